@@ -1,0 +1,712 @@
+// ct_build.hip -- bulk load on gfx950: Morton keys -> onesweep LSD radix sort of
+// (key, input position) -> gather of the caller's box arrays into sorted order
+// fused with the leaf-parent MBR reduce -> upper levels.
+//
+// Reference algorithm: src/rtree_bulkload_thrust.cuh:62-213 (key transform
+// :76-81, sort_by_key :83-86, level plan :101-115, reduce_by_key :126-171).
+#include "ct_internal.hpp"
+
+namespace ct {
+
+// ---------------------------------------------------------------------------
+// Block-wide exclusive scan over 256 threads (4 waves).  s_tmp holds 4 words.
+__device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* s_tmp,
+                                                       uint32_t* total) {
+  const int lane = lane_id(), w = threadIdx.x >> 6;
+  uint32_t x = v;
+  for (int o = 1; o < kWave; o <<= 1) {
+    uint32_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == kWave - 1) s_tmp[w] = x;
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+  for (int i = 0; i < 4; i++) {
+    uint32_t t = s_tmp[i];
+    if (i < w) off += t;
+    tot += t;
+  }
+  __syncthreads();
+  *total = tot;
+  return x - v + off;
+}
+
+// ---------------------------------------------------------------------------
+// Control block layout (device memory, zeroed by k_build_init).
+struct BuildCtrl {
+  uint32_t ext[4];                 // ordered-uint lox, loy, hix, hiy of centroids
+  uint32_t hist[kMaxDigits * 256];  // global digit histograms
+  uint32_t base[kMaxDigits * 256];  // exclusive digit offsets
+  uint32_t ticket[kMaxDigits];
+  int32_t active[kMaxDigits];      // pass p runs
+  int32_t src[kMaxDigits];         // pass p reads buffer src[p]
+  int32_t write_keys[kMaxDigits];  // pass p writes keys (not the last pass)
+  int32_t npass;                   // number of passes that run
+  int32_t first;                   // first pass that runs (-1: none)
+};
+
+__global__ __launch_bounds__(256) void k_build_init(BuildCtrl* c) {
+  for (int i = threadIdx.x; i < kMaxDigits * 256; i += 256) {
+    c->hist[i] = 0;
+    c->base[i] = 0;
+  }
+  if (threadIdx.x < kMaxDigits) {
+    c->ticket[threadIdx.x] = 0;
+    c->active[threadIdx.x] = 0;
+    c->src[threadIdx.x] = 0;
+    c->write_keys[threadIdx.x] = 0;
+  }
+  if (threadIdx.x == 0) {
+    c->ext[0] = 0xFFFFFFFFu;
+    c->ext[1] = 0xFFFFFFFFu;
+    c->ext[2] = 0u;
+    c->ext[3] = 0u;
+    c->npass = 0;
+    c->first = -1;
+  }
+}
+
+// Centroid extent (MORTON64 mode).  NaN centroids are skipped.
+__global__ __launch_bounds__(256) void k_extent(int64_t n, const float* __restrict__ xmin,
+                                                const float* __restrict__ ymin,
+                                                const float* __restrict__ xmax,
+                                                const float* __restrict__ ymax, BuildCtrl* c) {
+  uint32_t lx = 0xFFFFFFFFu, ly = 0xFFFFFFFFu, hx = 0u, hy = 0u;
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    float cx = (xmin[i] + xmax[i]) * 0.5f;
+    float cy = (ymin[i] + ymax[i]) * 0.5f;
+    if (cx == cx) {
+      uint32_t o = f2ord(cx);
+      lx = min(lx, o);
+      hx = max(hx, o);
+    }
+    if (cy == cy) {
+      uint32_t o = f2ord(cy);
+      ly = min(ly, o);
+      hy = max(hy, o);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    lx = min(lx, (uint32_t)__shfl_xor(lx, o));
+    ly = min(ly, (uint32_t)__shfl_xor(ly, o));
+    hx = max(hx, (uint32_t)__shfl_xor(hx, o));
+    hy = max(hy, (uint32_t)__shfl_xor(hy, o));
+  }
+  if (lane_id() == 0) {
+    atomicMin(&c->ext[0], lx);
+    atomicMin(&c->ext[1], ly);
+    atomicMax(&c->ext[2], hx);
+    atomicMax(&c->ext[3], hy);
+  }
+}
+
+// Extent -> (lo, scale) per axis, the same FP32 operations as the oracle.
+__device__ __forceinline__ void axis_params(uint32_t lo_o, uint32_t hi_o, float* lo, float* s) {
+  if (lo_o > hi_o) {  // no non-NaN centroid
+    *lo = 0.0f;
+    *s = 0.0f;
+    return;
+  }
+  float l = ord2f(lo_o), h = ord2f(hi_o);
+  float w = h - l;
+  *lo = l;
+  *s = (w > 0.0f) ? 4294967296.0f / w : 0.0f;
+}
+
+// Key generation + AoS staging copy + all-digit histograms + status zeroing.
+// REF32: key = z_order((ushort)cx, (ushort)cy) as in mbr2key
+// (src/rtree_bulkload_thrust.cuh:52-60), stored with the sign bit flipped so
+// that an unsigned radix sort yields the reference's signed int32 order.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_keygen(int64_t n, const float* __restrict__ xmin,
+                                                const float* __restrict__ ymin,
+                                                const float* __restrict__ xmax,
+                                                const float* __restrict__ ymax,
+                                                const int* __restrict__ id, BuildCtrl* c,
+                                                void* __restrict__ keys_out,
+                                                uint4* __restrict__ aos,
+                                                uint32_t* __restrict__ zero_ptr,
+                                                int64_t zero_words) {
+  constexpr int D = MODE == kKeyRef32 ? 4 : 8;
+  __shared__ uint32_t sh[D * 256];
+  for (int t = threadIdx.x; t < D * 256; t += 256) sh[t] = 0;
+  float lox = 0.f, loy = 0.f, sx = 0.f, sy = 0.f;
+  if (MODE == kKeyMorton64) {
+    axis_params(c->ext[0], c->ext[2], &lox, &sx);
+    axis_params(c->ext[1], c->ext[3], &loy, &sy);
+  }
+  __syncthreads();
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += stride) {
+    float a = xmin[i], b = ymin[i], cc = xmax[i], d = ymax[i];
+    int ident = id[i];
+    aos[2 * i] = make_uint4(__float_as_uint(a), __float_as_uint(b), __float_as_uint(cc),
+                            __float_as_uint(d));
+    aos[2 * i + 1] = make_uint4((uint32_t)ident, 0u, 0u, 0u);
+    if (MODE == kKeyRef32) {
+      float cx = (a + cc) / 2.0f;
+      float cy = (b + d) / 2.0f;
+      uint32_t z = (spread16(f2u16_sat(cy)) << 1) | spread16(f2u16_sat(cx));
+      uint32_t k = z ^ 0x80000000u;
+      ((uint32_t*)keys_out)[i] = k;
+#pragma unroll
+      for (int p = 0; p < 4; p++) atomicAdd(&sh[p * 256 + ((k >> (8 * p)) & 255u)], 1u);
+    } else {
+      float cx = (a + cc) * 0.5f;
+      float cy = (b + d) * 0.5f;
+      uint64_t k = (spread32(q32(cy, loy, sy)) << 1) | spread32(q32(cx, lox, sx));
+      ((uint64_t*)keys_out)[i] = k;
+#pragma unroll
+      for (int p = 0; p < 8; p++)
+        atomicAdd(&sh[p * 256 + (uint32_t)((k >> (8 * p)) & 255u)], 1u);
+    }
+  }
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < zero_words; i += stride) zero_ptr[i] = 0u;
+  __syncthreads();
+  for (int t = threadIdx.x; t < D * 256; t += 256)
+    if (sh[t]) atomicAdd(&c->hist[t], sh[t]);
+}
+
+// Pass plan: exclusive digit offsets per pass; a pass whose digit is the same
+// for every key is skipped (it would be the identity permutation).
+__global__ __launch_bounds__(256) void k_sort_plan(int64_t n, int D, BuildCtrl* c) {
+  __shared__ uint32_t s_tmp[4];
+  __shared__ int s_triv[kMaxDigits];
+  for (int p = 0; p < D; p++) {
+    uint32_t v = c->hist[p * 256 + threadIdx.x];
+    int triv = __syncthreads_or(v == (uint32_t)n);
+    uint32_t tot;
+    uint32_t ex = block_excl_scan256(v, s_tmp, &tot);
+    c->base[p * 256 + threadIdx.x] = ex;
+    if (threadIdx.x == 0) s_triv[p] = triv;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int J = 0, last = -1, first = -1;
+    for (int p = 0; p < D; p++) {
+      if (s_triv[p]) {
+        c->active[p] = 0;
+      } else {
+        c->active[p] = 1;
+        c->src[p] = J & 1;
+        c->write_keys[p] = 1;
+        if (first < 0) first = p;
+        last = p;
+        J++;
+      }
+    }
+    if (last >= 0) c->write_keys[last] = 0;
+    c->npass = J;
+    c->first = first;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// One onesweep LSD pass over 8 bits.  Tile = 256 threads x kSortItems keys,
+// warp-striped (wave w owns keys [w*64*I, (w+1)*64*I) of the tile, item i of
+// lane l is key w*64*I + i*64 + l), so ballot ranking in (item, lane) order is
+// the input order and the pass is stable.  Per-digit tile counts are chained
+// across tiles by decoupled look-back; tiles are numbered by an atomic ticket
+// so every tile waits only on tiles that have already started.
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr uint32_t kFlagA = 1u << 30, kFlagP = 2u << 30, kValMask = (1u << 30) - 1u;
+
+template <class KT>
+__global__ __launch_bounds__(kSortThreads) void k_onesweep(
+    int64_t n, int pass, KT* __restrict__ k0, KT* __restrict__ k1, uint32_t* __restrict__ v0,
+    uint32_t* __restrict__ v1, BuildCtrl* c, uint32_t* __restrict__ status, int keys_only) {
+  __shared__ uint32_t s_wh[4][256];
+  __shared__ uint32_t s_start[256];
+  __shared__ uint32_t s_adj[256];
+  __shared__ uint32_t s_tmp[4];
+  __shared__ uint32_t s_tile;
+  __shared__ KT s_buf[kSortTile];
+
+  if (!c->active[pass]) return;
+  const int t = threadIdx.x, w = t >> 6, lane = lane_id();
+  const int shift = 8 * pass;
+  if (t == 0) s_tile = atomicAdd(&c->ticket[pass], 1u);
+  for (int i = t; i < 4 * 256; i += kSortThreads) (&s_wh[0][0])[i] = 0;
+  __syncthreads();
+  const int64_t tile = s_tile;
+  const int src = c->src[pass];
+  const bool ident = pass == c->first;  // first pass: values are input positions
+  const KT* kin = src ? k1 : k0;
+  KT* kout = src ? k0 : k1;
+  const uint32_t* vin = src ? v1 : v0;
+  uint32_t* vout = src ? v0 : v1;
+  const bool wk = keys_only || c->write_keys[pass] != 0;
+
+  const int64_t tbase = tile * kSortTile;
+  const int64_t wbase = tbase + (int64_t)w * (kWave * kSortItems);
+  KT key[kSortItems];
+  uint32_t val[kSortItems];
+  uint32_t rank[kSortItems];
+#pragma unroll
+  for (int i = 0; i < kSortItems; i++) {
+    int64_t idx = wbase + i * kWave + lane;
+    bool valid = idx < n;
+    key[i] = valid ? kin[idx] : (KT)0;
+    val[i] = (valid && !keys_only) ? (ident ? (uint32_t)idx : vin[idx]) : 0u;
+  }
+  const uint64_t lt = lanemask_lt();
+#pragma unroll
+  for (int i = 0; i < kSortItems; i++) {
+    int64_t idx = wbase + i * kWave + lane;
+    bool valid = idx < n;
+    uint32_t d = (uint32_t)(key[i] >> shift) & 255u;
+    uint64_t m = ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      uint64_t bb = ballot((d >> b) & 1u);
+      m &= ((d >> b) & 1u) ? bb : ~bb;
+    }
+    uint32_t old = s_wh[w][d];
+    rank[i] = old + (uint32_t)popc64(m & lt);
+    if (valid && (m & lt) == 0) s_wh[w][d] = old + (uint32_t)popc64(m);
+  }
+  __syncthreads();
+
+  // thread t owns digit t: wave offsets, tile count, look-back, tile-local start
+  const int dg = t;
+  uint32_t c0 = s_wh[0][dg], c1 = s_wh[1][dg], c2 = s_wh[2][dg], c3 = s_wh[3][dg];
+  s_wh[0][dg] = 0;
+  s_wh[1][dg] = c0;
+  s_wh[2][dg] = c0 + c1;
+  s_wh[3][dg] = c0 + c1 + c2;
+  const uint32_t tot = c0 + c1 + c2 + c3;
+  uint32_t* st = status + (size_t)tile * 256;
+  uint32_t excl = 0;
+  if (tile == 0) {
+    st_agent(st + dg, kFlagP | tot);
+  } else {
+    st_agent(st + dg, kFlagA | tot);
+    for (int64_t j = tile - 1; j >= 0; j--) {
+      uint32_t wv;
+      do {
+        wv = ld_agent(status + (size_t)j * 256 + dg);
+      } while ((wv & ~kValMask) == 0u);
+      excl += wv & kValMask;
+      if ((wv & ~kValMask) == kFlagP) break;
+    }
+    st_agent(st + dg, kFlagP | (excl + tot));
+  }
+  uint32_t ttot;
+  const uint32_t start = block_excl_scan256(tot, s_tmp, &ttot);
+  s_start[dg] = start;
+  s_adj[dg] = c->base[pass * 256 + dg] + excl - start;
+  __syncthreads();
+
+  // local reorder of keys through LDS, then scatter runs of equal digits
+  uint32_t lpos[kSortItems];
+#pragma unroll
+  for (int i = 0; i < kSortItems; i++) {
+    int64_t idx = wbase + i * kWave + lane;
+    uint32_t d = (uint32_t)(key[i] >> shift) & 255u;
+    lpos[i] = s_start[d] + s_wh[w][d] + rank[i];
+    if (idx < n) s_buf[lpos[i]] = key[i];
+  }
+  __syncthreads();
+  const int64_t rem = n - tbase;
+  const int tn = rem < kSortTile ? (int)rem : kSortTile;
+  uint32_t gpos[kSortItems];
+#pragma unroll
+  for (int i = 0; i < kSortItems; i++) {
+    int j = t + i * kSortThreads;
+    gpos[i] = 0;
+    if (j < tn) {
+      KT kk = s_buf[j];
+      uint32_t d = (uint32_t)(kk >> shift) & 255u;
+      gpos[i] = s_adj[d] + (uint32_t)j;
+      if (wk) kout[gpos[i]] = kk;
+    }
+  }
+  if (keys_only) return;
+  __syncthreads();
+  uint32_t* s_val = (uint32_t*)s_buf;
+#pragma unroll
+  for (int i = 0; i < kSortItems; i++) {
+    int64_t idx = wbase + i * kWave + lane;
+    if (idx < n) s_val[lpos[i]] = val[i];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kSortItems; i++) {
+    int j = t + i * kSortThreads;
+    if (j < tn) vout[gpos[i]] = s_val[j];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Group reduction of an MBR over G consecutive lanes (G a power of two <= 64).
+// `has` marks lanes that hold a value; invalid lanes are skipped.
+template <int G>
+__device__ __forceinline__ void group_mbr(float& a, float& b, float& cc, float& d, bool has) {
+#pragma unroll
+  for (int o = 1; o < G; o <<= 1) {
+    float pa = __shfl_xor(a, o), pb = __shfl_xor(b, o);
+    float pc = __shfl_xor(cc, o), pd = __shfl_xor(d, o);
+    bool ph = __shfl_xor((int)has, o) != 0;
+    if (ph) {
+      if (has) {
+        a = mbr_min(a, pa);
+        b = mbr_min(b, pb);
+        cc = mbr_max(cc, pc);
+        d = mbr_max(d, pd);
+      } else {
+        a = pa;
+        b = pb;
+        cc = pc;
+        d = pd;
+        has = true;
+      }
+    }
+  }
+}
+
+// Gather the boxes into sorted order (in place on the caller's arrays, reading
+// the staging copy) and, for power-of-two F <= 64, build the leaf-parent level
+// in the same pass: node g covers sorted boxes [g*F, min(g*F+F, N)) with
+// pos = R_sz + g*F and len = count (src/rtree_bulkload_thrust.cuh:136-149).
+template <int G>
+__global__ __launch_bounds__(256) void k_gather_l0(int64_t n, const BuildCtrl* c,
+                                                   const uint32_t* __restrict__ v0,
+                                                   const uint32_t* __restrict__ v1,
+                                                   const uint4* __restrict__ aos, float* xmin,
+                                                   float* ymin, float* xmax, float* ymax,
+                                                   int* id, TreeView tr, int64_t l0start,
+                                                   int64_t r_sz) {
+  const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+  const int J = c->npass;
+  const uint32_t* vf = (J & 1) ? v1 : v0;
+  float a = 0.f, b = 0.f, cc = 0.f, d = 0.f;
+  const bool has = i < n;
+  if (has) {
+    uint32_t s = J ? vf[i] : (uint32_t)i;
+    uint4 r0 = aos[2 * (size_t)s];
+    uint4 r1 = aos[2 * (size_t)s + 1];
+    a = __uint_as_float(r0.x);
+    b = __uint_as_float(r0.y);
+    cc = __uint_as_float(r0.z);
+    d = __uint_as_float(r0.w);
+    xmin[i] = a;
+    ymin[i] = b;
+    xmax[i] = cc;
+    ymax[i] = d;
+    id[i] = (int)r1.x;
+  }
+  if constexpr (G > 0) {
+    group_mbr<G>(a, b, cc, d, has);
+    if (has && (i % G) == 0) {
+      int64_t g = i / G, node = l0start + g;
+      tr.xmin[node] = a;
+      tr.ymin[node] = b;
+      tr.xmax[node] = cc;
+      tr.ymax[node] = d;
+      tr.pos[node] = (int)(r_sz + i);
+      int64_t len = n - i;
+      tr.len[node] = (int)(len < G ? len : G);
+    }
+  }
+}
+
+// Upper level (or a level 0 that is not fused): child i of [0, cn) reads the
+// child arrays at cbase, parent i/G is written at pbase.  pos = absolute child
+// index (src/rtree_bulkload_thrust.cuh:151-168).
+template <int G>
+__global__ __launch_bounds__(256) void k_level_pow2(int64_t cn, const float* cx0, const float* cy0,
+                                                    const float* cx1, const float* cy1,
+                                                    int64_t pos_base, TreeView tr,
+                                                    int64_t pbase) {
+  const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+  float a = 0.f, b = 0.f, cc = 0.f, d = 0.f;
+  const bool has = i < cn;
+  if (has) {
+    a = cx0[i];
+    b = cy0[i];
+    cc = cx1[i];
+    d = cy1[i];
+  }
+  group_mbr<G>(a, b, cc, d, has);
+  if (has && (i % G) == 0) {
+    int64_t node = pbase + i / G;
+    tr.xmin[node] = a;
+    tr.ymin[node] = b;
+    tr.xmax[node] = cc;
+    tr.ymax[node] = d;
+    tr.pos[node] = (int)(pos_base + i);
+    int64_t len = cn - i;
+    tr.len[node] = (int)(len < G ? len : G);
+  }
+}
+
+// Any fanout: one thread per parent, left-to-right loop over its children.
+__global__ __launch_bounds__(256) void k_level_generic(int64_t cn, int F, const float* cx0,
+                                                       const float* cy0, const float* cx1,
+                                                       const float* cy1, int64_t pos_base,
+                                                       TreeView tr, int64_t pbase, int64_t pn) {
+  const int64_t g = blockIdx.x * 256ll + threadIdx.x;
+  if (g >= pn) return;
+  int64_t b0 = g * F, e = b0 + F;
+  if (e > cn) e = cn;
+  float a = cx0[b0], b = cy0[b0], cc = cx1[b0], d = cy1[b0];
+  for (int64_t i = b0 + 1; i < e; i++) {
+    a = mbr_min(a, cx0[i]);
+    b = mbr_min(b, cy0[i]);
+    cc = mbr_max(cc, cx1[i]);
+    d = mbr_max(d, cy1[i]);
+  }
+  int64_t node = pbase + g;
+  tr.xmin[node] = a;
+  tr.ymin[node] = b;
+  tr.xmax[node] = cc;
+  tr.ymax[node] = d;
+  tr.pos[node] = (int)(pos_base + b0);
+  tr.len[node] = (int)(e - b0);
+}
+
+// ---------------------------------------------------------------------------
+// Host side.
+static int pow2_group(int F) {
+  switch (F) {
+    case 2: case 4: case 8: case 16: case 32: case 64: return F;
+    default: return 0;
+  }
+}
+
+template <int G>
+static void launch_gather(hipStream_t s, int64_t n, const BuildCtrl* c, const uint32_t* v0,
+                          const uint32_t* v1, const uint4* aos, curtree_bbox_f32* box,
+                          TreeView tr, int64_t l0start, int64_t r_sz) {
+  unsigned grid = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(k_gather_l0<G>, dim3(grid), dim3(256), 0, s, n, c, v0, v1, aos, box->xmin,
+                     box->ymin, box->xmax, box->ymax, box->id, tr, l0start, r_sz);
+}
+
+static void launch_level(hipStream_t s, int F, int64_t cn, const float* cx0, const float* cy0,
+                         const float* cx1, const float* cy1, int64_t pos_base, TreeView tr,
+                         int64_t pbase, int64_t pn) {
+  int G = pow2_group(F);
+  unsigned grid = (unsigned)((cn + 255) / 256);
+  switch (G) {
+#define CT_LVL(g)                                                                              \
+  case g:                                                                                      \
+    hipLaunchKernelGGL(k_level_pow2<g>, dim3(grid), dim3(256), 0, s, cn, cx0, cy0, cx1, cy1,   \
+                       pos_base, tr, pbase);                                                   \
+    return;
+    CT_LVL(2) CT_LVL(4) CT_LVL(8) CT_LVL(16) CT_LVL(32) CT_LVL(64)
+#undef CT_LVL
+    default: {
+      unsigned pg = (unsigned)((pn + 255) / 256);
+      hipLaunchKernelGGL(k_level_generic, dim3(pg), dim3(256), 0, s, cn, F, cx0, cy0, cx1, cy1,
+                         pos_base, tr, pbase, pn);
+    }
+  }
+}
+
+int build_f32(Context& ctx, curtree_bbox_f32* box, curtree_rtree_f32* rt, int key_mode,
+              hipStream_t s, bool alloc) {
+  if (!box || !rt) return set_error(CURTREE_EINVAL, "build: null argument");
+  const int F = rt->fanout;
+  const int64_t n = box->sz;
+  if (F < 2) return set_error(CURTREE_EINVAL, "build: fanout must be >= 2 (got %d)", F);
+  if (n < 0) return set_error(CURTREE_EINVAL, "build: negative box count");
+  if (n >= (int64_t)1 << 30)
+    return set_error(CURTREE_EINVAL, "build: at most 2^30-1 boxes per tree (got %lld)",
+                     (long long)n);
+  if (key_mode != kKeyRef32 && key_mode != kKeyMorton64)
+    return set_error(CURTREE_EINVAL, "build: unknown key mode %d", key_mode);
+  if (n > 0 && (!box->xmin || !box->ymin || !box->xmax || !box->ymax || !box->id))
+    return set_error(CURTREE_EINVAL, "build: null box array");
+
+  LevelPlan lp;
+  level_plan(n, F, &lp);
+  if (lp.r_sz > INT32_MAX - n)
+    return set_error(CURTREE_EINVAL, "build: node count overflows int32 positions");
+  int rc;
+  if (alloc) {
+    rt->sz = 0;
+    rt->xmin = rt->ymin = rt->xmax = rt->ymax = nullptr;
+    rt->pos = rt->len = nullptr;
+    if (n == 0) return CURTREE_OK;
+    // node arrays: owned by the caller afterwards (freed with curtree_rt_d_free)
+    if ((rc = rt_alloc(rt, lp.r_sz))) return rc;
+  } else {
+    if (rt->sz != lp.r_sz)
+      return set_error(CURTREE_EINVAL, "build: tree holds %d nodes, %lld boxes need %lld", rt->sz,
+                       (long long)n, (long long)lp.r_sz);
+    if (n == 0) return CURTREE_OK;
+    if (!rt->xmin || !rt->ymin || !rt->xmax || !rt->ymax || !rt->pos || !rt->len)
+      return set_error(CURTREE_EINVAL, "build: null node array");
+  }
+  auto fail = [&](int code) {
+    if (alloc) rt_free(rt);
+    return code;
+  };
+
+  const int D = key_mode == kKeyRef32 ? 4 : 8;
+  const size_t kb = key_mode == kKeyRef32 ? 4 : 8;
+  const int64_t ntiles = (n + kSortTile - 1) / kSortTile;
+  const size_t status_words = (size_t)D * ntiles * 256;
+  Workspace& ws = ctx.ws;
+  if ((rc = ws.reserve(WsBuildKeys0, n * kb)) || (rc = ws.reserve(WsBuildKeys1, n * kb)) ||
+      (rc = ws.reserve(WsBuildVals0, n * 4)) || (rc = ws.reserve(WsBuildVals1, n * 4)) ||
+      (rc = ws.reserve(WsBuildAos, n * 32)) ||
+      (rc = ws.reserve(WsBuildStatus, status_words * 4)) ||
+      (rc = ws.reserve(WsBuildCtrl, sizeof(BuildCtrl))))
+    return fail(rc);
+  BuildCtrl* c = (BuildCtrl*)ws.ptr(WsBuildCtrl);
+  void* k0 = ws.ptr(WsBuildKeys0);
+  void* k1 = ws.ptr(WsBuildKeys1);
+  uint32_t* v0 = (uint32_t*)ws.ptr(WsBuildVals0);
+  uint32_t* v1 = (uint32_t*)ws.ptr(WsBuildVals1);
+  uint4* aos = (uint4*)ws.ptr(WsBuildAos);
+  uint32_t* status = (uint32_t*)ws.ptr(WsBuildStatus);
+
+  Prof& pf = ctx.prof;
+  pf.mark(s, "k_build_init");
+  hipLaunchKernelGGL(k_build_init, dim3(1), dim3(256), 0, s, c);
+  const unsigned sgrid = (unsigned)std::min<int64_t>((n + 1023) / 1024, 2048);
+  if (key_mode == kKeyMorton64) {
+    pf.mark(s, "k_extent");
+    hipLaunchKernelGGL(k_extent, dim3(sgrid), dim3(256), 0, s, n, box->xmin, box->ymin,
+                       box->xmax, box->ymax, c);
+  }
+  pf.mark(s, "k_keygen");
+  if (key_mode == kKeyRef32)
+    hipLaunchKernelGGL(k_keygen<kKeyRef32>, dim3(sgrid), dim3(256), 0, s, n, box->xmin,
+                       box->ymin, box->xmax, box->ymax, box->id, c, k0, aos, status,
+                       (int64_t)status_words);
+  else
+    hipLaunchKernelGGL(k_keygen<kKeyMorton64>, dim3(sgrid), dim3(256), 0, s, n, box->xmin,
+                       box->ymin, box->xmax, box->ymax, box->id, c, k0, aos, status,
+                       (int64_t)status_words);
+  pf.mark(s, "k_sort_plan");
+  hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(256), 0, s, n, D, c);
+  for (int p = 0; p < D; p++) {
+    pf.mark(s, "k_onesweep");
+    uint32_t* st = status + (size_t)p * ntiles * 256;
+    if (key_mode == kKeyRef32)
+      hipLaunchKernelGGL(k_onesweep<uint32_t>, dim3((unsigned)ntiles), dim3(kSortThreads), 0, s,
+                         n, p, (uint32_t*)k0, (uint32_t*)k1, v0, v1, c, st, 0);
+    else
+      hipLaunchKernelGGL(k_onesweep<uint64_t>, dim3((unsigned)ntiles), dim3(kSortThreads), 0, s,
+                         n, p, (uint64_t*)k0, (uint64_t*)k1, v0, v1, c, st, 0);
+  }
+
+  TreeView tr{rt->xmin, rt->ymin, rt->xmax, rt->ymax, rt->pos, rt->len};
+  const int64_t l0start = lp.r_sz - lp.n[0];
+  const int G = pow2_group(F);
+  pf.mark(s, "k_gather_l0");
+  switch (G) {
+    case 2: launch_gather<2>(s, n, c, v0, v1, aos, box, tr, l0start, lp.r_sz); break;
+    case 4: launch_gather<4>(s, n, c, v0, v1, aos, box, tr, l0start, lp.r_sz); break;
+    case 8: launch_gather<8>(s, n, c, v0, v1, aos, box, tr, l0start, lp.r_sz); break;
+    case 16: launch_gather<16>(s, n, c, v0, v1, aos, box, tr, l0start, lp.r_sz); break;
+    case 32: launch_gather<32>(s, n, c, v0, v1, aos, box, tr, l0start, lp.r_sz); break;
+    case 64: launch_gather<64>(s, n, c, v0, v1, aos, box, tr, l0start, lp.r_sz); break;
+    default:
+      launch_gather<0>(s, n, c, v0, v1, aos, box, tr, l0start, lp.r_sz);
+      pf.mark(s, "k_level");
+      launch_level(s, F, n, box->xmin, box->ymin, box->xmax, box->ymax, lp.r_sz, tr, l0start,
+                   lp.n[0]);
+  }
+  // upper levels: level l (l >= 1) reads level l-1
+  int64_t child = l0start;
+  for (int l = 1; l < lp.nl; l++) {
+    int64_t pstart = child - lp.n[l];
+    pf.mark(s, "k_level");
+    launch_level(s, F, lp.n[l - 1], rt->xmin + child, rt->ymin + child, rt->xmax + child,
+                 rt->ymax + child, child, tr, pstart, lp.n[l]);
+    child = pstart;
+  }
+  pf.mark(s, nullptr);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return fail(set_error(CURTREE_EHIP, "build: launch failed: %s", hipGetErrorString(e)));
+  return CURTREE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Canonical pair order: (query id, box id) lexicographic, as the reference's
+// two host stable sorts leave it (src/rtree_test_frontier.cu:102-105).  Pairs
+// are packed into one 64-bit key (query id high, box id with its sign bit
+// flipped low, so that unsigned order is signed int order) and radix sorted.
+__global__ __launch_bounds__(256) void k_pair_pack(int64_t n, const int* __restrict__ fid,
+                                                   const int* __restrict__ tid, BuildCtrl* c,
+                                                   uint64_t* __restrict__ keys,
+                                                   uint32_t* __restrict__ zero_ptr,
+                                                   int64_t zero_words) {
+  __shared__ uint32_t sh[8 * 256];
+  for (int t = threadIdx.x; t < 8 * 256; t += 256) sh[t] = 0;
+  __syncthreads();
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += stride) {
+    uint64_t k = ((uint64_t)((uint32_t)tid[i] ^ 0x80000000u) << 32) |
+                 (uint64_t)((uint32_t)fid[i] ^ 0x80000000u);
+    keys[i] = k;
+#pragma unroll
+    for (int p = 0; p < 8; p++) atomicAdd(&sh[p * 256 + (uint32_t)((k >> (8 * p)) & 255u)], 1u);
+  }
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < zero_words; i += stride) zero_ptr[i] = 0u;
+  __syncthreads();
+  for (int t = threadIdx.x; t < 8 * 256; t += 256)
+    if (sh[t]) atomicAdd(&c->hist[t], sh[t]);
+}
+
+__global__ __launch_bounds__(256) void k_pair_unpack(int64_t n, const BuildCtrl* c,
+                                                     const uint64_t* __restrict__ k0,
+                                                     const uint64_t* __restrict__ k1,
+                                                     int* __restrict__ fid,
+                                                     int* __restrict__ tid) {
+  const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t k = (c->npass & 1) ? k1[i] : k0[i];
+  tid[i] = (int)((uint32_t)(k >> 32) ^ 0x80000000u);
+  fid[i] = (int)((uint32_t)k ^ 0x80000000u);
+}
+
+int pairs_canonical_sort(Context& ctx, int* fid, int* tid, int64_t n, hipStream_t s) {
+  if (n < 0) return set_error(CURTREE_EINVAL, "canonical sort: negative count");
+  if (n <= 1) return CURTREE_OK;
+  if (!fid || !tid) return set_error(CURTREE_EINVAL, "canonical sort: null array");
+  if (n >= (int64_t)1 << 30)
+    return set_error(CURTREE_ERANGE, "canonical sort: at most 2^30-1 pairs per call");
+  const int64_t ntiles = (n + kSortTile - 1) / kSortTile;
+  const size_t status_words = (size_t)8 * ntiles * 256;
+  Workspace& ws = ctx.ws;
+  int rc;
+  if ((rc = ws.reserve(WsSortKeys0, n * 8)) || (rc = ws.reserve(WsSortKeys1, n * 8)) ||
+      (rc = ws.reserve(WsSortStatus, status_words * 4)) ||
+      (rc = ws.reserve(WsSortCtrl, sizeof(BuildCtrl))))
+    return rc;
+  BuildCtrl* c = (BuildCtrl*)ws.ptr(WsSortCtrl);
+  uint64_t* k0 = (uint64_t*)ws.ptr(WsSortKeys0);
+  uint64_t* k1 = (uint64_t*)ws.ptr(WsSortKeys1);
+  uint32_t* status = (uint32_t*)ws.ptr(WsSortStatus);
+  Prof& pf = ctx.prof;
+  pf.mark(s, "k_build_init");
+  hipLaunchKernelGGL(k_build_init, dim3(1), dim3(256), 0, s, c);
+  const unsigned sgrid = (unsigned)std::min<int64_t>((n + 1023) / 1024, 2048);
+  pf.mark(s, "k_pair_pack");
+  hipLaunchKernelGGL(k_pair_pack, dim3(sgrid), dim3(256), 0, s, n, fid, tid, c, k0, status,
+                     (int64_t)status_words);
+  pf.mark(s, "k_sort_plan");
+  hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(256), 0, s, n, 8, c);
+  for (int p = 0; p < 8; p++) {
+    pf.mark(s, "k_onesweep_pairs");
+    hipLaunchKernelGGL(k_onesweep<uint64_t>, dim3((unsigned)ntiles), dim3(kSortThreads), 0, s, n,
+                       p, k0, k1, (uint32_t*)nullptr, (uint32_t*)nullptr, c,
+                       status + (size_t)p * ntiles * 256, 1);
+  }
+  pf.mark(s, "k_pair_unpack");
+  hipLaunchKernelGGL(k_pair_unpack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, c, k0,
+                     k1, fid, tid);
+  pf.mark(s, nullptr);
+  CT_HIP(hipGetLastError());
+  return CURTREE_OK;
+}
+
+}  // namespace ct

@@ -1,0 +1,187 @@
+/*
+ * curtree_c.h -- C ABI of the MI355X (gfx950) R-tree library libcurtree_amd.so.
+ *
+ * The structs are layout-identical to the reference's data model
+ * (src/rtree.h:3-29): int sizes followed by raw device pointers.  Every entry
+ * point returns a status code (CURTREE_OK == 0); curtree_last_error() gives the
+ * message of the calling thread's last failure.  Device pointers are HIP device
+ * memory; `stream` is a hipStream_t (NULL = the null stream).  Entry points that
+ * mirror a synchronous reference call synchronise `stream` before returning.
+ */
+#ifndef CURTREE_C_H
+#define CURTREE_C_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CURTREE_API __attribute__((visibility("default")))
+
+/* status codes */
+#define CURTREE_OK 0
+#define CURTREE_EINVAL 1   /* bad argument */
+#define CURTREE_ENOMEM 2   /* device allocation failed */
+#define CURTREE_EHIP 3     /* HIP runtime error */
+#define CURTREE_ERANGE 4   /* result larger than the caller's buffer / int32 limit */
+#define CURTREE_EBUSY 5    /* serving queue full / budget exhausted */
+
+/* key modes for curtree_build_f32 */
+#define CURTREE_KEY_REF32 0    /* the reference's 16-bit-per-axis Z-order key, int32 order */
+#define CURTREE_KEY_MORTON64 1 /* 32 bits per axis over the centroid extent, uint64 order */
+
+/* query algorithms */
+#define CURTREE_ALGO_FRONTIER 0   /* level-synchronous BFS frontier in HBM */
+#define CURTREE_ALGO_PERSISTENT 1 /* persistent work-stealing frontier kernel */
+
+/* == BBOX<float>, src/rtree.h:3-11 */
+typedef struct curtree_bbox_f32 {
+  int sz;
+  float* xmin;
+  float* ymin;
+  float* xmax;
+  float* ymax;
+  int* id;
+} curtree_bbox_f32;
+
+/* == RTREE<float>, src/rtree.h:13-23.  Nodes root-level first; the root is
+ * node 0 and the leaf-parent level is the last block.  pos = first child (for
+ * leaf-parents: R_sz + first sorted-box position), len = child count. */
+typedef struct curtree_rtree_f32 {
+  int sz;
+  int fanout;
+  float* xmin;
+  float* ymin;
+  float* xmax;
+  float* ymax;
+  int* pos;
+  int* len;
+} curtree_rtree_f32;
+
+/* == IDPAIR, src/rtree.h:25-29: fid = data box id, tid = query index */
+typedef struct curtree_idpair {
+  int sz;
+  int* fid;
+  int* tid;
+} curtree_idpair;
+
+/* IDPAIR with a 64-bit count, for result sets beyond 2^31-1 pairs */
+typedef struct curtree_idpair64 {
+  int64_t sz;
+  int* fid;
+  int* tid;
+} curtree_idpair64;
+
+/* ---- bulk load ---------------------------------------------------------- */
+
+/* Replaces build_rtree<float> (src/rtree_bulkload_thrust.cuh:62-213).
+ * Sorts the caller's box arrays IN PLACE into key order (the in-place
+ * sort_by_key of :83-86), allocates the six node arrays (as rt_d_alloc,
+ * src/utility.cuh:114-135; free with curtree_rt_d_free) and fills them level
+ * by level.  rt->fanout must be set by the caller (2..any); rt->sz receives
+ * R_sz.  Synchronises `stream`. */
+CURTREE_API int curtree_build_f32(curtree_bbox_f32* box, curtree_rtree_f32* rt, int key_mode,
+                                  void* stream);
+
+/* Same as curtree_build_f32 but asynchronous: returns after enqueueing all
+ * kernels on `stream` (the node arrays are allocated before it returns). */
+CURTREE_API int curtree_build_f32_async(curtree_bbox_f32* box, curtree_rtree_f32* rt,
+                                        int key_mode, void* stream);
+
+/* Same as curtree_build_f32_async, into node arrays the caller allocated:
+ * rt->sz must equal the node count for box->sz boxes at rt->fanout (see
+ * curtree_tree_size).  Allocates no node arrays (work buffers are cached). */
+CURTREE_API int curtree_build_into_f32(curtree_bbox_f32* box, curtree_rtree_f32* rt,
+                                       int key_mode, void* stream);
+
+/* Node count R_sz and number of levels of a bulk-loaded tree of n boxes. */
+CURTREE_API int curtree_tree_size(int64_t n, int fanout, int64_t* r_sz, int* levels);
+
+/* ---- window query ------------------------------------------------------- */
+
+/* Replaces bfs_frontier<float> (src/rtree_bfs_frontier.cuh:292-529) and
+ * bfs_thrust<float> (src/rtree_bfs_thrust.cuh:58-256): all (box id, query
+ * index) pairs whose boxes intersect the query windows (closed intervals),
+ * in (query index, sorted box position) order -- the order the reference
+ * produces before its host sort.  `rbox` is the box set as sorted by the
+ * build.  Allocates out->fid/out->tid (free with curtree_idpair64_d_free).
+ * Never aborts on large frontiers.  Synchronises `stream`. */
+CURTREE_API int curtree_query_f32(const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox,
+                                  const curtree_bbox_f32* qbox, curtree_idpair64* out, int algo,
+                                  void* stream);
+
+/* Same result written into caller buffers of `cap` pairs.  *n_out receives the
+ * pair count; if it exceeds `cap` nothing is written and CURTREE_ERANGE is
+ * returned.  Synchronises `stream`. */
+CURTREE_API int curtree_query_into_f32(const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox,
+                                       const curtree_bbox_f32* qbox, int* fid, int* tid,
+                                       int64_t cap, int64_t* n_out, int algo, void* stream);
+
+/* Count pass only: number of pairs per query (`counts`, Q int64 in device
+ * memory, may be NULL) and in total (*n_out). */
+CURTREE_API int curtree_query_count_f32(const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox,
+                                        const curtree_bbox_f32* qbox, int64_t* counts,
+                                        int64_t* n_out, void* stream);
+
+/* Canonical result order: sorts pairs on the device into (query id, box id)
+ * lexicographic order -- the order the reference's two host stable sorts give
+ * (src/rtree_test_frontier.cu:102-105).  Pairs must arrive grouped by query
+ * (as every query entry point returns them). */
+CURTREE_API int curtree_pairs_canonical_sort(int* fid, int* tid, int64_t n, void* stream);
+
+/* ---- memory helpers (src/utility.cuh:16-205, over hipMalloc/hipFree) ----- */
+
+/* box_h2d: allocate device arrays for h->sz boxes and copy them in */
+CURTREE_API int curtree_box_h2d(curtree_bbox_f32* d, const curtree_bbox_f32* h);
+/* box_d2h: copy into host arrays the caller allocated (h->xmin.. of d->sz) */
+CURTREE_API int curtree_box_d2h(curtree_bbox_f32* h, const curtree_bbox_f32* d);
+CURTREE_API int curtree_box_d_alloc(curtree_bbox_f32* d, int sz);
+CURTREE_API int curtree_box_d_free(curtree_bbox_f32* d);
+CURTREE_API int curtree_rt_d_alloc(curtree_rtree_f32* rt, int sz);
+CURTREE_API int curtree_rt_d_free(curtree_rtree_f32* rt);
+CURTREE_API int curtree_idpair_d_alloc(curtree_idpair* p, int sz);
+CURTREE_API int curtree_idpair_d_free(curtree_idpair* p);
+CURTREE_API int curtree_idpair64_d_free(curtree_idpair64* p);
+/* idpair_d2h: copy into host arrays of d->sz entries the caller allocated */
+CURTREE_API int curtree_idpair64_d2h(int* h_fid, int* h_tid, const curtree_idpair64* d);
+
+/* raw device memory (used by non-C hosts: Python, Go, ...) */
+CURTREE_API int curtree_malloc(void** p, int64_t bytes);
+CURTREE_API int curtree_free(void* p);
+CURTREE_API int curtree_memcpy_h2d(void* dst, const void* src, int64_t bytes);
+CURTREE_API int curtree_memcpy_d2h(void* dst, const void* src, int64_t bytes);
+CURTREE_API int curtree_memcpy_d2d(void* dst, const void* src, int64_t bytes, void* stream);
+CURTREE_API int curtree_memset(void* dst, int value, int64_t bytes);
+CURTREE_API int curtree_synchronize(void* stream);
+CURTREE_API int curtree_stream_create(void** stream);
+CURTREE_API int curtree_stream_destroy(void* stream);
+
+/* ---- diagnostics -------------------------------------------------------- */
+
+CURTREE_API const char* curtree_last_error(void);
+CURTREE_API const char* curtree_version(void);
+/* device count, name of device 0, CU count, HBM bytes */
+CURTREE_API int curtree_device_info(int* count, char* name, int name_len, int* cus,
+                                    int64_t* hbm_bytes);
+/* Release the library's cached work buffers (they are reused across calls). */
+CURTREE_API int curtree_release_workspace(void);
+
+/* Per-kernel timing with HIP events on the launch stream.  While enabled, every
+ * kernel the library launches is bracketed by events; curtree_prof_get reports
+ * the summed milliseconds and launch count of kernel `name` since the last
+ * reset (events are resolved at the call, which synchronises). */
+CURTREE_API int curtree_prof_enable(int on);
+CURTREE_API int curtree_prof_reset(void);
+CURTREE_API int curtree_prof_get(const char* name, double* total_ms, int64_t* launches);
+/* Elapsed milliseconds between two events recorded with curtree_event_record. */
+CURTREE_API int curtree_event_create(void** ev);
+CURTREE_API int curtree_event_destroy(void* ev);
+CURTREE_API int curtree_event_record(void* ev, void* stream);
+CURTREE_API int curtree_event_elapsed(void* ev0, void* ev1, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CURTREE_C_H */

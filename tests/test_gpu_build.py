@@ -1,0 +1,133 @@
+"""GPU parity of the bulk load (through the C ABI) against the oracle.
+
+Bar: bit-exact.  The caller's box arrays after the in-place sort and all six
+node arrays must equal the oracle's byte for byte, in both key modes, for
+ragged sizes and every fanout the query supports, plus special values."""
+import numpy as np
+import pytest
+
+import oracle_py as orc
+from curtree_amd import datagen
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_build(boxes, ids, fanout, key_mode):
+    import curtree_amd as ct
+
+    d = ct.DeviceBoxes.from_host(*boxes, ids)
+    rt = ct.build_rtree(d, fanout, key_mode)
+    sb = d.to_host()
+    t = rt.to_host()
+    sz = rt.sz
+    rt.free()
+    d.free()
+    return sb, t, sz
+
+
+def assert_bits_equal(a, b, what):
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    assert a.shape == b.shape, what
+    bad = np.nonzero(a.view(np.uint32) != b.view(np.uint32))[0]
+    assert bad.size == 0, f"{what}: {bad.size} mismatches, first at {bad[:5]}"
+
+
+def check_against_oracle(boxes, ids, fanout, key_mode):
+    sb, t, sz = gpu_build(boxes, ids, fanout, key_mode)
+    osb, ot = orc.build_rtree(*boxes, ids, fanout, key_mode)
+    assert sz == ot["pos"].shape[0]
+    for i, name in enumerate(("xmin", "ymin", "xmax", "ymax", "id")):
+        assert_bits_equal(sb[i], osb[i], f"sorted {name}")
+    for name in ("xmin", "ymin", "xmax", "ymax", "pos", "len"):
+        assert_bits_equal(t[name], ot[name], f"node {name}")
+    return sb, t
+
+
+def test_ref32_unit_square_is_input_order():
+    # the configured [0,1)^2 data: every reference key is 0, so the tree is
+    # built over the input order (stable sort of equal keys)
+    n = 1_000_000
+    bx = datagen.uniform_boxes(n, seed=1)
+    ids = np.arange(n, dtype=np.int32)
+    sb, _ = check_against_oracle(bx, ids, 16, 0)
+    assert np.array_equal(sb[4], ids)
+
+
+@pytest.mark.parametrize("n", [1, 2, 15, 16, 17, 4095, 4097, 100_003])
+def test_ref32_integer_coordinates_signed_order(n):
+    r = np.random.default_rng(n)
+    x = r.integers(0, 65536, n).astype(np.float32)
+    y = r.integers(0, 65536, n).astype(np.float32)
+    x[: n // 3] = x[0]  # many ties
+    bx = (x - 0.5, y - 0.5, x + 0.5, y + 0.5)
+    ids = r.permutation(n).astype(np.int32)
+    check_against_oracle(bx, ids, 16, 0)
+
+
+@pytest.mark.parametrize("fanout", [2, 3, 4, 8, 10, 16, 32, 64])
+def test_morton64_fanouts(fanout):
+    n = 200_001
+    bx = datagen.mixture_boxes(n, seed=fanout)
+    ids = np.random.default_rng(fanout).permutation(n).astype(np.int32)
+    check_against_oracle(bx, ids, fanout, 1)
+
+
+def test_morton64_one_million_uniform():
+    n = 1_000_000
+    bx = datagen.uniform_boxes(n, seed=21)
+    check_against_oracle(bx, np.arange(n, dtype=np.int32), 16, 1)
+
+
+def test_special_values():
+    n = 50_000
+    bx = [a.copy() for a in datagen.uniform_boxes(n, seed=22, wmax=1e-2)]
+    r = np.random.default_rng(22)
+    for k, v in ((0, np.nan), (2, np.nan), (1, -0.0), (3, 0.0), (0, -0.0), (2, np.inf)):
+        idx = r.integers(0, n, 300)
+        bx[k][idx] = v
+    # degenerate points and identical boxes
+    bx[2][:1000] = bx[0][:1000]
+    bx[3][:1000] = bx[1][:1000]
+    for km in (0, 1):
+        for f in (16, 10):
+            check_against_oracle(tuple(bx), np.arange(n, dtype=np.int32), f, km)
+
+
+def test_all_equal_keys_and_tiny():
+    for n in (1, 2, 3, 64, 65, 4096, 4097):
+        bx = tuple(np.full(n, v, np.float32) for v in (0.25, 0.25, 0.5, 0.5))
+        for km in (0, 1):
+            for f in (2, 16, 64):
+                sb, _ = check_against_oracle(bx, np.arange(n, dtype=np.int32)[::-1].copy(), f, km)
+
+
+def test_empty_input():
+    import curtree_amd as ct
+
+    d = ct.DeviceBoxes(0)
+    rt = ct.build_rtree(d, 16, 1)
+    assert rt.sz == 0
+
+
+def test_large_ragged_properties():
+    # 2^24 + 1 boxes: beyond the reference's float level-count range
+    n = (1 << 24) + 1
+    bx = datagen.mixture_boxes(n, seed=23)
+    ids = np.arange(n, dtype=np.int32)
+    sb, t, sz = gpu_build(bx, ids, 16, 1)
+    lv, rs = orc.level_plan(n, 16)
+    assert sz == rs and lv[0] == 1_048_577
+    # the ids are a permutation and the keys of the sorted boxes are sorted
+    assert np.array_equal(np.sort(sb[4]), ids)
+    k = orc.keys_morton64(*sb[:4], ext=orc.centroid_extent(*bx))
+    assert np.all(k[1:] >= k[:-1])
+    # leaf-parent MBRs are the min/max of their boxes
+    start = rs - lv[0]
+    g = np.arange(0, n, 16)
+    assert np.array_equal(t["xmin"][start:], np.minimum.reduceat(sb[0], g))
+    assert np.array_equal(t["ymax"][start:], np.maximum.reduceat(sb[3], g))
+    # the permuted payload travels with its box
+    o = np.argsort(sb[4])
+    for i in range(4):
+        assert np.array_equal(sb[i][o], bx[i])

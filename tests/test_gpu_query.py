@@ -1,0 +1,177 @@
+"""GPU parity of the window query (through the C ABI).
+
+Bar: bit-exact integer results.  The raw pair list must equal the oracle's
+BFS output element for element (same (query, sorted box position) order the
+reference produces before its host sort), and the canonical (query id, box id)
+sorted pair set must equal brute force (src/utility.h:22-45) for every query
+checked."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle_py as orc
+from curtree_amd import datagen
+
+pytestmark = pytest.mark.gpu
+
+
+def setup(boxes, ids, windows, fanout, key_mode):
+    import curtree_amd as ct
+
+    d = ct.DeviceBoxes.from_host(*boxes, ids)
+    rt = ct.build_rtree(d, fanout, key_mode)
+    q = ct.DeviceBoxes.from_host(*windows[:4])
+    return d, rt, q
+
+
+def oracle_tree(boxes, ids, fanout, key_mode):
+    return orc.build_rtree(*boxes, ids, fanout, key_mode)
+
+
+def run_case(boxes, ids, windows, fanout, key_mode, brute_sample=None):
+    import curtree_amd as ct
+
+    d, rt, q = setup(boxes, ids, windows, fanout, key_mode)
+    fid, tid = ct.bfs_frontier(rt, d, q)
+    sb, t = oracle_tree(boxes, ids, fanout, key_mode)
+    ofid, otid, cnt = orc.bfs(t, sb, windows)
+    assert fid.shape == ofid.shape, (fid.shape, ofid.shape)
+    assert np.array_equal(tid, otid) and np.array_equal(fid, ofid)
+    nq = windows[0].shape[0]
+    sel = np.arange(nq) if brute_sample is None else np.sort(
+        np.random.default_rng(0).choice(nq, min(brute_sample, nq), replace=False))
+    sub = tuple(w[sel] for w in windows[:4])
+    bf, bt = orc.brute_force(tuple(boxes) + (ids,), sub)
+    bt = sel[bt].astype(np.int32)
+    m = np.isin(tid, sel)
+    gf, gt = orc.canonical(fid[m], tid[m])
+    cf, ct_ = orc.canonical(bf, bt)
+    assert np.array_equal(gt, ct_) and np.array_equal(gf, cf)
+    rt.free()
+    d.free()
+    q.free()
+    return fid, tid, cnt
+
+
+def test_plain_workload_one_million_boxes():
+    # 1e6 uniform boxes, F=16, 1e4 windows: the full CPU-runnable workload
+    n, nq = 1_000_000, 10_000
+    bx = datagen.uniform_boxes(n, seed=1)
+    qs = datagen.uniform_windows(nq, seed=2)
+    fid, tid, (X, T, H) = run_case(bx, np.arange(n, dtype=np.int32), qs, 16, 0, brute_sample=1000)
+    assert H == fid.shape[0] and H > 300_000
+    # same pair set with 64-bit Morton keys (different tree, same answer)
+    fid2, tid2, _ = run_case(bx, np.arange(n, dtype=np.int32), qs, 16, 1, brute_sample=200)
+    a = orc.canonical(fid, tid)
+    b = orc.canonical(fid2, tid2)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("fanout", [2, 3, 4, 8, 10, 16, 32, 64])
+def test_fanouts_clustered(fanout):
+    n = 60_000
+    bx = datagen.clustered_boxes(n, seed=30 + fanout, frac=0.3, sigma=0.01, wmax=0.005)
+    ids = np.random.default_rng(fanout).permutation(n).astype(np.int32)
+    qs = datagen.loguniform_windows(2000, seed=40 + fanout, lo=1e-4, hi=0.1)
+    run_case(bx, ids, qs, fanout, 1)
+
+
+def test_huge_windows_never_overflow():
+    # windows that would exceed the reference's 1024-slot LDS queue (it aborts)
+    n = 100_000
+    bx = datagen.uniform_boxes(n, seed=50)
+    qs = datagen.uniform_windows(64, seed=51, smax=1.0)
+    qs = (qs[0] - 0.3, qs[1] - 0.3, qs[2] + 0.3, qs[3] + 0.3)
+    fid, tid, (X, T, H) = run_case(bx, np.arange(n, dtype=np.int32), qs, 16, 1, brute_sample=8)
+    assert H > 64 * 1024
+
+
+def test_touching_points_and_specials():
+    n = 20_000
+    r = np.random.default_rng(60)
+    x = r.integers(0, 100, n).astype(np.float32) / 100
+    y = r.integers(0, 100, n).astype(np.float32) / 100
+    bx = (x, y, x, y)  # points on a grid: windows touch them exactly
+    g = np.arange(0, 100, 7, dtype=np.float32) / 100
+    qx, qy = np.meshgrid(g, g)
+    qx, qy = qx.ravel(), qy.ravel()
+    qs = (qx, qy, qx + np.float32(0.05), qy + np.float32(0.05))
+    for km in (0, 1):
+        run_case(bx, np.arange(n, dtype=np.int32), qs, 8, km)
+
+
+def test_empty_cases():
+    import curtree_amd as ct
+
+    bx = datagen.uniform_boxes(1000, seed=61)
+    d = ct.DeviceBoxes.from_host(*bx)
+    rt = ct.build_rtree(d, 16, 1)
+    q0 = ct.DeviceBoxes(0)
+    fid, tid = ct.bfs_frontier(rt, d, q0)
+    assert fid.size == 0
+    far = ct.DeviceBoxes.from_host(*(np.full(5, v, np.float32) for v in (5, 5, 6, 6)))
+    fid, tid = ct.bfs_frontier(rt, d, far)
+    assert fid.size == 0
+    one = ct.DeviceBoxes.from_host(*(np.array([v], np.float32) for v in (0.1, 0.1, 0.2, 0.2)),
+                                   np.array([42], np.int32))
+    rt1 = ct.build_rtree(one, 16, 0)
+    assert rt1.sz == 1
+    qa = ct.DeviceBoxes.from_host(*(np.array([v, 9], np.float32) for v in (0.0, 0.0, 0.1, 0.1)))
+    fid, tid = ct.bfs_frontier(rt1, one, qa)
+    assert fid.tolist() == [42] and tid.tolist() == [0]
+
+
+def test_query_into_and_counts():
+    import curtree_amd as ct
+
+    n = 50_000
+    bx = datagen.uniform_boxes(n, seed=70, wmax=0.01)
+    qs = datagen.uniform_windows(3000, seed=71, smax=0.05)
+    d, rt, q = setup(bx, np.arange(n, dtype=np.int32), qs, 16, 1)
+    total, counts = ct.query_count(rt, d, q, per_query=True)
+    fid, tid = ct.bfs_frontier(rt, d, q)
+    assert total == fid.size
+    assert np.array_equal(counts, np.bincount(tid, minlength=3000))
+    # too-small caller buffer: nothing written, the needed size is reported
+    p = ctypes.c_void_p()
+    ct._check(ct.lib.curtree_malloc(ctypes.byref(p), 8 * (total + 1)))
+    need = ct.query_into(rt, d, q, p.value, p.value + 4 * (total + 1), total - 1)
+    assert need == -total
+    got = ct.query_into(rt, d, q, p.value, p.value + 4 * (total + 1), total + 1)
+    assert got == total
+    f2 = ct._host_from_dev(p.value, total, np.int32)
+    t2 = ct._host_from_dev(p.value + 4 * (total + 1), total, np.int32)
+    assert np.array_equal(f2, fid) and np.array_equal(t2, tid)
+    ct.lib.curtree_free(p)
+
+
+def test_device_canonical_sort():
+    import curtree_amd as ct
+
+    n = 200_000
+    bx = datagen.mixture_boxes(n, seed=80)
+    ids = np.random.default_rng(80).permutation(n).astype(np.int32) - n // 2  # signed ids
+    qs = datagen.loguniform_windows(5000, seed=81, lo=1e-3, hi=0.05)
+    d, rt, q = setup(bx, ids, qs, 16, 1)
+    p = ct.query(rt, d, q)
+    raw_f, raw_t = p.to_host()
+    ct.canonical_sort(p)
+    f, t = p.to_host()
+    cf, ctt = orc.canonical(raw_f, raw_t)
+    assert np.array_equal(t, ctt) and np.array_equal(f, cf)
+    p.free()
+
+
+def test_repeated_calls_cycle_chained_scan_epochs():
+    # > 127 chained launches: the status words must be recycled correctly
+    import curtree_amd as ct
+
+    n = 5000
+    bx = datagen.uniform_boxes(n, seed=90, wmax=0.02)
+    qs = datagen.uniform_windows(100, seed=91, smax=0.05)
+    d, rt, q = setup(bx, np.arange(n, dtype=np.int32), qs, 4, 1)
+    ref = ct.bfs_frontier(rt, d, q)
+    for _ in range(60):  # 5 levels + 1 leaf pass per call
+        got = ct.bfs_frontier(rt, d, q)
+        assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
