@@ -67,6 +67,9 @@ def _load():
         "curtree_build_into_f32": (I, [P(BBox), P(RTreeStruct), I, V]),
         "curtree_tree_size": (I, [I64, I, P(I64), P(I)]),
         "curtree_rt_d_alloc": (I, [P(RTreeStruct), I]),
+        "curtree_read_mbr_csv": (I, [ctypes.c_char_p, I, ctypes.c_float, V, V, V, V, V]),
+        "curtree_tree_save": (I, [ctypes.c_char_p, P(RTreeStruct), P(BBox)]),
+        "curtree_tree_load": (I, [ctypes.c_char_p, P(RTreeStruct), P(BBox)]),
         "curtree_query_f32": (I, [P(RTreeStruct), P(BBox), P(BBox), P(IdPair64), I, V]),
         "curtree_query_into_f32": (I, [P(RTreeStruct), P(BBox), P(BBox), V, V, I64, P(I64), I, V]),
         "curtree_query_count_f32": (I, [P(RTreeStruct), P(BBox), P(BBox), V, P(I64), V]),
@@ -317,6 +320,26 @@ def canonical_sort(pairs, stream=None):
     (the host sorts of src/rtree_test_frontier.cu:102-105, on the device)."""
     _check(lib.curtree_pairs_canonical_sort(pairs.s.fid, pairs.s.tid, pairs.n, stream))
     return pairs
+
+
+def read_mbr(path, size, r=0.0):
+    """readMBR (src/utility.cpp:5-40): host arrays (xmin, ymin, xmax, ymax, id)."""
+    out = [np.empty(int(size), np.float32) for _ in range(4)] + [np.empty(int(size), np.int32)]
+    _check(lib.curtree_read_mbr_csv(os.fsencode(path), int(size), float(r),
+                                    *[a.ctypes.data for a in out]))
+    return tuple(out)
+
+
+def save_tree(path, rt, rbox):
+    _check(lib.curtree_tree_save(os.fsencode(path), ctypes.byref(rt.s), ctypes.byref(rbox.s)))
+
+
+def load_tree(path):
+    """Returns (RTree, DeviceBoxes) read from a file written by save_tree."""
+    rt = RTree(2)
+    boxes = DeviceBoxes(0)
+    _check(lib.curtree_tree_load(os.fsencode(path), ctypes.byref(rt.s), ctypes.byref(boxes.s)))
+    return rt, boxes
 
 
 def bfs_frontier(rt, rbox, qbox):

@@ -100,10 +100,13 @@ __global__ __launch_bounds__(kQThreads) void k_expand(int64_t n_in, const uint2*
   __shared__ uint64_t s_wtot[4];
   __shared__ uint64_t s_base;
   const int t = threadIdx.x, w = t >> 6, lane = lane_id();
+  const int sub = lane / G, c = lane % G;
+  for (;;) {  // tiles by ticket until all are taken (grid may be smaller)
+  __syncthreads();  // the previous tile is done with s_tile, s_wtot, s_base
   if (t == 0) s_tile = atomicAdd(&ctl->ticket[epoch], 1u);
   __syncthreads();
   const int64_t tile = s_tile;
-  const int sub = lane / G, c = lane % G;
+  if (tile >= ntiles) return;
   uint64_t masks[kQSteps];
   uint32_t qk[kQSteps], ck[kQSteps];
   uint64_t wcount = 0;
@@ -146,6 +149,7 @@ __global__ __launch_bounds__(kQThreads) void k_expand(int64_t n_in, const uint2*
     }
     b += (uint64_t)popc64(m);
   }
+  }
 }
 
 // Leaf level, count pass: test the boxes of each leaf-parent entry, store the
@@ -160,10 +164,13 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_count(
   __shared__ uint64_t s_wtot[4];
   __shared__ uint64_t s_base;
   const int t = threadIdx.x, w = t >> 6, lane = lane_id();
+  const int sub = lane / G, c = lane % G;
+  for (;;) {  // tiles by ticket until all are taken (grid may be smaller)
+  __syncthreads();  // the previous tile is done with s_tile, s_wtot, s_base
   if (t == 0) s_tile = atomicAdd(&ctl->ticket[epoch], 1u);
   __syncthreads();
   const int64_t tile = s_tile;
-  const int sub = lane / G, c = lane % G;
+  if (tile >= ntiles) return;
   uint64_t masks[kQSteps];
   uint64_t wcount = 0;
   const int64_t ebase = tile * S::kPerTile + (int64_t)w * S::kPerWave + sub;
@@ -210,6 +217,7 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_count(
     }
     b += (uint64_t)popc64(m);
   }
+  }
 }
 
 // Leaf level, write pass: G lanes per entry; lane c writes pair for box c if
@@ -222,28 +230,36 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_write(int64_t n_in, const ui
                                                           const uint64_t* __restrict__ eoff,
                                                           int* __restrict__ out_fid,
                                                           int* __restrict__ out_tid) {
-  const int64_t gt = blockIdx.x * (int64_t)kQThreads + threadIdx.x;
-  const int64_t e = gt / G;
-  const int c = (int)(gt % G);
-  if (e >= n_in) return;
-  const uint64_t m = emask[e];
-  if (!((m >> c) & 1ull)) return;
-  uint32_t node = 0, q;
-  if (fin) {
-    uint2 en = fin[e];
-    q = en.x;
-    node = en.y;
-  } else {
-    q = (uint32_t)e;
+  const int64_t total = n_in * G;
+  for (int64_t gt = blockIdx.x * (int64_t)kQThreads + threadIdx.x; gt < total;
+       gt += (int64_t)gridDim.x * kQThreads) {
+    const int64_t e = gt / G;
+    const int c = (int)(gt % G);
+    const uint64_t m = emask[e];
+    if (!((m >> c) & 1ull)) continue;
+    uint32_t node = 0, q;
+    if (fin) {
+      uint2 en = fin[e];
+      q = en.x;
+      node = en.y;
+    } else {
+      q = (uint32_t)e;
+    }
+    const uint64_t o = eoff[e] + (uint64_t)popc64(m & ((1ull << c) - 1ull));
+    const int64_t bi = (int64_t)tr.pos[node] - r_sz + c;
+    out_fid[o] = rid[bi];
+    out_tid[o] = (int)q;
   }
-  const uint64_t o = eoff[e] + (uint64_t)popc64(m & ((1ull << c) - 1ull));
-  const int64_t bi = (int64_t)tr.pos[node] - r_sz + c;
-  out_fid[o] = rid[bi];
-  out_tid[o] = (int)q;
 }
 
 // ---------------------------------------------------------------------------
 // Host side.
+// Blocks for a ticketed chained launch: enough to fill the chip; each block
+// loops over tiles, so the grid stays far below the 2^32 work-item limit.
+static unsigned chain_grid(int64_t ntiles) {
+  return (unsigned)std::min<int64_t>(ntiles, 8192);
+}
+
 static int next_pow2_group(int F) {
   int g = 2;
   while (g < F) g <<= 1;
@@ -312,7 +328,7 @@ static int run_query(Context& ctx, const QTree& tr, const QBoxes& qb, const QBox
       uint32_t epoch;
       if ((rc = chain_epoch(ctx, s, ntiles, &epoch))) return rc;
       pf.mark(s, "k_expand");
-      hipLaunchKernelGGL(k_expand<G>, dim3((unsigned)ntiles), dim3(kQThreads), 0, s, ncur, cur, tr,
+      hipLaunchKernelGGL(k_expand<G>, dim3(chain_grid(ntiles)), dim3(kQThreads), 0, s, ncur, cur, tr,
                          qb, (uint2*)ws.ptr(out_slot), cap, (uint64_t*)ws.ptr(WsQStatus), epoch,
                          (QCtrl*)ws.ptr(WsQCtrl), ntiles, r_sz);
       pf.mark(s, nullptr);
@@ -346,7 +362,7 @@ static int run_query(Context& ctx, const QTree& tr, const QBoxes& qb, const QBox
   if (qo->counts)
     hipLaunchKernelGGL(k_fill_u64, dim3(256), dim3(256), 0, s, (uint64_t*)qo->counts, Q, 0ull);
   pf.mark(s, "k_leaf_count");
-  hipLaunchKernelGGL(k_leaf_count<G>, dim3((unsigned)ntiles), dim3(kQThreads), 0, s, ncur, cur, tr,
+  hipLaunchKernelGGL(k_leaf_count<G>, dim3(chain_grid(ntiles)), dim3(kQThreads), 0, s, ncur, cur, tr,
                      qb, rb, r_sz, nbox, (uint64_t*)ws.ptr(WsQMask), (uint64_t*)ws.ptr(WsQOff),
                      (unsigned long long*)qo->counts, (uint64_t*)ws.ptr(WsQStatus), epoch,
                      (QCtrl*)ws.ptr(WsQCtrl), ntiles);
@@ -373,8 +389,9 @@ static int run_query(Context& ctx, const QTree& tr, const QBoxes& qb, const QBox
   qo->out_fid = fid;
   qo->out_tid = tid;
   const int64_t threads = ncur * G;
+  const unsigned wgrid = (unsigned)std::min<int64_t>((threads + kQThreads - 1) / kQThreads, 1 << 16);
   pf.mark(s, "k_leaf_write");
-  hipLaunchKernelGGL(k_leaf_write<G>, dim3((unsigned)((threads + kQThreads - 1) / kQThreads)),
+  hipLaunchKernelGGL(k_leaf_write<G>, dim3(wgrid),
                      dim3(kQThreads), 0, s, ncur, cur, tr, rid, r_sz, (const uint64_t*)ws.ptr(WsQMask),
                      (const uint64_t*)ws.ptr(WsQOff), fid, tid);
   pf.mark(s, nullptr);

@@ -130,6 +130,23 @@ CURTREE_API int curtree_query_count_f32(const curtree_rtree_f32* rt, const curtr
  * (as every query entry point returns them). */
 CURTREE_API int curtree_pairs_canonical_sort(int* fid, int* tid, int64_t n, void* stream);
 
+/* ---- ingest and persistence --------------------------------------------- */
+
+/* Replaces readMBR (src/utility.cpp:5-40): reads `size` lines
+ * "x1,y1,x2,y2,id,pid,nume" into host arrays the caller allocated; box =
+ * [min(x1,x2)-r, max(x1,x2)+r] x [min(y1,y2)-r, max(y1,y2)+r].  Open or parse
+ * failures return CURTREE_EINVAL (the reference exits / throws).  Host only. */
+CURTREE_API int curtree_read_mbr_csv(const char* path, int size, float r, float* xmin, float* ymin,
+                                     float* xmax, float* ymax, int* id);
+
+/* Write the sorted boxes and node arrays of one tree to a binary file / read
+ * them back into newly allocated device arrays (no reference counterpart: the
+ * reference never persists a tree; used to keep serving trees on disk). */
+CURTREE_API int curtree_tree_save(const char* path, const curtree_rtree_f32* rt,
+                                  const curtree_bbox_f32* rbox);
+CURTREE_API int curtree_tree_load(const char* path, curtree_rtree_f32* rt,
+                                  curtree_bbox_f32* rbox);
+
 /* ---- memory helpers (src/utility.cuh:16-205, over hipMalloc/hipFree) ----- */
 
 /* box_h2d: allocate device arrays for h->sz boxes and copy them in */

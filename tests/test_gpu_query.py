@@ -175,3 +175,22 @@ def test_repeated_calls_cycle_chained_scan_epochs():
     for _ in range(60):  # 5 levels + 1 leaf pass per call
         got = ct.bfs_frontier(rt, d, q)
         assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+
+
+def test_tree_save_load_round_trip(tmp_path):
+    import curtree_amd as ct
+
+    n = 30_000
+    bx = datagen.mixture_boxes(n, seed=95)
+    qs = datagen.loguniform_windows(500, seed=96, lo=1e-3, hi=0.05)
+    d, rt, q = setup(bx, np.arange(n, dtype=np.int32), qs, 16, 1)
+    ref = ct.bfs_frontier(rt, d, q)
+    path = str(tmp_path / "tree.ctree")
+    ct.save_tree(path, rt, d)
+    rt2, d2 = ct.load_tree(path)
+    assert rt2.sz == rt.sz and rt2.fanout == 16 and d2.n == n
+    a, b = rt.to_host(), rt2.to_host()
+    for k in a:
+        assert np.array_equal(a[k], b[k])
+    got = ct.bfs_frontier(rt2, d2, q)
+    assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
