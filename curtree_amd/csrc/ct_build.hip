@@ -33,8 +33,10 @@ __device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* s_t
 
 // ---------------------------------------------------------------------------
 // Control block layout (device memory, zeroed by k_build_init).
+constexpr int kExtSlots = 64;  // extent partials are spread over slots to avoid atomic contention
+
 struct BuildCtrl {
-  uint32_t ext[4];                 // ordered-uint lox, loy, hix, hiy of centroids
+  uint32_t ext[kExtSlots][4];      // ordered-uint lox, loy, hix, hiy of centroids, spread
   uint32_t hist[kMaxDigits * 256];  // global digit histograms
   uint32_t base[kMaxDigits * 256];  // exclusive digit offsets
   uint32_t ticket[kMaxDigits];
@@ -56,11 +58,13 @@ __global__ __launch_bounds__(256) void k_build_init(BuildCtrl* c) {
     c->src[threadIdx.x] = 0;
     c->write_keys[threadIdx.x] = 0;
   }
+  if (threadIdx.x < kExtSlots) {
+    c->ext[threadIdx.x][0] = 0xFFFFFFFFu;
+    c->ext[threadIdx.x][1] = 0xFFFFFFFFu;
+    c->ext[threadIdx.x][2] = 0u;
+    c->ext[threadIdx.x][3] = 0u;
+  }
   if (threadIdx.x == 0) {
-    c->ext[0] = 0xFFFFFFFFu;
-    c->ext[1] = 0xFFFFFFFFu;
-    c->ext[2] = 0u;
-    c->ext[3] = 0u;
     c->npass = 0;
     c->first = -1;
   }
@@ -92,12 +96,46 @@ __global__ __launch_bounds__(256) void k_extent(int64_t n, const float* __restri
     hx = max(hx, (uint32_t)__shfl_xor(hx, o));
     hy = max(hy, (uint32_t)__shfl_xor(hy, o));
   }
+  __shared__ uint32_t s_e[4][4];
+  const int w = threadIdx.x >> 6;
   if (lane_id() == 0) {
-    atomicMin(&c->ext[0], lx);
-    atomicMin(&c->ext[1], ly);
-    atomicMax(&c->ext[2], hx);
-    atomicMax(&c->ext[3], hy);
+    s_e[w][0] = lx;
+    s_e[w][1] = ly;
+    s_e[w][2] = hx;
+    s_e[w][3] = hy;
   }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int k = threadIdx.x;
+    uint32_t v = s_e[0][k];
+    for (int i = 1; i < 4; i++) v = k < 2 ? min(v, s_e[i][k]) : max(v, s_e[i][k]);
+    uint32_t* slot = &c->ext[blockIdx.x % kExtSlots][k];
+    if (k < 2) atomicMin(slot, v);
+    else atomicMax(slot, v);
+  }
+}
+
+// Final extent from the spread slots (every thread of the block gets it).
+__device__ __forceinline__ void read_extent(const BuildCtrl* c, uint32_t* out4) {
+  __shared__ uint32_t s_x[4];
+  if (threadIdx.x < 64) {
+    uint32_t lx = c->ext[threadIdx.x][0], ly = c->ext[threadIdx.x][1];
+    uint32_t hx = c->ext[threadIdx.x][2], hy = c->ext[threadIdx.x][3];
+    for (int o = 32; o > 0; o >>= 1) {
+      lx = min(lx, (uint32_t)__shfl_xor(lx, o));
+      ly = min(ly, (uint32_t)__shfl_xor(ly, o));
+      hx = max(hx, (uint32_t)__shfl_xor(hx, o));
+      hy = max(hy, (uint32_t)__shfl_xor(hy, o));
+    }
+    if (threadIdx.x == 0) {
+      s_x[0] = lx;
+      s_x[1] = ly;
+      s_x[2] = hx;
+      s_x[3] = hy;
+    }
+  }
+  __syncthreads();
+  for (int k = 0; k < 4; k++) out4[k] = s_x[k];
 }
 
 // Extent -> (lo, scale) per axis, the same FP32 operations as the oracle.
@@ -132,8 +170,10 @@ __global__ __launch_bounds__(256) void k_keygen(int64_t n, const float* __restri
   for (int t = threadIdx.x; t < D * 256; t += 256) sh[t] = 0;
   float lox = 0.f, loy = 0.f, sx = 0.f, sy = 0.f;
   if (MODE == kKeyMorton64) {
-    axis_params(c->ext[0], c->ext[2], &lox, &sx);
-    axis_params(c->ext[1], c->ext[3], &loy, &sy);
+    uint32_t e4[4];
+    read_extent(c, e4);
+    axis_params(e4[0], e4[2], &lox, &sx);
+    axis_params(e4[1], e4[3], &loy, &sy);
   }
   __syncthreads();
   const int64_t stride = (int64_t)gridDim.x * 256;
