@@ -91,6 +91,7 @@ def _load():
         "curtree_version": (ctypes.c_char_p, []),
         "curtree_device_info": (I, [P(I), ctypes.c_char_p, I, P(I), P(I64)]),
         "curtree_release_workspace": (I, []),
+        "curtree_set_option": (I, [ctypes.c_char_p, I64]),
         "curtree_prof_enable": (I, [I]),
         "curtree_prof_reset": (I, []),
         "curtree_prof_get": (I, [ctypes.c_char_p, P(ctypes.c_double), P(I64)]),
@@ -128,6 +129,11 @@ def _host_from_dev(ptr, n, dtype):
     if out.nbytes:
         _check(lib.curtree_memcpy_d2h(out.ctypes.data, ctypes.c_void_p(ptr), out.nbytes))
     return out
+
+
+def set_option(name, value):
+    """Process-wide tuning knob (see curtree_set_option in curtree_c.h)."""
+    _check(lib.curtree_set_option(name.encode(), int(value)))
 
 
 def synchronize(stream=None):

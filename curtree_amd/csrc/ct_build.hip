@@ -263,6 +263,7 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
   __shared__ uint32_t s_tmp[4];
   __shared__ uint32_t s_tile;
   __shared__ KT s_buf[kSortTile];
+  __shared__ uint8_t s_dig[kSortTile];
 
   if (!c->active[pass]) return;
   const int t = threadIdx.x, w = t >> 6, lane = lane_id();
@@ -282,14 +283,11 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
   const int64_t tbase = tile * kSortTile;
   const int64_t wbase = tbase + (int64_t)w * (kWave * kSortItems);
   KT key[kSortItems];
-  uint32_t val[kSortItems];
-  uint32_t rank[kSortItems];
+  uint32_t pos[kSortItems];  // rank within (wave, digit), later the tile-local position
 #pragma unroll
   for (int i = 0; i < kSortItems; i++) {
     int64_t idx = wbase + i * kWave + lane;
-    bool valid = idx < n;
-    key[i] = valid ? kin[idx] : (KT)0;
-    val[i] = (valid && !keys_only) ? (ident ? (uint32_t)idx : vin[idx]) : 0u;
+    key[i] = idx < n ? kin[idx] : (KT)0;
   }
   const uint64_t lt = lanemask_lt();
 #pragma unroll
@@ -304,7 +302,7 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
       m &= ((d >> b) & 1u) ? bb : ~bb;
     }
     uint32_t old = s_wh[w][d];
-    rank[i] = old + (uint32_t)popc64(m & lt);
+    pos[i] = old + (uint32_t)popc64(m & lt);
     if (valid && (m & lt) == 0) s_wh[w][d] = old + (uint32_t)popc64(m);
   }
   __syncthreads();
@@ -323,13 +321,32 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
     st_agent(st + dg, kFlagP | tot);
   } else {
     st_agent(st + dg, kFlagA | tot);
-    for (int64_t j = tile - 1; j >= 0; j--) {
-      uint32_t wv;
-      do {
-        wv = ld_agent(status + (size_t)j * 256 + dg);
-      } while ((wv & ~kValMask) == 0u);
-      excl += wv & kValMask;
-      if ((wv & ~kValMask) == kFlagP) break;
+    // look back 4 tiles per step (4 loads in flight); stop at the nearest
+    // inclusive prefix; tiles before the first count as prefix 0
+    int64_t j = tile - 1;
+    for (;;) {
+      uint32_t v[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        v[k] = (j - k >= 0) ? ld_agent(status + (size_t)(j - k) * 256 + dg) : kFlagP;
+      int stop = 4;
+      bool ready = true;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        if (stop < 4) break;
+        uint32_t f = v[k] & ~kValMask;
+        if (f == 0u) {
+          ready = false;
+          break;
+        }
+        if (f == kFlagP) stop = k;
+      }
+      if (!ready) continue;
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        if (k <= stop) excl += v[k] & kValMask;
+      if (stop < 4) break;
+      j -= 4;
     }
     st_agent(st + dg, kFlagP | (excl + tot));
   }
@@ -340,42 +357,46 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
   __syncthreads();
 
   // local reorder of keys through LDS, then scatter runs of equal digits
-  uint32_t lpos[kSortItems];
 #pragma unroll
   for (int i = 0; i < kSortItems; i++) {
     int64_t idx = wbase + i * kWave + lane;
     uint32_t d = (uint32_t)(key[i] >> shift) & 255u;
-    lpos[i] = s_start[d] + s_wh[w][d] + rank[i];
-    if (idx < n) s_buf[lpos[i]] = key[i];
+    pos[i] += s_start[d] + s_wh[w][d];
+    if (idx < n) s_buf[pos[i]] = key[i];
   }
   __syncthreads();
   const int64_t rem = n - tbase;
   const int tn = rem < kSortTile ? (int)rem : kSortTile;
-  uint32_t gpos[kSortItems];
 #pragma unroll
   for (int i = 0; i < kSortItems; i++) {
     int j = t + i * kSortThreads;
-    gpos[i] = 0;
     if (j < tn) {
       KT kk = s_buf[j];
       uint32_t d = (uint32_t)(kk >> shift) & 255u;
-      gpos[i] = s_adj[d] + (uint32_t)j;
-      if (wk) kout[gpos[i]] = kk;
+      s_dig[j] = (uint8_t)d;
+      if (wk) kout[s_adj[d] + (uint32_t)j] = kk;
     }
   }
   if (keys_only) return;
+  // values: load now (after the keys are out), reorder through LDS, scatter
+  uint32_t val[kSortItems];
+#pragma unroll
+  for (int i = 0; i < kSortItems; i++) {
+    int64_t idx = wbase + i * kWave + lane;
+    val[i] = idx < n ? (ident ? (uint32_t)idx : vin[idx]) : 0u;
+  }
   __syncthreads();
   uint32_t* s_val = (uint32_t*)s_buf;
 #pragma unroll
   for (int i = 0; i < kSortItems; i++) {
     int64_t idx = wbase + i * kWave + lane;
-    if (idx < n) s_val[lpos[i]] = val[i];
+    if (idx < n) s_val[pos[i]] = val[i];
   }
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < kSortItems; i++) {
     int j = t + i * kSortThreads;
-    if (j < tn) vout[gpos[i]] = s_val[j];
+    if (j < tn) vout[s_adj[s_dig[j]] + (uint32_t)j] = s_val[j];
   }
 }
 

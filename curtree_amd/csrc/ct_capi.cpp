@@ -431,6 +431,18 @@ CURTREE_API int curtree_release_workspace(void) {
   return CURTREE_OK;
 }
 
+CURTREE_API int curtree_set_option(const char* name, int64_t value) {
+  Context& c = context();
+  std::lock_guard<std::mutex> g(c.mu);
+  if (!name) return set_error(CURTREE_EINVAL, "set_option: null name");
+  if (!strcmp(name, "containment")) {
+    if (value < -1 || value > 1) return set_error(CURTREE_EINVAL, "containment: -1, 0 or 1");
+    c.opt_contain = (int)value;
+    return CURTREE_OK;
+  }
+  return set_error(CURTREE_EINVAL, "set_option: unknown option %s", name);
+}
+
 CURTREE_API int curtree_prof_enable(int on) {
   Context& c = context();
   std::lock_guard<std::mutex> g(c.mu);

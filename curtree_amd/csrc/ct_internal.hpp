@@ -60,6 +60,7 @@ enum WsSlot {
   WsQCtrl,
   WsQMask,
   WsQOff,
+  WsQSafe,
   WsSortKeys0,
   WsSortKeys1,
   WsSortVals0,
@@ -102,6 +103,8 @@ struct Context {
   Prof prof;
   uint32_t qepoch = 0;  // query chained-scan epoch, cycles through [0, kEpochs)
   size_t qstatus_words = 0;
+  int qfanout = 0;       // fanout of the tree being queried
+  int opt_contain = -1;  // containment shortcut: -1 auto, 0 off, 1 on
 };
 
 Context& context();
@@ -118,6 +121,7 @@ struct QueryOut {
   bool alloc = false;         // allocate exact-size output
   int64_t* counts = nullptr;  // optional per-query counts (device)
   bool count_only = false;
+  bool contain = true;        // use the containment shortcut (set by the entry point)
   // results
   int64_t n = 0;
   int* out_fid = nullptr;

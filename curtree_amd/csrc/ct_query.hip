@@ -34,7 +34,21 @@ struct QTree {
   const float* y1;
   const int* pos;
   const int* len;
+  const uint8_t* safe;  // per node: every box below is well-formed (or null: no shortcut)
 };
+
+// Frontier entries are (query, node | kFull).  kFull marks a node whose MBR
+// lies inside the query window and whose boxes are all well-formed
+// (xmin <= xmax, ymin <= ymax): every box below it then intersects the window
+// (box inside its leaf-parent MBR inside the window), so its subtree is
+// expanded without reading child MBRs or boxes.  Results are identical to
+// testing every child, as the reference does.
+constexpr uint32_t kFull = 0x80000000u;
+
+__device__ __forceinline__ bool contains(float x1, float y1, float x2, float y2, float m1,
+                                         float n1, float m2, float n2) {
+  return x1 <= m1 && m2 <= x2 && y1 <= n1 && n2 <= y2;
+}
 struct QBoxes {
   const float* x0;
   const float* y0;
@@ -117,19 +131,29 @@ __global__ __launch_bounds__(kQThreads) void k_expand(int64_t n_in, const uint2*
     bool hit = false;
     uint32_t q = 0, ci = 0;
     if (e < n_in) {
-      uint32_t node = 0;
+      uint32_t node = 0, full = 0;
       if (fin) {
         uint2 en = fin[e];
         q = en.x;
-        node = en.y;
+        node = en.y & ~kFull;
+        full = en.y & kFull;
       } else {
         q = (uint32_t)e;
       }
       const int p = tr.pos[node], L = tr.len[node];
       if (c < L && (int64_t)p + L <= r_sz) {  // children must be nodes
-        ci = (uint32_t)(p + c);
-        hit = intersect(qb.x0[q], qb.y0[q], qb.x1[q], qb.y1[q], tr.x0[ci], tr.y0[ci], tr.x1[ci],
-                        tr.y1[ci]);
+        const uint32_t child = (uint32_t)(p + c);
+        if (full) {
+          hit = true;
+          ci = child | kFull;
+        } else {
+          const float wx1 = qb.x0[q], wy1 = qb.y0[q], wx2 = qb.x1[q], wy2 = qb.y1[q];
+          const float m1 = tr.x0[child], n1 = tr.y0[child], m2 = tr.x1[child], n2 = tr.y1[child];
+          hit = intersect(wx1, wy1, wx2, wy2, m1, n1, m2, n2);
+          ci = child;
+          if (hit && tr.safe && contains(wx1, wy1, wx2, wy2, m1, n1, m2, n2) && tr.safe[child])
+            ci = child | kFull;
+        }
       }
     }
     const uint64_t m = ballot(hit);
@@ -150,6 +174,30 @@ __global__ __launch_bounds__(kQThreads) void k_expand(int64_t n_in, const uint2*
     b += (uint64_t)popc64(m);
   }
   }
+}
+
+// Box c of leaf-parent entry e: does it intersect the entry's window?
+__device__ __forceinline__ bool leaf_hit(int64_t e, int c, const uint2* __restrict__ fin,
+                                         const QTree& tr, const QBoxes& qb, const QBoxes& rb,
+                                         int64_t r_sz, int64_t nbox, uint32_t* qout,
+                                         int64_t* biout) {
+  uint32_t node = 0, full = 0, q;
+  if (fin) {
+    uint2 en = fin[e];
+    q = en.x;
+    node = en.y & ~kFull;
+    full = en.y & kFull;
+  } else {
+    q = (uint32_t)e;
+  }
+  *qout = q;
+  const int p = tr.pos[node], L = tr.len[node];
+  const int64_t bi = (int64_t)p - r_sz + c;
+  *biout = bi;
+  if (c >= L || bi < 0 || bi >= nbox) return false;  // children must be boxes
+  if (full) return true;
+  return intersect(qb.x0[q], qb.y0[q], qb.x1[q], qb.y1[q], rb.x0[bi], rb.y0[bi], rb.x1[bi],
+                   rb.y1[bi]);
 }
 
 // Leaf level, count pass: test the boxes of each leaf-parent entry, store the
@@ -180,20 +228,8 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_count(
     bool hit = false;
     uint32_t q = 0;
     if (e < n_in) {
-      uint32_t node = 0;
-      if (fin) {
-        uint2 en = fin[e];
-        q = en.x;
-        node = en.y;
-      } else {
-        q = (uint32_t)e;
-      }
-      const int p = tr.pos[node], L = tr.len[node];
-      const int64_t bi = (int64_t)p - r_sz + c;
-      if (c < L && bi >= 0 && bi < nbox) {  // children must be boxes
-        hit = intersect(qb.x0[q], qb.y0[q], qb.x1[q], qb.y1[q], rb.x0[bi], rb.y0[bi], rb.x1[bi],
-                        rb.y1[bi]);
-      }
+      int64_t bi;
+      hit = leaf_hit(e, c, fin, tr, qb, rb, r_sz, nbox, &q, &bi);
     }
     const uint64_t m = ballot(hit);
     masks[k] = m;
@@ -241,7 +277,7 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_write(int64_t n_in, const ui
     if (fin) {
       uint2 en = fin[e];
       q = en.x;
-      node = en.y;
+      node = en.y & ~kFull;
     } else {
       q = (uint32_t)e;
     }
@@ -250,6 +286,95 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_write(int64_t n_in, const ui
     out_fid[o] = rid[bi];
     out_tid[o] = (int)q;
   }
+}
+
+// Leaf level, single pass into a caller buffer of `cap` pairs: test, chain
+// the tile offsets, write pairs in (query, sorted position) order.  Pairs
+// beyond `cap` are dropped; the total is always counted.
+template <int G>
+__global__ __launch_bounds__(kQThreads) void k_leaf_fused(
+    int64_t n_in, const uint2* __restrict__ fin, QTree tr, QBoxes qb, QBoxes rb,
+    const int* __restrict__ rid, int64_t r_sz, int64_t nbox, int* __restrict__ out_fid,
+    int* __restrict__ out_tid, int64_t cap, uint64_t* status, uint32_t epoch, QCtrl* ctl,
+    int64_t ntiles) {
+  using S = QShape<G>;
+  __shared__ uint32_t s_tile;
+  __shared__ uint64_t s_wtot[4];
+  __shared__ uint64_t s_base;
+  const int t = threadIdx.x, w = t >> 6, lane = lane_id();
+  const int sub = lane / G, c = lane % G;
+  for (;;) {
+  __syncthreads();
+  if (t == 0) s_tile = atomicAdd(&ctl->ticket[epoch], 1u);
+  __syncthreads();
+  const int64_t tile = s_tile;
+  if (tile >= ntiles) return;
+  uint64_t masks[kQSteps];
+  uint32_t qk[kQSteps];
+  int64_t bk[kQSteps];
+  uint64_t wcount = 0;
+  const int64_t ebase = tile * S::kPerTile + (int64_t)w * S::kPerWave + sub;
+#pragma unroll
+  for (int k = 0; k < kQSteps; k++) {
+    const int64_t e = ebase + k * S::E;
+    bool hit = false;
+    uint32_t q = 0;
+    int64_t bi = 0;
+    if (e < n_in) hit = leaf_hit(e, c, fin, tr, qb, rb, r_sz, nbox, &q, &bi);
+    const uint64_t m = ballot(hit);
+    masks[k] = m;
+    qk[k] = q;
+    bk[k] = bi;
+    wcount += (uint64_t)popc64(m);
+  }
+  uint64_t b = chain_tile(wcount, tile, ntiles, status, epoch, &ctl->total, s_wtot, &s_base);
+  const uint64_t lt = lanemask_lt();
+#pragma unroll
+  for (int k = 0; k < kQSteps; k++) {
+    const uint64_t m = masks[k];
+    if ((m >> lane) & 1ull) {
+      const uint64_t o = b + (uint64_t)popc64(m & lt);
+      if (o < (uint64_t)cap) {
+        out_fid[o] = rid[bk[k]];
+        out_tid[o] = (int)qk[k];
+      }
+    }
+    b += (uint64_t)popc64(m);
+  }
+  }
+}
+
+// Well-formedness flags (xmin <= xmax and ymin <= ymax for every box below a
+// node; NaN counts as not well-formed).  Leaf-parents from their boxes,
+// upper levels as the AND of their children.
+template <int G>
+__global__ __launch_bounds__(256) void k_safe_leaf_pow2(int64_t nbox, QBoxes rb, uint8_t* safe0) {
+  const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+  const bool wf = i >= nbox || (rb.x0[i] <= rb.x1[i] && rb.y0[i] <= rb.y1[i]);
+  const uint64_t bad = ballot(!wf);
+  const int lane = lane_id();
+  const uint64_t gm = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+  if ((lane % G) == 0 && i < nbox) safe0[i / G] = ((bad >> (lane - lane % G)) & gm) == 0 ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void k_safe_leaf_generic(int64_t nbox, int F, QBoxes rb,
+                                                           uint8_t* safe0, int64_t n0) {
+  const int64_t g = blockIdx.x * 256ll + threadIdx.x;
+  if (g >= n0) return;
+  int64_t b = g * F, e = b + F < nbox ? b + F : nbox;
+  uint8_t ok = 1;
+  for (int64_t i = b; i < e; i++) ok &= (rb.x0[i] <= rb.x1[i] && rb.y0[i] <= rb.y1[i]) ? 1 : 0;
+  safe0[g] = ok;
+}
+
+__global__ __launch_bounds__(256) void k_safe_up(int64_t cn, int F, const uint8_t* child,
+                                                 uint8_t* parent, int64_t pn) {
+  const int64_t g = blockIdx.x * 256ll + threadIdx.x;
+  if (g >= pn) return;
+  int64_t b = g * F, e = b + F < cn ? b + F : cn;
+  uint8_t ok = 1;
+  for (int64_t i = b; i < e; i++) ok &= child[i];
+  parent[g] = ok;
 }
 
 // ---------------------------------------------------------------------------
@@ -304,14 +429,56 @@ static int read_total(Context& ctx, hipStream_t s, uint64_t* v) {
   return CURTREE_OK;
 }
 
+// Per-node well-formedness flags into WsQSafe (R_sz bytes).
+static int safe_flags(Context& ctx, const QTree& tr, const QBoxes& rb, int64_t r_sz, int64_t nbox,
+                      hipStream_t s) {
+  (void)tr;
+  Workspace& ws = ctx.ws;
+  int rc;
+  if ((rc = ws.reserve(WsQSafe, (size_t)r_sz))) return rc;
+  uint8_t* safe = (uint8_t*)ws.ptr(WsQSafe);
+  LevelPlan lp;
+  level_plan(nbox, ctx.qfanout, &lp);
+  const int F = ctx.qfanout;
+  int64_t start = r_sz - lp.n[0];
+  ctx.prof.mark(s, "k_safe");
+  switch (F) {
+#define CT_SAFE(g)                                                                              \
+  case g:                                                                                       \
+    hipLaunchKernelGGL(k_safe_leaf_pow2<g>, dim3((unsigned)((nbox + 255) / 256)), dim3(256), 0, s, \
+                       nbox, rb, safe + start);                                                 \
+    break;
+    CT_SAFE(2) CT_SAFE(4) CT_SAFE(8) CT_SAFE(16) CT_SAFE(32) CT_SAFE(64)
+#undef CT_SAFE
+    default:
+      hipLaunchKernelGGL(k_safe_leaf_generic, dim3((unsigned)((lp.n[0] + 255) / 256)), dim3(256),
+                         0, s, nbox, F, rb, safe + start, lp.n[0]);
+  }
+  for (int l = 1; l < lp.nl; l++) {
+    int64_t pstart = start - lp.n[l];
+    hipLaunchKernelGGL(k_safe_up, dim3((unsigned)((lp.n[l] + 255) / 256)), dim3(256), 0, s,
+                       lp.n[l - 1], F, (const uint8_t*)(safe + start), safe + pstart, lp.n[l]);
+    start = pstart;
+  }
+  ctx.prof.mark(s, nullptr);
+  CT_HIP(hipGetLastError());
+  return CURTREE_OK;
+}
+
 template <int G>
-static int run_query(Context& ctx, const QTree& tr, const QBoxes& qb, const QBoxes& rb,
+static int run_query(Context& ctx, const QTree& tr_in, const QBoxes& qb, const QBoxes& rb,
                      const int* rid, int64_t r_sz, int64_t nbox, int nl, int64_t Q,
                      QueryOut* qo, hipStream_t s) {
   using S = QShape<G>;
   Workspace& ws = ctx.ws;
   Prof& pf = ctx.prof;
   int rc;
+  QTree tr = tr_in;
+  tr.safe = nullptr;
+  if (qo->contain) {
+    if ((rc = safe_flags(ctx, tr, rb, r_sz, nbox, s))) return rc;
+    tr.safe = (const uint8_t*)ws.ptr(WsQSafe);
+  }
   const uint2* cur = nullptr;  // implicit (query, root) frontier
   int64_t ncur = Q;
   WsSlot cur_slot = WsQFrontB;
@@ -353,11 +520,30 @@ static int run_query(Context& ctx, const QTree& tr, const QBoxes& qb, const QBox
       hipLaunchKernelGGL(k_fill_u64, dim3(256), dim3(256), 0, s, (uint64_t*)qo->counts, Q, 0ull);
     return CURTREE_OK;
   }
-  // leaf level: count pass
   const int64_t ntiles = (ncur + S::kPerTile - 1) / S::kPerTile;
+  uint32_t epoch;
+  if (!qo->alloc && !qo->count_only && !qo->counts && qo->fid) {
+    // leaf level, single pass into the caller's buffer
+    if ((rc = chain_epoch(ctx, s, ntiles, &epoch))) return rc;
+    pf.mark(s, "k_leaf_fused");
+    hipLaunchKernelGGL(k_leaf_fused<G>, dim3(chain_grid(ntiles)), dim3(kQThreads), 0, s, ncur, cur,
+                       tr, qb, rb, rid, r_sz, nbox, qo->fid, qo->tid, qo->cap,
+                       (uint64_t*)ws.ptr(WsQStatus), epoch, (QCtrl*)ws.ptr(WsQCtrl), ntiles);
+    pf.mark(s, nullptr);
+    CT_HIP(hipGetLastError());
+    uint64_t H = 0;
+    if ((rc = read_total(ctx, s, &H))) return rc;
+    qo->n = (int64_t)H;
+    qo->out_fid = qo->fid;
+    qo->out_tid = qo->tid;
+    if ((int64_t)H > qo->cap)
+      return set_error(CURTREE_ERANGE, "query: %llu pairs exceed the buffer of %lld",
+                       (unsigned long long)H, (long long)qo->cap);
+    return CURTREE_OK;
+  }
+  // leaf level: count pass
   if ((rc = ws.reserve(WsQMask, (size_t)ncur * 8)) || (rc = ws.reserve(WsQOff, (size_t)ncur * 8)))
     return rc;
-  uint32_t epoch;
   if ((rc = chain_epoch(ctx, s, ntiles, &epoch))) return rc;
   if (qo->counts)
     hipLaunchKernelGGL(k_fill_u64, dim3(256), dim3(256), 0, s, (uint64_t*)qo->counts, Q, 0ull);
@@ -424,7 +610,11 @@ int query_frontier_f32(Context& ctx, const curtree_rtree_f32* rt, const curtree_
       !rbox->ymin || !rbox->xmax || !rbox->ymax || !rbox->id || !qbox->xmin || !qbox->ymin ||
       !qbox->xmax || !qbox->ymax)
     return set_error(CURTREE_EINVAL, "query: null device array");
-  QTree tr{rt->xmin, rt->ymin, rt->xmax, rt->ymax, rt->pos, rt->len};
+  QTree tr{rt->xmin, rt->ymin, rt->xmax, rt->ymax, rt->pos, rt->len, nullptr};
+  ctx.qfanout = F;
+  if (ctx.opt_contain == 0) qo->contain = false;
+  else if (ctx.opt_contain == 1) qo->contain = true;
+  else qo->contain = Q >= 256;  // the flags cost one pass over the boxes
   QBoxes qb{qbox->xmin, qbox->ymin, qbox->xmax, qbox->ymax};
   QBoxes rb{rbox->xmin, rbox->ymin, rbox->xmax, rbox->ymax};
   switch (next_pow2_group(F)) {

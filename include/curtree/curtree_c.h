@@ -111,9 +111,10 @@ CURTREE_API int curtree_query_f32(const curtree_rtree_f32* rt, const curtree_bbo
                                   const curtree_bbox_f32* qbox, curtree_idpair64* out, int algo,
                                   void* stream);
 
-/* Same result written into caller buffers of `cap` pairs.  *n_out receives the
- * pair count; if it exceeds `cap` nothing is written and CURTREE_ERANGE is
- * returned.  Synchronises `stream`. */
+/* Same result written into caller buffers of `cap` pairs, in one pass over the
+ * leaf level.  *n_out receives the pair count; if it exceeds `cap`,
+ * CURTREE_ERANGE is returned and the buffer contents are unspecified.
+ * Synchronises `stream`. */
 CURTREE_API int curtree_query_into_f32(const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox,
                                        const curtree_bbox_f32* qbox, int* fid, int* tid,
                                        int64_t cap, int64_t* n_out, int algo, void* stream);
@@ -181,6 +182,14 @@ CURTREE_API const char* curtree_version(void);
 /* device count, name of device 0, CU count, HBM bytes */
 CURTREE_API int curtree_device_info(int* count, char* name, int name_len, int* cus,
                                     int64_t* hbm_bytes);
+/* Tuning knobs (process-wide):
+ *   "containment"  -1 auto (default: on for >= 256 windows), 0 off, 1 on.  A node
+ *                  whose MBR lies inside a window and whose boxes are all
+ *                  well-formed is expanded without reading child MBRs/boxes;
+ *                  results are identical either way.
+ * Returns CURTREE_EINVAL for an unknown name or value. */
+CURTREE_API int curtree_set_option(const char* name, int64_t value);
+
 /* Release the library's cached work buffers (they are reused across calls). */
 CURTREE_API int curtree_release_workspace(void);
 
