@@ -54,6 +54,17 @@ class IdPair64(ctypes.Structure):
     _fields_ = [("sz", ctypes.c_int64), ("fid", ctypes.c_void_p), ("tid", ctypes.c_void_p)]
 
 
+class BatchStats(ctypes.Structure):
+    """curtree_batch_stats."""
+    _fields_ = [("pairs", ctypes.c_int64), ("chunks", ctypes.c_int64),
+                ("max_chunk_pairs", ctypes.c_int64), ("count_ms", ctypes.c_double),
+                ("write_ms", ctypes.c_double)]
+
+
+CHUNK_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                            ctypes.c_int64, ctypes.c_int64, ctypes.c_int64)
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(
@@ -74,6 +85,9 @@ def _load():
         "curtree_query_into_f32": (I, [P(RTreeStruct), P(BBox), P(BBox), V, V, I64, P(I64), I, V]),
         "curtree_query_count_f32": (I, [P(RTreeStruct), P(BBox), P(BBox), V, P(I64), V]),
         "curtree_pairs_canonical_sort": (I, [V, V, I64, V]),
+        "curtree_self_join_f32": (I, [P(RTreeStruct), P(BBox), ctypes.c_float, P(IdPair64), V]),
+        "curtree_query_batched_f32": (I, [P(RTreeStruct), P(BBox), P(BBox), I64, V, V,
+                                          P(BatchStats)]),
         "curtree_box_d_alloc": (I, [P(BBox), I]),
         "curtree_box_d_free": (I, [P(BBox)]),
         "curtree_rt_d_free": (I, [P(RTreeStruct)]),
@@ -319,6 +333,38 @@ def query_count(rt, rbox, qbox, per_query=False, stream=None):
         if cptr:
             lib.curtree_free(ctypes.c_void_p(cptr))
     return (n.value, counts) if per_query else n.value
+
+
+def query_batched(rt, rbox, qbox, budget_pairs, on_chunk=None):
+    """Batched query under a result budget: count pass, then one graph replay
+    per chunk of windows.  on_chunk(fid_ptr, tid_ptr, n, q_first, q_end) gets
+    each chunk's device pairs; return True (or None) to continue.  Returns the
+    curtree_batch_stats as a dict."""
+    def cb(_user, fid, tid, n, q0, q1):
+        if on_chunk is None:
+            return 0
+        try:
+            r = on_chunk(fid, tid, n, q0, q1)
+            return 0 if (r is None or r) else 1
+        except Exception:  # stop the batch; the error is reported below
+            import traceback
+            traceback.print_exc()
+            return 1
+    fn = CHUNK_FN(cb)
+    st = BatchStats()
+    _check(lib.curtree_query_batched_f32(ctypes.byref(rt.s), ctypes.byref(rbox.s),
+                                         ctypes.byref(qbox.s), int(budget_pairs),
+                                         ctypes.cast(fn, ctypes.c_void_p), None, ctypes.byref(st)))
+    return {k: getattr(st, k) for k, _ in BatchStats._fields_}
+
+
+def self_join(rt, rbox, r, stream=None):
+    """Distance self-join: pairs (box id, querying box id) of boxes within r of
+    each other (the reference drivers' queries = data boxes expanded by r)."""
+    out = IdPair64()
+    _check(lib.curtree_self_join_f32(ctypes.byref(rt.s), ctypes.byref(rbox.s), float(r),
+                                     ctypes.byref(out), stream))
+    return DevicePairs(out)
 
 
 def canonical_sort(pairs, stream=None):

@@ -243,6 +243,25 @@ CURTREE_API int curtree_query_count_f32(const curtree_rtree_f32* rt, const curtr
   return CURTREE_OK;
 }
 
+CURTREE_API int curtree_query_batched_f32(const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox,
+                                          const curtree_bbox_f32* qbox, int64_t budget_pairs,
+                                          curtree_chunk_fn fn, void* user,
+                                          curtree_batch_stats* stats) {
+  Context& c = context();
+  std::lock_guard<std::mutex> g(c.mu);
+  return query_batched_f32(c, rt, rbox, qbox, budget_pairs, fn, user, stats);
+}
+
+CURTREE_API int curtree_self_join_f32(const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox,
+                                      float r, curtree_idpair64* out, void* stream) {
+  Context& c = context();
+  std::lock_guard<std::mutex> g(c.mu);
+  int rc = self_join_f32(c, rt, rbox, r, out, as_stream(stream));
+  if (rc) return rc;
+  CT_HIP(hipStreamSynchronize(as_stream(stream)));
+  return CURTREE_OK;
+}
+
 CURTREE_API int curtree_pairs_canonical_sort(int* fid, int* tid, int64_t n, void* stream) {
   Context& c = context();
   std::lock_guard<std::mutex> g(c.mu);
@@ -426,8 +445,7 @@ CURTREE_API int curtree_release_workspace(void) {
   std::lock_guard<std::mutex> g(c.mu);
   CT_HIP(hipDeviceSynchronize());
   c.ws.release();
-  c.qstatus_words = 0;
-  c.qepoch = 0;
+  c.qe.release();
   return CURTREE_OK;
 }
 

@@ -61,6 +61,9 @@ enum WsSlot {
   WsQMask,
   WsQOff,
   WsQSafe,
+  WsJoinWin,
+  WsBatchCounts,
+  WsBatchOut,
   WsSortKeys0,
   WsSortKeys1,
   WsSortVals0,
@@ -97,13 +100,62 @@ struct Prof {
   hipEvent_t get_event();
 };
 
+// Device-side state of one query pipeline run (one chain of kernels: run
+// begin, one expansion per internal level, leaf stage).  Every size lives
+// here, so a run needs no host round trip and can be captured in a graph.
+constexpr int kMaxRunSlots = 40;
+struct QCtrl {
+  uint64_t count[kMaxRunSlots];  // count[l]: entries of frontier l (count[0]: queries of the run)
+  uint64_t total;                // pairs of the run
+  unsigned long long npieces;    // long box ranges queued for the piece writer
+  uint32_t overflow;             // nonzero: a frontier exceeded the buffer capacity
+  uint32_t zero_needed;          // status words must be cleared before this run
+  uint32_t epoch_base;           // chained-scan epoch of slot 0 of this run
+  uint32_t epoch_next;           // first free epoch (kEpochs: clear before the next run)
+  uint32_t chunk;                // next entry of the chunk plan (graph replays)
+  uint32_t pad;
+  int64_t q0, q1;                // query range of the run
+  uint32_t ticket[kMaxRunSlots];
+};
+
+// Work buffers of one query pipeline; one engine serves one stream at a time.
+struct QEngine {
+  void* front[2] = {nullptr, nullptr};  // (query, node) frontiers, uint2 entries
+  int64_t cap_front = 0;                // entries per frontier buffer
+  void* pieces = nullptr;               // long box ranges of contained subtrees
+  int64_t cap_pieces = 0;
+  uint64_t* status = nullptr;           // chained-scan status words
+  int64_t status_words = 0;
+  QCtrl* ctrl = nullptr;
+  QCtrl* host_ctrl = nullptr;           // pinned copy for read-back
+  uint8_t* safe = nullptr;              // per-node well-formedness flags
+  int64_t cap_safe = 0;
+  int64_t* plan = nullptr;              // chunk plan (q0, q1) pairs for graph replays
+  int64_t cap_plan = 0;
+  bool needs_clear = true;              // status/ctrl are fresh: clear on the next run
+  int reserve_front(int64_t entries);
+  int reserve_pieces(int64_t n);
+  int reserve_status(int64_t words);
+  int reserve_safe(int64_t nodes);
+  int reserve_plan(int64_t chunks);
+  int ensure_ctrl();
+  void release();
+  ~QEngine() { release(); }
+};
+
+// Batched path: its own stream (graphs cannot capture the null stream) and engine.
+struct BatchState {
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  QEngine qe;
+};
+
 struct Context {
   std::mutex mu;
   Workspace ws;
   Prof prof;
-  uint32_t qepoch = 0;  // query chained-scan epoch, cycles through [0, kEpochs)
-  size_t qstatus_words = 0;
-  int qfanout = 0;       // fanout of the tree being queried
+  QEngine qe;            // engine of the plain (synchronous) query entry points
+  BatchState batch;
   int opt_contain = -1;  // containment shortcut: -1 auto, 0 off, 1 on
 };
 
@@ -130,7 +182,22 @@ struct QueryOut {
 int query_frontier_f32(Context& ctx, const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox,
                        const curtree_bbox_f32* qbox, QueryOut* qo, hipStream_t s);
 
+// Same, on an explicit engine and stream, for queries [q0, q1) of qbox (pair
+// tids stay global query indices).  contain: -1 auto, 0 off, 1 on.
+int query_run_f32(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox,
+                  const curtree_bbox_f32* qbox, int64_t q0, int64_t q1, int contain, QueryOut* qo,
+                  hipStream_t s);
+
 int pairs_canonical_sort(Context& ctx, int* fid, int* tid, int64_t n, hipStream_t s);
+
+int query_batched_f32(Context& ctx, const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox,
+                      const curtree_bbox_f32* qbox, int64_t budget, curtree_chunk_fn fn,
+                      void* user, curtree_batch_stats* stats);
+// Launch the control-block reset now (outside any graph capture).
+void enqueue_ctrl_init(QEngine& qe, hipStream_t s);
+
+int self_join_f32(Context& ctx, const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox, float r,
+                  curtree_idpair64* out, hipStream_t s);
 
 #define CT_HIP(call)                                                                        \
   do {                                                                                      \

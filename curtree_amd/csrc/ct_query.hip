@@ -10,73 +10,80 @@
 // Each level is ONE kernel: G = next_pow2(F) lanes test the children of one
 // frontier entry, survivors are ranked by wave ballot, wave totals by a block
 // scan, and tiles are chained by a decoupled look-back, so the next frontier is
-// written compacted AND in (query, node) order with one read of the input.  The
-// leaf level runs count (test boxes, keep a hit mask per entry, chained 64-bit
-// offsets) and write (emit pairs from the masks, no re-test) passes, so the
-// output is allocated exactly and lands in (query, sorted box position) order.
-#include "ct_internal.hpp"
+// written compacted AND in (query, node) order with one read of the input.
+// Tiles are taken from a ticket counter by a capped grid of blocks that loop
+// until the level's tiles run out (work is balanced by (query, node) entry, not
+// by query, so one window with 1e5 hits spreads over many blocks).
+//
+// All sizes of a run live in device memory (QCtrl): a run is begin -> one
+// expansion per internal level -> leaf stage, with no host round trip, so the
+// batched path can capture a run in a hipGraph and replay it per chunk.  The
+// leaf stage either writes pairs in one pass into a buffer of known capacity,
+// or counts (hit mask + 64-bit offset per entry) and writes in a second pass
+// after the exact output was allocated.  Output order is (query, sorted box
+// position), the reference's pre-sort order.
+#include "ct_query.hpp"
 
 namespace ct {
-
-constexpr int kQThreads = 256;
-constexpr int kQSteps = 8;  // wave-steps per tile
-
-struct QCtrl {
-  uint64_t total;
-  uint64_t pad[7];
-  uint32_t ticket[kEpochs + 1];
-};
-
-struct QTree {
-  const float* x0;
-  const float* y0;
-  const float* x1;
-  const float* y1;
-  const int* pos;
-  const int* len;
-  const uint8_t* safe;  // per node: every box below is well-formed (or null: no shortcut)
-};
-
-// Frontier entries are (query, node | kFull).  kFull marks a node whose MBR
-// lies inside the query window and whose boxes are all well-formed
-// (xmin <= xmax, ymin <= ymax): every box below it then intersects the window
-// (box inside its leaf-parent MBR inside the window), so its subtree is
-// expanded without reading child MBRs or boxes.  Results are identical to
-// testing every child, as the reference does.
-constexpr uint32_t kFull = 0x80000000u;
-
-__device__ __forceinline__ bool contains(float x1, float y1, float x2, float y2, float m1,
-                                         float n1, float m2, float n2) {
-  return x1 <= m1 && m2 <= x2 && y1 <= n1 && n2 <= y2;
-}
-struct QBoxes {
-  const float* x0;
-  const float* y0;
-  const float* x1;
-  const float* y1;
-};
-
-template <int G>
-struct QShape {
-  static constexpr int E = kWave / G;                   // entries per wave-step
-  static constexpr int kPerWave = kQSteps * E;
-  static constexpr int kPerTile = (kQThreads / kWave) * kPerWave;
-  static constexpr uint64_t kGroupMask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
-};
 
 __global__ void k_fill_u64(uint64_t* p, int64_t n, uint64_t v) {
   for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = v;
 }
-__global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v) {
-  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = v;
+
+// Fresh control block: the first run clears the status words.
+__global__ void k_ctrl_init(QCtrl* c) {
+  if (threadIdx.x == 0) {
+    c->epoch_next = kEpochs;
+    c->chunk = 0;
+    c->overflow = 0;
+    c->total = 0;
+  }
 }
 
-// Tile prologue/epilogue shared by the chained kernels: wave totals -> tile
-// total -> look-back -> exclusive base of this wave.
+// Start of a run: query range (from the arguments, or the next chunk of
+// `plan`), epochs for its `nslots` chained launches, cleared tickets/counts.
+__global__ void k_run_begin(QCtrl* c, int64_t q0, int64_t q1, const int64_t* plan,
+                            uint32_t nslots) {
+  if (threadIdx.x == 0) {
+    if (plan) {
+      const uint32_t k = c->chunk++;
+      q0 = plan[2 * k];
+      q1 = plan[2 * k + 1];
+    }
+    uint32_t eb = c->epoch_next;
+    if (eb + nslots > (uint32_t)kEpochs) {
+      eb = 0;
+      c->zero_needed = 1;
+    } else {
+      c->zero_needed = 0;
+    }
+    c->epoch_base = eb;
+    c->epoch_next = eb + nslots;
+    c->q0 = q0;
+    c->q1 = q1;
+    c->count[0] = (uint64_t)(q1 - q0);
+    c->overflow = 0;
+    c->total = 0;
+    c->npieces = 0;
+  }
+  for (int i = threadIdx.x; i < kMaxRunSlots; i += blockDim.x) {
+    c->ticket[i] = 0;
+    if (i > 0) c->count[i] = 0;
+  }
+}
+
+__global__ void k_status_zero(const QCtrl* c, uint64_t* status, int64_t words) {
+  if (!c->zero_needed) return;
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < words; i += (int64_t)gridDim.x * 256)
+    status[i] = 0;
+}
+
+// Tile epilogue shared by the chained kernels: wave totals -> tile total ->
+// look-back -> exclusive base of this wave.  The last tile leaves the grand
+// total in *s_total.
 __device__ __forceinline__ uint64_t chain_tile(uint64_t wcount, int64_t tile, int64_t ntiles,
-                                               uint64_t* status, uint32_t epoch,
-                                               uint64_t* total_out, uint64_t* s_wtot,
-                                               uint64_t* s_base) {
+                                               uint64_t* status, uint32_t epoch, uint64_t* s_wtot,
+                                               uint64_t* s_base, uint64_t* s_total) {
   const int w = threadIdx.x >> 6, lane = lane_id();
   if (lane == 0) s_wtot[w] = wcount;
   __syncthreads();
@@ -92,7 +99,7 @@ __device__ __forceinline__ uint64_t chain_tile(uint64_t wcount, int64_t tile, in
     }
     if (lane == 0) {
       *s_base = excl;
-      if (tile == ntiles - 1) *total_out = excl + tt;
+      if (tile == ntiles - 1) *s_total = excl + tt;
     }
   }
   __syncthreads();
@@ -101,246 +108,263 @@ __device__ __forceinline__ uint64_t chain_tile(uint64_t wcount, int64_t tile, in
   return b;
 }
 
-// Internal level: entries (query, node) whose children are nodes.  fin == null
-// means the implicit first frontier (query e, root).
+// Take the next tile; returns -1 when the level's tiles are exhausted.
+__device__ __forceinline__ int64_t next_tile(uint32_t* ticket, int64_t ntiles, uint32_t* s_tile) {
+  __syncthreads();  // the previous tile is done with the shared words
+  if (threadIdx.x == 0) *s_tile = atomicAdd(ticket, 1u);
+  __syncthreads();
+  const int64_t t = *s_tile;
+  return t < ntiles ? t : -1;
+}
+
+// Internal level `slot`: entries (query, node) whose children are nodes.
+// fin == null: the implicit first frontier (query q0 + e, root).
 template <int G>
-__global__ __launch_bounds__(kQThreads) void k_expand(int64_t n_in, const uint2* __restrict__ fin,
-                                                      QTree tr, QBoxes qb,
+__global__ __launch_bounds__(kQThreads) void k_expand(int slot, QTree tr, QBoxes qb, QCtrl* c,
+                                                      const uint2* __restrict__ fin,
                                                       uint2* __restrict__ fout, int64_t cap,
-                                                      uint64_t* status, uint32_t epoch,
-                                                      QCtrl* ctl, int64_t ntiles, int64_t r_sz) {
+                                                      uint64_t* status, int64_t r_sz) {
   using S = QShape<G>;
   __shared__ uint32_t s_tile;
   __shared__ uint64_t s_wtot[4];
-  __shared__ uint64_t s_base;
+  __shared__ uint64_t s_base, s_total;
+  if (c->overflow) return;
+  const int64_t n_in = (int64_t)c->count[slot];
+  const uint32_t epoch = c->epoch_base + slot;
+  const int64_t q0 = c->q0;
+  const int64_t ntiles = (n_in + S::kPerTile - 1) / S::kPerTile;
   const int t = threadIdx.x, w = t >> 6, lane = lane_id();
-  const int sub = lane / G, c = lane % G;
-  for (;;) {  // tiles by ticket until all are taken (grid may be smaller)
-  __syncthreads();  // the previous tile is done with s_tile, s_wtot, s_base
-  if (t == 0) s_tile = atomicAdd(&ctl->ticket[epoch], 1u);
-  __syncthreads();
-  const int64_t tile = s_tile;
-  if (tile >= ntiles) return;
-  uint64_t masks[kQSteps];
-  uint32_t qk[kQSteps], ck[kQSteps];
-  uint64_t wcount = 0;
-  const int64_t ebase = tile * S::kPerTile + (int64_t)w * S::kPerWave + sub;
+  if (ntiles == 0) {
+    if (blockIdx.x == 0 && t == 0) c->count[slot + 1] = 0;
+    return;
+  }
+  const int sub = lane / G, c_ = lane % G;
+  int64_t tile;
+  while ((tile = next_tile(&c->ticket[slot], ntiles, &s_tile)) >= 0) {
+    uint64_t masks[kQSteps];
+    uint32_t qk[kQSteps], ck[kQSteps];
+    uint64_t wcount = 0;
+    const int64_t ebase = tile * S::kPerTile + (int64_t)w * S::kPerWave + sub;
 #pragma unroll
-  for (int k = 0; k < kQSteps; k++) {
-    const int64_t e = ebase + k * S::E;
-    bool hit = false;
-    uint32_t q = 0, ci = 0;
-    if (e < n_in) {
-      uint32_t node = 0, full = 0;
-      if (fin) {
-        uint2 en = fin[e];
-        q = en.x;
-        node = en.y & ~kFull;
-        full = en.y & kFull;
-      } else {
-        q = (uint32_t)e;
-      }
-      const int p = tr.pos[node], L = tr.len[node];
-      if (c < L && (int64_t)p + L <= r_sz) {  // children must be nodes
-        const uint32_t child = (uint32_t)(p + c);
-        if (full) {
-          hit = true;
-          ci = child | kFull;
+    for (int k = 0; k < kQSteps; k++) {
+      const int64_t e = ebase + k * S::E;
+      bool hit = false;
+      uint32_t q = 0, ci = 0;
+      if (e < n_in) {
+        uint32_t node = 0, full = 0;
+        if (fin) {
+          uint2 en = fin[e];
+          q = en.x;
+          node = en.y & ~kFull;
+          full = en.y & kFull;
         } else {
-          const float wx1 = qb.x0[q], wy1 = qb.y0[q], wx2 = qb.x1[q], wy2 = qb.y1[q];
-          const float m1 = tr.x0[child], n1 = tr.y0[child], m2 = tr.x1[child], n2 = tr.y1[child];
-          hit = intersect(wx1, wy1, wx2, wy2, m1, n1, m2, n2);
-          ci = child;
-          if (hit && tr.safe && contains(wx1, wy1, wx2, wy2, m1, n1, m2, n2) && tr.safe[child])
-            ci = child | kFull;
+          q = (uint32_t)(q0 + e);
+        }
+        if (full) {
+          // a contained subtree travels down as one entry (its box range)
+          hit = c_ == 0;
+          ci = node | kFull;
+        } else {
+          const int p = tr.pos[node], L = tr.len[node];
+          if (c_ < L && (int64_t)p + L <= r_sz) {  // children must be nodes
+            const uint32_t child = (uint32_t)(p + c_);
+            const float wx1 = qb.x0[q], wy1 = qb.y0[q], wx2 = qb.x1[q], wy2 = qb.y1[q];
+            const float m1 = tr.x0[child], n1 = tr.y0[child], m2 = tr.x1[child],
+                        n2 = tr.y1[child];
+            hit = intersect(wx1, wy1, wx2, wy2, m1, n1, m2, n2);
+            ci = child;
+            if (hit && tr.safe && contains(wx1, wy1, wx2, wy2, m1, n1, m2, n2) && tr.safe[child])
+              ci = child | kFull;
+          }
         }
       }
+      const uint64_t m = ballot(hit);
+      masks[k] = m;
+      qk[k] = q;
+      ck[k] = ci;
+      wcount += (uint64_t)popc64(m);
     }
-    const uint64_t m = ballot(hit);
-    masks[k] = m;
-    qk[k] = q;
-    ck[k] = ci;
-    wcount += (uint64_t)popc64(m);
-  }
-  uint64_t b = chain_tile(wcount, tile, ntiles, status, epoch, &ctl->total, s_wtot, &s_base);
-  const uint64_t lt = lanemask_lt();
+    uint64_t b = chain_tile(wcount, tile, ntiles, status, epoch, s_wtot, &s_base, &s_total);
+    if (tile == ntiles - 1 && t == 0) {
+      c->count[slot + 1] = s_total;
+      if (s_total > (uint64_t)cap) c->overflow = 1;
+    }
+    const uint64_t lt = lanemask_lt();
 #pragma unroll
-  for (int k = 0; k < kQSteps; k++) {
-    const uint64_t m = masks[k];
-    if ((m >> lane) & 1ull) {
-      const uint64_t o = b + (uint64_t)popc64(m & lt);
-      if (o < (uint64_t)cap) fout[o] = make_uint2(qk[k], ck[k]);
+    for (int k = 0; k < kQSteps; k++) {
+      const uint64_t m = masks[k];
+      if ((m >> lane) & 1ull) {
+        const uint64_t o = b + (uint64_t)popc64(m & lt);
+        if (o < (uint64_t)cap) fout[o] = make_uint2(qk[k], ck[k]);
+      }
+      b += (uint64_t)popc64(m);
     }
-    b += (uint64_t)popc64(m);
-  }
   }
 }
 
-// Box c of leaf-parent entry e: does it intersect the entry's window?
-__device__ __forceinline__ bool leaf_hit(int64_t e, int c, const uint2* __restrict__ fin,
-                                         const QTree& tr, const QBoxes& qb, const QBoxes& rb,
-                                         int64_t r_sz, int64_t nbox, uint32_t* qout,
-                                         int64_t* biout) {
-  uint32_t node = 0, full = 0, q;
-  if (fin) {
-    uint2 en = fin[e];
-    q = en.x;
-    node = en.y & ~kFull;
-    full = en.y & kFull;
-  } else {
-    q = (uint32_t)e;
-  }
-  *qout = q;
-  const int p = tr.pos[node], L = tr.len[node];
-  const int64_t bi = (int64_t)p - r_sz + c;
-  *biout = bi;
-  if (c >= L || bi < 0 || bi >= nbox) return false;  // children must be boxes
-  if (full) return true;
-  return intersect(qb.x0[q], qb.y0[q], qb.x1[q], qb.y1[q], rb.x0[bi], rb.y0[bi], rb.x1[bi],
-                   rb.y1[bi]);
+// Box range [first, first + cnt) below node v (a complete F-ary tree over the
+// sorted boxes: node i of level l covers boxes [i*F^(l+1), (i+1)*F^(l+1))).
+__device__ __forceinline__ void node_range(uint32_t v, const QLevels& L, int64_t nbox,
+                                           int64_t* first, int64_t* cnt) {
+  int lv = 0;
+#pragma unroll
+  for (int k = 0; k < kMaxLevels; k++)
+    if (k < L.nl && (int64_t)v >= L.lstart[k] && (int64_t)v < L.lstart[k] + L.lcount[k]) lv = k;
+  const int64_t f = ((int64_t)v - L.lstart[lv]) * L.span[lv];
+  const int64_t rem = nbox - f;
+  *first = f;
+  *cnt = rem < L.span[lv] ? rem : L.span[lv];
 }
 
-// Leaf level, count pass: test the boxes of each leaf-parent entry, store the
-// entry's hit mask and its exclusive pair offset (chained across tiles).
-template <int G>
-__global__ __launch_bounds__(kQThreads) void k_leaf_count(
-    int64_t n_in, const uint2* __restrict__ fin, QTree tr, QBoxes qb, QBoxes rb, int64_t r_sz,
-    int64_t nbox, uint64_t* __restrict__ emask, uint64_t* __restrict__ eoff, unsigned long long* counts,
-    uint64_t* status, uint32_t epoch, QCtrl* ctl, int64_t ntiles) {
+// Leaf stage.  Each entry is either a partial leaf-parent (its boxes are
+// tested: G lanes, one box each) or a contained subtree (all of its box range
+// hits, no reads).  Entry counts are prefix-summed across the wave, the waves
+// of the tile and the tiles (chained), giving every pair its output slot.
+//   COUNT: totals (and per-query totals) only.
+//   WRITE: partial entries and short ranges are written by their lanes; longer
+//          ranges are cut into pieces for k_write_pieces.  Slots beyond
+//          out_cap are dropped; the total is always counted.
+template <int G, bool WRITE>
+__global__ __launch_bounds__(kQThreads) void k_leaf_pass(
+    int slot, QTree tr, QBoxes qb, QBoxes rb, const int* __restrict__ rid, QLevels lv, QCtrl* c,
+    const uint2* __restrict__ fin, int64_t r_sz, int64_t nbox, int* __restrict__ out_fid,
+    int* __restrict__ out_tid, int64_t out_cap, QPiece* __restrict__ pieces, int64_t piece_cap,
+    unsigned long long* counts, uint64_t* status) {
   using S = QShape<G>;
   __shared__ uint32_t s_tile;
   __shared__ uint64_t s_wtot[4];
-  __shared__ uint64_t s_base;
+  __shared__ uint64_t s_base, s_total;
+  if (c->overflow) return;
+  const int64_t n_in = (int64_t)c->count[slot];
+  const uint32_t epoch = c->epoch_base + slot;
+  const int64_t q0 = c->q0;
+  const int64_t ntiles = (n_in + S::kPerTile - 1) / S::kPerTile;
   const int t = threadIdx.x, w = t >> 6, lane = lane_id();
-  const int sub = lane / G, c = lane % G;
-  for (;;) {  // tiles by ticket until all are taken (grid may be smaller)
-  __syncthreads();  // the previous tile is done with s_tile, s_wtot, s_base
-  if (t == 0) s_tile = atomicAdd(&ctl->ticket[epoch], 1u);
-  __syncthreads();
-  const int64_t tile = s_tile;
-  if (tile >= ntiles) return;
-  uint64_t masks[kQSteps];
-  uint64_t wcount = 0;
-  const int64_t ebase = tile * S::kPerTile + (int64_t)w * S::kPerWave + sub;
+  if (ntiles == 0) {
+    if (blockIdx.x == 0 && t == 0) c->total = 0;
+    return;
+  }
+  const int sub = lane / G, c_ = lane % G;
+  const int lead = sub * G;  // first lane of this lane's group
+  int64_t tile;
+  while ((tile = next_tile(&c->ticket[slot], ntiles, &s_tile)) >= 0) {
+    uint64_t gmask[kQSteps];   // the group's hit bits (partial entries)
+    uint64_t ecnt[kQSteps];    // pairs of this lane's entry
+    uint64_t epre[kQSteps];    // pairs of earlier entries in the same step
+    uint64_t stot[kQSteps];    // pairs of the whole step
+    int64_t efirst[kQSteps];   // first sorted box of the entry
+    uint32_t qk[kQSteps];
+    bool efull[kQSteps];
+    uint64_t wcount = 0;
+    const int64_t ebase = tile * S::kPerTile + (int64_t)w * S::kPerWave + sub;
 #pragma unroll
-  for (int k = 0; k < kQSteps; k++) {
-    const int64_t e = ebase + k * S::E;
-    bool hit = false;
-    uint32_t q = 0;
-    if (e < n_in) {
-      int64_t bi;
-      hit = leaf_hit(e, c, fin, tr, qb, rb, r_sz, nbox, &q, &bi);
-    }
-    const uint64_t m = ballot(hit);
-    masks[k] = m;
-    wcount += (uint64_t)popc64(m);
-    if (counts && c == 0 && e < n_in) {
-      // per-query totals: one atomic per entry with hits
-      const uint64_t me = (m >> (sub * G)) & S::kGroupMask;
-      if (me) atomicAdd(counts + q, (unsigned long long)popc64(me));
-    }
-  }
-  uint64_t b = chain_tile(wcount, tile, ntiles, status, epoch, &ctl->total, s_wtot, &s_base);
+    for (int k = 0; k < kQSteps; k++) {
+      const int64_t e = ebase + k * S::E;
+      bool hit = false, full = false;
+      uint32_t q = 0;
+      int64_t first = 0, rcnt = 0;
+      if (e < n_in) {
+        uint32_t node = 0;
+        if (fin) {
+          const uint2 en = fin[e];
+          q = en.x;
+          node = en.y & ~kFull;
+          full = (en.y & kFull) != 0;
+        } else {
+          q = (uint32_t)(q0 + e);
+        }
+        if (full) {
+          node_range(node, lv, nbox, &first, &rcnt);
+        } else {
+          const int p = tr.pos[node], L = tr.len[node];
+          first = (int64_t)p - r_sz;
+          const int64_t bi = first + c_;
+          if (c_ < L && bi >= 0 && bi < nbox)  // children must be boxes
+            hit = intersect(qb.x0[q], qb.y0[q], qb.x1[q], qb.y1[q], rb.x0[bi], rb.y0[bi],
+                            rb.x1[bi], rb.y1[bi]);
+        }
+      }
+      const uint64_t m = ballot(hit);
+      const uint64_t gm = (m >> lead) & S::kGroupMask;
+      const uint64_t cnt = full ? (uint64_t)rcnt : (uint64_t)popc64(gm);
+      // exclusive prefix of entry counts over the step's groups
+      uint64_t v = c_ == 0 ? cnt : 0ull, inc = v;
 #pragma unroll
-  for (int k = 0; k < kQSteps; k++) {
-    const int64_t e = ebase + k * S::E;
-    const uint64_t m = masks[k];
-    const uint64_t me = (m >> (sub * G)) & S::kGroupMask;
-    const uint64_t before = sub == 0 ? 0ull : (m & ((1ull << (sub * G)) - 1ull));
-    if (c == 0 && e < n_in) {
-      emask[e] = me;
-      eoff[e] = b + (uint64_t)popc64(before);
+      for (int o = 1; o < kWave; o <<= 1) {
+        const uint64_t y = __shfl_up(inc, o);
+        if (lane >= o) inc += y;
+      }
+      const uint64_t pre = __shfl(inc - v, lead);
+      const uint64_t tot = __shfl(inc, kWave - 1);
+      gmask[k] = gm;
+      ecnt[k] = cnt;
+      epre[k] = pre;
+      stot[k] = tot;
+      efirst[k] = first;
+      qk[k] = q;
+      efull[k] = full;
+      wcount += tot;
+      if (counts && c_ == 0 && e < n_in && cnt) atomicAdd(counts + q, (unsigned long long)cnt);
     }
-    b += (uint64_t)popc64(m);
-  }
-  }
-}
-
-// Leaf level, write pass: G lanes per entry; lane c writes pair for box c if
-// bit c of the entry's mask is set.
-template <int G>
-__global__ __launch_bounds__(kQThreads) void k_leaf_write(int64_t n_in, const uint2* __restrict__ fin,
-                                                          QTree tr, const int* __restrict__ rid,
-                                                          int64_t r_sz,
-                                                          const uint64_t* __restrict__ emask,
-                                                          const uint64_t* __restrict__ eoff,
-                                                          int* __restrict__ out_fid,
-                                                          int* __restrict__ out_tid) {
-  const int64_t total = n_in * G;
-  for (int64_t gt = blockIdx.x * (int64_t)kQThreads + threadIdx.x; gt < total;
-       gt += (int64_t)gridDim.x * kQThreads) {
-    const int64_t e = gt / G;
-    const int c = (int)(gt % G);
-    const uint64_t m = emask[e];
-    if (!((m >> c) & 1ull)) continue;
-    uint32_t node = 0, q;
-    if (fin) {
-      uint2 en = fin[e];
-      q = en.x;
-      node = en.y & ~kFull;
-    } else {
-      q = (uint32_t)e;
-    }
-    const uint64_t o = eoff[e] + (uint64_t)popc64(m & ((1ull << c) - 1ull));
-    const int64_t bi = (int64_t)tr.pos[node] - r_sz + c;
-    out_fid[o] = rid[bi];
-    out_tid[o] = (int)q;
-  }
-}
-
-// Leaf level, single pass into a caller buffer of `cap` pairs: test, chain
-// the tile offsets, write pairs in (query, sorted position) order.  Pairs
-// beyond `cap` are dropped; the total is always counted.
-template <int G>
-__global__ __launch_bounds__(kQThreads) void k_leaf_fused(
-    int64_t n_in, const uint2* __restrict__ fin, QTree tr, QBoxes qb, QBoxes rb,
-    const int* __restrict__ rid, int64_t r_sz, int64_t nbox, int* __restrict__ out_fid,
-    int* __restrict__ out_tid, int64_t cap, uint64_t* status, uint32_t epoch, QCtrl* ctl,
-    int64_t ntiles) {
-  using S = QShape<G>;
-  __shared__ uint32_t s_tile;
-  __shared__ uint64_t s_wtot[4];
-  __shared__ uint64_t s_base;
-  const int t = threadIdx.x, w = t >> 6, lane = lane_id();
-  const int sub = lane / G, c = lane % G;
-  for (;;) {
-  __syncthreads();
-  if (t == 0) s_tile = atomicAdd(&ctl->ticket[epoch], 1u);
-  __syncthreads();
-  const int64_t tile = s_tile;
-  if (tile >= ntiles) return;
-  uint64_t masks[kQSteps];
-  uint32_t qk[kQSteps];
-  int64_t bk[kQSteps];
-  uint64_t wcount = 0;
-  const int64_t ebase = tile * S::kPerTile + (int64_t)w * S::kPerWave + sub;
+    uint64_t b = chain_tile(wcount, tile, ntiles, status, epoch, s_wtot, &s_base, &s_total);
+    if (tile == ntiles - 1 && t == 0) c->total = s_total;
+    if (WRITE) {
 #pragma unroll
-  for (int k = 0; k < kQSteps; k++) {
-    const int64_t e = ebase + k * S::E;
-    bool hit = false;
-    uint32_t q = 0;
-    int64_t bi = 0;
-    if (e < n_in) hit = leaf_hit(e, c, fin, tr, qb, rb, r_sz, nbox, &q, &bi);
-    const uint64_t m = ballot(hit);
-    masks[k] = m;
-    qk[k] = q;
-    bk[k] = bi;
-    wcount += (uint64_t)popc64(m);
-  }
-  uint64_t b = chain_tile(wcount, tile, ntiles, status, epoch, &ctl->total, s_wtot, &s_base);
-  const uint64_t lt = lanemask_lt();
-#pragma unroll
-  for (int k = 0; k < kQSteps; k++) {
-    const uint64_t m = masks[k];
-    if ((m >> lane) & 1ull) {
-      const uint64_t o = b + (uint64_t)popc64(m & lt);
-      if (o < (uint64_t)cap) {
-        out_fid[o] = rid[bk[k]];
-        out_tid[o] = (int)qk[k];
+      for (int k = 0; k < kQSteps; k++) {
+        const uint64_t off = b + epre[k];  // the entry's first output slot
+        if (!efull[k]) {
+          const uint64_t gm = gmask[k];
+          if ((gm >> c_) & 1ull) {
+            const uint64_t o = off + (uint64_t)popc64(gm & ((1ull << c_) - 1ull));
+            if (o < (uint64_t)out_cap) {
+              out_fid[o] = rid[efirst[k] + c_];
+              out_tid[o] = (int)qk[k];
+            }
+          }
+        } else if (ecnt[k] <= (uint64_t)(4 * G)) {
+          for (uint64_t j = c_; j < ecnt[k]; j += G) {
+            if (off + j < (uint64_t)out_cap) {
+              out_fid[off + j] = rid[efirst[k] + (int64_t)j];
+              out_tid[off + j] = (int)qk[k];
+            }
+          }
+        } else if (c_ == 0) {
+          const uint64_t np = (ecnt[k] + kPiece - 1) / kPiece;
+          const uint64_t at = atomicAdd(&c->npieces, (unsigned long long)np);
+          for (uint64_t p = 0; p < np; p++) {
+            if (at + p >= (uint64_t)piece_cap) break;  // only when the total exceeds out_cap
+            const uint64_t rest = ecnt[k] - p * kPiece;
+            QPiece pc;
+            pc.off = off + p * kPiece;
+            pc.first = efirst[k] + (int64_t)(p * kPiece);
+            pc.len = (uint32_t)(rest < kPiece ? rest : kPiece);
+            pc.q = qk[k];
+            pieces[at + p] = pc;
+          }
+        }
+        b += stot[k];
       }
     }
-    b += (uint64_t)popc64(m);
   }
+}
+
+// Long box ranges of contained subtrees: one block per piece, coalesced id
+// reads and pair writes.
+__global__ __launch_bounds__(256) void k_write_pieces(const QCtrl* c, const QPiece* __restrict__ pieces,
+                                                      int64_t piece_cap, const int* __restrict__ rid,
+                                                      int* __restrict__ out_fid,
+                                                      int* __restrict__ out_tid, int64_t out_cap) {
+  if (c->overflow) return;
+  const int64_t np = (int64_t)c->npieces < piece_cap ? (int64_t)c->npieces : piece_cap;
+  for (int64_t p = blockIdx.x; p < np; p += gridDim.x) {
+    const QPiece pc = pieces[p];
+    for (uint32_t j = threadIdx.x; j < pc.len; j += 256) {
+      const uint64_t o = pc.off + j;
+      if (o < (uint64_t)out_cap) {
+        out_fid[o] = rid[pc.first + j];
+        out_tid[o] = (int)pc.q;
+      }
+    }
   }
 }
 
@@ -378,75 +402,110 @@ __global__ __launch_bounds__(256) void k_safe_up(int64_t cn, int F, const uint8_
 }
 
 // ---------------------------------------------------------------------------
-// Host side.
-// Blocks for a ticketed chained launch: enough to fill the chip; each block
-// loops over tiles, so the grid stays far below the 2^32 work-item limit.
-static unsigned chain_grid(int64_t ntiles) {
-  return (unsigned)std::min<int64_t>(ntiles, 8192);
+// Host side: engine buffers.
+static int dev_alloc(void** p, size_t bytes) {
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  if (hipMalloc(p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return set_error(CURTREE_ENOMEM, "query: cannot allocate %zu bytes of work memory", bytes);
+  }
+  return CURTREE_OK;
 }
 
-static int next_pow2_group(int F) {
+int QEngine::reserve_front(int64_t entries) {
+  if (entries <= cap_front) return CURTREE_OK;
+  int rc;
+  if ((rc = dev_alloc(&front[0], (size_t)entries * 8)) ||
+      (rc = dev_alloc(&front[1], (size_t)entries * 8))) {
+    cap_front = 0;
+    return rc;
+  }
+  cap_front = entries;
+  return CURTREE_OK;
+}
+
+int QEngine::reserve_pieces(int64_t n) {
+  if (n <= cap_pieces) return CURTREE_OK;
+  n = std::max<int64_t>(n + n / 4, 4096);
+  int rc = dev_alloc(&pieces, (size_t)n * sizeof(QPiece));
+  cap_pieces = rc ? 0 : n;
+  return rc;
+}
+
+int QEngine::reserve_status(int64_t words) {
+  if (words <= status_words) return CURTREE_OK;
+  words = std::max<int64_t>(words + words / 2, 1 << 16);
+  int rc = dev_alloc((void**)&status, (size_t)words * 8);
+  status_words = rc ? 0 : words;
+  needs_clear = true;
+  return rc;
+}
+
+int QEngine::reserve_safe(int64_t nodes) {
+  if (nodes <= cap_safe) return CURTREE_OK;
+  int rc = dev_alloc((void**)&safe, (size_t)std::max<int64_t>(nodes, 256));
+  cap_safe = rc ? 0 : nodes;
+  return rc;
+}
+
+int QEngine::reserve_plan(int64_t chunks) {
+  if (chunks <= cap_plan) return CURTREE_OK;
+  int rc = dev_alloc((void**)&plan, (size_t)chunks * 16);
+  cap_plan = rc ? 0 : chunks;
+  return rc;
+}
+
+int QEngine::ensure_ctrl() {
+  if (ctrl) return CURTREE_OK;
+  int rc = dev_alloc((void**)&ctrl, sizeof(QCtrl));
+  if (rc) return rc;
+  if (hipHostMalloc((void**)&host_ctrl, sizeof(QCtrl), hipHostMallocDefault) != hipSuccess) {
+    host_ctrl = nullptr;
+    return set_error(CURTREE_ENOMEM, "query: cannot allocate pinned control block");
+  }
+  needs_clear = true;
+  return CURTREE_OK;
+}
+
+void QEngine::release() {
+  void* ps[] = {front[0], front[1], pieces, status, ctrl, safe, plan};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  if (host_ctrl) (void)hipHostFree(host_ctrl);
+  front[0] = front[1] = pieces = nullptr;
+  status = nullptr;
+  ctrl = nullptr;
+  host_ctrl = nullptr;
+  safe = nullptr;
+  plan = nullptr;
+  cap_front = cap_pieces = status_words = cap_safe = cap_plan = 0;
+  needs_clear = true;
+}
+
+int next_pow2_group(int F) {
   int g = 2;
   while (g < F) g <<= 1;
   return g;
 }
 
-// Fresh epoch for one chained launch of `ntiles` tiles; zeroes the status
-// words and tickets when the array grows or the epoch counter wraps.
-static int chain_epoch(Context& ctx, hipStream_t s, int64_t ntiles, uint32_t* epoch) {
-  Workspace& ws = ctx.ws;
-  int rc;
-  bool zero = false;
-  if ((size_t)ntiles > ctx.qstatus_words) {
-    size_t words = std::max<size_t>((size_t)ntiles + ntiles / 2, 1 << 16);
-    if ((rc = ws.reserve(WsQStatus, words * 8))) return rc;
-    ctx.qstatus_words = words;
-    zero = true;
-  }
-  if (!ws.ptr(WsQCtrl)) {
-    if ((rc = ws.reserve(WsQCtrl, sizeof(QCtrl)))) return rc;
-    zero = true;
-  }
-  if (zero || ctx.qepoch + 1 >= (uint32_t)kEpochs) {
-    hipLaunchKernelGGL(k_fill_u64, dim3(1024), dim3(256), 0, s, (uint64_t*)ws.ptr(WsQStatus),
-                       (int64_t)ctx.qstatus_words, 0ull);
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, s,
-                       (uint32_t*)ws.ptr(WsQCtrl), (int64_t)(sizeof(QCtrl) / 4), 0u);
-    ctx.qepoch = 0;
-  } else {
-    ctx.qepoch++;
-  }
-  *epoch = ctx.qepoch;
-  return CURTREE_OK;
-}
+int per_tile(int G) { return (kQThreads / kWave) * kQSteps * (kWave / G); }
 
-static int read_total(Context& ctx, hipStream_t s, uint64_t* v) {
-  static uint64_t* host = nullptr;
-  if (!host) CT_HIP(hipHostMalloc((void**)&host, 64, hipHostMallocDefault));
-  CT_HIP(hipMemcpyAsync(host, &((QCtrl*)ctx.ws.ptr(WsQCtrl))->total, 8, hipMemcpyDeviceToHost, s));
-  CT_HIP(hipStreamSynchronize(s));
-  *v = *host;
-  return CURTREE_OK;
-}
-
-// Per-node well-formedness flags into WsQSafe (R_sz bytes).
-static int safe_flags(Context& ctx, const QTree& tr, const QBoxes& rb, int64_t r_sz, int64_t nbox,
-                      hipStream_t s) {
-  (void)tr;
-  Workspace& ws = ctx.ws;
+int compute_safe(QEngine& qe, Prof& pf, const QBoxes& rb, int F, int64_t r_sz, int64_t nbox,
+                 hipStream_t s) {
   int rc;
-  if ((rc = ws.reserve(WsQSafe, (size_t)r_sz))) return rc;
-  uint8_t* safe = (uint8_t*)ws.ptr(WsQSafe);
+  if ((rc = qe.reserve_safe(r_sz))) return rc;
+  uint8_t* safe = qe.safe;
   LevelPlan lp;
-  level_plan(nbox, ctx.qfanout, &lp);
-  const int F = ctx.qfanout;
+  level_plan(nbox, F, &lp);
   int64_t start = r_sz - lp.n[0];
-  ctx.prof.mark(s, "k_safe");
+  pf.mark(s, "k_safe");
   switch (F) {
-#define CT_SAFE(g)                                                                              \
-  case g:                                                                                       \
-    hipLaunchKernelGGL(k_safe_leaf_pow2<g>, dim3((unsigned)((nbox + 255) / 256)), dim3(256), 0, s, \
-                       nbox, rb, safe + start);                                                 \
+#define CT_SAFE(g)                                                                        \
+  case g:                                                                                 \
+    hipLaunchKernelGGL(k_safe_leaf_pow2<g>, dim3((unsigned)((nbox + 255) / 256)), dim3(256), \
+                       0, s, nbox, rb, safe + start);                                     \
     break;
     CT_SAFE(2) CT_SAFE(4) CT_SAFE(8) CT_SAFE(16) CT_SAFE(32) CT_SAFE(64)
 #undef CT_SAFE
@@ -460,171 +519,223 @@ static int safe_flags(Context& ctx, const QTree& tr, const QBoxes& rb, int64_t r
                        lp.n[l - 1], F, (const uint8_t*)(safe + start), safe + pstart, lp.n[l]);
     start = pstart;
   }
-  ctx.prof.mark(s, nullptr);
+  pf.mark(s, nullptr);
   CT_HIP(hipGetLastError());
   return CURTREE_OK;
 }
 
 template <int G>
-static int run_query(Context& ctx, const QTree& tr_in, const QBoxes& qb, const QBoxes& rb,
-                     const int* rid, int64_t r_sz, int64_t nbox, int nl, int64_t Q,
-                     QueryOut* qo, hipStream_t s) {
-  using S = QShape<G>;
-  Workspace& ws = ctx.ws;
-  Prof& pf = ctx.prof;
-  int rc;
-  QTree tr = tr_in;
-  tr.safe = nullptr;
-  if (qo->contain) {
-    if ((rc = safe_flags(ctx, tr, rb, r_sz, nbox, s))) return rc;
-    tr.safe = (const uint8_t*)ws.ptr(WsQSafe);
+void enqueue_run_g(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
+  const unsigned nslots = (unsigned)a.nl;  // nl-1 expansions + 1 leaf stage
+  if (qe.needs_clear) {
+    hipLaunchKernelGGL(k_ctrl_init, dim3(1), dim3(64), 0, s, qe.ctrl);
+    qe.needs_clear = false;
   }
-  const uint2* cur = nullptr;  // implicit (query, root) frontier
-  int64_t ncur = Q;
-  WsSlot cur_slot = WsQFrontB;
-  for (int l = 0; l + 1 < nl; l++) {
-    const WsSlot out_slot = cur_slot == WsQFrontA ? WsQFrontB : WsQFrontA;
-    const int64_t ntiles = (ncur + S::kPerTile - 1) / S::kPerTile;
-    int64_t cap = (int64_t)(ws.cap[out_slot] / sizeof(uint2));
-    if (cap < std::max<int64_t>(2 * ncur, 1 << 20)) {
-      cap = std::max<int64_t>(2 * ncur, 1 << 20);
-      if ((rc = ws.reserve(out_slot, (size_t)cap * sizeof(uint2)))) return rc;
-    }
-    uint64_t total = 0;
-    for (int attempt = 0; attempt < 2; attempt++) {
-      uint32_t epoch;
-      if ((rc = chain_epoch(ctx, s, ntiles, &epoch))) return rc;
-      pf.mark(s, "k_expand");
-      hipLaunchKernelGGL(k_expand<G>, dim3(chain_grid(ntiles)), dim3(kQThreads), 0, s, ncur, cur, tr,
-                         qb, (uint2*)ws.ptr(out_slot), cap, (uint64_t*)ws.ptr(WsQStatus), epoch,
-                         (QCtrl*)ws.ptr(WsQCtrl), ntiles, r_sz);
-      pf.mark(s, nullptr);
-      CT_HIP(hipGetLastError());
-      if ((rc = read_total(ctx, s, &total))) return rc;
-      if (total <= (uint64_t)cap) break;
-      // frontier larger than the buffer: grow and run the level again
-      cap = (int64_t)(total + total / 4);
-      if ((rc = ws.reserve(out_slot, (size_t)cap * sizeof(uint2)))) return rc;
-    }
-    if (total > 0xFFFFFFFFull)
-      return set_error(CURTREE_ERANGE, "query: frontier of %llu entries exceeds 2^32",
-                       (unsigned long long)total);
-    ncur = (int64_t)total;
-    cur = (const uint2*)ws.ptr(out_slot);
-    cur_slot = out_slot;
-    if (ncur == 0) break;
+  pf.mark(s, "k_run_begin");
+  hipLaunchKernelGGL(k_run_begin, dim3(1), dim3(64), 0, s, qe.ctrl, a.q0, a.q1, a.plan, nslots);
+  hipLaunchKernelGGL(k_status_zero, dim3(1024), dim3(256), 0, s, (const QCtrl*)qe.ctrl, qe.status,
+                     qe.status_words);
+  const uint2* cur = nullptr;
+  int which = 0;
+  for (int l = 0; l + 1 < a.nl; l++) {
+    uint2* out = (uint2*)qe.front[which];
+    pf.mark(s, "k_expand");
+    hipLaunchKernelGGL(k_expand<G>, dim3(kChainGrid), dim3(kQThreads), 0, s, l, a.tr, a.qb,
+                       qe.ctrl, cur, out, qe.cap_front, qe.status, a.r_sz);
+    cur = out;
+    which ^= 1;
   }
-  qo->n = 0;
-  if (ncur == 0) {
-    if (qo->counts)
-      hipLaunchKernelGGL(k_fill_u64, dim3(256), dim3(256), 0, s, (uint64_t*)qo->counts, Q, 0ull);
-    return CURTREE_OK;
+  const int leaf = a.nl - 1;
+  if (a.write) {
+    pf.mark(s, "k_leaf_pass");
+    hipLaunchKernelGGL((k_leaf_pass<G, true>), dim3(kChainGrid), dim3(kQThreads), 0, s, leaf, a.tr,
+                       a.qb, a.rb, a.rid, a.lv, qe.ctrl, cur, a.r_sz, a.nbox, a.out_fid,
+                       a.out_tid, a.out_cap, (QPiece*)qe.pieces, qe.cap_pieces, a.counts,
+                       qe.status);
+    pf.mark(s, "k_write_pieces");
+    hipLaunchKernelGGL(k_write_pieces, dim3(4096), dim3(256), 0, s, (const QCtrl*)qe.ctrl,
+                       (const QPiece*)qe.pieces, qe.cap_pieces, a.rid, a.out_fid, a.out_tid,
+                       a.out_cap);
+  } else {
+    pf.mark(s, "k_leaf_count");
+    hipLaunchKernelGGL((k_leaf_pass<G, false>), dim3(kChainGrid), dim3(kQThreads), 0, s, leaf, a.tr,
+                       a.qb, a.rb, a.rid, a.lv, qe.ctrl, cur, a.r_sz, a.nbox, (int*)nullptr,
+                       (int*)nullptr, (int64_t)0, (QPiece*)nullptr, (int64_t)0, a.counts,
+                       qe.status);
   }
-  const int64_t ntiles = (ncur + S::kPerTile - 1) / S::kPerTile;
-  uint32_t epoch;
-  if (!qo->alloc && !qo->count_only && !qo->counts && qo->fid) {
-    // leaf level, single pass into the caller's buffer
-    if ((rc = chain_epoch(ctx, s, ntiles, &epoch))) return rc;
-    pf.mark(s, "k_leaf_fused");
-    hipLaunchKernelGGL(k_leaf_fused<G>, dim3(chain_grid(ntiles)), dim3(kQThreads), 0, s, ncur, cur,
-                       tr, qb, rb, rid, r_sz, nbox, qo->fid, qo->tid, qo->cap,
-                       (uint64_t*)ws.ptr(WsQStatus), epoch, (QCtrl*)ws.ptr(WsQCtrl), ntiles);
-    pf.mark(s, nullptr);
-    CT_HIP(hipGetLastError());
-    uint64_t H = 0;
-    if ((rc = read_total(ctx, s, &H))) return rc;
-    qo->n = (int64_t)H;
-    qo->out_fid = qo->fid;
-    qo->out_tid = qo->tid;
-    if ((int64_t)H > qo->cap)
-      return set_error(CURTREE_ERANGE, "query: %llu pairs exceed the buffer of %lld",
-                       (unsigned long long)H, (long long)qo->cap);
-    return CURTREE_OK;
-  }
-  // leaf level: count pass
-  if ((rc = ws.reserve(WsQMask, (size_t)ncur * 8)) || (rc = ws.reserve(WsQOff, (size_t)ncur * 8)))
-    return rc;
-  if ((rc = chain_epoch(ctx, s, ntiles, &epoch))) return rc;
-  if (qo->counts)
-    hipLaunchKernelGGL(k_fill_u64, dim3(256), dim3(256), 0, s, (uint64_t*)qo->counts, Q, 0ull);
-  pf.mark(s, "k_leaf_count");
-  hipLaunchKernelGGL(k_leaf_count<G>, dim3(chain_grid(ntiles)), dim3(kQThreads), 0, s, ncur, cur, tr,
-                     qb, rb, r_sz, nbox, (uint64_t*)ws.ptr(WsQMask), (uint64_t*)ws.ptr(WsQOff),
-                     (unsigned long long*)qo->counts, (uint64_t*)ws.ptr(WsQStatus), epoch,
-                     (QCtrl*)ws.ptr(WsQCtrl), ntiles);
   pf.mark(s, nullptr);
-  CT_HIP(hipGetLastError());
-  uint64_t H = 0;
-  if ((rc = read_total(ctx, s, &H))) return rc;
-  qo->n = (int64_t)H;
-  if (qo->count_only || H == 0) return CURTREE_OK;
-  int* fid = qo->fid;
-  int* tid = qo->tid;
-  if (qo->alloc) {
-    CT_HIP(hipMalloc((void**)&fid, H * sizeof(int)));
-    hipError_t e2 = hipMalloc((void**)&tid, H * sizeof(int));
-    if (e2 != hipSuccess) {
-      (void)hipFree(fid);
-      return set_error(CURTREE_ENOMEM, "query: cannot allocate %llu result pairs",
-                       (unsigned long long)H);
-    }
-  } else if ((int64_t)H > qo->cap) {
-    return set_error(CURTREE_ERANGE, "query: %llu pairs exceed the buffer of %lld",
-                     (unsigned long long)H, (long long)qo->cap);
+}
+
+void enqueue_run(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
+  switch (a.G) {
+    case 2: enqueue_run_g<2>(qe, pf, a, s); break;
+    case 4: enqueue_run_g<4>(qe, pf, a, s); break;
+    case 8: enqueue_run_g<8>(qe, pf, a, s); break;
+    case 16: enqueue_run_g<16>(qe, pf, a, s); break;
+    case 32: enqueue_run_g<32>(qe, pf, a, s); break;
+    default: enqueue_run_g<64>(qe, pf, a, s); break;
   }
-  qo->out_fid = fid;
-  qo->out_tid = tid;
-  const int64_t threads = ncur * G;
-  const unsigned wgrid = (unsigned)std::min<int64_t>((threads + kQThreads - 1) / kQThreads, 1 << 16);
-  pf.mark(s, "k_leaf_write");
-  hipLaunchKernelGGL(k_leaf_write<G>, dim3(wgrid),
-                     dim3(kQThreads), 0, s, ncur, cur, tr, rid, r_sz, (const uint64_t*)ws.ptr(WsQMask),
-                     (const uint64_t*)ws.ptr(WsQOff), fid, tid);
-  pf.mark(s, nullptr);
-  CT_HIP(hipGetLastError());
+}
+
+int64_t pieces_needed(const QEngine& qe, const QRunArgs& a) {
+  // every long range is at most one leaf entry and cut into kPiece slots
+  return qe.cap_front + a.out_cap / (int64_t)kPiece + 1;
+}
+
+void enqueue_ctrl_init(QEngine& qe, hipStream_t s) {
+  hipLaunchKernelGGL(k_ctrl_init, dim3(1), dim3(64), 0, s, qe.ctrl);
+  qe.needs_clear = false;
+}
+
+void enqueue_fill_counts(int64_t* counts, int64_t n, hipStream_t s) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_fill_u64, dim3(256), dim3(256), 0, s, (uint64_t*)counts, n, 0ull);
+}
+
+int read_ctrl(QEngine& qe, hipStream_t s) {
+  CT_HIP(hipMemcpyAsync(qe.host_ctrl, qe.ctrl, sizeof(QCtrl), hipMemcpyDeviceToHost, s));
+  CT_HIP(hipStreamSynchronize(s));
   return CURTREE_OK;
 }
 
-int query_frontier_f32(Context& ctx, const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox,
-                       const curtree_bbox_f32* qbox, QueryOut* qo, hipStream_t s) {
-  if (!rt || !rbox || !qbox || !qo) return set_error(CURTREE_EINVAL, "query: null argument");
+int prepare_query(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox,
+                  const curtree_bbox_f32* qbox, int contain, int64_t nq_hint, QRunArgs* a,
+                  hipStream_t s) {
+  if (!rt || !rbox || !qbox) return set_error(CURTREE_EINVAL, "query: null argument");
   const int64_t N = rbox->sz, Q = qbox->sz;
   const int F = rt->fanout;
-  qo->n = 0;
   if (N < 0 || Q < 0) return set_error(CURTREE_EINVAL, "query: negative size");
   if (F < 2) return set_error(CURTREE_EINVAL, "query: fanout must be >= 2");
   if (F > 64)
     return set_error(CURTREE_EINVAL, "query: fanout %d > 64 is not supported by the GPU query", F);
-  if (N == 0 || Q == 0) {
-    if (qo->counts && Q > 0)
-      hipLaunchKernelGGL(k_fill_u64, dim3(256), dim3(256), 0, s, (uint64_t*)qo->counts, Q, 0ull);
-    return CURTREE_OK;
-  }
+  if (Q > (int64_t)0xFFFFFFFF) return set_error(CURTREE_EINVAL, "query: too many windows");
+  *a = QRunArgs();
+  a->nbox = N;
+  a->F = F;
+  a->G = next_pow2_group(F);
+  if (N == 0) return CURTREE_OK;
   LevelPlan lp;
   level_plan(N, F, &lp);
   if (lp.r_sz != rt->sz)
     return set_error(CURTREE_EINVAL,
                      "query: tree size %d does not match %lld boxes at fanout %d (expected %lld)",
                      rt->sz, (long long)N, F, (long long)lp.r_sz);
+  if (lp.nl + 1 > kMaxRunSlots) return set_error(CURTREE_EINVAL, "query: tree too deep");
   if (!rt->xmin || !rt->ymin || !rt->xmax || !rt->ymax || !rt->pos || !rt->len || !rbox->xmin ||
-      !rbox->ymin || !rbox->xmax || !rbox->ymax || !rbox->id || !qbox->xmin || !qbox->ymin ||
-      !qbox->xmax || !qbox->ymax)
+      !rbox->ymin || !rbox->xmax || !rbox->ymax || !rbox->id ||
+      (Q > 0 && (!qbox->xmin || !qbox->ymin || !qbox->xmax || !qbox->ymax)))
     return set_error(CURTREE_EINVAL, "query: null device array");
-  QTree tr{rt->xmin, rt->ymin, rt->xmax, rt->ymax, rt->pos, rt->len, nullptr};
-  ctx.qfanout = F;
-  if (ctx.opt_contain == 0) qo->contain = false;
-  else if (ctx.opt_contain == 1) qo->contain = true;
-  else qo->contain = Q >= 256;  // the flags cost one pass over the boxes
-  QBoxes qb{qbox->xmin, qbox->ymin, qbox->xmax, qbox->ymax};
-  QBoxes rb{rbox->xmin, rbox->ymin, rbox->xmax, rbox->ymax};
-  switch (next_pow2_group(F)) {
-    case 2: return run_query<2>(ctx, tr, qb, rb, rbox->id, lp.r_sz, N, lp.nl, Q, qo, s);
-    case 4: return run_query<4>(ctx, tr, qb, rb, rbox->id, lp.r_sz, N, lp.nl, Q, qo, s);
-    case 8: return run_query<8>(ctx, tr, qb, rb, rbox->id, lp.r_sz, N, lp.nl, Q, qo, s);
-    case 16: return run_query<16>(ctx, tr, qb, rb, rbox->id, lp.r_sz, N, lp.nl, Q, qo, s);
-    case 32: return run_query<32>(ctx, tr, qb, rb, rbox->id, lp.r_sz, N, lp.nl, Q, qo, s);
-    default: return run_query<64>(ctx, tr, qb, rb, rbox->id, lp.r_sz, N, lp.nl, Q, qo, s);
+  int rc;
+  if ((rc = qe.ensure_ctrl())) return rc;
+  a->tr = QTree{rt->xmin, rt->ymin, rt->xmax, rt->ymax, rt->pos, rt->len, nullptr};
+  a->qb = QBoxes{qbox->xmin, qbox->ymin, qbox->xmax, qbox->ymax};
+  a->rb = QBoxes{rbox->xmin, rbox->ymin, rbox->xmax, rbox->ymax};
+  a->rid = rbox->id;
+  a->r_sz = lp.r_sz;
+  a->nl = lp.nl;
+  a->lv.nl = lp.nl;
+  {
+    int64_t start = lp.r_sz, span = 1;
+    for (int l = 0; l < lp.nl; l++) {
+      start -= lp.n[l];
+      span = span > ((int64_t)1 << 40) / F ? ((int64_t)1 << 40) : span * F;
+      a->lv.lstart[l] = start;
+      a->lv.lcount[l] = lp.n[l];
+      a->lv.span[l] = span;
+    }
   }
+  // the flags cost one pass over the boxes; worth it for larger batches
+  const bool use = contain == 1 || (contain < 0 && nq_hint >= 256);
+  if (use) {
+    if ((rc = compute_safe(qe, pf, a->rb, F, lp.r_sz, N, s))) return rc;
+    a->tr.safe = qe.safe;
+  }
+  return CURTREE_OK;
+}
+
+int query_run_f32(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox,
+                  const curtree_bbox_f32* qbox, int64_t q0, int64_t q1, int contain, QueryOut* qo,
+                  hipStream_t s) {
+  if (!qo || !qbox) return set_error(CURTREE_EINVAL, "query: null argument");
+  qo->n = 0;
+  if (q0 < 0 || q1 > qbox->sz || q0 > q1) return set_error(CURTREE_EINVAL, "query: bad range");
+  QRunArgs a;
+  int rc;
+  if ((rc = prepare_query(qe, pf, rt, rbox, qbox, contain, q1 - q0, &a, s))) return rc;
+  const int64_t nq = q1 - q0;
+  if (a.nbox == 0 || nq == 0) {
+    if (qo->counts) enqueue_fill_counts(qo->counts + q0, nq, s);
+    return CURTREE_OK;
+  }
+  a.q0 = q0;
+  a.q1 = q1;
+  a.counts = (unsigned long long*)qo->counts;
+  // caller buffer: one writing run; otherwise count first
+  const bool direct = !qo->alloc && !qo->count_only && !qo->counts && qo->fid;
+  a.write = direct;
+  a.out_fid = qo->fid;
+  a.out_tid = qo->tid;
+  a.out_cap = direct ? qo->cap : 0;
+  if ((rc = qe.reserve_front(std::max<int64_t>(2 * nq, 1 << 20)))) return rc;
+  for (int pass = 0; pass < 2; pass++) {
+    for (int attempt = 0;; attempt++) {
+      if (a.counts) enqueue_fill_counts(qo->counts + q0, nq, s);
+      if ((rc = qe.reserve_status(std::max<int64_t>(qe.cap_front, nq) / per_tile(a.G) + 2)))
+        return rc;
+      if (a.write && (rc = qe.reserve_pieces(pieces_needed(qe, a)))) return rc;
+      enqueue_run(qe, pf, a, s);
+      CT_HIP(hipGetLastError());
+      if ((rc = read_ctrl(qe, s))) return rc;
+      if (!qe.host_ctrl->overflow) break;
+      if (attempt >= 3) return set_error(CURTREE_ENOMEM, "query: frontier kept overflowing");
+      // grow to the largest frontier the run reported and run again
+      uint64_t need = 0;
+      for (int l = 1; l < a.nl; l++) need = std::max<uint64_t>(need, qe.host_ctrl->count[l]);
+      if (need > 0xFFFFFFFFull)
+        return set_error(CURTREE_ERANGE, "query: frontier of %llu entries exceeds 2^32",
+                         (unsigned long long)need);
+      if ((rc = qe.reserve_front((int64_t)(need + need / 4 + 1024)))) return rc;
+    }
+    const uint64_t H = qe.host_ctrl->total;
+    qo->n = (int64_t)H;
+    if (a.write) {
+      qo->out_fid = a.out_fid;
+      qo->out_tid = a.out_tid;
+      if ((int64_t)H > a.out_cap)
+        return set_error(CURTREE_ERANGE, "query: %llu pairs exceed the buffer of %lld",
+                         (unsigned long long)H, (long long)a.out_cap);
+      return CURTREE_OK;
+    }
+    if (qo->count_only || H == 0) return CURTREE_OK;
+    // second pass: write into an exact allocation (or the caller's buffer)
+    if (qo->alloc) {
+      int* fid = nullptr;
+      int* tid = nullptr;
+      CT_HIP(hipMalloc((void**)&fid, H * sizeof(int)));
+      if (hipMalloc((void**)&tid, H * sizeof(int)) != hipSuccess) {
+        (void)hipFree(fid);
+        return set_error(CURTREE_ENOMEM, "query: cannot allocate %llu result pairs",
+                         (unsigned long long)H);
+      }
+      a.out_fid = fid;
+      a.out_tid = tid;
+      a.out_cap = (int64_t)H;
+    } else if ((int64_t)H > qo->cap) {
+      return set_error(CURTREE_ERANGE, "query: %llu pairs exceed the buffer of %lld",
+                       (unsigned long long)H, (long long)qo->cap);
+    } else {
+      a.out_cap = qo->cap;
+    }
+    a.write = true;
+    a.counts = nullptr;
+  }
+  return CURTREE_OK;
+}
+
+int query_frontier_f32(Context& ctx, const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox,
+                       const curtree_bbox_f32* qbox, QueryOut* qo, hipStream_t s) {
+  if (!qbox) return set_error(CURTREE_EINVAL, "query: null argument");
+  return query_run_f32(ctx.qe, ctx.prof, rt, rbox, qbox, 0, qbox->sz, ctx.opt_contain, qo, s);
 }
 
 }  // namespace ct

@@ -125,11 +125,43 @@ CURTREE_API int curtree_query_count_f32(const curtree_rtree_f32* rt, const curtr
                                         const curtree_bbox_f32* qbox, int64_t* counts,
                                         int64_t* n_out, void* stream);
 
+/* Batched query of a window set whose pairs may exceed device memory (no
+ * reference counterpart: the reference runs one batch and aborts on large
+ * frontiers, src/rtree_bfs_frontier.cuh:385-403).  Pass 1 counts every
+ * window's pairs; windows are then cut into chunks of consecutive windows
+ * whose pairs fit `budget_pairs` (a window with more pairs than the budget
+ * gets a chunk of its own); pass 2 replays one captured hipGraph per chunk.
+ * `fn` receives each chunk's pairs in device memory, in (query, sorted box
+ * position) order, valid until it returns; a nonzero return stops the batch. */
+typedef struct curtree_batch_stats {
+  int64_t pairs;           /* pairs delivered */
+  int64_t chunks;          /* graph replays */
+  int64_t max_chunk_pairs; /* largest chunk */
+  double count_ms;         /* pass 1, device time */
+  double write_ms;         /* pass 2, device time summed over chunks */
+} curtree_batch_stats;
+typedef int (*curtree_chunk_fn)(void* user, const int* fid, const int* tid, int64_t n,
+                                int64_t q_first, int64_t q_end);
+CURTREE_API int curtree_query_batched_f32(const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox,
+                                          const curtree_bbox_f32* qbox, int64_t budget_pairs,
+                                          curtree_chunk_fn fn, void* user,
+                                          curtree_batch_stats* stats);
+
 /* Canonical result order: sorts pairs on the device into (query id, box id)
  * lexicographic order -- the order the reference's two host stable sorts give
  * (src/rtree_test_frontier.cu:102-105).  Pairs must arrive grouped by query
  * (as every query entry point returns them). */
 CURTREE_API int curtree_pairs_canonical_sort(int* fid, int* tid, int64_t n, void* stream);
+
+/* Distance self-join: all (box id, querying box id) pairs whose boxes are
+ * within r of each other along both axes -- the tree queried with every data
+ * box expanded by r, as the reference's drivers do (query file = data file
+ * read with r = 10, src/rtree_test_frontier.cu:84-85; Boost check
+ * src/verify.hpp:75-104).  `rbox` is the sorted box set of the build; tid
+ * holds the querying box's id.  Order: (querying box's sorted position, box
+ * sorted position).  Allocates out->fid/tid.  Synchronises `stream`. */
+CURTREE_API int curtree_self_join_f32(const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox,
+                                      float r, curtree_idpair64* out, void* stream);
 
 /* ---- ingest and persistence --------------------------------------------- */
 
