@@ -73,6 +73,10 @@ def main():
     ap.add_argument("--key", choices=["morton64", "ref32"], default="morton64")
     ap.add_argument("--cpu-sample", type=int, default=4_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--query-tile-steps", type=int, default=0,
+                    help="wave-steps per query frontier tile (4, 8, 16; 0: library default)")
+    ap.add_argument("--query-grid", type=int, default=0,
+                    help="blocks per chained query launch (0: library default)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -84,6 +88,10 @@ def main():
     import curtree_amd as ct
     from curtree_amd import datagen
 
+    if args.query_tile_steps:
+        ct.set_option("query_tile_steps", args.query_tile_steps)
+    if args.query_grid:
+        ct.set_option("query_grid", args.query_grid)
     key_mode = ct.KEY_MORTON64 if args.key == "morton64" else ct.KEY_REF32
     kbits = 64 if key_mode == ct.KEY_MORTON64 else 32
     n, nq, F = args.boxes, args.windows, args.fanout
@@ -108,8 +116,20 @@ def main():
     # size the work buffers and the pair output once, outside the timed region
     reset()
     rt = ct.build_rtree(work, F, key_mode)
+    # the reference's traversal (no containment shortcut) gives the work
+    # counts of the algorithmic query bytes: X expansions, T tests, H hits
+    ct.set_option("containment", 0)
     H = ct.query_count(rt, work, q)
+    ref_stats = ct.query_last_stats()
+    ct.set_option("containment", -1)
+    assert ct.query_count(rt, work, q) == H
+    run_stats = ct.query_last_stats()
     rt.free()
+    X = sum(ref_stats["entries"][1:])
+    # every expanded entry tests all children of its node (the last node of a
+    # level may hold fewer than F; the difference is below 1e-6 here)
+    T = sum(ref_stats["entries"]) * F
+    q_bytes = 16 * nq + 8 * X + 16 * T + 12 * H
     import ctypes
     fbuf, tbuf = ctypes.c_void_p(), ctypes.c_void_p()
     ct._check(ct.lib.curtree_malloc(ctypes.byref(fbuf), 4 * max(H, 1)))
@@ -202,6 +222,11 @@ def main():
                   "frac_of_8TBs": build_gbs / HBM_PEAK_GBS},
         "query": {"ms": float(query_ms.mean()), "pairs": int(H),
                   "pairs_per_s": H / (query_ms.mean() * 1e-3),
+                  "reference_work": {"X": int(X), "T": int(T), "H": int(H),
+                                     "frontier": ref_stats["entries"]},
+                  "frontier": run_stats["entries"], "pieces": run_stats["pieces"],
+                  "algorithmic_bytes": int(q_bytes),
+                  "achieved_GBs": q_bytes / (query_ms.mean() * 1e-3) / 1e9,
                   "step_p50_ms": float(np.percentile(step_ms, 50)),
                   "step_p99_ms": float(np.percentile(step_ms, 99))},
         "kernels": kernels,

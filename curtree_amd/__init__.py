@@ -20,7 +20,8 @@ __all__ = [
 ]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libcurtree_amd.so")
+# CURTREE_AMD_LIB selects another build of the same library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("CURTREE_AMD_LIB") or os.path.join(HERE, "libcurtree_amd.so")
 
 KEY_REF32 = 0
 KEY_MORTON64 = 1
@@ -61,6 +62,20 @@ class BatchStats(ctypes.Structure):
                 ("write_ms", ctypes.c_double)]
 
 
+class ServerOpts(ctypes.Structure):
+    """curtree_server_opts."""
+    _fields_ = [("slots", ctypes.c_int), ("budget_bytes", ctypes.c_int64),
+                ("max_batch_windows", ctypes.c_int64), ("batching", ctypes.c_int)]
+
+
+class ServerStats(ctypes.Structure):
+    """curtree_server_stats."""
+    _fields_ = [("requests_done", ctypes.c_int64), ("pairs_done", ctypes.c_int64),
+                ("batches", ctypes.c_int64), ("queued", ctypes.c_int64),
+                ("p50_us", ctypes.c_double), ("p99_us", ctypes.c_double),
+                ("span_us", ctypes.c_double)]
+
+
 CHUNK_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                             ctypes.c_int64, ctypes.c_int64, ctypes.c_int64)
 
@@ -88,6 +103,14 @@ def _load():
         "curtree_self_join_f32": (I, [P(RTreeStruct), P(BBox), ctypes.c_float, P(IdPair64), V]),
         "curtree_query_batched_f32": (I, [P(RTreeStruct), P(BBox), P(BBox), I64, V, V,
                                           P(BatchStats)]),
+        "curtree_server_create": (I, [P(ServerOpts), P(V)]),
+        "curtree_server_add_tree": (I, [V, P(RTreeStruct), P(BBox), P(I)]),
+        "curtree_server_submit": (I, [V, I, P(BBox), P(I64)]),
+        "curtree_server_poll": (I, [V, I64, P(I), P(IdPair64), P(ctypes.c_double)]),
+        "curtree_server_release": (I, [V, I64]),
+        "curtree_server_get_stats": (I, [V, P(ServerStats)]),
+        "curtree_server_reset_stats": (I, [V]),
+        "curtree_server_destroy": (I, [V]),
         "curtree_box_d_alloc": (I, [P(BBox), I]),
         "curtree_box_d_free": (I, [P(BBox)]),
         "curtree_rt_d_free": (I, [P(RTreeStruct)]),
@@ -105,6 +128,7 @@ def _load():
         "curtree_version": (ctypes.c_char_p, []),
         "curtree_device_info": (I, [P(I), ctypes.c_char_p, I, P(I), P(I64)]),
         "curtree_release_workspace": (I, []),
+        "curtree_query_last_stats": (I, [P(I64), I, P(I), P(I64), P(I64)]),
         "curtree_set_option": (I, [ctypes.c_char_p, I64]),
         "curtree_prof_enable": (I, [I]),
         "curtree_prof_reset": (I, []),
@@ -143,6 +167,16 @@ def _host_from_dev(ptr, n, dtype):
     if out.nbytes:
         _check(lib.curtree_memcpy_d2h(out.ctypes.data, ctypes.c_void_p(ptr), out.nbytes))
     return out
+
+
+def query_last_stats():
+    """Frontier sizes of the last plain query (see curtree_query_last_stats)."""
+    ent = (ctypes.c_int64 * 64)()
+    nl, pairs, pieces = ctypes.c_int(), ctypes.c_int64(), ctypes.c_int64()
+    _check(lib.curtree_query_last_stats(ent, 64, ctypes.byref(nl), ctypes.byref(pairs),
+                                        ctypes.byref(pieces)))
+    return {"entries": [int(ent[i]) for i in range(nl.value)], "pairs": pairs.value,
+            "pieces": pieces.value}
 
 
 def set_option(name, value):
@@ -232,6 +266,17 @@ class RTree:
             "pos": _host_from_dev(self.s.pos, n, np.int32),
             "len": _host_from_dev(self.s.len, n, np.int32),
         }
+
+    def upload(self, host):
+        """Overwrite the node arrays with host arrays of the same size (the
+        dict layout of to_host)."""
+        for k, t in (("xmin", np.float32), ("ymin", np.float32), ("xmax", np.float32),
+                     ("ymax", np.float32), ("pos", np.int32), ("len", np.int32)):
+            a = np.ascontiguousarray(host[k], dtype=t)
+            assert a.shape == (self.sz,)
+            if a.nbytes:
+                _check(lib.curtree_memcpy_h2d(ctypes.c_void_p(getattr(self.s, k)), a.ctypes.data,
+                                              a.nbytes))
 
     def free(self):
         _check(lib.curtree_rt_d_free(ctypes.byref(self.s)))
@@ -356,6 +401,82 @@ def query_batched(rt, rbox, qbox, budget_pairs, on_chunk=None):
                                          ctypes.byref(qbox.s), int(budget_pairs),
                                          ctypes.cast(fn, ctypes.c_void_p), None, ctypes.byref(st)))
     return {k: getattr(st, k) for k, _ in BatchStats._fields_}
+
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
+
+
+class Server:
+    """Serving runtime: resident trees, continuous batching of window requests
+    (see curtree_server_* in curtree_c.h)."""
+
+    def __init__(self, slots=4, budget_bytes=16 << 30, max_batch_windows=1 << 20, batching=True):
+        o = ServerOpts(int(slots), int(budget_bytes), int(max_batch_windows), 1 if batching else 0)
+        p = ctypes.c_void_p()
+        _check(lib.curtree_server_create(ctypes.byref(o), ctypes.byref(p)))
+        self.p = p.value
+        self._keep = {}
+
+    def add_tree(self, rt, rbox):
+        tid = ctypes.c_int()
+        _check(lib.curtree_server_add_tree(self.p, ctypes.byref(rt.s), ctypes.byref(rbox.s),
+                                           ctypes.byref(tid)))
+        return tid.value
+
+    def submit(self, tree_id, windows):
+        """windows: a BBox struct of device arrays (kept alive until released)."""
+        rid = ctypes.c_int64()
+        _check(lib.curtree_server_submit(self.p, int(tree_id), ctypes.byref(windows),
+                                         ctypes.byref(rid)))
+        self._keep[rid.value] = windows
+        return rid.value
+
+    def poll(self, rid):
+        """(state, DevicePairs-like (n, fid_ptr, tid_ptr) or None, latency_us)."""
+        st = ctypes.c_int()
+        out = IdPair64()
+        lat = ctypes.c_double()
+        rc = lib.curtree_server_poll(self.p, int(rid), ctypes.byref(st), ctypes.byref(out),
+                                     ctypes.byref(lat))
+        if st.value == REQ_FAILED:
+            raise CurtreeError(rc, lib.curtree_last_error().decode(errors="replace"))
+        _check(rc)
+        res = (int(out.sz), out.fid, out.tid) if st.value == REQ_DONE else None
+        return st.value, res, lat.value
+
+    def release(self, rid):
+        _check(lib.curtree_server_release(self.p, int(rid)))
+        self._keep.pop(rid, None)
+
+    def stats(self):
+        s = ServerStats()
+        _check(lib.curtree_server_get_stats(self.p, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_}
+
+    def reset_stats(self):
+        _check(lib.curtree_server_reset_stats(self.p))
+
+    def close(self):
+        if self.p:
+            _check(lib.curtree_server_destroy(self.p))
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def window_view(boxes, start, count):
+    """A BBox struct viewing windows [start, start+count) of device boxes."""
+    v = BBox()
+    v.sz = int(count)
+    for f in ("xmin", "ymin", "xmax", "ymax"):
+        base = getattr(boxes.s, f)
+        setattr(v, f, (base or 0) + 4 * int(start))
+    v.id = None
+    return v
 
 
 def self_join(rt, rbox, r, stream=None):

@@ -46,7 +46,11 @@ int query_batched_f32(Context& ctx, const curtree_rtree_f32* rt, const curtree_b
                       void* user, curtree_batch_stats* stats) {
   if (!rt || !rbox || !qbox || budget <= 0)
     return set_error(CURTREE_EINVAL, "batched: bad argument");
-  BatchState& bs = ctx.batch;
+  BatchState& bs = ctx.batch;  // caller holds bs.mu
+  // the context lock covers the setup (profiler, options); the chunk loop
+  // below touches batch-owned resources only and runs without it, so chunk
+  // callbacks may call other entry points
+  std::unique_lock<std::mutex> lk(ctx.mu);
   int rc;
   if (!bs.stream) CT_HIP(hipStreamCreateWithFlags(&bs.stream, hipStreamNonBlocking));
   if (!bs.ev0) {
@@ -62,8 +66,8 @@ int query_batched_f32(Context& ctx, const curtree_rtree_f32* rt, const curtree_b
     return CURTREE_OK;
   }
   // ---- pass 1: per-window pair counts
-  if ((rc = ctx.ws.reserve(WsBatchCounts, (size_t)Q * 8))) return rc;
-  int64_t* counts = (int64_t*)ctx.ws.ptr(WsBatchCounts);
+  if ((rc = bs.ws.reserve(WsBatchCounts, (size_t)Q * 8))) return rc;
+  int64_t* counts = (int64_t*)bs.ws.ptr(WsBatchCounts);
   CT_HIP(hipEventRecord(bs.ev0, s));
   QueryOut qo;
   qo.count_only = true;
@@ -90,10 +94,10 @@ int query_batched_f32(Context& ctx, const curtree_rtree_f32* rt, const curtree_b
   if ((rc = qe.reserve_front(std::max<int64_t>((int64_t)front_need + 1024, 1 << 16)))) return rc;
   int64_t max_q = 0;
   for (int64_t k = 0; k < nchunks; k++) max_q = std::max(max_q, plan[2 * k + 1] - plan[2 * k]);
-  if ((rc = qe.reserve_status(std::max<int64_t>(qe.cap_front, max_q) / per_tile(a.G) + 2)))
+  if ((rc = qe.reserve_status(std::max<int64_t>(qe.cap_front, max_q) / per_tile(a.G, a.steps) + 2)))
     return rc;
-  if ((rc = ctx.ws.reserve(WsBatchOut, (size_t)out_cap * 8))) return rc;
-  int* fid = (int*)ctx.ws.ptr(WsBatchOut);
+  if ((rc = bs.ws.reserve(WsBatchOut, (size_t)out_cap * 8))) return rc;
+  int* fid = (int*)bs.ws.ptr(WsBatchOut);
   int* tid = fid + out_cap;
   a.write = true;
   a.out_fid = fid;
@@ -125,6 +129,7 @@ int query_batched_f32(Context& ctx, const curtree_rtree_f32* rt, const curtree_b
     return set_error(CURTREE_EHIP, "batched: graph instantiation failed: %s",
                      hipGetErrorString(ce));
   }
+  lk.unlock();
   rc = CURTREE_OK;
   double write_ms = 0.0;
   for (int64_t k = 0; k < nchunks && !rc; k++) {

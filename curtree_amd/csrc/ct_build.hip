@@ -9,9 +9,10 @@
 namespace ct {
 
 // ---------------------------------------------------------------------------
-// Block-wide exclusive scan over 256 threads (4 waves).  s_tmp holds 4 words.
-__device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* s_tmp,
-                                                       uint32_t* total) {
+// Block-wide exclusive scan over NW waves.  s_tmp holds NW words.
+template <int NW>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_tmp,
+                                                    uint32_t* total) {
   const int lane = lane_id(), w = threadIdx.x >> 6;
   uint32_t x = v;
   for (int o = 1; o < kWave; o <<= 1) {
@@ -21,7 +22,7 @@ __device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* s_t
   if (lane == kWave - 1) s_tmp[w] = x;
   __syncthreads();
   uint32_t off = 0, tot = 0;
-  for (int i = 0; i < 4; i++) {
+  for (int i = 0; i < NW; i++) {
     uint32_t t = s_tmp[i];
     if (i < w) off += t;
     tot += t;
@@ -216,7 +217,7 @@ __global__ __launch_bounds__(256) void k_sort_plan(int64_t n, int D, BuildCtrl* 
     uint32_t v = c->hist[p * 256 + threadIdx.x];
     int triv = __syncthreads_or(v == (uint32_t)n);
     uint32_t tot;
-    uint32_t ex = block_excl_scan256(v, s_tmp, &tot);
+    uint32_t ex = block_excl_scan<4>(v, s_tmp, &tot);
     c->base[p * 256 + threadIdx.x] = ex;
     if (threadIdx.x == 0) s_triv[p] = triv;
   }
@@ -248,19 +249,30 @@ __global__ __launch_bounds__(256) void k_sort_plan(int64_t n, int D, BuildCtrl* 
 // the input order and the pass is stable.  Per-digit tile counts are chained
 // across tiles by decoupled look-back; tiles are numbered by an atomic ticket
 // so every tile waits only on tiles that have already started.
-constexpr int kSortThreads = 256;
-constexpr int kSortItems = 16;
+#ifndef CT_SORT_THREADS
+#define CT_SORT_THREADS 256
+#endif
+#ifndef CT_SORT_ITEMS
+#define CT_SORT_ITEMS 16
+#endif
+#ifndef CT_LOOKBACK
+#define CT_LOOKBACK 4
+#endif
+constexpr int kSortThreads = CT_SORT_THREADS;  // 256 or 512
+constexpr int kSortWaves = kSortThreads / kWave;
+constexpr int kSortItems = CT_SORT_ITEMS;
 constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr uint32_t kFlagA = 1u << 30, kFlagP = 2u << 30, kValMask = (1u << 30) - 1u;
+static_assert(kSortThreads >= 256, "one thread per digit");
 
 template <class KT>
 __global__ __launch_bounds__(kSortThreads) void k_onesweep(
     int64_t n, int pass, KT* __restrict__ k0, KT* __restrict__ k1, uint32_t* __restrict__ v0,
     uint32_t* __restrict__ v1, BuildCtrl* c, uint32_t* __restrict__ status, int keys_only) {
-  __shared__ uint32_t s_wh[4][256];
+  __shared__ uint32_t s_wh[kSortWaves][256];
   __shared__ uint32_t s_start[256];
   __shared__ uint32_t s_adj[256];
-  __shared__ uint32_t s_tmp[4];
+  __shared__ uint32_t s_tmp[kSortWaves];
   __shared__ uint32_t s_tile;
   __shared__ KT s_buf[kSortTile];
   __shared__ uint8_t s_dig[kSortTile];
@@ -269,7 +281,7 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
   const int t = threadIdx.x, w = t >> 6, lane = lane_id();
   const int shift = 8 * pass;
   if (t == 0) s_tile = atomicAdd(&c->ticket[pass], 1u);
-  for (int i = t; i < 4 * 256; i += kSortThreads) (&s_wh[0][0])[i] = 0;
+  for (int i = t; i < kSortWaves * 256; i += kSortThreads) (&s_wh[0][0])[i] = 0;
   __syncthreads();
   const int64_t tile = s_tile;
   const int src = c->src[pass];
@@ -307,53 +319,58 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
   }
   __syncthreads();
 
-  // thread t owns digit t: wave offsets, tile count, look-back, tile-local start
+  // threads 0..255 own one digit each: wave offsets, tile count, look-back
   const int dg = t;
-  uint32_t c0 = s_wh[0][dg], c1 = s_wh[1][dg], c2 = s_wh[2][dg], c3 = s_wh[3][dg];
-  s_wh[0][dg] = 0;
-  s_wh[1][dg] = c0;
-  s_wh[2][dg] = c0 + c1;
-  s_wh[3][dg] = c0 + c1 + c2;
-  const uint32_t tot = c0 + c1 + c2 + c3;
-  uint32_t* st = status + (size_t)tile * 256;
-  uint32_t excl = 0;
-  if (tile == 0) {
-    st_agent(st + dg, kFlagP | tot);
-  } else {
-    st_agent(st + dg, kFlagA | tot);
-    // look back 4 tiles per step (4 loads in flight); stop at the nearest
-    // inclusive prefix; tiles before the first count as prefix 0
-    int64_t j = tile - 1;
-    for (;;) {
-      uint32_t v[4];
+  uint32_t tot = 0, excl = 0;
+  if (t < 256) {
 #pragma unroll
-      for (int k = 0; k < 4; k++)
-        v[k] = (j - k >= 0) ? ld_agent(status + (size_t)(j - k) * 256 + dg) : kFlagP;
-      int stop = 4;
-      bool ready = true;
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        if (stop < 4) break;
-        uint32_t f = v[k] & ~kValMask;
-        if (f == 0u) {
-          ready = false;
-          break;
-        }
-        if (f == kFlagP) stop = k;
-      }
-      if (!ready) continue;
-#pragma unroll
-      for (int k = 0; k < 4; k++)
-        if (k <= stop) excl += v[k] & kValMask;
-      if (stop < 4) break;
-      j -= 4;
+    for (int i = 0; i < kSortWaves; i++) {
+      const uint32_t ci = s_wh[i][dg];
+      s_wh[i][dg] = tot;
+      tot += ci;
     }
-    st_agent(st + dg, kFlagP | (excl + tot));
+    uint32_t* st = status + (size_t)tile * 256;
+    if (tile == 0) {
+      st_agent(st + dg, kFlagP | tot);
+    } else {
+      st_agent(st + dg, kFlagA | tot);
+      // look back kLB tiles per step (kLB loads in flight); stop at the
+      // nearest inclusive prefix; tiles before the first count as prefix 0
+      constexpr int kLB = CT_LOOKBACK;
+      int64_t j = tile - 1;
+      for (;;) {
+        uint32_t v[kLB];
+#pragma unroll
+        for (int k = 0; k < kLB; k++)
+          v[k] = (j - k >= 0) ? ld_agent(status + (size_t)(j - k) * 256 + dg) : kFlagP;
+        int stop = kLB;
+        bool ready = true;
+#pragma unroll
+        for (int k = 0; k < kLB; k++) {
+          if (stop < kLB) break;
+          uint32_t f = v[k] & ~kValMask;
+          if (f == 0u) {
+            ready = false;
+            break;
+          }
+          if (f == kFlagP) stop = k;
+        }
+        if (!ready) continue;
+#pragma unroll
+        for (int k = 0; k < kLB; k++)
+          if (k <= stop) excl += v[k] & kValMask;
+        if (stop < kLB) break;
+        j -= kLB;
+      }
+      st_agent(st + dg, kFlagP | (excl + tot));
+    }
   }
   uint32_t ttot;
-  const uint32_t start = block_excl_scan256(tot, s_tmp, &ttot);
-  s_start[dg] = start;
-  s_adj[dg] = c->base[pass * 256 + dg] + excl - start;
+  const uint32_t start = block_excl_scan<kSortWaves>(tot, s_tmp, &ttot);
+  if (t < 256) {
+    s_start[dg] = start;
+    s_adj[dg] = c->base[pass * 256 + dg] + excl - start;
+  }
   __syncthreads();
 
   // local reorder of keys through LDS, then scatter runs of equal digits

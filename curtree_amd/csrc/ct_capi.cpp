@@ -8,7 +8,7 @@
 #include <stdio.h>
 #include <string.h>
 
-#include "ct_internal.hpp"
+#include "ct_query.hpp"
 
 namespace ct {
 
@@ -248,7 +248,9 @@ CURTREE_API int curtree_query_batched_f32(const curtree_rtree_f32* rt, const cur
                                           curtree_chunk_fn fn, void* user,
                                           curtree_batch_stats* stats) {
   Context& c = context();
-  std::lock_guard<std::mutex> g(c.mu);
+  std::unique_lock<std::mutex> b(c.batch.mu, std::try_to_lock);
+  if (!b.owns_lock())
+    return set_error(CURTREE_EBUSY, "batched query already running (nested in a chunk callback?)");
   return query_batched_f32(c, rt, rbox, qbox, budget_pairs, fn, user, stats);
 }
 
@@ -443,9 +445,25 @@ CURTREE_API int curtree_device_info(int* count, char* name, int name_len, int* c
 CURTREE_API int curtree_release_workspace(void) {
   Context& c = context();
   std::lock_guard<std::mutex> g(c.mu);
+  std::unique_lock<std::mutex> b(c.batch.mu, std::try_to_lock);
+  if (!b.owns_lock()) return set_error(CURTREE_EBUSY, "release_workspace: batched query running");
   CT_HIP(hipDeviceSynchronize());
   c.ws.release();
   c.qe.release();
+  c.batch.ws.release();
+  c.batch.qe.release();
+  return CURTREE_OK;
+}
+
+CURTREE_API int curtree_query_last_stats(int64_t* entries, int cap, int* levels, int64_t* pairs,
+                                         int64_t* pieces) {
+  Context& c = context();
+  std::lock_guard<std::mutex> g(c.mu);
+  const int nl = c.qe.host_ctrl ? c.qe.last_nl : 0;
+  if (levels) *levels = nl;
+  for (int l = 0; l < nl && l < cap && entries; l++) entries[l] = (int64_t)c.qe.host_ctrl->count[l];
+  if (pairs) *pairs = nl ? (int64_t)c.qe.host_ctrl->total : 0;
+  if (pieces) *pieces = nl ? (int64_t)c.qe.host_ctrl->npieces : 0;
   return CURTREE_OK;
 }
 
@@ -456,6 +474,22 @@ CURTREE_API int curtree_set_option(const char* name, int64_t value) {
   if (!strcmp(name, "containment")) {
     if (value < -1 || value > 1) return set_error(CURTREE_EINVAL, "containment: -1, 0 or 1");
     c.opt_contain = (int)value;
+    return CURTREE_OK;
+  }
+  if (!strcmp(name, "query_tile_steps")) {
+    if (value != 4 && value != 8 && value != 16)
+      return set_error(CURTREE_EINVAL, "query_tile_steps: 4, 8 or 16");
+    qtuning().steps = (int)value;
+    return CURTREE_OK;
+  }
+  if (!strcmp(name, "box_staging")) {
+    if (value < -1 || value > 1) return set_error(CURTREE_EINVAL, "box_staging: -1, 0 or 1");
+    qtuning().staging = (int)value;
+    return CURTREE_OK;
+  }
+  if (!strcmp(name, "query_grid")) {
+    if (value < 1 || value > 65535) return set_error(CURTREE_EINVAL, "query_grid: 1..65535");
+    qtuning().grid = (int)value;
     return CURTREE_OK;
   }
   return set_error(CURTREE_EINVAL, "set_option: unknown option %s", name);

@@ -104,6 +104,15 @@ struct Prof {
 // begin, one expansion per internal level, leaf stage).  Every size lives
 // here, so a run needs no host round trip and can be captured in a graph.
 constexpr int kMaxRunSlots = 40;
+// Level table of a bulk-loaded tree, leaf-parent level first: node i of level
+// l (global index lstart[l] + i) covers sorted boxes [i*span[l], (i+1)*span[l]).
+constexpr int kMaxLevels = 32;
+struct QLevels {
+  int nl;
+  int64_t lstart[kMaxLevels];
+  int64_t lcount[kMaxLevels];
+  int64_t span[kMaxLevels];
+};
 struct QCtrl {
   uint64_t count[kMaxRunSlots];  // count[l]: entries of frontier l (count[0]: queries of the run)
   uint64_t total;                // pairs of the run
@@ -116,6 +125,7 @@ struct QCtrl {
   uint32_t pad;
   int64_t q0, q1;                // query range of the run
   uint32_t ticket[kMaxRunSlots];
+  QLevels lv;                    // levels of the run's tree (written by the run's begin)
 };
 
 // Work buffers of one query pipeline; one engine serves one stream at a time.
@@ -132,7 +142,15 @@ struct QEngine {
   int64_t cap_safe = 0;
   int64_t* plan = nullptr;              // chunk plan (q0, q1) pairs for graph replays
   int64_t cap_plan = 0;
+  float4* box4 = nullptr;               // AoS copies staged for one query: boxes,
+  int64_t cap_box4 = 0;                 // node MBRs and windows
+  float4* node4 = nullptr;
+  int64_t cap_node4 = 0;
+  float4* win4 = nullptr;
+  int64_t cap_win4 = 0;
+  uint32_t* canon = nullptr;            // the tree has the bulk-load layout (device word)
   bool needs_clear = true;              // status/ctrl are fresh: clear on the next run
+  int last_nl = 0;                      // depths of the last run read back (host_ctrl)
   int reserve_front(int64_t entries);
   int reserve_pieces(int64_t n);
   int reserve_status(int64_t words);
@@ -143,11 +161,14 @@ struct QEngine {
   ~QEngine() { release(); }
 };
 
-// Batched path: its own stream (graphs cannot capture the null stream) and engine.
+// Resources of the batched entry point, owned by it alone: its chunk loop runs
+// without the context lock so that chunk callbacks may call the library.
 struct BatchState {
+  std::mutex mu;  // one batched call at a time
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   QEngine qe;
+  Workspace ws;   // WsBatchCounts, WsBatchOut
 };
 
 struct Context {

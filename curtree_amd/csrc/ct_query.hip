@@ -43,8 +43,9 @@ __global__ void k_ctrl_init(QCtrl* c) {
 // Start of a run: query range (from the arguments, or the next chunk of
 // `plan`), epochs for its `nslots` chained launches, cleared tickets/counts.
 __global__ void k_run_begin(QCtrl* c, int64_t q0, int64_t q1, const int64_t* plan,
-                            uint32_t nslots) {
+                            uint32_t nslots, QLevels lv) {
   if (threadIdx.x == 0) {
+    c->lv = lv;
     if (plan) {
       const uint32_t k = c->chunk++;
       q0 = plan[2 * k];
@@ -108,7 +109,9 @@ __device__ __forceinline__ uint64_t chain_tile(uint64_t wcount, int64_t tile, in
   return b;
 }
 
-// Take the next tile; returns -1 when the level's tiles are exhausted.
+// Take the next tile; returns -1 when the level's tiles are exhausted.  The
+// ticket is taken when the block is ready to work on it: a tile taken early
+// would hold up the look-back of every later tile until its block gets to it.
 __device__ __forceinline__ int64_t next_tile(uint32_t* ticket, int64_t ntiles, uint32_t* s_tile) {
   __syncthreads();  // the previous tile is done with the shared words
   if (threadIdx.x == 0) *s_tile = atomicAdd(ticket, 1u);
@@ -119,12 +122,12 @@ __device__ __forceinline__ int64_t next_tile(uint32_t* ticket, int64_t ntiles, u
 
 // Internal level `slot`: entries (query, node) whose children are nodes.
 // fin == null: the implicit first frontier (query q0 + e, root).
-template <int G>
+template <int G, int STEPS>
 __global__ __launch_bounds__(kQThreads) void k_expand(int slot, QTree tr, QBoxes qb, QCtrl* c,
                                                       const uint2* __restrict__ fin,
                                                       uint2* __restrict__ fout, int64_t cap,
                                                       uint64_t* status, int64_t r_sz) {
-  using S = QShape<G>;
+  using S = QShape<G, STEPS>;
   __shared__ uint32_t s_tile;
   __shared__ uint64_t s_wtot[4];
   __shared__ uint64_t s_base, s_total;
@@ -139,14 +142,21 @@ __global__ __launch_bounds__(kQThreads) void k_expand(int slot, QTree tr, QBoxes
     return;
   }
   const int sub = lane / G, c_ = lane % G;
+  // bulk-load layout: the entries of this launch sit on level nl-1-slot and
+  // node i's children are nodes cs + i*F .. of the level below
+  const bool canon = tr.canon && *tr.canon;
+  const int lvl = c->lv.nl - 1 - slot;
+  const int64_t ls = c->lv.lstart[lvl];
+  const int64_t cs = lvl > 0 ? c->lv.lstart[lvl - 1] : 0;
+  const int64_t cc = lvl > 0 ? c->lv.lcount[lvl - 1] : 0;
   int64_t tile;
   while ((tile = next_tile(&c->ticket[slot], ntiles, &s_tile)) >= 0) {
-    uint64_t masks[kQSteps];
-    uint32_t qk[kQSteps], ck[kQSteps];
+    uint64_t masks[STEPS];
+    uint32_t qk[STEPS], ck[STEPS];
     uint64_t wcount = 0;
     const int64_t ebase = tile * S::kPerTile + (int64_t)w * S::kPerWave + sub;
 #pragma unroll
-    for (int k = 0; k < kQSteps; k++) {
+    for (int k = 0; k < STEPS; k++) {
       const int64_t e = ebase + k * S::E;
       bool hit = false;
       uint32_t q = 0, ci = 0;
@@ -165,15 +175,25 @@ __global__ __launch_bounds__(kQThreads) void k_expand(int slot, QTree tr, QBoxes
           hit = c_ == 0;
           ci = node | kFull;
         } else {
-          const int p = tr.pos[node], L = tr.len[node];
-          if (c_ < L && (int64_t)p + L <= r_sz) {  // children must be nodes
+          int64_t p, L;
+          if (canon) {
+            const int64_t f = ((int64_t)node - ls) * tr.F;
+            p = cs + f;
+            L = cc - f < tr.F ? cc - f : tr.F;
+          } else {
+            p = tr.pos[node];
+            L = tr.len[node];
+          }
+          if (c_ < L && p + L <= r_sz) {  // children must be nodes
             const uint32_t child = (uint32_t)(p + c_);
-            const float wx1 = qb.x0[q], wy1 = qb.y0[q], wx2 = qb.x1[q], wy2 = qb.y1[q];
-            const float m1 = tr.x0[child], n1 = tr.y0[child], m2 = tr.x1[child],
-                        n2 = tr.y1[child];
-            hit = intersect(wx1, wy1, wx2, wy2, m1, n1, m2, n2);
+            const float4 wq = ld_box(qb, q);
+            const float4 mb = ld_box(tr.mbr, child);
+            hit = intersect(wq.x, wq.y, wq.z, wq.w, mb.x, mb.y, mb.z, mb.w);
             ci = child;
-            if (hit && tr.safe && contains(wx1, wy1, wx2, wy2, m1, n1, m2, n2) && tr.safe[child])
+            // (a contained subtree's box range comes from the level table,
+            // so the shortcut needs the bulk-load layout)
+            if (hit && canon && tr.safe &&
+                contains(wq.x, wq.y, wq.z, wq.w, mb.x, mb.y, mb.z, mb.w) && tr.safe[child])
               ci = child | kFull;
           }
         }
@@ -191,7 +211,7 @@ __global__ __launch_bounds__(kQThreads) void k_expand(int slot, QTree tr, QBoxes
     }
     const uint64_t lt = lanemask_lt();
 #pragma unroll
-    for (int k = 0; k < kQSteps; k++) {
+    for (int k = 0; k < STEPS; k++) {
       const uint64_t m = masks[k];
       if ((m >> lane) & 1ull) {
         const uint64_t o = b + (uint64_t)popc64(m & lt);
@@ -204,41 +224,109 @@ __global__ __launch_bounds__(kQThreads) void k_expand(int slot, QTree tr, QBoxes
 
 // Box range [first, first + cnt) below node v (a complete F-ary tree over the
 // sorted boxes: node i of level l covers boxes [i*F^(l+1), (i+1)*F^(l+1))).
-__device__ __forceinline__ void node_range(uint32_t v, const QLevels& L, int64_t nbox,
+// Levels are stored root level first, so lstart falls with l and v's level is
+// the first l with v >= lstart[l]; contained subtrees sit low in the tree, so
+// the loop usually ends after one or two steps.
+__device__ __forceinline__ void node_range(uint32_t v, const QLevels* L, int64_t nbox,
                                            int64_t* first, int64_t* cnt) {
-  int lv = 0;
-#pragma unroll
-  for (int k = 0; k < kMaxLevels; k++)
-    if (k < L.nl && (int64_t)v >= L.lstart[k] && (int64_t)v < L.lstart[k] + L.lcount[k]) lv = k;
-  const int64_t f = ((int64_t)v - L.lstart[lv]) * L.span[lv];
+  const int nl = L->nl;
+  int lv = nl - 1;
+  for (int k = 0; k < nl; k++) {
+    if ((int64_t)v >= L->lstart[k]) {
+      lv = k;
+      break;
+    }
+  }
+  const int64_t span = L->span[lv];
+  const int64_t f = ((int64_t)v - L->lstart[lv]) * span;
   const int64_t rem = nbox - f;
   *first = f;
-  *cnt = rem < L.span[lv] ? rem : L.span[lv];
+  *cnt = rem < span ? rem : span;
 }
 
 // Leaf stage.  Each entry is either a partial leaf-parent (its boxes are
 // tested: G lanes, one box each) or a contained subtree (all of its box range
-// hits, no reads).  Entry counts are prefix-summed across the wave, the waves
-// of the tile and the tiles (chained), giving every pair its output slot.
-//   COUNT: totals (and per-query totals) only.
-//   WRITE: partial entries and short ranges are written by their lanes; longer
-//          ranges are cut into pieces for k_write_pieces.  Slots beyond
-//          out_cap are dropped; the total is always counted.
-template <int G, bool WRITE>
+// hits, no reads).
+//   COUNT: per-wave totals chained across tiles (and per-query totals).
+//   WRITE: each entry's state (hit mask or range length, first box, query)
+//          is staged in LDS; a tile-wide scan gives every entry its output
+//          offset, the chained look-back the tile's base.  The block then
+//          writes the tile's pairs as one flat, coalesced sequence (each slot
+//          finds its entry by binary search); ranges longer than kFlatMax are
+//          cut into pieces for k_write_pieces.  Slots beyond out_cap are
+//          dropped; the total is always counted.  Nothing is held in
+//          registers across the look-back, which keeps occupancy high.
+constexpr uint32_t kQRange = 0x80000000u;
+constexpr uint32_t kQNone = 0xFFFFFFFFu;
+constexpr uint64_t kFlatMax = 2048;  // longer ranges go to the piece writer
+
+// Position of the r-th (0-based) set bit of m (r < popcount(m)).
+__device__ __forceinline__ uint32_t nth_set_bit(uint64_t m, uint32_t r) {
+  uint32_t pos = 0;
+#pragma unroll
+  for (int w = 32; w > 0; w >>= 1) {
+    const uint64_t low = m & ((1ull << w) - 1ull);
+    const uint32_t c = (uint32_t)__popcll(low);
+    if (r >= c) {
+      r -= c;
+      m >>= w;
+      pos += w;
+    } else {
+      m = low;
+    }
+  }
+  return pos;
+}
+
+// Exclusive scan of one u64 per thread over a 256-thread block.
+__device__ __forceinline__ uint64_t block_excl_scan64(uint64_t v, uint64_t* s_tmp, uint64_t* total) {
+  const int lane = lane_id(), w = threadIdx.x >> 6;
+  uint64_t x = v;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint64_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == kWave - 1) s_tmp[w] = x;
+  __syncthreads();
+  uint64_t off = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < kQThreads / kWave; i++) {
+    const uint64_t t = s_tmp[i];
+    if (i < w) off += t;
+    tot += t;
+  }
+  __syncthreads();
+  *total = tot;
+  return x - v + off;
+}
+
+template <int G, int STEPS, bool WRITE>
 __global__ __launch_bounds__(kQThreads) void k_leaf_pass(
-    int slot, QTree tr, QBoxes qb, QBoxes rb, const int* __restrict__ rid, QLevels lv, QCtrl* c,
+    int slot, QTree tr, QBoxes qb, QBoxes rb, const int* __restrict__ rid, QCtrl* c,
     const uint2* __restrict__ fin, int64_t r_sz, int64_t nbox, int* __restrict__ out_fid,
     int* __restrict__ out_tid, int64_t out_cap, QPiece* __restrict__ pieces, int64_t piece_cap,
     unsigned long long* counts, uint64_t* status) {
-  using S = QShape<G>;
+  using S = QShape<G, STEPS>;
+  constexpr int P = S::kPerTile;                     // entries per tile
+  constexpr int R = (P + kQThreads - 1) / kQThreads;  // entries per thread in the tile scan
+  constexpr int PW = WRITE ? P : 1;
   __shared__ uint32_t s_tile;
   __shared__ uint64_t s_wtot[4];
   __shared__ uint64_t s_base, s_total;
+  __shared__ uint64_t s_val[PW];   // partial: hit mask; range: length
+  __shared__ uint64_t s_off[PW];   // tile-relative first output slot
+  __shared__ int64_t s_first[PW];  // first sorted box
+  __shared__ uint32_t s_q[PW];     // query | kQRange, kQNone past the end
+  __shared__ uint32_t s_foff[PW];  // tile-relative first flat slot
+  __shared__ uint32_t s_ftot;
+  __shared__ uint64_t s_scan[kQThreads / kWave];
   if (c->overflow) return;
   const int64_t n_in = (int64_t)c->count[slot];
   const uint32_t epoch = c->epoch_base + slot;
   const int64_t q0 = c->q0;
-  const int64_t ntiles = (n_in + S::kPerTile - 1) / S::kPerTile;
+  const QLevels* lv = &c->lv;
+  const int64_t ntiles = (n_in + P - 1) / P;
   const int t = threadIdx.x, w = t >> 6, lane = lane_id();
   if (ntiles == 0) {
     if (blockIdx.x == 0 && t == 0) c->total = 0;
@@ -246,20 +334,17 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_pass(
   }
   const int sub = lane / G, c_ = lane % G;
   const int lead = sub * G;  // first lane of this lane's group
+  // bulk-load layout: leaf-parent i holds sorted boxes i*F .. i*F+F-1
+  const bool canon = tr.canon && *tr.canon;
+  const int64_t ls = lv->lstart[0];
   int64_t tile;
   while ((tile = next_tile(&c->ticket[slot], ntiles, &s_tile)) >= 0) {
-    uint64_t gmask[kQSteps];   // the group's hit bits (partial entries)
-    uint64_t ecnt[kQSteps];    // pairs of this lane's entry
-    uint64_t epre[kQSteps];    // pairs of earlier entries in the same step
-    uint64_t stot[kQSteps];    // pairs of the whole step
-    int64_t efirst[kQSteps];   // first sorted box of the entry
-    uint32_t qk[kQSteps];
-    bool efull[kQSteps];
     uint64_t wcount = 0;
-    const int64_t ebase = tile * S::kPerTile + (int64_t)w * S::kPerWave + sub;
+    const int64_t tbase = tile * P;
 #pragma unroll
-    for (int k = 0; k < kQSteps; k++) {
-      const int64_t e = ebase + k * S::E;
+    for (int k = 0; k < STEPS; k++) {
+      const int li = w * S::kPerWave + k * S::E + sub;  // entry index within the tile
+      const int64_t e = tbase + li;
       bool hit = false, full = false;
       uint32_t q = 0;
       int64_t first = 0, rcnt = 0;
@@ -276,73 +361,122 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_pass(
         if (full) {
           node_range(node, lv, nbox, &first, &rcnt);
         } else {
-          const int p = tr.pos[node], L = tr.len[node];
-          first = (int64_t)p - r_sz;
+          int64_t L;
+          if (canon) {
+            first = ((int64_t)node - ls) * tr.F;
+            L = nbox - first < tr.F ? nbox - first : tr.F;
+          } else {
+            first = (int64_t)tr.pos[node] - r_sz;
+            L = tr.len[node];
+          }
           const int64_t bi = first + c_;
-          if (c_ < L && bi >= 0 && bi < nbox)  // children must be boxes
-            hit = intersect(qb.x0[q], qb.y0[q], qb.x1[q], qb.y1[q], rb.x0[bi], rb.y0[bi],
-                            rb.x1[bi], rb.y1[bi]);
+          if (c_ < L && bi >= 0 && bi < nbox) {  // children must be boxes
+            const float4 wq = ld_box(qb, q);
+            const float4 bx = ld_box(rb, bi);
+            hit = intersect(wq.x, wq.y, wq.z, wq.w, bx.x, bx.y, bx.z, bx.w);
+          }
         }
       }
       const uint64_t m = ballot(hit);
       const uint64_t gm = (m >> lead) & S::kGroupMask;
       const uint64_t cnt = full ? (uint64_t)rcnt : (uint64_t)popc64(gm);
-      // exclusive prefix of entry counts over the step's groups
-      uint64_t v = c_ == 0 ? cnt : 0ull, inc = v;
-#pragma unroll
-      for (int o = 1; o < kWave; o <<= 1) {
-        const uint64_t y = __shfl_up(inc, o);
-        if (lane >= o) inc += y;
+      if (c_ == 0) {
+        if (WRITE) {
+          s_val[li] = full ? cnt : gm;
+          s_off[li] = cnt;
+          s_first[li] = first;
+          s_q[li] = e < n_in ? (q | (full ? kQRange : 0u)) : kQNone;
+        } else {
+          if (counts && e < n_in && cnt) atomicAdd(counts + q, (unsigned long long)cnt);
+        }
       }
-      const uint64_t pre = __shfl(inc - v, lead);
-      const uint64_t tot = __shfl(inc, kWave - 1);
-      gmask[k] = gm;
-      ecnt[k] = cnt;
-      epre[k] = pre;
-      stot[k] = tot;
-      efirst[k] = first;
-      qk[k] = q;
-      efull[k] = full;
-      wcount += tot;
-      if (counts && c_ == 0 && e < n_in && cnt) atomicAdd(counts + q, (unsigned long long)cnt);
+      if (!WRITE) {
+        // wave total of this step (group leaders hold the counts)
+        uint64_t v = c_ == 0 ? cnt : 0ull;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        wcount += v;
+      }
     }
-    uint64_t b = chain_tile(wcount, tile, ntiles, status, epoch, s_wtot, &s_base, &s_total);
+    if (WRITE) {
+      __syncthreads();
+      // tile-wide exclusive scans of the entry counts, in entry order: all
+      // pairs (the output slots) and the flat ones (partial entries and
+      // ranges up to kFlatMax, written by the whole block below)
+      uint64_t loc[R];
+      uint64_t sum = 0, fsum = 0;
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int li = t * R + r;
+        loc[r] = li < P ? s_off[li] : 0ull;
+        sum += loc[r];
+        if (li < P && !((s_q[li] & kQRange) && s_q[li] != kQNone && loc[r] > kFlatMax))
+          fsum += loc[r];
+      }
+      uint64_t ttot, ftot;
+      uint64_t ex = block_excl_scan64(sum, s_scan, &ttot);
+      uint64_t fex = block_excl_scan64(fsum, s_scan, &ftot);
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int li = t * R + r;
+        if (li < P) {
+          const bool piece = (s_q[li] & kQRange) && s_q[li] != kQNone && loc[r] > kFlatMax;
+          s_off[li] = ex;
+          s_foff[li] = (uint32_t)fex;
+          if (!piece) fex += loc[r];
+          if (counts && loc[r] && s_q[li] != kQNone)
+            atomicAdd(counts + (s_q[li] & ~kQRange), (unsigned long long)loc[r]);
+        }
+        ex += loc[r];
+      }
+      if (t == 0) s_ftot = (uint32_t)ftot;
+      wcount = w == 0 ? ttot : 0ull;  // the tile total enters through wave 0
+    }
+    chain_tile(wcount, tile, ntiles, status, epoch, s_wtot, &s_base, &s_total);
     if (tile == ntiles - 1 && t == 0) c->total = s_total;
     if (WRITE) {
+      const uint64_t base = s_base;
+      // long ranges: pieces for k_write_pieces
 #pragma unroll
-      for (int k = 0; k < kQSteps; k++) {
-        const uint64_t off = b + epre[k];  // the entry's first output slot
-        if (!efull[k]) {
-          const uint64_t gm = gmask[k];
-          if ((gm >> c_) & 1ull) {
-            const uint64_t o = off + (uint64_t)popc64(gm & ((1ull << c_) - 1ull));
-            if (o < (uint64_t)out_cap) {
-              out_fid[o] = rid[efirst[k] + c_];
-              out_tid[o] = (int)qk[k];
-            }
-          }
-        } else if (ecnt[k] <= (uint64_t)(4 * G)) {
-          for (uint64_t j = c_; j < ecnt[k]; j += G) {
-            if (off + j < (uint64_t)out_cap) {
-              out_fid[off + j] = rid[efirst[k] + (int64_t)j];
-              out_tid[off + j] = (int)qk[k];
-            }
-          }
-        } else if (c_ == 0) {
-          const uint64_t np = (ecnt[k] + kPiece - 1) / kPiece;
-          const uint64_t at = atomicAdd(&c->npieces, (unsigned long long)np);
-          for (uint64_t p = 0; p < np; p++) {
-            if (at + p >= (uint64_t)piece_cap) break;  // only when the total exceeds out_cap
-            const uint64_t rest = ecnt[k] - p * kPiece;
-            QPiece pc;
-            pc.off = off + p * kPiece;
-            pc.first = efirst[k] + (int64_t)(p * kPiece);
-            pc.len = (uint32_t)(rest < kPiece ? rest : kPiece);
-            pc.q = qk[k];
-            pieces[at + p] = pc;
-          }
+      for (int r = 0; r < R; r++) {
+        const int li = t * R + r;
+        if (li >= P) break;
+        const uint32_t qf = s_q[li];
+        if (qf == kQNone || !(qf & kQRange)) continue;
+        const uint64_t ev = s_val[li];
+        if (ev <= kFlatMax) continue;
+        const uint64_t off = base + s_off[li];
+        const uint64_t np = (ev + kPiece - 1) / kPiece;
+        const uint64_t at = atomicAdd(&c->npieces, (unsigned long long)np);
+        for (uint64_t p = 0; p < np; p++) {
+          if (at + p >= (uint64_t)piece_cap) break;  // only when the total exceeds out_cap
+          const uint64_t rest = ev - p * kPiece;
+          QPiece pc;
+          pc.off = off + p * kPiece;
+          pc.first = s_first[li] + (int64_t)(p * kPiece);
+          pc.len = (uint32_t)(rest < kPiece ? rest : kPiece);
+          pc.q = qf & ~kQRange;
+          pieces[at + p] = pc;
         }
-        b += stot[k];
+      }
+      // flat slots, load-balanced over the block: slot j belongs to the last
+      // entry whose flat offset is <= j (binary search), coalesced writes
+      const uint32_t ftot = s_ftot;
+#pragma unroll 4
+      for (uint32_t j = t; j < ftot; j += kQThreads) {
+        int lo = 0;
+#pragma unroll
+        for (int step = P / 2; step > 0; step >>= 1)
+          if (s_foff[lo + step] <= j) lo += step;
+        const uint32_t qf = s_q[lo];
+        const uint32_t r = j - s_foff[lo];
+        uint32_t bpos = r;
+        if (!(qf & kQRange)) bpos = nth_set_bit(s_val[lo], r);
+        const uint64_t o = base + s_off[lo] + r;
+        if (o < (uint64_t)out_cap) {
+          out_fid[o] = rid[s_first[lo] + bpos];
+          out_tid[o] = (int)(qf & ~kQRange);
+        }
       }
     }
   }
@@ -470,9 +604,12 @@ int QEngine::ensure_ctrl() {
 }
 
 void QEngine::release() {
-  void* ps[] = {front[0], front[1], pieces, status, ctrl, safe, plan};
+  void* ps[] = {front[0], front[1], pieces, status, ctrl, safe, plan, box4, node4, win4, canon};
   for (void* p : ps)
     if (p) (void)hipFree(p);
+  box4 = node4 = win4 = nullptr;
+  canon = nullptr;
+  cap_box4 = cap_node4 = cap_win4 = 0;
   if (host_ctrl) (void)hipHostFree(host_ctrl);
   front[0] = front[1] = pieces = nullptr;
   status = nullptr;
@@ -484,13 +621,82 @@ void QEngine::release() {
   needs_clear = true;
 }
 
+QTuning& qtuning() {
+  static QTuning t;
+  return t;
+}
+
 int next_pow2_group(int F) {
   int g = 2;
   while (g < F) g <<= 1;
   return g;
 }
 
-int per_tile(int G) { return (kQThreads / kWave) * kQSteps * (kWave / G); }
+int per_tile(int G, int steps) { return (kQThreads / kWave) * steps * (kWave / G); }
+
+__global__ void k_set_u32(uint32_t* p, uint32_t v) {
+  if (threadIdx.x == 0) *p = v;
+}
+
+// Nodes [start, start+count) of one level must point at F consecutive
+// children starting at child0 + i*F (the last one fewer: nchild in all).
+__global__ __launch_bounds__(256) void k_check_level(int64_t start, int64_t count, int64_t child0,
+                                                     int64_t nchild, int F, const int* pos,
+                                                     const int* len, uint32_t* flag) {
+  const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+  if (i >= count) return;
+  const int64_t f = i * F;
+  const int64_t want_len = nchild - f < F ? nchild - f : F;
+  if ((int64_t)pos[start + i] != child0 + f || (int64_t)len[start + i] != want_len) *flag = 0u;
+}
+
+int check_layout(uint32_t** flag, const curtree_rtree_f32* rt, int64_t nbox, hipStream_t s) {
+  if (!*flag && hipMalloc((void**)flag, 256) != hipSuccess) {
+    (void)hipGetLastError();
+    *flag = nullptr;
+    return set_error(CURTREE_ENOMEM, "query: cannot allocate a flag");
+  }
+  LevelPlan lp;
+  level_plan(nbox, rt->fanout, &lp);
+  hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, s, *flag, lp.nl > 0 ? 1u : 0u);
+  int64_t start = lp.r_sz;
+  for (int l = 0; l < lp.nl; l++) {
+    start -= lp.n[l];
+    const int64_t child0 = l == 0 ? lp.r_sz : start + lp.n[l];
+    const int64_t nchild = l == 0 ? nbox : lp.n[l - 1];
+    hipLaunchKernelGGL(k_check_level, dim3((unsigned)((lp.n[l] + 255) / 256)), dim3(256), 0, s,
+                       start, lp.n[l], child0, nchild, rt->fanout, (const int*)rt->pos,
+                       (const int*)rt->len, *flag);
+  }
+  CT_HIP(hipGetLastError());
+  return CURTREE_OK;
+}
+
+__global__ __launch_bounds__(256) void k_stage4(int64_t n, QBoxes src, float4* __restrict__ dst) {
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    dst[i] = make_float4(src.x0[i], src.y0[i], src.x1[i], src.y1[i]);
+}
+
+int stage_boxes(float4** dst, int64_t* cap, const QBoxes& src, int64_t n, hipStream_t s) {
+  if (n > *cap) {
+    if (*dst) (void)hipFree(*dst);
+    *dst = nullptr;
+    *cap = 0;
+    const int64_t want = n + n / 8 + 1024;
+    if (hipMalloc((void**)dst, (size_t)want * sizeof(float4)) != hipSuccess) {
+      (void)hipGetLastError();
+      *dst = nullptr;
+      return set_error(CURTREE_ENOMEM, "query: cannot stage %lld boxes", (long long)n);
+    }
+    *cap = want;
+  }
+  if (n > 0) {
+    const unsigned grid = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_stage4, dim3(grid), dim3(256), 0, s, n, src, *dst);
+  }
+  CT_HIP(hipGetLastError());
+  return CURTREE_OK;
+}
 
 int compute_safe(QEngine& qe, Prof& pf, const QBoxes& rb, int F, int64_t r_sz, int64_t nbox,
                  hipStream_t s) {
@@ -524,7 +730,7 @@ int compute_safe(QEngine& qe, Prof& pf, const QBoxes& rb, int F, int64_t r_sz, i
   return CURTREE_OK;
 }
 
-template <int G>
+template <int G, int STEPS>
 void enqueue_run_g(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
   const unsigned nslots = (unsigned)a.nl;  // nl-1 expansions + 1 leaf stage
   if (qe.needs_clear) {
@@ -532,7 +738,8 @@ void enqueue_run_g(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
     qe.needs_clear = false;
   }
   pf.mark(s, "k_run_begin");
-  hipLaunchKernelGGL(k_run_begin, dim3(1), dim3(64), 0, s, qe.ctrl, a.q0, a.q1, a.plan, nslots);
+  hipLaunchKernelGGL(k_run_begin, dim3(1), dim3(64), 0, s, qe.ctrl, a.q0, a.q1, a.plan, nslots,
+                     a.lv);
   hipLaunchKernelGGL(k_status_zero, dim3(1024), dim3(256), 0, s, (const QCtrl*)qe.ctrl, qe.status,
                      qe.status_words);
   const uint2* cur = nullptr;
@@ -540,7 +747,7 @@ void enqueue_run_g(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
   for (int l = 0; l + 1 < a.nl; l++) {
     uint2* out = (uint2*)qe.front[which];
     pf.mark(s, "k_expand");
-    hipLaunchKernelGGL(k_expand<G>, dim3(kChainGrid), dim3(kQThreads), 0, s, l, a.tr, a.qb,
+    hipLaunchKernelGGL((k_expand<G, STEPS>), dim3(a.grid), dim3(kQThreads), 0, s, l, a.tr, a.qb,
                        qe.ctrl, cur, out, qe.cap_front, qe.status, a.r_sz);
     cur = out;
     which ^= 1;
@@ -548,8 +755,8 @@ void enqueue_run_g(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
   const int leaf = a.nl - 1;
   if (a.write) {
     pf.mark(s, "k_leaf_pass");
-    hipLaunchKernelGGL((k_leaf_pass<G, true>), dim3(kChainGrid), dim3(kQThreads), 0, s, leaf, a.tr,
-                       a.qb, a.rb, a.rid, a.lv, qe.ctrl, cur, a.r_sz, a.nbox, a.out_fid,
+    hipLaunchKernelGGL((k_leaf_pass<G, STEPS, true>), dim3(a.grid), dim3(kQThreads), 0, s, leaf, a.tr,
+                       a.qb, a.rb, a.rid, qe.ctrl, cur, a.r_sz, a.nbox, a.out_fid,
                        a.out_tid, a.out_cap, (QPiece*)qe.pieces, qe.cap_pieces, a.counts,
                        qe.status);
     pf.mark(s, "k_write_pieces");
@@ -558,22 +765,31 @@ void enqueue_run_g(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
                        a.out_cap);
   } else {
     pf.mark(s, "k_leaf_count");
-    hipLaunchKernelGGL((k_leaf_pass<G, false>), dim3(kChainGrid), dim3(kQThreads), 0, s, leaf, a.tr,
-                       a.qb, a.rb, a.rid, a.lv, qe.ctrl, cur, a.r_sz, a.nbox, (int*)nullptr,
+    hipLaunchKernelGGL((k_leaf_pass<G, STEPS, false>), dim3(a.grid), dim3(kQThreads), 0, s, leaf, a.tr,
+                       a.qb, a.rb, a.rid, qe.ctrl, cur, a.r_sz, a.nbox, (int*)nullptr,
                        (int*)nullptr, (int64_t)0, (QPiece*)nullptr, (int64_t)0, a.counts,
                        qe.status);
   }
   pf.mark(s, nullptr);
 }
 
+template <int G>
+static void enqueue_run_steps(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
+  switch (a.steps) {
+    case 4: enqueue_run_g<G, 4>(qe, pf, a, s); break;
+    case 16: enqueue_run_g<G, 16>(qe, pf, a, s); break;
+    default: enqueue_run_g<G, 8>(qe, pf, a, s); break;
+  }
+}
+
 void enqueue_run(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
   switch (a.G) {
-    case 2: enqueue_run_g<2>(qe, pf, a, s); break;
-    case 4: enqueue_run_g<4>(qe, pf, a, s); break;
-    case 8: enqueue_run_g<8>(qe, pf, a, s); break;
-    case 16: enqueue_run_g<16>(qe, pf, a, s); break;
-    case 32: enqueue_run_g<32>(qe, pf, a, s); break;
-    default: enqueue_run_g<64>(qe, pf, a, s); break;
+    case 2: enqueue_run_steps<2>(qe, pf, a, s); break;
+    case 4: enqueue_run_steps<4>(qe, pf, a, s); break;
+    case 8: enqueue_run_steps<8>(qe, pf, a, s); break;
+    case 16: enqueue_run_steps<16>(qe, pf, a, s); break;
+    case 32: enqueue_run_steps<32>(qe, pf, a, s); break;
+    default: enqueue_run_steps<64>(qe, pf, a, s); break;
   }
 }
 
@@ -610,6 +826,8 @@ int prepare_query(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curt
     return set_error(CURTREE_EINVAL, "query: fanout %d > 64 is not supported by the GPU query", F);
   if (Q > (int64_t)0xFFFFFFFF) return set_error(CURTREE_EINVAL, "query: too many windows");
   *a = QRunArgs();
+  a->steps = qtuning().steps;
+  a->grid = qtuning().grid;
   a->nbox = N;
   a->F = F;
   a->G = next_pow2_group(F);
@@ -627,9 +845,12 @@ int prepare_query(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curt
     return set_error(CURTREE_EINVAL, "query: null device array");
   int rc;
   if ((rc = qe.ensure_ctrl())) return rc;
-  a->tr = QTree{rt->xmin, rt->ymin, rt->xmax, rt->ymax, rt->pos, rt->len, nullptr};
-  a->qb = QBoxes{qbox->xmin, qbox->ymin, qbox->xmax, qbox->ymax};
-  a->rb = QBoxes{rbox->xmin, rbox->ymin, rbox->xmax, rbox->ymax};
+  a->tr.mbr = QBoxes{rt->xmin, rt->ymin, rt->xmax, rt->ymax, nullptr};
+  a->tr.pos = rt->pos;
+  a->tr.len = rt->len;
+  a->tr.F = F;
+  a->qb = QBoxes{qbox->xmin, qbox->ymin, qbox->xmax, qbox->ymax, nullptr};
+  a->rb = QBoxes{rbox->xmin, rbox->ymin, rbox->xmax, rbox->ymax, nullptr};
   a->rid = rbox->id;
   a->r_sz = lp.r_sz;
   a->nl = lp.nl;
@@ -649,6 +870,26 @@ int prepare_query(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curt
   if (use) {
     if ((rc = compute_safe(qe, pf, a->rb, F, lp.r_sz, N, s))) return rc;
     a->tr.safe = qe.safe;
+  }
+  // AoS staging costs a read and a write of every box: worth it when the
+  // batch is large against the tree (callers that keep trees resident
+  // (contain == 0 here) stage once themselves)
+  const int st = qtuning().staging;
+  const bool stage = st == 1 || (st < 0 && contain != 0 && nq_hint >= 256 && nq_hint * 256 >= N);
+  if (stage) {
+    pf.mark(s, "k_stage4");
+    if ((rc = stage_boxes(&qe.box4, &qe.cap_box4, a->rb, N, s)) ||
+        (rc = stage_boxes(&qe.node4, &qe.cap_node4, a->tr.mbr, lp.r_sz, s)) ||
+        (rc = stage_boxes(&qe.win4, &qe.cap_win4, a->qb, Q, s)))
+      return rc;
+    pf.mark(s, nullptr);
+    a->rb.b4 = qe.box4;
+    a->tr.mbr.b4 = qe.node4;
+    a->qb.b4 = qe.win4;
+  }
+  if (use || stage) {
+    if ((rc = check_layout(&qe.canon, rt, N, s))) return rc;
+    a->tr.canon = qe.canon;
   }
   return CURTREE_OK;
 }
@@ -680,21 +921,38 @@ int query_run_f32(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curt
   for (int pass = 0; pass < 2; pass++) {
     for (int attempt = 0;; attempt++) {
       if (a.counts) enqueue_fill_counts(qo->counts + q0, nq, s);
-      if ((rc = qe.reserve_status(std::max<int64_t>(qe.cap_front, nq) / per_tile(a.G) + 2)))
+      if ((rc = qe.reserve_status(std::max<int64_t>(qe.cap_front, nq) / per_tile(a.G, a.steps) + 2)))
         return rc;
       if (a.write && (rc = qe.reserve_pieces(pieces_needed(qe, a)))) return rc;
       enqueue_run(qe, pf, a, s);
       CT_HIP(hipGetLastError());
       if ((rc = read_ctrl(qe, s))) return rc;
+      qe.last_nl = a.nl;
       if (!qe.host_ctrl->overflow) break;
-      if (attempt >= 3) return set_error(CURTREE_ENOMEM, "query: frontier kept overflowing");
-      // grow to the largest frontier the run reported and run again
+      // an overflow stops the run at the level that did not fit, so every
+      // attempt learns at least one more level: nl attempts always suffice
+      if (attempt > a.nl) return set_error(CURTREE_ENOMEM, "query: frontier kept overflowing");
+      const uint64_t* cnt = qe.host_ctrl->count;
       uint64_t need = 0;
-      for (int l = 1; l < a.nl; l++) need = std::max<uint64_t>(need, qe.host_ctrl->count[l]);
+      int deepest = 1;
+      for (int l = 1; l < a.nl; l++) {
+        if (cnt[l] > need) need = cnt[l];
+        if (cnt[l]) deepest = l;
+      }
       if (need > 0xFFFFFFFFull)
         return set_error(CURTREE_ERANGE, "query: frontier of %llu entries exceeds 2^32",
                          (unsigned long long)need);
-      if ((rc = qe.reserve_front((int64_t)(need + need / 4 + 1024)))) return rc;
+      // the levels below the cut are unknown: extrapolate the last growth
+      // factor (at most F) to save reruns, falling back to the known size
+      uint64_t guess = need + need / 4 + 1024;
+      if (deepest + 1 < a.nl && deepest >= 2 && cnt[deepest - 1] > 0) {
+        double r = (double)cnt[deepest] / (double)cnt[deepest - 1];
+        r = std::min<double>(std::max(r, 1.0), (double)a.F);
+        guess = std::min<uint64_t>((uint64_t)((double)need * r * 1.25) + 1024, 0xFFFFFFFFull);
+      }
+      if (qe.reserve_front((int64_t)guess) != CURTREE_OK &&
+          (rc = qe.reserve_front((int64_t)(need + need / 4 + 1024))))
+        return rc;
     }
     const uint64_t H = qe.host_ctrl->total;
     qo->n = (int64_t)H;

@@ -7,24 +7,46 @@
 namespace ct {
 
 constexpr int kQThreads = 256;
-constexpr int kQSteps = 8;       // wave-steps per chained tile
-constexpr int kChainGrid = 8192;  // blocks of a chained launch (they loop over tiles)
+// Tile of a chained launch: 4 waves x STEPS wave-steps x (64/G) entries.
+// STEPS (4, 8 or 16) and the grid (blocks looping over tiles, i.e. waves per
+// CU) are process-wide tuning knobs (curtree_set_option), picked by the
+// autotuner.
+constexpr int kDefaultSteps = 8;
+constexpr int kDefaultChainGrid = 8192;
+struct QTuning {
+  int steps = kDefaultSteps;
+  int grid = kDefaultChainGrid;
+  int staging = -1;  // AoS box staging: -1 auto, 0 off, 1 on
+};
+QTuning& qtuning();
 
-struct QTree {
-  const float* x0;
-  const float* y0;
-  const float* x1;
-  const float* y1;
-  const int* pos;
-  const int* len;
-  const uint8_t* safe;  // per node: every box below is well-formed (or null: no shortcut)
-};
+// Boxes are read from the caller's SoA arrays, or from an AoS copy (one
+// float4 per box: xmin, ymin, xmax, ymax) when the query stages one: a group's
+// F consecutive boxes are then F*16 contiguous bytes read by one dwordx4 load
+// per lane, where SoA needs four loads touching four half-used cache lines.
 struct QBoxes {
-  const float* x0;
-  const float* y0;
-  const float* x1;
-  const float* y1;
+  const float* x0 = nullptr;
+  const float* y0 = nullptr;
+  const float* x1 = nullptr;
+  const float* y1 = nullptr;
+  const float4* b4 = nullptr;  // AoS copy (or null)
 };
+struct QTree {
+  QBoxes mbr;                   // node MBRs
+  const int* pos = nullptr;
+  const int* len = nullptr;
+  const uint8_t* safe = nullptr;  // per node: every box below is well-formed (or null: no shortcut)
+  // Device word, nonzero when every node's pos/len is the bulk-load layout
+  // (children of node i of a level are i*F .. i*F+F-1 of the level below):
+  // child ranges are then computed, not loaded.  Null: always load.
+  const uint32_t* canon = nullptr;
+  int F = 0;
+};
+
+__device__ __forceinline__ float4 ld_box(const QBoxes& b, int64_t i) {
+  if (b.b4) return b.b4[i];
+  return make_float4(b.x0[i], b.y0[i], b.x1[i], b.y1[i]);
+}
 
 // Frontier entries are (query, node | kFull).  kFull marks a node whose MBR
 // lies inside the query window and whose boxes are all well-formed
@@ -39,22 +61,12 @@ __device__ __forceinline__ bool contains(float x1, float y1, float x2, float y2,
   return x1 <= m1 && m2 <= x2 && y1 <= n1 && n2 <= y2;
 }
 
-template <int G>
+template <int G, int STEPS>
 struct QShape {
   static constexpr int E = kWave / G;  // entries per wave-step
-  static constexpr int kPerWave = kQSteps * E;
+  static constexpr int kPerWave = STEPS * E;
   static constexpr int kPerTile = (kQThreads / kWave) * kPerWave;
   static constexpr uint64_t kGroupMask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
-};
-
-// Level table of a bulk-loaded tree, leaf-parent level first: node i of level
-// l (global index lstart[l] + i) covers sorted boxes [i*span[l], (i+1)*span[l]).
-constexpr int kMaxLevels = 32;
-struct QLevels {
-  int nl;
-  int64_t lstart[kMaxLevels];
-  int64_t lcount[kMaxLevels];
-  int64_t span[kMaxLevels];
 };
 
 // A piece of a contained subtree's box range, written by one block.
@@ -75,6 +87,8 @@ struct QRunArgs {
   const int* rid = nullptr;
   int64_t r_sz = 0, nbox = 0;
   int nl = 0, F = 0, G = 0;
+  int steps = kDefaultSteps;      // wave-steps per tile: 4, 8 or 16
+  int grid = kDefaultChainGrid;   // blocks of each chained launch
   int64_t q0 = 0, q1 = 0;
   const int64_t* plan = nullptr;  // take (q0, q1) from the next plan chunk instead
   bool write = false;             // leaf stage writes pairs (else counts only)
@@ -85,13 +99,18 @@ struct QRunArgs {
 };
 
 int next_pow2_group(int F);
-int per_tile(int G);
+int per_tile(int G, int steps);
 // Validate, fill `a` (tree/boxes/levels) and compute the containment flags.
 int prepare_query(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox,
                   const curtree_bbox_f32* qbox, int contain, int64_t nq_hint, QRunArgs* a,
                   hipStream_t s);
 int compute_safe(QEngine& qe, Prof& pf, const QBoxes& rb, int F, int64_t r_sz, int64_t nbox,
                  hipStream_t s);
+// Checks that rt's pos/len arrays hold the bulk-load layout; the result lands
+// in the device word *flag (allocated on first use), no host synchronisation.
+int check_layout(uint32_t** flag, const curtree_rtree_f32* rt, int64_t nbox, hipStream_t s);
+// AoS copy of n boxes into *dst (grown as needed; *cap in boxes).
+int stage_boxes(float4** dst, int64_t* cap, const QBoxes& src, int64_t n, hipStream_t s);
 // Enqueue one run (begin, expansions, leaf stage, piece writer); no host
 // synchronisation.  Needs qe.cap_pieces >= pieces_needed(qe, a).
 void enqueue_run(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s);

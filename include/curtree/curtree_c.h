@@ -147,6 +147,44 @@ CURTREE_API int curtree_query_batched_f32(const curtree_rtree_f32* rt, const cur
                                           curtree_chunk_fn fn, void* user,
                                           curtree_batch_stats* stats);
 
+/* ---- serving (no reference counterpart) ----------------------------------
+ * Several bulk-loaded trees stay resident; window requests of any size are
+ * submitted concurrently and served by continuous batching: a dispatcher
+ * thread merges queued requests on the same tree into one query batch per
+ * free slot (stream), admits a batch when its pairs fit the result budget,
+ * and overlaps the slots' work on the device.  A request's result (device
+ * pairs, query indices relative to the request, (query, sorted box position)
+ * order) is identical alone and in any mix, and stays valid until released.
+ * Window arrays passed to submit must stay valid until the request is done. */
+#define CURTREE_REQ_QUEUED 0
+#define CURTREE_REQ_RUNNING 1
+#define CURTREE_REQ_DONE 2
+#define CURTREE_REQ_FAILED 3
+typedef struct curtree_server curtree_server;
+typedef struct curtree_server_opts {
+  int slots;                 /* concurrent batches (streams), 1..16 */
+  int64_t budget_bytes;      /* result memory (8 bytes per pair) */
+  int64_t max_batch_windows; /* window cap of one batch (and of one request) */
+  int batching;              /* 0: each request alone (plain per-request launches) */
+} curtree_server_opts;
+typedef struct curtree_server_stats {
+  int64_t requests_done, pairs_done, batches, queued;
+  double p50_us, p99_us; /* submit -> completion latency */
+  double span_us;        /* first submit -> last completion */
+} curtree_server_stats;
+CURTREE_API int curtree_server_create(const curtree_server_opts* opts, curtree_server** out);
+CURTREE_API int curtree_server_add_tree(curtree_server* s, const curtree_rtree_f32* rt,
+                                        const curtree_bbox_f32* rbox, int* tree_id);
+CURTREE_API int curtree_server_submit(curtree_server* s, int tree_id,
+                                      const curtree_bbox_f32* windows, int64_t* request_id);
+/* state: CURTREE_REQ_*; result/latency filled once done */
+CURTREE_API int curtree_server_poll(curtree_server* s, int64_t request_id, int* state,
+                                    curtree_idpair64* result, double* latency_us);
+CURTREE_API int curtree_server_release(curtree_server* s, int64_t request_id);
+CURTREE_API int curtree_server_get_stats(curtree_server* s, curtree_server_stats* stats);
+CURTREE_API int curtree_server_reset_stats(curtree_server* s);
+CURTREE_API int curtree_server_destroy(curtree_server* s);
+
 /* Canonical result order: sorts pairs on the device into (query id, box id)
  * lexicographic order -- the order the reference's two host stable sorts give
  * (src/rtree_test_frontier.cu:102-105).  Pairs must arrive grouped by query
@@ -214,11 +252,30 @@ CURTREE_API const char* curtree_version(void);
 /* device count, name of device 0, CU count, HBM bytes */
 CURTREE_API int curtree_device_info(int* count, char* name, int name_len, int* cus,
                                     int64_t* hbm_bytes);
+/* Frontier sizes of the last plain query run in this process (diagnostics
+ * for measurement; no counterpart in the reference): entries[l] = (window,
+ * node) entries processed at depth l (l = 0: one per window at the root; the
+ * last depth: leaf-parent entries plus contained-subtree entries), *levels =
+ * number of depths, *pairs = pairs of the run, *pieces = long box ranges of
+ * contained subtrees written by the piece writer.  Zero levels before any
+ * query. */
+CURTREE_API int curtree_query_last_stats(int64_t* entries, int cap, int* levels, int64_t* pairs,
+                                         int64_t* pieces);
+
 /* Tuning knobs (process-wide):
  *   "containment"  -1 auto (default: on for >= 256 windows), 0 off, 1 on.  A node
  *                  whose MBR lies inside a window and whose boxes are all
  *                  well-formed is expanded without reading child MBRs/boxes;
  *                  results are identical either way.
+ *   "query_tile_steps"  4, 8 (default) or 16: wave-steps per frontier tile
+ *                  (tile = 4 waves x steps x 64/next_pow2(F) entries).
+ *   "query_grid"   blocks of each chained query launch (default 8192; they
+ *                  loop over tiles), i.e. waves per CU.
+ *   "box_staging"  -1 (default: auto), 0 off, 1 on: a query copies boxes,
+ *                  node MBRs and windows into AoS float4 arrays first (one
+ *                  extra pass over the boxes; auto when the batch has at
+ *                  least 256 windows and at least 1/256 as many windows as
+ *                  boxes).  Results are identical either way.
  * Returns CURTREE_EINVAL for an unknown name or value. */
 CURTREE_API int curtree_set_option(const char* name, int64_t value);
 

@@ -201,10 +201,11 @@ def test_tree_save_load_round_trip(tmp_path):
     assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
 
 
-@pytest.mark.parametrize("contain", [0, 1])
-def test_containment_shortcut_exact_with_malformed_boxes(contain):
+@pytest.mark.parametrize("contain,staging", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_containment_shortcut_exact_with_malformed_boxes(contain, staging):
     # inverted and NaN boxes make a subtree unsafe for the shortcut; the
-    # result must equal the oracle (which tests every child) either way
+    # result must equal the oracle (which tests every child) either way, with
+    # boxes read from the caller's SoA arrays or from the staged AoS copy
     import curtree_amd as ct
 
     n = 80_000
@@ -215,6 +216,7 @@ def test_containment_shortcut_exact_with_malformed_boxes(contain):
     bx[1][r.integers(0, n, 50)] = np.nan
     qs = datagen.loguniform_windows(3000, seed=98, lo=1e-3, hi=0.3)
     ct.set_option("containment", contain)
+    ct.set_option("box_staging", staging)
     try:
         for f, km in ((16, 1), (10, 1), (8, 0)):
             run_case(tuple(bx), np.arange(n, dtype=np.int32), qs, f, km, brute_sample=0)
@@ -230,3 +232,35 @@ def test_containment_shortcut_exact_with_malformed_boxes(contain):
         ct.lib.curtree_free(p)
     finally:
         ct.set_option("containment", -1)
+        ct.set_option("box_staging", -1)
+
+
+@pytest.mark.parametrize("contain,staging", [(0, 0), (1, 1)])
+def test_non_bulk_load_layout_follows_pos_len(contain, staging):
+    # Two leaf-parents trade places (pos, len and MBR travel together), so the
+    # nodes are no longer in bulk-load order.  The computed-child fast path
+    # must notice and follow pos/len, as the oracle does, with the containment
+    # shortcut off for such a tree.
+    import curtree_amd as ct
+
+    n = 60_000
+    bx = datagen.uniform_boxes(n, seed=141, wmax=2e-3)
+    qs = datagen.loguniform_windows(2000, seed=142, lo=1e-3, hi=0.3)
+    d, rt, q = setup(bx, np.arange(n, dtype=np.int32), qs, 16, 1)
+    h = rt.to_host()
+    r_sz = rt.sz
+    n0 = (n + 15) // 16
+    a, b = r_sz - n0 + 3, r_sz - n0 + 1000
+    for k in h:
+        h[k][a], h[k][b] = h[k][b].copy(), h[k][a].copy()
+    rt.upload(h)
+    ct.set_option("containment", contain)
+    ct.set_option("box_staging", staging)
+    try:
+        fid, tid = ct.bfs_frontier(rt, d, q)
+    finally:
+        ct.set_option("containment", -1)
+        ct.set_option("box_staging", -1)
+    sb = d.to_host()
+    ofid, otid, _ = orc.bfs(h, sb, qs)
+    assert np.array_equal(fid, ofid) and np.array_equal(tid, otid)
