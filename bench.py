@@ -12,6 +12,21 @@ is outside the timed events.
 
 Prints one JSON line.  `value` is bulk-load boxes/s; the query's pairs/s and
 per-step latency percentiles are in `query`.
+
+Other modes (same JSON contract, one line):
+  --mode batched  50,000,000 boxes (90% uniform + 10% in 5 Gaussian clusters,
+                  sigma 1e-3, sides U[0,1e-3]); 4,000,000 windows per launch,
+                  side log-uniform [1e-5, 0.05] (0 to ~1e5 hits per window);
+                  the batched entry point (count pass, chunks under a pair
+                  budget, one graph replay per chunk).  value = pairs/s.
+  --mode serve    8 resident trees of 1e7..2e8 boxes (uniform, clustered,
+                  road-like segments); Poisson request arrivals, each request
+                  on a random tree with 1..65,536 windows (log-uniform count),
+                  side log-uniform [1e-5, 2e-3]; 16 GB result budget.  The
+                  request rate is swept for plain per-request launches (one
+                  request at a time) and for the serving runtime (continuous
+                  batching over 8 streams); value = served pairs/s at the plain
+                  mode's p99 latency (at its highest sustained rate).
 """
 import argparse
 import json
@@ -62,13 +77,200 @@ def cpu_baseline(boxes, fanout, key_mode, sample):
                       f"{sample:,} boxes of the same workload, 1 thread, {dt:.2f} s"}
 
 
+def bench_batched(args, ct, datagen):
+    n = args.boxes if args.boxes_set else 50_000_000
+    nq = args.windows if args.windows_set else 4_000_000
+    F = args.fanout
+    t0 = time.time()
+    boxes = datagen.clustered_boxes(n, seed=5, frac=0.1, k=5, sigma=1e-3, wmax=1e-3)
+    wins = datagen.loguniform_windows(nq, seed=6, lo=1e-5, hi=0.05)
+    print(f"[bench] generated {n:,} boxes, {nq:,} windows in {time.time() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    d = ct.DeviceBoxes.from_host(*boxes, np.arange(n, dtype=np.int32))
+    del boxes
+    rt = ct.build_rtree(d, F, ct.KEY_MORTON64)
+    q = ct.DeviceBoxes.from_host(*wins)
+    chunk_pairs = []
+
+    def on_chunk(fid, tid, npairs, q0, q1):
+        chunk_pairs.append(npairs)  # pairs stay on the device; a consumer would read them here
+        return True
+
+    def step():
+        chunk_pairs.clear()
+        ct.synchronize()
+        t = time.perf_counter()
+        st = ct.query_batched(rt, d, q, args.budget_pairs, on_chunk)
+        ct.synchronize()
+        return time.perf_counter() - t, st
+
+    for _ in range(args.warmup):
+        step()
+    print("[bench] warmup done", file=sys.stderr, flush=True)
+    rec = []
+    for i in range(args.steps):
+        dt, st = step()
+        rec.append((dt, st))
+        print(f"[bench] step {i}: {dt * 1e3:.1f} ms, {st['pairs']:,} pairs in {st['chunks']} "
+              f"chunks (count {st['count_ms']:.1f} ms, write {st['write_ms']:.1f} ms)",
+              file=sys.stderr, flush=True)
+    ms = np.array([r[0] for r in rec]) * 1e3
+    st = rec[-1][1]
+    total, counts = ct.query_count(rt, d, q, per_query=True)
+    out = {
+        "metric": METRIC, "value": st["pairs"] / (ms.mean() * 1e-3), "unit": "pairs/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": float(ms.mean()),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"batched uneven windows: {n:,} boxes (90% uniform + 10% in 5 "
+                               f"Gaussian clusters), {nq:,} windows side log-uniform [1e-5,0.05], "
+                               f"fanout {F}, budget {args.budget_pairs:,} pairs per chunk",
+                   "boxes": n, "windows": nq, "fanout": F, "budget_pairs": args.budget_pairs},
+        "batched": {"pairs": int(st["pairs"]), "chunks": int(st["chunks"]),
+                    "max_chunk_pairs": int(st["max_chunk_pairs"]),
+                    "count_ms": float(st["count_ms"]), "write_ms": float(st["write_ms"]),
+                    "hits_per_window": {"max": int(counts.max()), "mean": float(counts.mean()),
+                                        "zero_frac": float((counts == 0).mean())},
+                    "step_p50_ms": float(np.percentile(ms, 50)),
+                    "step_p99_ms": float(np.percentile(ms, 99))},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def _poisson_run(ct, sv, tree_ids, pool, npool, lam, duration, seed, drain_s=20.0):
+    """Poisson arrivals at `lam` requests/s for `duration` s; latency from the
+    arrival time to the client seeing the completion."""
+    r = np.random.default_rng(seed)
+    sv.reset_stats()
+    t0 = time.perf_counter()
+    next_t = t0 + r.exponential(1.0 / lam)
+    arrivals = {}
+    lat = []
+    offered = 0
+    t_end = t0
+    while True:
+        now = time.perf_counter()
+        while next_t <= now and next_t - t0 < duration:
+            k = int(np.exp(r.uniform(0, np.log(65536))))
+            start = int(r.integers(0, npool - k))
+            tree = int(r.integers(0, len(tree_ids)))
+            rid = sv.submit(tree_ids[tree], ct.window_view(pool, start, k))
+            arrivals[rid] = next_t
+            offered += 1
+            next_t += r.exponential(1.0 / lam)
+        done_ids = sv.completed()
+        if done_ids:
+            t = time.perf_counter()
+            for rid in done_ids:
+                lat.append(t - arrivals.pop(rid))
+                sv.release(rid)  # the pairs were consumed on the device
+            t_end = t
+        if now - t0 >= duration and not arrivals:
+            break
+        if now - t0 >= duration + drain_s:
+            break  # overloaded: the backlog does not drain
+        time.sleep(50e-6)
+    done = len(lat)
+    sst = sv.stats()
+    pairs = sst["pairs_done"]
+    lat_ms = np.array(lat) * 1e3 if lat else np.array([np.inf])
+    span = max(t_end - t0, 1e-9)
+    return {"rate": lam, "offered": offered, "done": done, "pairs": int(pairs),
+            "pairs_per_s": pairs / span, "p50_ms": float(np.percentile(lat_ms, 50)),
+            "p99_ms": float(np.percentile(lat_ms, 99)),
+            "batches": int(sst["batches"]), "launch_busy": sst["launch_us"] * 1e-6 / span,
+            "kept_up": bool(done == offered and span <= duration * 1.2 + 0.5)}
+
+
+def bench_serve(args, ct, datagen):
+    scale = args.serve_scale
+    sizes = [int(x * scale) for x in (1e7, 2e7, 4e7, 6e7, 8e7, 1.2e8, 1.6e8, 2e8)]
+    kinds = ["uniform", "clustered", "roadlike"]
+    trees = []
+    t0 = time.time()
+    for i, n in enumerate(sizes):
+        kind = kinds[i % 3]
+        if kind == "uniform":
+            bx = datagen.uniform_boxes(n, seed=70 + i, wmax=1e-3)
+        elif kind == "clustered":
+            bx = datagen.clustered_boxes(n, seed=70 + i, frac=0.3, k=20, sigma=5e-3, wmax=1e-3)
+        else:
+            bx = datagen.roadlike_boxes(n, seed=70 + i)
+        d = ct.DeviceBoxes.from_host(*bx, np.arange(n, dtype=np.int32))
+        del bx
+        rt = ct.build_rtree(d, args.fanout, ct.KEY_MORTON64)
+        trees.append((rt, d, kind))
+        print(f"[bench] tree {i}: {n:,} {kind} boxes ({time.time() - t0:.0f} s)", file=sys.stderr,
+              flush=True)
+    npool = 4_000_000
+    pool = ct.DeviceBoxes.from_host(*datagen.loguniform_windows(npool, seed=80, lo=1e-5, hi=2e-3))
+    budget = args.serve_budget_gb << 30
+    modes = {"plain": dict(slots=1, batching=False),
+             "served": dict(slots=args.serve_slots, batching=True)}
+    res = {}
+    for name, kw in modes.items():
+        sv = ct.Server(budget_bytes=budget, max_batch_windows=1 << 20, **kw)
+        ids = [sv.add_tree(rt, d) for rt, d, _ in trees]
+        pts = []
+        _poisson_run(ct, sv, ids, pool, npool, 200.0, 0.5, seed=1)  # warm-up
+        lam = args.serve_rate0
+        while lam <= args.serve_rate_max:
+            p = _poisson_run(ct, sv, ids, pool, npool, lam, args.serve_seconds, seed=int(lam))
+            pts.append(p)
+            print(f"[bench] {name} {lam:.0f} req/s: {p['pairs_per_s'] / 1e9:.2f} Gpairs/s, "
+                  f"p50 {p['p50_ms']:.2f} ms, p99 {p['p99_ms']:.2f} ms, kept up {p['kept_up']}, "
+                  f"{p['batches']} batches, dispatcher launching {p['launch_busy']:.0%}",
+                  file=sys.stderr, flush=True)
+            if not p["kept_up"]:
+                break
+            lam *= args.serve_rate_step
+        sv.close()
+        res[name] = pts
+    # the plain mode's knee: its highest rate that kept up without a queue
+    # building (median latency within 3x of the lightest load's)
+    p50_0 = res["plain"][0]["p50_ms"]
+    ok_plain = [p for p in res["plain"] if p["kept_up"] and p["p50_ms"] <= 3 * p50_0]
+    ref = (ok_plain or res["plain"][:1])[-1]
+    ok_served = [p for p in res["served"] if p["kept_up"] and p["p99_ms"] <= ref["p99_ms"]]
+    best = max(ok_served, key=lambda p: p["pairs_per_s"]) if ok_served else None
+    value = best["pairs_per_s"] if best else 0.0
+    out = {
+        "metric": METRIC, "value": value, "unit": "pairs/s", "n_gpus": 1,
+        "steps": len(res["plain"]) + len(res["served"]), "warmup": 1,
+        "ms_per_step": args.serve_seconds * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "serving: 8 resident trees of "
+                               + "/".join(f"{n / 1e6:g}M" for n in sizes)
+                               + " boxes (uniform, clustered, road-like), Poisson arrivals, "
+                               "1..65536 windows per request (log-uniform), side log-uniform "
+                               f"[1e-5,2e-3], {args.serve_budget_gb} GB result budget",
+                   "trees": sizes, "fanout": args.fanout},
+        "serve": {"plain": res["plain"], "served": res["served"], "plain_knee_rate": ref["rate"],
+                  "p99_ref_ms": ref["p99_ms"], "plain_pairs_per_s": ref["pairs_per_s"],
+                  "served_pairs_per_s_at_equal_p99": value,
+                  "ratio": value / ref["pairs_per_s"] if ref["pairs_per_s"] else None},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--boxes", type=int, default=100_000_000)
-    ap.add_argument("--windows", type=int, default=1_000_000)
+    ap.add_argument("--mode", choices=["plain", "batched", "serve"], default="plain")
+    ap.add_argument("--boxes", type=int, default=None)
+    ap.add_argument("--windows", type=int, default=None)
+    ap.add_argument("--budget-pairs", type=int, default=256 << 20,
+                    help="batched mode: pairs per chunk (8 bytes each)")
+    ap.add_argument("--serve-scale", type=float, default=1.0,
+                    help="serve mode: tree sizes relative to 1e7..2e8 boxes")
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--serve-rate0", type=float, default=250.0)
+    ap.add_argument("--serve-rate-max", type=float, default=64000.0)
+    ap.add_argument("--serve-rate-step", type=float, default=1.5)
+    ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-budget-gb", type=int, default=16)
     ap.add_argument("--fanout", type=int, default=16)
     ap.add_argument("--key", choices=["morton64", "ref32"], default="morton64")
     ap.add_argument("--cpu-sample", type=int, default=4_000_000)
@@ -92,6 +294,16 @@ def main():
         ct.set_option("query_tile_steps", args.query_tile_steps)
     if args.query_grid:
         ct.set_option("query_grid", args.query_grid)
+    args.boxes_set = args.boxes is not None
+    args.windows_set = args.windows is not None
+    if args.mode == "batched":
+        return bench_batched(args, ct, datagen)
+    if args.mode == "serve":
+        return bench_serve(args, ct, datagen)
+    if not args.boxes_set:
+        args.boxes = 100_000_000
+    if not args.windows_set:
+        args.windows = 1_000_000
     key_mode = ct.KEY_MORTON64 if args.key == "morton64" else ct.KEY_REF32
     kbits = 64 if key_mode == ct.KEY_MORTON64 else 32
     n, nq, F = args.boxes, args.windows, args.fanout

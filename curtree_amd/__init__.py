@@ -73,7 +73,7 @@ class ServerStats(ctypes.Structure):
     _fields_ = [("requests_done", ctypes.c_int64), ("pairs_done", ctypes.c_int64),
                 ("batches", ctypes.c_int64), ("queued", ctypes.c_int64),
                 ("p50_us", ctypes.c_double), ("p99_us", ctypes.c_double),
-                ("span_us", ctypes.c_double)]
+                ("span_us", ctypes.c_double), ("launch_us", ctypes.c_double)]
 
 
 CHUNK_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
@@ -107,6 +107,7 @@ def _load():
         "curtree_server_add_tree": (I, [V, P(RTreeStruct), P(BBox), P(I)]),
         "curtree_server_submit": (I, [V, I, P(BBox), P(I64)]),
         "curtree_server_poll": (I, [V, I64, P(I), P(IdPair64), P(ctypes.c_double)]),
+        "curtree_server_completed": (I, [V, P(I64), I, P(I)]),
         "curtree_server_release": (I, [V, I64]),
         "curtree_server_get_stats": (I, [V, P(ServerStats)]),
         "curtree_server_reset_stats": (I, [V]),
@@ -443,6 +444,13 @@ class Server:
         _check(rc)
         res = (int(out.sz), out.fid, out.tid) if st.value == REQ_DONE else None
         return st.value, res, lat.value
+
+    def completed(self, cap=4096):
+        """Ids of requests finished since the last call (completion queue)."""
+        ids = (ctypes.c_int64 * cap)()
+        n = ctypes.c_int()
+        _check(lib.curtree_server_completed(self.p, ids, cap, ctypes.byref(n)))
+        return [int(ids[i]) for i in range(n.value)]
 
     def release(self, rid):
         _check(lib.curtree_server_release(self.p, int(rid)))

@@ -488,7 +488,7 @@ CURTREE_API int curtree_set_option(const char* name, int64_t value) {
     return CURTREE_OK;
   }
   if (!strcmp(name, "query_grid")) {
-    if (value < 1 || value > 65535) return set_error(CURTREE_EINVAL, "query_grid: 1..65535");
+    if (value < 0 || value > 65535) return set_error(CURTREE_EINVAL, "query_grid: 0..65535");
     qtuning().grid = (int)value;
     return CURTREE_OK;
   }

@@ -740,7 +740,7 @@ void enqueue_run_g(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
   pf.mark(s, "k_run_begin");
   hipLaunchKernelGGL(k_run_begin, dim3(1), dim3(64), 0, s, qe.ctrl, a.q0, a.q1, a.plan, nslots,
                      a.lv);
-  hipLaunchKernelGGL(k_status_zero, dim3(1024), dim3(256), 0, s, (const QCtrl*)qe.ctrl, qe.status,
+  hipLaunchKernelGGL(k_status_zero, dim3(256), dim3(256), 0, s, (const QCtrl*)qe.ctrl, qe.status,
                      qe.status_words);
   const uint2* cur = nullptr;
   int which = 0;
@@ -827,7 +827,11 @@ int prepare_query(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curt
   if (Q > (int64_t)0xFFFFFFFF) return set_error(CURTREE_EINVAL, "query: too many windows");
   *a = QRunArgs();
   a->steps = qtuning().steps;
-  a->grid = qtuning().grid;
+  // a block loops over tiles, so any grid is correct; small batches get a
+  // small grid (cheaper launches that leave room for concurrent runs)
+  a->grid = qtuning().grid > 0
+                ? qtuning().grid
+                : (int)std::min<int64_t>(kDefaultChainGrid, std::max<int64_t>(512, nq_hint / 8));
   a->nbox = N;
   a->F = F;
   a->G = next_pow2_group(F);

@@ -15,7 +15,7 @@ constexpr int kDefaultSteps = 8;
 constexpr int kDefaultChainGrid = 8192;
 struct QTuning {
   int steps = kDefaultSteps;
-  int grid = kDefaultChainGrid;
+  int grid = 0;      // blocks per chained launch; 0: by batch size, up to kDefaultChainGrid
   int staging = -1;  // AoS box staging: -1 auto, 0 off, 1 on
 };
 QTuning& qtuning();

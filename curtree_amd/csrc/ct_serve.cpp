@@ -3,16 +3,19 @@
 //
 // A dispatcher thread owns S slots (stream + query engine + window staging).
 // Whenever a slot is free it takes the oldest queued request, adds every other
-// queued request on the same tree (up to a window cap), counts the batch's
-// pairs on the slot's stream, reserves that many pairs in the result pool
-// (the memory budget; a batch waits while the pool is full), and enqueues the
-// writing run.  Slots run concurrently on their own streams, so requests that
-// arrive while others are on the device are admitted as slots and memory free
-// up and their work overlaps.  A request's pairs are a contiguous slice of its
-// batch's output; their query indices are rebased to the request, so the
-// result is the same alone and in any mix.  Results stay in the pool until the
-// client releases them.  With batching off each request runs alone on one
-// slot: the plain per-request launches the serving mode is compared with.
+// queued request on the same tree (up to a window cap), reserves result space
+// in the pool (the memory budget; a batch waits while the pool is full) and
+// runs the batch's query writing straight into it, with per-window counts.
+// When the pairs fit the reservation shrinks to them: one pass, one host
+// round trip per batch; otherwise a second pass writes into an exact
+// reservation.  Slots run concurrently on their own streams and the
+// dispatcher only polls events, so requests that arrive while others are on
+// the device are admitted as slots and memory free up, and their work
+// overlaps.  A request's pairs are a contiguous slice of its batch's output;
+// their query indices are rebased to the request on the device, so the result
+// is the same alone and in any mix.  Results stay in the pool until the
+// client releases them.  With batching off each request runs alone: the plain
+// per-request launches the serving mode is compared with.
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -45,11 +48,23 @@ __global__ void k_stage(const ServeJob* __restrict__ jobs, int njobs, float* __r
   }
 }
 
-// Query indices of each request's pairs relative to the request.
-__global__ void k_rebase(const ServeJob* __restrict__ jobs, int njobs, int* __restrict__ tid) {
-  for (int j = blockIdx.x; j < njobs; j += gridDim.x) {
-    const ServeJob jb = jobs[j];
-    for (int64_t i = threadIdx.x; i < jb.n; i += 256) tid[jb.dst + i] -= (int)jb.base;
+// Query indices of a batch's pairs relative to their request: the request of
+// a pair is the last one whose first batch window is <= the pair's window
+// (binary search over the requests' first windows).  The pair count is read
+// from the run's control block, so no host round trip is needed.
+__global__ void k_rebase_search(const QCtrl* __restrict__ c, int64_t cap,
+                                const int64_t* __restrict__ first_win, int nreq,
+                                int* __restrict__ tid) {
+  const int64_t n = (int64_t)c->total < cap ? (int64_t)c->total : cap;
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const int64_t t = tid[i];
+    int lo = 0, hi = nreq - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (first_win[mid] <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    tid[i] = (int)(t - first_win[lo]);
   }
 }
 
@@ -81,6 +96,11 @@ struct Pool {
       }
     }
     return -1;
+  }
+  int64_t largest() const {
+    int64_t m = 0;
+    for (auto& f : free_) m = std::max(m, f.second);
+    return m;
   }
   void release(int64_t off, int64_t n) {
     if (n == 0) return;
@@ -119,20 +139,39 @@ struct Tree {
   uint32_t* canon = nullptr;  // owned: bulk-load layout flag (device word)
 };
 
+// A slot runs one batch at a time.  Pass 1 writes the batch's pairs straight
+// into a pool reservation of `reserve` pairs (with per-window counts), rebases
+// query indices on the device and copies the counts back; if the pairs fit,
+// the reservation shrinks to them and the batch is done after ONE pass and
+// one host round trip.  Otherwise the counts give the exact size: a pass 2
+// writes into an exact reservation.  Every step is enqueued on the slot's
+// stream and finishes on an event, so the dispatcher never waits on the
+// device and all slots' work overlaps.
+enum SlotState { kIdle, kWaitPool1, kPass1, kWaitPool2, kPass2 };
+
 struct Slot {
   hipStream_t s = nullptr;
-  hipEvent_t done = nullptr;
+  hipEvent_t ev = nullptr;
   QEngine qe;
   float* win = nullptr;  // staging: 4 arrays of win_cap windows
   int64_t win_cap = 0;
-  int64_t* counts = nullptr;  // per-window pair counts
-  std::vector<int64_t> hcounts;
-  ServeJob* jobs = nullptr;     // device descriptors (stage, then rebase)
-  ServeJob* hjobs = nullptr;    // pinned host copy
+  int64_t* counts = nullptr;   // per-window pair counts (device)
+  int64_t* hcounts = nullptr;  // pinned copy
+  ServeJob* jobs = nullptr;    // device descriptors (window staging)
+  ServeJob* hjobs = nullptr;   // pinned host copy
+  int64_t* first = nullptr;    // device: first batch window of each request
+  int64_t* hfirst = nullptr;   // pinned host copy
   int jobs_cap = 0;
-  bool busy = false;
+  SlotState state = kIdle;
+  // device work decided under the server lock and enqueued outside it
+  enum Todo { kNone, kBegin, kPass, kGrowPass } todo = kNone;
+  int64_t grow_to = 0;         // kGrowPass: frontier entries
+  Tree tree;                   // the batch's tree (a copy: trees may grow)
+  int attempts = 0;
+  QRunArgs a;                  // the batch's run
+  int64_t nwin = 0;
   std::vector<int64_t> reqs;
-  int64_t pool_off = 0, pool_len = 0;
+  int64_t pool_off = 0, pool_len = 0;  // current reservation
 };
 
 }  // namespace
@@ -143,9 +182,11 @@ struct Server {
   std::vector<Slot> slots;
   std::unordered_map<int64_t, Request> reqs;
   std::deque<int64_t> queue;
+  std::deque<int64_t> completed;  // finished request ids not yet reported
   Pool pool;
   int* pool_fid = nullptr;
   int* pool_tid = nullptr;
+  int64_t reserve = 0;  // pass-1 reservation (pairs)
   Prof prof;  // never enabled
   std::mutex mu;
   std::condition_variable cv;
@@ -154,15 +195,20 @@ struct Server {
   int64_t next_id = 1;
   std::vector<double> lat_us;
   int64_t pairs_done = 0, batches = 0, requests_done = 0;
-  uint64_t pool_gen = 0, busy_gen = ~0ull;  // releases so far / at the last full-pool stall
+  double launch_us = 0;  // dispatcher time spent enqueuing device work
+  uint64_t pool_gen = 0;
   double t_first = 0, t_last = 0;
-  std::string fatal;
 
   int start();
   void loop();
-  bool harvest();
+  bool step_slot(Slot& sl);
   bool launch_one();
-  int run_batch(Slot& sl, std::vector<int64_t>& batch, int64_t* pool_off, int64_t* pool_len);
+  int begin_batch(Slot& sl);
+  int try_pass(Slot& sl, bool exact);
+  int enqueue_pass(Slot& sl);
+  int after_pass(Slot& sl);
+  int run_todo(Slot& sl);
+  void finish(Slot& sl, int err);
 };
 
 int Server::start() {
@@ -171,66 +217,61 @@ int Server::start() {
   const int64_t cap = std::max<int64_t>(opt.max_batch_windows, 1);
   for (auto& sl : slots) {
     CT_HIP(hipStreamCreateWithFlags(&sl.s, hipStreamNonBlocking));
-    CT_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+    CT_HIP(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming));
     CT_HIP(hipMalloc((void**)&sl.win, (size_t)cap * 16));
     CT_HIP(hipMalloc((void**)&sl.counts, (size_t)cap * 8));
+    CT_HIP(hipHostMalloc((void**)&sl.hcounts, (size_t)cap * 8, hipHostMallocDefault));
     sl.win_cap = cap;
-    sl.hcounts.resize((size_t)cap);
     sl.jobs_cap = 4096;
-    CT_HIP(hipMalloc((void**)&sl.jobs, (size_t)sl.jobs_cap * 2 * sizeof(ServeJob)));
-    CT_HIP(hipHostMalloc((void**)&sl.hjobs, (size_t)sl.jobs_cap * 2 * sizeof(ServeJob),
+    CT_HIP(hipMalloc((void**)&sl.jobs, (size_t)sl.jobs_cap * sizeof(ServeJob)));
+    CT_HIP(hipHostMalloc((void**)&sl.hjobs, (size_t)sl.jobs_cap * sizeof(ServeJob),
                          hipHostMallocDefault));
+    CT_HIP(hipMalloc((void**)&sl.first, (size_t)sl.jobs_cap * 8));
+    CT_HIP(hipHostMalloc((void**)&sl.hfirst, (size_t)sl.jobs_cap * 8, hipHostMallocDefault));
     if ((rc = sl.qe.ensure_ctrl())) return rc;
   }
   const int64_t pairs = std::max<int64_t>(opt.budget_bytes / 8, 1);
   CT_HIP(hipMalloc((void**)&pool_fid, (size_t)pairs * 4));
   CT_HIP(hipMalloc((void**)&pool_tid, (size_t)pairs * 4));
   pool.init(pairs);
+  reserve = std::max<int64_t>(pairs / (2 * (int64_t)opt.slots), 1);
   th = std::thread([this] { loop(); });
   return CURTREE_OK;
 }
 
-// Completed slots -> their requests are done, their pool space stays with the
-// requests until released.
-bool Server::harvest() {
-  bool any = false;
-  for (auto& sl : slots) {
-    if (!sl.busy) continue;
-    hipError_t e = hipEventQuery(sl.done);
-    if (e == hipErrorNotReady) continue;
-    const double t = now_us();
-    for (int64_t id : sl.reqs) {
-      auto it = reqs.find(id);
-      if (it == reqs.end()) continue;
-      Request& r = it->second;
-      if (e != hipSuccess) {
-        r.state = CURTREE_REQ_FAILED;
-        r.err = CURTREE_EHIP;
-      } else {
-        r.state = CURTREE_REQ_DONE;
-      }
-      r.t_done = t;
+// The batch's requests are finished (err == 0: done, their pairs stay in the
+// pool until released; else failed, the reservation is returned).
+void Server::finish(Slot& sl, int err) {
+  const double t = now_us();
+  if (err && sl.pool_len > 0) {
+    pool.release(sl.pool_off, sl.pool_len);
+    pool_gen++;
+  }
+  for (int64_t id : sl.reqs) {
+    auto it = reqs.find(id);
+    if (it == reqs.end()) continue;
+    Request& r = it->second;
+    r.state = err ? CURTREE_REQ_FAILED : CURTREE_REQ_DONE;
+    r.err = err;
+    r.t_done = t;
+    if (!err) {
       lat_us.push_back(t - r.t_submit);
       pairs_done += r.npairs;
       requests_done++;
-      t_last = t;
     }
-    sl.reqs.clear();
-    sl.busy = false;
-    any = true;
+    completed.push_back(id);
+    t_last = t;
   }
-  return any;
+  sl.reqs.clear();
+  sl.pool_off = sl.pool_len = 0;
+  sl.state = kIdle;
 }
 
-// Count, reserve, write: the batch's requests (same tree) on slot `sl`.  The
-// host waits for the count only; the write runs on while other slots start.
-int Server::run_batch(Slot& sl, std::vector<int64_t>& batch, int64_t* pool_off,
-                      int64_t* pool_len) {
-  Tree& tr = trees[(size_t)reqs[batch[0]].tree];
-  const int nj = (int)batch.size();
+// Under the lock: the batch's window descriptors and request offsets.
+static int64_t fill_jobs(Slot& sl, std::unordered_map<int64_t, Request>& reqs) {
   int64_t nwin = 0;
-  for (int j = 0; j < nj; j++) {
-    const Request& r = reqs[batch[(size_t)j]];
+  for (size_t j = 0; j < sl.reqs.size(); j++) {
+    const Request& r = reqs[sl.reqs[j]];
     ServeJob& jb = sl.hjobs[j];
     jb.src[0] = r.win.xmin;
     jb.src[1] = r.win.ymin;
@@ -239,117 +280,219 @@ int Server::run_batch(Slot& sl, std::vector<int64_t>& batch, int64_t* pool_off,
     jb.n = r.win.sz;
     jb.dst = nwin;
     jb.base = nwin;
+    sl.hfirst[j] = nwin;
     nwin += r.win.sz;
   }
+  return nwin;
+}
+
+// Outside the lock: stage the batch's windows and enqueue pass 1 (the pool
+// reservation is already held).
+int Server::begin_batch(Slot& sl) {
+  const Tree& tr = sl.tree;
+  const int nj = (int)sl.reqs.size();
+  const int64_t nwin = sl.nwin;
   CT_HIP(hipMemcpyAsync(sl.jobs, sl.hjobs, (size_t)nj * sizeof(ServeJob), hipMemcpyHostToDevice,
                         sl.s));
+  CT_HIP(hipMemcpyAsync(sl.first, sl.hfirst, (size_t)nj * 8, hipMemcpyHostToDevice, sl.s));
   hipLaunchKernelGGL(k_stage, dim3((unsigned)std::min(nj, 1024)), dim3(256), 0, sl.s,
                      (const ServeJob*)sl.jobs, nj, sl.win, sl.win_cap);
   curtree_bbox_f32 q{(int)nwin, sl.win, sl.win + sl.win_cap, sl.win + 2 * sl.win_cap,
                      sl.win + 3 * sl.win_cap, nullptr};
-  // count pass with per-window counts (the tree's cached containment flags)
-  QRunArgs a;
+  // the tree's cached containment flags, AoS copies and layout check
   int rc;
-  if ((rc = prepare_query(sl.qe, prof, &tr.rt, &tr.rbox, &q, 0, nwin, &a, sl.s))) return rc;
-  a.tr.safe = tr.safe;
-  a.rb.b4 = tr.box4;
-  a.tr.mbr.b4 = tr.node4;
-  a.tr.canon = tr.canon;
-  a.q0 = 0;
-  a.q1 = nwin;
-  a.write = false;
-  a.counts = (unsigned long long*)sl.counts;
+  if ((rc = prepare_query(sl.qe, prof, &tr.rt, &tr.rbox, &q, 0, nwin, &sl.a, sl.s))) return rc;
+  sl.a.tr.safe = tr.safe;
+  sl.a.rb.b4 = tr.box4;
+  sl.a.tr.mbr.b4 = tr.node4;
+  sl.a.tr.canon = tr.canon;
+  sl.a.q0 = 0;
+  sl.a.q1 = nwin;
   if ((rc = sl.qe.reserve_front(std::max<int64_t>(2 * nwin, 1 << 16)))) return rc;
-  for (int attempt = 0;; attempt++) {
-    enqueue_fill_counts(sl.counts, nwin, sl.s);
-    if ((rc = sl.qe.reserve_status(std::max<int64_t>(sl.qe.cap_front, nwin) / per_tile(a.G, a.steps) + 2)))
-      return rc;
-    enqueue_run(sl.qe, prof, a, sl.s);
-    CT_HIP(hipGetLastError());
-    if ((rc = read_ctrl(sl.qe, sl.s))) return rc;
-    if (!sl.qe.host_ctrl->overflow) break;
-    if (attempt >= 3) return set_error(CURTREE_ENOMEM, "serve: frontier kept overflowing");
-    uint64_t need = 0;
-    for (int l = 1; l < a.nl; l++) need = std::max<uint64_t>(need, sl.qe.host_ctrl->count[l]);
-    if ((rc = sl.qe.reserve_front((int64_t)(need + need / 4 + 1024)))) return rc;
+  return enqueue_pass(sl);
+}
+
+// Outside the lock: the slot's pending device work.
+int Server::run_todo(Slot& sl) {
+  const Slot::Todo t = sl.todo;
+  sl.todo = Slot::kNone;
+  int rc;
+  switch (t) {
+    case Slot::kBegin:
+      return begin_batch(sl);
+    case Slot::kGrowPass:
+      if ((rc = sl.qe.reserve_front(sl.grow_to))) return rc;
+      return enqueue_pass(sl);
+    case Slot::kPass:
+      return enqueue_pass(sl);
+    default:
+      return CURTREE_OK;
   }
-  const int64_t H = (int64_t)sl.qe.host_ctrl->total;
-  if (H > pool.cap)
-    return set_error(CURTREE_ERANGE, "serve: %lld pairs exceed the result budget", (long long)H);
-  int64_t off = 0;
-  if (H > 0) {
-    off = pool.alloc(H);
-    if (off < 0) return CURTREE_EBUSY;  // wait for released results
+}
+
+// Reserve pool space (pass 1: the standing reservation or the largest free
+// block; pass 2: exactly the counted pairs) and enqueue the pass, or wait.
+int Server::try_pass(Slot& sl, bool exact) {
+  int64_t want = 0;
+  if (exact) {
+    want = sl.qe.host_ctrl->total;
+    if (want > pool.cap)
+      return set_error(CURTREE_ERANGE, "serve: %lld pairs exceed the result budget",
+                       (long long)want);
+  } else {
+    want = std::min<int64_t>(reserve, std::max<int64_t>(pool.largest(), 0));
+    if (want < std::min<int64_t>(reserve, 1 << 16)) want = -1;  // too little free: wait
   }
-  CT_HIP(hipMemcpyAsync(sl.hcounts.data(), sl.counts, (size_t)nwin * 8, hipMemcpyDeviceToHost,
-                        sl.s));
-  CT_HIP(hipStreamSynchronize(sl.s));
-  if (H > 0) {
-    // write pass into the reserved pool slice
-    a.write = true;
-    a.counts = nullptr;
-    a.out_fid = pool_fid + off;
-    a.out_tid = pool_tid + off;
-    a.out_cap = H;
-    if ((rc = sl.qe.reserve_pieces(pieces_needed(sl.qe, a)))) {
-      pool.release(off, H);
-      return rc;
-    }
-    enqueue_run(sl.qe, prof, a, sl.s);
+  const int64_t off = want > 0 ? pool.alloc(want) : (want == 0 ? 0 : -1);
+  if (off < 0) {
+    sl.state = exact ? kWaitPool2 : kWaitPool1;
+    return CURTREE_OK;
   }
-  // per-request slices of the batch output; query indices rebased per request
-  int64_t p0 = 0;
-  int nr = 0;
-  ServeJob* rj = sl.hjobs + sl.jobs_cap;
-  for (int j = 0; j < nj; j++) {
-    Request& r = reqs[batch[(size_t)j]];
-    const int64_t w0 = sl.hjobs[j].dst;
-    int64_t np = 0;
-    for (int64_t i = 0; i < r.win.sz; i++) np += sl.hcounts[(size_t)(w0 + i)];
-    r.pool_off = off + p0;
-    r.npairs = np;
-    r.state = CURTREE_REQ_RUNNING;
-    if (np > 0 && w0 > 0) {
-      ServeJob& jb = rj[nr++];
-      jb.n = np;
-      jb.dst = off + p0;
-      jb.base = w0;
-    }
-    p0 += np;
-  }
-  if (nr > 0) {
-    CT_HIP(hipMemcpyAsync(sl.jobs + sl.jobs_cap, rj, (size_t)nr * sizeof(ServeJob),
-                          hipMemcpyHostToDevice, sl.s));
-    hipLaunchKernelGGL(k_rebase, dim3((unsigned)std::min(nr, 1024)), dim3(256), 0, sl.s,
-                       (const ServeJob*)(sl.jobs + sl.jobs_cap), nr, pool_tid);
-  }
-  CT_HIP(hipEventRecord(sl.done, sl.s));
-  *pool_off = off;
-  *pool_len = H;
+  sl.pool_off = off;
+  sl.pool_len = want;
+  const bool first = sl.state == kIdle || sl.state == kWaitPool1;
+  sl.state = exact ? kPass2 : kPass1;
+  sl.todo = first && !exact ? Slot::kBegin : Slot::kPass;
   return CURTREE_OK;
 }
 
-// Admit the next batch onto a free slot.  Returns true if something launched.
+int Server::enqueue_pass(Slot& sl) {
+  int rc;
+  QRunArgs& a = sl.a;
+  const bool pass1 = sl.state == kPass1;
+  if (a.nbox == 0 || sl.nwin == 0) {
+    enqueue_fill_counts(sl.counts, sl.nwin, sl.s);
+    sl.qe.host_ctrl->total = 0;
+    sl.qe.host_ctrl->overflow = 0;
+  } else {
+    a.write = true;
+    a.counts = pass1 ? (unsigned long long*)sl.counts : nullptr;
+    a.out_fid = pool_fid + sl.pool_off;
+    a.out_tid = pool_tid + sl.pool_off;
+    a.out_cap = sl.pool_len;
+    if ((rc = sl.qe.reserve_status(std::max<int64_t>(sl.qe.cap_front, sl.nwin) /
+                                       per_tile(a.G, a.steps) + 2)) ||
+        (rc = sl.qe.reserve_pieces(pieces_needed(sl.qe, a))))
+      return rc;
+    if (pass1) enqueue_fill_counts(sl.counts, sl.nwin, sl.s);
+    enqueue_run(sl.qe, prof, a, sl.s);
+    if (sl.reqs.size() > 1 && sl.pool_len > 0)
+      hipLaunchKernelGGL(k_rebase_search, dim3(1024), dim3(256), 0, sl.s,
+                         (const QCtrl*)sl.qe.ctrl, sl.pool_len, (const int64_t*)sl.first,
+                         (int)sl.reqs.size(), pool_tid + sl.pool_off);
+    CT_HIP(hipMemcpyAsync(sl.qe.host_ctrl, sl.qe.ctrl, sizeof(QCtrl), hipMemcpyDeviceToHost,
+                          sl.s));
+  }
+  if (pass1)
+    CT_HIP(hipMemcpyAsync(sl.hcounts, sl.counts, (size_t)sl.nwin * 8, hipMemcpyDeviceToHost,
+                          sl.s));
+  CT_HIP(hipEventRecord(sl.ev, sl.s));
+  return CURTREE_OK;
+}
+
+// A pass finished: rerun after a frontier overflow; after pass 1 shrink the
+// reservation to the pairs (done) or go for an exact pass 2.
+int Server::after_pass(Slot& sl) {
+  QRunArgs& a = sl.a;
+  const QCtrl& hc = *sl.qe.host_ctrl;
+  const bool ran = a.nbox > 0 && sl.nwin > 0;
+  if (ran && hc.overflow) {
+    if (++sl.attempts > a.nl + 1)
+      return set_error(CURTREE_ENOMEM, "serve: frontier kept overflowing");
+    uint64_t need = 0;
+    for (int l = 1; l < a.nl; l++) need = std::max<uint64_t>(need, hc.count[l]);
+    sl.grow_to = (int64_t)(need + need / 4 + 1024);
+    sl.todo = Slot::kGrowPass;
+    return CURTREE_OK;
+  }
+  const int64_t H = ran ? (int64_t)hc.total : 0;
+  if (sl.state == kPass1) {
+    // per-request slices from the per-window counts
+    int64_t p0 = 0;
+    for (size_t j = 0; j < sl.reqs.size(); j++) {
+      Request& r = reqs[sl.reqs[j]];
+      const int64_t w0 = sl.hfirst[j];
+      int64_t np = 0;
+      for (int64_t i = 0; i < r.win.sz; i++) np += sl.hcounts[w0 + i];
+      r.pool_off = p0;  // relative until the final reservation is known
+      r.npairs = np;
+      r.state = CURTREE_REQ_RUNNING;
+      p0 += np;
+    }
+    if (H > sl.pool_len) {
+      // did not fit: return the reservation, write again into an exact one
+      if (sl.pool_len > 0) {
+        pool.release(sl.pool_off, sl.pool_len);
+        pool_gen++;
+      }
+      sl.pool_off = sl.pool_len = 0;
+      return try_pass(sl, true);
+    }
+    if (H < sl.pool_len) {
+      pool.release(sl.pool_off + H, sl.pool_len - H);
+      pool_gen++;
+      sl.pool_len = H;
+    }
+  }
+  for (int64_t id : sl.reqs) reqs[id].pool_off += sl.pool_off;
+  finish(sl, CURTREE_OK);
+  return CURTREE_OK;
+}
+
+// Advance one slot if its pending step finished.  Returns true on progress.
+bool Server::step_slot(Slot& sl) {
+  int rc = CURTREE_OK;
+  if (sl.todo != Slot::kNone) return false;  // not enqueued yet
+  switch (sl.state) {
+    case kIdle:
+      return false;
+    case kPass1:
+    case kPass2: {
+      hipError_t e = hipEventQuery(sl.ev);
+      if (e == hipErrorNotReady) return false;
+      if (e != hipSuccess)
+        rc = set_error(CURTREE_EHIP, "serve: %s", hipGetErrorString(e));
+      else
+        rc = after_pass(sl);
+      break;
+    }
+    case kWaitPool1:
+    case kWaitPool2: {
+      const SlotState was = sl.state;
+      rc = try_pass(sl, was == kWaitPool2);
+      if (!rc && sl.state == was) return false;
+      break;
+    }
+  }
+  if (rc) finish(sl, rc);
+  return true;
+}
+
+// Admit the oldest request (and, with batching, every queued request on the
+// same tree that fits) onto a free slot.  Returns true if something started.
 bool Server::launch_one() {
   if (queue.empty()) return false;
-  if (busy_gen == pool_gen) return false;  // the pool was full; nothing released since
+  // a batch waiting for result memory goes first: no new batch may take the
+  // space that releases free up
+  for (auto& sl : slots)
+    if (sl.state == kWaitPool1 || sl.state == kWaitPool2) return false;
   Slot* free_slot = nullptr;
   for (auto& sl : slots)
-    if (!sl.busy) {
+    if (sl.state == kIdle) {
       free_slot = &sl;
       break;
     }
   if (!free_slot) return false;
-  // the oldest request decides the tree; same-tree requests join it
-  std::vector<int64_t> batch;
+  Slot& sl = *free_slot;
   const int tree = reqs[queue.front()].tree;
   int64_t nwin = 0;
+  sl.reqs.clear();
   for (auto it = queue.begin(); it != queue.end();) {
     const Request& r = reqs[*it];
-    const bool first = batch.empty();
-    if (r.tree == tree && (first || (opt.batching && nwin + r.win.sz <= free_slot->win_cap &&
-                                     (int)batch.size() < free_slot->jobs_cap))) {
-      batch.push_back(*it);
+    const bool first = sl.reqs.empty();
+    if (r.tree == tree && (first || (opt.batching && nwin + r.win.sz <= sl.win_cap &&
+                                     (int)sl.reqs.size() < sl.jobs_cap))) {
+      sl.reqs.push_back(*it);
       nwin += r.win.sz;
       it = queue.erase(it);
       if (!opt.batching) break;
@@ -357,40 +500,45 @@ bool Server::launch_one() {
       ++it;
     }
   }
-  int64_t off = 0, len = 0;
-  int rc = run_batch(*free_slot, batch, &off, &len);
-  if (rc == CURTREE_EBUSY) {
-    // not enough result memory now: requeue in front, keep their order
-    for (auto it = batch.rbegin(); it != batch.rend(); ++it) queue.push_front(*it);
-    busy_gen = pool_gen;
-    return false;
-  }
-  if (rc) {
-    for (int64_t id : batch) {
-      Request& r = reqs[id];
-      r.state = CURTREE_REQ_FAILED;
-      r.err = rc;
-      r.t_done = now_us();
-    }
-    return true;
-  }
-  free_slot->busy = true;
-  free_slot->reqs = batch;
   batches++;
+  sl.tree = trees[(size_t)tree];
+  sl.nwin = fill_jobs(sl, reqs);
+  sl.attempts = 0;
+  const int rc = try_pass(sl, false);  // reserve now, stage outside the lock
+  if (rc) finish(sl, rc);
   return true;
 }
 
 void Server::loop() {
   std::unique_lock<std::mutex> lk(mu);
   while (!stop) {
-    bool progressed = harvest();
+    bool progressed = false;
+    for (auto& sl : slots) progressed |= step_slot(sl);
     while (launch_one()) progressed = true;
+    // device work outside the lock: clients are not held up by launches
+    bool any = false;
+    for (auto& sl : slots) any |= sl.todo != Slot::kNone;
+    if (any) {
+      std::vector<std::pair<Slot*, int>> failed;
+      lk.unlock();
+      const double t0 = now_us();
+      for (auto& sl : slots)
+        if (sl.todo != Slot::kNone) {
+          const int rc = run_todo(sl);
+          if (rc) failed.push_back({&sl, rc});
+        }
+      const double dt = now_us() - t0;
+      lk.lock();
+      launch_us += dt;
+      for (auto& f : failed) finish(*f.first, f.second);
+      progressed = true;
+    }
     bool busy = false;
-    for (auto& sl : slots) busy |= sl.busy;
+    for (auto& sl : slots) busy |= sl.state != kIdle;
     if (!progressed) {
       if (busy) {
         lk.unlock();
-        std::this_thread::sleep_for(std::chrono::microseconds(20));
+        std::this_thread::sleep_for(std::chrono::microseconds(10));
         lk.lock();
       } else {
         cv.wait_for(lk, std::chrono::milliseconds(5));
@@ -503,6 +651,20 @@ CURTREE_API int curtree_server_poll(curtree_server* s, int64_t request_id, int* 
   return CURTREE_OK;
 }
 
+CURTREE_API int curtree_server_completed(curtree_server* s, int64_t* ids, int cap, int* n) {
+  Server* sv = (Server*)s;
+  if (!sv || !n || cap < 0 || (cap > 0 && !ids))
+    return set_error(CURTREE_EINVAL, "server_completed: bad argument");
+  std::lock_guard<std::mutex> g(sv->mu);
+  int k = 0;
+  while (k < cap && !sv->completed.empty()) {
+    ids[k++] = sv->completed.front();
+    sv->completed.pop_front();
+  }
+  *n = k;
+  return CURTREE_OK;
+}
+
 CURTREE_API int curtree_server_release(curtree_server* s, int64_t request_id) {
   Server* sv = (Server*)s;
   if (!sv) return set_error(CURTREE_EINVAL, "server_release: null");
@@ -538,6 +700,7 @@ CURTREE_API int curtree_server_get_stats(curtree_server* s, curtree_server_stats
   st->p50_us = pct(0.50);
   st->p99_us = pct(0.99);
   st->span_us = sv->t_last > sv->t_first ? sv->t_last - sv->t_first : 0.0;
+  st->launch_us = sv->launch_us;
   return CURTREE_OK;
 }
 
@@ -547,6 +710,7 @@ CURTREE_API int curtree_server_reset_stats(curtree_server* s) {
   std::lock_guard<std::mutex> g(sv->mu);
   sv->lat_us.clear();
   sv->pairs_done = sv->batches = sv->requests_done = 0;
+  sv->launch_us = 0;
   sv->t_first = sv->t_last = 0;
   return CURTREE_OK;
 }
@@ -563,11 +727,14 @@ CURTREE_API int curtree_server_destroy(curtree_server* s) {
   (void)hipDeviceSynchronize();
   for (auto& sl : sv->slots) {
     if (sl.s) (void)hipStreamDestroy(sl.s);
-    if (sl.done) (void)hipEventDestroy(sl.done);
+    if (sl.ev) (void)hipEventDestroy(sl.ev);
     if (sl.win) (void)hipFree(sl.win);
     if (sl.counts) (void)hipFree(sl.counts);
+    if (sl.hcounts) (void)hipHostFree(sl.hcounts);
     if (sl.jobs) (void)hipFree(sl.jobs);
     if (sl.hjobs) (void)hipHostFree(sl.hjobs);
+    if (sl.first) (void)hipFree(sl.first);
+    if (sl.hfirst) (void)hipHostFree(sl.hfirst);
     sl.qe.release();
   }
   for (auto& t : sv->trees) {

@@ -171,6 +171,7 @@ typedef struct curtree_server_stats {
   int64_t requests_done, pairs_done, batches, queued;
   double p50_us, p99_us; /* submit -> completion latency */
   double span_us;        /* first submit -> last completion */
+  double launch_us;      /* dispatcher time spent enqueuing device work */
 } curtree_server_stats;
 CURTREE_API int curtree_server_create(const curtree_server_opts* opts, curtree_server** out);
 CURTREE_API int curtree_server_add_tree(curtree_server* s, const curtree_rtree_f32* rt,
@@ -180,6 +181,10 @@ CURTREE_API int curtree_server_submit(curtree_server* s, int tree_id,
 /* state: CURTREE_REQ_*; result/latency filled once done */
 CURTREE_API int curtree_server_poll(curtree_server* s, int64_t request_id, int* state,
                                     curtree_idpair64* result, double* latency_us);
+/* Completion queue: ids of requests that finished (done or failed) since the
+ * last call, oldest first, at most cap of them; *n = how many.  Each id is
+ * reported once; poll it for the result. */
+CURTREE_API int curtree_server_completed(curtree_server* s, int64_t* ids, int cap, int* n);
 CURTREE_API int curtree_server_release(curtree_server* s, int64_t request_id);
 CURTREE_API int curtree_server_get_stats(curtree_server* s, curtree_server_stats* stats);
 CURTREE_API int curtree_server_reset_stats(curtree_server* s);
@@ -269,8 +274,9 @@ CURTREE_API int curtree_query_last_stats(int64_t* entries, int cap, int* levels,
  *                  results are identical either way.
  *   "query_tile_steps"  4, 8 (default) or 16: wave-steps per frontier tile
  *                  (tile = 4 waves x steps x 64/next_pow2(F) entries).
- *   "query_grid"   blocks of each chained query launch (default 8192; they
- *                  loop over tiles), i.e. waves per CU.
+ *   "query_grid"   blocks of each chained query launch (they loop over
+ *                  tiles), i.e. waves per CU; 0 (default): by batch size,
+ *                  windows/8 clamped to [512, 8192].
  *   "box_staging"  -1 (default: auto), 0 off, 1 on: a query copies boxes,
  *                  node MBRs and windows into AoS float4 arrays first (one
  *                  extra pass over the boxes; auto when the batch has at

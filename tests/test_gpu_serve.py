@@ -47,6 +47,7 @@ def test_results_identical_alone_and_in_the_mix(batching, slots):
     ids = [sv.add_tree(rt, d) for rt, d in trees]
     pending = {}
     got = {}
+    reported = set()
     k = 0
     t_end = time.time() + 120
     while (k < nreq or pending) and time.time() < t_end:
@@ -56,13 +57,16 @@ def test_results_identical_alone_and_in_the_mix(batching, slots):
                 v = ct.window_view(pool, starts[k], sizes[k])
                 pending[sv.submit(ids[which[k]], v)] = k
                 k += 1
-        for rid in list(pending):
+        # finished requests come from the completion queue, each id once
+        for rid in sv.completed():
+            assert rid in pending and rid not in reported
+            reported.add(rid)
             st, res, lat = sv.poll(rid)
-            if st == ct.REQ_DONE:
-                n, fp, tp = res
-                got[pending.pop(rid)] = (ct._host_from_dev(fp, n, np.int32),
-                                         ct._host_from_dev(tp, n, np.int32), lat)
-                sv.release(rid)
+            assert st == ct.REQ_DONE
+            n, fp, tp = res
+            got[pending.pop(rid)] = (ct._host_from_dev(fp, n, np.int32),
+                                     ct._host_from_dev(tp, n, np.int32), lat)
+            sv.release(rid)
         time.sleep(0.0005)
     stats = sv.stats()
     sv.close()
