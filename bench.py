@@ -386,7 +386,7 @@ def main():
     query_ms = np.array([r[1] for r in rec])
     step_ms = build_ms + query_ms
     kernels = {}
-    for name in ("k_build_init", "k_extent", "k_keygen", "k_sort_plan", "k_onesweep",
+    for name in ("k_build_init", "k_extent", "k_keygen", "k_sort_plan", "k_onesweep", "k_fixup",
                  "k_gather_l0", "k_level", "k_safe", "k_run_begin", "k_expand", "k_leaf_count",
                  "k_leaf_pass", "k_write_pieces"):
         tot, cnt = ct.Prof.get(name)

@@ -36,16 +36,24 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_tmp,
 // Control block layout (device memory, zeroed by k_build_init).
 constexpr int kExtSlots = 64;  // extent partials are spread over slots to avoid atomic contention
 
+// Sort steps: one onesweep pass each, over digit digit[step].  A plain LSD
+// sort runs digits 0..D-1; the 64-bit Morton sort runs the high half first
+// (digits 4..7), fixes runs of equal high halves in place (k_fixup) and only
+// if a run is too long for that runs the full LSD sort (steps 4..11).
+constexpr int kMaxSteps = 12;
 struct BuildCtrl {
   uint32_t ext[kExtSlots][4];      // ordered-uint lox, loy, hix, hiy of centroids, spread
   uint32_t hist[kMaxDigits * 256];  // global digit histograms
   uint32_t base[kMaxDigits * 256];  // exclusive digit offsets
-  uint32_t ticket[kMaxDigits];
-  int32_t active[kMaxDigits];      // pass p runs
-  int32_t src[kMaxDigits];         // pass p reads buffer src[p]
-  int32_t write_keys[kMaxDigits];  // pass p writes keys (not the last pass)
-  int32_t npass;                   // number of passes that run
-  int32_t first;                   // first pass that runs (-1: none)
+  uint32_t ticket[kMaxSteps];
+  int32_t digit[kMaxSteps];        // digit of each step
+  int32_t active[kMaxSteps];       // step s runs
+  int32_t src[kMaxSteps];          // step s reads buffer src[s]
+  int32_t write_keys[kMaxSteps];   // step s writes keys (not the last step)
+  int32_t npass;                   // steps that ran so far (planned): parity of the data
+  int32_t first;                   // first step that runs (-1: none)
+  uint32_t fallback;               // k_fixup found a run too long to fix in place
+  uint32_t trivial;                // bit d: digit d is the same for every key
 };
 
 __global__ __launch_bounds__(256) void k_build_init(BuildCtrl* c) {
@@ -53,8 +61,9 @@ __global__ __launch_bounds__(256) void k_build_init(BuildCtrl* c) {
     c->hist[i] = 0;
     c->base[i] = 0;
   }
-  if (threadIdx.x < kMaxDigits) {
+  if (threadIdx.x < kMaxSteps) {
     c->ticket[threadIdx.x] = 0;
+    c->digit[threadIdx.x] = 0;
     c->active[threadIdx.x] = 0;
     c->src[threadIdx.x] = 0;
     c->write_keys[threadIdx.x] = 0;
@@ -68,6 +77,8 @@ __global__ __launch_bounds__(256) void k_build_init(BuildCtrl* c) {
   if (threadIdx.x == 0) {
     c->npass = 0;
     c->first = -1;
+    c->fallback = 0;
+    c->trivial = 0;
   }
 }
 
@@ -208,55 +219,79 @@ __global__ __launch_bounds__(256) void k_keygen(int64_t n, const float* __restri
     if (sh[t]) atomicAdd(&c->hist[t], sh[t]);
 }
 
-// Pass plan: exclusive digit offsets per pass; a pass whose digit is the same
-// for every key is skipped (it would be the identity permutation).
-__global__ __launch_bounds__(256) void k_sort_plan(int64_t n, int D, BuildCtrl* c) {
+// Step plan: exclusive digit offsets for digits 0..D-1 and, for steps
+// [s0, s0+ns) over the digits packed 4 bits each in `digits` (lowest first),
+// which steps run and from which buffer.  A step whose digit is the same for
+// every key is skipped (it would be the identity permutation).  `need_one`:
+// at least one step must run (a later stage reads the sorted buffers).
+__device__ void plan_steps(BuildCtrl* c, int s0, int ns, uint64_t digits, bool need_one) {
+  int J = c->npass, last = -1, first = c->first;
+  for (int k = 0; k < ns; k++) {
+    const int st = s0 + k, d = (int)((digits >> (4 * k)) & 15u);
+    c->digit[st] = d;
+    c->ticket[st] = 0;
+    const bool triv = (c->trivial >> d) & 1u;
+    if (triv && !(need_one && k == ns - 1 && J == c->npass && last < 0)) {
+      c->active[st] = 0;
+    } else {
+      c->active[st] = 1;
+      c->src[st] = J & 1;
+      c->write_keys[st] = 1;
+      if (first < 0) first = st;
+      last = st;
+      J++;
+    }
+  }
+  c->npass = J;
+  c->first = first;
+}
+
+__global__ __launch_bounds__(256) void k_sort_plan(int64_t n, int D, BuildCtrl* c, int ns,
+                                                   uint64_t digits, int keep_last_keys) {
   __shared__ uint32_t s_tmp[4];
-  __shared__ int s_triv[kMaxDigits];
+  __shared__ uint32_t s_triv;
+  if (threadIdx.x == 0) s_triv = 0;
+  __syncthreads();
   for (int p = 0; p < D; p++) {
     uint32_t v = c->hist[p * 256 + threadIdx.x];
     int triv = __syncthreads_or(v == (uint32_t)n);
     uint32_t tot;
     uint32_t ex = block_excl_scan<4>(v, s_tmp, &tot);
     c->base[p * 256 + threadIdx.x] = ex;
-    if (threadIdx.x == 0) s_triv[p] = triv;
+    if (threadIdx.x == 0 && triv) s_triv |= 1u << p;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    int J = 0, last = -1, first = -1;
-    for (int p = 0; p < D; p++) {
-      if (s_triv[p]) {
-        c->active[p] = 0;
-      } else {
-        c->active[p] = 1;
-        c->src[p] = J & 1;
-        c->write_keys[p] = 1;
-        if (first < 0) first = p;
-        last = p;
-        J++;
-      }
-    }
-    if (last >= 0) c->write_keys[last] = 0;
-    c->npass = J;
-    c->first = first;
+    c->trivial = s_triv;
+    plan_steps(c, 0, ns, digits, keep_last_keys != 0);
+    // the last step writes values only, unless a later stage reads the keys
+    if (!keep_last_keys)
+      for (int st = ns - 1; st >= 0; st--)
+        if (c->active[st]) {
+          c->write_keys[st] = 0;
+          break;
+        }
   }
 }
 
 // ---------------------------------------------------------------------------
-// One onesweep LSD pass over 8 bits.  Tile = 256 threads x kSortItems keys,
+// One onesweep LSD pass over 8 bits.  Tile = kSortThreads x kSortItems keys,
 // warp-striped (wave w owns keys [w*64*I, (w+1)*64*I) of the tile, item i of
 // lane l is key w*64*I + i*64 + l), so ballot ranking in (item, lane) order is
 // the input order and the pass is stable.  Per-digit tile counts are chained
 // across tiles by decoupled look-back; tiles are numbered by an atomic ticket
 // so every tile waits only on tiles that have already started.
 #ifndef CT_SORT_THREADS
-#define CT_SORT_THREADS 256
+#define CT_SORT_THREADS 512
 #endif
 #ifndef CT_SORT_ITEMS
-#define CT_SORT_ITEMS 16
+#define CT_SORT_ITEMS 12
 #endif
 #ifndef CT_LOOKBACK
 #define CT_LOOKBACK 4
+#endif
+#ifndef CT_VALS_EARLY
+#define CT_VALS_EARLY 0
 #endif
 constexpr int kSortThreads = CT_SORT_THREADS;  // 256 or 512
 constexpr int kSortWaves = kSortThreads / kWave;
@@ -267,7 +302,7 @@ static_assert(kSortThreads >= 256, "one thread per digit");
 
 template <class KT>
 __global__ __launch_bounds__(kSortThreads) void k_onesweep(
-    int64_t n, int pass, KT* __restrict__ k0, KT* __restrict__ k1, uint32_t* __restrict__ v0,
+    int64_t n, int step, KT* __restrict__ k0, KT* __restrict__ k1, uint32_t* __restrict__ v0,
     uint32_t* __restrict__ v1, BuildCtrl* c, uint32_t* __restrict__ status, int keys_only) {
   __shared__ uint32_t s_wh[kSortWaves][256];
   __shared__ uint32_t s_start[256];
@@ -277,20 +312,21 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
   __shared__ KT s_buf[kSortTile];
   __shared__ uint8_t s_dig[kSortTile];
 
-  if (!c->active[pass]) return;
+  if (!c->active[step]) return;
   const int t = threadIdx.x, w = t >> 6, lane = lane_id();
+  const int pass = c->digit[step];
   const int shift = 8 * pass;
-  if (t == 0) s_tile = atomicAdd(&c->ticket[pass], 1u);
+  if (t == 0) s_tile = atomicAdd(&c->ticket[step], 1u);
   for (int i = t; i < kSortWaves * 256; i += kSortThreads) (&s_wh[0][0])[i] = 0;
   __syncthreads();
   const int64_t tile = s_tile;
-  const int src = c->src[pass];
-  const bool ident = pass == c->first;  // first pass: values are input positions
+  const int src = c->src[step];
+  const bool ident = step == c->first;  // first step: values are input positions
   const KT* kin = src ? k1 : k0;
   KT* kout = src ? k0 : k1;
   const uint32_t* vin = src ? v1 : v0;
   uint32_t* vout = src ? v0 : v1;
-  const bool wk = keys_only || c->write_keys[pass] != 0;
+  const bool wk = keys_only || c->write_keys[step] != 0;
 
   const int64_t tbase = tile * kSortTile;
   const int64_t wbase = tbase + (int64_t)w * (kWave * kSortItems);
@@ -301,6 +337,15 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
     int64_t idx = wbase + i * kWave + lane;
     key[i] = idx < n ? kin[idx] : (KT)0;
   }
+#if CT_VALS_EARLY
+  // values in flight with the keys (one memory round trip per tile)
+  uint32_t val[kSortItems];
+#pragma unroll
+  for (int i = 0; i < kSortItems; i++) {
+    int64_t idx = wbase + i * kWave + lane;
+    val[i] = (idx < n && !keys_only) ? (ident ? (uint32_t)idx : vin[idx]) : 0u;
+  }
+#endif
   const uint64_t lt = lanemask_lt();
 #pragma unroll
   for (int i = 0; i < kSortItems; i++) {
@@ -395,13 +440,15 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
     }
   }
   if (keys_only) return;
-  // values: load now (after the keys are out), reorder through LDS, scatter
+  // values: reorder through LDS, scatter
+#if !CT_VALS_EARLY
   uint32_t val[kSortItems];
 #pragma unroll
   for (int i = 0; i < kSortItems; i++) {
     int64_t idx = wbase + i * kWave + lane;
     val[i] = idx < n ? (ident ? (uint32_t)idx : vin[idx]) : 0u;
   }
+#endif
   __syncthreads();
   uint32_t* s_val = (uint32_t*)s_buf;
 #pragma unroll
@@ -415,6 +462,139 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
     int j = t + i * kSortThreads;
     if (j < tn) vout[s_adj[s_dig[j]] + (uint32_t)j] = s_val[j];
   }
+}
+
+// ---------------------------------------------------------------------------
+// Runs of keys with equal high halves, after the high-half passes: each run is
+// sorted in place by (low half, input position), which makes the whole array
+// sorted by (key, input position) -- the stable full-key order -- without the
+// four low-digit passes.  With n boxes spread over 2^32 high-half cells runs
+// are short; a run that may be longer than the halo (extreme duplicate
+// prefixes) sets `fallback` and the full LSD sort runs instead.
+//   A block owns the run starts in a tile of kFixTile positions and loads
+// kFixMax more as a halo; runs of 2..32 are insertion-sorted by their owner
+// thread in LDS, longer ones by a block-wide bitonic network, then written
+// back (keys and values, so that fallback passes see consistent data).
+constexpr int kFixTile = 2048;
+constexpr int kFixMax = 1024;
+constexpr int kFixSpan = kFixTile + kFixMax;
+constexpr int kFixNet = 2048;  // bitonic width: next power of two of kFixMax + 1
+
+__global__ __launch_bounds__(256) void k_fixup(int64_t n, BuildCtrl* c, uint64_t* k0,
+                                               uint64_t* k1, uint32_t* v0, uint32_t* v1) {
+  __shared__ uint32_t s_hi[kFixSpan + 1];  // s_hi[j + 1]: high half of key tb + j
+  __shared__ uint64_t s_ck[kFixSpan];      // (low half << 32) | input position
+  __shared__ uint64_t s_net[kFixNet];      // bitonic scratch
+  __shared__ int s_nbig;
+  __shared__ int s_big[kFixTile / 33 + 1][2];
+  const int cur = c->npass & 1;
+  uint64_t* keys = cur ? k1 : k0;
+  uint32_t* vals = cur ? v1 : v0;
+  const int64_t tb = (int64_t)blockIdx.x * kFixTile;
+  const int64_t te = tb + kFixTile < n ? tb + kFixTile : n;
+  const int64_t le = te + kFixMax < n ? te + kFixMax : n;
+  const int nl = (int)(le - tb), nown = (int)(te - tb);
+  const int t = threadIdx.x;
+  if (t == 0) s_nbig = 0;
+  for (int j = t; j < nl; j += 256) {
+    const uint64_t k = keys[tb + j];
+    s_hi[j + 1] = (uint32_t)(k >> 32);
+    s_ck[j] = ((k & 0xFFFFFFFFull) << 32) | vals[tb + j];
+  }
+  // the element before the tile (a value unequal to the first for tile 0)
+  if (t == 0) s_hi[0] = tb > 0 ? (uint32_t)(keys[tb - 1] >> 32) : ~(uint32_t)(keys[0] >> 32);
+  __syncthreads();
+  // run starts owned by this tile; L = run length within the loaded span
+  constexpr int R = kFixTile / 256;
+  int my_len[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int j = t + r * 256;
+    my_len[r] = 0;
+    if (j >= nown || s_hi[j] == s_hi[j + 1]) continue;  // not a run start
+    const uint32_t h = s_hi[j + 1];
+    int L = 1;
+    while (j + L < nl && s_hi[j + L + 1] == h) L++;
+    if (L < 2) continue;
+    if (j + L == nl && le < n) {  // may continue past the halo
+      atomicOr(&c->fallback, 1u);
+      continue;
+    }
+    my_len[r] = L;
+    if (L <= 32) {
+      for (int a = j + 1; a < j + L; a++) {  // insertion sort, owner thread only
+        const uint64_t v = s_ck[a];
+        int b = a - 1;
+        while (b >= j && s_ck[b] > v) {
+          s_ck[b + 1] = s_ck[b];
+          b--;
+        }
+        s_ck[b + 1] = v;
+      }
+    } else {
+      const int q = atomicAdd(&s_nbig, 1);
+      s_big[q][0] = j;
+      s_big[q][1] = L;
+    }
+  }
+  __syncthreads();
+  const int nbig = s_nbig;
+  for (int q = 0; q < nbig; q++) {  // longer runs: bitonic network over the block
+    const int j = s_big[q][0], L = s_big[q][1];
+    int m = 1;
+    while (m < L) m <<= 1;
+    for (int i = t; i < m; i += 256) s_net[i] = i < L ? s_ck[j + i] : ~0ull;
+    __syncthreads();
+    for (int k = 2; k <= m; k <<= 1) {
+      for (int d = k >> 1; d > 0; d >>= 1) {
+        for (int i = t; i < m; i += 256) {
+          const int p = i ^ d;
+          if (p > i) {
+            const uint64_t x = s_net[i], y = s_net[p];
+            if ((x > y) == ((i & k) == 0)) {
+              s_net[i] = y;
+              s_net[p] = x;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (int i = t; i < L; i += 256) s_ck[j + i] = s_net[i];
+    __syncthreads();
+  }
+  // write back the sorted runs (owner thread)
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int j = t + r * 256, L = my_len[r];
+    if (L < 2) continue;
+    const uint64_t hi = (uint64_t)s_hi[j + 1] << 32;
+    for (int i = j; i < j + L; i++) {
+      const uint64_t v = s_ck[i];
+      keys[tb + i] = hi | (v >> 32);
+      vals[tb + i] = (uint32_t)v;
+    }
+  }
+}
+
+// After k_fixup: if a run was too long, plan the full LSD sort (steps 4..11
+// over digits 0..7) from the current order; else nothing more runs.
+__global__ void k_fallback_plan(BuildCtrl* c) {
+  if (threadIdx.x != 0 || !c->fallback) return;
+  plan_steps(c, 4, 8, 0x76543210ull, false);
+  for (int st = kMaxSteps - 1; st >= 4; st--)
+    if (c->active[st]) {
+      c->write_keys[st] = 0;
+      break;
+    }
+}
+
+// Status words of the fallback steps (cleared only when they will run).
+__global__ __launch_bounds__(256) void k_fallback_zero(const BuildCtrl* c, uint32_t* p,
+                                                       int64_t words) {
+  if (!c->fallback) return;
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < words; i += (int64_t)gridDim.x * 256)
+    p[i] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -627,7 +807,10 @@ int build_f32(Context& ctx, curtree_bbox_f32* box, curtree_rtree_f32* rt, int ke
   const int D = key_mode == kKeyRef32 ? 4 : 8;
   const size_t kb = key_mode == kKeyRef32 ? 4 : 8;
   const int64_t ntiles = (n + kSortTile - 1) / kSortTile;
-  const size_t status_words = (size_t)D * ntiles * 256;
+  const bool m64 = key_mode == kKeyMorton64;
+  const int nsteps = m64 ? kMaxSteps : 4;  // Morton: 4 high-half steps + 8 fallback steps
+  const size_t step_words = (size_t)ntiles * 256;
+  const size_t status_words = (size_t)nsteps * step_words;
   Workspace& ws = ctx.ws;
   if ((rc = ws.reserve(WsBuildKeys0, n * kb)) || (rc = ws.reserve(WsBuildKeys1, n * kb)) ||
       (rc = ws.reserve(WsBuildVals0, n * 4)) || (rc = ws.reserve(WsBuildVals1, n * 4)) ||
@@ -652,26 +835,37 @@ int build_f32(Context& ctx, curtree_bbox_f32* box, curtree_rtree_f32* rt, int ke
     hipLaunchKernelGGL(k_extent, dim3(sgrid), dim3(256), 0, s, n, box->xmin, box->ymin,
                        box->xmax, box->ymax, c);
   }
+  // keygen clears the status words of the first four steps
   pf.mark(s, "k_keygen");
   if (key_mode == kKeyRef32)
     hipLaunchKernelGGL(k_keygen<kKeyRef32>, dim3(sgrid), dim3(256), 0, s, n, box->xmin,
                        box->ymin, box->xmax, box->ymax, box->id, c, k0, aos, status,
-                       (int64_t)status_words);
+                       (int64_t)(4 * step_words));
   else
     hipLaunchKernelGGL(k_keygen<kKeyMorton64>, dim3(sgrid), dim3(256), 0, s, n, box->xmin,
                        box->ymin, box->xmax, box->ymax, box->id, c, k0, aos, status,
-                       (int64_t)status_words);
+                       (int64_t)(4 * step_words));
   pf.mark(s, "k_sort_plan");
-  hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(256), 0, s, n, D, c);
-  for (int p = 0; p < D; p++) {
+  // REF32: digits 0..3; Morton: the high half (digits 4..7) first
+  hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(256), 0, s, n, D, c, 4,
+                     m64 ? 0x7654ull : 0x3210ull, m64 ? 1 : 0);
+  for (int st = 0; st < nsteps; st++) {
+    if (st == 4) {
+      pf.mark(s, "k_fixup");
+      hipLaunchKernelGGL(k_fixup, dim3((unsigned)((n + kFixTile - 1) / kFixTile)), dim3(256), 0,
+                         s, n, c, (uint64_t*)k0, (uint64_t*)k1, v0, v1);
+      hipLaunchKernelGGL(k_fallback_plan, dim3(1), dim3(64), 0, s, c);
+      hipLaunchKernelGGL(k_fallback_zero, dim3(1024), dim3(256), 0, s, (const BuildCtrl*)c,
+                         status + 4 * step_words, (int64_t)(8 * step_words));
+    }
     pf.mark(s, "k_onesweep");
-    uint32_t* st = status + (size_t)p * ntiles * 256;
+    uint32_t* stw = status + (size_t)st * step_words;
     if (key_mode == kKeyRef32)
       hipLaunchKernelGGL(k_onesweep<uint32_t>, dim3((unsigned)ntiles), dim3(kSortThreads), 0, s,
-                         n, p, (uint32_t*)k0, (uint32_t*)k1, v0, v1, c, st, 0);
+                         n, st, (uint32_t*)k0, (uint32_t*)k1, v0, v1, c, stw, 0);
     else
       hipLaunchKernelGGL(k_onesweep<uint64_t>, dim3((unsigned)ntiles), dim3(kSortThreads), 0, s,
-                         n, p, (uint64_t*)k0, (uint64_t*)k1, v0, v1, c, st, 0);
+                         n, st, (uint64_t*)k0, (uint64_t*)k1, v0, v1, c, stw, 0);
   }
 
   TreeView tr{rt->xmin, rt->ymin, rt->xmax, rt->ymax, rt->pos, rt->len};
@@ -772,7 +966,7 @@ int pairs_canonical_sort(Context& ctx, int* fid, int* tid, int64_t n, hipStream_
   hipLaunchKernelGGL(k_pair_pack, dim3(sgrid), dim3(256), 0, s, n, fid, tid, c, k0, status,
                      (int64_t)status_words);
   pf.mark(s, "k_sort_plan");
-  hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(256), 0, s, n, 8, c);
+  hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(256), 0, s, n, 8, c, 8, 0x76543210ull, 1);
   for (int p = 0; p < 8; p++) {
     pf.mark(s, "k_onesweep_pairs");
     hipLaunchKernelGGL(k_onesweep<uint64_t>, dim3((unsigned)ntiles), dim3(kSortThreads), 0, s, n,

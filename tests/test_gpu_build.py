@@ -131,3 +131,38 @@ def test_large_ragged_properties():
     o = np.argsort(sb[4])
     for i in range(4):
         assert np.array_equal(sb[i][o], bx[i])
+
+
+def _clustered_points(sizes, seed, background):
+    """Point boxes: clusters whose centroids share one 1/65536 cell per axis
+    (so their 64-bit Morton keys agree in the high 32 bits and differ below),
+    plus uniform background points and two corners fixing the extent."""
+    r = np.random.default_rng(seed)
+    xs, ys = [r.random(background)], [r.random(background)]
+    for s in sizes:
+        cx, cy = r.integers(1000, 64000, 2)
+        xs.append((cx + 0.5 + r.uniform(-0.45, 0.45, s)) / 65536.0)
+        ys.append((cy + 0.5 + r.uniform(-0.45, 0.45, s)) / 65536.0)
+    xs.append(np.array([0.0, 1.0]))
+    ys.append(np.array([0.0, 1.0]))
+    x = np.concatenate(xs).astype(np.float32)
+    y = np.concatenate(ys).astype(np.float32)
+    p = r.permutation(x.size)
+    x, y = x[p], y[p]
+    return (x, y, x.copy(), y.copy())
+
+
+@pytest.mark.parametrize("case", ["short_runs", "long_runs", "too_long_runs"])
+def test_morton64_equal_high_half_runs(case):
+    # the 64-bit sort orders by the high half, then sorts runs of equal high
+    # halves by (low half, input position) in place: runs of 2..32 by one
+    # thread, 33..1025 by a block network; a longer run takes the full
+    # eight-pass path.  Every case must equal the oracle's stable sort.
+    r = np.random.default_rng(7)
+    sizes = {"short_runs": r.integers(2, 33, 3000),
+             "long_runs": r.integers(33, 1026, 40),
+             "too_long_runs": [5000, 40, 3]}[case]
+    bx = _clustered_points(sizes, seed=len(case), background=100_000)
+    n = bx[0].size
+    for f in (16, 10):
+        check_against_oracle(bx, np.arange(n, dtype=np.int32), f, 1)
