@@ -53,6 +53,7 @@ struct BuildCtrl {
   int32_t npass;                   // steps that ran so far (planned): parity of the data
   int32_t first;                   // first step that runs (-1: none)
   uint32_t fallback;               // k_fixup found a run too long to fix in place
+  uint32_t nruns;                  // long runs listed for k_fixup_long
   uint32_t trivial;                // bit d: digit d is the same for every key
 };
 
@@ -78,6 +79,7 @@ __global__ __launch_bounds__(256) void k_build_init(BuildCtrl* c) {
     c->npass = 0;
     c->first = -1;
     c->fallback = 0;
+    c->nruns = 0;
     c->trivial = 0;
   }
 }
@@ -469,90 +471,117 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
 // sorted in place by (low half, input position), which makes the whole array
 // sorted by (key, input position) -- the stable full-key order -- without the
 // four low-digit passes.  With n boxes spread over 2^32 high-half cells runs
-// are short; a run that may be longer than the halo (extreme duplicate
-// prefixes) sets `fallback` and the full LSD sort runs instead.
-//   A block owns the run starts in a tile of kFixTile positions and loads
-// kFixMax more as a halo; runs of 2..32 are insertion-sorted by their owner
-// thread in LDS, longer ones by a block-wide bitonic network, then written
-// back (keys and values, so that fallback passes see consistent data).
-constexpr int kFixTile = 2048;
-constexpr int kFixMax = 1024;
-constexpr int kFixSpan = kFixTile + kFixMax;
-constexpr int kFixNet = 2048;  // bitonic width: next power of two of kFixMax + 1
+// are short.
+//   k_fixup: a block loads a tile (plus a halo of kFixShort) into LDS; the
+// owner thread of each run of 2..kFixShort starting in the tile sorts it by
+// insertion, and the block writes the rewritten positions back (keys and
+// values, so that fallback passes see consistent data).  Longer runs are
+// listed for k_fixup_long (one block per run, bitonic in LDS, up to
+// kFixLong); a longer run, or a full list, sets `fallback` and the full LSD
+// sort runs instead.
+constexpr int kFixShort = 16;
+constexpr int kFixLong = 4096;
+constexpr int kFixLongSlots = 4096;  // blocks of k_fixup_long (they loop over the list)
+
+__device__ __forceinline__ uint32_t key_hi(const uint64_t* k, int64_t i) {
+  return (uint32_t)(k[i] >> 32);
+}
+
+constexpr int kFixTile = 2048;  // run starts owned by one block of k_fixup
+constexpr int kFixSpan = kFixTile + kFixShort;
 
 __global__ __launch_bounds__(256) void k_fixup(int64_t n, BuildCtrl* c, uint64_t* k0,
-                                               uint64_t* k1, uint32_t* v0, uint32_t* v1) {
-  __shared__ uint32_t s_hi[kFixSpan + 1];  // s_hi[j + 1]: high half of key tb + j
+                                               uint64_t* k1, uint32_t* v0, uint32_t* v1,
+                                               uint32_t* runs, uint32_t runs_cap) {
+  __shared__ uint32_t s_hi[kFixSpan + 1];  // s_hi[j + 1]: high half of position tb + j
   __shared__ uint64_t s_ck[kFixSpan];      // (low half << 32) | input position
-  __shared__ uint64_t s_net[kFixNet];      // bitonic scratch
-  __shared__ int s_nbig;
-  __shared__ int s_big[kFixTile / 33 + 1][2];
+  __shared__ uint8_t s_dirty[kFixSpan];    // position rewritten by its run's owner
   const int cur = c->npass & 1;
   uint64_t* keys = cur ? k1 : k0;
   uint32_t* vals = cur ? v1 : v0;
   const int64_t tb = (int64_t)blockIdx.x * kFixTile;
   const int64_t te = tb + kFixTile < n ? tb + kFixTile : n;
-  const int64_t le = te + kFixMax < n ? te + kFixMax : n;
+  const int64_t le = te + kFixShort < n ? te + kFixShort : n;
   const int nl = (int)(le - tb), nown = (int)(te - tb);
   const int t = threadIdx.x;
-  if (t == 0) s_nbig = 0;
   for (int j = t; j < nl; j += 256) {
     const uint64_t k = keys[tb + j];
     s_hi[j + 1] = (uint32_t)(k >> 32);
     s_ck[j] = ((k & 0xFFFFFFFFull) << 32) | vals[tb + j];
+    s_dirty[j] = 0;
   }
-  // the element before the tile (a value unequal to the first for tile 0)
+  // the position before the tile (a value unequal to the first for tile 0)
   if (t == 0) s_hi[0] = tb > 0 ? (uint32_t)(keys[tb - 1] >> 32) : ~(uint32_t)(keys[0] >> 32);
   __syncthreads();
-  // run starts owned by this tile; L = run length within the loaded span
-  constexpr int R = kFixTile / 256;
-  int my_len[R];
-#pragma unroll
-  for (int r = 0; r < R; r++) {
-    const int j = t + r * 256;
-    my_len[r] = 0;
-    if (j >= nown || s_hi[j] == s_hi[j + 1]) continue;  // not a run start
+  // owner threads: runs of 2..kFixShort starting in the tile, insertion sort
+  for (int j = t; j < nown; j += 256) {
     const uint32_t h = s_hi[j + 1];
-    int L = 1;
-    while (j + L < nl && s_hi[j + L + 1] == h) L++;
-    if (L < 2) continue;
-    if (j + L == nl && le < n) {  // may continue past the halo
-      atomicOr(&c->fallback, 1u);
+    if (s_hi[j] == h || j + 1 >= nl || s_hi[j + 2] != h) continue;  // no start of a run >= 2
+    int L = 2;
+    while (L <= kFixShort && j + L < nl && s_hi[j + L + 1] == h) L++;
+    if (L > kFixShort || (j + L == nl && le < n)) {  // long (or reaching past the halo)
+      const uint32_t at = atomicAdd(&c->nruns, 1u);
+      if (at < runs_cap) runs[at] = (uint32_t)(tb + j);
+      else atomicOr(&c->fallback, 1u);
       continue;
     }
-    my_len[r] = L;
-    if (L <= 32) {
-      for (int a = j + 1; a < j + L; a++) {  // insertion sort, owner thread only
-        const uint64_t v = s_ck[a];
-        int b = a - 1;
-        while (b >= j && s_ck[b] > v) {
-          s_ck[b + 1] = s_ck[b];
-          b--;
-        }
-        s_ck[b + 1] = v;
+    for (int a2 = j + 1; a2 < j + L; a2++) {
+      const uint64_t v = s_ck[a2];
+      int b2 = a2 - 1;
+      while (b2 >= j && s_ck[b2] > v) {
+        s_ck[b2 + 1] = s_ck[b2];
+        b2--;
       }
-    } else {
-      const int q = atomicAdd(&s_nbig, 1);
-      s_big[q][0] = j;
-      s_big[q][1] = L;
+      s_ck[b2 + 1] = v;
     }
+    for (int q = j; q < j + L; q++) s_dirty[q] = 1;
   }
   __syncthreads();
-  const int nbig = s_nbig;
-  for (int q = 0; q < nbig; q++) {  // longer runs: bitonic network over the block
-    const int j = s_big[q][0], L = s_big[q][1];
+  // coalesced write-back of the rewritten positions
+  for (int j = t; j < nl; j += 256) {
+    if (!s_dirty[j]) continue;
+    const uint64_t v = s_ck[j];
+    keys[tb + j] = ((uint64_t)s_hi[j + 1] << 32) | (v >> 32);
+    vals[tb + j] = (uint32_t)v;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_fixup_long(int64_t n, BuildCtrl* c, uint64_t* k0,
+                                                    uint64_t* k1, uint32_t* v0, uint32_t* v1,
+                                                    const uint32_t* runs, uint32_t runs_cap) {
+  __shared__ uint64_t s_net[kFixLong];
+  __shared__ int s_len;
+  const uint32_t nr = c->nruns < runs_cap ? c->nruns : runs_cap;
+  if (blockIdx.x >= nr || c->fallback) return;
+  const int cur = c->npass & 1;
+  uint64_t* keys = cur ? k1 : k0;
+  uint32_t* vals = cur ? v1 : v0;
+  const int t = threadIdx.x;
+  for (uint32_t q = blockIdx.x; q < nr; q += gridDim.x) {
+    const int64_t i = runs[q];
+    const uint32_t h = key_hi(keys, i);
+    if (t == 0) {
+      int64_t L = kFixShort;
+      while (L <= kFixLong && i + L < n && key_hi(keys, i + L) == h) L++;
+      s_len = (int)L;
+      if (L > kFixLong) atomicOr(&c->fallback, 1u);
+    }
+    __syncthreads();
+    const int L = s_len;
+    if (L > kFixLong) return;  // the full LSD sort takes over
     int m = 1;
     while (m < L) m <<= 1;
-    for (int i = t; i < m; i += 256) s_net[i] = i < L ? s_ck[j + i] : ~0ull;
+    for (int j = t; j < m; j += 256)
+      s_net[j] = j < L ? ((keys[i + j] & 0xFFFFFFFFull) << 32) | vals[i + j] : ~0ull;
     __syncthreads();
     for (int k = 2; k <= m; k <<= 1) {
       for (int d = k >> 1; d > 0; d >>= 1) {
-        for (int i = t; i < m; i += 256) {
-          const int p = i ^ d;
-          if (p > i) {
-            const uint64_t x = s_net[i], y = s_net[p];
-            if ((x > y) == ((i & k) == 0)) {
-              s_net[i] = y;
+        for (int j = t; j < m; j += 256) {
+          const int p = j ^ d;
+          if (p > j) {
+            const uint64_t x = s_net[j], y = s_net[p];
+            if ((x > y) == ((j & k) == 0)) {
+              s_net[j] = y;
               s_net[p] = x;
             }
           }
@@ -560,20 +589,12 @@ __global__ __launch_bounds__(256) void k_fixup(int64_t n, BuildCtrl* c, uint64_t
         __syncthreads();
       }
     }
-    for (int i = t; i < L; i += 256) s_ck[j + i] = s_net[i];
-    __syncthreads();
-  }
-  // write back the sorted runs (owner thread)
-#pragma unroll
-  for (int r = 0; r < R; r++) {
-    const int j = t + r * 256, L = my_len[r];
-    if (L < 2) continue;
-    const uint64_t hi = (uint64_t)s_hi[j + 1] << 32;
-    for (int i = j; i < j + L; i++) {
-      const uint64_t v = s_ck[i];
-      keys[tb + i] = hi | (v >> 32);
-      vals[tb + i] = (uint32_t)v;
+    const uint64_t hi = (uint64_t)h << 32;
+    for (int j = t; j < L; j += 256) {
+      keys[i + j] = hi | (s_net[j] >> 32);
+      vals[i + j] = (uint32_t)s_net[j];
     }
+    __syncthreads();
   }
 }
 
@@ -818,6 +839,10 @@ int build_f32(Context& ctx, curtree_bbox_f32* box, curtree_rtree_f32* rt, int ke
       (rc = ws.reserve(WsBuildStatus, status_words * 4)) ||
       (rc = ws.reserve(WsBuildCtrl, sizeof(BuildCtrl))))
     return fail(rc);
+  // long runs of equal high halves (64-bit keys): at most n / (kFixShort + 1)
+  const uint32_t runs_cap = m64 ? (uint32_t)std::min<int64_t>(n / (kFixShort + 1) + 1, 1 << 22) : 1;
+  if ((rc = ws.reserve(WsBuildRuns, (size_t)runs_cap * 4))) return fail(rc);
+  uint32_t* runs = (uint32_t*)ws.ptr(WsBuildRuns);
   BuildCtrl* c = (BuildCtrl*)ws.ptr(WsBuildCtrl);
   void* k0 = ws.ptr(WsBuildKeys0);
   void* k1 = ws.ptr(WsBuildKeys1);
@@ -852,8 +877,11 @@ int build_f32(Context& ctx, curtree_bbox_f32* box, curtree_rtree_f32* rt, int ke
   for (int st = 0; st < nsteps; st++) {
     if (st == 4) {
       pf.mark(s, "k_fixup");
-      hipLaunchKernelGGL(k_fixup, dim3((unsigned)((n + kFixTile - 1) / kFixTile)), dim3(256), 0,
-                         s, n, c, (uint64_t*)k0, (uint64_t*)k1, v0, v1);
+      const unsigned fgrid = (unsigned)((n + kFixTile - 1) / kFixTile);
+      hipLaunchKernelGGL(k_fixup, dim3(fgrid), dim3(256), 0, s, n, c, (uint64_t*)k0,
+                         (uint64_t*)k1, v0, v1, runs, runs_cap);
+      hipLaunchKernelGGL(k_fixup_long, dim3(kFixLongSlots), dim3(256), 0, s, n, c, (uint64_t*)k0,
+                         (uint64_t*)k1, v0, v1, (const uint32_t*)runs, runs_cap);
       hipLaunchKernelGGL(k_fallback_plan, dim3(1), dim3(64), 0, s, c);
       hipLaunchKernelGGL(k_fallback_zero, dim3(1024), dim3(256), 0, s, (const BuildCtrl*)c,
                          status + 4 * step_words, (int64_t)(8 * step_words));

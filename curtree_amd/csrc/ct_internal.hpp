@@ -54,6 +54,7 @@ enum WsSlot {
   WsBuildAos,
   WsBuildStatus,
   WsBuildCtrl,
+  WsBuildRuns,
   WsQFrontA,
   WsQFrontB,
   WsQStatus,
