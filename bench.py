@@ -80,7 +80,14 @@ def cpu_baseline(boxes, fanout, key_mode, sample):
 def bench_batched(args, ct, datagen):
     n = args.boxes if args.boxes_set else 50_000_000
     nq = args.windows if args.windows_set else 4_000_000
-    F = args.fanout
+    tuning = None
+    tf = os.path.join(ROOT, "curtree_amd", "tuning.json")
+    if not args.no_tuning and os.path.exists(tf):
+        with open(tf) as f:
+            tuning = json.load(f)["tuned"]["batched"]
+        ct.set_option("query_tile_steps", tuning["tile_steps"])
+        ct.set_option("query_grid", tuning["grid"])
+    F = args.fanout or (tuning["fanout"] if tuning else 16)
     t0 = time.time()
     boxes = datagen.clustered_boxes(n, seed=5, frac=0.1, k=5, sigma=1e-3, wmax=1e-3)
     wins = datagen.loguniform_windows(nq, seed=6, lo=1e-5, hi=0.05)
@@ -125,7 +132,8 @@ def bench_batched(args, ct, datagen):
         "config": {"workload": f"batched uneven windows: {n:,} boxes (90% uniform + 10% in 5 "
                                f"Gaussian clusters), {nq:,} windows side log-uniform [1e-5,0.05], "
                                f"fanout {F}, budget {args.budget_pairs:,} pairs per chunk",
-                   "boxes": n, "windows": nq, "fanout": F, "budget_pairs": args.budget_pairs},
+                   "boxes": n, "windows": nq, "fanout": F, "budget_pairs": args.budget_pairs,
+                   "tuning": tuning and {k: tuning[k] for k in ("fanout", "tile_steps", "grid")}},
         "batched": {"pairs": int(st["pairs"]), "chunks": int(st["chunks"]),
                     "max_chunk_pairs": int(st["max_chunk_pairs"]),
                     "count_ms": float(st["count_ms"]), "write_ms": float(st["write_ms"]),
@@ -271,9 +279,13 @@ def main():
     ap.add_argument("--serve-rate-step", type=float, default=1.5)
     ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--serve-budget-gb", type=int, default=16)
-    ap.add_argument("--fanout", type=int, default=16)
+    ap.add_argument("--fanout", type=int, default=None,
+                    help="default: 16 (plain, serve), the tuned one (batched)")
+    ap.add_argument("--no-tuning", action="store_true",
+                    help="batched mode: library defaults instead of curtree_amd/tuning.json")
     ap.add_argument("--key", choices=["morton64", "ref32"], default="morton64")
-    ap.add_argument("--cpu-sample", type=int, default=4_000_000)
+    ap.add_argument("--cpu-sample", type=int, default=0,
+                    help="boxes timed in the CPU baseline (0: the whole workload)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--query-tile-steps", type=int, default=0,
                     help="wave-steps per query frontier tile (4, 8, 16; 0: library default)")
@@ -296,6 +308,8 @@ def main():
         ct.set_option("query_grid", args.query_grid)
     args.boxes_set = args.boxes is not None
     args.windows_set = args.windows is not None
+    if args.mode != "batched" and args.fanout is None:
+        args.fanout = 16
     if args.mode == "batched":
         return bench_batched(args, ct, datagen)
     if args.mode == "serve":
@@ -387,6 +401,7 @@ def main():
     step_ms = build_ms + query_ms
     kernels = {}
     for name in ("k_build_init", "k_extent", "k_keygen", "k_sort_plan", "k_onesweep", "k_fixup",
+                 "k_onesweep_fallback",
                  "k_gather_l0", "k_level", "k_safe", "k_run_begin", "k_expand", "k_leaf_count",
                  "k_leaf_pass", "k_write_pieces"):
         tot, cnt = ct.Prof.get(name)
@@ -448,7 +463,8 @@ def main():
                      "algorithmic_bytes_per_launch": dom_bytes},
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(boxes, F, key_mode, min(args.cpu_sample, n))
+        out["cpu_baseline"] = cpu_baseline(boxes, F, key_mode,
+                                           min(args.cpu_sample, n) if args.cpu_sample else n)
     ct.lib.curtree_free(fbuf)
     ct.lib.curtree_free(tbuf)
     print(json.dumps(out), flush=True)

@@ -405,6 +405,7 @@ def query_batched(rt, rbox, qbox, budget_pairs, on_chunk=None):
 
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
+kQThreads = 256  # threads per block of the chained query kernels
 
 
 class Server:

@@ -886,7 +886,8 @@ int build_f32(Context& ctx, curtree_bbox_f32* box, curtree_rtree_f32* rt, int ke
       hipLaunchKernelGGL(k_fallback_zero, dim3(1024), dim3(256), 0, s, (const BuildCtrl*)c,
                          status + 4 * step_words, (int64_t)(8 * step_words));
     }
-    pf.mark(s, "k_onesweep");
+    // fallback steps exit at once unless k_fixup asked for them
+    pf.mark(s, st < 4 ? "k_onesweep" : "k_onesweep_fallback");
     uint32_t* stw = status + (size_t)st * step_words;
     if (key_mode == kKeyRef32)
       hipLaunchKernelGGL(k_onesweep<uint32_t>, dim3((unsigned)ntiles), dim3(kSortThreads), 0, s,

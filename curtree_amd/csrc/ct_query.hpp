@@ -12,7 +12,7 @@ constexpr int kQThreads = 256;
 // CU) are process-wide tuning knobs (curtree_set_option), picked by the
 // autotuner.
 constexpr int kDefaultSteps = 8;
-constexpr int kDefaultChainGrid = 8192;
+constexpr int kDefaultChainGrid = 2048;  // resident blocks: 8 per CU on 256 CUs
 struct QTuning {
   int steps = kDefaultSteps;
   int grid = 0;      // blocks per chained launch; 0: by batch size, up to kDefaultChainGrid

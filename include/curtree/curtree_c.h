@@ -276,7 +276,7 @@ CURTREE_API int curtree_query_last_stats(int64_t* entries, int cap, int* levels,
  *                  (tile = 4 waves x steps x 64/next_pow2(F) entries).
  *   "query_grid"   blocks of each chained query launch (they loop over
  *                  tiles), i.e. waves per CU; 0 (default): by batch size,
- *                  windows/8 clamped to [512, 8192].
+ *                  windows/8 clamped to [512, 2048].
  *   "box_staging"  -1 (default: auto), 0 off, 1 on: a query copies boxes,
  *                  node MBRs and windows into AoS float4 arrays first (one
  *                  extra pass over the boxes; auto when the batch has at
