@@ -403,7 +403,7 @@ def main():
     for name in ("k_build_init", "k_extent", "k_keygen", "k_sort_plan", "k_onesweep", "k_fixup",
                  "k_onesweep_fallback",
                  "k_gather_l0", "k_level", "k_safe", "k_run_begin", "k_expand", "k_leaf_count",
-                 "k_leaf_pass", "k_write_pieces"):
+                 "k_leaf_pass", "k_tile_scan", "k_leaf_write", "k_write_pieces"):
         tot, cnt = ct.Prof.get(name)
         if cnt:
             kernels[name] = {"avg_ms": tot / cnt, "launches_per_step": cnt / args.steps,

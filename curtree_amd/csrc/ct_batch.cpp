@@ -91,9 +91,12 @@ int query_batched_f32(Context& ctx, const curtree_rtree_f32* rt, const curtree_b
   // ---- buffers sized once for every chunk (the graph holds their addresses)
   QRunArgs a;
   if ((rc = prepare_query(qe, ctx.prof, rt, rbox, qbox, ctx.opt_contain, Q, &a, s))) return rc;
-  if ((rc = qe.reserve_front(std::max<int64_t>((int64_t)front_need + 1024, 1 << 16)))) return rc;
   int64_t max_q = 0;
   for (int64_t k = 0; k < nchunks; k++) max_q = std::max(max_q, plan[2 * k + 1] - plan[2 * k]);
+  // frontiers and leaf entries (a one-level tree has one leaf entry per window)
+  if ((rc = qe.reserve_front(
+           std::max<int64_t>(std::max<int64_t>((int64_t)front_need + 1024, max_q), 1 << 16))))
+    return rc;
   if ((rc = qe.reserve_status(std::max<int64_t>(qe.cap_front, max_q) / per_tile(a.G, a.steps) + 2)))
     return rc;
   if ((rc = bs.ws.reserve(WsBatchOut, (size_t)out_cap * 8))) return rc;

@@ -130,9 +130,18 @@ struct QCtrl {
 };
 
 // Work buffers of one query pipeline; one engine serves one stream at a time.
+// One leaf entry between the leaf kernels: hit mask (partial leaf-parent) or
+// range length (contained subtree), first sorted box, query | range flag.
+struct alignas(16) QLeaf {
+  uint64_t val;
+  uint32_t first;
+  uint32_t qf;
+};
 struct QEngine {
   void* front[2] = {nullptr, nullptr};  // (query, node) frontiers, uint2 entries
-  int64_t cap_front = 0;                // entries per frontier buffer
+  int64_t cap_front = 0;                // entries per frontier buffer (and leaf entries)
+  QLeaf* est = nullptr;                 // leaf entries of a writing run
+  uint64_t* ttot = nullptr;             // pairs per leaf tile, then their exclusive scan
   void* pieces = nullptr;               // long box ranges of contained subtrees
   int64_t cap_pieces = 0;
   uint64_t* status = nullptr;           // chained-scan status words

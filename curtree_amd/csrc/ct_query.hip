@@ -89,7 +89,9 @@ __device__ __forceinline__ uint64_t chain_tile(uint64_t wcount, int64_t tile, in
   if (lane == 0) s_wtot[w] = wcount;
   __syncthreads();
   if (w == 0) {
-    uint64_t tt = s_wtot[0] + s_wtot[1] + s_wtot[2] + s_wtot[3];
+    uint64_t tt = 0;
+#pragma unroll
+    for (int i = 0; i < kQWaves; i++) tt += s_wtot[i];
     uint64_t excl = 0;
     if (tile == 0) {
       if (lane == 0) st_agent(status, st_pack(2u * epoch + 2u, tt));
@@ -129,7 +131,7 @@ __global__ __launch_bounds__(kQThreads) void k_expand(int slot, QTree tr, QBoxes
                                                       uint64_t* status, int64_t r_sz) {
   using S = QShape<G, STEPS>;
   __shared__ uint32_t s_tile;
-  __shared__ uint64_t s_wtot[4];
+  __shared__ uint64_t s_wtot[kQWaves];
   __shared__ uint64_t s_base, s_total;
   if (c->overflow) return;
   const int64_t n_in = (int64_t)c->count[slot];
@@ -246,19 +248,23 @@ __device__ __forceinline__ void node_range(uint32_t v, const QLevels* L, int64_t
 
 // Leaf stage.  Each entry is either a partial leaf-parent (its boxes are
 // tested: G lanes, one box each) or a contained subtree (all of its box range
-// hits, no reads).
-//   COUNT: per-wave totals chained across tiles (and per-query totals).
-//   WRITE: each entry's state (hit mask or range length, first box, query)
-//          is staged in LDS; a tile-wide scan gives every entry its output
-//          offset, the chained look-back the tile's base.  The block then
-//          writes the tile's pairs as one flat, coalesced sequence (each slot
-//          finds its entry by binary search); ranges longer than kFlatMax are
-//          cut into pieces for k_write_pieces.  Slots beyond out_cap are
-//          dropped; the total is always counted.  Nothing is held in
-//          registers across the look-back, which keeps occupancy high.
+// hits, no reads).  Three launches, none of which waits on another tile:
+//   k_leaf_count  tests the boxes; per entry it stores the hit mask (or range
+//                 length), first box and query (16 B), per tile the pair
+//                 count; totals and per-query counts by atomics.  Count-only
+//                 runs stop here and store nothing.
+//   k_tile_scan   exclusive scan of the tile counts (a short chain: one link
+//                 per kQThreads * 8 tiles).
+//   k_leaf_write  per tile: loads the entries into LDS, scans them locally,
+//                 adds the tile's base and writes the tile's pairs as one
+//                 flat, coalesced sequence (each slot finds its entry by
+//                 binary search); ranges longer than kFlatMax are cut into
+//                 pieces for k_write_pieces.  Slots beyond out_cap are dropped.
+// Output order is the entries' order: (query, node), then box position.
 constexpr uint32_t kQRange = 0x80000000u;
 constexpr uint32_t kQNone = 0xFFFFFFFFu;
 constexpr uint64_t kFlatMax = 2048;  // longer ranges go to the piece writer
+constexpr int kScanPer = 8;          // tile counts per thread in k_tile_scan
 
 // Position of the r-th (0-based) set bit of m (r < popcount(m)).
 __device__ __forceinline__ uint32_t nth_set_bit(uint64_t m, uint32_t r) {
@@ -278,7 +284,7 @@ __device__ __forceinline__ uint32_t nth_set_bit(uint64_t m, uint32_t r) {
   return pos;
 }
 
-// Exclusive scan of one u64 per thread over a 256-thread block.
+// Exclusive scan of one u64 per thread over the block.
 __device__ __forceinline__ uint64_t block_excl_scan64(uint64_t v, uint64_t* s_tmp, uint64_t* total) {
   const int lane = lane_id(), w = threadIdx.x >> 6;
   uint64_t x = v;
@@ -291,7 +297,7 @@ __device__ __forceinline__ uint64_t block_excl_scan64(uint64_t v, uint64_t* s_tm
   __syncthreads();
   uint64_t off = 0, tot = 0;
 #pragma unroll
-  for (int i = 0; i < kQThreads / kWave; i++) {
+  for (int i = 0; i < kQWaves; i++) {
     const uint64_t t = s_tmp[i];
     if (i < w) off += t;
     tot += t;
@@ -301,37 +307,22 @@ __device__ __forceinline__ uint64_t block_excl_scan64(uint64_t v, uint64_t* s_tm
   return x - v + off;
 }
 
-template <int G, int STEPS, bool WRITE>
-__global__ __launch_bounds__(kQThreads) void k_leaf_pass(
-    int slot, QTree tr, QBoxes qb, QBoxes rb, const int* __restrict__ rid, QCtrl* c,
-    const uint2* __restrict__ fin, int64_t r_sz, int64_t nbox, int* __restrict__ out_fid,
-    int* __restrict__ out_tid, int64_t out_cap, QPiece* __restrict__ pieces, int64_t piece_cap,
-    unsigned long long* counts, uint64_t* status) {
+template <int G, int STEPS, bool STORE>
+__global__ __launch_bounds__(kQThreads) void k_leaf_count(
+    int slot, QTree tr, QBoxes qb, QBoxes rb, QCtrl* c, const uint2* __restrict__ fin,
+    int64_t r_sz, int64_t nbox, QLeaf* __restrict__ est, uint64_t* __restrict__ ttot,
+    unsigned long long* counts) {
   using S = QShape<G, STEPS>;
-  constexpr int P = S::kPerTile;                     // entries per tile
-  constexpr int R = (P + kQThreads - 1) / kQThreads;  // entries per thread in the tile scan
-  constexpr int PW = WRITE ? P : 1;
+  constexpr int P = S::kPerTile;
   __shared__ uint32_t s_tile;
-  __shared__ uint64_t s_wtot[4];
-  __shared__ uint64_t s_base, s_total;
-  __shared__ uint64_t s_val[PW];   // partial: hit mask; range: length
-  __shared__ uint64_t s_off[PW];   // tile-relative first output slot
-  __shared__ int64_t s_first[PW];  // first sorted box
-  __shared__ uint32_t s_q[PW];     // query | kQRange, kQNone past the end
-  __shared__ uint32_t s_foff[PW];  // tile-relative first flat slot
-  __shared__ uint32_t s_ftot;
-  __shared__ uint64_t s_scan[kQThreads / kWave];
+  __shared__ uint64_t s_red[kQWaves];
   if (c->overflow) return;
   const int64_t n_in = (int64_t)c->count[slot];
-  const uint32_t epoch = c->epoch_base + slot;
   const int64_t q0 = c->q0;
   const QLevels* lv = &c->lv;
   const int64_t ntiles = (n_in + P - 1) / P;
   const int t = threadIdx.x, w = t >> 6, lane = lane_id();
-  if (ntiles == 0) {
-    if (blockIdx.x == 0 && t == 0) c->total = 0;
-    return;
-  }
+  if (ntiles == 0) return;  // the run's total stays 0
   const int sub = lane / G, c_ = lane % G;
   const int lead = sub * G;  // first lane of this lane's group
   // bulk-load layout: leaf-parent i holds sorted boxes i*F .. i*F+F-1
@@ -339,12 +330,11 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_pass(
   const int64_t ls = lv->lstart[0];
   int64_t tile;
   while ((tile = next_tile(&c->ticket[slot], ntiles, &s_tile)) >= 0) {
-    uint64_t wcount = 0;
+    uint64_t wsum = 0;
     const int64_t tbase = tile * P;
 #pragma unroll
     for (int k = 0; k < STEPS; k++) {
-      const int li = w * S::kPerWave + k * S::E + sub;  // entry index within the tile
-      const int64_t e = tbase + li;
+      const int64_t e = tbase + w * S::kPerWave + k * S::E + sub;
       bool hit = false, full = false;
       uint32_t q = 0;
       int64_t first = 0, rcnt = 0;
@@ -380,103 +370,175 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_pass(
       const uint64_t m = ballot(hit);
       const uint64_t gm = (m >> lead) & S::kGroupMask;
       const uint64_t cnt = full ? (uint64_t)rcnt : (uint64_t)popc64(gm);
-      if (c_ == 0) {
-        if (WRITE) {
-          s_val[li] = full ? cnt : gm;
-          s_off[li] = cnt;
-          s_first[li] = first;
-          s_q[li] = e < n_in ? (q | (full ? kQRange : 0u)) : kQNone;
-        } else {
-          if (counts && e < n_in && cnt) atomicAdd(counts + q, (unsigned long long)cnt);
+      if (c_ == 0 && e < n_in) {
+        if (STORE) {
+          QLeaf x;
+          x.val = full ? cnt : gm;
+          x.first = (uint32_t)first;
+          x.qf = q | (full ? kQRange : 0u);
+          est[e] = x;
         }
+        if (counts && cnt) atomicAdd(counts + q, (unsigned long long)cnt);
       }
-      if (!WRITE) {
-        // wave total of this step (group leaders hold the counts)
-        uint64_t v = c_ == 0 ? cnt : 0ull;
+      uint64_t v = c_ == 0 ? cnt : 0ull;
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-        wcount += v;
-      }
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+      wsum += v;
     }
-    if (WRITE) {
-      __syncthreads();
-      // tile-wide exclusive scans of the entry counts, in entry order: all
-      // pairs (the output slots) and the flat ones (partial entries and
-      // ranges up to kFlatMax, written by the whole block below)
-      uint64_t loc[R];
-      uint64_t sum = 0, fsum = 0;
+    if (lane == 0) s_red[w] = wsum;
+    __syncthreads();
+    if (t == 0) {
+      uint64_t tot = 0;
 #pragma unroll
-      for (int r = 0; r < R; r++) {
-        const int li = t * R + r;
-        loc[r] = li < P ? s_off[li] : 0ull;
+      for (int i = 0; i < kQWaves; i++) tot += s_red[i];
+      if (STORE) ttot[tile] = tot;
+      if (tot) atomicAdd((unsigned long long*)&c->total, (unsigned long long)tot);
+    }
+  }
+}
+
+// Exclusive scan of the leaf tile counts in place (tiles of P entries).
+__global__ __launch_bounds__(kQThreads) void k_tile_scan(int slot, int P, QCtrl* c,
+                                                         uint64_t* __restrict__ ttot,
+                                                         uint64_t* status) {
+  __shared__ uint32_t s_tile;
+  __shared__ uint64_t s_wtot[kQWaves];
+  __shared__ uint64_t s_base, s_total;
+  if (c->overflow) return;
+  const int64_t n = ((int64_t)c->count[slot - 1] + P - 1) / P;
+  constexpr int T = kQThreads * kScanPer;
+  const int64_t ntiles = (n + T - 1) / T;
+  const uint32_t epoch = c->epoch_base + slot;
+  const int t = threadIdx.x, lane = lane_id();
+  int64_t tile;
+  while ((tile = next_tile(&c->ticket[slot], ntiles, &s_tile)) >= 0) {
+    const int64_t b0 = tile * T + (int64_t)t * kScanPer;
+    uint64_t v[kScanPer];
+    uint64_t sum = 0;
+#pragma unroll
+    for (int j = 0; j < kScanPer; j++) {
+      v[j] = b0 + j < n ? ttot[b0 + j] : 0ull;
+      sum += v[j];
+    }
+    uint64_t x = sum;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+      const uint64_t y = __shfl_up(x, o);
+      if (lane >= o) x += y;
+    }
+    const uint64_t wtot = __shfl(x, kWave - 1);
+    uint64_t b = chain_tile(wtot, tile, ntiles, status, epoch, s_wtot, &s_base, &s_total);
+    b += x - sum;
+#pragma unroll
+    for (int j = 0; j < kScanPer; j++) {
+      if (b0 + j < n) ttot[b0 + j] = b;
+      b += v[j];
+    }
+  }
+}
+
+template <int G, int STEPS>
+__global__ __launch_bounds__(kQThreads) void k_leaf_write(
+    int slot, QCtrl* c, const QLeaf* __restrict__ est, const uint64_t* __restrict__ tbase_arr,
+    const int* __restrict__ rid, int* __restrict__ out_fid, int* __restrict__ out_tid,
+    int64_t out_cap, QPiece* __restrict__ pieces, int64_t piece_cap) {
+  using S = QShape<G, STEPS>;
+  constexpr int P = S::kPerTile;                     // entries per tile
+  constexpr int R = (P + kQThreads - 1) / kQThreads;  // entries per thread
+  __shared__ uint64_t s_val[P];   // partial: hit mask; range: length
+  __shared__ uint64_t s_off[P];   // tile-relative first output slot
+  __shared__ uint32_t s_first[P];  // first sorted box
+  __shared__ uint32_t s_q[P];     // query | kQRange, kQNone past the end
+  __shared__ uint32_t s_foff[P];  // tile-relative first flat slot
+  __shared__ uint32_t s_ftot;
+  __shared__ uint64_t s_scan[kQWaves];
+  if (c->overflow) return;
+  const int64_t n_in = (int64_t)c->count[slot];
+  const int64_t ntiles = (n_in + P - 1) / P;
+  const int t = threadIdx.x;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    __syncthreads();  // the previous tile is done with the shared arrays
+    // the tile's entries, and scans of all pairs and of the flat ones
+    // (partial entries and ranges up to kFlatMax, written by the block)
+    uint64_t loc[R];
+    uint64_t sum = 0, fsum = 0;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int li = t * R + r;
+      loc[r] = 0;
+      if (li < P) {
+        const int64_t e = tile * P + li;
+        uint32_t qf = kQNone;
+        if (e < n_in) {
+          const QLeaf x = est[e];
+          qf = x.qf;
+          s_val[li] = x.val;
+          s_first[li] = x.first;
+          loc[r] = (qf & kQRange) ? x.val : (uint64_t)popc64(x.val);
+        }
+        s_q[li] = qf;
         sum += loc[r];
-        if (li < P && !((s_q[li] & kQRange) && s_q[li] != kQNone && loc[r] > kFlatMax))
-          fsum += loc[r];
+        if (!((qf & kQRange) && qf != kQNone && loc[r] > kFlatMax)) fsum += loc[r];
       }
-      uint64_t ttot, ftot;
-      uint64_t ex = block_excl_scan64(sum, s_scan, &ttot);
-      uint64_t fex = block_excl_scan64(fsum, s_scan, &ftot);
-#pragma unroll
-      for (int r = 0; r < R; r++) {
-        const int li = t * R + r;
-        if (li < P) {
-          const bool piece = (s_q[li] & kQRange) && s_q[li] != kQNone && loc[r] > kFlatMax;
-          s_off[li] = ex;
-          s_foff[li] = (uint32_t)fex;
-          if (!piece) fex += loc[r];
-          if (counts && loc[r] && s_q[li] != kQNone)
-            atomicAdd(counts + (s_q[li] & ~kQRange), (unsigned long long)loc[r]);
-        }
-        ex += loc[r];
-      }
-      if (t == 0) s_ftot = (uint32_t)ftot;
-      wcount = w == 0 ? ttot : 0ull;  // the tile total enters through wave 0
     }
-    chain_tile(wcount, tile, ntiles, status, epoch, s_wtot, &s_base, &s_total);
-    if (tile == ntiles - 1 && t == 0) c->total = s_total;
-    if (WRITE) {
-      const uint64_t base = s_base;
-      // long ranges: pieces for k_write_pieces
+    uint64_t ttot, ftot;
+    uint64_t ex = block_excl_scan64(sum, s_scan, &ttot);
+    uint64_t fex = block_excl_scan64(fsum, s_scan, &ftot);
 #pragma unroll
-      for (int r = 0; r < R; r++) {
-        const int li = t * R + r;
-        if (li >= P) break;
+    for (int r = 0; r < R; r++) {
+      const int li = t * R + r;
+      if (li < P) {
         const uint32_t qf = s_q[li];
-        if (qf == kQNone || !(qf & kQRange)) continue;
-        const uint64_t ev = s_val[li];
-        if (ev <= kFlatMax) continue;
-        const uint64_t off = base + s_off[li];
-        const uint64_t np = (ev + kPiece - 1) / kPiece;
-        const uint64_t at = atomicAdd(&c->npieces, (unsigned long long)np);
-        for (uint64_t p = 0; p < np; p++) {
-          if (at + p >= (uint64_t)piece_cap) break;  // only when the total exceeds out_cap
-          const uint64_t rest = ev - p * kPiece;
-          QPiece pc;
-          pc.off = off + p * kPiece;
-          pc.first = s_first[li] + (int64_t)(p * kPiece);
-          pc.len = (uint32_t)(rest < kPiece ? rest : kPiece);
-          pc.q = qf & ~kQRange;
-          pieces[at + p] = pc;
-        }
+        const bool piece = (qf & kQRange) && qf != kQNone && loc[r] > kFlatMax;
+        s_off[li] = ex;
+        s_foff[li] = (uint32_t)fex;
+        if (!piece) fex += loc[r];
       }
-      // flat slots, load-balanced over the block: slot j belongs to the last
-      // entry whose flat offset is <= j (binary search), coalesced writes
-      const uint32_t ftot = s_ftot;
-#pragma unroll 4
-      for (uint32_t j = t; j < ftot; j += kQThreads) {
-        int lo = 0;
+      ex += loc[r];
+    }
+    if (t == 0) s_ftot = (uint32_t)ftot;
+    __syncthreads();
+    const uint64_t base = tbase_arr[tile];
+    // long ranges: pieces for k_write_pieces
 #pragma unroll
-        for (int step = P / 2; step > 0; step >>= 1)
-          if (s_foff[lo + step] <= j) lo += step;
-        const uint32_t qf = s_q[lo];
-        const uint32_t r = j - s_foff[lo];
-        uint32_t bpos = r;
-        if (!(qf & kQRange)) bpos = nth_set_bit(s_val[lo], r);
-        const uint64_t o = base + s_off[lo] + r;
-        if (o < (uint64_t)out_cap) {
-          out_fid[o] = rid[s_first[lo] + bpos];
-          out_tid[o] = (int)(qf & ~kQRange);
-        }
+    for (int r = 0; r < R; r++) {
+      const int li = t * R + r;
+      if (li >= P) break;
+      const uint32_t qf = s_q[li];
+      if (qf == kQNone || !(qf & kQRange)) continue;
+      const uint64_t ev = s_val[li];
+      if (ev <= kFlatMax) continue;
+      const uint64_t off = base + s_off[li];
+      const uint64_t np = (ev + kPiece - 1) / kPiece;
+      const uint64_t at = atomicAdd(&c->npieces, (unsigned long long)np);
+      for (uint64_t p = 0; p < np; p++) {
+        if (at + p >= (uint64_t)piece_cap) break;  // only when the total exceeds out_cap
+        const uint64_t rest = ev - p * kPiece;
+        QPiece pc;
+        pc.off = off + p * kPiece;
+        pc.first = (int64_t)s_first[li] + (int64_t)(p * kPiece);
+        pc.len = (uint32_t)(rest < kPiece ? rest : kPiece);
+        pc.q = qf & ~kQRange;
+        pieces[at + p] = pc;
+      }
+    }
+    // flat slots, load-balanced over the block: slot j belongs to the last
+    // entry whose flat offset is <= j (binary search), coalesced writes
+    const uint32_t fn = s_ftot;
+#pragma unroll 4
+    for (uint32_t j = t; j < fn; j += kQThreads) {
+      int lo = 0;
+#pragma unroll
+      for (int step = P / 2; step > 0; step >>= 1)
+        if (s_foff[lo + step] <= j) lo += step;
+      const uint32_t qf = s_q[lo];
+      const uint32_t r = j - s_foff[lo];
+      uint32_t bpos = r;
+      if (!(qf & kQRange)) bpos = nth_set_bit(s_val[lo], r);
+      const uint64_t o = base + s_off[lo] + r;
+      if (o < (uint64_t)out_cap) {
+        out_fid[o] = rid[(int64_t)s_first[lo] + bpos];
+        out_tid[o] = (int)(qf & ~kQRange);
       }
     }
   }
@@ -551,8 +613,11 @@ static int dev_alloc(void** p, size_t bytes) {
 int QEngine::reserve_front(int64_t entries) {
   if (entries <= cap_front) return CURTREE_OK;
   int rc;
+  // leaf entries (one QLeaf each) and tile counts (tiles hold >= 16 entries)
   if ((rc = dev_alloc(&front[0], (size_t)entries * 8)) ||
-      (rc = dev_alloc(&front[1], (size_t)entries * 8))) {
+      (rc = dev_alloc(&front[1], (size_t)entries * 8)) ||
+      (rc = dev_alloc((void**)&est, (size_t)entries * sizeof(QLeaf))) ||
+      (rc = dev_alloc((void**)&ttot, (size_t)(entries / 16 + 2) * 8))) {
     cap_front = 0;
     return rc;
   }
@@ -604,10 +669,13 @@ int QEngine::ensure_ctrl() {
 }
 
 void QEngine::release() {
-  void* ps[] = {front[0], front[1], pieces, status, ctrl, safe, plan, box4, node4, win4, canon};
+  void* ps[] = {front[0], front[1], pieces, status, ctrl,  safe, plan,
+                box4,     node4,    win4,   canon,  est,   ttot};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   box4 = node4 = win4 = nullptr;
+  est = nullptr;
+  ttot = nullptr;
   canon = nullptr;
   cap_box4 = cap_node4 = cap_win4 = 0;
   if (host_ctrl) (void)hipHostFree(host_ctrl);
@@ -732,7 +800,8 @@ int compute_safe(QEngine& qe, Prof& pf, const QBoxes& rb, int F, int64_t r_sz, i
 
 template <int G, int STEPS>
 void enqueue_run_g(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
-  const unsigned nslots = (unsigned)a.nl;  // nl-1 expansions + 1 leaf stage
+  // nl-1 expansions, the leaf stage and the tile scan
+  const unsigned nslots = (unsigned)a.nl + 1;
   if (qe.needs_clear) {
     hipLaunchKernelGGL(k_ctrl_init, dim3(1), dim3(64), 0, s, qe.ctrl);
     qe.needs_clear = false;
@@ -754,21 +823,26 @@ void enqueue_run_g(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
   }
   const int leaf = a.nl - 1;
   if (a.write) {
+    using S = QShape<G, STEPS>;
     pf.mark(s, "k_leaf_pass");
-    hipLaunchKernelGGL((k_leaf_pass<G, STEPS, true>), dim3(a.grid), dim3(kQThreads), 0, s, leaf, a.tr,
-                       a.qb, a.rb, a.rid, qe.ctrl, cur, a.r_sz, a.nbox, a.out_fid,
-                       a.out_tid, a.out_cap, (QPiece*)qe.pieces, qe.cap_pieces, a.counts,
-                       qe.status);
+    hipLaunchKernelGGL((k_leaf_count<G, STEPS, true>), dim3(a.grid), dim3(kQThreads), 0, s, leaf,
+                       a.tr, a.qb, a.rb, qe.ctrl, cur, a.r_sz, a.nbox, qe.est, qe.ttot, a.counts);
+    pf.mark(s, "k_tile_scan");
+    hipLaunchKernelGGL(k_tile_scan, dim3(64), dim3(kQThreads), 0, s, leaf + 1, S::kPerTile,
+                       qe.ctrl, qe.ttot, qe.status);
+    pf.mark(s, "k_leaf_write");
+    hipLaunchKernelGGL((k_leaf_write<G, STEPS>), dim3(a.grid), dim3(kQThreads), 0, s, leaf,
+                       qe.ctrl, (const QLeaf*)qe.est, (const uint64_t*)qe.ttot, a.rid, a.out_fid,
+                       a.out_tid, a.out_cap, (QPiece*)qe.pieces, qe.cap_pieces);
     pf.mark(s, "k_write_pieces");
     hipLaunchKernelGGL(k_write_pieces, dim3(4096), dim3(256), 0, s, (const QCtrl*)qe.ctrl,
                        (const QPiece*)qe.pieces, qe.cap_pieces, a.rid, a.out_fid, a.out_tid,
                        a.out_cap);
   } else {
     pf.mark(s, "k_leaf_count");
-    hipLaunchKernelGGL((k_leaf_pass<G, STEPS, false>), dim3(a.grid), dim3(kQThreads), 0, s, leaf, a.tr,
-                       a.qb, a.rb, a.rid, qe.ctrl, cur, a.r_sz, a.nbox, (int*)nullptr,
-                       (int*)nullptr, (int64_t)0, (QPiece*)nullptr, (int64_t)0, a.counts,
-                       qe.status);
+    hipLaunchKernelGGL((k_leaf_count<G, STEPS, false>), dim3(a.grid), dim3(kQThreads), 0, s, leaf,
+                       a.tr, a.qb, a.rb, qe.ctrl, cur, a.r_sz, a.nbox, (QLeaf*)nullptr,
+                       (uint64_t*)nullptr, a.counts);
   }
   pf.mark(s, nullptr);
 }
@@ -842,7 +916,7 @@ int prepare_query(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curt
     return set_error(CURTREE_EINVAL,
                      "query: tree size %d does not match %lld boxes at fanout %d (expected %lld)",
                      rt->sz, (long long)N, F, (long long)lp.r_sz);
-  if (lp.nl + 1 > kMaxRunSlots) return set_error(CURTREE_EINVAL, "query: tree too deep");
+  if (lp.nl + 2 > kMaxRunSlots) return set_error(CURTREE_EINVAL, "query: tree too deep");
   if (!rt->xmin || !rt->ymin || !rt->xmax || !rt->ymax || !rt->pos || !rt->len || !rbox->xmin ||
       !rbox->ymin || !rbox->xmax || !rbox->ymax || !rbox->id ||
       (Q > 0 && (!qbox->xmin || !qbox->ymin || !qbox->xmax || !qbox->ymax)))

@@ -6,7 +6,11 @@
 
 namespace ct {
 
-constexpr int kQThreads = 256;
+#ifndef CT_QTHREADS
+#define CT_QTHREADS 256
+#endif
+constexpr int kQThreads = CT_QTHREADS;  // threads per block of the chained query kernels
+constexpr int kQWaves = kQThreads / 64;
 // Tile of a chained launch: 4 waves x STEPS wave-steps x (64/G) entries.
 // STEPS (4, 8 or 16) and the grid (blocks looping over tiles, i.e. waves per
 // CU) are process-wide tuning knobs (curtree_set_option), picked by the
