@@ -25,7 +25,7 @@ Other modes (same JSON contract, one line):
                   side log-uniform [1e-5, 2e-3]; 16 GB result budget.  The
                   request rate is swept for plain per-request launches (one
                   request at a time) and for the serving runtime (continuous
-                  batching over 8 streams); value = served pairs/s at the plain
+                  batching over 4 streams); value = served pairs/s at the plain
                   mode's p99 latency (at its highest sustained rate).
 """
 import argparse
@@ -217,7 +217,7 @@ def bench_serve(args, ct, datagen):
              "served": dict(slots=args.serve_slots, batching=True)}
     res = {}
     for name, kw in modes.items():
-        sv = ct.Server(budget_bytes=budget, max_batch_windows=1 << 20, **kw)
+        sv = ct.Server(budget_bytes=budget, max_batch_windows=args.serve_batch_windows, **kw)
         ids = [sv.add_tree(rt, d) for rt, d, _ in trees]
         pts = []
         _poisson_run(ct, sv, ids, pool, npool, 200.0, 0.5, seed=1)  # warm-up
@@ -277,7 +277,9 @@ def main():
     ap.add_argument("--serve-rate0", type=float, default=250.0)
     ap.add_argument("--serve-rate-max", type=float, default=64000.0)
     ap.add_argument("--serve-rate-step", type=float, default=1.5)
-    ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-batch-windows", type=int, default=1 << 20,
+                    help="serve mode: window cap of one batch (and of one request)")
     ap.add_argument("--serve-budget-gb", type=int, default=16)
     ap.add_argument("--fanout", type=int, default=None,
                     help="default: 16 (plain, serve), the tuned one (batched)")

@@ -1,17 +1,18 @@
 // ct_serve.cpp -- serving: several bulk-loaded trees resident in HBM, window
 // requests of any size submitted concurrently, continuous batching.
 //
-// A dispatcher thread owns S slots (stream + query engine + window staging).
-// Whenever a slot is free it takes the oldest queued request, adds every other
-// queued request on the same tree (up to a window cap), reserves result space
-// in the pool (the memory budget; a batch waits while the pool is full) and
-// runs the batch's query writing straight into it, with per-window counts.
-// When the pairs fit the reservation shrinks to them: one pass, one host
+// The server owns S slots (stream + query engine + window staging), each run
+// by a worker thread.  Whenever its slot is free, a worker takes the oldest
+// queued request, adds every other queued request on the same tree (up to a
+// window cap), reserves result space in the pool (the memory budget; a batch
+// waits while the pool is full) and runs the batch's query writing straight
+// into it, with per-window counts.  When the pairs fit the reservation shrinks
+// to them: one pass, one host
 // round trip per batch; otherwise a second pass writes into an exact
-// reservation.  Slots run concurrently on their own streams and the
-// dispatcher only polls events, so requests that arrive while others are on
-// the device are admitted as slots and memory free up, and their work
-// overlaps.  A request's pairs are a contiguous slice of its batch's output;
+// reservation.  Slots run concurrently on their own streams, and a worker
+// enqueues and waits without the server lock, so requests that arrive while
+// others are on the device are admitted as slots and memory free up, and
+// their work overlaps.  A request's pairs are a contiguous slice of its batch's output;
 // their query indices are rebased to the request on the device, so the result
 // is the same alone and in any mix.  Results stay in the pool until the
 // client releases them.  With batching off each request runs alone: the plain
@@ -190,19 +191,18 @@ struct Server {
   Prof prof;  // never enabled
   std::mutex mu;
   std::condition_variable cv;
-  std::thread th;
+  std::vector<std::thread> workers;  // one per slot
   bool stop = false;
   int64_t next_id = 1;
   std::vector<double> lat_us;
   int64_t pairs_done = 0, batches = 0, requests_done = 0;
-  double launch_us = 0;  // dispatcher time spent enqueuing device work
+  double launch_us = 0;  // worker time spent enqueuing device work (summed over slots)
   uint64_t pool_gen = 0;
   double t_first = 0, t_last = 0;
 
   int start();
-  void loop();
-  bool step_slot(Slot& sl);
-  bool launch_one();
+  void worker(Slot& sl);
+  bool take_batch(Slot& sl);
   int begin_batch(Slot& sl);
   int try_pass(Slot& sl, bool exact);
   int enqueue_pass(Slot& sl);
@@ -235,7 +235,7 @@ int Server::start() {
   CT_HIP(hipMalloc((void**)&pool_tid, (size_t)pairs * 4));
   pool.init(pairs);
   reserve = std::max<int64_t>(pairs / (2 * (int64_t)opt.slots), 1);
-  th = std::thread([this] { loop(); });
+  for (auto& sl : slots) workers.emplace_back([this, &sl] { worker(sl); });
   return CURTREE_OK;
 }
 
@@ -439,51 +439,14 @@ int Server::after_pass(Slot& sl) {
   return CURTREE_OK;
 }
 
-// Advance one slot if its pending step finished.  Returns true on progress.
-bool Server::step_slot(Slot& sl) {
-  int rc = CURTREE_OK;
-  if (sl.todo != Slot::kNone) return false;  // not enqueued yet
-  switch (sl.state) {
-    case kIdle:
-      return false;
-    case kPass1:
-    case kPass2: {
-      hipError_t e = hipEventQuery(sl.ev);
-      if (e == hipErrorNotReady) return false;
-      if (e != hipSuccess)
-        rc = set_error(CURTREE_EHIP, "serve: %s", hipGetErrorString(e));
-      else
-        rc = after_pass(sl);
-      break;
-    }
-    case kWaitPool1:
-    case kWaitPool2: {
-      const SlotState was = sl.state;
-      rc = try_pass(sl, was == kWaitPool2);
-      if (!rc && sl.state == was) return false;
-      break;
-    }
-  }
-  if (rc) finish(sl, rc);
-  return true;
-}
-
 // Admit the oldest request (and, with batching, every queued request on the
-// same tree that fits) onto a free slot.  Returns true if something started.
-bool Server::launch_one() {
+// same tree that fits) onto free slot `sl`.  Returns true if a batch started.
+bool Server::take_batch(Slot& sl) {
   if (queue.empty()) return false;
   // a batch waiting for result memory goes first: no new batch may take the
   // space that releases free up
-  for (auto& sl : slots)
-    if (sl.state == kWaitPool1 || sl.state == kWaitPool2) return false;
-  Slot* free_slot = nullptr;
-  for (auto& sl : slots)
-    if (sl.state == kIdle) {
-      free_slot = &sl;
-      break;
-    }
-  if (!free_slot) return false;
-  Slot& sl = *free_slot;
+  for (auto& o : slots)
+    if (o.state == kWaitPool1 || o.state == kWaitPool2) return false;
   const int tree = reqs[queue.front()].tree;
   int64_t nwin = 0;
   sl.reqs.clear();
@@ -509,41 +472,37 @@ bool Server::launch_one() {
   return true;
 }
 
-void Server::loop() {
+// One worker thread per slot: it takes a batch when its slot is free, does
+// the slot's device work outside the server lock and waits on the slot's own
+// event, so the slots' launch work and waits proceed in parallel.
+void Server::worker(Slot& sl) {
   std::unique_lock<std::mutex> lk(mu);
   while (!stop) {
-    bool progressed = false;
-    for (auto& sl : slots) progressed |= step_slot(sl);
-    while (launch_one()) progressed = true;
-    // device work outside the lock: clients are not held up by launches
-    bool any = false;
-    for (auto& sl : slots) any |= sl.todo != Slot::kNone;
-    if (any) {
-      std::vector<std::pair<Slot*, int>> failed;
+    if (sl.state == kIdle) {
+      if (!take_batch(sl)) cv.wait_for(lk, std::chrono::milliseconds(5));
+      continue;
+    }
+    int rc = CURTREE_OK;
+    if (sl.todo != Slot::kNone) {
       lk.unlock();
       const double t0 = now_us();
-      for (auto& sl : slots)
-        if (sl.todo != Slot::kNone) {
-          const int rc = run_todo(sl);
-          if (rc) failed.push_back({&sl, rc});
-        }
+      rc = run_todo(sl);
       const double dt = now_us() - t0;
       lk.lock();
       launch_us += dt;
-      for (auto& f : failed) finish(*f.first, f.second);
-      progressed = true;
+    } else if (sl.state == kPass1 || sl.state == kPass2) {
+      lk.unlock();
+      const hipError_t e = hipEventSynchronize(sl.ev);
+      lk.lock();
+      rc = e != hipSuccess ? set_error(CURTREE_EHIP, "serve: %s", hipGetErrorString(e))
+                           : after_pass(sl);
+    } else {  // waiting for result memory
+      const SlotState was = sl.state;
+      rc = try_pass(sl, was == kWaitPool2);
+      if (!rc && sl.state == was) cv.wait_for(lk, std::chrono::milliseconds(1));
     }
-    bool busy = false;
-    for (auto& sl : slots) busy |= sl.state != kIdle;
-    if (!progressed) {
-      if (busy) {
-        lk.unlock();
-        std::this_thread::sleep_for(std::chrono::microseconds(10));
-        lk.lock();
-      } else {
-        cv.wait_for(lk, std::chrono::milliseconds(5));
-      }
-    }
+    if (rc) finish(sl, rc);
+    if (sl.state == kIdle) cv.notify_all();  // released pool space, a free slot
   }
 }
 
@@ -628,7 +587,7 @@ CURTREE_API int curtree_server_submit(curtree_server* s, int tree_id,
   sv->reqs[r.id] = r;
   sv->queue.push_back(r.id);
   *request_id = r.id;
-  sv->cv.notify_one();
+  sv->cv.notify_all();
   return CURTREE_OK;
 }
 
@@ -678,7 +637,7 @@ CURTREE_API int curtree_server_release(curtree_server* s, int64_t request_id) {
     sv->pool_gen++;
   }
   sv->reqs.erase(it);
-  sv->cv.notify_one();
+  sv->cv.notify_all();
   return CURTREE_OK;
 }
 
@@ -722,8 +681,9 @@ CURTREE_API int curtree_server_destroy(curtree_server* s) {
     std::lock_guard<std::mutex> g(sv->mu);
     sv->stop = true;
   }
-  sv->cv.notify_one();
-  if (sv->th.joinable()) sv->th.join();
+  sv->cv.notify_all();
+  for (auto& w : sv->workers)
+    if (w.joinable()) w.join();
   (void)hipDeviceSynchronize();
   for (auto& sl : sv->slots) {
     if (sl.s) (void)hipStreamDestroy(sl.s);

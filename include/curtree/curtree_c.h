@@ -171,7 +171,7 @@ typedef struct curtree_server_stats {
   int64_t requests_done, pairs_done, batches, queued;
   double p50_us, p99_us; /* submit -> completion latency */
   double span_us;        /* first submit -> last completion */
-  double launch_us;      /* dispatcher time spent enqueuing device work */
+  double launch_us;      /* time spent enqueuing device work, summed over slots */
 } curtree_server_stats;
 CURTREE_API int curtree_server_create(const curtree_server_opts* opts, curtree_server** out);
 CURTREE_API int curtree_server_add_tree(curtree_server* s, const curtree_rtree_f32* rt,
