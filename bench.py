@@ -23,12 +23,16 @@ Other modes (same JSON contract, one line):
                   road-like segments); Poisson request arrivals, each request
                   on a random tree with 1..65,536 windows (log-uniform count),
                   side log-uniform [1e-5, 2e-3]; 16 GB result budget.  The
-                  request rate is swept for plain per-request launches (one
-                  request at a time) and for the serving runtime (continuous
-                  batching over 4 streams); value = served pairs/s at the plain
-                  mode's p99 latency (at its highest sustained rate).
+                  request rate is swept for plain per-request launches (the
+                  plain entry point, one request at a time; and the server
+                  with batching off on one stream) and for the serving runtime
+                  (continuous batching over 4 streams); value = served pairs/s
+                  at the plain entry point's p99 latency (at its highest
+                  sustained rate); the comparison with the unbatched server is
+                  reported beside it.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -190,6 +194,62 @@ def _poisson_run(ct, sv, tree_ids, pool, npool, lam, duration, seed, drain_s=20.
             "kept_up": bool(done == offered and span <= duration * 1.2 + 0.5)}
 
 
+class _Win:
+    """A window view in the shape the query wrappers take (`.s`)."""
+
+    def __init__(self, s):
+        self.s = s
+
+
+def _plain_entry_run(ct, trees, pool, npool, lam, duration, seed, buf, cap, drain_s=20.0):
+    """The same Poisson request stream served by the plain entry point, one
+    request at a time (curtree_query_into into a preallocated buffer),
+    latency from the arrival time to the call's return."""
+    r = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    next_t = t0 + r.exponential(1.0 / lam)
+    lat, pairs, offered, t_end = [], 0, 0, t0
+    while next_t - t0 < duration:
+        while time.perf_counter() < next_t:
+            pass
+        k = int(np.exp(r.uniform(0, np.log(65536))))
+        start = int(r.integers(0, npool - k))
+        rt, d, _ = trees[int(r.integers(0, len(trees)))]
+        v = _Win(ct.window_view(pool, start, k))
+        got = ct.query_into(rt, d, v, buf[0], buf[1], cap)
+        if got < 0:  # more pairs than the buffer: an allocating query
+            p = ct.query(rt, d, v)
+            got = p.n
+            p.free()
+        t = time.perf_counter()
+        lat.append(t - next_t)
+        pairs += got
+        offered += 1
+        t_end = t
+        next_t += r.exponential(1.0 / lam)
+        if t - t0 > duration + drain_s:
+            break
+    lat_ms = np.array(lat) * 1e3 if lat else np.array([np.inf])
+    span = max(t_end - t0, 1e-9)
+    return {"rate": lam, "offered": offered, "done": len(lat), "pairs": int(pairs),
+            "pairs_per_s": pairs / span, "p50_ms": float(np.percentile(lat_ms, 50)),
+            "p99_ms": float(np.percentile(lat_ms, 99)),
+            "kept_up": bool(span <= duration * 1.2 + 0.5 and next_t - t0 >= duration)}
+
+
+def _knee(points):
+    """Highest rate that kept up without a queue building (median latency
+    within 3x of the lightest load's)."""
+    p50_0 = points[0]["p50_ms"]
+    ok = [p for p in points if p["kept_up"] and p["p50_ms"] <= 3 * p50_0]
+    return (ok or points[:1])[-1]
+
+
+def _at_p99(points, p99):
+    ok = [p for p in points if p["kept_up"] and p["p99_ms"] <= p99]
+    return max(ok, key=lambda p: p["pairs_per_s"]) if ok else None
+
+
 def bench_serve(args, ct, datagen):
     scale = args.serve_scale
     sizes = [int(x * scale) for x in (1e7, 2e7, 4e7, 6e7, 8e7, 1.2e8, 1.6e8, 2e8)]
@@ -234,14 +294,33 @@ def bench_serve(args, ct, datagen):
             lam *= args.serve_rate_step
         sv.close()
         res[name] = pts
-    # the plain mode's knee: its highest rate that kept up without a queue
-    # building (median latency within 3x of the lightest load's)
-    p50_0 = res["plain"][0]["p50_ms"]
-    ok_plain = [p for p in res["plain"] if p["kept_up"] and p["p50_ms"] <= 3 * p50_0]
-    ref = (ok_plain or res["plain"][:1])[-1]
-    ok_served = [p for p in res["served"] if p["kept_up"] and p["p99_ms"] <= ref["p99_ms"]]
-    best = max(ok_served, key=lambda p: p["pairs_per_s"]) if ok_served else None
-    value = best["pairs_per_s"] if best else 0.0
+    # the plain entry point per request (what a caller does without the server)
+    cap = 256 << 20
+    fb, tb = ctypes.c_void_p(), ctypes.c_void_p()
+    ct._check(ct.lib.curtree_malloc(ctypes.byref(fb), 4 * cap))
+    ct._check(ct.lib.curtree_malloc(ctypes.byref(tb), 4 * cap))
+    pts = []
+    _plain_entry_run(ct, trees, pool, npool, 200.0, 0.5, 1, (fb.value, tb.value), cap)
+    lam = args.serve_rate0 / 4
+    while lam <= args.serve_rate_max:
+        p = _plain_entry_run(ct, trees, pool, npool, lam, args.serve_seconds, int(lam),
+                             (fb.value, tb.value), cap)
+        pts.append(p)
+        print(f"[bench] plain entry point {lam:.0f} req/s: {p['pairs_per_s'] / 1e9:.2f} Gpairs/s, "
+              f"p50 {p['p50_ms']:.2f} ms, p99 {p['p99_ms']:.2f} ms, kept up {p['kept_up']}",
+              file=sys.stderr, flush=True)
+        if not p["kept_up"]:
+            break
+        lam *= args.serve_rate_step
+    ct.lib.curtree_free(fb)
+    ct.lib.curtree_free(tb)
+    res["plain_entry"] = pts
+    # equal-p99 comparisons: served pairs/s at each baseline's knee p99
+    ref = _knee(res["plain"])
+    ref_e = _knee(res["plain_entry"])
+    best = _at_p99(res["served"], ref["p99_ms"])
+    best_e = _at_p99(res["served"], ref_e["p99_ms"])
+    value = best_e["pairs_per_s"] if best_e else 0.0
     out = {
         "metric": METRIC, "value": value, "unit": "pairs/s", "n_gpus": 1,
         "steps": len(res["plain"]) + len(res["served"]), "warmup": 1,
@@ -253,10 +332,21 @@ def bench_serve(args, ct, datagen):
                                "1..65536 windows per request (log-uniform), side log-uniform "
                                f"[1e-5,2e-3], {args.serve_budget_gb} GB result budget",
                    "trees": sizes, "fanout": args.fanout},
-        "serve": {"plain": res["plain"], "served": res["served"], "plain_knee_rate": ref["rate"],
-                  "p99_ref_ms": ref["p99_ms"], "plain_pairs_per_s": ref["pairs_per_s"],
-                  "served_pairs_per_s_at_equal_p99": value,
-                  "ratio": value / ref["pairs_per_s"] if ref["pairs_per_s"] else None},
+        "serve": {"plain": res["plain"], "served": res["served"],
+                  "plain_entry": res["plain_entry"],
+                  "vs_plain_entry_point": {
+                      "baseline": "curtree_query_into per request, one at a time",
+                      "knee_rate": ref_e["rate"], "p99_ref_ms": ref_e["p99_ms"],
+                      "plain_pairs_per_s": ref_e["pairs_per_s"],
+                      "served_pairs_per_s_at_equal_p99": value,
+                      "ratio": value / ref_e["pairs_per_s"] if ref_e["pairs_per_s"] else None},
+                  "vs_unbatched_server": {
+                      "baseline": "the server with batching off and one slot",
+                      "knee_rate": ref["rate"], "p99_ref_ms": ref["p99_ms"],
+                      "plain_pairs_per_s": ref["pairs_per_s"],
+                      "served_pairs_per_s_at_equal_p99": best["pairs_per_s"] if best else 0.0,
+                      "ratio": (best["pairs_per_s"] / ref["pairs_per_s"])
+                               if best and ref["pairs_per_s"] else None}},
     }
     print(json.dumps(out), flush=True)
 
@@ -274,15 +364,15 @@ def main():
     ap.add_argument("--serve-scale", type=float, default=1.0,
                     help="serve mode: tree sizes relative to 1e7..2e8 boxes")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
-    ap.add_argument("--serve-rate0", type=float, default=250.0)
+    ap.add_argument("--serve-rate0", type=float, default=2000.0)
     ap.add_argument("--serve-rate-max", type=float, default=64000.0)
-    ap.add_argument("--serve-rate-step", type=float, default=1.5)
+    ap.add_argument("--serve-rate-step", type=float, default=1.2)
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-batch-windows", type=int, default=1 << 20,
                     help="serve mode: window cap of one batch (and of one request)")
     ap.add_argument("--serve-budget-gb", type=int, default=16)
     ap.add_argument("--fanout", type=int, default=None,
-                    help="default: 16 (plain, serve), the tuned one (batched)")
+                    help="default: 16 (plain), 32 (serve), the tuned one (batched)")
     ap.add_argument("--no-tuning", action="store_true",
                     help="batched mode: library defaults instead of curtree_amd/tuning.json")
     ap.add_argument("--key", choices=["morton64", "ref32"], default="morton64")
@@ -310,8 +400,10 @@ def main():
         ct.set_option("query_grid", args.query_grid)
     args.boxes_set = args.boxes is not None
     args.windows_set = args.windows is not None
-    if args.mode != "batched" and args.fanout is None:
+    if args.mode == "plain" and args.fanout is None:
         args.fanout = 16
+    if args.mode == "serve" and args.fanout is None:
+        args.fanout = 32
     if args.mode == "batched":
         return bench_batched(args, ct, datagen)
     if args.mode == "serve":
@@ -358,7 +450,6 @@ def main():
     # level may hold fewer than F; the difference is below 1e-6 here)
     T = sum(ref_stats["entries"]) * F
     q_bytes = 16 * nq + 8 * X + 16 * T + 12 * H
-    import ctypes
     fbuf, tbuf = ctypes.c_void_p(), ctypes.c_void_p()
     ct._check(ct.lib.curtree_malloc(ctypes.byref(fbuf), 4 * max(H, 1)))
     ct._check(ct.lib.curtree_malloc(ctypes.byref(tbuf), 4 * max(H, 1)))

@@ -568,9 +568,16 @@ __global__ __launch_bounds__(256) void k_write_pieces(const QCtrl* c, const QPie
 // node; NaN counts as not well-formed).  Leaf-parents from their boxes,
 // upper levels as the AND of their children.
 template <int G>
-__global__ __launch_bounds__(256) void k_safe_leaf_pow2(int64_t nbox, QBoxes rb, uint8_t* safe0) {
+// (box4 non-null: the AoS copy of the boxes is written in the same pass)
+__global__ __launch_bounds__(256) void k_safe_leaf_pow2(int64_t nbox, QBoxes rb, uint8_t* safe0,
+                                                        float4* __restrict__ box4) {
   const int64_t i = blockIdx.x * 256ll + threadIdx.x;
-  const bool wf = i >= nbox || (rb.x0[i] <= rb.x1[i] && rb.y0[i] <= rb.y1[i]);
+  float4 b = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (i < nbox) {
+    b = make_float4(rb.x0[i], rb.y0[i], rb.x1[i], rb.y1[i]);
+    if (box4) box4[i] = b;
+  }
+  const bool wf = i >= nbox || (b.x <= b.z && b.y <= b.w);
   const uint64_t bad = ballot(!wf);
   const int lane = lane_id();
   const uint64_t gm = G == 64 ? ~0ull : ((1ull << G) - 1ull);
@@ -745,7 +752,7 @@ __global__ __launch_bounds__(256) void k_stage4(int64_t n, QBoxes src, float4* _
     dst[i] = make_float4(src.x0[i], src.y0[i], src.x1[i], src.y1[i]);
 }
 
-int stage_boxes(float4** dst, int64_t* cap, const QBoxes& src, int64_t n, hipStream_t s) {
+int reserve_f4(float4** dst, int64_t* cap, int64_t n) {
   if (n > *cap) {
     if (*dst) (void)hipFree(*dst);
     *dst = nullptr;
@@ -758,6 +765,12 @@ int stage_boxes(float4** dst, int64_t* cap, const QBoxes& src, int64_t n, hipStr
     }
     *cap = want;
   }
+  return CURTREE_OK;
+}
+
+int stage_boxes(float4** dst, int64_t* cap, const QBoxes& src, int64_t n, hipStream_t s) {
+  int rc;
+  if ((rc = reserve_f4(dst, cap, n))) return rc;
   if (n > 0) {
     const unsigned grid = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
     hipLaunchKernelGGL(k_stage4, dim3(grid), dim3(256), 0, s, n, src, *dst);
@@ -767,8 +780,9 @@ int stage_boxes(float4** dst, int64_t* cap, const QBoxes& src, int64_t n, hipStr
 }
 
 int compute_safe(QEngine& qe, Prof& pf, const QBoxes& rb, int F, int64_t r_sz, int64_t nbox,
-                 hipStream_t s) {
+                 hipStream_t s, float4* box4, bool* staged) {
   int rc;
+  if (staged) *staged = false;
   if ((rc = qe.reserve_safe(r_sz))) return rc;
   uint8_t* safe = qe.safe;
   LevelPlan lp;
@@ -779,7 +793,8 @@ int compute_safe(QEngine& qe, Prof& pf, const QBoxes& rb, int F, int64_t r_sz, i
 #define CT_SAFE(g)                                                                        \
   case g:                                                                                 \
     hipLaunchKernelGGL(k_safe_leaf_pow2<g>, dim3((unsigned)((nbox + 255) / 256)), dim3(256), \
-                       0, s, nbox, rb, safe + start);                                     \
+                       0, s, nbox, rb, safe + start, box4);                               \
+    if (staged) *staged = box4 != nullptr;                                                \
     break;
     CT_SAFE(2) CT_SAFE(4) CT_SAFE(8) CT_SAFE(16) CT_SAFE(32) CT_SAFE(64)
 #undef CT_SAFE
@@ -945,18 +960,21 @@ int prepare_query(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curt
   }
   // the flags cost one pass over the boxes; worth it for larger batches
   const bool use = contain == 1 || (contain < 0 && nq_hint >= 256);
-  if (use) {
-    if ((rc = compute_safe(qe, pf, a->rb, F, lp.r_sz, N, s))) return rc;
-    a->tr.safe = qe.safe;
-  }
   // AoS staging costs a read and a write of every box: worth it when the
   // batch is large against the tree (callers that keep trees resident
   // (contain == 0 here) stage once themselves)
   const int st = qtuning().staging;
   const bool stage = st == 1 || (st < 0 && contain != 0 && nq_hint >= 256 && nq_hint * 256 >= N);
+  bool staged = false;
+  if (stage && (rc = reserve_f4(&qe.box4, &qe.cap_box4, N))) return rc;
+  if (use) {  // the flag pass also writes the AoS boxes when both are wanted
+    if ((rc = compute_safe(qe, pf, a->rb, F, lp.r_sz, N, s, stage ? qe.box4 : nullptr, &staged)))
+      return rc;
+    a->tr.safe = qe.safe;
+  }
   if (stage) {
     pf.mark(s, "k_stage4");
-    if ((rc = stage_boxes(&qe.box4, &qe.cap_box4, a->rb, N, s)) ||
+    if ((!staged && (rc = stage_boxes(&qe.box4, &qe.cap_box4, a->rb, N, s))) ||
         (rc = stage_boxes(&qe.node4, &qe.cap_node4, a->tr.mbr, lp.r_sz, s)) ||
         (rc = stage_boxes(&qe.win4, &qe.cap_win4, a->qb, Q, s)))
       return rc;

@@ -108,8 +108,11 @@ int per_tile(int G, int steps);
 int prepare_query(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox,
                   const curtree_bbox_f32* qbox, int contain, int64_t nq_hint, QRunArgs* a,
                   hipStream_t s);
+// Well-formedness flags of every node; with box4 (power-of-two F) the same
+// pass writes the AoS copy of the boxes and sets *staged.
 int compute_safe(QEngine& qe, Prof& pf, const QBoxes& rb, int F, int64_t r_sz, int64_t nbox,
-                 hipStream_t s);
+                 hipStream_t s, float4* box4 = nullptr, bool* staged = nullptr);
+int reserve_f4(float4** dst, int64_t* cap, int64_t n);
 // Checks that rt's pos/len arrays hold the bulk-load layout; the result lands
 // in the device word *flag (allocated on first use), no host synchronisation.
 int check_layout(uint32_t** flag, const curtree_rtree_f32* rt, int64_t nbox, hipStream_t s);
