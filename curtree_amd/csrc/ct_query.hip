@@ -328,6 +328,7 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_count(
   // bulk-load layout: leaf-parent i holds sorted boxes i*F .. i*F+F-1
   const bool canon = tr.canon && *tr.canon;
   const int64_t ls = lv->lstart[0];
+  uint64_t block_total = 0;  // thread 0: one atomic per block, not per tile
   int64_t tile;
   while ((tile = next_tile(&c->ticket[slot], ntiles, &s_tile)) >= 0) {
     uint64_t wsum = 0;
@@ -392,9 +393,11 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_count(
 #pragma unroll
       for (int i = 0; i < kQWaves; i++) tot += s_red[i];
       if (STORE) ttot[tile] = tot;
-      if (tot) atomicAdd((unsigned long long*)&c->total, (unsigned long long)tot);
+      block_total += tot;
     }
   }
+  if (threadIdx.x == 0 && block_total)
+    atomicAdd((unsigned long long*)&c->total, (unsigned long long)block_total);
 }
 
 // Exclusive scan of the leaf tile counts in place (tiles of P entries).
