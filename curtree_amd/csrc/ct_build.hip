@@ -165,59 +165,106 @@ __device__ __forceinline__ void axis_params(uint32_t lo_o, uint32_t hi_o, float*
   *s = (w > 0.0f) ? 4294967296.0f / w : 0.0f;
 }
 
-// Key generation + AoS staging copy + all-digit histograms + status zeroing.
-// REF32: key = z_order((ushort)cx, (ushort)cy) as in mbr2key
+// ---------------------------------------------------------------------------
+// Keys.  REF32: key = z_order((ushort)cx, (ushort)cy) as in mbr2key
 // (src/rtree_bulkload_thrust.cuh:52-60), stored with the sign bit flipped so
 // that an unsigned radix sort yields the reference's signed int32 order.
+// MORTON64: 32-bit quantised centroids over the centroid extent, interleaved.
+// The payload sort runs on a 32-bit key: REF32's key, or the high half of the
+// Morton key (ties of the high half are settled on the low half afterwards).
+struct KeyParams {
+  float lox, sx, loy, sy;
+};
+
+// Block-wide: every thread of the block calls it.
 template <int MODE>
-__global__ __launch_bounds__(256) void k_keygen(int64_t n, const float* __restrict__ xmin,
-                                                const float* __restrict__ ymin,
-                                                const float* __restrict__ xmax,
-                                                const float* __restrict__ ymax,
-                                                const int* __restrict__ id, BuildCtrl* c,
-                                                void* __restrict__ keys_out,
-                                                uint4* __restrict__ aos,
-                                                uint32_t* __restrict__ zero_ptr,
-                                                int64_t zero_words) {
-  constexpr int D = MODE == kKeyRef32 ? 4 : 8;
-  __shared__ uint32_t sh[D * 256];
-  for (int t = threadIdx.x; t < D * 256; t += 256) sh[t] = 0;
-  float lox = 0.f, loy = 0.f, sx = 0.f, sy = 0.f;
+__device__ __forceinline__ KeyParams key_params(const BuildCtrl* c) {
+  KeyParams p{0.f, 0.f, 0.f, 0.f};
   if (MODE == kKeyMorton64) {
     uint32_t e4[4];
     read_extent(c, e4);
-    axis_params(e4[0], e4[2], &lox, &sx);
-    axis_params(e4[1], e4[3], &loy, &sy);
+    axis_params(e4[0], e4[2], &p.lox, &p.sx);
+    axis_params(e4[1], e4[3], &p.loy, &p.sy);
   }
+  return p;
+}
+
+__device__ __forceinline__ uint64_t morton64(float a, float b, float cc, float d,
+                                             const KeyParams& p) {
+  const float cx = (a + cc) * 0.5f;
+  const float cy = (b + d) * 0.5f;
+  return (spread32(q32(cy, p.loy, p.sy)) << 1) | spread32(q32(cx, p.lox, p.sx));
+}
+
+__device__ __forceinline__ uint32_t ref32(float a, float b, float cc, float d) {
+  const float cx = (a + cc) / 2.0f;
+  const float cy = (b + d) / 2.0f;
+  return ((spread16(f2u16_sat(cy)) << 1) | spread16(f2u16_sat(cx))) ^ 0x80000000u;
+}
+
+template <int MODE>
+__device__ __forceinline__ uint32_t sort_key32(float a, float b, float cc, float d,
+                                               const KeyParams& p) {
+  if (MODE == kKeyRef32) return ref32(a, b, cc, d);
+  return (uint32_t)(morton64(a, b, cc, d, p) >> 32);
+}
+
+// Histograms of the four digits of the 32-bit sort key, and the status words
+// of the sort steps cleared.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_hist(int64_t n, const float* __restrict__ xmin,
+                                              const float* __restrict__ ymin,
+                                              const float* __restrict__ xmax,
+                                              const float* __restrict__ ymax, BuildCtrl* c,
+                                              uint32_t* __restrict__ zero_ptr,
+                                              int64_t zero_words) {
+  __shared__ uint32_t sh[4 * 256];
+  for (int t = threadIdx.x; t < 4 * 256; t += 256) sh[t] = 0;
+  const KeyParams kp = key_params<MODE>(c);
   __syncthreads();
   const int64_t stride = (int64_t)gridDim.x * 256;
   for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += stride) {
-    float a = xmin[i], b = ymin[i], cc = xmax[i], d = ymax[i];
-    int ident = id[i];
-    aos[2 * i] = make_uint4(__float_as_uint(a), __float_as_uint(b), __float_as_uint(cc),
-                            __float_as_uint(d));
-    aos[2 * i + 1] = make_uint4((uint32_t)ident, 0u, 0u, 0u);
-    if (MODE == kKeyRef32) {
-      float cx = (a + cc) / 2.0f;
-      float cy = (b + d) / 2.0f;
-      uint32_t z = (spread16(f2u16_sat(cy)) << 1) | spread16(f2u16_sat(cx));
-      uint32_t k = z ^ 0x80000000u;
-      ((uint32_t*)keys_out)[i] = k;
+    const uint32_t k = sort_key32<MODE>(xmin[i], ymin[i], xmax[i], ymax[i], kp);
 #pragma unroll
-      for (int p = 0; p < 4; p++) atomicAdd(&sh[p * 256 + ((k >> (8 * p)) & 255u)], 1u);
-    } else {
-      float cx = (a + cc) * 0.5f;
-      float cy = (b + d) * 0.5f;
-      uint64_t k = (spread32(q32(cy, loy, sy)) << 1) | spread32(q32(cx, lox, sx));
-      ((uint64_t*)keys_out)[i] = k;
-#pragma unroll
-      for (int p = 0; p < 8; p++)
-        atomicAdd(&sh[p * 256 + (uint32_t)((k >> (8 * p)) & 255u)], 1u);
-    }
+    for (int p = 0; p < 4; p++) atomicAdd(&sh[p * 256 + ((k >> (8 * p)) & 255u)], 1u);
   }
   for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < zero_words; i += stride) zero_ptr[i] = 0u;
   __syncthreads();
-  for (int t = threadIdx.x; t < D * 256; t += 256)
+  for (int t = threadIdx.x; t < 4 * 256; t += 256)
+    if (sh[t]) atomicAdd(&c->hist[t], sh[t]);
+}
+
+// 64-bit keys + AoS staging copy + all-digit histograms: the input of the
+// full 8-pass LSD sort, which runs only when the high-half sort leaves a run
+// of equal high halves too long to settle in place (`gate`: skip unless
+// c->fallback; the keys go to the buffer the next step reads).
+__global__ __launch_bounds__(256) void k_keygen64(int64_t n, const float* __restrict__ xmin,
+                                                  const float* __restrict__ ymin,
+                                                  const float* __restrict__ xmax,
+                                                  const float* __restrict__ ymax,
+                                                  const int* __restrict__ id, BuildCtrl* c,
+                                                  uint64_t* __restrict__ k0,
+                                                  uint64_t* __restrict__ k1,
+                                                  uint4* __restrict__ aos, int gate) {
+  __shared__ uint32_t sh[8 * 256];
+  if (gate && !c->fallback) return;
+  for (int t = threadIdx.x; t < 8 * 256; t += 256) sh[t] = 0;
+  const KeyParams kp = key_params<kKeyMorton64>(c);
+  uint64_t* keys_out = (c->npass & 1) ? k1 : k0;
+  __syncthreads();
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += stride) {
+    const float a = xmin[i], b = ymin[i], cc = xmax[i], d = ymax[i];
+    aos[2 * i] = make_uint4(__float_as_uint(a), __float_as_uint(b), __float_as_uint(cc),
+                            __float_as_uint(d));
+    aos[2 * i + 1] = make_uint4((uint32_t)id[i], 0u, 0u, 0u);
+    const uint64_t k = morton64(a, b, cc, d, kp);
+    keys_out[i] = k;
+#pragma unroll
+    for (int p = 0; p < 8; p++) atomicAdd(&sh[p * 256 + (uint32_t)((k >> (8 * p)) & 255u)], 1u);
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < 8 * 256; t += 256)
     if (sh[t]) atomicAdd(&c->hist[t], sh[t]);
 }
 
@@ -248,10 +295,14 @@ __device__ void plan_steps(BuildCtrl* c, int s0, int ns, uint64_t digits, bool n
   c->first = first;
 }
 
-__global__ __launch_bounds__(256) void k_sort_plan(int64_t n, int D, BuildCtrl* c, int ns,
-                                                   uint64_t digits, int keep_last_keys) {
+// Digit offsets for digits 0..D-1 from the histograms and the plan of steps
+// [s0, s0+ns).  `gate`: plan only if the fallback sort was asked for (its
+// first step then reads input positions as values again).
+__global__ __launch_bounds__(256) void k_sort_plan(int64_t n, int D, BuildCtrl* c, int s0, int ns,
+                                                   uint64_t digits, int keep_last_keys, int gate) {
   __shared__ uint32_t s_tmp[4];
   __shared__ uint32_t s_triv;
+  if (gate && !c->fallback) return;
   if (threadIdx.x == 0) s_triv = 0;
   __syncthreads();
   for (int p = 0; p < D; p++) {
@@ -265,10 +316,12 @@ __global__ __launch_bounds__(256) void k_sort_plan(int64_t n, int D, BuildCtrl* 
   __syncthreads();
   if (threadIdx.x == 0) {
     c->trivial = s_triv;
-    plan_steps(c, 0, ns, digits, keep_last_keys != 0);
+    if (gate) c->first = -1;
+    // (the fallback's gather reads the values its steps write: one must run)
+    plan_steps(c, s0, ns, digits, keep_last_keys != 0 || gate != 0);
     // the last step writes values only, unless a later stage reads the keys
     if (!keep_last_keys)
-      for (int st = ns - 1; st >= 0; st--)
+      for (int st = s0 + ns - 1; st >= s0; st--)
         if (c->active[st]) {
           c->write_keys[st] = 0;
           break;
@@ -277,9 +330,9 @@ __global__ __launch_bounds__(256) void k_sort_plan(int64_t n, int D, BuildCtrl* 
 }
 
 // ---------------------------------------------------------------------------
-// One onesweep LSD pass over 8 bits.  Tile = kSortThreads x kSortItems keys,
-// warp-striped (wave w owns keys [w*64*I, (w+1)*64*I) of the tile, item i of
-// lane l is key w*64*I + i*64 + l), so ballot ranking in (item, lane) order is
+// Onesweep LSD passes over 8 bits.  Tile = kSortThreads x kSortItems items,
+// warp-striped (wave w owns items [w*64*I, (w+1)*64*I) of the tile, item i of
+// lane l is item w*64*I + i*64 + l), so ballot ranking in (item, lane) order is
 // the input order and the pass is stable.  Per-digit tile counts are chained
 // across tiles by decoupled look-back; tiles are numbered by an atomic ticket
 // so every tile waits only on tiles that have already started.
@@ -292,9 +345,6 @@ __global__ __launch_bounds__(256) void k_sort_plan(int64_t n, int D, BuildCtrl* 
 #ifndef CT_LOOKBACK
 #define CT_LOOKBACK 4
 #endif
-#ifndef CT_VALS_EARLY
-#define CT_VALS_EARLY 0
-#endif
 constexpr int kSortThreads = CT_SORT_THREADS;  // 256 or 512
 constexpr int kSortWaves = kSortThreads / kWave;
 constexpr int kSortItems = CT_SORT_ITEMS;
@@ -302,67 +352,38 @@ constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr uint32_t kFlagA = 1u << 30, kFlagP = 2u << 30, kValMask = (1u << 30) - 1u;
 static_assert(kSortThreads >= 256, "one thread per digit");
 
-template <class KT>
-__global__ __launch_bounds__(kSortThreads) void k_onesweep(
-    int64_t n, int step, KT* __restrict__ k0, KT* __restrict__ k1, uint32_t* __restrict__ v0,
-    uint32_t* __restrict__ v1, BuildCtrl* c, uint32_t* __restrict__ status, int keys_only) {
-  __shared__ uint32_t s_wh[kSortWaves][256];
-  __shared__ uint32_t s_start[256];
-  __shared__ uint32_t s_adj[256];
-  __shared__ uint32_t s_tmp[kSortWaves];
-  __shared__ uint32_t s_tile;
-  __shared__ KT s_buf[kSortTile];
-  __shared__ uint8_t s_dig[kSortTile];
+struct SweepShared {
+  uint32_t wh[kSortWaves][256];  // per-wave digit counts, then wave offsets
+  uint32_t start[256];           // first tile-local position of each digit
+  uint32_t adj[256];             // output position of tile-local position j: adj[d] + j
+  uint32_t tmp[kSortWaves];
+  uint32_t tile;
+};
 
-  if (!c->active[step]) return;
+// Stable rank of the tile's items by their digits dig[i] and the per-digit
+// look-back.  On return pos[i] is item i's position in the tile reordered by
+// digit, and the item at reordered position j with digit d belongs at output
+// position sh.adj[d] + j.  sh.wh must be zero on entry (and synchronised).
+__device__ __forceinline__ void sweep_rank(const uint32_t (&dig)[kSortItems],
+                                           uint32_t (&pos)[kSortItems], int64_t wbase,
+                                           int64_t n, int64_t tile, const uint32_t* gbase,
+                                           uint32_t* status, SweepShared& sh) {
   const int t = threadIdx.x, w = t >> 6, lane = lane_id();
-  const int pass = c->digit[step];
-  const int shift = 8 * pass;
-  if (t == 0) s_tile = atomicAdd(&c->ticket[step], 1u);
-  for (int i = t; i < kSortWaves * 256; i += kSortThreads) (&s_wh[0][0])[i] = 0;
-  __syncthreads();
-  const int64_t tile = s_tile;
-  const int src = c->src[step];
-  const bool ident = step == c->first;  // first step: values are input positions
-  const KT* kin = src ? k1 : k0;
-  KT* kout = src ? k0 : k1;
-  const uint32_t* vin = src ? v1 : v0;
-  uint32_t* vout = src ? v0 : v1;
-  const bool wk = keys_only || c->write_keys[step] != 0;
-
-  const int64_t tbase = tile * kSortTile;
-  const int64_t wbase = tbase + (int64_t)w * (kWave * kSortItems);
-  KT key[kSortItems];
-  uint32_t pos[kSortItems];  // rank within (wave, digit), later the tile-local position
-#pragma unroll
-  for (int i = 0; i < kSortItems; i++) {
-    int64_t idx = wbase + i * kWave + lane;
-    key[i] = idx < n ? kin[idx] : (KT)0;
-  }
-#if CT_VALS_EARLY
-  // values in flight with the keys (one memory round trip per tile)
-  uint32_t val[kSortItems];
-#pragma unroll
-  for (int i = 0; i < kSortItems; i++) {
-    int64_t idx = wbase + i * kWave + lane;
-    val[i] = (idx < n && !keys_only) ? (ident ? (uint32_t)idx : vin[idx]) : 0u;
-  }
-#endif
   const uint64_t lt = lanemask_lt();
 #pragma unroll
   for (int i = 0; i < kSortItems; i++) {
-    int64_t idx = wbase + i * kWave + lane;
-    bool valid = idx < n;
-    uint32_t d = (uint32_t)(key[i] >> shift) & 255u;
+    const int64_t idx = wbase + i * kWave + lane;
+    const bool valid = idx < n;
+    const uint32_t d = dig[i];
     uint64_t m = ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; b++) {
-      uint64_t bb = ballot((d >> b) & 1u);
+      const uint64_t bb = ballot((d >> b) & 1u);
       m &= ((d >> b) & 1u) ? bb : ~bb;
     }
-    uint32_t old = s_wh[w][d];
+    const uint32_t old = sh.wh[w][d];
     pos[i] = old + (uint32_t)popc64(m & lt);
-    if (valid && (m & lt) == 0) s_wh[w][d] = old + (uint32_t)popc64(m);
+    if (valid && (m & lt) == 0) sh.wh[w][d] = old + (uint32_t)popc64(m);
   }
   __syncthreads();
 
@@ -372,8 +393,8 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
   if (t < 256) {
 #pragma unroll
     for (int i = 0; i < kSortWaves; i++) {
-      const uint32_t ci = s_wh[i][dg];
-      s_wh[i][dg] = tot;
+      const uint32_t ci = sh.wh[i][dg];
+      sh.wh[i][dg] = tot;
       tot += ci;
     }
     uint32_t* st = status + (size_t)tile * 256;
@@ -395,7 +416,7 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
 #pragma unroll
         for (int k = 0; k < kLB; k++) {
           if (stop < kLB) break;
-          uint32_t f = v[k] & ~kValMask;
+          const uint32_t f = v[k] & ~kValMask;
           if (f == 0u) {
             ready = false;
             break;
@@ -413,19 +434,58 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
     }
   }
   uint32_t ttot;
-  const uint32_t start = block_excl_scan<kSortWaves>(tot, s_tmp, &ttot);
+  const uint32_t start = block_excl_scan<kSortWaves>(tot, sh.tmp, &ttot);
   if (t < 256) {
-    s_start[dg] = start;
-    s_adj[dg] = c->base[pass * 256 + dg] + excl - start;
+    sh.start[dg] = start;
+    sh.adj[dg] = gbase[dg] + excl - start;
   }
   __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kSortItems; i++) pos[i] += sh.start[dig[i]] + sh.wh[w][dig[i]];
+}
+
+// One 8-bit pass of (key, u32 value) or keys alone: the full 8-pass LSD sort
+// of 64-bit keys (fallback of the Morton sort, canonical pair order).
+template <class KT>
+__global__ __launch_bounds__(kSortThreads) void k_onesweep(
+    int64_t n, int step, KT* __restrict__ k0, KT* __restrict__ k1, uint32_t* __restrict__ v0,
+    uint32_t* __restrict__ v1, BuildCtrl* c, uint32_t* __restrict__ status, int keys_only) {
+  __shared__ SweepShared sh;
+  __shared__ KT s_buf[kSortTile];
+  __shared__ uint8_t s_dig[kSortTile];
+
+  if (!c->active[step]) return;
+  const int t = threadIdx.x, w = t >> 6, lane = lane_id();
+  const int pass = c->digit[step];
+  const int shift = 8 * pass;
+  if (t == 0) sh.tile = atomicAdd(&c->ticket[step], 1u);
+  for (int i = t; i < kSortWaves * 256; i += kSortThreads) (&sh.wh[0][0])[i] = 0;
+  __syncthreads();
+  const int64_t tile = sh.tile;
+  const int src = c->src[step];
+  const bool ident = step == c->first;  // first step: values are input positions
+  const KT* kin = src ? k1 : k0;
+  KT* kout = src ? k0 : k1;
+  const uint32_t* vin = src ? v1 : v0;
+  uint32_t* vout = src ? v0 : v1;
+  const bool wk = keys_only || c->write_keys[step] != 0;
+
+  const int64_t tbase = tile * kSortTile;
+  const int64_t wbase = tbase + (int64_t)w * (kWave * kSortItems);
+  KT key[kSortItems];
+  uint32_t dig[kSortItems], pos[kSortItems];
+#pragma unroll
+  for (int i = 0; i < kSortItems; i++) {
+    const int64_t idx = wbase + i * kWave + lane;
+    key[i] = idx < n ? kin[idx] : (KT)0;
+    dig[i] = (uint32_t)(key[i] >> shift) & 255u;
+  }
+  sweep_rank(dig, pos, wbase, n, tile, c->base + pass * 256, status, sh);
 
   // local reorder of keys through LDS, then scatter runs of equal digits
 #pragma unroll
   for (int i = 0; i < kSortItems; i++) {
-    int64_t idx = wbase + i * kWave + lane;
-    uint32_t d = (uint32_t)(key[i] >> shift) & 255u;
-    pos[i] += s_start[d] + s_wh[w][d];
+    const int64_t idx = wbase + i * kWave + lane;
     if (idx < n) s_buf[pos[i]] = key[i];
   }
   __syncthreads();
@@ -433,146 +493,246 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
   const int tn = rem < kSortTile ? (int)rem : kSortTile;
 #pragma unroll
   for (int i = 0; i < kSortItems; i++) {
-    int j = t + i * kSortThreads;
+    const int j = t + i * kSortThreads;
     if (j < tn) {
-      KT kk = s_buf[j];
-      uint32_t d = (uint32_t)(kk >> shift) & 255u;
+      const KT kk = s_buf[j];
+      const uint32_t d = (uint32_t)(kk >> shift) & 255u;
       s_dig[j] = (uint8_t)d;
-      if (wk) kout[s_adj[d] + (uint32_t)j] = kk;
+      if (wk) kout[sh.adj[d] + (uint32_t)j] = kk;
     }
   }
   if (keys_only) return;
   // values: reorder through LDS, scatter
-#if !CT_VALS_EARLY
   uint32_t val[kSortItems];
 #pragma unroll
   for (int i = 0; i < kSortItems; i++) {
-    int64_t idx = wbase + i * kWave + lane;
+    const int64_t idx = wbase + i * kWave + lane;
     val[i] = idx < n ? (ident ? (uint32_t)idx : vin[idx]) : 0u;
   }
-#endif
   __syncthreads();
   uint32_t* s_val = (uint32_t*)s_buf;
 #pragma unroll
   for (int i = 0; i < kSortItems; i++) {
-    int64_t idx = wbase + i * kWave + lane;
+    const int64_t idx = wbase + i * kWave + lane;
     if (idx < n) s_val[pos[i]] = val[i];
   }
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < kSortItems; i++) {
-    int j = t + i * kSortThreads;
-    if (j < tn) vout[s_adj[s_dig[j]] + (uint32_t)j] = s_val[j];
+    const int j = t + i * kSortThreads;
+    if (j < tn) vout[sh.adj[s_dig[j]] + (uint32_t)j] = s_val[j];
   }
+}
+
+// One 8-bit pass of the payload sort: the items are (key32 << 32 | id) plus
+// the box as two float2 halves (xmin, ymin) and (xmax, ymax), all moved by
+// every pass, so that the sorted boxes need no gather.  The first pass that
+// runs reads the caller's SoA arrays and computes the keys itself; later
+// passes read the staging buffers the previous pass wrote.  The three 8-B
+// fields are reordered through LDS one after the other, so the tile's LDS
+// buffer stays 8 B per item.
+// Moves the ranked tile of the payload sort: keys, then the two box halves,
+// each reordered through LDS and scattered in runs of equal digits.  FULL:
+// every item of the tile exists (no predicated memory operations, so nothing
+// waits on earlier stores at block joins).
+template <bool FULL>
+__device__ __forceinline__ void sweep_move(
+    const uint64_t (&kv)[kSortItems], const uint32_t (&pos)[kSortItems], int64_t wbase, int64_t n,
+    int tn, int shift, bool first, const float* __restrict__ xmin, const float* __restrict__ ymin,
+    const float* __restrict__ xmax, const float* __restrict__ ymax, const float2* __restrict__ lin,
+    const float2* __restrict__ hin, uint64_t* __restrict__ kout, float2* __restrict__ lout,
+    float2* __restrict__ hout, SweepShared& sh, uint64_t* s_buf, uint8_t* s_dig) {
+  const int t = threadIdx.x, lane = lane_id();
+  // the box halves are loaded first: their loads are then older than every
+  // store of this tile, and waiting for them never waits for a store (the
+  // first pass reads the caller's arrays again: recently read, cache hits)
+  float2 vlo[kSortItems], vhi[kSortItems];
+  if (first) {
+#pragma unroll
+    for (int i = 0; i < kSortItems; i++) {
+      const int64_t idx = wbase + i * kWave + lane;
+      const bool ok = FULL || idx < n;
+      vlo[i] = ok ? make_float2(xmin[idx], ymin[idx]) : make_float2(0.f, 0.f);
+      vhi[i] = ok ? make_float2(xmax[idx], ymax[idx]) : make_float2(0.f, 0.f);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < kSortItems; i++) {
+      const int64_t idx = wbase + i * kWave + lane;
+      const bool ok = FULL || idx < n;
+      vlo[i] = ok ? lin[idx] : make_float2(0.f, 0.f);
+      vhi[i] = ok ? hin[idx] : make_float2(0.f, 0.f);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < kSortItems; i++)
+    if (FULL || wbase + i * kWave + lane < n) s_buf[pos[i]] = kv[i];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kSortItems; i++) {
+    const int j = t + i * kSortThreads;
+    if (FULL || j < tn) {
+      const uint64_t kk = s_buf[j];
+      const uint32_t d = (uint32_t)(kk >> shift) & 255u;
+      s_dig[j] = (uint8_t)d;
+      kout[sh.adj[d] + (uint32_t)j] = kk;
+    }
+  }
+  float2* s_f2 = (float2*)s_buf;
+#pragma unroll
+  for (int half = 0; half < 2; half++) {
+    __syncthreads();  // the previous field's LDS reads are done
+#pragma unroll
+    for (int i = 0; i < kSortItems; i++)
+      if (FULL || wbase + i * kWave + lane < n) s_f2[pos[i]] = half ? vhi[i] : vlo[i];
+    __syncthreads();
+    float2* hdst = half ? hout : lout;
+#pragma unroll
+    for (int i = 0; i < kSortItems; i++) {
+      const int j = t + i * kSortThreads;
+      if (FULL || j < tn) hdst[sh.adj[s_dig[j]] + (uint32_t)j] = s_f2[j];
+    }
+  }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kSortThreads) void k_sweep_kv(
+    int64_t n, int step, const float* __restrict__ xmin, const float* __restrict__ ymin,
+    const float* __restrict__ xmax, const float* __restrict__ ymax, const int* __restrict__ id,
+    uint64_t* __restrict__ kv0, uint64_t* __restrict__ kv1, float2* __restrict__ lo0,
+    float2* __restrict__ lo1, float2* __restrict__ hi0, float2* __restrict__ hi1, BuildCtrl* c,
+    uint32_t* __restrict__ status) {
+  __shared__ SweepShared sh;
+  __shared__ uint64_t s_buf[kSortTile];
+  __shared__ uint8_t s_dig[kSortTile];
+
+  if (!c->active[step]) return;
+  const int t = threadIdx.x, w = t >> 6, lane = lane_id();
+  const int pass = c->digit[step];
+  const int shift = 32 + 8 * pass;
+  const bool first = step == c->first;
+  KeyParams kp{0.f, 0.f, 0.f, 0.f};
+  if (MODE == kKeyMorton64 && first) kp = key_params<MODE>(c);
+  if (t == 0) sh.tile = atomicAdd(&c->ticket[step], 1u);
+  for (int i = t; i < kSortWaves * 256; i += kSortThreads) (&sh.wh[0][0])[i] = 0;
+  __syncthreads();
+  const int64_t tile = sh.tile;
+  const int src = c->src[step];
+  const uint64_t* kin = src ? kv1 : kv0;
+  uint64_t* kout = src ? kv0 : kv1;
+  const float2* lin = src ? lo1 : lo0;
+  float2* lout = src ? lo0 : lo1;
+  const float2* hin = src ? hi1 : hi0;
+  float2* hout = src ? hi0 : hi1;
+
+  const int64_t tbase = tile * kSortTile;
+  const int64_t wbase = tbase + (int64_t)w * (kWave * kSortItems);
+  uint64_t kv[kSortItems];
+  uint32_t dig[kSortItems], pos[kSortItems];
+  // keys (each load loop issues all of its loads before the first use)
+  if (first) {
+    float a[kSortItems], b[kSortItems], cc[kSortItems], d[kSortItems];
+    int ident[kSortItems];
+#pragma unroll
+    for (int i = 0; i < kSortItems; i++) {
+      const int64_t idx = wbase + i * kWave + lane;
+      const bool ok = idx < n;
+      a[i] = ok ? xmin[idx] : 0.f;
+      b[i] = ok ? ymin[idx] : 0.f;
+      cc[i] = ok ? xmax[idx] : 0.f;
+      d[i] = ok ? ymax[idx] : 0.f;
+      ident[i] = ok ? id[idx] : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < kSortItems; i++)
+      kv[i] = ((uint64_t)sort_key32<MODE>(a[i], b[i], cc[i], d[i], kp) << 32) | (uint32_t)ident[i];
+  } else {
+#pragma unroll
+    for (int i = 0; i < kSortItems; i++) {
+      const int64_t idx = wbase + i * kWave + lane;
+      kv[i] = idx < n ? kin[idx] : 0ull;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < kSortItems; i++) dig[i] = (uint32_t)(kv[i] >> shift) & 255u;
+  sweep_rank(dig, pos, wbase, n, tile, c->base + pass * 256, status, sh);
+
+  const int64_t rem = n - tbase;
+  if (rem >= kSortTile)  // full tile: no predicated loads or stores
+    sweep_move<true>(kv, pos, wbase, n, kSortTile, shift, first, xmin, ymin, xmax, ymax, lin, hin,
+                     kout, lout, hout, sh, s_buf, s_dig);
+  else
+    sweep_move<false>(kv, pos, wbase, n, (int)rem, shift, first, xmin, ymin, xmax, ymax, lin, hin,
+                      kout, lout, hout, sh, s_buf, s_dig);
 }
 
 // ---------------------------------------------------------------------------
-// Runs of keys with equal high halves, after the high-half passes: each run is
-// sorted in place by (low half, input position), which makes the whole array
-// sorted by (key, input position) -- the stable full-key order -- without the
-// four low-digit passes.  With n boxes spread over 2^32 high-half cells runs
-// are short.
-//   k_fixup: a block loads a tile (plus a halo of kFixShort) into LDS; the
-// owner thread of each run of 2..kFixShort starting in the tile sorts it by
-// insertion, and the block writes the rewritten positions back (keys and
-// values, so that fallback passes see consistent data).  Longer runs are
-// listed for k_fixup_long (one block per run, bitonic in LDS, up to
-// kFixLong); a longer run, or a full list, sets `fallback` and the full LSD
-// sort runs instead.
+// Ties of the high half (64-bit Morton keys).  After the high-half passes the
+// items are sorted by the high 32 bits, and a stable sort leaves each run of
+// equal high halves in input order.  Sorting every run by (low half, position
+// in the run) then gives exactly the order of a stable sort on the full key,
+// so the four low-digit passes are not needed.  With n boxes spread over 2^32
+// high-half cells runs are short.
+//   Runs of 2..kFixShort are sorted by k_final, in LDS, by every block whose
+// tile they touch (each block writes only its own positions).  Longer runs are
+// listed by the block whose tile holds their start, sorted by k_fixup_long
+// (one block per run, bitonic in LDS, up to kFixLong), and the leaf-parent
+// nodes over them recomputed by k_fixup_groups.  A longer run, or a full list,
+// sets `fallback`: the full 8-pass LSD sort of 64-bit keys then runs from the
+// order reached so far (in which equal full keys are still in input order).
 constexpr int kFixShort = 16;
 constexpr int kFixLong = 4096;
 constexpr int kFixLongSlots = 4096;  // blocks of k_fixup_long (they loop over the list)
+#ifndef CT_FINAL_GRID
+#define CT_FINAL_GRID 4096
+#endif
+constexpr int kFinTile = 2048;       // positions owned by one block of k_final (multiple of 64)
+constexpr int kFinSpan = kFinTile + 2 * kFixShort;
 
-__device__ __forceinline__ uint32_t key_hi(const uint64_t* k, int64_t i) {
-  return (uint32_t)(k[i] >> 32);
-}
-
-constexpr int kFixTile = 2048;  // run starts owned by one block of k_fixup
-constexpr int kFixSpan = kFixTile + kFixShort;
-
-__global__ __launch_bounds__(256) void k_fixup(int64_t n, BuildCtrl* c, uint64_t* k0,
-                                               uint64_t* k1, uint32_t* v0, uint32_t* v1,
-                                               uint32_t* runs, uint32_t runs_cap) {
-  __shared__ uint32_t s_hi[kFixSpan + 1];  // s_hi[j + 1]: high half of position tb + j
-  __shared__ uint64_t s_ck[kFixSpan];      // (low half << 32) | input position
-  __shared__ uint8_t s_dirty[kFixSpan];    // position rewritten by its run's owner
-  const int cur = c->npass & 1;
-  uint64_t* keys = cur ? k1 : k0;
-  uint32_t* vals = cur ? v1 : v0;
-  const int64_t tb = (int64_t)blockIdx.x * kFixTile;
-  const int64_t te = tb + kFixTile < n ? tb + kFixTile : n;
-  const int64_t le = te + kFixShort < n ? te + kFixShort : n;
-  const int nl = (int)(le - tb), nown = (int)(te - tb);
-  const int t = threadIdx.x;
-  for (int j = t; j < nl; j += 256) {
-    const uint64_t k = keys[tb + j];
-    s_hi[j + 1] = (uint32_t)(k >> 32);
-    s_ck[j] = ((k & 0xFFFFFFFFull) << 32) | vals[tb + j];
-    s_dirty[j] = 0;
-  }
-  // the position before the tile (a value unequal to the first for tile 0)
-  if (t == 0) s_hi[0] = tb > 0 ? (uint32_t)(keys[tb - 1] >> 32) : ~(uint32_t)(keys[0] >> 32);
-  __syncthreads();
-  // owner threads: runs of 2..kFixShort starting in the tile, insertion sort
-  for (int j = t; j < nown; j += 256) {
-    const uint32_t h = s_hi[j + 1];
-    if (s_hi[j] == h || j + 1 >= nl || s_hi[j + 2] != h) continue;  // no start of a run >= 2
-    int L = 2;
-    while (L <= kFixShort && j + L < nl && s_hi[j + L + 1] == h) L++;
-    if (L > kFixShort || (j + L == nl && le < n)) {  // long (or reaching past the halo)
-      const uint32_t at = atomicAdd(&c->nruns, 1u);
-      if (at < runs_cap) runs[at] = (uint32_t)(tb + j);
-      else atomicOr(&c->fallback, 1u);
-      continue;
-    }
-    for (int a2 = j + 1; a2 < j + L; a2++) {
-      const uint64_t v = s_ck[a2];
-      int b2 = a2 - 1;
-      while (b2 >= j && s_ck[b2] > v) {
-        s_ck[b2 + 1] = s_ck[b2];
-        b2--;
-      }
-      s_ck[b2 + 1] = v;
-    }
-    for (int q = j; q < j + L; q++) s_dirty[q] = 1;
-  }
-  __syncthreads();
-  // coalesced write-back of the rewritten positions
-  for (int j = t; j < nl; j += 256) {
-    if (!s_dirty[j]) continue;
-    const uint64_t v = s_ck[j];
-    keys[tb + j] = ((uint64_t)s_hi[j + 1] << 32) | (v >> 32);
-    vals[tb + j] = (uint32_t)v;
-  }
-}
-
-__global__ __launch_bounds__(256) void k_fixup_long(int64_t n, BuildCtrl* c, uint64_t* k0,
-                                                    uint64_t* k1, uint32_t* v0, uint32_t* v1,
-                                                    const uint32_t* runs, uint32_t runs_cap) {
+__global__ __launch_bounds__(256) void k_fixup_long(
+    int64_t n, BuildCtrl* c, const uint64_t* __restrict__ kv0, const uint64_t* __restrict__ kv1,
+    const float2* __restrict__ lo0, const float2* __restrict__ lo1, const float2* __restrict__ hi0,
+    const float2* __restrict__ hi1, float* __restrict__ xmin, float* __restrict__ ymin,
+    float* __restrict__ xmax, float* __restrict__ ymax, int* __restrict__ id, uint32_t* runs,
+    uint32_t runs_cap) {
   __shared__ uint64_t s_net[kFixLong];
   __shared__ int s_len;
   const uint32_t nr = c->nruns < runs_cap ? c->nruns : runs_cap;
   if (blockIdx.x >= nr || c->fallback) return;
+  const KeyParams kp = key_params<kKeyMorton64>(c);
   const int cur = c->npass & 1;
-  uint64_t* keys = cur ? k1 : k0;
-  uint32_t* vals = cur ? v1 : v0;
+  const uint64_t* kv = cur ? kv1 : kv0;
+  const float2* lo = cur ? lo1 : lo0;
+  const float2* hi = cur ? hi1 : hi0;
   const int t = threadIdx.x;
   for (uint32_t q = blockIdx.x; q < nr; q += gridDim.x) {
     const int64_t i = runs[q];
-    const uint32_t h = key_hi(keys, i);
-    if (t == 0) {
-      int64_t L = kFixShort;
-      while (L <= kFixLong && i + L < n && key_hi(keys, i + L) == h) L++;
-      s_len = (int)L;
-      if (L > kFixLong) atomicOr(&c->fallback, 1u);
+    const uint32_t h = (uint32_t)(kv[i] >> 32);
+    // length: the first offset whose high half differs (kFixLong + 1: longer)
+    if (t == 0) s_len = kFixLong + 1;
+    __syncthreads();
+    for (int b0 = kFixShort; b0 <= kFixLong; b0 += 256) {
+      const int j = b0 + t;
+      const bool end = j <= kFixLong && (i + j >= n || (uint32_t)(kv[i + j] >> 32) != h);
+      if (end) atomicMin(&s_len, j);
+      if (__syncthreads_or(end)) break;
     }
     __syncthreads();
     const int L = s_len;
-    if (L > kFixLong) return;  // the full LSD sort takes over
+    if (L > kFixLong) {  // the full LSD sort takes over
+      if (t == 0) atomicOr(&c->fallback, 1u);
+      return;
+    }
     int m = 1;
     while (m < L) m <<= 1;
-    for (int j = t; j < m; j += 256)
-      s_net[j] = j < L ? ((keys[i + j] & 0xFFFFFFFFull) << 32) | vals[i + j] : ~0ull;
+    for (int j = t; j < m; j += 256) {
+      uint64_t v = ~0ull;
+      if (j < L) {
+        const float2 l = lo[i + j], hh = hi[i + j];
+        v = ((uint64_t)(uint32_t)morton64(l.x, l.y, hh.x, hh.y, kp) << 32) | (uint32_t)j;
+      }
+      s_net[j] = v;
+    }
     __syncthreads();
     for (int k = 2; k <= m; k <<= 1) {
       for (int d = k >> 1; d > 0; d >>= 1) {
@@ -589,66 +749,265 @@ __global__ __launch_bounds__(256) void k_fixup_long(int64_t n, BuildCtrl* c, uin
         __syncthreads();
       }
     }
-    const uint64_t hi = (uint64_t)h << 32;
+    // the run's boxes in final order, from the (unchanged) staging buffers
     for (int j = t; j < L; j += 256) {
-      keys[i + j] = hi | (s_net[j] >> 32);
-      vals[i + j] = (uint32_t)s_net[j];
+      const int64_t sp = i + (uint32_t)s_net[j];
+      const float2 l = lo[sp], hh = hi[sp];
+      xmin[i + j] = l.x;
+      ymin[i + j] = l.y;
+      xmax[i + j] = hh.x;
+      ymax[i + j] = hh.y;
+      id[i + j] = (int)(uint32_t)kv[sp];
     }
+    if (t == 0) runs[runs_cap + q] = (uint32_t)L;
     __syncthreads();
   }
 }
 
-// After k_fixup: if a run was too long, plan the full LSD sort (steps 4..11
-// over digits 0..7) from the current order; else nothing more runs.
-__global__ void k_fallback_plan(BuildCtrl* c) {
-  if (threadIdx.x != 0 || !c->fallback) return;
-  plan_steps(c, 4, 8, 0x76543210ull, false);
-  for (int st = kMaxSteps - 1; st >= 4; st--)
-    if (c->active[st]) {
-      c->write_keys[st] = 0;
-      break;
-    }
-}
-
-// Status words of the fallback steps (cleared only when they will run).
-__global__ __launch_bounds__(256) void k_fallback_zero(const BuildCtrl* c, uint32_t* p,
-                                                       int64_t words) {
+// Status words of the fallback steps and the histograms (cleared only when
+// the fallback sort will run).
+__global__ __launch_bounds__(256) void k_fallback_zero(BuildCtrl* c, uint32_t* p, int64_t words) {
   if (!c->fallback) return;
+  if (blockIdx.x == 0)
+    for (int i = threadIdx.x; i < kMaxDigits * 256; i += 256) c->hist[i] = 0;
   for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < words; i += (int64_t)gridDim.x * 256)
     p[i] = 0u;
 }
 
 // ---------------------------------------------------------------------------
-// Group reduction of an MBR over G consecutive lanes (G a power of two <= 64).
-// `has` marks lanes that hold a value; invalid lanes are skipped.
+// Group reduction of an MBR over G consecutive lanes (G a power of two <= 64;
+// every lane of the group ends with the result).  `has` marks lanes that hold
+// a value; invalid lanes are skipped.
+//   The reduction runs on ordered integer keys: f2ord maps floats to uint32 in
+// float order with -0 below +0, and NaN takes the key that loses every
+// comparison (all ones for the min, zero for the max), which an invalid lane
+// takes as well.  A u32 min/max over the keys then gives exactly the bits of
+// the mbr_min/mbr_max fold: the least (greatest) non-NaN value, and the
+// canonical NaN when every value is NaN.  Partners within 16 lanes come from
+// DPP lane permutations (quad xor 1, quad xor 2, half-row mirror, row mirror:
+// each pairs the lane with one in the other half of its 2-, 4-, 8- or
+// 16-lane block), wider ones from ds_bpermute.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_perm(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, 0xF, 0xF, false);
+}
+
+__device__ __forceinline__ uint32_t min_key(float x, bool has) {
+  return (has && x == x) ? f2ord(x) : 0xFFFFFFFFu;
+}
+__device__ __forceinline__ uint32_t max_key(float x, bool has) {
+  return (has && x == x) ? f2ord(x) : 0u;
+}
+
+template <int CTRL>
+__device__ __forceinline__ void key_step_dpp(uint32_t& a, uint32_t& b, uint32_t& cc, uint32_t& d) {
+  a = min(a, dpp_perm<CTRL>(a));
+  b = min(b, dpp_perm<CTRL>(b));
+  cc = max(cc, dpp_perm<CTRL>(cc));
+  d = max(d, dpp_perm<CTRL>(d));
+}
+
 template <int G>
 __device__ __forceinline__ void group_mbr(float& a, float& b, float& cc, float& d, bool has) {
+  uint32_t ka = min_key(a, has), kb = min_key(b, has);
+  uint32_t kc = max_key(cc, has), kd = max_key(d, has);
+  if (G >= 2) key_step_dpp<0xB1>(ka, kb, kc, kd);   // quad_perm [1,0,3,2]
+  if (G >= 4) key_step_dpp<0x4E>(ka, kb, kc, kd);   // quad_perm [2,3,0,1]
+  if (G >= 8) key_step_dpp<0x141>(ka, kb, kc, kd);  // row_half_mirror
+  if (G >= 16) key_step_dpp<0x140>(ka, kb, kc, kd); // row_mirror
 #pragma unroll
-  for (int o = 1; o < G; o <<= 1) {
-    float pa = __shfl_xor(a, o), pb = __shfl_xor(b, o);
-    float pc = __shfl_xor(cc, o), pd = __shfl_xor(d, o);
-    bool ph = __shfl_xor((int)has, o) != 0;
-    if (ph) {
-      if (has) {
-        a = mbr_min(a, pa);
-        b = mbr_min(b, pb);
-        cc = mbr_max(cc, pc);
-        d = mbr_max(d, pd);
-      } else {
-        a = pa;
-        b = pb;
-        cc = pc;
-        d = pd;
-        has = true;
-      }
+  for (int o = 16; o < G; o <<= 1) {
+    ka = min(ka, (uint32_t)__shfl_xor((int)ka, o));
+    kb = min(kb, (uint32_t)__shfl_xor((int)kb, o));
+    kc = max(kc, (uint32_t)__shfl_xor((int)kc, o));
+    kd = max(kd, (uint32_t)__shfl_xor((int)kd, o));
+  }
+  a = ka == 0xFFFFFFFFu ? canon_nan() : ord2f(ka);
+  b = kb == 0xFFFFFFFFu ? canon_nan() : ord2f(kb);
+  cc = kc == 0u ? canon_nan() : ord2f(kc);
+  d = kd == 0u ? canon_nan() : ord2f(kd);
+}
+
+// Leaf-parent node g over sorted boxes [g*G, min(g*G+G, n)) from a group
+// reduction over G lanes (lanes of a group hold consecutive positions, the
+// group's first lane writes): pos = R_sz + g*G, len = count
+// (src/rtree_bulkload_thrust.cuh:136-149).
+template <int G>
+__device__ __forceinline__ void leaf_parent(int64_t i, int64_t n, float a, float b, float cc,
+                                            float d, bool has, const TreeView& tr,
+                                            int64_t l0start, int64_t r_sz, bool mbr_only) {
+  group_mbr<G>(a, b, cc, d, has);
+  if (has && (i % G) == 0) {
+    const int64_t node = l0start + i / G;
+    tr.xmin[node] = a;
+    tr.ymin[node] = b;
+    tr.xmax[node] = cc;
+    tr.ymax[node] = d;
+    if (!mbr_only) {
+      tr.pos[node] = (int)(r_sz + i);
+      const int64_t len = n - i;
+      tr.len[node] = (int)(len < G ? len : G);
     }
   }
 }
 
-// Gather the boxes into sorted order (in place on the caller's arrays, reading
-// the staging copy) and, for power-of-two F <= 64, build the leaf-parent level
-// in the same pass: node g covers sorted boxes [g*F, min(g*F+F, N)) with
-// pos = R_sz + g*F and len = count (src/rtree_bulkload_thrust.cuh:136-149).
+// Last stage of the payload sort: the boxes written to the caller's arrays in
+// final order and, for power-of-two F <= 64, the leaf-parent level built in
+// the same pass.  64-bit keys: runs of 2..kFixShort equal high halves that
+// touch the block's tile are sorted by (low half, position) in LDS, over a
+// window of the tile plus kFixShort positions each side; longer runs are
+// listed for k_fixup_long and written in high-half order here.
+template <int MODE, int G>
+__global__ __launch_bounds__(256) void k_final(
+    int64_t n, BuildCtrl* c, const uint64_t* __restrict__ kv0, const uint64_t* __restrict__ kv1,
+    const float2* __restrict__ lo0, const float2* __restrict__ lo1, const float2* __restrict__ hi0,
+    const float2* __restrict__ hi1, float* __restrict__ xmin, float* __restrict__ ymin,
+    float* __restrict__ xmax, float* __restrict__ ymax, int* __restrict__ id, TreeView tr,
+    int64_t l0start, int64_t r_sz, uint32_t* runs, uint32_t runs_cap) {
+  constexpr bool M64 = MODE == kKeyMorton64;
+  constexpr int kPer = kFinTile / 256;               // owned positions per thread
+  constexpr int kWin = (kFinSpan + 255) / 256;       // window positions per thread
+  __shared__ uint64_t s_kv[M64 ? kFinSpan : 1];      // the window's (key32 << 32 | id)
+  // (low half << 32) | window position of the box that lands at window position j
+  __shared__ uint64_t s_ck[M64 ? kFinSpan : 1];
+  const int cur = c->npass & 1;
+  const uint64_t* kv = cur ? kv1 : kv0;
+  const float2* lo = cur ? lo1 : lo0;
+  const float2* hi = cur ? hi1 : hi0;
+  const int t = threadIdx.x;
+  const KeyParams kp = key_params<MODE>(c);
+  const int64_t ntiles = (n + kFinTile - 1) / kFinTile;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    __syncthreads();  // the previous tile is done with the shared arrays
+    const int64_t tb = tile * kFinTile;
+    const int64_t te = tb + kFinTile < n ? tb + kFinTile : n;
+    int64_t ws = tb;
+    if (M64) {
+      ws = tb > kFixShort ? tb - kFixShort : 0;
+      const int64_t we = te + kFixShort < n ? te + kFixShort : n;
+      const int wn = (int)(we - ws);
+      uint64_t wv[kWin];
+#pragma unroll
+      for (int r = 0; r < kWin; r++) {
+        const int j = t + 256 * r;
+        wv[r] = j < wn ? kv[ws + j] : 0ull;
+      }
+#pragma unroll
+      for (int r = 0; r < kWin; r++) {
+        const int j = t + 256 * r;
+        if (j < wn) {
+          s_kv[j] = wv[r];
+          s_ck[j] = (uint64_t)j;
+        }
+      }
+      __syncthreads();
+      for (int r = 0; r < kWin; r++) {
+        const int j = t + 256 * r;
+        if (j >= wn) break;
+        const uint32_t h = (uint32_t)(s_kv[j] >> 32);
+        const int64_t gs = ws + j;
+        // a run starts at j (the window's first position only at the array
+        // start: a run through it that reaches the tile is a long one)
+        if (j == 0 ? gs != 0 : (uint32_t)(s_kv[j - 1] >> 32) == h) continue;
+        if (j + 1 >= wn || (uint32_t)(s_kv[j + 1] >> 32) != h) continue;  // a single item
+        int L = 2;
+        while (L <= kFixShort && j + L < wn && (uint32_t)(s_kv[j + L] >> 32) == h) L++;
+        if (L > kFixShort || (j + L == wn && we < n)) {  // long (or running past the window)
+          if (gs >= tb && gs < te) {
+            const uint32_t at = atomicAdd(&c->nruns, 1u);
+            if (at < runs_cap) runs[at] = (uint32_t)gs;
+            else atomicOr(&c->fallback, 1u);
+          }
+          continue;
+        }
+        if (gs + L <= tb || gs >= te) continue;  // no position in the tile
+        for (int a2 = 0; a2 < L; a2++) {
+          const float2 l = lo[gs + a2], hh = hi[gs + a2];
+          s_ck[j + a2] =
+              ((uint64_t)(uint32_t)morton64(l.x, l.y, hh.x, hh.y, kp) << 32) | (uint32_t)(j + a2);
+        }
+        for (int a2 = j + 1; a2 < j + L; a2++) {
+          const uint64_t v = s_ck[a2];
+          int b2 = a2 - 1;
+          while (b2 >= j && s_ck[b2] > v) {
+            s_ck[b2 + 1] = s_ck[b2];
+            b2--;
+          }
+          s_ck[b2 + 1] = v;
+        }
+      }
+      __syncthreads();
+    }
+    // owned positions: every load in flight before the first store
+    int64_t sp[kPer];
+    float2 l[kPer], hh[kPer];
+    uint32_t ident[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const int64_t gp = tb + t + 256 * k;
+      sp[k] = gp;
+      if (M64 && gp < n) sp[k] = ws + (uint32_t)s_ck[gp - ws];
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      l[k] = hh[k] = make_float2(0.f, 0.f);
+      ident[k] = 0;
+      if (tb + t + 256 * k < n) {
+        l[k] = lo[sp[k]];
+        hh[k] = hi[sp[k]];
+        ident[k] = M64 ? (uint32_t)s_kv[sp[k] - ws] : (uint32_t)kv[sp[k]];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const int64_t gp = tb + t + 256 * k;
+      const bool has = gp < n;
+      if (has) {
+        xmin[gp] = l[k].x;
+        ymin[gp] = l[k].y;
+        xmax[gp] = hh[k].x;
+        ymax[gp] = hh[k].y;
+        id[gp] = (int)ident[k];
+      }
+      if constexpr (G > 0)
+        leaf_parent<G>(gp, n, l[k].x, l[k].y, hh[k].x, hh[k].y, has, tr, l0start, r_sz, false);
+    }
+  }
+}
+
+// Leaf-parent MBRs over the runs k_fixup_long rewrote (nodes shared by two
+// runs are computed twice, identically).
+template <int G>
+__global__ __launch_bounds__(256) void k_fixup_groups(
+    int64_t n, const BuildCtrl* c, const uint32_t* __restrict__ runs, uint32_t runs_cap,
+    const float* __restrict__ xmin, const float* __restrict__ ymin,
+    const float* __restrict__ xmax, const float* __restrict__ ymax, TreeView tr,
+    int64_t l0start, int64_t r_sz) {
+  const uint32_t nr = c->nruns < runs_cap ? c->nruns : runs_cap;
+  if (blockIdx.x >= nr || c->fallback) return;
+  for (uint32_t q = blockIdx.x; q < nr; q += gridDim.x) {
+    const int64_t i = runs[q], L = runs[runs_cap + q];
+    const int64_t p0 = (i / G) * G;
+    int64_t p1 = ((i + L - 1) / G + 1) * G;
+    if (p1 > n) p1 = n;
+    for (int64_t base = p0; base < p1; base += 256) {
+      const int64_t p = base + threadIdx.x;
+      const bool has = p < p1;
+      float a = 0.f, b = 0.f, cc = 0.f, d = 0.f;
+      if (has) {
+        a = xmin[p];
+        b = ymin[p];
+        cc = xmax[p];
+        d = ymax[p];
+      }
+      leaf_parent<G>(p, n, a, b, cc, d, has, tr, l0start, r_sz, true);
+    }
+  }
+}
+
+// Fallback sort's last stage: gather the boxes into sorted order (in place on
+// the caller's arrays, reading the 32-B staging records) and, for power-of-two
+// F <= 64, the leaf-parent level.  Runs only if c->fallback is set.
 template <int G>
 __global__ __launch_bounds__(256) void k_gather_l0(int64_t n, const BuildCtrl* c,
                                                    const uint32_t* __restrict__ v0,
@@ -657,37 +1016,27 @@ __global__ __launch_bounds__(256) void k_gather_l0(int64_t n, const BuildCtrl* c
                                                    float* ymin, float* xmax, float* ymax,
                                                    int* id, TreeView tr, int64_t l0start,
                                                    int64_t r_sz) {
-  const int64_t i = blockIdx.x * 256ll + threadIdx.x;
-  const int J = c->npass;
-  const uint32_t* vf = (J & 1) ? v1 : v0;
-  float a = 0.f, b = 0.f, cc = 0.f, d = 0.f;
-  const bool has = i < n;
-  if (has) {
-    uint32_t s = J ? vf[i] : (uint32_t)i;
-    uint4 r0 = aos[2 * (size_t)s];
-    uint4 r1 = aos[2 * (size_t)s + 1];
-    a = __uint_as_float(r0.x);
-    b = __uint_as_float(r0.y);
-    cc = __uint_as_float(r0.z);
-    d = __uint_as_float(r0.w);
-    xmin[i] = a;
-    ymin[i] = b;
-    xmax[i] = cc;
-    ymax[i] = d;
-    id[i] = (int)r1.x;
-  }
-  if constexpr (G > 0) {
-    group_mbr<G>(a, b, cc, d, has);
-    if (has && (i % G) == 0) {
-      int64_t g = i / G, node = l0start + g;
-      tr.xmin[node] = a;
-      tr.ymin[node] = b;
-      tr.xmax[node] = cc;
-      tr.ymax[node] = d;
-      tr.pos[node] = (int)(r_sz + i);
-      int64_t len = n - i;
-      tr.len[node] = (int)(len < G ? len : G);
+  if (!c->fallback) return;
+  const uint32_t* vf = (c->npass & 1) ? v1 : v0;
+  for (int64_t base = blockIdx.x * 256ll; base < n; base += (int64_t)gridDim.x * 256) {
+    const int64_t i = base + threadIdx.x;
+    float a = 0.f, b = 0.f, cc = 0.f, d = 0.f;
+    const bool has = i < n;
+    if (has) {
+      const uint32_t s = vf[i];
+      const uint4 r0 = aos[2 * (size_t)s];
+      const uint4 r1 = aos[2 * (size_t)s + 1];
+      a = __uint_as_float(r0.x);
+      b = __uint_as_float(r0.y);
+      cc = __uint_as_float(r0.z);
+      d = __uint_as_float(r0.w);
+      xmin[i] = a;
+      ymin[i] = b;
+      xmax[i] = cc;
+      ymax[i] = d;
+      id[i] = (int)r1.x;
     }
+    if constexpr (G > 0) leaf_parent<G>(i, n, a, b, cc, d, has, tr, l0start, r_sz, false);
   }
 }
 
@@ -755,13 +1104,72 @@ static int pow2_group(int F) {
   }
 }
 
-template <int G>
-static void launch_gather(hipStream_t s, int64_t n, const BuildCtrl* c, const uint32_t* v0,
-                          const uint32_t* v1, const uint4* aos, curtree_bbox_f32* box,
-                          TreeView tr, int64_t l0start, int64_t r_sz) {
-  unsigned grid = (unsigned)((n + 255) / 256);
-  hipLaunchKernelGGL(k_gather_l0<G>, dim3(grid), dim3(256), 0, s, n, c, v0, v1, aos, box->xmin,
-                     box->ymin, box->xmax, box->ymax, box->id, tr, l0start, r_sz);
+struct SortBufs {
+  uint64_t* kv[2];
+  float2* lo[2];
+  float2* hi[2];
+  uint64_t* k64[2];  // fallback: 64-bit keys (the kv buffers)
+  uint32_t* v[2];    // fallback: positions
+  uint4* aos;        // fallback: 32-B staging records (the lo/hi buffers)
+  uint32_t* status;
+  size_t step_words;
+  uint32_t* runs;
+  uint32_t runs_cap;
+};
+
+// Final order, leaf-parent level and, for 64-bit keys, the tie stages.
+template <int MODE, int G>
+static void launch_tail(Prof& pf, hipStream_t s, int64_t n, BuildCtrl* c, const SortBufs& b,
+                        curtree_bbox_f32* box, TreeView tr, int64_t l0start, int64_t r_sz) {
+  pf.mark(s, "k_final");
+  hipLaunchKernelGGL((k_final<MODE, G>),
+                     dim3((unsigned)std::min<int64_t>((n + kFinTile - 1) / kFinTile, CT_FINAL_GRID)),
+                     dim3(256), 0, s, n, c, b.kv[0], b.kv[1], b.lo[0], b.lo[1], b.hi[0], b.hi[1],
+                     box->xmin, box->ymin, box->xmax, box->ymax, box->id, tr, l0start, r_sz, b.runs,
+                     b.runs_cap);
+  if (MODE != kKeyMorton64) return;
+  pf.mark(s, "k_fixup_long");
+  hipLaunchKernelGGL(k_fixup_long, dim3(kFixLongSlots), dim3(256), 0, s, n, c, b.kv[0], b.kv[1],
+                     b.lo[0], b.lo[1], b.hi[0], b.hi[1], box->xmin, box->ymin, box->xmax, box->ymax,
+                     box->id, b.runs, b.runs_cap);
+  if (G > 0)
+    hipLaunchKernelGGL(k_fixup_groups<G>, dim3(kFixLongSlots), dim3(256), 0, s, n,
+                       (const BuildCtrl*)c, (const uint32_t*)b.runs, b.runs_cap, box->xmin,
+                       box->ymin, box->xmax, box->ymax, tr, l0start, r_sz);
+  // the full LSD sort of 64-bit keys: every launch exits at once unless a
+  // run of equal high halves was too long to settle in place
+  pf.mark(s, "k_fallback");
+  hipLaunchKernelGGL(k_fallback_zero, dim3(1024), dim3(256), 0, s, c, b.status + 4 * b.step_words,
+                     (int64_t)(8 * b.step_words));
+  const unsigned sgrid = (unsigned)std::min<int64_t>((n + 1023) / 1024, 2048);
+  hipLaunchKernelGGL(k_keygen64, dim3(sgrid), dim3(256), 0, s, n, (const float*)box->xmin,
+                     (const float*)box->ymin, (const float*)box->xmax, (const float*)box->ymax,
+                     (const int*)box->id, c, b.k64[0], b.k64[1], b.aos, 1);
+  hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(256), 0, s, n, 8, c, 4, 8, 0x76543210ull, 0, 1);
+  const int64_t ntiles = (n + kSortTile - 1) / kSortTile;
+  for (int st = 4; st < kMaxSteps; st++)
+    hipLaunchKernelGGL(k_onesweep<uint64_t>, dim3((unsigned)ntiles), dim3(kSortThreads), 0, s, n,
+                       st, b.k64[0], b.k64[1], b.v[0], b.v[1], c,
+                       b.status + (size_t)st * b.step_words, 0);
+  const unsigned ggrid = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_gather_l0<G>, dim3(ggrid), dim3(256), 0, s, n, (const BuildCtrl*)c,
+                     (const uint32_t*)b.v[0], (const uint32_t*)b.v[1], (const uint4*)b.aos,
+                     box->xmin, box->ymin, box->xmax, box->ymax, box->id, tr, l0start, r_sz);
+}
+
+template <int MODE>
+static void launch_tail_g(Prof& pf, hipStream_t s, int G, int64_t n, BuildCtrl* c,
+                          const SortBufs& b, curtree_bbox_f32* box, TreeView tr, int64_t l0start,
+                          int64_t r_sz) {
+  switch (G) {
+    case 2: launch_tail<MODE, 2>(pf, s, n, c, b, box, tr, l0start, r_sz); break;
+    case 4: launch_tail<MODE, 4>(pf, s, n, c, b, box, tr, l0start, r_sz); break;
+    case 8: launch_tail<MODE, 8>(pf, s, n, c, b, box, tr, l0start, r_sz); break;
+    case 16: launch_tail<MODE, 16>(pf, s, n, c, b, box, tr, l0start, r_sz); break;
+    case 32: launch_tail<MODE, 32>(pf, s, n, c, b, box, tr, l0start, r_sz); break;
+    case 64: launch_tail<MODE, 64>(pf, s, n, c, b, box, tr, l0start, r_sz); break;
+    default: launch_tail<MODE, 0>(pf, s, n, c, b, box, tr, l0start, r_sz);
+  }
 }
 
 static void launch_level(hipStream_t s, int F, int64_t cn, const float* cx0, const float* cy0,
@@ -825,94 +1233,87 @@ int build_f32(Context& ctx, curtree_bbox_f32* box, curtree_rtree_f32* rt, int ke
     return code;
   };
 
-  const int D = key_mode == kKeyRef32 ? 4 : 8;
-  const size_t kb = key_mode == kKeyRef32 ? 4 : 8;
   const int64_t ntiles = (n + kSortTile - 1) / kSortTile;
   const bool m64 = key_mode == kKeyMorton64;
   const int nsteps = m64 ? kMaxSteps : 4;  // Morton: 4 high-half steps + 8 fallback steps
   const size_t step_words = (size_t)ntiles * 256;
   const size_t status_words = (size_t)nsteps * step_words;
   Workspace& ws = ctx.ws;
-  if ((rc = ws.reserve(WsBuildKeys0, n * kb)) || (rc = ws.reserve(WsBuildKeys1, n * kb)) ||
-      (rc = ws.reserve(WsBuildVals0, n * 4)) || (rc = ws.reserve(WsBuildVals1, n * 4)) ||
+  // payload sort: (key32, id) and the box halves, double-buffered (48 B/box);
+  // the fallback reuses them for 64-bit keys and 32-B staging records
+  if ((rc = ws.reserve(WsBuildKeys0, n * 8)) || (rc = ws.reserve(WsBuildKeys1, n * 8)) ||
       (rc = ws.reserve(WsBuildAos, n * 32)) ||
+      (rc = ws.reserve(WsBuildVals0, m64 ? n * 4 : 4)) ||
+      (rc = ws.reserve(WsBuildVals1, m64 ? n * 4 : 4)) ||
       (rc = ws.reserve(WsBuildStatus, status_words * 4)) ||
       (rc = ws.reserve(WsBuildCtrl, sizeof(BuildCtrl))))
     return fail(rc);
-  // long runs of equal high halves (64-bit keys): at most n / (kFixShort + 1)
+  // long runs of equal high halves (64-bit keys): at most n / (kFixShort + 1),
+  // start and length of each
   const uint32_t runs_cap = m64 ? (uint32_t)std::min<int64_t>(n / (kFixShort + 1) + 1, 1 << 22) : 1;
-  if ((rc = ws.reserve(WsBuildRuns, (size_t)runs_cap * 4))) return fail(rc);
-  uint32_t* runs = (uint32_t*)ws.ptr(WsBuildRuns);
+  if ((rc = ws.reserve(WsBuildRuns, (size_t)runs_cap * 8))) return fail(rc);
   BuildCtrl* c = (BuildCtrl*)ws.ptr(WsBuildCtrl);
-  void* k0 = ws.ptr(WsBuildKeys0);
-  void* k1 = ws.ptr(WsBuildKeys1);
-  uint32_t* v0 = (uint32_t*)ws.ptr(WsBuildVals0);
-  uint32_t* v1 = (uint32_t*)ws.ptr(WsBuildVals1);
-  uint4* aos = (uint4*)ws.ptr(WsBuildAos);
-  uint32_t* status = (uint32_t*)ws.ptr(WsBuildStatus);
+  SortBufs sb;
+  sb.kv[0] = (uint64_t*)ws.ptr(WsBuildKeys0);
+  sb.kv[1] = (uint64_t*)ws.ptr(WsBuildKeys1);
+  float2* f2 = (float2*)ws.ptr(WsBuildAos);
+  sb.lo[0] = f2;
+  sb.hi[0] = f2 + n;
+  sb.lo[1] = f2 + 2 * n;
+  sb.hi[1] = f2 + 3 * n;
+  sb.k64[0] = sb.kv[0];
+  sb.k64[1] = sb.kv[1];
+  sb.v[0] = (uint32_t*)ws.ptr(WsBuildVals0);
+  sb.v[1] = (uint32_t*)ws.ptr(WsBuildVals1);
+  sb.aos = (uint4*)ws.ptr(WsBuildAos);
+  sb.status = (uint32_t*)ws.ptr(WsBuildStatus);
+  sb.step_words = step_words;
+  sb.runs = (uint32_t*)ws.ptr(WsBuildRuns);
+  sb.runs_cap = runs_cap;
 
   Prof& pf = ctx.prof;
   pf.mark(s, "k_build_init");
   hipLaunchKernelGGL(k_build_init, dim3(1), dim3(256), 0, s, c);
   const unsigned sgrid = (unsigned)std::min<int64_t>((n + 1023) / 1024, 2048);
-  if (key_mode == kKeyMorton64) {
+  if (m64) {
     pf.mark(s, "k_extent");
     hipLaunchKernelGGL(k_extent, dim3(sgrid), dim3(256), 0, s, n, box->xmin, box->ymin,
                        box->xmax, box->ymax, c);
   }
-  // keygen clears the status words of the first four steps
-  pf.mark(s, "k_keygen");
-  if (key_mode == kKeyRef32)
-    hipLaunchKernelGGL(k_keygen<kKeyRef32>, dim3(sgrid), dim3(256), 0, s, n, box->xmin,
-                       box->ymin, box->xmax, box->ymax, box->id, c, k0, aos, status,
-                       (int64_t)(4 * step_words));
+  // histograms of the 32-bit sort key; clears the status words of steps 0..3
+  pf.mark(s, "k_hist");
+  if (m64)
+    hipLaunchKernelGGL(k_hist<kKeyMorton64>, dim3(sgrid), dim3(256), 0, s, n, box->xmin,
+                       box->ymin, box->xmax, box->ymax, c, sb.status, (int64_t)(4 * step_words));
   else
-    hipLaunchKernelGGL(k_keygen<kKeyMorton64>, dim3(sgrid), dim3(256), 0, s, n, box->xmin,
-                       box->ymin, box->xmax, box->ymax, box->id, c, k0, aos, status,
-                       (int64_t)(4 * step_words));
+    hipLaunchKernelGGL(k_hist<kKeyRef32>, dim3(sgrid), dim3(256), 0, s, n, box->xmin, box->ymin,
+                       box->xmax, box->ymax, c, sb.status, (int64_t)(4 * step_words));
   pf.mark(s, "k_sort_plan");
-  // REF32: digits 0..3; Morton: the high half (digits 4..7) first
-  hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(256), 0, s, n, D, c, 4,
-                     m64 ? 0x7654ull : 0x3210ull, m64 ? 1 : 0);
-  for (int st = 0; st < nsteps; st++) {
-    if (st == 4) {
-      pf.mark(s, "k_fixup");
-      const unsigned fgrid = (unsigned)((n + kFixTile - 1) / kFixTile);
-      hipLaunchKernelGGL(k_fixup, dim3(fgrid), dim3(256), 0, s, n, c, (uint64_t*)k0,
-                         (uint64_t*)k1, v0, v1, runs, runs_cap);
-      hipLaunchKernelGGL(k_fixup_long, dim3(kFixLongSlots), dim3(256), 0, s, n, c, (uint64_t*)k0,
-                         (uint64_t*)k1, v0, v1, (const uint32_t*)runs, runs_cap);
-      hipLaunchKernelGGL(k_fallback_plan, dim3(1), dim3(64), 0, s, c);
-      hipLaunchKernelGGL(k_fallback_zero, dim3(1024), dim3(256), 0, s, (const BuildCtrl*)c,
-                         status + 4 * step_words, (int64_t)(8 * step_words));
-    }
-    // fallback steps exit at once unless k_fixup asked for them
-    pf.mark(s, st < 4 ? "k_onesweep" : "k_onesweep_fallback");
-    uint32_t* stw = status + (size_t)st * step_words;
-    if (key_mode == kKeyRef32)
-      hipLaunchKernelGGL(k_onesweep<uint32_t>, dim3((unsigned)ntiles), dim3(kSortThreads), 0, s,
-                         n, st, (uint32_t*)k0, (uint32_t*)k1, v0, v1, c, stw, 0);
+  hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(256), 0, s, n, 4, c, 0, 4, 0x3210ull, 1, 0);
+  for (int st = 0; st < 4; st++) {
+    pf.mark(s, "k_onesweep");
+    uint32_t* stw = sb.status + (size_t)st * step_words;
+    if (m64)
+      hipLaunchKernelGGL(k_sweep_kv<kKeyMorton64>, dim3((unsigned)ntiles), dim3(kSortThreads), 0,
+                         s, n, st, box->xmin, box->ymin, box->xmax, box->ymax, box->id, sb.kv[0],
+                         sb.kv[1], sb.lo[0], sb.lo[1], sb.hi[0], sb.hi[1], c, stw);
     else
-      hipLaunchKernelGGL(k_onesweep<uint64_t>, dim3((unsigned)ntiles), dim3(kSortThreads), 0, s,
-                         n, st, (uint64_t*)k0, (uint64_t*)k1, v0, v1, c, stw, 0);
+      hipLaunchKernelGGL(k_sweep_kv<kKeyRef32>, dim3((unsigned)ntiles), dim3(kSortThreads), 0, s,
+                         n, st, box->xmin, box->ymin, box->xmax, box->ymax, box->id, sb.kv[0],
+                         sb.kv[1], sb.lo[0], sb.lo[1], sb.hi[0], sb.hi[1], c, stw);
   }
 
   TreeView tr{rt->xmin, rt->ymin, rt->xmax, rt->ymax, rt->pos, rt->len};
   const int64_t l0start = lp.r_sz - lp.n[0];
   const int G = pow2_group(F);
-  pf.mark(s, "k_gather_l0");
-  switch (G) {
-    case 2: launch_gather<2>(s, n, c, v0, v1, aos, box, tr, l0start, lp.r_sz); break;
-    case 4: launch_gather<4>(s, n, c, v0, v1, aos, box, tr, l0start, lp.r_sz); break;
-    case 8: launch_gather<8>(s, n, c, v0, v1, aos, box, tr, l0start, lp.r_sz); break;
-    case 16: launch_gather<16>(s, n, c, v0, v1, aos, box, tr, l0start, lp.r_sz); break;
-    case 32: launch_gather<32>(s, n, c, v0, v1, aos, box, tr, l0start, lp.r_sz); break;
-    case 64: launch_gather<64>(s, n, c, v0, v1, aos, box, tr, l0start, lp.r_sz); break;
-    default:
-      launch_gather<0>(s, n, c, v0, v1, aos, box, tr, l0start, lp.r_sz);
-      pf.mark(s, "k_level");
-      launch_level(s, F, n, box->xmin, box->ymin, box->xmax, box->ymax, lp.r_sz, tr, l0start,
-                   lp.n[0]);
+  if (m64)
+    launch_tail_g<kKeyMorton64>(pf, s, G, n, c, sb, box, tr, l0start, lp.r_sz);
+  else
+    launch_tail_g<kKeyRef32>(pf, s, G, n, c, sb, box, tr, l0start, lp.r_sz);
+  if (G == 0) {
+    pf.mark(s, "k_level");
+    launch_level(s, F, n, box->xmin, box->ymin, box->xmax, box->ymax, lp.r_sz, tr, l0start,
+                 lp.n[0]);
   }
   // upper levels: level l (l >= 1) reads level l-1
   int64_t child = l0start;
@@ -995,7 +1396,7 @@ int pairs_canonical_sort(Context& ctx, int* fid, int* tid, int64_t n, hipStream_
   hipLaunchKernelGGL(k_pair_pack, dim3(sgrid), dim3(256), 0, s, n, fid, tid, c, k0, status,
                      (int64_t)status_words);
   pf.mark(s, "k_sort_plan");
-  hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(256), 0, s, n, 8, c, 8, 0x76543210ull, 1);
+  hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(256), 0, s, n, 8, c, 0, 8, 0x76543210ull, 1, 0);
   for (int p = 0; p < 8; p++) {
     pf.mark(s, "k_onesweep_pairs");
     hipLaunchKernelGGL(k_onesweep<uint64_t>, dim3((unsigned)ntiles), dim3(kSortThreads), 0, s, n,
