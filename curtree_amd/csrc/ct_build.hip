@@ -225,8 +225,20 @@ __global__ __launch_bounds__(256) void k_hist(int64_t n, const float* __restrict
   const int64_t stride = (int64_t)gridDim.x * 256;
   for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += stride) {
     const uint32_t k = sort_key32<MODE>(xmin[i], ymin[i], xmax[i], ymax[i], kp);
+    // a digit that the whole wave shares is added once (degenerate keys, such
+    // as REF32 on unit-square data, would otherwise serialise on one address)
+    const uint64_t act = ballot(true);
 #pragma unroll
-    for (int p = 0; p < 4; p++) atomicAdd(&sh[p * 256 + ((k >> (8 * p)) & 255u)], 1u);
+    for (int p = 0; p < 4; p++) {
+      const uint32_t d = (k >> (8 * p)) & 255u;
+      const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+      const uint64_t same = ballot(d == d0);
+      if (same == act) {
+        if (lane_id() == __ffsll((long long)act) - 1) atomicAdd(&sh[p * 256 + d0], (uint32_t)popc64(act));
+      } else {
+        atomicAdd(&sh[p * 256 + d], 1u);
+      }
+    }
   }
   for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < zero_words; i += stride) zero_ptr[i] = 0u;
   __syncthreads();
@@ -865,10 +877,12 @@ __global__ __launch_bounds__(256) void k_final(
     float* __restrict__ xmax, float* __restrict__ ymax, int* __restrict__ id, TreeView tr,
     int64_t l0start, int64_t r_sz, uint32_t* runs, uint32_t runs_cap) {
   constexpr bool M64 = MODE == kKeyMorton64;
-  constexpr int kPer = kFinTile / 256;               // owned positions per thread
-  constexpr int kWin = (kFinSpan + 255) / 256;       // window positions per thread
-  __shared__ uint64_t s_kv[M64 ? kFinSpan : 1];      // the window's (key32 << 32 | id)
-  // (low half << 32) | window position of the box that lands at window position j
+  constexpr int kPer = kFinTile / 256;  // owned positions per thread
+  // the window: slot j holds position tb - kFixShort + j; the owned position
+  // tb + t + 256k sits in slot kFixShort + t + 256k, and threads 0..31 also
+  // hold one halo slot each (0..15 before the tile, kFinTile+16.. after it)
+  __shared__ uint64_t s_kv[M64 ? kFinSpan : 1];  // (key32 << 32 | id)
+  // (low half << 32) | slot of the box that lands in slot j
   __shared__ uint64_t s_ck[M64 ? kFinSpan : 1];
   const int cur = c->npass & 1;
   const uint64_t* kv = cur ? kv1 : kv0;
@@ -881,38 +895,61 @@ __global__ __launch_bounds__(256) void k_final(
     __syncthreads();  // the previous tile is done with the shared arrays
     const int64_t tb = tile * kFinTile;
     const int64_t te = tb + kFinTile < n ? tb + kFinTile : n;
-    int64_t ws = tb;
+    uint64_t k8[kPer];
+    float2 l8[kPer], h8[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const int64_t gp = tb + t + 256 * k;
+      const bool ok = gp < n;
+      k8[k] = ok ? kv[gp] : 0ull;
+      l8[k] = ok ? lo[gp] : make_float2(0.f, 0.f);
+      h8[k] = ok ? hi[gp] : make_float2(0.f, 0.f);
+    }
     if (M64) {
-      ws = tb > kFixShort ? tb - kFixShort : 0;
+      const int64_t ws = tb - kFixShort;  // (negative on the first tile)
       const int64_t we = te + kFixShort < n ? te + kFixShort : n;
-      const int wn = (int)(we - ws);
-      uint64_t wv[kWin];
-#pragma unroll
-      for (int r = 0; r < kWin; r++) {
-        const int j = t + 256 * r;
-        wv[r] = j < wn ? kv[ws + j] : 0ull;
-      }
-#pragma unroll
-      for (int r = 0; r < kWin; r++) {
-        const int j = t + 256 * r;
-        if (j < wn) {
-          s_kv[j] = wv[r];
-          s_ck[j] = (uint64_t)j;
+      const int jlo = tb == 0 ? kFixShort : 0, jhi = (int)(we - ws);
+      int hj = -1;
+      uint64_t hk = 0;
+      float2 hl = make_float2(0.f, 0.f), hh = hl;
+      if (t < 2 * kFixShort) {
+        hj = t < kFixShort ? t : kFinTile + t;
+        if (hj >= jlo && hj < jhi) {
+          hk = kv[ws + hj];
+          hl = lo[ws + hj];
+          hh = hi[ws + hj];
+        } else {
+          hj = -1;
         }
       }
+      // every slot's item and (low half, slot); runs are sorted on these
+#pragma unroll
+      for (int k = 0; k < kPer; k++) {
+        const int j = kFixShort + t + 256 * k;
+        if (j < jhi) {
+          s_kv[j] = k8[k];
+          s_ck[j] =
+              ((uint64_t)(uint32_t)morton64(l8[k].x, l8[k].y, h8[k].x, h8[k].y, kp) << 32) |
+              (uint32_t)j;
+        }
+      }
+      if (hj >= 0) {
+        s_kv[hj] = hk;
+        s_ck[hj] = ((uint64_t)(uint32_t)morton64(hl.x, hl.y, hh.x, hh.y, kp) << 32) | (uint32_t)hj;
+      }
       __syncthreads();
-      for (int r = 0; r < kWin; r++) {
-        const int j = t + 256 * r;
-        if (j >= wn) break;
+      for (int r = 0; r <= kPer; r++) {
+        const int j = r < kPer ? kFixShort + t + 256 * r : hj;
+        if (j < jlo || j >= jhi) continue;
         const uint32_t h = (uint32_t)(s_kv[j] >> 32);
         const int64_t gs = ws + j;
-        // a run starts at j (the window's first position only at the array
-        // start: a run through it that reaches the tile is a long one)
-        if (j == 0 ? gs != 0 : (uint32_t)(s_kv[j - 1] >> 32) == h) continue;
-        if (j + 1 >= wn || (uint32_t)(s_kv[j + 1] >> 32) != h) continue;  // a single item
+        // a run starts at j (the window's first slot only at the array start:
+        // a run through it that reaches the tile is a long one)
+        if (j == jlo ? gs != 0 : (uint32_t)(s_kv[j - 1] >> 32) == h) continue;
+        if (j + 1 >= jhi || (uint32_t)(s_kv[j + 1] >> 32) != h) continue;  // a single item
         int L = 2;
-        while (L <= kFixShort && j + L < wn && (uint32_t)(s_kv[j + L] >> 32) == h) L++;
-        if (L > kFixShort || (j + L == wn && we < n)) {  // long (or running past the window)
+        while (L <= kFixShort && j + L < jhi && (uint32_t)(s_kv[j + L] >> 32) == h) L++;
+        if (L > kFixShort || (j + L == jhi && we < n)) {  // long (or running past the window)
           if (gs >= tb && gs < te) {
             const uint32_t at = atomicAdd(&c->nruns, 1u);
             if (at < runs_cap) runs[at] = (uint32_t)gs;
@@ -921,11 +958,6 @@ __global__ __launch_bounds__(256) void k_final(
           continue;
         }
         if (gs + L <= tb || gs >= te) continue;  // no position in the tile
-        for (int a2 = 0; a2 < L; a2++) {
-          const float2 l = lo[gs + a2], hh = hi[gs + a2];
-          s_ck[j + a2] =
-              ((uint64_t)(uint32_t)morton64(l.x, l.y, hh.x, hh.y, kp) << 32) | (uint32_t)(j + a2);
-        }
         for (int a2 = j + 1; a2 < j + L; a2++) {
           const uint64_t v = s_ck[a2];
           int b2 = a2 - 1;
@@ -937,25 +969,18 @@ __global__ __launch_bounds__(256) void k_final(
         }
       }
       __syncthreads();
-    }
-    // owned positions: every load in flight before the first store
-    int64_t sp[kPer];
-    float2 l[kPer], hh[kPer];
-    uint32_t ident[kPer];
+      // positions that receive another slot's box (inside sorted runs)
 #pragma unroll
-    for (int k = 0; k < kPer; k++) {
-      const int64_t gp = tb + t + 256 * k;
-      sp[k] = gp;
-      if (M64 && gp < n) sp[k] = ws + (uint32_t)s_ck[gp - ws];
-    }
-#pragma unroll
-    for (int k = 0; k < kPer; k++) {
-      l[k] = hh[k] = make_float2(0.f, 0.f);
-      ident[k] = 0;
-      if (tb + t + 256 * k < n) {
-        l[k] = lo[sp[k]];
-        hh[k] = hi[sp[k]];
-        ident[k] = M64 ? (uint32_t)s_kv[sp[k] - ws] : (uint32_t)kv[sp[k]];
+      for (int k = 0; k < kPer; k++) {
+        const int j = kFixShort + t + 256 * k;
+        if (j < jhi) {
+          const uint32_t js = (uint32_t)s_ck[j];
+          if (js != (uint32_t)j) {
+            l8[k] = lo[ws + js];
+            h8[k] = hi[ws + js];
+            k8[k] = s_kv[js];
+          }
+        }
       }
     }
 #pragma unroll
@@ -963,14 +988,14 @@ __global__ __launch_bounds__(256) void k_final(
       const int64_t gp = tb + t + 256 * k;
       const bool has = gp < n;
       if (has) {
-        xmin[gp] = l[k].x;
-        ymin[gp] = l[k].y;
-        xmax[gp] = hh[k].x;
-        ymax[gp] = hh[k].y;
-        id[gp] = (int)ident[k];
+        xmin[gp] = l8[k].x;
+        ymin[gp] = l8[k].y;
+        xmax[gp] = h8[k].x;
+        ymax[gp] = h8[k].y;
+        id[gp] = (int)(uint32_t)k8[k];
       }
       if constexpr (G > 0)
-        leaf_parent<G>(gp, n, l[k].x, l[k].y, hh[k].x, hh[k].y, has, tr, l0start, r_sz, false);
+        leaf_parent<G>(gp, n, l8[k].x, l8[k].y, h8[k].x, h8[k].y, has, tr, l0start, r_sz, false);
     }
   }
 }
