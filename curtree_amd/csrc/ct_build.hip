@@ -234,7 +234,8 @@ __global__ __launch_bounds__(256) void k_hist(int64_t n, const float* __restrict
       const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
       const uint64_t same = ballot(d == d0);
       if (same == act) {
-        if (lane_id() == __ffsll((long long)act) - 1) atomicAdd(&sh[p * 256 + d0], (uint32_t)popc64(act));
+        if (lane_id() == __ffsll((long long)act) - 1)
+          atomicAdd(&sh[p * 256 + d0], (uint32_t)popc64(act));
       } else {
         atomicAdd(&sh[p * 256 + d], 1u);
       }
@@ -362,6 +363,9 @@ constexpr int kSortWaves = kSortThreads / kWave;
 constexpr int kSortItems = CT_SORT_ITEMS;
 constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr uint32_t kFlagA = 1u << 30, kFlagP = 2u << 30, kValMask = (1u << 30) - 1u;
+// grid of the 64-bit LSD passes (fallback sort, pair order): blocks loop over
+// tickets, so a launch that has nothing to do costs few blocks
+constexpr int64_t kOnesweepGrid = 2048;
 static_assert(kSortThreads >= 256, "one thread per digit");
 
 struct SweepShared {
@@ -470,10 +474,6 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
   const int t = threadIdx.x, w = t >> 6, lane = lane_id();
   const int pass = c->digit[step];
   const int shift = 8 * pass;
-  if (t == 0) sh.tile = atomicAdd(&c->ticket[step], 1u);
-  for (int i = t; i < kSortWaves * 256; i += kSortThreads) (&sh.wh[0][0])[i] = 0;
-  __syncthreads();
-  const int64_t tile = sh.tile;
   const int src = c->src[step];
   const bool ident = step == c->first;  // first step: values are input positions
   const KT* kin = src ? k1 : k0;
@@ -481,58 +481,70 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
   const uint32_t* vin = src ? v1 : v0;
   uint32_t* vout = src ? v0 : v1;
   const bool wk = keys_only || c->write_keys[step] != 0;
+  const int64_t ntiles = (n + kSortTile - 1) / kSortTile;
+  // blocks loop over tickets when the grid is smaller than the tile count
+  for (;;) {
+    if (t == 0) sh.tile = atomicAdd(&c->ticket[step], 1u);
+    for (int i = t; i < kSortWaves * 256; i += kSortThreads) (&sh.wh[0][0])[i] = 0;
+    __syncthreads();
+    const int64_t tile = sh.tile;
+    if (tile >= ntiles) break;
 
-  const int64_t tbase = tile * kSortTile;
-  const int64_t wbase = tbase + (int64_t)w * (kWave * kSortItems);
-  KT key[kSortItems];
-  uint32_t dig[kSortItems], pos[kSortItems];
+    const int64_t tbase = tile * kSortTile;
+    const int64_t wbase = tbase + (int64_t)w * (kWave * kSortItems);
+    KT key[kSortItems];
+    uint32_t dig[kSortItems], pos[kSortItems];
 #pragma unroll
-  for (int i = 0; i < kSortItems; i++) {
-    const int64_t idx = wbase + i * kWave + lane;
-    key[i] = idx < n ? kin[idx] : (KT)0;
-    dig[i] = (uint32_t)(key[i] >> shift) & 255u;
-  }
-  sweep_rank(dig, pos, wbase, n, tile, c->base + pass * 256, status, sh);
-
-  // local reorder of keys through LDS, then scatter runs of equal digits
-#pragma unroll
-  for (int i = 0; i < kSortItems; i++) {
-    const int64_t idx = wbase + i * kWave + lane;
-    if (idx < n) s_buf[pos[i]] = key[i];
-  }
-  __syncthreads();
-  const int64_t rem = n - tbase;
-  const int tn = rem < kSortTile ? (int)rem : kSortTile;
-#pragma unroll
-  for (int i = 0; i < kSortItems; i++) {
-    const int j = t + i * kSortThreads;
-    if (j < tn) {
-      const KT kk = s_buf[j];
-      const uint32_t d = (uint32_t)(kk >> shift) & 255u;
-      s_dig[j] = (uint8_t)d;
-      if (wk) kout[sh.adj[d] + (uint32_t)j] = kk;
+    for (int i = 0; i < kSortItems; i++) {
+      const int64_t idx = wbase + i * kWave + lane;
+      key[i] = idx < n ? kin[idx] : (KT)0;
+      dig[i] = (uint32_t)(key[i] >> shift) & 255u;
     }
-  }
-  if (keys_only) return;
-  // values: reorder through LDS, scatter
-  uint32_t val[kSortItems];
+    sweep_rank(dig, pos, wbase, n, tile, c->base + pass * 256, status, sh);
+
+    // local reorder of keys through LDS, then scatter runs of equal digits
 #pragma unroll
-  for (int i = 0; i < kSortItems; i++) {
-    const int64_t idx = wbase + i * kWave + lane;
-    val[i] = idx < n ? (ident ? (uint32_t)idx : vin[idx]) : 0u;
-  }
-  __syncthreads();
-  uint32_t* s_val = (uint32_t*)s_buf;
+    for (int i = 0; i < kSortItems; i++) {
+      const int64_t idx = wbase + i * kWave + lane;
+      if (idx < n) s_buf[pos[i]] = key[i];
+    }
+    __syncthreads();
+    const int64_t rem = n - tbase;
+    const int tn = rem < kSortTile ? (int)rem : kSortTile;
 #pragma unroll
-  for (int i = 0; i < kSortItems; i++) {
-    const int64_t idx = wbase + i * kWave + lane;
-    if (idx < n) s_val[pos[i]] = val[i];
-  }
-  __syncthreads();
+    for (int i = 0; i < kSortItems; i++) {
+      const int j = t + i * kSortThreads;
+      if (j < tn) {
+        const KT kk = s_buf[j];
+        const uint32_t d = (uint32_t)(kk >> shift) & 255u;
+        s_dig[j] = (uint8_t)d;
+        if (wk) kout[sh.adj[d] + (uint32_t)j] = kk;
+      }
+    }
+    if (!keys_only) {
+      // values: reorder through LDS, scatter
+      uint32_t val[kSortItems];
 #pragma unroll
-  for (int i = 0; i < kSortItems; i++) {
-    const int j = t + i * kSortThreads;
-    if (j < tn) vout[sh.adj[s_dig[j]] + (uint32_t)j] = s_val[j];
+      for (int i = 0; i < kSortItems; i++) {
+        const int64_t idx = wbase + i * kWave + lane;
+        val[i] = idx < n ? (ident ? (uint32_t)idx : vin[idx]) : 0u;
+      }
+      __syncthreads();
+      uint32_t* s_val = (uint32_t*)s_buf;
+#pragma unroll
+      for (int i = 0; i < kSortItems; i++) {
+        const int64_t idx = wbase + i * kWave + lane;
+        if (idx < n) s_val[pos[i]] = val[i];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < kSortItems; i++) {
+        const int j = t + i * kSortThreads;
+        if (j < tn) vout[sh.adj[s_dig[j]] + (uint32_t)j] = s_val[j];
+      }
+    }
+    if ((int64_t)gridDim.x >= ntiles) break;  // one tile per block
+    __syncthreads();  // the tile is done with the shared arrays
   }
 }
 
@@ -1148,7 +1160,8 @@ static void launch_tail(Prof& pf, hipStream_t s, int64_t n, BuildCtrl* c, const 
                         curtree_bbox_f32* box, TreeView tr, int64_t l0start, int64_t r_sz) {
   pf.mark(s, "k_final");
   hipLaunchKernelGGL((k_final<MODE, G>),
-                     dim3((unsigned)std::min<int64_t>((n + kFinTile - 1) / kFinTile, CT_FINAL_GRID)),
+                     dim3((unsigned)std::min<int64_t>((n + kFinTile - 1) / kFinTile,
+                                                      CT_FINAL_GRID)),
                      dim3(256), 0, s, n, c, b.kv[0], b.kv[1], b.lo[0], b.lo[1], b.hi[0], b.hi[1],
                      box->xmin, box->ymin, box->xmax, box->ymax, box->id, tr, l0start, r_sz, b.runs,
                      b.runs_cap);
@@ -1173,8 +1186,8 @@ static void launch_tail(Prof& pf, hipStream_t s, int64_t n, BuildCtrl* c, const 
   hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(256), 0, s, n, 8, c, 4, 8, 0x76543210ull, 0, 1);
   const int64_t ntiles = (n + kSortTile - 1) / kSortTile;
   for (int st = 4; st < kMaxSteps; st++)
-    hipLaunchKernelGGL(k_onesweep<uint64_t>, dim3((unsigned)ntiles), dim3(kSortThreads), 0, s, n,
-                       st, b.k64[0], b.k64[1], b.v[0], b.v[1], c,
+    hipLaunchKernelGGL(k_onesweep<uint64_t>, dim3((unsigned)std::min(ntiles, kOnesweepGrid)),
+                       dim3(kSortThreads), 0, s, n, st, b.k64[0], b.k64[1], b.v[0], b.v[1], c,
                        b.status + (size_t)st * b.step_words, 0);
   const unsigned ggrid = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
   hipLaunchKernelGGL(k_gather_l0<G>, dim3(ggrid), dim3(256), 0, s, n, (const BuildCtrl*)c,
@@ -1424,9 +1437,9 @@ int pairs_canonical_sort(Context& ctx, int* fid, int* tid, int64_t n, hipStream_
   hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(256), 0, s, n, 8, c, 0, 8, 0x76543210ull, 1, 0);
   for (int p = 0; p < 8; p++) {
     pf.mark(s, "k_onesweep_pairs");
-    hipLaunchKernelGGL(k_onesweep<uint64_t>, dim3((unsigned)ntiles), dim3(kSortThreads), 0, s, n,
-                       p, k0, k1, (uint32_t*)nullptr, (uint32_t*)nullptr, c,
-                       status + (size_t)p * ntiles * 256, 1);
+    hipLaunchKernelGGL(k_onesweep<uint64_t>, dim3((unsigned)std::min(ntiles, kOnesweepGrid)),
+                       dim3(kSortThreads), 0, s, n, p, k0, k1, (uint32_t*)nullptr,
+                       (uint32_t*)nullptr, c, status + (size_t)p * ntiles * 256, 1);
   }
   pf.mark(s, "k_pair_unpack");
   hipLaunchKernelGGL(k_pair_unpack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, c, k0,
