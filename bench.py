@@ -56,12 +56,19 @@ def build_bytes_per_box(kbits, fanout):
 
 
 def kernel_bytes(name, n, kbits, fanout):
-    """Algorithmic bytes of one launch of build kernel `name` over n boxes."""
-    kb = kbits // 8
+    """Algorithmic bytes of one launch of build kernel `name` over n boxes.
+
+    The sort passes carry the payload: each moves a (32-bit key, id) item and
+    the 16-byte box, read and written (48 B/box; the first pass reads the
+    caller's 20-byte SoA box instead of 24 staged bytes).  The final pass reads
+    the sorted items and boxes and writes the caller's arrays and the
+    leaf-parent nodes.  The whole build is also reported against the formula
+    of build_bytes_per_box, which prices the reference's algorithm."""
     return {
-        "k_keygen": (16 + kb) * n,
-        "k_onesweep": 2 * (kb + 4) * n,
-        "k_gather_l0": (4 + 20 + 20 + 16 + 24 / fanout) * n,
+        "k_extent": 16 * n,
+        "k_hist": 16 * n,
+        "k_onesweep": 2 * (8 + 16) * n,
+        "k_final": (8 + 16 + 20 + 24 / fanout) * n,
     }.get(name)
 
 
@@ -383,6 +390,9 @@ def main():
                     help="wave-steps per query frontier tile (4, 8, 16; 0: library default)")
     ap.add_argument("--query-grid", type=int, default=0,
                     help="blocks per chained query launch (0: library default)")
+    ap.add_argument("--expand-split", type=int, default=-1, choices=[-1, 0, 1],
+                    help="internal levels: -1 by run size (library default), 0 chained scan, "
+                         "1 count/scan/write")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -398,6 +408,8 @@ def main():
         ct.set_option("query_tile_steps", args.query_tile_steps)
     if args.query_grid:
         ct.set_option("query_grid", args.query_grid)
+    if args.expand_split != -1:
+        ct.set_option("expand_split", args.expand_split)
     args.boxes_set = args.boxes is not None
     args.windows_set = args.windows is not None
     if args.mode == "plain" and args.fanout is None:
@@ -493,10 +505,10 @@ def main():
     query_ms = np.array([r[1] for r in rec])
     step_ms = build_ms + query_ms
     kernels = {}
-    for name in ("k_build_init", "k_extent", "k_keygen", "k_sort_plan", "k_onesweep", "k_fixup",
-                 "k_onesweep_fallback",
-                 "k_gather_l0", "k_level", "k_safe", "k_run_begin", "k_expand", "k_leaf_count",
-                 "k_leaf_pass", "k_tile_scan", "k_leaf_write", "k_write_pieces"):
+    for name in ("k_build_init", "k_extent", "k_hist", "k_sort_plan", "k_onesweep", "k_final",
+                 "k_fixup_long", "k_fallback", "k_level", "k_safe", "k_run_begin", "k_expand",
+                 "k_expand_count", "k_expand_write", "k_leaf_count", "k_leaf_pass", "k_tile_scan",
+                 "k_leaf_write", "k_write_pieces"):
         tot, cnt = ct.Prof.get(name)
         if cnt:
             kernels[name] = {"avg_ms": tot / cnt, "launches_per_step": cnt / args.steps,

@@ -487,6 +487,11 @@ CURTREE_API int curtree_set_option(const char* name, int64_t value) {
     qtuning().staging = (int)value;
     return CURTREE_OK;
   }
+  if (!strcmp(name, "expand_split")) {
+    if (value < -1 || value > 1) return set_error(CURTREE_EINVAL, "expand_split: -1, 0 or 1");
+    qtuning().expand = (int)value;
+    return CURTREE_OK;
+  }
   if (!strcmp(name, "query_grid")) {
     if (value < 0 || value > 65535) return set_error(CURTREE_EINVAL, "query_grid: 0..65535");
     qtuning().grid = (int)value;

@@ -104,7 +104,7 @@ struct Prof {
 // Device-side state of one query pipeline run (one chain of kernels: run
 // begin, one expansion per internal level, leaf stage).  Every size lives
 // here, so a run needs no host round trip and can be captured in a graph.
-constexpr int kMaxRunSlots = 40;
+constexpr int kMaxRunSlots = 72;
 // Level table of a bulk-loaded tree, leaf-parent level first: node i of level
 // l (global index lstart[l] + i) covers sorted boxes [i*span[l], (i+1)*span[l]).
 constexpr int kMaxLevels = 32;
@@ -137,10 +137,19 @@ struct alignas(16) QLeaf {
   uint32_t first;
   uint32_t qf;
 };
+// One internal entry between the expansion kernels: hit mask and the mask of
+// hits that are contained, well-formed subtrees, query, first child.
+struct alignas(8) QExp {
+  uint64_t hit;
+  uint64_t full;
+  uint32_t q;
+  uint32_t first;
+};
+constexpr size_t kEntryBytes = sizeof(QExp) > sizeof(QLeaf) ? sizeof(QExp) : sizeof(QLeaf);
 struct QEngine {
   void* front[2] = {nullptr, nullptr};  // (query, node) frontiers, uint2 entries
   int64_t cap_front = 0;                // entries per frontier buffer (and leaf entries)
-  QLeaf* est = nullptr;                 // leaf entries of a writing run
+  void* est = nullptr;                  // per-entry state between count and write kernels
   uint64_t* ttot = nullptr;             // pairs per leaf tile, then their exclusive scan
   void* pieces = nullptr;               // long box ranges of contained subtrees
   int64_t cap_pieces = 0;

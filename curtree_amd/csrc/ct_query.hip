@@ -400,15 +400,16 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_count(
     atomicAdd((unsigned long long*)&c->total, (unsigned long long)block_total);
 }
 
-// Exclusive scan of the leaf tile counts in place (tiles of P entries).
-__global__ __launch_bounds__(kQThreads) void k_tile_scan(int slot, int P, QCtrl* c,
+// Exclusive scan in place of the tile counts of frontier `cslot` (tiles of P
+// entries); `slot` names the launch's ticket and epoch.
+__global__ __launch_bounds__(kQThreads) void k_tile_scan(int slot, int cslot, int P, QCtrl* c,
                                                          uint64_t* __restrict__ ttot,
                                                          uint64_t* status) {
   __shared__ uint32_t s_tile;
   __shared__ uint64_t s_wtot[kQWaves];
   __shared__ uint64_t s_base, s_total;
   if (c->overflow) return;
-  const int64_t n = ((int64_t)c->count[slot - 1] + P - 1) / P;
+  const int64_t n = ((int64_t)c->count[cslot] + P - 1) / P;
   constexpr int T = kQThreads * kScanPer;
   const int64_t ntiles = (n + T - 1) / T;
   const uint32_t epoch = c->epoch_base + slot;
@@ -436,6 +437,199 @@ __global__ __launch_bounds__(kQThreads) void k_tile_scan(int slot, int P, QCtrl*
     for (int j = 0; j < kScanPer; j++) {
       if (b0 + j < n) ttot[b0 + j] = b;
       b += v[j];
+    }
+  }
+}
+
+// Internal level `slot` without a tile chain (large runs): entries (query,
+// node) whose children are nodes; fin == null: the implicit first frontier
+// (query q0 + e, root).  Three launches, none of which waits on another tile:
+//   k_expand_count  tests the children of each entry; per entry it stores the
+//                   hit mask, the contained-and-safe mask, the query and the
+//                   first child (24 B), per tile the survivor count; the
+//                   level's total by one atomic per block.
+//   k_tile_scan     exclusive scan of the tile counts.
+//   k_expand_write  per tile: loads the entries into LDS, scans them locally,
+//                   adds the tile's base and writes the survivors (query,
+//                   child | FULL) as one flat, coalesced sequence in
+//                   (query, node) order.
+// A contained subtree travels down as one FULL entry (its box range is taken
+// at the leaf stage); it survives every level as its own single child.
+template <int G, int STEPS>
+__global__ __launch_bounds__(kQThreads) void k_expand_count(int slot, QTree tr, QBoxes qb,
+                                                            QCtrl* c, const uint2* __restrict__ fin,
+                                                            QExp* __restrict__ est,
+                                                            uint64_t* __restrict__ ttot,
+                                                            int64_t r_sz) {
+  using S = QShape<G, STEPS>;
+  constexpr int P = S::kPerTile;
+  __shared__ uint32_t s_tile;
+  __shared__ uint64_t s_red[kQWaves];
+  if (c->overflow) return;
+  const int64_t n_in = (int64_t)c->count[slot];
+  const int64_t q0 = c->q0;
+  const int64_t ntiles = (n_in + P - 1) / P;
+  const int t = threadIdx.x, w = t >> 6, lane = lane_id();
+  if (ntiles == 0) return;  // the next level's count stays 0
+  const int sub = lane / G, c_ = lane % G;
+  const int lead = sub * G;
+  // bulk-load layout: the entries of this launch sit on level nl-1-slot and
+  // node i's children are nodes cs + i*F .. of the level below
+  const bool canon = tr.canon && *tr.canon;
+  const int lvl = c->lv.nl - 1 - slot;
+  const int64_t ls = c->lv.lstart[lvl];
+  const int64_t cs = lvl > 0 ? c->lv.lstart[lvl - 1] : 0;
+  const int64_t cc = lvl > 0 ? c->lv.lcount[lvl - 1] : 0;
+  uint64_t block_total = 0;
+  int64_t tile;
+  while ((tile = next_tile(&c->ticket[slot], ntiles, &s_tile)) >= 0) {
+    uint64_t wsum = 0;
+    const int64_t tbase = tile * P;
+#pragma unroll
+    for (int k = 0; k < STEPS; k++) {
+      const int64_t e = tbase + w * S::kPerWave + k * S::E + sub;
+      bool hit = false, cont = false;
+      uint32_t q = 0, first = 0;
+      if (e < n_in) {
+        uint32_t node = 0, full = 0;
+        if (fin) {
+          const uint2 en = fin[e];
+          q = en.x;
+          node = en.y & ~kFull;
+          full = en.y & kFull;
+        } else {
+          q = (uint32_t)(q0 + e);
+        }
+        if (full) {
+          hit = c_ == 0;
+          cont = hit;
+          first = node;
+        } else {
+          int64_t p, L;
+          if (canon) {
+            const int64_t f = ((int64_t)node - ls) * tr.F;
+            p = cs + f;
+            L = cc - f < tr.F ? cc - f : tr.F;
+          } else {
+            p = tr.pos[node];
+            L = tr.len[node];
+          }
+          first = (uint32_t)p;
+          if (c_ < L && p + L <= r_sz) {  // children must be nodes
+            const uint32_t child = (uint32_t)(p + c_);
+            const float4 wq = ld_box(qb, q);
+            const float4 mb = ld_box(tr.mbr, child);
+            hit = intersect(wq.x, wq.y, wq.z, wq.w, mb.x, mb.y, mb.z, mb.w);
+            // (a contained subtree's box range comes from the level table,
+            // so the shortcut needs the bulk-load layout)
+            cont = hit && canon && tr.safe &&
+                   contains(wq.x, wq.y, wq.z, wq.w, mb.x, mb.y, mb.z, mb.w) && tr.safe[child];
+          }
+        }
+      }
+      const uint64_t gm = (ballot(hit) >> lead) & S::kGroupMask;
+      const uint64_t fm = (ballot(cont) >> lead) & S::kGroupMask;
+      if (c_ == 0 && e < n_in) {
+        QExp x;
+        x.hit = gm;
+        x.full = fm;
+        x.q = q;
+        x.first = first;
+        est[e] = x;
+      }
+      uint64_t v = c_ == 0 ? (uint64_t)popc64(gm) : 0ull;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+      wsum += v;
+    }
+    if (lane == 0) s_red[w] = wsum;
+    __syncthreads();
+    if (t == 0) {
+      uint64_t tot = 0;
+#pragma unroll
+      for (int i = 0; i < kQWaves; i++) tot += s_red[i];
+      ttot[tile] = tot;
+      block_total += tot;
+    }
+  }
+  if (t == 0 && block_total)
+    atomicAdd((unsigned long long*)&c->count[slot + 1], (unsigned long long)block_total);
+}
+
+template <int G, int STEPS>
+__global__ __launch_bounds__(kQThreads) void k_expand_write(int slot, QCtrl* c,
+                                                            const QExp* __restrict__ est,
+                                                            const uint64_t* __restrict__ tbase_arr,
+                                                            uint2* __restrict__ fout, int64_t cap) {
+  using S = QShape<G, STEPS>;
+  constexpr int P = S::kPerTile;
+  constexpr int R = (P + kQThreads - 1) / kQThreads;
+  __shared__ uint64_t s_hit[P];
+  __shared__ uint64_t s_full[P];
+  __shared__ uint32_t s_q[P];
+  __shared__ uint32_t s_first[P];
+  __shared__ uint32_t s_off[P];  // tile-relative first output slot
+  __shared__ uint32_t s_tot;
+  __shared__ uint64_t s_scan[kQWaves];
+  if (c->overflow) return;
+  if ((int64_t)c->count[slot + 1] > cap) {  // the next frontier does not fit
+    if (blockIdx.x == 0 && threadIdx.x == 0) c->overflow = 1;
+    return;
+  }
+  const int64_t n_in = (int64_t)c->count[slot];
+  const int64_t ntiles = (n_in + P - 1) / P;
+  const int t = threadIdx.x;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    __syncthreads();  // the previous tile is done with the shared arrays
+    uint64_t loc[R];
+    uint64_t sum = 0;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int li = t * R + r;
+      loc[r] = 0;
+      if (li < P) {
+        const int64_t e = tile * P + li;
+        uint64_t h = 0, f = 0;
+        uint32_t q = 0, fc = 0;
+        if (e < n_in) {
+          const QExp x = est[e];
+          h = x.hit;
+          f = x.full;
+          q = x.q;
+          fc = x.first;
+        }
+        s_hit[li] = h;
+        s_full[li] = f;
+        s_q[li] = q;
+        s_first[li] = fc;
+        loc[r] = (uint64_t)popc64(h);
+        sum += loc[r];
+      }
+    }
+    uint64_t ttot;
+    uint64_t ex = block_excl_scan64(sum, s_scan, &ttot);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int li = t * R + r;
+      if (li < P) s_off[li] = (uint32_t)ex;
+      ex += loc[r];
+    }
+    if (t == 0) s_tot = (uint32_t)ttot;
+    __syncthreads();
+    const uint64_t base = tbase_arr[tile];
+    const uint32_t tn = s_tot;
+#pragma unroll 4
+    for (uint32_t j = t; j < tn; j += kQThreads) {
+      int lo = 0;
+#pragma unroll
+      for (int step = P / 2; step > 0; step >>= 1)
+        if (s_off[lo + step] <= j) lo += step;
+      const uint32_t r = j - s_off[lo];
+      const uint32_t bit = nth_set_bit(s_hit[lo], r);
+      const uint32_t child = s_first[lo] + bit;
+      const uint32_t fl = ((s_full[lo] >> bit) & 1ull) ? kFull : 0u;
+      const uint64_t o = base + j;
+      if (o < (uint64_t)cap) fout[o] = make_uint2(s_q[lo], child | fl);
     }
   }
 }
@@ -623,10 +817,11 @@ static int dev_alloc(void** p, size_t bytes) {
 int QEngine::reserve_front(int64_t entries) {
   if (entries <= cap_front) return CURTREE_OK;
   int rc;
-  // leaf entries (one QLeaf each) and tile counts (tiles hold >= 16 entries)
+  // per-entry state (a QLeaf or QExp each) and tile counts (tiles hold >= 16
+  // entries)
   if ((rc = dev_alloc(&front[0], (size_t)entries * 8)) ||
       (rc = dev_alloc(&front[1], (size_t)entries * 8)) ||
-      (rc = dev_alloc((void**)&est, (size_t)entries * sizeof(QLeaf))) ||
+      (rc = dev_alloc(&est, (size_t)entries * kEntryBytes)) ||
       (rc = dev_alloc((void**)&ttot, (size_t)(entries / 16 + 2) * 8))) {
     cap_front = 0;
     return rc;
@@ -816,10 +1011,18 @@ int compute_safe(QEngine& qe, Prof& pf, const QBoxes& rb, int F, int64_t r_sz, i
   return CURTREE_OK;
 }
 
+// Runs of at least this many queries expand without a tile chain.
+constexpr int64_t kSplitMinQueries = 32768;
+
 template <int G, int STEPS>
 void enqueue_run_g(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
-  // nl-1 expansions, the leaf stage and the tile scan
-  const unsigned nslots = (unsigned)a.nl + 1;
+  using S = QShape<G, STEPS>;
+  // slots: expansions 0..nl-2, the leaf stage nl-1 and its tile scan nl, the
+  // expansions' tile scans nl+1..2nl-1
+  const unsigned nslots = 2u * (unsigned)a.nl;
+  const int ex = qtuning().expand;
+  const bool split =
+      ex == 1 || (ex < 0 && (a.plan != nullptr || a.q1 - a.q0 >= kSplitMinQueries));
   if (qe.needs_clear) {
     hipLaunchKernelGGL(k_ctrl_init, dim3(1), dim3(64), 0, s, qe.ctrl);
     qe.needs_clear = false;
@@ -833,20 +1036,33 @@ void enqueue_run_g(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
   int which = 0;
   for (int l = 0; l + 1 < a.nl; l++) {
     uint2* out = (uint2*)qe.front[which];
-    pf.mark(s, "k_expand");
-    hipLaunchKernelGGL((k_expand<G, STEPS>), dim3(a.grid), dim3(kQThreads), 0, s, l, a.tr, a.qb,
-                       qe.ctrl, cur, out, qe.cap_front, qe.status, a.r_sz);
+    if (split) {
+      pf.mark(s, "k_expand_count");
+      hipLaunchKernelGGL((k_expand_count<G, STEPS>), dim3(a.grid), dim3(kQThreads), 0, s, l,
+                         a.tr, a.qb, qe.ctrl, cur, (QExp*)qe.est, qe.ttot, a.r_sz);
+      pf.mark(s, "k_tile_scan");
+      hipLaunchKernelGGL(k_tile_scan, dim3(64), dim3(kQThreads), 0, s, a.nl + 1 + l, l,
+                         S::kPerTile, qe.ctrl, qe.ttot, qe.status);
+      pf.mark(s, "k_expand_write");
+      hipLaunchKernelGGL((k_expand_write<G, STEPS>), dim3(a.grid), dim3(kQThreads), 0, s, l,
+                         qe.ctrl, (const QExp*)qe.est, (const uint64_t*)qe.ttot, out,
+                         qe.cap_front);
+    } else {
+      pf.mark(s, "k_expand");
+      hipLaunchKernelGGL((k_expand<G, STEPS>), dim3(a.grid), dim3(kQThreads), 0, s, l, a.tr,
+                         a.qb, qe.ctrl, cur, out, qe.cap_front, qe.status, a.r_sz);
+    }
     cur = out;
     which ^= 1;
   }
   const int leaf = a.nl - 1;
   if (a.write) {
-    using S = QShape<G, STEPS>;
     pf.mark(s, "k_leaf_pass");
     hipLaunchKernelGGL((k_leaf_count<G, STEPS, true>), dim3(a.grid), dim3(kQThreads), 0, s, leaf,
-                       a.tr, a.qb, a.rb, qe.ctrl, cur, a.r_sz, a.nbox, qe.est, qe.ttot, a.counts);
+                       a.tr, a.qb, a.rb, qe.ctrl, cur, a.r_sz, a.nbox, (QLeaf*)qe.est, qe.ttot,
+                       a.counts);
     pf.mark(s, "k_tile_scan");
-    hipLaunchKernelGGL(k_tile_scan, dim3(64), dim3(kQThreads), 0, s, leaf + 1, S::kPerTile,
+    hipLaunchKernelGGL(k_tile_scan, dim3(64), dim3(kQThreads), 0, s, leaf + 1, leaf, S::kPerTile,
                        qe.ctrl, qe.ttot, qe.status);
     pf.mark(s, "k_leaf_write");
     hipLaunchKernelGGL((k_leaf_write<G, STEPS>), dim3(a.grid), dim3(kQThreads), 0, s, leaf,
@@ -934,7 +1150,7 @@ int prepare_query(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curt
     return set_error(CURTREE_EINVAL,
                      "query: tree size %d does not match %lld boxes at fanout %d (expected %lld)",
                      rt->sz, (long long)N, F, (long long)lp.r_sz);
-  if (lp.nl + 2 > kMaxRunSlots) return set_error(CURTREE_EINVAL, "query: tree too deep");
+  if (2 * lp.nl > kMaxRunSlots) return set_error(CURTREE_EINVAL, "query: tree too deep");
   if (!rt->xmin || !rt->ymin || !rt->xmax || !rt->ymax || !rt->pos || !rt->len || !rbox->xmin ||
       !rbox->ymin || !rbox->xmax || !rbox->ymax || !rbox->id ||
       (Q > 0 && (!qbox->xmin || !qbox->ymin || !qbox->xmax || !qbox->ymax)))

@@ -21,6 +21,7 @@ struct QTuning {
   int steps = kDefaultSteps;
   int grid = 0;      // blocks per chained launch; 0: by batch size, up to kDefaultChainGrid
   int staging = -1;  // AoS box staging: -1 auto, 0 off, 1 on
+  int expand = -1;   // internal levels: -1 by run size, 0 chained scan, 1 count/scan/write
 };
 QTuning& qtuning();
 
@@ -96,6 +97,7 @@ struct QRunArgs {
   int64_t q0 = 0, q1 = 0;
   const int64_t* plan = nullptr;  // take (q0, q1) from the next plan chunk instead
   bool write = false;             // leaf stage writes pairs (else counts only)
+  bool split = false;             // internal levels as count/scan/write (no tile chain)
   int* out_fid = nullptr;
   int* out_tid = nullptr;
   int64_t out_cap = 0;

@@ -282,6 +282,11 @@ CURTREE_API int curtree_query_last_stats(int64_t* entries, int cap, int* levels,
  *                  extra pass over the boxes; auto when the batch has at
  *                  least 256 windows and at least 1/256 as many windows as
  *                  boxes).  Results are identical either way.
+ *   "expand_split" -1 (default: by run size), 0, 1: internal tree levels are
+ *                  expanded by one chained-scan kernel (0) or by count, tile
+ *                  scan and write kernels with no chain between tiles (1;
+ *                  auto for runs of at least 32768 windows and for batched
+ *                  chunks).  Results are identical either way.
  * Returns CURTREE_EINVAL for an unknown name or value. */
 CURTREE_API int curtree_set_option(const char* name, int64_t value);
 

@@ -264,3 +264,25 @@ def test_non_bulk_load_layout_follows_pos_len(contain, staging):
     sb = d.to_host()
     ofid, otid, _ = orc.bfs(h, sb, qs)
     assert np.array_equal(fid, ofid) and np.array_equal(tid, otid)
+
+
+@pytest.mark.parametrize("split", [0, 1])
+@pytest.mark.parametrize("fanout,key_mode", [(16, 1), (10, 1), (4, 0), (64, 1)])
+def test_expansion_paths(split, fanout, key_mode):
+    # internal levels by the chained-scan kernel (0) or by count / tile scan /
+    # write with no chain between tiles (1): the same pairs in the same order
+    import curtree_amd as ct
+
+    n = 80_000
+    bx = datagen.clustered_boxes(n, seed=150 + fanout, frac=0.3, sigma=0.01, wmax=0.005)
+    qs = datagen.loguniform_windows(3000, seed=151, lo=1e-4, hi=0.2)
+    big = datagen.uniform_windows(64, seed=152, smax=1.0)
+    big = (big[0] - 0.3, big[1] - 0.3, big[2] + 0.3, big[3] + 0.3)
+    ct.set_option("expand_split", split)
+    try:
+        run_case(bx, np.arange(n, dtype=np.int32), qs, fanout, key_mode, brute_sample=300)
+        # windows over most of the domain: with F=4 the leaf-parent frontier
+        # outgrows the first buffers and the run is repeated with larger ones
+        run_case(bx, np.arange(n, dtype=np.int32), big, fanout, key_mode, brute_sample=4)
+    finally:
+        ct.set_option("expand_split", -1)
