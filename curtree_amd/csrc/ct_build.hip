@@ -350,18 +350,25 @@ __global__ __launch_bounds__(256) void k_sort_plan(int64_t n, int D, BuildCtrl* 
 // across tiles by decoupled look-back; tiles are numbered by an atomic ticket
 // so every tile waits only on tiles that have already started.
 #ifndef CT_SORT_THREADS
-#define CT_SORT_THREADS 512
+#define CT_SORT_THREADS 1024
 #endif
 #ifndef CT_SORT_ITEMS
-#define CT_SORT_ITEMS 12
+#define CT_SORT_ITEMS 10
+#endif
+#ifndef CT_SWEEP_ITEMS
+#define CT_SWEEP_ITEMS 12
 #endif
 #ifndef CT_LOOKBACK
 #define CT_LOOKBACK 4
 #endif
-constexpr int kSortThreads = CT_SORT_THREADS;  // 256 or 512
+// One block per CU: large tiles make long runs of equal digits (fewer written
+// cache lines shared with the neighbouring tile) and a short look-back chain.
+constexpr int kSortThreads = CT_SORT_THREADS;  // 256 .. 1024
 constexpr int kSortWaves = kSortThreads / kWave;
-constexpr int kSortItems = CT_SORT_ITEMS;
+constexpr int kSortItems = CT_SORT_ITEMS;      // 64-bit LSD passes
 constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kSweepItems = CT_SWEEP_ITEMS;    // payload passes
+constexpr int kSweepTile = kSortThreads * kSweepItems;
 constexpr uint32_t kFlagA = 1u << 30, kFlagP = 2u << 30, kValMask = (1u << 30) - 1u;
 // grid of the 64-bit LSD passes (fallback sort, pair order): blocks loop over
 // tickets, so a launch that has nothing to do costs few blocks
@@ -380,14 +387,15 @@ struct SweepShared {
 // look-back.  On return pos[i] is item i's position in the tile reordered by
 // digit, and the item at reordered position j with digit d belongs at output
 // position sh.adj[d] + j.  sh.wh must be zero on entry (and synchronised).
-__device__ __forceinline__ void sweep_rank(const uint32_t (&dig)[kSortItems],
-                                           uint32_t (&pos)[kSortItems], int64_t wbase,
+template <int I>
+__device__ __forceinline__ void sweep_rank(const uint32_t (&dig)[I], uint32_t (&pos)[I],
+                                           int64_t wbase,
                                            int64_t n, int64_t tile, const uint32_t* gbase,
                                            uint32_t* status, SweepShared& sh) {
   const int t = threadIdx.x, w = t >> 6, lane = lane_id();
   const uint64_t lt = lanemask_lt();
 #pragma unroll
-  for (int i = 0; i < kSortItems; i++) {
+  for (int i = 0; i < I; i++) {
     const int64_t idx = wbase + i * kWave + lane;
     const bool valid = idx < n;
     const uint32_t d = dig[i];
@@ -457,7 +465,7 @@ __device__ __forceinline__ void sweep_rank(const uint32_t (&dig)[kSortItems],
   }
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < kSortItems; i++) pos[i] += sh.start[dig[i]] + sh.wh[w][dig[i]];
+  for (int i = 0; i < I; i++) pos[i] += sh.start[dig[i]] + sh.wh[w][dig[i]];
 }
 
 // One 8-bit pass of (key, u32 value) or keys alone: the full 8-pass LSD sort
@@ -561,7 +569,7 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
 // waits on earlier stores at block joins).
 template <bool FULL>
 __device__ __forceinline__ void sweep_move(
-    const uint64_t (&kv)[kSortItems], const uint32_t (&pos)[kSortItems], int64_t wbase, int64_t n,
+    const uint64_t (&kv)[kSweepItems], const uint32_t (&pos)[kSweepItems], int64_t wbase, int64_t n,
     int tn, int shift, bool first, const float* __restrict__ xmin, const float* __restrict__ ymin,
     const float* __restrict__ xmax, const float* __restrict__ ymax, const float2* __restrict__ lin,
     const float2* __restrict__ hin, uint64_t* __restrict__ kout, float2* __restrict__ lout,
@@ -570,10 +578,10 @@ __device__ __forceinline__ void sweep_move(
   // the box halves are loaded first: their loads are then older than every
   // store of this tile, and waiting for them never waits for a store (the
   // first pass reads the caller's arrays again: recently read, cache hits)
-  float2 vlo[kSortItems], vhi[kSortItems];
+  float2 vlo[kSweepItems], vhi[kSweepItems];
   if (first) {
 #pragma unroll
-    for (int i = 0; i < kSortItems; i++) {
+    for (int i = 0; i < kSweepItems; i++) {
       const int64_t idx = wbase + i * kWave + lane;
       const bool ok = FULL || idx < n;
       vlo[i] = ok ? make_float2(xmin[idx], ymin[idx]) : make_float2(0.f, 0.f);
@@ -581,7 +589,7 @@ __device__ __forceinline__ void sweep_move(
     }
   } else {
 #pragma unroll
-    for (int i = 0; i < kSortItems; i++) {
+    for (int i = 0; i < kSweepItems; i++) {
       const int64_t idx = wbase + i * kWave + lane;
       const bool ok = FULL || idx < n;
       vlo[i] = ok ? lin[idx] : make_float2(0.f, 0.f);
@@ -589,11 +597,11 @@ __device__ __forceinline__ void sweep_move(
     }
   }
 #pragma unroll
-  for (int i = 0; i < kSortItems; i++)
+  for (int i = 0; i < kSweepItems; i++)
     if (FULL || wbase + i * kWave + lane < n) s_buf[pos[i]] = kv[i];
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < kSortItems; i++) {
+  for (int i = 0; i < kSweepItems; i++) {
     const int j = t + i * kSortThreads;
     if (FULL || j < tn) {
       const uint64_t kk = s_buf[j];
@@ -607,12 +615,12 @@ __device__ __forceinline__ void sweep_move(
   for (int half = 0; half < 2; half++) {
     __syncthreads();  // the previous field's LDS reads are done
 #pragma unroll
-    for (int i = 0; i < kSortItems; i++)
+    for (int i = 0; i < kSweepItems; i++)
       if (FULL || wbase + i * kWave + lane < n) s_f2[pos[i]] = half ? vhi[i] : vlo[i];
     __syncthreads();
     float2* hdst = half ? hout : lout;
 #pragma unroll
-    for (int i = 0; i < kSortItems; i++) {
+    for (int i = 0; i < kSweepItems; i++) {
       const int j = t + i * kSortThreads;
       if (FULL || j < tn) hdst[sh.adj[s_dig[j]] + (uint32_t)j] = s_f2[j];
     }
@@ -627,8 +635,8 @@ __global__ __launch_bounds__(kSortThreads) void k_sweep_kv(
     float2* __restrict__ lo1, float2* __restrict__ hi0, float2* __restrict__ hi1, BuildCtrl* c,
     uint32_t* __restrict__ status) {
   __shared__ SweepShared sh;
-  __shared__ uint64_t s_buf[kSortTile];
-  __shared__ uint8_t s_dig[kSortTile];
+  __shared__ uint64_t s_buf[kSweepTile];
+  __shared__ uint8_t s_dig[kSweepTile];
 
   if (!c->active[step]) return;
   const int t = threadIdx.x, w = t >> 6, lane = lane_id();
@@ -649,16 +657,16 @@ __global__ __launch_bounds__(kSortThreads) void k_sweep_kv(
   const float2* hin = src ? hi1 : hi0;
   float2* hout = src ? hi0 : hi1;
 
-  const int64_t tbase = tile * kSortTile;
-  const int64_t wbase = tbase + (int64_t)w * (kWave * kSortItems);
-  uint64_t kv[kSortItems];
-  uint32_t dig[kSortItems], pos[kSortItems];
+  const int64_t tbase = tile * kSweepTile;
+  const int64_t wbase = tbase + (int64_t)w * (kWave * kSweepItems);
+  uint64_t kv[kSweepItems];
+  uint32_t dig[kSweepItems], pos[kSweepItems];
   // keys (each load loop issues all of its loads before the first use)
   if (first) {
-    float a[kSortItems], b[kSortItems], cc[kSortItems], d[kSortItems];
-    int ident[kSortItems];
+    float a[kSweepItems], b[kSweepItems], cc[kSweepItems], d[kSweepItems];
+    int ident[kSweepItems];
 #pragma unroll
-    for (int i = 0; i < kSortItems; i++) {
+    for (int i = 0; i < kSweepItems; i++) {
       const int64_t idx = wbase + i * kWave + lane;
       const bool ok = idx < n;
       a[i] = ok ? xmin[idx] : 0.f;
@@ -668,22 +676,22 @@ __global__ __launch_bounds__(kSortThreads) void k_sweep_kv(
       ident[i] = ok ? id[idx] : 0;
     }
 #pragma unroll
-    for (int i = 0; i < kSortItems; i++)
+    for (int i = 0; i < kSweepItems; i++)
       kv[i] = ((uint64_t)sort_key32<MODE>(a[i], b[i], cc[i], d[i], kp) << 32) | (uint32_t)ident[i];
   } else {
 #pragma unroll
-    for (int i = 0; i < kSortItems; i++) {
+    for (int i = 0; i < kSweepItems; i++) {
       const int64_t idx = wbase + i * kWave + lane;
       kv[i] = idx < n ? kin[idx] : 0ull;
     }
   }
 #pragma unroll
-  for (int i = 0; i < kSortItems; i++) dig[i] = (uint32_t)(kv[i] >> shift) & 255u;
+  for (int i = 0; i < kSweepItems; i++) dig[i] = (uint32_t)(kv[i] >> shift) & 255u;
   sweep_rank(dig, pos, wbase, n, tile, c->base + pass * 256, status, sh);
 
   const int64_t rem = n - tbase;
-  if (rem >= kSortTile)  // full tile: no predicated loads or stores
-    sweep_move<true>(kv, pos, wbase, n, kSortTile, shift, first, xmin, ymin, xmax, ymax, lin, hin,
+  if (rem >= kSweepTile)  // full tile: no predicated loads or stores
+    sweep_move<true>(kv, pos, wbase, n, kSweepTile, shift, first, xmin, ymin, xmax, ymax, lin, hin,
                      kout, lout, hout, sh, s_buf, s_dig);
   else
     sweep_move<false>(kv, pos, wbase, n, (int)rem, shift, first, xmin, ymin, xmax, ymax, lin, hin,
@@ -1271,10 +1279,11 @@ int build_f32(Context& ctx, curtree_bbox_f32* box, curtree_rtree_f32* rt, int ke
     return code;
   };
 
-  const int64_t ntiles = (n + kSortTile - 1) / kSortTile;
+  const int64_t ntiles = (n + kSweepTile - 1) / kSweepTile;      // payload passes
+  const int64_t ntiles_lsd = (n + kSortTile - 1) / kSortTile;    // 64-bit LSD passes
   const bool m64 = key_mode == kKeyMorton64;
   const int nsteps = m64 ? kMaxSteps : 4;  // Morton: 4 high-half steps + 8 fallback steps
-  const size_t step_words = (size_t)ntiles * 256;
+  const size_t step_words = (size_t)std::max(ntiles, ntiles_lsd) * 256;
   const size_t status_words = (size_t)nsteps * step_words;
   Workspace& ws = ctx.ws;
   // payload sort: (key32, id) and the box halves, double-buffered (48 B/box);
