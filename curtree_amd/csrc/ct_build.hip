@@ -209,13 +209,14 @@ __device__ __forceinline__ uint32_t sort_key32(float a, float b, float cc, float
   return (uint32_t)(morton64(a, b, cc, d, p) >> 32);
 }
 
-// Histograms of the four digits of the 32-bit sort key, and the status words
-// of the sort steps cleared.
+// The 32-bit sort keys (read by the first pass), the histograms of their four
+// digits, and the status words of the sort steps cleared.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_hist(int64_t n, const float* __restrict__ xmin,
                                               const float* __restrict__ ymin,
                                               const float* __restrict__ xmax,
                                               const float* __restrict__ ymax, BuildCtrl* c,
+                                              uint32_t* __restrict__ keys,
                                               uint32_t* __restrict__ zero_ptr,
                                               int64_t zero_words) {
   __shared__ uint32_t sh[4 * 256];
@@ -225,6 +226,7 @@ __global__ __launch_bounds__(256) void k_hist(int64_t n, const float* __restrict
   const int64_t stride = (int64_t)gridDim.x * 256;
   for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += stride) {
     const uint32_t k = sort_key32<MODE>(xmin[i], ymin[i], xmax[i], ymax[i], kp);
+    keys[i] = k;
     // a digit that the whole wave shares is added once (degenerate keys, such
     // as REF32 on unit-square data, would otherwise serialise on one address)
     const uint64_t act = ballot(true);
@@ -330,8 +332,9 @@ __global__ __launch_bounds__(256) void k_sort_plan(int64_t n, int D, BuildCtrl* 
   if (threadIdx.x == 0) {
     c->trivial = s_triv;
     if (gate) c->first = -1;
-    // (the fallback's gather reads the values its steps write: one must run)
-    plan_steps(c, s0, ns, digits, keep_last_keys != 0 || gate != 0);
+    // at least one step runs: the stage after the sort reads the buffers a
+    // step writes
+    plan_steps(c, s0, ns, digits, true);
     // the last step writes values only, unless a later stage reads the keys
     if (!keep_last_keys)
       for (int st = s0 + ns - 1; st >= s0; st--)
@@ -556,13 +559,13 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
   }
 }
 
-// One 8-bit pass of the payload sort: the items are (key32 << 32 | id) plus
-// the box as two float2 halves (xmin, ymin) and (xmax, ymax), all moved by
-// every pass, so that the sorted boxes need no gather.  The first pass that
-// runs reads the caller's SoA arrays and computes the keys itself; later
-// passes read the staging buffers the previous pass wrote.  The three 8-B
-// fields are reordered through LDS one after the other, so the tile's LDS
-// buffer stays 8 B per item.
+// Payload sort passes: the items are (key32 << 32 | id) plus the box as two
+// float2 halves (xmin, ymin) and (xmax, ymax), all moved by every pass, so
+// that the sorted boxes need no gather.  The first pass that runs reads the
+// keys k_hist stored and the caller's SoA arrays; later passes read the
+// staging buffers the previous pass wrote.  The three 8-B fields are
+// reordered through LDS one after the other, so the tile's LDS buffer stays
+// 8 B per item.
 // Moves the ranked tile of the payload sort: keys, then the two box halves,
 // each reordered through LDS and scattered in runs of equal digits.  FULL:
 // every item of the tile exists (no predicated memory operations, so nothing
@@ -572,12 +575,12 @@ __device__ __forceinline__ void sweep_move(
     const uint64_t (&kv)[kSweepItems], const uint32_t (&pos)[kSweepItems], int64_t wbase, int64_t n,
     int tn, int shift, bool first, const float* __restrict__ xmin, const float* __restrict__ ymin,
     const float* __restrict__ xmax, const float* __restrict__ ymax, const float2* __restrict__ lin,
-    const float2* __restrict__ hin, uint64_t* __restrict__ kout, float2* __restrict__ lout,
-    float2* __restrict__ hout, SweepShared& sh, uint64_t* s_buf, uint8_t* s_dig) {
+    const float2* __restrict__ hin, uint64_t* __restrict__ kout, uint32_t* __restrict__ idout,
+    float2* __restrict__ lout, float2* __restrict__ hout, SweepShared& sh, uint64_t* s_buf,
+    uint8_t* s_dig) {
   const int t = threadIdx.x, lane = lane_id();
   // the box halves are loaded first: their loads are then older than every
-  // store of this tile, and waiting for them never waits for a store (the
-  // first pass reads the caller's arrays again: recently read, cache hits)
+  // store of this tile, and waiting for them never waits for a store
   float2 vlo[kSweepItems], vhi[kSweepItems];
   if (first) {
 #pragma unroll
@@ -607,7 +610,8 @@ __device__ __forceinline__ void sweep_move(
       const uint64_t kk = s_buf[j];
       const uint32_t d = (uint32_t)(kk >> shift) & 255u;
       s_dig[j] = (uint8_t)d;
-      kout[sh.adj[d] + (uint32_t)j] = kk;
+      if (idout) idout[sh.adj[d] + (uint32_t)j] = (uint32_t)kk;
+      else kout[sh.adj[d] + (uint32_t)j] = kk;
     }
   }
   float2* s_f2 = (float2*)s_buf;
@@ -627,12 +631,15 @@ __device__ __forceinline__ void sweep_move(
   }
 }
 
-template <int MODE>
+// One 8-bit pass of the payload sort.  The first pass that runs reads the
+// sort keys k_hist stored and the caller's arrays; the last one writes the ids
+// alone (k_final recomputes the keys from the boxes).
 __global__ __launch_bounds__(kSortThreads) void k_sweep_kv(
     int64_t n, int step, const float* __restrict__ xmin, const float* __restrict__ ymin,
     const float* __restrict__ xmax, const float* __restrict__ ymax, const int* __restrict__ id,
-    uint64_t* __restrict__ kv0, uint64_t* __restrict__ kv1, float2* __restrict__ lo0,
-    float2* __restrict__ lo1, float2* __restrict__ hi0, float2* __restrict__ hi1, BuildCtrl* c,
+    const uint32_t* __restrict__ keys, uint64_t* __restrict__ kv0, uint64_t* __restrict__ kv1,
+    uint32_t* __restrict__ ids, float2* __restrict__ lo0, float2* __restrict__ lo1,
+    float2* __restrict__ hi0, float2* __restrict__ hi1, BuildCtrl* c,
     uint32_t* __restrict__ status) {
   __shared__ SweepShared sh;
   __shared__ uint64_t s_buf[kSweepTile];
@@ -643,8 +650,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sweep_kv(
   const int pass = c->digit[step];
   const int shift = 32 + 8 * pass;
   const bool first = step == c->first;
-  KeyParams kp{0.f, 0.f, 0.f, 0.f};
-  if (MODE == kKeyMorton64 && first) kp = key_params<MODE>(c);
+  const bool last = c->write_keys[step] == 0;
   if (t == 0) sh.tile = atomicAdd(&c->ticket[step], 1u);
   for (int i = t; i < kSortWaves * 256; i += kSortThreads) (&sh.wh[0][0])[i] = 0;
   __syncthreads();
@@ -661,23 +667,19 @@ __global__ __launch_bounds__(kSortThreads) void k_sweep_kv(
   const int64_t wbase = tbase + (int64_t)w * (kWave * kSweepItems);
   uint64_t kv[kSweepItems];
   uint32_t dig[kSweepItems], pos[kSweepItems];
-  // keys (each load loop issues all of its loads before the first use)
+  // items (each load loop issues all of its loads before the first use)
   if (first) {
-    float a[kSweepItems], b[kSweepItems], cc[kSweepItems], d[kSweepItems];
+    uint32_t k32[kSweepItems];
     int ident[kSweepItems];
 #pragma unroll
     for (int i = 0; i < kSweepItems; i++) {
       const int64_t idx = wbase + i * kWave + lane;
       const bool ok = idx < n;
-      a[i] = ok ? xmin[idx] : 0.f;
-      b[i] = ok ? ymin[idx] : 0.f;
-      cc[i] = ok ? xmax[idx] : 0.f;
-      d[i] = ok ? ymax[idx] : 0.f;
+      k32[i] = ok ? keys[idx] : 0u;
       ident[i] = ok ? id[idx] : 0;
     }
 #pragma unroll
-    for (int i = 0; i < kSweepItems; i++)
-      kv[i] = ((uint64_t)sort_key32<MODE>(a[i], b[i], cc[i], d[i], kp) << 32) | (uint32_t)ident[i];
+    for (int i = 0; i < kSweepItems; i++) kv[i] = ((uint64_t)k32[i] << 32) | (uint32_t)ident[i];
   } else {
 #pragma unroll
     for (int i = 0; i < kSweepItems; i++) {
@@ -689,13 +691,14 @@ __global__ __launch_bounds__(kSortThreads) void k_sweep_kv(
   for (int i = 0; i < kSweepItems; i++) dig[i] = (uint32_t)(kv[i] >> shift) & 255u;
   sweep_rank(dig, pos, wbase, n, tile, c->base + pass * 256, status, sh);
 
+  uint32_t* idout = last ? ids : nullptr;
   const int64_t rem = n - tbase;
   if (rem >= kSweepTile)  // full tile: no predicated loads or stores
     sweep_move<true>(kv, pos, wbase, n, kSweepTile, shift, first, xmin, ymin, xmax, ymax, lin, hin,
-                     kout, lout, hout, sh, s_buf, s_dig);
+                     kout, idout, lout, hout, sh, s_buf, s_dig);
   else
     sweep_move<false>(kv, pos, wbase, n, (int)rem, shift, first, xmin, ymin, xmax, ymax, lin, hin,
-                      kout, lout, hout, sh, s_buf, s_dig);
+                      kout, idout, lout, hout, sh, s_buf, s_dig);
 }
 
 // ---------------------------------------------------------------------------
@@ -722,8 +725,7 @@ constexpr int kFinTile = 2048;       // positions owned by one block of k_final 
 constexpr int kFinSpan = kFinTile + 2 * kFixShort;
 
 __global__ __launch_bounds__(256) void k_fixup_long(
-    int64_t n, BuildCtrl* c, const uint64_t* __restrict__ kv0, const uint64_t* __restrict__ kv1,
-    const float2* __restrict__ lo0, const float2* __restrict__ lo1, const float2* __restrict__ hi0,
+    int64_t n, BuildCtrl* c, const uint32_t* __restrict__ ids, const float2* __restrict__ lo0, const float2* __restrict__ lo1, const float2* __restrict__ hi0,
     const float2* __restrict__ hi1, float* __restrict__ xmin, float* __restrict__ ymin,
     float* __restrict__ xmax, float* __restrict__ ymax, int* __restrict__ id, uint32_t* runs,
     uint32_t runs_cap) {
@@ -733,19 +735,26 @@ __global__ __launch_bounds__(256) void k_fixup_long(
   if (blockIdx.x >= nr || c->fallback) return;
   const KeyParams kp = key_params<kKeyMorton64>(c);
   const int cur = c->npass & 1;
-  const uint64_t* kv = cur ? kv1 : kv0;
   const float2* lo = cur ? lo1 : lo0;
   const float2* hi = cur ? hi1 : hi0;
   const int t = threadIdx.x;
   for (uint32_t q = blockIdx.x; q < nr; q += gridDim.x) {
     const int64_t i = runs[q];
-    const uint32_t h = (uint32_t)(kv[i] >> 32);
+    const uint32_t h = (uint32_t)(morton64(lo[i].x, lo[i].y, hi[i].x, hi[i].y, kp) >> 32);
     // length: the first offset whose high half differs (kFixLong + 1: longer)
     if (t == 0) s_len = kFixLong + 1;
     __syncthreads();
     for (int b0 = kFixShort; b0 <= kFixLong; b0 += 256) {
       const int j = b0 + t;
-      const bool end = j <= kFixLong && (i + j >= n || (uint32_t)(kv[i + j] >> 32) != h);
+      bool end = false;
+      if (j <= kFixLong) {
+        if (i + j >= n) {
+          end = true;
+        } else {
+          const float2 l = lo[i + j], hh = hi[i + j];
+          end = (uint32_t)(morton64(l.x, l.y, hh.x, hh.y, kp) >> 32) != h;
+        }
+      }
       if (end) atomicMin(&s_len, j);
       if (__syncthreads_or(end)) break;
     }
@@ -789,7 +798,7 @@ __global__ __launch_bounds__(256) void k_fixup_long(
       ymin[i + j] = l.y;
       xmax[i + j] = hh.x;
       ymax[i + j] = hh.y;
-      id[i + j] = (int)(uint32_t)kv[sp];
+      id[i + j] = (int)ids[sp];
     }
     if (t == 0) runs[runs_cap + q] = (uint32_t)L;
     __syncthreads();
@@ -891,8 +900,7 @@ __device__ __forceinline__ void leaf_parent(int64_t i, int64_t n, float a, float
 // listed for k_fixup_long and written in high-half order here.
 template <int MODE, int G>
 __global__ __launch_bounds__(256) void k_final(
-    int64_t n, BuildCtrl* c, const uint64_t* __restrict__ kv0, const uint64_t* __restrict__ kv1,
-    const float2* __restrict__ lo0, const float2* __restrict__ lo1, const float2* __restrict__ hi0,
+    int64_t n, BuildCtrl* c, const uint32_t* __restrict__ ids, const float2* __restrict__ lo0, const float2* __restrict__ lo1, const float2* __restrict__ hi0,
     const float2* __restrict__ hi1, float* __restrict__ xmin, float* __restrict__ ymin,
     float* __restrict__ xmax, float* __restrict__ ymax, int* __restrict__ id, TreeView tr,
     int64_t l0start, int64_t r_sz, uint32_t* runs, uint32_t runs_cap) {
@@ -901,11 +909,10 @@ __global__ __launch_bounds__(256) void k_final(
   // the window: slot j holds position tb - kFixShort + j; the owned position
   // tb + t + 256k sits in slot kFixShort + t + 256k, and threads 0..31 also
   // hold one halo slot each (0..15 before the tile, kFinTile+16.. after it)
-  __shared__ uint64_t s_kv[M64 ? kFinSpan : 1];  // (key32 << 32 | id)
+  __shared__ uint32_t s_hi[M64 ? kFinSpan : 1];  // high half of the slot's key
   // (low half << 32) | slot of the box that lands in slot j
   __shared__ uint64_t s_ck[M64 ? kFinSpan : 1];
   const int cur = c->npass & 1;
-  const uint64_t* kv = cur ? kv1 : kv0;
   const float2* lo = cur ? lo1 : lo0;
   const float2* hi = cur ? hi1 : hi0;
   const int t = threadIdx.x;
@@ -915,13 +922,13 @@ __global__ __launch_bounds__(256) void k_final(
     __syncthreads();  // the previous tile is done with the shared arrays
     const int64_t tb = tile * kFinTile;
     const int64_t te = tb + kFinTile < n ? tb + kFinTile : n;
-    uint64_t k8[kPer];
+    uint32_t k8[kPer];  // ids
     float2 l8[kPer], h8[kPer];
 #pragma unroll
     for (int k = 0; k < kPer; k++) {
       const int64_t gp = tb + t + 256 * k;
       const bool ok = gp < n;
-      k8[k] = ok ? kv[gp] : 0ull;
+      k8[k] = ok ? ids[gp] : 0u;
       l8[k] = ok ? lo[gp] : make_float2(0.f, 0.f);
       h8[k] = ok ? hi[gp] : make_float2(0.f, 0.f);
     }
@@ -930,45 +937,44 @@ __global__ __launch_bounds__(256) void k_final(
       const int64_t we = te + kFixShort < n ? te + kFixShort : n;
       const int jlo = tb == 0 ? kFixShort : 0, jhi = (int)(we - ws);
       int hj = -1;
-      uint64_t hk = 0;
       float2 hl = make_float2(0.f, 0.f), hh = hl;
       if (t < 2 * kFixShort) {
         hj = t < kFixShort ? t : kFinTile + t;
         if (hj >= jlo && hj < jhi) {
-          hk = kv[ws + hj];
           hl = lo[ws + hj];
           hh = hi[ws + hj];
         } else {
           hj = -1;
         }
       }
-      // every slot's item and (low half, slot); runs are sorted on these
+      // every slot's key, recomputed from its box: the high half finds the
+      // runs, (low half, slot) orders them
 #pragma unroll
       for (int k = 0; k < kPer; k++) {
         const int j = kFixShort + t + 256 * k;
         if (j < jhi) {
-          s_kv[j] = k8[k];
-          s_ck[j] =
-              ((uint64_t)(uint32_t)morton64(l8[k].x, l8[k].y, h8[k].x, h8[k].y, kp) << 32) |
-              (uint32_t)j;
+          const uint64_t key = morton64(l8[k].x, l8[k].y, h8[k].x, h8[k].y, kp);
+          s_hi[j] = (uint32_t)(key >> 32);
+          s_ck[j] = ((uint64_t)(uint32_t)key << 32) | (uint32_t)j;
         }
       }
       if (hj >= 0) {
-        s_kv[hj] = hk;
-        s_ck[hj] = ((uint64_t)(uint32_t)morton64(hl.x, hl.y, hh.x, hh.y, kp) << 32) | (uint32_t)hj;
+        const uint64_t key = morton64(hl.x, hl.y, hh.x, hh.y, kp);
+        s_hi[hj] = (uint32_t)(key >> 32);
+        s_ck[hj] = ((uint64_t)(uint32_t)key << 32) | (uint32_t)hj;
       }
       __syncthreads();
       for (int r = 0; r <= kPer; r++) {
         const int j = r < kPer ? kFixShort + t + 256 * r : hj;
         if (j < jlo || j >= jhi) continue;
-        const uint32_t h = (uint32_t)(s_kv[j] >> 32);
+        const uint32_t h = s_hi[j];
         const int64_t gs = ws + j;
         // a run starts at j (the window's first slot only at the array start:
         // a run through it that reaches the tile is a long one)
-        if (j == jlo ? gs != 0 : (uint32_t)(s_kv[j - 1] >> 32) == h) continue;
-        if (j + 1 >= jhi || (uint32_t)(s_kv[j + 1] >> 32) != h) continue;  // a single item
+        if (j == jlo ? gs != 0 : s_hi[j - 1] == h) continue;
+        if (j + 1 >= jhi || s_hi[j + 1] != h) continue;  // a single item
         int L = 2;
-        while (L <= kFixShort && j + L < jhi && (uint32_t)(s_kv[j + L] >> 32) == h) L++;
+        while (L <= kFixShort && j + L < jhi && s_hi[j + L] == h) L++;
         if (L > kFixShort || (j + L == jhi && we < n)) {  // long (or running past the window)
           if (gs >= tb && gs < te) {
             const uint32_t at = atomicAdd(&c->nruns, 1u);
@@ -998,7 +1004,7 @@ __global__ __launch_bounds__(256) void k_final(
           if (js != (uint32_t)j) {
             l8[k] = lo[ws + js];
             h8[k] = hi[ws + js];
-            k8[k] = s_kv[js];
+            k8[k] = ids[ws + js];
           }
         }
       }
@@ -1012,7 +1018,7 @@ __global__ __launch_bounds__(256) void k_final(
         ymin[gp] = l8[k].y;
         xmax[gp] = h8[k].x;
         ymax[gp] = h8[k].y;
-        id[gp] = (int)(uint32_t)k8[k];
+        id[gp] = (int)k8[k];
       }
       if constexpr (G > 0)
         leaf_parent<G>(gp, n, l8[k].x, l8[k].y, h8[k].x, h8[k].y, has, tr, l0start, r_sz, false);
@@ -1151,6 +1157,8 @@ static int pow2_group(int F) {
 
 struct SortBufs {
   uint64_t* kv[2];
+  uint32_t* keys;    // sort keys of the input order (k_hist -> first pass)
+  uint32_t* ids;     // ids in sorted order (last pass -> final pass)
   float2* lo[2];
   float2* hi[2];
   uint64_t* k64[2];  // fallback: 64-bit keys (the kv buffers)
@@ -1170,15 +1178,15 @@ static void launch_tail(Prof& pf, hipStream_t s, int64_t n, BuildCtrl* c, const 
   hipLaunchKernelGGL((k_final<MODE, G>),
                      dim3((unsigned)std::min<int64_t>((n + kFinTile - 1) / kFinTile,
                                                       CT_FINAL_GRID)),
-                     dim3(256), 0, s, n, c, b.kv[0], b.kv[1], b.lo[0], b.lo[1], b.hi[0], b.hi[1],
-                     box->xmin, box->ymin, box->xmax, box->ymax, box->id, tr, l0start, r_sz, b.runs,
-                     b.runs_cap);
+                     dim3(256), 0, s, n, c, (const uint32_t*)b.ids, b.lo[0], b.lo[1], b.hi[0],
+                     b.hi[1], box->xmin, box->ymin, box->xmax, box->ymax, box->id, tr, l0start,
+                     r_sz, b.runs, b.runs_cap);
   if (MODE != kKeyMorton64) return;
   pf.mark(s, "k_fixup_long");
-  hipLaunchKernelGGL(k_fixup_long, dim3(kFixLongSlots), dim3(256), 0, s, n, c, b.kv[0], b.kv[1],
-                     b.lo[0], b.lo[1], b.hi[0], b.hi[1], box->xmin, box->ymin, box->xmax, box->ymax,
-                     box->id, b.runs, b.runs_cap);
-  if (G > 0)
+  hipLaunchKernelGGL(k_fixup_long, dim3(kFixLongSlots), dim3(256), 0, s, n, c,
+                     (const uint32_t*)b.ids, b.lo[0], b.lo[1], b.hi[0], b.hi[1], box->xmin,
+                     box->ymin, box->xmax, box->ymax, box->id, b.runs, b.runs_cap);
+  if constexpr (G > 0)
     hipLaunchKernelGGL(k_fixup_groups<G>, dim3(kFixLongSlots), dim3(256), 0, s, n,
                        (const BuildCtrl*)c, (const uint32_t*)b.runs, b.runs_cap, box->xmin,
                        box->ymin, box->xmax, box->ymax, tr, l0start, r_sz);
@@ -1290,8 +1298,7 @@ int build_f32(Context& ctx, curtree_bbox_f32* box, curtree_rtree_f32* rt, int ke
   // the fallback reuses them for 64-bit keys and 32-B staging records
   if ((rc = ws.reserve(WsBuildKeys0, n * 8)) || (rc = ws.reserve(WsBuildKeys1, n * 8)) ||
       (rc = ws.reserve(WsBuildAos, n * 32)) ||
-      (rc = ws.reserve(WsBuildVals0, m64 ? n * 4 : 4)) ||
-      (rc = ws.reserve(WsBuildVals1, m64 ? n * 4 : 4)) ||
+      (rc = ws.reserve(WsBuildVals0, n * 4)) || (rc = ws.reserve(WsBuildVals1, n * 4)) ||
       (rc = ws.reserve(WsBuildStatus, status_words * 4)) ||
       (rc = ws.reserve(WsBuildCtrl, sizeof(BuildCtrl))))
     return fail(rc);
@@ -1312,6 +1319,8 @@ int build_f32(Context& ctx, curtree_bbox_f32* box, curtree_rtree_f32* rt, int ke
   sb.k64[1] = sb.kv[1];
   sb.v[0] = (uint32_t*)ws.ptr(WsBuildVals0);
   sb.v[1] = (uint32_t*)ws.ptr(WsBuildVals1);
+  sb.ids = sb.v[0];   // (the fallback's positions come after the final pass)
+  sb.keys = sb.v[1];
   sb.aos = (uint4*)ws.ptr(WsBuildAos);
   sb.status = (uint32_t*)ws.ptr(WsBuildStatus);
   sb.step_words = step_words;
@@ -1327,27 +1336,24 @@ int build_f32(Context& ctx, curtree_bbox_f32* box, curtree_rtree_f32* rt, int ke
     hipLaunchKernelGGL(k_extent, dim3(sgrid), dim3(256), 0, s, n, box->xmin, box->ymin,
                        box->xmax, box->ymax, c);
   }
-  // histograms of the 32-bit sort key; clears the status words of steps 0..3
+  // sort keys and their histograms; clears the status words of steps 0..3
   pf.mark(s, "k_hist");
   if (m64)
     hipLaunchKernelGGL(k_hist<kKeyMorton64>, dim3(sgrid), dim3(256), 0, s, n, box->xmin,
-                       box->ymin, box->xmax, box->ymax, c, sb.status, (int64_t)(4 * step_words));
+                       box->ymin, box->xmax, box->ymax, c, sb.keys, sb.status,
+                       (int64_t)(4 * step_words));
   else
     hipLaunchKernelGGL(k_hist<kKeyRef32>, dim3(sgrid), dim3(256), 0, s, n, box->xmin, box->ymin,
-                       box->xmax, box->ymax, c, sb.status, (int64_t)(4 * step_words));
+                       box->xmax, box->ymax, c, sb.keys, sb.status, (int64_t)(4 * step_words));
+  // (the last step that runs writes the ids alone)
   pf.mark(s, "k_sort_plan");
-  hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(256), 0, s, n, 4, c, 0, 4, 0x3210ull, 1, 0);
+  hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(256), 0, s, n, 4, c, 0, 4, 0x3210ull, 0, 0);
   for (int st = 0; st < 4; st++) {
     pf.mark(s, "k_onesweep");
-    uint32_t* stw = sb.status + (size_t)st * step_words;
-    if (m64)
-      hipLaunchKernelGGL(k_sweep_kv<kKeyMorton64>, dim3((unsigned)ntiles), dim3(kSortThreads), 0,
-                         s, n, st, box->xmin, box->ymin, box->xmax, box->ymax, box->id, sb.kv[0],
-                         sb.kv[1], sb.lo[0], sb.lo[1], sb.hi[0], sb.hi[1], c, stw);
-    else
-      hipLaunchKernelGGL(k_sweep_kv<kKeyRef32>, dim3((unsigned)ntiles), dim3(kSortThreads), 0, s,
-                         n, st, box->xmin, box->ymin, box->xmax, box->ymax, box->id, sb.kv[0],
-                         sb.kv[1], sb.lo[0], sb.lo[1], sb.hi[0], sb.hi[1], c, stw);
+    hipLaunchKernelGGL(k_sweep_kv, dim3((unsigned)ntiles), dim3(kSortThreads), 0, s, n, st,
+                       box->xmin, box->ymin, box->xmax, box->ymax, box->id,
+                       (const uint32_t*)sb.keys, sb.kv[0], sb.kv[1], sb.ids, sb.lo[0], sb.lo[1],
+                       sb.hi[0], sb.hi[1], c, sb.status + (size_t)st * step_words);
   }
 
   TreeView tr{rt->xmin, rt->ymin, rt->xmax, rt->ymax, rt->pos, rt->len};
