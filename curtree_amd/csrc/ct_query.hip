@@ -314,7 +314,6 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_count(
     unsigned long long* counts) {
   using S = QShape<G, STEPS>;
   constexpr int P = S::kPerTile;
-  __shared__ uint32_t s_tile;
   __shared__ uint64_t s_red[kQWaves];
   if (c->overflow) return;
   const int64_t n_in = (int64_t)c->count[slot];
@@ -329,8 +328,9 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_count(
   const bool canon = tr.canon && *tr.canon;
   const int64_t ls = lv->lstart[0];
   uint64_t block_total = 0;  // thread 0: one atomic per block, not per tile
-  int64_t tile;
-  while ((tile = next_tile(&c->ticket[slot], ntiles, &s_tile)) >= 0) {
+  // tiles are independent (no chain): a fixed stride over them, no ticket
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    __syncthreads();  // the previous tile is done with the shared words
     uint64_t wsum = 0;
     const int64_t tbase = tile * P;
 #pragma unroll
@@ -463,7 +463,6 @@ __global__ __launch_bounds__(kQThreads) void k_expand_count(int slot, QTree tr, 
                                                             int64_t r_sz) {
   using S = QShape<G, STEPS>;
   constexpr int P = S::kPerTile;
-  __shared__ uint32_t s_tile;
   __shared__ uint64_t s_red[kQWaves];
   if (c->overflow) return;
   const int64_t n_in = (int64_t)c->count[slot];
@@ -481,8 +480,9 @@ __global__ __launch_bounds__(kQThreads) void k_expand_count(int slot, QTree tr, 
   const int64_t cs = lvl > 0 ? c->lv.lstart[lvl - 1] : 0;
   const int64_t cc = lvl > 0 ? c->lv.lcount[lvl - 1] : 0;
   uint64_t block_total = 0;
-  int64_t tile;
-  while ((tile = next_tile(&c->ticket[slot], ntiles, &s_tile)) >= 0) {
+  // tiles are independent (no chain): a fixed stride over them, no ticket
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    __syncthreads();  // the previous tile is done with the shared words
     uint64_t wsum = 0;
     const int64_t tbase = tile * P;
 #pragma unroll
@@ -1014,6 +1014,62 @@ int compute_safe(QEngine& qe, Prof& pf, const QBoxes& rb, int F, int64_t r_sz, i
 // Runs of at least this many queries expand without a tile chain.
 constexpr int64_t kSplitMinQueries = 32768;
 
+// Blocks of `kernel` (kQThreads threads) that the device holds at once.  The
+// chain-free count kernels stride over their tiles with a fixed step, so
+// they are launched with no more blocks than can run together (a block that
+// starts late would add its whole share of tiles to the end of the launch).
+static unsigned resident_grid(const void* kernel) {
+  int dev = 0, cus = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kQThreads, 0) != hipSuccess ||
+      cus <= 0 || per <= 0) {
+    (void)hipGetLastError();
+    return 1024;
+  }
+  return (unsigned)(cus * per);
+}
+
+template <int G, int STEPS, bool STORE>
+static unsigned leaf_count_grid() {
+  static const unsigned g = resident_grid((const void*)k_leaf_count<G, STEPS, STORE>);
+  return g;
+}
+
+template <int G, int STEPS>
+static unsigned expand_count_grid() {
+  static const unsigned g = resident_grid((const void*)k_expand_count<G, STEPS>);
+  return g;
+}
+
+// Queries the occupancy of the count kernels for (G, steps) ahead of any
+// graph capture (the first query of each is cached).
+template <int G>
+static void warm_grids_g(int steps) {
+  switch (steps) {
+#define CT_WARM(st)                          \
+  case st:                                   \
+    (void)expand_count_grid<G, st>();        \
+    (void)leaf_count_grid<G, st, true>();    \
+    (void)leaf_count_grid<G, st, false>();   \
+    break;
+    CT_WARM(4) CT_WARM(16)
+    default: CT_WARM(8)
+#undef CT_WARM
+  }
+}
+
+static void warm_grids(int G, int steps) {
+  switch (G) {
+    case 2: warm_grids_g<2>(steps); break;
+    case 4: warm_grids_g<4>(steps); break;
+    case 8: warm_grids_g<8>(steps); break;
+    case 16: warm_grids_g<16>(steps); break;
+    case 32: warm_grids_g<32>(steps); break;
+    default: warm_grids_g<64>(steps); break;
+  }
+}
+
 template <int G, int STEPS>
 void enqueue_run_g(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
   using S = QShape<G, STEPS>;
@@ -1038,7 +1094,9 @@ void enqueue_run_g(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
     uint2* out = (uint2*)qe.front[which];
     if (split) {
       pf.mark(s, "k_expand_count");
-      hipLaunchKernelGGL((k_expand_count<G, STEPS>), dim3(a.grid), dim3(kQThreads), 0, s, l,
+      hipLaunchKernelGGL((k_expand_count<G, STEPS>),
+                         dim3(std::min<unsigned>(a.grid, expand_count_grid<G, STEPS>())),
+                         dim3(kQThreads), 0, s, l,
                          a.tr, a.qb, qe.ctrl, cur, (QExp*)qe.est, qe.ttot, a.r_sz);
       pf.mark(s, "k_tile_scan");
       hipLaunchKernelGGL(k_tile_scan, dim3(64), dim3(kQThreads), 0, s, a.nl + 1 + l, l,
@@ -1058,7 +1116,9 @@ void enqueue_run_g(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
   const int leaf = a.nl - 1;
   if (a.write) {
     pf.mark(s, "k_leaf_pass");
-    hipLaunchKernelGGL((k_leaf_count<G, STEPS, true>), dim3(a.grid), dim3(kQThreads), 0, s, leaf,
+    hipLaunchKernelGGL((k_leaf_count<G, STEPS, true>),
+                       dim3(std::min<unsigned>(a.grid, leaf_count_grid<G, STEPS, true>())),
+                       dim3(kQThreads), 0, s, leaf,
                        a.tr, a.qb, a.rb, qe.ctrl, cur, a.r_sz, a.nbox, (QLeaf*)qe.est, qe.ttot,
                        a.counts);
     pf.mark(s, "k_tile_scan");
@@ -1074,7 +1134,9 @@ void enqueue_run_g(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
                        a.out_cap);
   } else {
     pf.mark(s, "k_leaf_count");
-    hipLaunchKernelGGL((k_leaf_count<G, STEPS, false>), dim3(a.grid), dim3(kQThreads), 0, s, leaf,
+    hipLaunchKernelGGL((k_leaf_count<G, STEPS, false>),
+                       dim3(std::min<unsigned>(a.grid, leaf_count_grid<G, STEPS, false>())),
+                       dim3(kQThreads), 0, s, leaf,
                        a.tr, a.qb, a.rb, qe.ctrl, cur, a.r_sz, a.nbox, (QLeaf*)nullptr,
                        (uint64_t*)nullptr, a.counts);
   }
@@ -1143,6 +1205,7 @@ int prepare_query(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curt
   a->nbox = N;
   a->F = F;
   a->G = next_pow2_group(F);
+  warm_grids(a->G, a->steps);
   if (N == 0) return CURTREE_OK;
   LevelPlan lp;
   level_plan(N, F, &lp);
