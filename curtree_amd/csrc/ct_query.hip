@@ -314,77 +314,116 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_count(
     unsigned long long* counts) {
   using S = QShape<G, STEPS>;
   constexpr int P = S::kPerTile;
+  // memory round trips per tile: the entries of all steps, then the windows
+  // and boxes of H steps at a time
+  constexpr int H = STEPS < 4 ? STEPS : 4;
+  static_assert(STEPS % H == 0, "steps in whole groups");
   __shared__ uint64_t s_red[kQWaves];
+  __shared__ QLevels s_lv;  // the level table (contained ranges)
   if (c->overflow) return;
   const int64_t n_in = (int64_t)c->count[slot];
   const int64_t q0 = c->q0;
-  const QLevels* lv = &c->lv;
   const int64_t ntiles = (n_in + P - 1) / P;
   const int t = threadIdx.x, w = t >> 6, lane = lane_id();
   if (ntiles == 0) return;  // the run's total stays 0
+  for (int i = t; i < (int)(sizeof(QLevels) / 4); i += kQThreads)
+    ((uint32_t*)&s_lv)[i] = ((const uint32_t*)&c->lv)[i];
   const int sub = lane / G, c_ = lane % G;
   const int lead = sub * G;  // first lane of this lane's group
   // bulk-load layout: leaf-parent i holds sorted boxes i*F .. i*F+F-1
   const bool canon = tr.canon && *tr.canon;
-  const int64_t ls = lv->lstart[0];
+  const int64_t ls = c->lv.lstart[0];
   uint64_t block_total = 0;  // thread 0: one atomic per block, not per tile
   // tiles are independent (no chain): a fixed stride over them, no ticket
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    __syncthreads();  // the previous tile is done with the shared words
+    __syncthreads();  // the previous tile is done with the shared words (and s_lv is in)
     uint64_t wsum = 0;
     const int64_t tbase = tile * P;
+    // the tile's entries (a position past the frontier reads its last entry,
+    // unused)
+    uint2 ent[STEPS];
+    if (fin) {
 #pragma unroll
-    for (int k = 0; k < STEPS; k++) {
-      const int64_t e = tbase + w * S::kPerWave + k * S::E + sub;
-      bool hit = false, full = false;
-      uint32_t q = 0;
-      int64_t first = 0, rcnt = 0;
-      if (e < n_in) {
-        uint32_t node = 0;
-        if (fin) {
-          const uint2 en = fin[e];
-          q = en.x;
-          node = en.y & ~kFull;
-          full = (en.y & kFull) != 0;
+      for (int k = 0; k < STEPS; k++) {
+        const int64_t e = tbase + w * S::kPerWave + k * S::E + sub;
+        ent[k] = fin[e < n_in ? e : n_in - 1];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < STEPS; k++) {
+        const int64_t e = tbase + w * S::kPerWave + k * S::E + sub;
+        ent[k] = make_uint2((uint32_t)(q0 + (e < n_in ? e : n_in - 1)), 0u);
+      }
+    }
+#pragma unroll
+    for (int k0 = 0; k0 < STEPS; k0 += H) {
+      // the box range of each entry
+      uint32_t q[H];
+      int64_t first[H], cnt[H];  // contained: range length; partial: children
+      bool full[H];
+      const uint2* en = ent + k0;
+#pragma unroll
+      for (int k = 0; k < H; k++) {
+        q[k] = en[k].x;
+        const uint32_t node = en[k].y & ~kFull;
+        full[k] = (en[k].y & kFull) != 0;
+        if (full[k]) {
+          node_range(node, &s_lv, nbox, &first[k], &cnt[k]);
         } else {
-          q = (uint32_t)(q0 + e);
-        }
-        if (full) {
-          node_range(node, lv, nbox, &first, &rcnt);
-        } else {
-          int64_t L;
-          if (canon) {
-            first = ((int64_t)node - ls) * tr.F;
-            L = nbox - first < tr.F ? nbox - first : tr.F;
-          } else {
-            first = (int64_t)tr.pos[node] - r_sz;
-            L = tr.len[node];
-          }
-          const int64_t bi = first + c_;
-          if (c_ < L && bi >= 0 && bi < nbox) {  // children must be boxes
-            const float4 wq = ld_box(qb, q);
-            const float4 bx = ld_box(rb, bi);
-            hit = intersect(wq.x, wq.y, wq.z, wq.w, bx.x, bx.y, bx.z, bx.w);
-          }
+          first[k] = ((int64_t)node - ls) * tr.F;
+          cnt[k] = nbox - first[k] < tr.F ? nbox - first[k] : tr.F;
         }
       }
-      const uint64_t m = ballot(hit);
-      const uint64_t gm = (m >> lead) & S::kGroupMask;
-      const uint64_t cnt = full ? (uint64_t)rcnt : (uint64_t)popc64(gm);
-      if (c_ == 0 && e < n_in) {
-        if (STORE) {
-          QLeaf x;
-          x.val = full ? cnt : gm;
-          x.first = (uint32_t)first;
-          x.qf = q | (full ? kQRange : 0u);
-          est[e] = x;
-        }
-        if (counts && cnt) atomicAdd(counts + q, (unsigned long long)cnt);
-      }
-      uint64_t v = c_ == 0 ? cnt : 0ull;
+      if (!canon) {  // any other layout: the node's pos/len
+        int pos[H], len[H];
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-      wsum += v;
+        for (int k = 0; k < H; k++) {
+          const uint32_t node = en[k].y & ~kFull;
+          pos[k] = tr.pos[node];
+          len[k] = tr.len[node];
+        }
+#pragma unroll
+        for (int k = 0; k < H; k++)
+          if (!full[k]) {
+            first[k] = (int64_t)pos[k] - r_sz;
+            cnt[k] = len[k];
+          }
+      }
+      // windows and boxes (a lane with no child reads box 0, unused)
+      float4 wq[H], bx[H];
+      bool test[H];
+#pragma unroll
+      for (int k = 0; k < H; k++) {
+        const int64_t bi = first[k] + c_;
+        test[k] = !full[k] && c_ < cnt[k] && bi >= 0 && bi < nbox;  // children must be boxes
+        wq[k] = ld_box(qb, q[k]);
+        bx[k] = ld_box(rb, test[k] ? bi : 0);
+      }
+#pragma unroll
+      for (int k = 0; k < H; k++) {
+        const int64_t e = tbase + w * S::kPerWave + (k0 + k) * S::E + sub;
+        const bool valid = e < n_in;
+        const bool hit = valid && test[k] &&
+                         intersect(wq[k].x, wq[k].y, wq[k].z, wq[k].w, bx[k].x, bx[k].y, bx[k].z,
+                                   bx[k].w);
+        const uint64_t m = ballot(hit);
+        const uint64_t gm = (m >> lead) & S::kGroupMask;
+        const uint64_t nh = full[k] ? (uint64_t)cnt[k] : (uint64_t)popc64(gm);
+        if (c_ == 0 && valid) {
+          if (STORE) {
+            QLeaf x;
+            x.val = full[k] ? nh : gm;
+            x.first = (uint32_t)first[k];
+            x.qf = q[k] | (full[k] ? kQRange : 0u);
+            est[e] = x;
+          }
+          if (counts && nh) atomicAdd(counts + q[k], (unsigned long long)nh);
+        }
+        uint64_t v = (c_ == 0 && valid) ? nh : 0ull;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        wsum += v;
+      }
     }
     if (lane == 0) s_red[w] = wsum;
     __syncthreads();
@@ -463,6 +502,10 @@ __global__ __launch_bounds__(kQThreads) void k_expand_count(int slot, QTree tr, 
                                                             int64_t r_sz) {
   using S = QShape<G, STEPS>;
   constexpr int P = S::kPerTile;
+  // memory round trips per tile: the entries of all steps, then the windows,
+  // child MBRs and flags of H steps at a time
+  constexpr int H = STEPS < 4 ? STEPS : 4;
+  static_assert(STEPS % H == 0, "steps in whole groups");
   __shared__ uint64_t s_red[kQWaves];
   if (c->overflow) return;
   const int64_t n_in = (int64_t)c->count[slot];
@@ -475,6 +518,7 @@ __global__ __launch_bounds__(kQThreads) void k_expand_count(int slot, QTree tr, 
   // bulk-load layout: the entries of this launch sit on level nl-1-slot and
   // node i's children are nodes cs + i*F .. of the level below
   const bool canon = tr.canon && *tr.canon;
+  const bool shortcut = canon && tr.safe != nullptr;
   const int lvl = c->lv.nl - 1 - slot;
   const int64_t ls = c->lv.lstart[lvl];
   const int64_t cs = lvl > 0 ? c->lv.lstart[lvl - 1] : 0;
@@ -485,62 +529,95 @@ __global__ __launch_bounds__(kQThreads) void k_expand_count(int slot, QTree tr, 
     __syncthreads();  // the previous tile is done with the shared words
     uint64_t wsum = 0;
     const int64_t tbase = tile * P;
+    // the tile's entries (a position past the frontier reads its last entry,
+    // unused)
+    uint2 ent[STEPS];
+    if (fin) {
 #pragma unroll
-    for (int k = 0; k < STEPS; k++) {
-      const int64_t e = tbase + w * S::kPerWave + k * S::E + sub;
-      bool hit = false, cont = false;
-      uint32_t q = 0, first = 0;
-      if (e < n_in) {
-        uint32_t node = 0, full = 0;
-        if (fin) {
-          const uint2 en = fin[e];
-          q = en.x;
-          node = en.y & ~kFull;
-          full = en.y & kFull;
-        } else {
-          q = (uint32_t)(q0 + e);
+      for (int k = 0; k < STEPS; k++) {
+        const int64_t e = tbase + w * S::kPerWave + k * S::E + sub;
+        ent[k] = fin[e < n_in ? e : n_in - 1];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < STEPS; k++) {
+        const int64_t e = tbase + w * S::kPerWave + k * S::E + sub;
+        ent[k] = make_uint2((uint32_t)(q0 + (e < n_in ? e : n_in - 1)), 0u);
+      }
+    }
+#pragma unroll
+    for (int k0 = 0; k0 < STEPS; k0 += H) {
+      const uint2* en = ent + k0;
+      // children of each entry (a contained entry is its own single child)
+      int64_t p[H], L[H];
+#pragma unroll
+      for (int k = 0; k < H; k++) {
+        const int64_t f = ((int64_t)(en[k].y & ~kFull) - ls) * tr.F;
+        p[k] = cs + f;
+        L[k] = cc - f < tr.F ? cc - f : tr.F;
+      }
+      if (!canon) {  // any other layout: the node's pos/len
+        int pos[H], len[H];
+#pragma unroll
+        for (int k = 0; k < H; k++) {
+          const uint32_t node = en[k].y & ~kFull;
+          pos[k] = tr.pos[node];
+          len[k] = tr.len[node];
         }
+#pragma unroll
+        for (int k = 0; k < H; k++) {
+          p[k] = pos[k];
+          L[k] = len[k];
+        }
+      }
+      // windows, child MBRs and well-formed flags (a lane with no child
+      // reads node 0, unused)
+      float4 wq[H], mb[H];
+      uint8_t sf[H];
+      bool test[H];
+#pragma unroll
+      for (int k = 0; k < H; k++) {
+        const bool full = (en[k].y & kFull) != 0;
+        test[k] = !full && c_ < L[k] && p[k] + L[k] <= r_sz;  // children must be nodes
+        const uint32_t child = test[k] ? (uint32_t)(p[k] + c_) : 0u;
+        wq[k] = ld_box(qb, en[k].x);
+        mb[k] = ld_box(tr.mbr, child);
+        sf[k] = shortcut ? tr.safe[child] : 0;
+      }
+#pragma unroll
+      for (int k = 0; k < H; k++) {
+        const int64_t e = tbase + w * S::kPerWave + (k0 + k) * S::E + sub;
+        const bool valid = e < n_in;
+        const bool full = (en[k].y & kFull) != 0;
+        bool hit, cont;
         if (full) {
-          hit = c_ == 0;
+          hit = valid && c_ == 0;
           cont = hit;
-          first = node;
         } else {
-          int64_t p, L;
-          if (canon) {
-            const int64_t f = ((int64_t)node - ls) * tr.F;
-            p = cs + f;
-            L = cc - f < tr.F ? cc - f : tr.F;
-          } else {
-            p = tr.pos[node];
-            L = tr.len[node];
-          }
-          first = (uint32_t)p;
-          if (c_ < L && p + L <= r_sz) {  // children must be nodes
-            const uint32_t child = (uint32_t)(p + c_);
-            const float4 wq = ld_box(qb, q);
-            const float4 mb = ld_box(tr.mbr, child);
-            hit = intersect(wq.x, wq.y, wq.z, wq.w, mb.x, mb.y, mb.z, mb.w);
-            // (a contained subtree's box range comes from the level table,
-            // so the shortcut needs the bulk-load layout)
-            cont = hit && canon && tr.safe &&
-                   contains(wq.x, wq.y, wq.z, wq.w, mb.x, mb.y, mb.z, mb.w) && tr.safe[child];
-          }
+          hit = valid && test[k] &&
+                intersect(wq[k].x, wq[k].y, wq[k].z, wq[k].w, mb[k].x, mb[k].y, mb[k].z, mb[k].w);
+          // (a contained subtree's box range comes from the level table, so
+          // the shortcut needs the bulk-load layout)
+          cont = hit && shortcut &&
+                 contains(wq[k].x, wq[k].y, wq[k].z, wq[k].w, mb[k].x, mb[k].y, mb[k].z,
+                          mb[k].w) &&
+                 sf[k];
         }
-      }
-      const uint64_t gm = (ballot(hit) >> lead) & S::kGroupMask;
-      const uint64_t fm = (ballot(cont) >> lead) & S::kGroupMask;
-      if (c_ == 0 && e < n_in) {
-        QExp x;
-        x.hit = gm;
-        x.full = fm;
-        x.q = q;
-        x.first = first;
-        est[e] = x;
-      }
-      uint64_t v = c_ == 0 ? (uint64_t)popc64(gm) : 0ull;
+        const uint64_t gm = (ballot(hit) >> lead) & S::kGroupMask;
+        const uint64_t fm = (ballot(cont) >> lead) & S::kGroupMask;
+        if (c_ == 0 && valid) {
+          QExp x;
+          x.hit = gm;
+          x.full = fm;
+          x.q = en[k].x;
+          x.first = full ? (en[k].y & ~kFull) : (uint32_t)p[k];
+          est[e] = x;
+        }
+        uint64_t v = (c_ == 0 && valid) ? (uint64_t)popc64(gm) : 0ull;
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-      wsum += v;
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        wsum += v;
+      }
     }
     if (lane == 0) s_red[w] = wsum;
     __syncthreads();
