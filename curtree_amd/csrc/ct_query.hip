@@ -494,6 +494,9 @@ __global__ __launch_bounds__(kQThreads) void k_tile_scan(int slot, int cslot, in
 //                   (query, node) order.
 // A contained subtree travels down as one FULL entry (its box range is taken
 // at the leaf stage); it survives every level as its own single child.
+#ifndef CT_EXPAND_H
+#define CT_EXPAND_H 2
+#endif
 template <int G, int STEPS>
 __global__ __launch_bounds__(kQThreads) void k_expand_count(int slot, QTree tr, QBoxes qb,
                                                             QCtrl* c, const uint2* __restrict__ fin,
@@ -504,7 +507,7 @@ __global__ __launch_bounds__(kQThreads) void k_expand_count(int slot, QTree tr, 
   constexpr int P = S::kPerTile;
   // memory round trips per tile: the entries of all steps, then the windows,
   // child MBRs and flags of H steps at a time
-  constexpr int H = STEPS < 4 ? STEPS : 4;
+  constexpr int H = STEPS < CT_EXPAND_H ? STEPS : CT_EXPAND_H;
   static_assert(STEPS % H == 0, "steps in whole groups");
   __shared__ uint64_t s_red[kQWaves];
   if (c->overflow) return;
@@ -797,23 +800,39 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_write(
       }
     }
     // flat slots, load-balanced over the block: slot j belongs to the last
-    // entry whose flat offset is <= j (binary search), coalesced writes
+    // entry whose flat offset is <= j (binary search), coalesced writes; the
+    // box ids of kWriteU slots per thread are loaded together
+    constexpr int kWriteU = 8;
     const uint32_t fn = s_ftot;
-#pragma unroll 4
-    for (uint32_t j = t; j < fn; j += kQThreads) {
-      int lo = 0;
+    for (uint32_t j0 = 0; j0 < fn; j0 += kWriteU * kQThreads) {
+      uint64_t o[kWriteU];
+      int64_t bi[kWriteU];
+      uint32_t qv[kWriteU];
 #pragma unroll
-      for (int step = P / 2; step > 0; step >>= 1)
-        if (s_foff[lo + step] <= j) lo += step;
-      const uint32_t qf = s_q[lo];
-      const uint32_t r = j - s_foff[lo];
-      uint32_t bpos = r;
-      if (!(qf & kQRange)) bpos = nth_set_bit(s_val[lo], r);
-      const uint64_t o = base + s_off[lo] + r;
-      if (o < (uint64_t)out_cap) {
-        out_fid[o] = rid[(int64_t)s_first[lo] + bpos];
-        out_tid[o] = (int)(qf & ~kQRange);
+      for (int u = 0; u < kWriteU; u++) {
+        const uint32_t j = j0 + u * kQThreads + t;
+        const uint32_t jc = j < fn ? j : fn - 1;
+        int lo = 0;
+#pragma unroll
+        for (int step = P / 2; step > 0; step >>= 1)
+          if (s_foff[lo + step] <= jc) lo += step;
+        const uint32_t qf = s_q[lo];
+        const uint32_t r = jc - s_foff[lo];
+        uint32_t bpos = r;
+        if (!(qf & kQRange)) bpos = nth_set_bit(s_val[lo], r);
+        o[u] = j < fn ? base + s_off[lo] + r : ~0ull;
+        bi[u] = (int64_t)s_first[lo] + bpos;
+        qv[u] = qf & ~kQRange;
       }
+      int idv[kWriteU];
+#pragma unroll
+      for (int u = 0; u < kWriteU; u++) idv[u] = rid[bi[u]];
+#pragma unroll
+      for (int u = 0; u < kWriteU; u++)
+        if (o[u] < (uint64_t)out_cap) {
+          out_fid[o[u]] = idv[u];
+          out_tid[o[u]] = (int)qv[u];
+        }
     }
   }
 }
@@ -1092,9 +1111,10 @@ int compute_safe(QEngine& qe, Prof& pf, const QBoxes& rb, int F, int64_t r_sz, i
 constexpr int64_t kSplitMinQueries = 32768;
 
 // Blocks of `kernel` (kQThreads threads) that the device holds at once.  The
-// chain-free count kernels stride over their tiles with a fixed step, so
-// they are launched with no more blocks than can run together (a block that
-// starts late would add its whole share of tiles to the end of the launch).
+// chain-free count and write kernels stride over their tiles with a fixed
+// step, so they are launched with no more blocks than can run together (a
+// block that starts late would add its whole share of tiles to the end of
+// the launch).
 static unsigned resident_grid(const void* kernel) {
   int dev = 0, cus = 0, per = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
@@ -1119,6 +1139,18 @@ static unsigned expand_count_grid() {
   return g;
 }
 
+template <int G, int STEPS>
+static unsigned expand_write_grid() {
+  static const unsigned g = resident_grid((const void*)k_expand_write<G, STEPS>);
+  return g;
+}
+
+template <int G, int STEPS>
+static unsigned leaf_write_grid() {
+  static const unsigned g = resident_grid((const void*)k_leaf_write<G, STEPS>);
+  return g;
+}
+
 // Queries the occupancy of the count kernels for (G, steps) ahead of any
 // graph capture (the first query of each is cached).
 template <int G>
@@ -1127,8 +1159,10 @@ static void warm_grids_g(int steps) {
 #define CT_WARM(st)                          \
   case st:                                   \
     (void)expand_count_grid<G, st>();        \
+    (void)expand_write_grid<G, st>();        \
     (void)leaf_count_grid<G, st, true>();    \
     (void)leaf_count_grid<G, st, false>();   \
+    (void)leaf_write_grid<G, st>();          \
     break;
     CT_WARM(4) CT_WARM(16)
     default: CT_WARM(8)
@@ -1179,7 +1213,9 @@ void enqueue_run_g(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
       hipLaunchKernelGGL(k_tile_scan, dim3(64), dim3(kQThreads), 0, s, a.nl + 1 + l, l,
                          S::kPerTile, qe.ctrl, qe.ttot, qe.status);
       pf.mark(s, "k_expand_write");
-      hipLaunchKernelGGL((k_expand_write<G, STEPS>), dim3(a.grid), dim3(kQThreads), 0, s, l,
+      hipLaunchKernelGGL((k_expand_write<G, STEPS>),
+                         dim3(std::min<unsigned>(a.grid, expand_write_grid<G, STEPS>())),
+                         dim3(kQThreads), 0, s, l,
                          qe.ctrl, (const QExp*)qe.est, (const uint64_t*)qe.ttot, out,
                          qe.cap_front);
     } else {
@@ -1202,7 +1238,9 @@ void enqueue_run_g(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_tile_scan, dim3(64), dim3(kQThreads), 0, s, leaf + 1, leaf, S::kPerTile,
                        qe.ctrl, qe.ttot, qe.status);
     pf.mark(s, "k_leaf_write");
-    hipLaunchKernelGGL((k_leaf_write<G, STEPS>), dim3(a.grid), dim3(kQThreads), 0, s, leaf,
+    hipLaunchKernelGGL((k_leaf_write<G, STEPS>),
+                       dim3(std::min<unsigned>(a.grid, leaf_write_grid<G, STEPS>())),
+                       dim3(kQThreads), 0, s, leaf,
                        qe.ctrl, (const QLeaf*)qe.est, (const uint64_t*)qe.ttot, a.rid, a.out_fid,
                        a.out_tid, a.out_cap, (QPiece*)qe.pieces, qe.cap_pieces);
     pf.mark(s, "k_write_pieces");
