@@ -358,8 +358,9 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_count(
 #pragma unroll
     for (int k0 = 0; k0 < STEPS; k0 += H) {
       // the box range of each entry
+      // (box positions and counts fit 32 bits: a tree holds < 2^30 boxes)
       uint32_t q[H];
-      int64_t first[H], cnt[H];  // contained: range length; partial: children
+      int first[H], cnt[H];  // contained: range length; partial: children
       bool full[H];
       const uint2* en = ent + k0;
 #pragma unroll
@@ -368,10 +369,14 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_count(
         const uint32_t node = en[k].y & ~kFull;
         full[k] = (en[k].y & kFull) != 0;
         if (full[k]) {
-          node_range(node, &s_lv, nbox, &first[k], &cnt[k]);
+          int64_t f, n;
+          node_range(node, &s_lv, nbox, &f, &n);
+          first[k] = (int)f;
+          cnt[k] = (int)n;
         } else {
-          first[k] = ((int64_t)node - ls) * tr.F;
-          cnt[k] = nbox - first[k] < tr.F ? nbox - first[k] : tr.F;
+          const int64_t f = ((int64_t)node - ls) * tr.F;
+          first[k] = (int)f;
+          cnt[k] = (int)(nbox - f < tr.F ? nbox - f : tr.F);
         }
       }
       if (!canon) {  // any other layout: the node's pos/len
@@ -385,7 +390,7 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_count(
 #pragma unroll
         for (int k = 0; k < H; k++)
           if (!full[k]) {
-            first[k] = (int64_t)pos[k] - r_sz;
+            first[k] = (int)((int64_t)pos[k] - r_sz);
             cnt[k] = len[k];
           }
       }
@@ -394,7 +399,7 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_count(
       bool test[H];
 #pragma unroll
       for (int k = 0; k < H; k++) {
-        const int64_t bi = first[k] + c_;
+        const int bi = first[k] + c_;
         test[k] = !full[k] && c_ < cnt[k] && bi >= 0 && bi < nbox;  // children must be boxes
         wq[k] = ld_box(qb, q[k]);
         bx[k] = ld_box(rb, test[k] ? bi : 0);
@@ -552,12 +557,13 @@ __global__ __launch_bounds__(kQThreads) void k_expand_count(int slot, QTree tr, 
     for (int k0 = 0; k0 < STEPS; k0 += H) {
       const uint2* en = ent + k0;
       // children of each entry (a contained entry is its own single child)
-      int64_t p[H], L[H];
+      // (node indices fit 32 bits)
+      int p[H], L[H];
 #pragma unroll
       for (int k = 0; k < H; k++) {
         const int64_t f = ((int64_t)(en[k].y & ~kFull) - ls) * tr.F;
-        p[k] = cs + f;
-        L[k] = cc - f < tr.F ? cc - f : tr.F;
+        p[k] = (int)(cs + f);
+        L[k] = (int)(cc - f < tr.F ? cc - f : tr.F);
       }
       if (!canon) {  // any other layout: the node's pos/len
         int pos[H], len[H];
@@ -581,7 +587,7 @@ __global__ __launch_bounds__(kQThreads) void k_expand_count(int slot, QTree tr, 
 #pragma unroll
       for (int k = 0; k < H; k++) {
         const bool full = (en[k].y & kFull) != 0;
-        test[k] = !full && c_ < L[k] && p[k] + L[k] <= r_sz;  // children must be nodes
+        test[k] = !full && c_ < L[k] && (int64_t)p[k] + L[k] <= r_sz;  // children must be nodes
         const uint32_t child = test[k] ? (uint32_t)(p[k] + c_) : 0u;
         wq[k] = ld_box(qb, en[k].x);
         mb[k] = ld_box(tr.mbr, child);
@@ -642,7 +648,11 @@ __global__ __launch_bounds__(kQThreads) void k_expand_write(int slot, QCtrl* c,
                                                             const uint64_t* __restrict__ tbase_arr,
                                                             uint2* __restrict__ fout, int64_t cap) {
   using S = QShape<G, STEPS>;
-  constexpr int P = S::kPerTile;
+  // a write tile is M count tiles (at least one entry per thread): its base is
+  // the first count tile's scanned total
+  constexpr int PC = S::kPerTile;
+  constexpr int M = PC < kQThreads ? kQThreads / PC : 1;
+  constexpr int P = PC * M;                           // entries per write tile
   constexpr int R = (P + kQThreads - 1) / kQThreads;
   __shared__ uint64_t s_hit[P];
   __shared__ uint64_t s_full[P];
@@ -696,7 +706,7 @@ __global__ __launch_bounds__(kQThreads) void k_expand_write(int slot, QCtrl* c,
     }
     if (t == 0) s_tot = (uint32_t)ttot;
     __syncthreads();
-    const uint64_t base = tbase_arr[tile];
+    const uint64_t base = tbase_arr[tile * M];
     const uint32_t tn = s_tot;
 #pragma unroll 4
     for (uint32_t j = t; j < tn; j += kQThreads) {
@@ -720,7 +730,11 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_write(
     const int* __restrict__ rid, int* __restrict__ out_fid, int* __restrict__ out_tid,
     int64_t out_cap, QPiece* __restrict__ pieces, int64_t piece_cap) {
   using S = QShape<G, STEPS>;
-  constexpr int P = S::kPerTile;                     // entries per tile
+  // a write tile is M count tiles (at least one entry per thread): its base is
+  // the first count tile's scanned total
+  constexpr int PC = S::kPerTile;
+  constexpr int M = PC < kQThreads ? kQThreads / PC : 1;
+  constexpr int P = PC * M;                           // entries per write tile
   constexpr int R = (P + kQThreads - 1) / kQThreads;  // entries per thread
   __shared__ uint64_t s_val[P];   // partial: hit mask; range: length
   __shared__ uint64_t s_off[P];   // tile-relative first output slot
@@ -775,7 +789,7 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_write(
     }
     if (t == 0) s_ftot = (uint32_t)ftot;
     __syncthreads();
-    const uint64_t base = tbase_arr[tile];
+    const uint64_t base = tbase_arr[tile * M];
     // long ranges: pieces for k_write_pieces
 #pragma unroll
     for (int r = 0; r < R; r++) {
