@@ -381,7 +381,7 @@ def main():
     ap.add_argument("--fanout", type=int, default=None,
                     help="default: 16 (plain), 32 (serve), the tuned one (batched)")
     ap.add_argument("--no-tuning", action="store_true",
-                    help="batched mode: library defaults instead of curtree_amd/tuning.json")
+                    help="library defaults instead of curtree_amd/tuning.json (plain, batched)")
     ap.add_argument("--key", choices=["morton64", "ref32"], default="morton64")
     ap.add_argument("--cpu-sample", type=int, default=0,
                     help="boxes timed in the CPU baseline (0: the whole workload)")
@@ -414,6 +414,18 @@ def main():
     args.windows_set = args.windows is not None
     if args.mode == "plain" and args.fanout is None:
         args.fanout = 16
+    args.plain_tuning = None
+    tf = os.path.join(ROOT, "curtree_amd", "tuning.json")
+    if args.mode == "plain" and not args.no_tuning and os.path.exists(tf):
+        # the tuned query knobs (the fanout stays the workload's)
+        with open(tf) as f:
+            tp = json.load(f)["tuned"].get("plain")
+        if tp:
+            args.plain_tuning = {k: tp[k] for k in ("tile_steps", "grid")}
+            if not args.query_tile_steps:
+                ct.set_option("query_tile_steps", tp["tile_steps"])
+            if not args.query_grid:
+                ct.set_option("query_grid", tp["grid"])
     if args.mode == "serve" and args.fanout is None:
         args.fanout = 32
     if args.mode == "batched":
@@ -548,7 +560,8 @@ def main():
                                f"(64-component Gaussian mixture, log-normal sizes), fanout {F}, "
                                f"{kbits}-bit Morton keys; {nq:,} windows side log-uniform "
                                f"[1e-4,1e-2]",
-                   "boxes": n, "windows": nq, "fanout": F, "key_bits": kbits},
+                   "boxes": n, "windows": nq, "fanout": F, "key_bits": kbits,
+                   "tuning": args.plain_tuning},
         "build": {"ms": float(build_ms.mean()), "boxes_per_s": n / (build_ms.mean() * 1e-3),
                   "algorithmic_bytes_per_box": bpb, "achieved_GBs": build_gbs,
                   "frac_of_8TBs": build_gbs / HBM_PEAK_GBS},
