@@ -59,16 +59,17 @@ def kernel_bytes(name, n, kbits, fanout):
     """Algorithmic bytes of one launch of build kernel `name` over n boxes.
 
     The sort passes carry the payload: each moves a (32-bit key, id) item and
-    the 16-byte box, read and written (48 B/box; the first pass reads the
-    caller's 20-byte SoA box instead of 24 staged bytes).  The final pass reads
-    the sorted items and boxes and writes the caller's arrays and the
-    leaf-parent nodes.  The whole build is also reported against the formula
+    the 16-byte box, read and written (48 B/box; the last pass writes the id
+    alone: 44).  The histogram pass also stores the 32-bit keys.  The final
+    pass reads the sorted ids and boxes and writes the caller's arrays and
+    the leaf-parent nodes.  The whole build is also reported against the formula
     of build_bytes_per_box, which prices the reference's algorithm."""
     return {
         "k_extent": 16 * n,
-        "k_hist": 16 * n,
-        "k_onesweep": 2 * (8 + 16) * n,
-        "k_final": (8 + 16 + 20 + 24 / fanout) * n,
+        "k_hist": (16 + 4) * n,
+        # three passes of 48 B/box and a last one that writes the id alone
+        "k_onesweep": (3 * 48 + 44) / 4 * n,
+        "k_final": (4 + 16 + 20 + 24 / fanout) * n,
     }.get(name)
 
 
