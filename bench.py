@@ -68,7 +68,7 @@ def kernel_bytes(name, n, kbits, fanout):
         "k_extent": 16 * n,
         "k_hist": (16 + 4) * n,
         # three passes of 48 B/box and a last one that writes the id alone
-        "k_onesweep": (3 * 48 + 44) / 4 * n,
+        "k_sweep_kv": (3 * 48 + 44) / 4 * n,
         "k_final": (4 + 16 + 20 + 24 / fanout) * n,
     }.get(name)
 
@@ -518,9 +518,9 @@ def main():
     query_ms = np.array([r[1] for r in rec])
     step_ms = build_ms + query_ms
     kernels = {}
-    for name in ("k_build_init", "k_extent", "k_hist", "k_sort_plan", "k_onesweep", "k_final",
+    for name in ("k_build_init", "k_extent", "k_hist", "k_sort_plan", "k_sweep_kv", "k_final",
                  "k_fixup_long", "k_fallback", "k_level", "k_safe", "k_run_begin", "k_expand",
-                 "k_expand_count", "k_expand_write", "k_leaf_count", "k_leaf_pass", "k_tile_scan",
+                 "k_expand_count", "k_expand_write", "k_leaf_count", "k_tile_scan",
                  "k_leaf_write", "k_write_pieces"):
         tot, cnt = ct.Prof.get(name)
         if cnt:

@@ -1349,7 +1349,7 @@ int build_f32(Context& ctx, curtree_bbox_f32* box, curtree_rtree_f32* rt, int ke
   pf.mark(s, "k_sort_plan");
   hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(256), 0, s, n, 4, c, 0, 4, 0x3210ull, 0, 0);
   for (int st = 0; st < 4; st++) {
-    pf.mark(s, "k_onesweep");
+    pf.mark(s, "k_sweep_kv");
     hipLaunchKernelGGL(k_sweep_kv, dim3((unsigned)ntiles), dim3(kSortThreads), 0, s, n, st,
                        box->xmin, box->ymin, box->xmax, box->ymax, box->id,
                        (const uint32_t*)sb.keys, sb.kv[0], sb.kv[1], sb.ids, sb.lo[0], sb.lo[1],

@@ -1242,7 +1242,7 @@ void enqueue_run_g(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
   }
   const int leaf = a.nl - 1;
   if (a.write) {
-    pf.mark(s, "k_leaf_pass");
+    pf.mark(s, "k_leaf_count");
     hipLaunchKernelGGL((k_leaf_count<G, STEPS, true>),
                        dim3(std::min<unsigned>(a.grid, leaf_count_grid<G, STEPS, true>())),
                        dim3(kQThreads), 0, s, leaf,
