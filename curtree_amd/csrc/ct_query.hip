@@ -307,6 +307,9 @@ __device__ __forceinline__ uint64_t block_excl_scan64(uint64_t v, uint64_t* s_tm
   return x - v + off;
 }
 
+#ifndef CT_LEAF_H
+#define CT_LEAF_H 2
+#endif
 template <int G, int STEPS, bool STORE>
 __global__ __launch_bounds__(kQThreads) void k_leaf_count(
     int slot, QTree tr, QBoxes qb, QBoxes rb, QCtrl* c, const uint2* __restrict__ fin,
@@ -316,7 +319,7 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_count(
   constexpr int P = S::kPerTile;
   // memory round trips per tile: the entries of all steps, then the windows
   // and boxes of H steps at a time
-  constexpr int H = STEPS < 4 ? STEPS : 4;
+  constexpr int H = STEPS < CT_LEAF_H ? STEPS : CT_LEAF_H;
   static_assert(STEPS % H == 0, "steps in whole groups");
   __shared__ uint64_t s_red[kQWaves];
   __shared__ QLevels s_lv;  // the level table (contained ranges)
@@ -724,6 +727,9 @@ __global__ __launch_bounds__(kQThreads) void k_expand_write(int slot, QCtrl* c,
   }
 }
 
+#ifndef CT_WRITE_U
+#define CT_WRITE_U 4
+#endif
 template <int G, int STEPS>
 __global__ __launch_bounds__(kQThreads) void k_leaf_write(
     int slot, QCtrl* c, const QLeaf* __restrict__ est, const uint64_t* __restrict__ tbase_arr,
@@ -816,7 +822,7 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_write(
     // flat slots, load-balanced over the block: slot j belongs to the last
     // entry whose flat offset is <= j (binary search), coalesced writes; the
     // box ids of kWriteU slots per thread are loaded together
-    constexpr int kWriteU = 8;
+    constexpr int kWriteU = CT_WRITE_U;
     const uint32_t fn = s_ftot;
     for (uint32_t j0 = 0; j0 < fn; j0 += kWriteU * kQThreads) {
       uint64_t o[kWriteU];

@@ -73,6 +73,20 @@ def test_morton64_fanouts(fanout):
     check_against_oracle(bx, ids, fanout, 1)
 
 
+@pytest.mark.parametrize("n", [2047, 2048, 2049, 10241, 12287, 12288, 12289, 24577])
+def test_tile_boundaries(n):
+    # sizes around the tiles of the final pass (2048 positions), the 64-bit
+    # LSD passes (10240 keys) and the payload passes (12288 items), with runs
+    # of equal high halves crossing tile edges
+    r = np.random.default_rng(n)
+    g = r.integers(0, 64, n).astype(np.float32) / np.float32(64)  # 64 x 64 cells
+    h = r.integers(0, 64, n).astype(np.float32) / np.float32(64)
+    bx = (g, h, g + np.float32(1e-3), h + np.float32(1e-3))
+    ids = r.permutation(n).astype(np.int32)
+    for f, km in ((16, 1), (10, 1), (8, 0)):
+        check_against_oracle(bx, ids, f, km)
+
+
 def test_morton64_one_million_uniform():
     n = 1_000_000
     bx = datagen.uniform_boxes(n, seed=21)
