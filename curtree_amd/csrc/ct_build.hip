@@ -1,6 +1,7 @@
-// ct_build.hip -- bulk load on gfx950: Morton keys -> onesweep LSD radix sort of
-// (key, input position) -> gather of the caller's box arrays into sorted order
-// fused with the leaf-parent MBR reduce -> upper levels.
+// ct_build.hip -- bulk load on gfx950: Morton keys -> onesweep LSD radix sort
+// of 32-bit keys that carries the boxes and ids (no gather) -> final pass:
+// ties of the 64-bit keys settled, the caller's box arrays written in sorted
+// order and the leaf-parent level reduced -> upper levels.
 //
 // Reference algorithm: src/rtree_bulkload_thrust.cuh:62-213 (key transform
 // :76-81, sort_by_key :83-86, level plan :101-115, reduce_by_key :126-171).
@@ -36,10 +37,11 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_tmp,
 // Control block layout (device memory, zeroed by k_build_init).
 constexpr int kExtSlots = 64;  // extent partials are spread over slots to avoid atomic contention
 
-// Sort steps: one onesweep pass each, over digit digit[step].  A plain LSD
-// sort runs digits 0..D-1; the 64-bit Morton sort runs the high half first
-// (digits 4..7), fixes runs of equal high halves in place (k_fixup) and only
-// if a run is too long for that runs the full LSD sort (steps 4..11).
+// Sort steps: one onesweep pass each, over digit digit[step].  The payload
+// sort runs steps 0..3 over the four digits of its 32-bit key (REF32's key or
+// the high half of the Morton key); runs of equal high halves are settled by
+// k_final / k_fixup_long, and only if a run is too long for that does the
+// full 64-bit LSD sort run (steps 4..11).  The pair sort runs steps 0..7.
 constexpr int kMaxSteps = 12;
 struct BuildCtrl {
   uint32_t ext[kExtSlots][4];      // ordered-uint lox, loy, hix, hiy of centroids, spread
@@ -49,10 +51,10 @@ struct BuildCtrl {
   int32_t digit[kMaxSteps];        // digit of each step
   int32_t active[kMaxSteps];       // step s runs
   int32_t src[kMaxSteps];          // step s reads buffer src[s]
-  int32_t write_keys[kMaxSteps];   // step s writes keys (not the last step)
+  int32_t write_keys[kMaxSteps];   // step s writes keys (not the last step: values / ids only)
   int32_t npass;                   // steps that ran so far (planned): parity of the data
   int32_t first;                   // first step that runs (-1: none)
-  uint32_t fallback;               // k_fixup found a run too long to fix in place
+  uint32_t fallback;               // a run is too long to settle in place: full 64-bit sort
   uint32_t nruns;                  // long runs listed for k_fixup_long
   uint32_t trivial;                // bit d: digit d is the same for every key
 };
@@ -249,7 +251,7 @@ __global__ __launch_bounds__(256) void k_hist(int64_t n, const float* __restrict
     if (sh[t]) atomicAdd(&c->hist[t], sh[t]);
 }
 
-// 64-bit keys + AoS staging copy + all-digit histograms: the input of the
+// 64-bit keys + 32-B staging records + all-digit histograms: the input of the
 // full 8-pass LSD sort, which runs only when the high-half sort leaves a run
 // of equal high halves too long to settle in place (`gate`: skip unless
 // c->fallback; the keys go to the buffer the next step reads).
