@@ -727,7 +727,8 @@ constexpr int kFinTile = 2048;       // positions owned by one block of k_final 
 constexpr int kFinSpan = kFinTile + 2 * kFixShort;
 
 __global__ __launch_bounds__(256) void k_fixup_long(
-    int64_t n, BuildCtrl* c, const uint32_t* __restrict__ ids, const float2* __restrict__ lo0, const float2* __restrict__ lo1, const float2* __restrict__ hi0,
+    int64_t n, BuildCtrl* c, const uint32_t* __restrict__ ids, const float2* __restrict__ lo0,
+    const float2* __restrict__ lo1, const float2* __restrict__ hi0,
     const float2* __restrict__ hi1, float* __restrict__ xmin, float* __restrict__ ymin,
     float* __restrict__ xmax, float* __restrict__ ymax, int* __restrict__ id, uint32_t* runs,
     uint32_t runs_cap) {
@@ -902,7 +903,8 @@ __device__ __forceinline__ void leaf_parent(int64_t i, int64_t n, float a, float
 // listed for k_fixup_long and written in high-half order here.
 template <int MODE, int G>
 __global__ __launch_bounds__(256) void k_final(
-    int64_t n, BuildCtrl* c, const uint32_t* __restrict__ ids, const float2* __restrict__ lo0, const float2* __restrict__ lo1, const float2* __restrict__ hi0,
+    int64_t n, BuildCtrl* c, const uint32_t* __restrict__ ids, const float2* __restrict__ lo0,
+    const float2* __restrict__ lo1, const float2* __restrict__ hi0,
     const float2* __restrict__ hi1, float* __restrict__ xmin, float* __restrict__ ymin,
     float* __restrict__ xmax, float* __restrict__ ymax, int* __restrict__ id, TreeView tr,
     int64_t l0start, int64_t r_sz, uint32_t* runs, uint32_t runs_cap) {
