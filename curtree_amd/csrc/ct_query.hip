@@ -323,6 +323,10 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_count(
   static_assert(STEPS % H == 0, "steps in whole groups");
   __shared__ uint64_t s_red[kQWaves];
   __shared__ QLevels s_lv;  // the level table (contained ranges)
+  // the tile's entries, written out once at the end of the tile: a store
+  // between two rounds of loads would make the second wait for it (stores
+  // and loads share one counter)
+  __shared__ QLeaf s_est[STORE ? P : 1];
   if (c->overflow) return;
   const int64_t n_in = (int64_t)c->count[slot];
   const int64_t q0 = c->q0;
@@ -423,7 +427,7 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_count(
             x.val = full[k] ? nh : gm;
             x.first = (uint32_t)first[k];
             x.qf = q[k] | (full[k] ? kQRange : 0u);
-            est[e] = x;
+            s_est[e - tbase] = x;
           }
           if (counts && nh) atomicAdd(counts + q[k], (unsigned long long)nh);
         }
@@ -435,6 +439,9 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_count(
     }
     if (lane == 0) s_red[w] = wsum;
     __syncthreads();
+    if (STORE)
+      for (int i = t; i < P; i += kQThreads)
+        if (tbase + i < n_in) est[tbase + i] = s_est[i];
     if (t == 0) {
       uint64_t tot = 0;
 #pragma unroll
@@ -518,6 +525,9 @@ __global__ __launch_bounds__(kQThreads) void k_expand_count(int slot, QTree tr, 
   constexpr int H = STEPS < CT_EXPAND_H ? STEPS : CT_EXPAND_H;
   static_assert(STEPS % H == 0, "steps in whole groups");
   __shared__ uint64_t s_red[kQWaves];
+  // the tile's entries, written out once at the end of the tile (a store
+  // between two rounds of loads would make the second wait for it)
+  __shared__ QExp s_est[P];
   if (c->overflow) return;
   const int64_t n_in = (int64_t)c->count[slot];
   const int64_t q0 = c->q0;
@@ -623,7 +633,7 @@ __global__ __launch_bounds__(kQThreads) void k_expand_count(int slot, QTree tr, 
           x.full = fm;
           x.q = en[k].x;
           x.first = full ? (en[k].y & ~kFull) : (uint32_t)p[k];
-          est[e] = x;
+          s_est[e - tbase] = x;
         }
         uint64_t v = (c_ == 0 && valid) ? (uint64_t)popc64(gm) : 0ull;
 #pragma unroll
@@ -633,6 +643,8 @@ __global__ __launch_bounds__(kQThreads) void k_expand_count(int slot, QTree tr, 
     }
     if (lane == 0) s_red[w] = wsum;
     __syncthreads();
+    for (int i = t; i < P; i += kQThreads)
+      if (tbase + i < n_in) est[tbase + i] = s_est[i];
     if (t == 0) {
       uint64_t tot = 0;
 #pragma unroll
@@ -822,12 +834,14 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_write(
     // flat slots, load-balanced over the block: slot j belongs to the last
     // entry whose flat offset is <= j (binary search), coalesced writes; the
     // box ids of kWriteU slots per thread are loaded together
+    // (software-pipelined: the ids of the next batch are loaded before this
+    // batch is stored, so no load waits behind a store of its own batch)
     constexpr int kWriteU = CT_WRITE_U;
+    constexpr uint32_t kStep = kWriteU * kQThreads;
     const uint32_t fn = s_ftot;
-    for (uint32_t j0 = 0; j0 < fn; j0 += kWriteU * kQThreads) {
-      uint64_t o[kWriteU];
+    auto batch = [&](uint32_t j0, uint64_t (&o)[kWriteU], int (&idv)[kWriteU],
+                     uint32_t (&qv)[kWriteU]) {
       int64_t bi[kWriteU];
-      uint32_t qv[kWriteU];
 #pragma unroll
       for (int u = 0; u < kWriteU; u++) {
         const uint32_t j = j0 + u * kQThreads + t;
@@ -844,15 +858,35 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_write(
         bi[u] = (int64_t)s_first[lo] + bpos;
         qv[u] = qf & ~kQRange;
       }
-      int idv[kWriteU];
 #pragma unroll
       for (int u = 0; u < kWriteU; u++) idv[u] = rid[bi[u]];
+    };
+    if (fn > 0) {
+      uint64_t oA[kWriteU];
+      int idA[kWriteU];
+      uint32_t qA[kWriteU];
+      batch(0, oA, idA, qA);
+      for (uint32_t j0 = 0; j0 < fn; j0 += kStep) {
+        uint64_t oB[kWriteU];
+        int idB[kWriteU];
+        uint32_t qB[kWriteU];
+        const bool more = j0 + kStep < fn;
+        if (more) batch(j0 + kStep, oB, idB, qB);
 #pragma unroll
-      for (int u = 0; u < kWriteU; u++)
-        if (o[u] < (uint64_t)out_cap) {
-          out_fid[o[u]] = idv[u];
-          out_tid[o[u]] = (int)qv[u];
+        for (int u = 0; u < kWriteU; u++)
+          if (oA[u] < (uint64_t)out_cap) {
+            out_fid[oA[u]] = idA[u];
+            out_tid[oA[u]] = (int)qA[u];
+          }
+        if (more) {
+#pragma unroll
+          for (int u = 0; u < kWriteU; u++) {
+            oA[u] = oB[u];
+            idA[u] = idB[u];
+            qA[u] = qB[u];
+          }
         }
+      }
     }
   }
 }
