@@ -1,7 +1,7 @@
 // ct_build.hip -- bulk load on gfx950: Morton keys -> onesweep LSD radix sort
-// of 32-bit keys that carries the boxes and ids (no gather) -> final pass:
-// ties of the 64-bit keys settled, the caller's box arrays written in sorted
-// order and the leaf-parent level reduced -> upper levels.
+// of 32-bit keys that carries the boxes and ids (no gather; its last pass
+// writes the caller's arrays) -> final pass, in place: ties of the 64-bit keys
+// settled and the leaf-parent level reduced -> upper levels.
 //
 // Reference algorithm: src/rtree_bulkload_thrust.cuh:62-213 (key transform
 // :76-81, sort_by_key :83-86, level plan :101-115, reduce_by_key :126-171).
@@ -51,11 +51,12 @@ struct BuildCtrl {
   int32_t digit[kMaxSteps];        // digit of each step
   int32_t active[kMaxSteps];       // step s runs
   int32_t src[kMaxSteps];          // step s reads buffer src[s]
-  int32_t write_keys[kMaxSteps];   // step s writes keys (not the last step: values / ids only)
+  int32_t write_keys[kMaxSteps];   // step s writes keys (not the last step: values only)
   int32_t npass;                   // steps that ran so far (planned): parity of the data
   int32_t first;                   // first step that runs (-1: none)
   uint32_t fallback;               // a run is too long to settle in place: full 64-bit sort
-  uint32_t nruns;                  // long runs listed for k_fixup_long
+  uint32_t nruns;                  // runs listed by k_final
+  uint32_t nlong;                  // of them longer than 64 (k_fixup_long)
   uint32_t trivial;                // bit d: digit d is the same for every key
 };
 
@@ -82,6 +83,7 @@ __global__ __launch_bounds__(256) void k_build_init(BuildCtrl* c) {
     c->first = -1;
     c->fallback = 0;
     c->nruns = 0;
+    c->nlong = 0;
     c->trivial = 0;
   }
 }
@@ -198,6 +200,28 @@ __device__ __forceinline__ uint64_t morton64(float a, float b, float cc, float d
   return (spread32(q32(cy, p.loy, p.sy)) << 1) | spread32(q32(cx, p.lox, p.sx));
 }
 
+// The two halves of morton64 in comparable form without interleaving: the
+// cell (qy_hi16 << 16) | qx_hi16, equal exactly when the high halves are, and
+// *lo = (qy_lo16 << 16) | qx_lo16 (order: morton_less16).
+__device__ __forceinline__ uint32_t morton_cell(float a, float b, float cc, float d,
+                                                const KeyParams& p, uint32_t* lo) {
+  const float cx = (a + cc) * 0.5f;
+  const float cy = (b + d) * 0.5f;
+  const uint32_t qx = q32(cx, p.lox, p.sx), qy = q32(cy, p.loy, p.sy);
+  *lo = (qy << 16) | (qx & 0xFFFFu);
+  return (qy & 0xFFFF0000u) | (qx >> 16);
+}
+
+// Order of the low halves of two Morton keys given as morton_cell's *lo: the
+// key interleaves the coordinates with y above x at every bit, so the
+// coordinate whose difference has the higher top bit decides (y on a tie).
+__device__ __forceinline__ bool morton_less16(uint32_t a, uint32_t b) {
+  const uint32_t d = a ^ b;
+  const uint32_t dx = d & 0xFFFFu, dy = d >> 16;
+  const bool xdec = dy < dx && dy < (dy ^ dx);  // top bit of dy below that of dx
+  return xdec ? (a & 0xFFFFu) < (b & 0xFFFFu) : (a >> 16) < (b >> 16);
+}
+
 __device__ __forceinline__ uint32_t ref32(float a, float b, float cc, float d) {
   const float cx = (a + cc) / 2.0f;
   const float cy = (b + d) / 2.0f;
@@ -288,23 +312,32 @@ __global__ __launch_bounds__(256) void k_keygen64(int64_t n, const float* __rest
 // Step plan: exclusive digit offsets for digits 0..D-1 and, for steps
 // [s0, s0+ns) over the digits packed 4 bits each in `digits` (lowest first),
 // which steps run and from which buffer.  A step whose digit is the same for
-// every key is skipped (it would be the identity permutation).  `need_one`:
-// at least one step must run (a later stage reads the sorted buffers).
-__device__ void plan_steps(BuildCtrl* c, int s0, int ns, uint64_t digits, bool need_one) {
-  int J = c->npass, last = -1, first = c->first;
+// every key is skipped (it would be the identity permutation), except that at
+// least `min_steps` steps run: the first trivial steps are kept when fewer
+// digits vary (a later stage reads the buffers a step writes; the payload
+// sort's last step must not be its first, which reads the arrays it writes).
+__device__ void plan_steps(BuildCtrl* c, int s0, int ns, uint64_t digits, int min_steps) {
+  int varying = 0;
+  for (int k = 0; k < ns; k++)
+    if (!((c->trivial >> (int)((digits >> (4 * k)) & 15u)) & 1u)) varying++;
+  int force = min_steps - varying;  // identity steps that still run
+  int J = c->npass, first = c->first;
   for (int k = 0; k < ns; k++) {
     const int st = s0 + k, d = (int)((digits >> (4 * k)) & 15u);
     c->digit[st] = d;
     c->ticket[st] = 0;
-    const bool triv = (c->trivial >> d) & 1u;
-    if (triv && !(need_one && k == ns - 1 && J == c->npass && last < 0)) {
+    bool run = !((c->trivial >> d) & 1u);
+    if (!run && force > 0) {
+      run = true;
+      force--;
+    }
+    if (!run) {
       c->active[st] = 0;
     } else {
       c->active[st] = 1;
       c->src[st] = J & 1;
       c->write_keys[st] = 1;
       if (first < 0) first = st;
-      last = st;
       J++;
     }
   }
@@ -313,10 +346,12 @@ __device__ void plan_steps(BuildCtrl* c, int s0, int ns, uint64_t digits, bool n
 }
 
 // Digit offsets for digits 0..D-1 from the histograms and the plan of steps
-// [s0, s0+ns).  `gate`: plan only if the fallback sort was asked for (its
-// first step then reads input positions as values again).
+// [s0, s0+ns) (at least min_steps of them run).  `gate`: plan only if the
+// fallback sort was asked for (its first step then reads input positions as
+// values again).
 __global__ __launch_bounds__(256) void k_sort_plan(int64_t n, int D, BuildCtrl* c, int s0, int ns,
-                                                   uint64_t digits, int keep_last_keys, int gate) {
+                                                   uint64_t digits, int keep_last_keys, int gate,
+                                                   int min_steps) {
   __shared__ uint32_t s_tmp[4];
   __shared__ uint32_t s_triv;
   if (gate && !c->fallback) return;
@@ -334,9 +369,7 @@ __global__ __launch_bounds__(256) void k_sort_plan(int64_t n, int D, BuildCtrl* 
   if (threadIdx.x == 0) {
     c->trivial = s_triv;
     if (gate) c->first = -1;
-    // at least one step runs: the stage after the sort reads the buffers a
-    // step writes
-    plan_steps(c, s0, ns, digits, true);
+    plan_steps(c, s0, ns, digits, min_steps);
     // the last step writes values only, unless a later stage reads the keys
     if (!keep_last_keys)
       for (int st = s0 + ns - 1; st >= s0; st--)
@@ -568,18 +601,83 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
 // staging buffers the previous pass wrote.  The three 8-B fields are
 // reordered through LDS one after the other, so the tile's LDS buffer stays
 // 8 B per item.
+// ---------------------------------------------------------------------------
+// Runs of equal keys inside one wave, whose lanes hold 64 consecutive items.
+// dpp_perm moves a value by a DPP lane pattern; a lane whose source lies
+// outside the wave, or outside the active lanes, reads 0 (so every lane of
+// the wave runs these helpers; with no old value to keep, a move folds into
+// the min/max that uses it).
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_perm(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, true);
+}
+// whole-wave shifts by one lane: lane i takes lane i - 1 (shr) or i + 1 (shl)
+constexpr int kDppWaveShr1 = 0x138;
+constexpr int kDppWaveShl1 = 0x130;
+
+// First and last lane of this lane's run of equal keys (eqp / eqn: the same
+// key as the previous / next item); -1 / kWave when the run goes on beyond
+// the wave.
+__device__ __forceinline__ void run_bounds(bool eqp, bool eqn, int* st, int* en) {
+  const int lane = lane_id();
+  const uint64_t S = ballot(!eqp && eqn), E = ballot(eqp && !eqn);
+  const uint64_t le = S & (lanemask_lt() | (1ull << lane)), ge = E & ~lanemask_lt();
+  *st = le ? 63 - __clzll((long long)le) : -1;
+  *en = ge ? __ffsll((long long)ge) - 1 : kWave;
+}
+
+// Destination lane of this lane's item when every run marked `local` (lanes
+// st..en) is sorted by (low half, lane), low halves in morton_cell form;
+// other lanes stay.  Runs of two (nearly all runs) swap when the second key
+// is strictly less; longer runs rank each member against the others, shifted
+// in one lane per step (a lower lane with an equal key comes first).
+__device__ __forceinline__ int local_run_dest(bool local, int st, int en, uint32_t lo) {
+  const int lane = lane_id();
+  const uint32_t lo_next = dpp_perm<kDppWaveShl1>(lo), lo_prev = dpp_perm<kDppWaveShr1>(lo);
+  int dest = lane;
+  if (local && en - st == 1) {
+    const bool head = lane == st;
+    if (head ? morton_less16(lo_next, lo) : morton_less16(lo, lo_prev))
+      dest = head ? lane + 1 : lane - 1;
+  }
+  const bool longer = local && en - st > 1;
+  if (ballot(longer)) {
+    uint32_t vl = lo, vr = lo;
+    int rank = 0;
+    for (int d = 1; ballot(longer && d <= en - st); d++) {
+      vl = dpp_perm<kDppWaveShr1>(vl);  // key of lane - d
+      vr = dpp_perm<kDppWaveShl1>(vr);  // key of lane + d
+      if (longer && lane - d >= st && !morton_less16(lo, vl)) rank++;
+      if (longer && lane + d <= en && morton_less16(vr, lo)) rank++;
+    }
+    if (longer) dest = st + rank;
+  }
+  return dest;
+}
+
+// Forward permutation over the wave: this lane's box and id go to lane `dest`
+// (dest is a permutation of the lanes).
+__device__ __forceinline__ void wave_permute(int dest, float4& b, int& v) {
+  const int a = dest << 2;
+  b.x = __int_as_float(__builtin_amdgcn_ds_permute(a, __float_as_int(b.x)));
+  b.y = __int_as_float(__builtin_amdgcn_ds_permute(a, __float_as_int(b.y)));
+  b.z = __int_as_float(__builtin_amdgcn_ds_permute(a, __float_as_int(b.z)));
+  b.w = __int_as_float(__builtin_amdgcn_ds_permute(a, __float_as_int(b.w)));
+  v = __builtin_amdgcn_ds_permute(a, v);
+}
+
 // Moves the ranked tile of the payload sort: keys, then the two box halves,
 // each reordered through LDS and scattered in runs of equal digits.  FULL:
 // every item of the tile exists (no predicated memory operations, so nothing
 // waits on earlier stores at block joins).
 template <bool FULL>
 __device__ __forceinline__ void sweep_move(
-    const uint64_t (&kv)[kSweepItems], const uint32_t (&pos)[kSweepItems], int64_t wbase, int64_t n,
-    int tn, int shift, bool first, const float* __restrict__ xmin, const float* __restrict__ ymin,
-    const float* __restrict__ xmax, const float* __restrict__ ymax, const float2* __restrict__ lin,
-    const float2* __restrict__ hin, uint64_t* __restrict__ kout, uint32_t* __restrict__ idout,
-    float2* __restrict__ lout, float2* __restrict__ hout, SweepShared& sh, uint64_t* s_buf,
-    uint8_t* s_dig) {
+    const uint64_t (&kv)[kSweepItems], const uint32_t (&pos)[kSweepItems], int64_t wbase,
+    int64_t n, int tn, int shift, bool first, bool last, float* __restrict__ xmin,
+    float* __restrict__ ymin, float* __restrict__ xmax, float* __restrict__ ymax,
+    int* __restrict__ id, const float2* __restrict__ lin, const float2* __restrict__ hin,
+    uint64_t* __restrict__ kout, float2* __restrict__ lout, float2* __restrict__ hout,
+    SweepShared& sh, uint64_t* s_buf, uint8_t* s_dig) {
   const int t = threadIdx.x, lane = lane_id();
   // the box halves are loaded first: their loads are then older than every
   // store of this tile, and waiting for them never waits for a store
@@ -605,15 +703,27 @@ __device__ __forceinline__ void sweep_move(
   for (int i = 0; i < kSweepItems; i++)
     if (FULL || wbase + i * kWave + lane < n) s_buf[pos[i]] = kv[i];
   __syncthreads();
+  if (last) {  // the ids, into the caller's array
 #pragma unroll
-  for (int i = 0; i < kSweepItems; i++) {
-    const int j = t + i * kSortThreads;
-    if (FULL || j < tn) {
-      const uint64_t kk = s_buf[j];
-      const uint32_t d = (uint32_t)(kk >> shift) & 255u;
-      s_dig[j] = (uint8_t)d;
-      if (idout) idout[sh.adj[d] + (uint32_t)j] = (uint32_t)kk;
-      else kout[sh.adj[d] + (uint32_t)j] = kk;
+    for (int i = 0; i < kSweepItems; i++) {
+      const int j = t + i * kSortThreads;
+      if (FULL || j < tn) {
+        const uint64_t kk = s_buf[j];
+        const uint32_t d = (uint32_t)(kk >> shift) & 255u;
+        s_dig[j] = (uint8_t)d;
+        id[sh.adj[d] + (uint32_t)j] = (int)(uint32_t)kk;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < kSweepItems; i++) {
+      const int j = t + i * kSortThreads;
+      if (FULL || j < tn) {
+        const uint64_t kk = s_buf[j];
+        const uint32_t d = (uint32_t)(kk >> shift) & 255u;
+        s_dig[j] = (uint8_t)d;
+        kout[sh.adj[d] + (uint32_t)j] = kk;
+      }
     }
   }
   float2* s_f2 = (float2*)s_buf;
@@ -624,25 +734,40 @@ __device__ __forceinline__ void sweep_move(
     for (int i = 0; i < kSweepItems; i++)
       if (FULL || wbase + i * kWave + lane < n) s_f2[pos[i]] = half ? vhi[i] : vlo[i];
     __syncthreads();
-    float2* hdst = half ? hout : lout;
+    if (last) {  // one coordinate array each
+      float* dx = half ? xmax : xmin;
+      float* dy = half ? ymax : ymin;
 #pragma unroll
-    for (int i = 0; i < kSweepItems; i++) {
-      const int j = t + i * kSortThreads;
-      if (FULL || j < tn) hdst[sh.adj[s_dig[j]] + (uint32_t)j] = s_f2[j];
+      for (int i = 0; i < kSweepItems; i++) {
+        const int j = t + i * kSortThreads;
+        if (FULL || j < tn) {
+          const float2 v = s_f2[j];
+          const uint32_t o = sh.adj[s_dig[j]] + (uint32_t)j;
+          dx[o] = v.x;
+          dy[o] = v.y;
+        }
+      }
+    } else {
+      float2* hdst = half ? hout : lout;
+#pragma unroll
+      for (int i = 0; i < kSweepItems; i++) {
+        const int j = t + i * kSortThreads;
+        if (FULL || j < tn) hdst[sh.adj[s_dig[j]] + (uint32_t)j] = s_f2[j];
+      }
     }
   }
 }
 
 // One 8-bit pass of the payload sort.  The first pass that runs reads the
-// sort keys k_hist stored and the caller's arrays; the last one writes the ids
-// alone (k_final recomputes the keys from the boxes).
+// sort keys k_hist stored and the caller's arrays; the last one writes the
+// ids and boxes into the caller's arrays in the order of the 32-bit key (the
+// plan runs at least two passes, so the last never reads what it writes).
 __global__ __launch_bounds__(kSortThreads) void k_sweep_kv(
-    int64_t n, int step, const float* __restrict__ xmin, const float* __restrict__ ymin,
-    const float* __restrict__ xmax, const float* __restrict__ ymax, const int* __restrict__ id,
+    int64_t n, int step, float* __restrict__ xmin, float* __restrict__ ymin,
+    float* __restrict__ xmax, float* __restrict__ ymax, int* __restrict__ id,
     const uint32_t* __restrict__ keys, uint64_t* __restrict__ kv0, uint64_t* __restrict__ kv1,
-    uint32_t* __restrict__ ids, float2* __restrict__ lo0, float2* __restrict__ lo1,
-    float2* __restrict__ hi0, float2* __restrict__ hi1, BuildCtrl* c,
-    uint32_t* __restrict__ status) {
+    float2* __restrict__ lo0, float2* __restrict__ lo1, float2* __restrict__ hi0,
+    float2* __restrict__ hi1, BuildCtrl* c, uint32_t* __restrict__ status) {
   __shared__ SweepShared sh;
   __shared__ uint64_t s_buf[kSweepTile];
   __shared__ uint8_t s_dig[kSweepTile];
@@ -693,87 +818,146 @@ __global__ __launch_bounds__(kSortThreads) void k_sweep_kv(
   for (int i = 0; i < kSweepItems; i++) dig[i] = (uint32_t)(kv[i] >> shift) & 255u;
   sweep_rank(dig, pos, wbase, n, tile, c->base + pass * 256, status, sh);
 
-  uint32_t* idout = last ? ids : nullptr;
   const int64_t rem = n - tbase;
   if (rem >= kSweepTile)  // full tile: no predicated loads or stores
-    sweep_move<true>(kv, pos, wbase, n, kSweepTile, shift, first, xmin, ymin, xmax, ymax, lin, hin,
-                     kout, idout, lout, hout, sh, s_buf, s_dig);
+    sweep_move<true>(kv, pos, wbase, n, kSweepTile, shift, first, last, xmin, ymin, xmax, ymax,
+                     id, lin, hin, kout, lout, hout, sh, s_buf, s_dig);
   else
-    sweep_move<false>(kv, pos, wbase, n, (int)rem, shift, first, xmin, ymin, xmax, ymax, lin, hin,
-                      kout, idout, lout, hout, sh, s_buf, s_dig);
+    sweep_move<false>(kv, pos, wbase, n, (int)rem, shift, first, last, xmin, ymin, xmax, ymax,
+                      id, lin, hin, kout, lout, hout, sh, s_buf, s_dig);
 }
 
 // ---------------------------------------------------------------------------
 // Ties of the high half (64-bit Morton keys).  After the high-half passes the
-// items are sorted by the high 32 bits, and a stable sort leaves each run of
-// equal high halves in input order.  Sorting every run by (low half, position
-// in the run) then gives exactly the order of a stable sort on the full key,
-// so the four low-digit passes are not needed.  With n boxes spread over 2^32
-// high-half cells runs are short.
-//   Runs of 2..kFixShort are sorted by k_final, in LDS, by every block whose
-// tile they touch (each block writes only its own positions).  Longer runs are
-// listed by the block whose tile holds their start, sorted by k_fixup_long
-// (one block per run, bitonic in LDS, up to kFixLong), and the leaf-parent
-// nodes over them recomputed by k_fixup_groups.  A longer run, or a full list,
-// sets `fallback`: the full 8-pass LSD sort of 64-bit keys then runs from the
-// order reached so far (in which equal full keys are still in input order).
-constexpr int kFixShort = 16;
+// caller's arrays are sorted by the high 32 bits, and a stable sort leaves each
+// run of equal high halves in input order.  Sorting every run by (low half,
+// position in the run) then gives exactly the order of a stable sort on the
+// full key, so the four low-digit passes are not needed.  With n boxes spread
+// over 2^32 high-half cells runs are short.  The runs are sorted in place:
+//   k_final sorts, within one wave, every run that lies inside one chunk of 64
+// positions (a wave's lanes), unless it holds the first or last position of
+// the block's tile: the neighbouring blocks read those two positions, so they
+// never change during the launch.  Every other run is listed by the chunk
+// that holds its start; k_run_len measures the listed runs (reading only),
+// k_fixup_short sorts those of up to 64 boxes (one wave each), k_fixup_long
+// the longer ones up to kFixLong (one block each, bitonic in LDS), and
+// k_fixup_groups recomputes the leaf-parent nodes over them.  A longer run, or
+// a full list, sets `fallback`: the full 8-pass LSD sort of 64-bit keys then
+// runs from the order reached so far (in which equal full keys are still in
+// input order).
 constexpr int kFixLong = 4096;
-constexpr int kFixLongSlots = 4096;  // blocks of k_fixup_long (they loop over the list)
+constexpr int kFixSlots = 4096;      // blocks of the fixup launches (they loop over the list)
 #ifndef CT_FINAL_GRID
 #define CT_FINAL_GRID 4096
 #endif
 constexpr int kFinTile = 2048;       // positions owned by one block of k_final (multiple of 64)
-constexpr int kFinSpan = kFinTile + 2 * kFixShort;
 
-__global__ __launch_bounds__(256) void k_fixup_long(
-    int64_t n, BuildCtrl* c, const uint32_t* __restrict__ ids, const float2* __restrict__ lo0,
-    const float2* __restrict__ lo1, const float2* __restrict__ hi0,
-    const float2* __restrict__ hi1, float* __restrict__ xmin, float* __restrict__ ymin,
-    float* __restrict__ xmax, float* __restrict__ ymax, int* __restrict__ id, uint32_t* runs,
-    uint32_t runs_cap) {
-  __shared__ uint64_t s_net[kFixLong];
-  __shared__ int s_len;
+// Length of every listed run (one wave per run; nothing is written to the
+// boxes, so every wave reads stable data).  Runs of more than 64 boxes are
+// listed again for k_fixup_long; one longer than kFixLong sets `fallback`.
+__global__ __launch_bounds__(256) void k_run_len(int64_t n, BuildCtrl* c,
+                                                 const float* __restrict__ xmin,
+                                                 const float* __restrict__ ymin,
+                                                 const float* __restrict__ xmax,
+                                                 const float* __restrict__ ymax, uint32_t* runs,
+                                                 uint32_t runs_cap) {
+  // (no early exit on `fallback`: this launch sets it, so blocks could see
+  // it differently)
   const uint32_t nr = c->nruns < runs_cap ? c->nruns : runs_cap;
-  if (blockIdx.x >= nr || c->fallback) return;
+  if (blockIdx.x * 4u >= nr) return;
   const KeyParams kp = key_params<kKeyMorton64>(c);
-  const int cur = c->npass & 1;
-  const float2* lo = cur ? lo1 : lo0;
-  const float2* hi = cur ? hi1 : hi0;
-  const int t = threadIdx.x;
-  for (uint32_t q = blockIdx.x; q < nr; q += gridDim.x) {
+  const int lane = lane_id();
+  const uint32_t nw = gridDim.x * 4u;
+  for (uint32_t q = (blockIdx.x * 256u + threadIdx.x) >> 6; q < nr; q += nw) {
     const int64_t i = runs[q];
-    const uint32_t h = (uint32_t)(morton64(lo[i].x, lo[i].y, hi[i].x, hi[i].y, kp) >> 32);
-    // length: the first offset whose high half differs (kFixLong + 1: longer)
-    if (t == 0) s_len = kFixLong + 1;
-    __syncthreads();
-    for (int b0 = kFixShort; b0 <= kFixLong; b0 += 256) {
-      const int j = b0 + t;
-      bool end = false;
-      if (j <= kFixLong) {
-        if (i + j >= n) {
-          end = true;
-        } else {
-          const float2 l = lo[i + j], hh = hi[i + j];
-          end = (uint32_t)(morton64(l.x, l.y, hh.x, hh.y, kp) >> 32) != h;
-        }
+    const uint32_t h0 = (uint32_t)(morton64(xmin[i], ymin[i], xmax[i], ymax[i], kp) >> 32);
+    int L = kFixLong + 1;
+    for (int64_t b = i; b - i <= kFixLong; b += kWave) {
+      const int64_t p = b + lane;
+      const bool eq =
+          p < n && (uint32_t)(morton64(xmin[p], ymin[p], xmax[p], ymax[p], kp) >> 32) == h0;
+      const uint64_t ne = ballot(!eq);
+      if (ne) {
+        L = (int)(b - i) + __ffsll((long long)ne) - 1;
+        break;
       }
-      if (end) atomicMin(&s_len, j);
-      if (__syncthreads_or(end)) break;
     }
-    __syncthreads();
-    const int L = s_len;
-    if (L > kFixLong) {  // the full LSD sort takes over
-      if (t == 0) atomicOr(&c->fallback, 1u);
-      return;
+    if (lane == 0) {
+      runs[runs_cap + q] = (uint32_t)L;
+      if (L > kFixLong) {
+        atomicOr(&c->fallback, 1u);
+      } else if (L > kWave) {
+        const uint32_t at = atomicAdd(&c->nlong, 1u);
+        runs[2 * (size_t)runs_cap + at] = q;
+      }
     }
+  }
+}
+
+// Listed runs of up to 64 boxes, one wave each: lane l holds position i + l.
+__global__ __launch_bounds__(256) void k_fixup_short(int64_t n, BuildCtrl* c,
+                                                     float* __restrict__ xmin,
+                                                     float* __restrict__ ymin,
+                                                     float* __restrict__ xmax,
+                                                     float* __restrict__ ymax,
+                                                     int* __restrict__ id,
+                                                     const uint32_t* __restrict__ runs,
+                                                     uint32_t runs_cap) {
+  const uint32_t nr = c->nruns < runs_cap ? c->nruns : runs_cap;
+  if (blockIdx.x * 4u >= nr || c->fallback) return;
+  const KeyParams kp = key_params<kKeyMorton64>(c);
+  const int lane = lane_id();
+  const uint32_t nw = gridDim.x * 4u;
+  for (uint32_t q = (blockIdx.x * 256u + threadIdx.x) >> 6; q < nr; q += nw) {
+    const int L = (int)runs[runs_cap + q];
+    if (L > kWave) continue;
+    const int64_t p = (int64_t)runs[q] + lane;
+    const bool member = lane < L;
+    float4 b = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (member) b = make_float4(xmin[p], ymin[p], xmax[p], ymax[p]);
+    uint32_t lo;
+    (void)morton_cell(b.x, b.y, b.z, b.w, kp, &lo);
+    const int dest = local_run_dest(member, 0, L - 1, lo);
+    int v = dest != lane ? id[p] : 0;
+    wave_permute(dest, b, v);
+    if (dest != lane) {
+      xmin[p] = b.x;
+      ymin[p] = b.y;
+      xmax[p] = b.z;
+      ymax[p] = b.w;
+      id[p] = v;
+    }
+  }
+}
+
+// Listed runs of 65..kFixLong boxes, one block each.
+__global__ __launch_bounds__(256) void k_fixup_long(int64_t n, BuildCtrl* c,
+                                                    float* __restrict__ xmin,
+                                                    float* __restrict__ ymin,
+                                                    float* __restrict__ xmax,
+                                                    float* __restrict__ ymax,
+                                                    int* __restrict__ id,
+                                                    const uint32_t* __restrict__ runs,
+                                                    uint32_t runs_cap) {
+  constexpr int kPer = kFixLong / 256;
+  __shared__ uint64_t s_net[kFixLong];
+  const uint32_t nl = c->nlong;
+  if (blockIdx.x >= nl || c->fallback) return;
+  const KeyParams kp = key_params<kKeyMorton64>(c);
+  const int t = threadIdx.x;
+  for (uint32_t r = blockIdx.x; r < nl; r += gridDim.x) {
+    const uint32_t q = runs[2 * (size_t)runs_cap + r];
+    const int64_t i = runs[q];
+    const int L = (int)runs[runs_cap + q];
     int m = 1;
     while (m < L) m <<= 1;
+    __syncthreads();  // the previous run is done with s_net
     for (int j = t; j < m; j += 256) {
       uint64_t v = ~0ull;
       if (j < L) {
-        const float2 l = lo[i + j], hh = hi[i + j];
-        v = ((uint64_t)(uint32_t)morton64(l.x, l.y, hh.x, hh.y, kp) << 32) | (uint32_t)j;
+        const int64_t p = i + j;
+        v = ((uint64_t)(uint32_t)morton64(xmin[p], ymin[p], xmax[p], ymax[p], kp) << 32) |
+            (uint32_t)j;
       }
       s_net[j] = v;
     }
@@ -793,18 +977,33 @@ __global__ __launch_bounds__(256) void k_fixup_long(
         __syncthreads();
       }
     }
-    // the run's boxes in final order, from the (unchanged) staging buffers
-    for (int j = t; j < L; j += 256) {
-      const int64_t sp = i + (uint32_t)s_net[j];
-      const float2 l = lo[sp], hh = hi[sp];
-      xmin[i + j] = l.x;
-      ymin[i + j] = l.y;
-      xmax[i + j] = hh.x;
-      ymax[i + j] = hh.y;
-      id[i + j] = (int)ids[sp];
+    // every box of the run read (into registers) before any is written back
+    float4 rb[kPer];
+    int rid[kPer];
+    uint32_t src[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const int j = t + 256 * k;
+      src[k] = j < L ? (uint32_t)s_net[j] : (uint32_t)j;
+      if (src[k] != (uint32_t)j) {
+        const int64_t sp = i + src[k];
+        rb[k] = make_float4(xmin[sp], ymin[sp], xmax[sp], ymax[sp]);
+        rid[k] = id[sp];
+      }
     }
-    if (t == 0) runs[runs_cap + q] = (uint32_t)L;
     __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const int j = t + 256 * k;
+      if (src[k] != (uint32_t)j) {
+        const int64_t p = i + j;
+        xmin[p] = rb[k].x;
+        ymin[p] = rb[k].y;
+        xmax[p] = rb[k].z;
+        ymax[p] = rb[k].w;
+        id[p] = rid[k];
+      }
+    }
   }
 }
 
@@ -831,11 +1030,6 @@ __global__ __launch_bounds__(256) void k_fallback_zero(BuildCtrl* c, uint32_t* p
 // DPP lane permutations (quad xor 1, quad xor 2, half-row mirror, row mirror:
 // each pairs the lane with one in the other half of its 2-, 4-, 8- or
 // 16-lane block), wider ones from ds_bpermute.
-template <int CTRL>
-__device__ __forceinline__ uint32_t dpp_perm(uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, 0xF, 0xF, false);
-}
-
 __device__ __forceinline__ uint32_t min_key(float x, bool has) {
   return (has && x == x) ? f2ord(x) : 0xFFFFFFFFu;
 }
@@ -876,161 +1070,162 @@ __device__ __forceinline__ void group_mbr(float& a, float& b, float& cc, float& 
 // reduction over G lanes (lanes of a group hold consecutive positions, the
 // group's first lane writes): pos = R_sz + g*G, len = count
 // (src/rtree_bulkload_thrust.cuh:136-149).
+// (positions, node indices and pos values fit 32 bits: n < 2^30 and
+// r_sz + n < 2^31)
 template <int G>
-__device__ __forceinline__ void leaf_parent(int64_t i, int64_t n, float a, float b, float cc,
+__device__ __forceinline__ void leaf_parent(uint32_t i, uint32_t n, float a, float b, float cc,
                                             float d, bool has, const TreeView& tr,
-                                            int64_t l0start, int64_t r_sz, bool mbr_only) {
+                                            uint32_t l0start, uint32_t r_sz, bool mbr_only) {
   group_mbr<G>(a, b, cc, d, has);
   if (has && (i % G) == 0) {
-    const int64_t node = l0start + i / G;
+    const uint32_t node = l0start + i / G;
     tr.xmin[node] = a;
     tr.ymin[node] = b;
     tr.xmax[node] = cc;
     tr.ymax[node] = d;
     if (!mbr_only) {
       tr.pos[node] = (int)(r_sz + i);
-      const int64_t len = n - i;
-      tr.len[node] = (int)(len < G ? len : G);
+      const uint32_t len = n - i;
+      tr.len[node] = (int)(len < (uint32_t)G ? len : (uint32_t)G);
     }
   }
 }
 
-// Last stage of the payload sort: the boxes written to the caller's arrays in
-// final order and, for power-of-two F <= 64, the leaf-parent level built in
-// the same pass.  64-bit keys: runs of 2..kFixShort equal high halves that
-// touch the block's tile are sorted by (low half, position) in LDS, over a
-// window of the tile plus kFixShort positions each side; longer runs are
-// listed for k_fixup_long and written in high-half order here.
+// Last stage of the payload sort, in place on the caller's arrays (sorted by
+// the 32-bit key): for 64-bit keys the runs of equal high halves that lie in
+// one chunk of 64 positions sorted by (low half, position) within the wave,
+// the others listed; for power-of-two F <= 64 the leaf-parent level.  Thread
+// t holds positions tb + t + 256k, so each wave holds chunks of 64 consecutive
+// positions, and a group of G <= 64 never leaves its chunk.
+constexpr int kFinWaves = 256 / kWave;
+constexpr int kFinChunks = kFinTile / kWave;
+constexpr int kListCap = 512;  // listed runs buffered per block (at most kFinChunks + 1 per tile)
+
 template <int MODE, int G>
-__global__ __launch_bounds__(256) void k_final(
-    int64_t n, BuildCtrl* c, const uint32_t* __restrict__ ids, const float2* __restrict__ lo0,
-    const float2* __restrict__ lo1, const float2* __restrict__ hi0,
-    const float2* __restrict__ hi1, float* __restrict__ xmin, float* __restrict__ ymin,
-    float* __restrict__ xmax, float* __restrict__ ymax, int* __restrict__ id, TreeView tr,
-    int64_t l0start, int64_t r_sz, uint32_t* runs, uint32_t runs_cap) {
+__global__ __launch_bounds__(256) void k_final(int64_t n, BuildCtrl* c, float* __restrict__ xmin,
+                                               float* __restrict__ ymin,
+                                               float* __restrict__ xmax,
+                                               float* __restrict__ ymax, int* __restrict__ id,
+                                               TreeView tr, int64_t l0start, int64_t r_sz,
+                                               uint32_t* runs, uint32_t runs_cap) {
   constexpr bool M64 = MODE == kKeyMorton64;
-  constexpr int kPer = kFinTile / 256;  // owned positions per thread
-  // the window: slot j holds position tb - kFixShort + j; the owned position
-  // tb + t + 256k sits in slot kFixShort + t + 256k, and threads 0..31 also
-  // hold one halo slot each (0..15 before the tile, kFinTile+16.. after it)
-  __shared__ uint32_t s_hi[M64 ? kFinSpan : 1];  // high half of the slot's key
-  // (low half << 32) | slot of the box that lands in slot j
-  __shared__ uint64_t s_ck[M64 ? kFinSpan : 1];
-  const int cur = c->npass & 1;
-  const float2* lo = cur ? lo1 : lo0;
-  const float2* hi = cur ? hi1 : hi0;
-  const int t = threadIdx.x;
+  constexpr int kPer = kFinTile / 256;  // positions per thread
+  // high halves at each chunk's first and last position, and at the
+  // positions just before and after the tile
+  __shared__ uint32_t s_eh[2][M64 ? kFinChunks : 1];
+  __shared__ uint32_t s_nb[2];
+  __shared__ uint32_t s_list[M64 ? kListCap : 1];
+  __shared__ uint32_t s_nlist, s_base;
+  const int t = threadIdx.x, w = t >> 6, lane = lane_id();
   const KeyParams kp = key_params<MODE>(c);
-  const int64_t ntiles = (n + kFinTile - 1) / kFinTile;
-  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    __syncthreads();  // the previous tile is done with the shared arrays
-    const int64_t tb = tile * kFinTile;
-    const int64_t te = tb + kFinTile < n ? tb + kFinTile : n;
-    uint32_t k8[kPer];  // ids
-    float2 l8[kPer], h8[kPer];
+  // listed runs go to the global list in one atomic per block and flush
+  auto flush = [&]() {
+    const uint32_t cnt = s_nlist;  // (the same for every thread: read after a barrier)
+    if (cnt == 0) return;
+    if (t == 0) s_base = atomicAdd(&c->nruns, cnt);
+    __syncthreads();
+    const uint32_t base = s_base;
+    for (uint32_t i = t; i < cnt; i += 256) {
+      if (base + i < runs_cap) runs[base + i] = s_list[i];
+      else atomicOr(&c->fallback, 1u);
+    }
+    __syncthreads();
+    if (t == 0) s_nlist = 0;
+  };
+  if (M64 && t == 0) s_nlist = 0;
+  const uint32_t un = (uint32_t)n;  // (n < 2^30: positions fit 32 bits)
+  // one tile; FULL: every position of the tile exists (no bounds checks)
+  auto tile_body = [&](auto full_tag, uint32_t tb) {
+    constexpr bool FULL = decltype(full_tag)::value;
+    float4 b8[kPer];
+    int v8[kPer];  // ids (64-bit keys: they move with the boxes)
 #pragma unroll
     for (int k = 0; k < kPer; k++) {
-      const int64_t gp = tb + t + 256 * k;
-      const bool ok = gp < n;
-      k8[k] = ok ? ids[gp] : 0u;
-      l8[k] = ok ? lo[gp] : make_float2(0.f, 0.f);
-      h8[k] = ok ? hi[gp] : make_float2(0.f, 0.f);
+      const uint32_t gp = tb + t + 256u * k;
+      const bool ok = FULL || gp < un;
+      b8[k] = ok ? make_float4(xmin[gp], ymin[gp], xmax[gp], ymax[gp])
+                 : make_float4(0.f, 0.f, 0.f, 0.f);
+      if (M64) v8[k] = ok ? id[gp] : 0;
     }
-    if (M64) {
-      const int64_t ws = tb - kFixShort;  // (negative on the first tile)
-      const int64_t we = te + kFixShort < n ? te + kFixShort : n;
-      const int jlo = tb == 0 ? kFixShort : 0, jhi = (int)(we - ws);
-      int hj = -1;
-      float2 hl = make_float2(0.f, 0.f), hh = hl;
-      if (t < 2 * kFixShort) {
-        hj = t < kFixShort ? t : kFinTile + t;
-        if (hj >= jlo && hj < jhi) {
-          hl = lo[ws + hj];
-          hh = hi[ws + hj];
-        } else {
-          hj = -1;
-        }
+    if constexpr (M64) {
+      const uint32_t te = FULL ? tb + kFinTile : un;
+      const bool before = tb > 0, after = te < un;
+      if (t < 2 && (t == 0 ? before : after)) {
+        const uint32_t p = t == 0 ? tb - 1 : te;
+        uint32_t lo;
+        s_nb[t] = morton_cell(xmin[p], ymin[p], xmax[p], ymax[p], kp, &lo);
       }
-      // every slot's key, recomputed from its box: the high half finds the
-      // runs, (low half, slot) orders them
+      // high halves compared as cells, low halves as coordinates
+      uint32_t h8[kPer], l8[kPer];
 #pragma unroll
       for (int k = 0; k < kPer; k++) {
-        const int j = kFixShort + t + 256 * k;
-        if (j < jhi) {
-          const uint64_t key = morton64(l8[k].x, l8[k].y, h8[k].x, h8[k].y, kp);
-          s_hi[j] = (uint32_t)(key >> 32);
-          s_ck[j] = ((uint64_t)(uint32_t)key << 32) | (uint32_t)j;
-        }
-      }
-      if (hj >= 0) {
-        const uint64_t key = morton64(hl.x, hl.y, hh.x, hh.y, kp);
-        s_hi[hj] = (uint32_t)(key >> 32);
-        s_ck[hj] = ((uint64_t)(uint32_t)key << 32) | (uint32_t)hj;
+        h8[k] = morton_cell(b8[k].x, b8[k].y, b8[k].z, b8[k].w, kp, &l8[k]);
+        if (lane == 0) s_eh[0][k * kFinWaves + w] = h8[k];
+        if (lane == kWave - 1) s_eh[1][k * kFinWaves + w] = h8[k];
       }
       __syncthreads();
-      for (int r = 0; r <= kPer; r++) {
-        const int j = r < kPer ? kFixShort + t + 256 * r : hj;
-        if (j < jlo || j >= jhi) continue;
-        const uint32_t h = s_hi[j];
-        const int64_t gs = ws + j;
-        // a run starts at j (the window's first slot only at the array start:
-        // a run through it that reaches the tile is a long one)
-        if (j == jlo ? gs != 0 : s_hi[j - 1] == h) continue;
-        if (j + 1 >= jhi || s_hi[j + 1] != h) continue;  // a single item
-        int L = 2;
-        while (L <= kFixShort && j + L < jhi && s_hi[j + L] == h) L++;
-        if (L > kFixShort || (j + L == jhi && we < n)) {  // long (or running past the window)
-          if (gs >= tb && gs < te) {
-            const uint32_t at = atomicAdd(&c->nruns, 1u);
-            if (at < runs_cap) runs[at] = (uint32_t)gs;
-            else atomicOr(&c->fallback, 1u);
-          }
-          continue;
-        }
-        if (gs + L <= tb || gs >= te) continue;  // no position in the tile
-        for (int a2 = j + 1; a2 < j + L; a2++) {
-          const uint64_t v = s_ck[a2];
-          int b2 = a2 - 1;
-          while (b2 >= j && s_ck[b2] > v) {
-            s_ck[b2 + 1] = s_ck[b2];
-            b2--;
-          }
-          s_ck[b2 + 1] = v;
-        }
-      }
-      __syncthreads();
-      // positions that receive another slot's box (inside sorted runs)
 #pragma unroll
       for (int k = 0; k < kPer; k++) {
-        const int j = kFixShort + t + 256 * k;
-        if (j < jhi) {
-          const uint32_t js = (uint32_t)s_ck[j];
-          if (js != (uint32_t)j) {
-            l8[k] = lo[ws + js];
-            h8[k] = hi[ws + js];
-            k8[k] = ids[ws + js];
-          }
+        const uint32_t gp = tb + t + 256u * k;
+        const int ci = k * kFinWaves + w;  // chunk index in the tile
+        const uint32_t h = h8[k];
+        // same cell as the previous / next position (lanes 0 and 63: the
+        // neighbouring chunk's edge from LDS)
+        bool eqp = dpp_perm<kDppWaveShr1>(h) == h, eqn = dpp_perm<kDppWaveShl1>(h) == h;
+        if (lane == 0) eqp = (ci > 0 || before) && (ci > 0 ? s_eh[1][ci - 1] : s_nb[0]) == h;
+        if (lane == kWave - 1)
+          eqn = (ci + 1 < kFinChunks ? s_eh[0][ci + 1] : s_nb[1]) == h;
+        eqp = eqp && (FULL || gp < un);
+        eqn = eqn && gp + 1 < un;
+        int st, en;  // this lane's run in the chunk (-1 / kWave: beyond it)
+        run_bounds(eqp, eqn, &st, &en);
+        const bool inrun = eqp || eqn;
+        const bool edge = (ci == 0 && st == 0 && before) ||
+                          (ci == kFinChunks - 1 && en == kWave - 1 && after);
+        const bool local = inrun && st >= 0 && en < kWave && !edge;
+        if (inrun && st == lane && !local) s_list[atomicAdd(&s_nlist, 1u)] = gp;
+        if (!ballot(local)) continue;
+        // (runs that the last sort pass could not settle: split between its
+        // chunks or tiles)
+        const int dest = local_run_dest(local, st, en, l8[k]);
+        const uint64_t moved = ballot(dest != lane);
+        if (!moved) continue;
+        wave_permute(dest, b8[k], v8[k]);
+        // whole 32-byte sectors (8 positions) around the moved boxes are
+        // written: a partly written sector costs the memory a read first
+        if (((moved >> (lane & ~7)) & 0xFFull) && (FULL || gp < un)) {
+          xmin[gp] = b8[k].x;
+          ymin[gp] = b8[k].y;
+          xmax[gp] = b8[k].z;
+          ymax[gp] = b8[k].w;
+          id[gp] = v8[k];
         }
       }
     }
+    if constexpr (G > 0) {
 #pragma unroll
-    for (int k = 0; k < kPer; k++) {
-      const int64_t gp = tb + t + 256 * k;
-      const bool has = gp < n;
-      if (has) {
-        xmin[gp] = l8[k].x;
-        ymin[gp] = l8[k].y;
-        xmax[gp] = h8[k].x;
-        ymax[gp] = h8[k].y;
-        id[gp] = (int)k8[k];
+      for (int k = 0; k < kPer; k++) {
+        const uint32_t gp = tb + t + 256u * k;
+        leaf_parent<G>(gp, un, b8[k].x, b8[k].y, b8[k].z, b8[k].w, FULL || gp < un, tr,
+                       (uint32_t)l0start, (uint32_t)r_sz, false);
       }
-      if constexpr (G > 0)
-        leaf_parent<G>(gp, n, l8[k].x, l8[k].y, h8[k].x, h8[k].y, has, tr, l0start, r_sz, false);
     }
+  };
+  const uint32_t ntiles = (un + kFinTile - 1) / kFinTile;
+  for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    __syncthreads();  // the previous tile is done with the shared words
+    if (M64 && s_nlist > (uint32_t)(kListCap - 2 * kFinChunks)) flush();
+    const uint32_t tb = tile * kFinTile;
+    if (tb + kFinTile <= un) tile_body(std::true_type{}, tb);
+    else tile_body(std::false_type{}, tb);
+  }
+  if constexpr (M64) {
+    __syncthreads();
+    flush();
   }
 }
 
-// Leaf-parent MBRs over the runs k_fixup_long rewrote (nodes shared by two
+// Leaf-parent MBRs over the listed runs, one wave each (nodes shared by two
 // runs are computed twice, identically).
 template <int G>
 __global__ __launch_bounds__(256) void k_fixup_groups(
@@ -1039,14 +1234,15 @@ __global__ __launch_bounds__(256) void k_fixup_groups(
     const float* __restrict__ xmax, const float* __restrict__ ymax, TreeView tr,
     int64_t l0start, int64_t r_sz) {
   const uint32_t nr = c->nruns < runs_cap ? c->nruns : runs_cap;
-  if (blockIdx.x >= nr || c->fallback) return;
-  for (uint32_t q = blockIdx.x; q < nr; q += gridDim.x) {
+  if (blockIdx.x * 4u >= nr || c->fallback) return;
+  const uint32_t nw = gridDim.x * 4u;
+  for (uint32_t q = (blockIdx.x * 256u + threadIdx.x) >> 6; q < nr; q += nw) {
     const int64_t i = runs[q], L = runs[runs_cap + q];
     const int64_t p0 = (i / G) * G;
     int64_t p1 = ((i + L - 1) / G + 1) * G;
     if (p1 > n) p1 = n;
-    for (int64_t base = p0; base < p1; base += 256) {
-      const int64_t p = base + threadIdx.x;
+    for (int64_t base = p0; base < p1; base += kWave) {
+      const int64_t p = base + lane_id();
       const bool has = p < p1;
       float a = 0.f, b = 0.f, cc = 0.f, d = 0.f;
       if (has) {
@@ -1162,7 +1358,6 @@ static int pow2_group(int F) {
 struct SortBufs {
   uint64_t* kv[2];
   uint32_t* keys;    // sort keys of the input order (k_hist -> first pass)
-  uint32_t* ids;     // ids in sorted order (last pass -> final pass)
   float2* lo[2];
   float2* hi[2];
   uint64_t* k64[2];  // fallback: 64-bit keys (the kv buffers)
@@ -1174,24 +1369,31 @@ struct SortBufs {
   uint32_t runs_cap;
 };
 
-// Final order, leaf-parent level and, for 64-bit keys, the tie stages.
+// Ties of the 64-bit keys and the leaf-parent level, in place on the caller's
+// arrays (nothing to do for 32-bit keys and a fanout without the fused level).
 template <int MODE, int G>
 static void launch_tail(Prof& pf, hipStream_t s, int64_t n, BuildCtrl* c, const SortBufs& b,
                         curtree_bbox_f32* box, TreeView tr, int64_t l0start, int64_t r_sz) {
+  if (MODE != kKeyMorton64 && G == 0) return;
   pf.mark(s, "k_final");
   hipLaunchKernelGGL((k_final<MODE, G>),
                      dim3((unsigned)std::min<int64_t>((n + kFinTile - 1) / kFinTile,
                                                       CT_FINAL_GRID)),
-                     dim3(256), 0, s, n, c, (const uint32_t*)b.ids, b.lo[0], b.lo[1], b.hi[0],
-                     b.hi[1], box->xmin, box->ymin, box->xmax, box->ymax, box->id, tr, l0start,
-                     r_sz, b.runs, b.runs_cap);
+                     dim3(256), 0, s, n, c, box->xmin, box->ymin, box->xmax, box->ymax, box->id,
+                     tr, l0start, r_sz, b.runs, b.runs_cap);
   if (MODE != kKeyMorton64) return;
   pf.mark(s, "k_fixup_long");
-  hipLaunchKernelGGL(k_fixup_long, dim3(kFixLongSlots), dim3(256), 0, s, n, c,
-                     (const uint32_t*)b.ids, b.lo[0], b.lo[1], b.hi[0], b.hi[1], box->xmin,
-                     box->ymin, box->xmax, box->ymax, box->id, b.runs, b.runs_cap);
+  hipLaunchKernelGGL(k_run_len, dim3(kFixSlots), dim3(256), 0, s, n, c, (const float*)box->xmin,
+                     (const float*)box->ymin, (const float*)box->xmax, (const float*)box->ymax,
+                     b.runs, b.runs_cap);
+  hipLaunchKernelGGL(k_fixup_short, dim3(kFixSlots), dim3(256), 0, s, n, c, box->xmin,
+                     box->ymin, box->xmax, box->ymax, box->id, (const uint32_t*)b.runs,
+                     b.runs_cap);
+  hipLaunchKernelGGL(k_fixup_long, dim3(kFixSlots), dim3(256), 0, s, n, c, box->xmin,
+                     box->ymin, box->xmax, box->ymax, box->id, (const uint32_t*)b.runs,
+                     b.runs_cap);
   if constexpr (G > 0)
-    hipLaunchKernelGGL(k_fixup_groups<G>, dim3(kFixLongSlots), dim3(256), 0, s, n,
+    hipLaunchKernelGGL(k_fixup_groups<G>, dim3(kFixSlots), dim3(256), 0, s, n,
                        (const BuildCtrl*)c, (const uint32_t*)b.runs, b.runs_cap, box->xmin,
                        box->ymin, box->xmax, box->ymax, tr, l0start, r_sz);
   // the full LSD sort of 64-bit keys: every launch exits at once unless a
@@ -1203,7 +1405,8 @@ static void launch_tail(Prof& pf, hipStream_t s, int64_t n, BuildCtrl* c, const 
   hipLaunchKernelGGL(k_keygen64, dim3(sgrid), dim3(256), 0, s, n, (const float*)box->xmin,
                      (const float*)box->ymin, (const float*)box->xmax, (const float*)box->ymax,
                      (const int*)box->id, c, b.k64[0], b.k64[1], b.aos, 1);
-  hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(256), 0, s, n, 8, c, 4, 8, 0x76543210ull, 0, 1);
+  hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(256), 0, s, n, 8, c, 4, 8, 0x76543210ull, 0, 1,
+                     1);
   const int64_t ntiles = (n + kSortTile - 1) / kSortTile;
   for (int st = 4; st < kMaxSteps; st++)
     hipLaunchKernelGGL(k_onesweep<uint64_t>, dim3((unsigned)std::min(ntiles, kOnesweepGrid)),
@@ -1306,10 +1509,10 @@ int build_f32(Context& ctx, curtree_bbox_f32* box, curtree_rtree_f32* rt, int ke
       (rc = ws.reserve(WsBuildStatus, status_words * 4)) ||
       (rc = ws.reserve(WsBuildCtrl, sizeof(BuildCtrl))))
     return fail(rc);
-  // long runs of equal high halves (64-bit keys): at most n / (kFixShort + 1),
-  // start and length of each
-  const uint32_t runs_cap = m64 ? (uint32_t)std::min<int64_t>(n / (kFixShort + 1) + 1, 1 << 22) : 1;
-  if ((rc = ws.reserve(WsBuildRuns, (size_t)runs_cap * 8))) return fail(rc);
+  // listed runs of equal high halves (64-bit keys): start, length and the
+  // list of the long ones; at most two per chunk of 64 positions
+  const uint32_t runs_cap = m64 ? (uint32_t)(2 * ((n + kWave - 1) / kWave) + 2) : 1;
+  if ((rc = ws.reserve(WsBuildRuns, (size_t)runs_cap * 12))) return fail(rc);
   BuildCtrl* c = (BuildCtrl*)ws.ptr(WsBuildCtrl);
   SortBufs sb;
   sb.kv[0] = (uint64_t*)ws.ptr(WsBuildKeys0);
@@ -1323,7 +1526,6 @@ int build_f32(Context& ctx, curtree_bbox_f32* box, curtree_rtree_f32* rt, int ke
   sb.k64[1] = sb.kv[1];
   sb.v[0] = (uint32_t*)ws.ptr(WsBuildVals0);
   sb.v[1] = (uint32_t*)ws.ptr(WsBuildVals1);
-  sb.ids = sb.v[0];   // (the fallback's positions come after the final pass)
   sb.keys = sb.v[1];
   sb.aos = (uint4*)ws.ptr(WsBuildAos);
   sb.status = (uint32_t*)ws.ptr(WsBuildStatus);
@@ -1349,15 +1551,16 @@ int build_f32(Context& ctx, curtree_bbox_f32* box, curtree_rtree_f32* rt, int ke
   else
     hipLaunchKernelGGL(k_hist<kKeyRef32>, dim3(sgrid), dim3(256), 0, s, n, box->xmin, box->ymin,
                        box->xmax, box->ymax, c, sb.keys, sb.status, (int64_t)(4 * step_words));
-  // (the last step that runs writes the ids alone)
+  // (the last step that runs writes the caller's arrays)
   pf.mark(s, "k_sort_plan");
-  hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(256), 0, s, n, 4, c, 0, 4, 0x3210ull, 0, 0);
+  hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(256), 0, s, n, 4, c, 0, 4, 0x3210ull, 0, 0, 2);
   for (int st = 0; st < 4; st++) {
     pf.mark(s, "k_sweep_kv");
     hipLaunchKernelGGL(k_sweep_kv, dim3((unsigned)ntiles), dim3(kSortThreads), 0, s, n, st,
-                       box->xmin, box->ymin, box->xmax, box->ymax, box->id,
-                       (const uint32_t*)sb.keys, sb.kv[0], sb.kv[1], sb.ids, sb.lo[0], sb.lo[1],
-                       sb.hi[0], sb.hi[1], c, sb.status + (size_t)st * step_words);
+                       box->xmin,
+                       box->ymin, box->xmax, box->ymax, box->id, (const uint32_t*)sb.keys,
+                       sb.kv[0], sb.kv[1], sb.lo[0], sb.lo[1], sb.hi[0], sb.hi[1], c,
+                       sb.status + (size_t)st * step_words);
   }
 
   TreeView tr{rt->xmin, rt->ymin, rt->xmax, rt->ymax, rt->pos, rt->len};
@@ -1453,7 +1656,8 @@ int pairs_canonical_sort(Context& ctx, int* fid, int* tid, int64_t n, hipStream_
   hipLaunchKernelGGL(k_pair_pack, dim3(sgrid), dim3(256), 0, s, n, fid, tid, c, k0, status,
                      (int64_t)status_words);
   pf.mark(s, "k_sort_plan");
-  hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(256), 0, s, n, 8, c, 0, 8, 0x76543210ull, 1, 0);
+  hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(256), 0, s, n, 8, c, 0, 8, 0x76543210ull, 1, 0,
+                     1);
   for (int p = 0; p < 8; p++) {
     pf.mark(s, "k_onesweep_pairs");
     hipLaunchKernelGGL(k_onesweep<uint64_t>, dim3((unsigned)std::min(ntiles, kOnesweepGrid)),

@@ -43,12 +43,13 @@ __device__ __forceinline__ float mbr_max(float a, float b) {
 
 // Order-preserving map float -> uint32 (for atomic min/max of non-NaN values;
 // -0 maps below +0).
+// (negative: all bits flipped; otherwise the sign bit set)
 __device__ __forceinline__ uint32_t f2ord(float f) {
-  uint32_t u = __float_as_uint(f);
-  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+  const uint32_t u = __float_as_uint(f);
+  return u ^ ((uint32_t)((int32_t)u >> 31) | 0x80000000u);
 }
 __device__ __forceinline__ float ord2f(uint32_t u) {
-  return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
+  return __uint_as_float(u ^ ((uint32_t)((int32_t)~u >> 31) | 0x80000000u));
 }
 
 // ---------------------------------------------------------------------------

@@ -166,14 +166,16 @@ def _clustered_points(sizes, seed, background):
     return (x, y, x.copy(), y.copy())
 
 
-@pytest.mark.parametrize("case", ["short_runs", "long_runs", "too_long_runs"])
+@pytest.mark.parametrize("case", ["short_runs", "wave_runs", "long_runs", "too_long_runs"])
 def test_morton64_equal_high_half_runs(case):
     # the 64-bit sort orders by the high half, then sorts runs of equal high
-    # halves by (low half, input position) in place: runs of 2..32 by one
-    # thread, 33..1025 by a block network; a longer run takes the full
-    # eight-pass path.  Every case must equal the oracle's stable sort.
+    # halves by (low half, input position) in place: runs inside one chunk of
+    # 64 positions by its wave, other runs of up to 64 by one wave each, up to
+    # 4096 by a block network; a longer run takes the full eight-pass path.
+    # Every case must equal the oracle's stable sort.
     r = np.random.default_rng(7)
     sizes = {"short_runs": r.integers(2, 33, 3000),
+             "wave_runs": r.integers(48, 81, 400),
              "long_runs": r.integers(33, 1026, 40),
              "too_long_runs": [5000, 40, 3]}[case]
     bx = _clustered_points(sizes, seed=len(case), background=100_000)
