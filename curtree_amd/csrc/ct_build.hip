@@ -235,14 +235,14 @@ __device__ __forceinline__ uint32_t sort_key32(float a, float b, float cc, float
   return (uint32_t)(morton64(a, b, cc, d, p) >> 32);
 }
 
-// The 32-bit sort keys (read by the first pass), the histograms of their four
-// digits, and the status words of the sort steps cleared.
+// The histograms of the four digits of the 32-bit sort keys (the first sort
+// pass computes the keys again from the boxes it reads), and the status words
+// of the sort steps cleared.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_hist(int64_t n, const float* __restrict__ xmin,
                                               const float* __restrict__ ymin,
                                               const float* __restrict__ xmax,
                                               const float* __restrict__ ymax, BuildCtrl* c,
-                                              uint32_t* __restrict__ keys,
                                               uint32_t* __restrict__ zero_ptr,
                                               int64_t zero_words) {
   __shared__ uint32_t sh[4 * 256];
@@ -252,7 +252,6 @@ __global__ __launch_bounds__(256) void k_hist(int64_t n, const float* __restrict
   const int64_t stride = (int64_t)gridDim.x * 256;
   for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += stride) {
     const uint32_t k = sort_key32<MODE>(xmin[i], ymin[i], xmax[i], ymax[i], kp);
-    keys[i] = k;
     // a digit that the whole wave shares is added once (degenerate keys, such
     // as REF32 on unit-square data, would otherwise serialise on one address)
     const uint64_t act = ballot(true);
@@ -315,18 +314,22 @@ __global__ __launch_bounds__(256) void k_keygen64(int64_t n, const float* __rest
 // every key is skipped (it would be the identity permutation), except that at
 // least `min_steps` steps run: the first trivial steps are kept when fewer
 // digits vary (a later stage reads the buffers a step writes; the payload
-// sort's last step must not be its first, which reads the arrays it writes).
-__device__ void plan_steps(BuildCtrl* c, int s0, int ns, uint64_t digits, int min_steps) {
+// sort's last step must not be its first, which reads the arrays it writes),
+// and that `keep_first` keeps the range's first step (the payload sort's
+// first launch is the one that computes the keys).
+__device__ void plan_steps(BuildCtrl* c, int s0, int ns, uint64_t digits, int min_steps,
+                           bool keep_first) {
   int varying = 0;
   for (int k = 0; k < ns; k++)
-    if (!((c->trivial >> (int)((digits >> (4 * k)) & 15u)) & 1u)) varying++;
+    if (!((c->trivial >> (int)((digits >> (4 * k)) & 15u)) & 1u) || (keep_first && k == 0))
+      varying++;
   int force = min_steps - varying;  // identity steps that still run
   int J = c->npass, first = c->first;
   for (int k = 0; k < ns; k++) {
     const int st = s0 + k, d = (int)((digits >> (4 * k)) & 15u);
     c->digit[st] = d;
     c->ticket[st] = 0;
-    bool run = !((c->trivial >> d) & 1u);
+    bool run = !((c->trivial >> d) & 1u) || (keep_first && k == 0);
     if (!run && force > 0) {
       run = true;
       force--;
@@ -346,12 +349,12 @@ __device__ void plan_steps(BuildCtrl* c, int s0, int ns, uint64_t digits, int mi
 }
 
 // Digit offsets for digits 0..D-1 from the histograms and the plan of steps
-// [s0, s0+ns) (at least min_steps of them run).  `gate`: plan only if the
-// fallback sort was asked for (its first step then reads input positions as
-// values again).
+// [s0, s0+ns) (at least min_steps of them run; keep_first: the first one
+// runs).  `gate`: plan only if the fallback sort was asked for (its first step
+// then reads input positions as values again).
 __global__ __launch_bounds__(256) void k_sort_plan(int64_t n, int D, BuildCtrl* c, int s0, int ns,
                                                    uint64_t digits, int keep_last_keys, int gate,
-                                                   int min_steps) {
+                                                   int min_steps, int keep_first) {
   __shared__ uint32_t s_tmp[4];
   __shared__ uint32_t s_triv;
   if (gate && !c->fallback) return;
@@ -369,7 +372,7 @@ __global__ __launch_bounds__(256) void k_sort_plan(int64_t n, int D, BuildCtrl* 
   if (threadIdx.x == 0) {
     c->trivial = s_triv;
     if (gate) c->first = -1;
-    plan_steps(c, s0, ns, digits, min_steps);
+    plan_steps(c, s0, ns, digits, min_steps, keep_first != 0);
     // the last step writes values only, unless a later stage reads the keys
     if (!keep_last_keys)
       for (int st = s0 + ns - 1; st >= s0; st--)
@@ -670,26 +673,30 @@ __device__ __forceinline__ void wave_permute(int dest, float4& b, int& v) {
 // each reordered through LDS and scattered in runs of equal digits.  FULL:
 // every item of the tile exists (no predicated memory operations, so nothing
 // waits on earlier stores at block joins).
-template <bool FULL>
+template <bool FULL, bool FIRST>
 __device__ __forceinline__ void sweep_move(
-    const uint64_t (&kv)[kSweepItems], const uint32_t (&pos)[kSweepItems], int64_t wbase,
-    int64_t n, int tn, int shift, bool first, bool last, float* __restrict__ xmin,
-    float* __restrict__ ymin, float* __restrict__ xmax, float* __restrict__ ymax,
-    int* __restrict__ id, const float2* __restrict__ lin, const float2* __restrict__ hin,
-    uint64_t* __restrict__ kout, float2* __restrict__ lout, float2* __restrict__ hout,
-    SweepShared& sh, uint64_t* s_buf, uint8_t* s_dig) {
+    uint64_t (&kv)[kSweepItems], const uint32_t (&pos)[kSweepItems],
+    float2 (&vlo)[kSweepItems], float2 (&vhi)[kSweepItems], int64_t wbase, int64_t n, int tn,
+    int shift, bool last, float* __restrict__ xmin, float* __restrict__ ymin,
+    float* __restrict__ xmax, float* __restrict__ ymax, int* __restrict__ id,
+    const float2* __restrict__ lin, const float2* __restrict__ hin, uint64_t* __restrict__ kout,
+    float2* __restrict__ lout, float2* __restrict__ hout, SweepShared& sh, uint64_t* s_buf,
+    uint8_t* s_dig) {
   const int t = threadIdx.x, lane = lane_id();
-  // the box halves are loaded first: their loads are then older than every
-  // store of this tile, and waiting for them never waits for a store
-  float2 vlo[kSweepItems], vhi[kSweepItems];
-  if (first) {
+  // later passes load the box halves here, first: their loads are then older
+  // than every store of this tile, and waiting for them never waits for a
+  // store (the first pass read them to compute the keys, and left the low
+  // halves in LDS at the item's tile index)
+  if (FIRST) {
+    const int64_t tbase = wbase - (threadIdx.x >> 6) * (kWave * kSweepItems);
+    const int* pid = id + wbase + lane;
 #pragma unroll
     for (int i = 0; i < kSweepItems; i++) {
-      const int64_t idx = wbase + i * kWave + lane;
-      const bool ok = FULL || idx < n;
-      vlo[i] = ok ? make_float2(xmin[idx], ymin[idx]) : make_float2(0.f, 0.f);
-      vhi[i] = ok ? make_float2(xmax[idx], ymax[idx]) : make_float2(0.f, 0.f);
+      const bool ok = FULL || wbase + i * kWave + lane < n;
+      kv[i] |= (uint32_t)(ok ? pid[i * kWave] : 0);
+      vlo[i] = ((const float2*)s_buf)[(wbase - tbase) + i * kWave + lane];
     }
+    __syncthreads();  // every stashed half is read before the keys overwrite LDS
   } else {
 #pragma unroll
     for (int i = 0; i < kSweepItems; i++) {
@@ -758,16 +765,18 @@ __device__ __forceinline__ void sweep_move(
   }
 }
 
-// One 8-bit pass of the payload sort.  The first pass that runs reads the
-// sort keys k_hist stored and the caller's arrays; the last one writes the
-// ids and boxes into the caller's arrays in the order of the 32-bit key (the
-// plan runs at least two passes, so the last never reads what it writes).
+// One 8-bit pass of the payload sort.  FIRST (step 0, which the plan always
+// runs): reads the caller's arrays and computes the keys of MODE from the
+// boxes; the last pass writes the ids and boxes into the caller's arrays in
+// the order of the 32-bit key (the plan runs at least two passes, so the last
+// never reads what it writes).
+template <bool FIRST, int MODE>
 __global__ __launch_bounds__(kSortThreads) void k_sweep_kv(
     int64_t n, int step, float* __restrict__ xmin, float* __restrict__ ymin,
     float* __restrict__ xmax, float* __restrict__ ymax, int* __restrict__ id,
-    const uint32_t* __restrict__ keys, uint64_t* __restrict__ kv0, uint64_t* __restrict__ kv1,
-    float2* __restrict__ lo0, float2* __restrict__ lo1, float2* __restrict__ hi0,
-    float2* __restrict__ hi1, BuildCtrl* c, uint32_t* __restrict__ status) {
+    uint64_t* __restrict__ kv0, uint64_t* __restrict__ kv1, float2* __restrict__ lo0,
+    float2* __restrict__ lo1, float2* __restrict__ hi0, float2* __restrict__ hi1, BuildCtrl* c,
+    uint32_t* __restrict__ status) {
   __shared__ SweepShared sh;
   __shared__ uint64_t s_buf[kSweepTile];
   __shared__ uint8_t s_dig[kSweepTile];
@@ -776,8 +785,9 @@ __global__ __launch_bounds__(kSortThreads) void k_sweep_kv(
   const int t = threadIdx.x, w = t >> 6, lane = lane_id();
   const int pass = c->digit[step];
   const int shift = 32 + 8 * pass;
-  const bool first = step == c->first;
   const bool last = c->write_keys[step] == 0;
+  KeyParams kp{0.f, 0.f, 0.f, 0.f};
+  if (FIRST) kp = key_params<MODE>(c);
   if (t == 0) sh.tile = atomicAdd(&c->ticket[step], 1u);
   for (int i = t; i < kSortWaves * 256; i += kSortThreads) (&sh.wh[0][0])[i] = 0;
   __syncthreads();
@@ -794,19 +804,26 @@ __global__ __launch_bounds__(kSortThreads) void k_sweep_kv(
   const int64_t wbase = tbase + (int64_t)w * (kWave * kSweepItems);
   uint64_t kv[kSweepItems];
   uint32_t dig[kSweepItems], pos[kSweepItems];
+  float2 vlo[kSweepItems], vhi[kSweepItems];
   // items (each load loop issues all of its loads before the first use)
-  if (first) {
-    uint32_t k32[kSweepItems];
-    int ident[kSweepItems];
+  if (FIRST) {
+    // (one base per array and constant offsets: few address registers; the
+    // ids join the keys after the ranking)
+    const int64_t b0 = wbase + lane;
+    const float *px0 = xmin + b0, *py0 = ymin + b0, *px1 = xmax + b0, *py1 = ymax + b0;
 #pragma unroll
     for (int i = 0; i < kSweepItems; i++) {
-      const int64_t idx = wbase + i * kWave + lane;
-      const bool ok = idx < n;
-      k32[i] = ok ? keys[idx] : 0u;
-      ident[i] = ok ? id[idx] : 0;
+      const bool ok = b0 + i * kWave < n;
+      vlo[i] = ok ? make_float2(px0[i * kWave], py0[i * kWave]) : make_float2(0.f, 0.f);
+      vhi[i] = ok ? make_float2(px1[i * kWave], py1[i * kWave]) : make_float2(0.f, 0.f);
     }
 #pragma unroll
-    for (int i = 0; i < kSweepItems; i++) kv[i] = ((uint64_t)k32[i] << 32) | (uint32_t)ident[i];
+    for (int i = 0; i < kSweepItems; i++) {
+      kv[i] = (uint64_t)sort_key32<MODE>(vlo[i].x, vlo[i].y, vhi[i].x, vhi[i].y, kp) << 32;
+      // the low halves wait in LDS (unused until the moves) while the tile
+      // is ranked: registers for all of the box would not fit
+      ((float2*)s_buf)[(wbase - tbase) + i * kWave + lane] = vlo[i];
+    }
   } else {
 #pragma unroll
     for (int i = 0; i < kSweepItems; i++) {
@@ -820,11 +837,11 @@ __global__ __launch_bounds__(kSortThreads) void k_sweep_kv(
 
   const int64_t rem = n - tbase;
   if (rem >= kSweepTile)  // full tile: no predicated loads or stores
-    sweep_move<true>(kv, pos, wbase, n, kSweepTile, shift, first, last, xmin, ymin, xmax, ymax,
-                     id, lin, hin, kout, lout, hout, sh, s_buf, s_dig);
+    sweep_move<true, FIRST>(kv, pos, vlo, vhi, wbase, n, kSweepTile, shift, last, xmin, ymin,
+                            xmax, ymax, id, lin, hin, kout, lout, hout, sh, s_buf, s_dig);
   else
-    sweep_move<false>(kv, pos, wbase, n, (int)rem, shift, first, last, xmin, ymin, xmax, ymax,
-                      id, lin, hin, kout, lout, hout, sh, s_buf, s_dig);
+    sweep_move<false, FIRST>(kv, pos, vlo, vhi, wbase, n, (int)rem, shift, last, xmin, ymin,
+                             xmax, ymax, id, lin, hin, kout, lout, hout, sh, s_buf, s_dig);
 }
 
 // ---------------------------------------------------------------------------
@@ -1406,7 +1423,7 @@ static void launch_tail(Prof& pf, hipStream_t s, int64_t n, BuildCtrl* c, const 
                      (const float*)box->ymin, (const float*)box->xmax, (const float*)box->ymax,
                      (const int*)box->id, c, b.k64[0], b.k64[1], b.aos, 1);
   hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(256), 0, s, n, 8, c, 4, 8, 0x76543210ull, 0, 1,
-                     1);
+                     1, 0);
   const int64_t ntiles = (n + kSortTile - 1) / kSortTile;
   for (int st = 4; st < kMaxSteps; st++)
     hipLaunchKernelGGL(k_onesweep<uint64_t>, dim3((unsigned)std::min(ntiles, kOnesweepGrid)),
@@ -1542,25 +1559,26 @@ int build_f32(Context& ctx, curtree_bbox_f32* box, curtree_rtree_f32* rt, int ke
     hipLaunchKernelGGL(k_extent, dim3(sgrid), dim3(256), 0, s, n, box->xmin, box->ymin,
                        box->xmax, box->ymax, c);
   }
-  // sort keys and their histograms; clears the status words of steps 0..3
+  // histograms of the sort keys' digits; clears the status words of steps 0..3
   pf.mark(s, "k_hist");
   if (m64)
     hipLaunchKernelGGL(k_hist<kKeyMorton64>, dim3(sgrid), dim3(256), 0, s, n, box->xmin,
-                       box->ymin, box->xmax, box->ymax, c, sb.keys, sb.status,
-                       (int64_t)(4 * step_words));
+                       box->ymin, box->xmax, box->ymax, c, sb.status, (int64_t)(4 * step_words));
   else
     hipLaunchKernelGGL(k_hist<kKeyRef32>, dim3(sgrid), dim3(256), 0, s, n, box->xmin, box->ymin,
-                       box->xmax, box->ymax, c, sb.keys, sb.status, (int64_t)(4 * step_words));
+                       box->xmax, box->ymax, c, sb.status, (int64_t)(4 * step_words));
   // (the last step that runs writes the caller's arrays)
   pf.mark(s, "k_sort_plan");
-  hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(256), 0, s, n, 4, c, 0, 4, 0x3210ull, 0, 0, 2);
+  hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(256), 0, s, n, 4, c, 0, 4, 0x3210ull, 0, 0, 2,
+                     1);
   for (int st = 0; st < 4; st++) {
     pf.mark(s, "k_sweep_kv");
-    hipLaunchKernelGGL(k_sweep_kv, dim3((unsigned)ntiles), dim3(kSortThreads), 0, s, n, st,
-                       box->xmin,
-                       box->ymin, box->xmax, box->ymax, box->id, (const uint32_t*)sb.keys,
-                       sb.kv[0], sb.kv[1], sb.lo[0], sb.lo[1], sb.hi[0], sb.hi[1], c,
-                       sb.status + (size_t)st * step_words);
+    auto kern = st > 0 ? k_sweep_kv<false, 0>
+                : m64     ? k_sweep_kv<true, kKeyMorton64>
+                          : k_sweep_kv<true, kKeyRef32>;
+    hipLaunchKernelGGL(kern, dim3((unsigned)ntiles), dim3(kSortThreads), 0, s, n, st, box->xmin,
+                       box->ymin, box->xmax, box->ymax, box->id, sb.kv[0], sb.kv[1], sb.lo[0],
+                       sb.lo[1], sb.hi[0], sb.hi[1], c, sb.status + (size_t)st * step_words);
   }
 
   TreeView tr{rt->xmin, rt->ymin, rt->xmax, rt->ymax, rt->pos, rt->len};
@@ -1657,7 +1675,7 @@ int pairs_canonical_sort(Context& ctx, int* fid, int* tid, int64_t n, hipStream_
                      (int64_t)status_words);
   pf.mark(s, "k_sort_plan");
   hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(256), 0, s, n, 8, c, 0, 8, 0x76543210ull, 1, 0,
-                     1);
+                     1, 0);
   for (int p = 0; p < 8; p++) {
     pf.mark(s, "k_onesweep_pairs");
     hipLaunchKernelGGL(k_onesweep<uint64_t>, dim3((unsigned)std::min(ntiles, kOnesweepGrid)),
