@@ -250,21 +250,35 @@ __global__ __launch_bounds__(256) void k_hist(int64_t n, const float* __restrict
   const KeyParams kp = key_params<MODE>(c);
   __syncthreads();
   const int64_t stride = (int64_t)gridDim.x * 256;
-  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += stride) {
-    const uint32_t k = sort_key32<MODE>(xmin[i], ymin[i], xmax[i], ymax[i], kp);
-    // a digit that the whole wave shares is added once (degenerate keys, such
-    // as REF32 on unit-square data, would otherwise serialise on one address)
-    const uint64_t act = ballot(true);
+  constexpr int U = 4;  // boxes per thread and step, all loaded before any is used
+  for (int64_t i0 = blockIdx.x * (256ll * U) + threadIdx.x; i0 < n; i0 += stride * U) {
+    float4 b[U];
 #pragma unroll
-    for (int p = 0; p < 4; p++) {
-      const uint32_t d = (k >> (8 * p)) & 255u;
-      const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
-      const uint64_t same = ballot(d == d0);
-      if (same == act) {
-        if (lane_id() == __ffsll((long long)act) - 1)
-          atomicAdd(&sh[p * 256 + d0], (uint32_t)popc64(act));
-      } else {
-        atomicAdd(&sh[p * 256 + d], 1u);
+    for (int u = 0; u < U; u++) {
+      const int64_t i = i0 + 256 * u;
+      b[u] = i < n ? make_float4(xmin[i], ymin[i], xmax[i], ymax[i])
+                   : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const bool ok = i0 + 256 * u < n;
+      const uint32_t k = sort_key32<MODE>(b[u].x, b[u].y, b[u].z, b[u].w, kp);
+      // a digit that the whole wave shares is added once (degenerate keys,
+      // such as REF32 on unit-square data, would otherwise serialise on one
+      // address)
+      const uint64_t act = ballot(ok);
+      if (!act) break;
+#pragma unroll
+      for (int p = 0; p < 4; p++) {
+        const uint32_t d = (k >> (8 * p)) & 255u;
+        const uint32_t d0 = __builtin_amdgcn_readlane(d, __ffsll((long long)act) - 1);
+        const uint64_t same = ballot(ok && d == d0);
+        if (same == act) {
+          if (lane_id() == __ffsll((long long)act) - 1)
+            atomicAdd(&sh[p * 256 + d0], (uint32_t)popc64(act));
+        } else if (ok) {
+          atomicAdd(&sh[p * 256 + d], 1u);
+        }
       }
     }
   }
@@ -411,9 +425,10 @@ constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr int kSweepItems = CT_SWEEP_ITEMS;    // payload passes
 constexpr int kSweepTile = kSortThreads * kSweepItems;
 constexpr uint32_t kFlagA = 1u << 30, kFlagP = 2u << 30, kValMask = (1u << 30) - 1u;
-// grid of the 64-bit LSD passes (fallback sort, pair order): blocks loop over
-// tickets, so a launch that has nothing to do costs few blocks
-constexpr int64_t kOnesweepGrid = 2048;
+// grid of the 64-bit LSD passes (fallback sort, pair order): one block per CU
+// (their LDS admits no second) looping over tickets, so a launch that has
+// nothing to do costs one round of blocks
+constexpr int64_t kOnesweepGrid = 256;
 static_assert(kSortThreads >= 256, "one thread per digit");
 
 struct SweepShared {
@@ -864,6 +879,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sweep_kv(
 // input order).
 constexpr int kFixLong = 4096;
 constexpr int kFixSlots = 4096;      // blocks of the fixup launches (they loop over the list)
+constexpr int kFixLongSlots = 256;   // blocks of k_fixup_long (rare work)
 #ifndef CT_FINAL_GRID
 #define CT_FINAL_GRID 4096
 #endif
@@ -1406,7 +1422,7 @@ static void launch_tail(Prof& pf, hipStream_t s, int64_t n, BuildCtrl* c, const 
   hipLaunchKernelGGL(k_fixup_short, dim3(kFixSlots), dim3(256), 0, s, n, c, box->xmin,
                      box->ymin, box->xmax, box->ymax, box->id, (const uint32_t*)b.runs,
                      b.runs_cap);
-  hipLaunchKernelGGL(k_fixup_long, dim3(kFixSlots), dim3(256), 0, s, n, c, box->xmin,
+  hipLaunchKernelGGL(k_fixup_long, dim3(kFixLongSlots), dim3(256), 0, s, n, c, box->xmin,
                      box->ymin, box->xmax, box->ymax, box->id, (const uint32_t*)b.runs,
                      b.runs_cap);
   if constexpr (G > 0)
@@ -1416,7 +1432,7 @@ static void launch_tail(Prof& pf, hipStream_t s, int64_t n, BuildCtrl* c, const 
   // the full LSD sort of 64-bit keys: every launch exits at once unless a
   // run of equal high halves was too long to settle in place
   pf.mark(s, "k_fallback");
-  hipLaunchKernelGGL(k_fallback_zero, dim3(1024), dim3(256), 0, s, c, b.status + 4 * b.step_words,
+  hipLaunchKernelGGL(k_fallback_zero, dim3(256), dim3(256), 0, s, c, b.status + 4 * b.step_words,
                      (int64_t)(8 * b.step_words));
   const unsigned sgrid = (unsigned)std::min<int64_t>((n + 1023) / 1024, 2048);
   hipLaunchKernelGGL(k_keygen64, dim3(sgrid), dim3(256), 0, s, n, (const float*)box->xmin,
@@ -1429,7 +1445,7 @@ static void launch_tail(Prof& pf, hipStream_t s, int64_t n, BuildCtrl* c, const 
     hipLaunchKernelGGL(k_onesweep<uint64_t>, dim3((unsigned)std::min(ntiles, kOnesweepGrid)),
                        dim3(kSortThreads), 0, s, n, st, b.k64[0], b.k64[1], b.v[0], b.v[1], c,
                        b.status + (size_t)st * b.step_words, 0);
-  const unsigned ggrid = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
+  const unsigned ggrid = (unsigned)std::min<int64_t>((n + 255) / 256, 2048);
   hipLaunchKernelGGL(k_gather_l0<G>, dim3(ggrid), dim3(256), 0, s, n, (const BuildCtrl*)c,
                      (const uint32_t*)b.v[0], (const uint32_t*)b.v[1], (const uint4*)b.aos,
                      box->xmin, box->ymin, box->xmax, box->ymax, box->id, tr, l0start, r_sz);
