@@ -902,10 +902,12 @@ __global__ __launch_bounds__(256) void k_run_len(int64_t n, BuildCtrl* c,
   const int lane = lane_id();
   const uint32_t nw = gridDim.x * 4u;
   for (uint32_t q = (blockIdx.x * 256u + threadIdx.x) >> 6; q < nr; q += nw) {
+    int L = (int)runs[runs_cap + q];
+    if (L > 0 && L <= kWave) continue;  // (k_final measured it)
     const int64_t i = runs[q];
     const uint32_t h0 = (uint32_t)(morton64(xmin[i], ymin[i], xmax[i], ymax[i], kp) >> 32);
-    int L = kFixLong + 1;
-    for (int64_t b = i; b - i <= kFixLong; b += kWave) {
+    if (L == 0) L = kFixLong + 1;
+    for (int64_t b = i; L > kFixLong && b - i <= kFixLong; b += kWave) {
       const int64_t p = b + lane;
       const bool eq =
           p < n && (uint32_t)(morton64(xmin[p], ymin[p], xmax[p], ymax[p], kp) >> 32) == h0;
@@ -1147,10 +1149,36 @@ __global__ __launch_bounds__(256) void k_final(int64_t n, BuildCtrl* c, float* _
   // positions just before and after the tile
   __shared__ uint32_t s_eh[2][M64 ? kFinChunks : 1];
   __shared__ uint32_t s_nb[2];
+  // listed runs: start position and length (kOpen while the tile is open: a
+  // run that goes on beyond its chunk)
   __shared__ uint32_t s_list[M64 ? kListCap : 1];
+  __shared__ uint32_t s_len[M64 ? kListCap : 1];
+  // per chunk: how many of its first positions continue the previous chunk's
+  // run (kWave + 1: all of them, and the run goes on)
+  __shared__ uint8_t s_cont[M64 ? kFinChunks : 1];
   __shared__ uint32_t s_nlist, s_base;
+  constexpr uint32_t kOpen = 0x80000000u;
   const int t = threadIdx.x, w = t >> 6, lane = lane_id();
   const KeyParams kp = key_params<MODE>(c);
+  // lengths of the tile's open runs from the chunks after them (0: the run
+  // goes on into the next tile, k_run_len measures it)
+  auto close_runs = [&](uint32_t from, uint32_t tb) {
+    for (uint32_t i = from + t; i < s_nlist; i += 256) {
+      if (!(s_len[i] & kOpen)) continue;
+      const uint32_t off = s_list[i] - tb;  // the start's chunk and lane
+      uint32_t L = kWave - (off % kWave);
+      int ci = (int)(off / kWave) + 1;
+      for (; ci < kFinChunks; ci++) {
+        const uint32_t v = s_cont[ci];
+        if (v <= (uint32_t)kWave) {
+          L += v;
+          break;
+        }
+        L += kWave;
+      }
+      s_len[i] = ci < kFinChunks ? L : 0u;
+    }
+  };
   // listed runs go to the global list in one atomic per block and flush
   auto flush = [&]() {
     const uint32_t cnt = s_nlist;  // (the same for every thread: read after a barrier)
@@ -1159,8 +1187,12 @@ __global__ __launch_bounds__(256) void k_final(int64_t n, BuildCtrl* c, float* _
     __syncthreads();
     const uint32_t base = s_base;
     for (uint32_t i = t; i < cnt; i += 256) {
-      if (base + i < runs_cap) runs[base + i] = s_list[i];
-      else atomicOr(&c->fallback, 1u);
+      if (base + i < runs_cap) {
+        runs[base + i] = s_list[i];
+        runs[runs_cap + base + i] = s_len[i];
+      } else {
+        atomicOr(&c->fallback, 1u);
+      }
     }
     __syncthreads();
     if (t == 0) s_nlist = 0;
@@ -1216,7 +1248,12 @@ __global__ __launch_bounds__(256) void k_final(int64_t n, BuildCtrl* c, float* _
         const bool edge = (ci == 0 && st == 0 && before) ||
                           (ci == kFinChunks - 1 && en == kWave - 1 && after);
         const bool local = inrun && st >= 0 && en < kWave && !edge;
-        if (inrun && st == lane && !local) s_list[atomicAdd(&s_nlist, 1u)] = gp;
+        if (lane == 0) s_cont[ci] = (uint8_t)(eqp ? (en < kWave ? en + 1 : kWave + 1) : 0);
+        if (inrun && st == lane && !local) {
+          const uint32_t at = atomicAdd(&s_nlist, 1u);
+          s_list[at] = gp;
+          s_len[at] = en < kWave ? (uint32_t)(en - st + 1) : kOpen;
+        }
         if (!ballot(local)) continue;
         // (runs that the last sort pass could not settle: split between its
         // chunks or tiles)
@@ -1245,14 +1282,29 @@ __global__ __launch_bounds__(256) void k_final(int64_t n, BuildCtrl* c, float* _
     }
   };
   const uint32_t ntiles = (un + kFinTile - 1) / kFinTile;
+  // the previous tile's first entry and first position (the same in every thread)
+  uint32_t opened = 0, prev_tb = 0;
   for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     __syncthreads();  // the previous tile is done with the shared words
-    if (M64 && s_nlist > (uint32_t)(kListCap - 2 * kFinChunks)) flush();
+    if (M64) {
+      // (before this tile's chunks overwrite s_cont)
+      close_runs(opened, prev_tb);
+      if (s_nlist > (uint32_t)(kListCap - 2 * kFinChunks)) {
+        __syncthreads();
+        flush();
+        opened = 0;
+      } else {
+        opened = s_nlist;
+      }
+    }
     const uint32_t tb = tile * kFinTile;
     if (tb + kFinTile <= un) tile_body(std::true_type{}, tb);
     else tile_body(std::false_type{}, tb);
+    prev_tb = tb;
   }
   if constexpr (M64) {
+    __syncthreads();
+    close_runs(opened, prev_tb);
     __syncthreads();
     flush();
   }
