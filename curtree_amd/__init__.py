@@ -132,6 +132,7 @@ def _load():
         "curtree_query_last_stats": (I, [P(I64), I, P(I), P(I64), P(I64)]),
         "curtree_set_option": (I, [ctypes.c_char_p, I64]),
         "curtree_prof_enable": (I, [I]),
+        "curtree_prof_only": (I, [ctypes.c_char_p]),
         "curtree_prof_reset": (I, []),
         "curtree_prof_get": (I, [ctypes.c_char_p, P(ctypes.c_double), P(I64)]),
         "curtree_event_create": (I, [P(V)]),
@@ -546,6 +547,11 @@ class Prof:
     @staticmethod
     def enable(on=True):
         _check(lib.curtree_prof_enable(1 if on else 0))
+
+    @staticmethod
+    def only(name=None):
+        """Bracket only the launches of kernel `name` (None: every kernel)."""
+        _check(lib.curtree_prof_only(name.encode() if name else None))
 
     @staticmethod
     def reset():

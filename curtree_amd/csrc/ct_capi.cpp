@@ -113,7 +113,7 @@ void Prof::mark(hipStream_t s, const char* name) {
     open_name = nullptr;
     open_ev = nullptr;
   }
-  if (name) {
+  if (name && (only.empty() || only == name)) {
     open_ev = get_event();
     (void)hipEventRecord(open_ev, s);
     open_stream = s;
@@ -504,6 +504,13 @@ CURTREE_API int curtree_prof_enable(int on) {
   Context& c = context();
   std::lock_guard<std::mutex> g(c.mu);
   c.prof.on = on != 0;
+  return CURTREE_OK;
+}
+
+CURTREE_API int curtree_prof_only(const char* name) {
+  Context& c = context();
+  std::lock_guard<std::mutex> g(c.mu);
+  c.prof.only = name ? name : "";
   return CURTREE_OK;
 }
 

@@ -85,6 +85,7 @@ struct Workspace {
 // Optional per-kernel event timing.
 struct Prof {
   bool on = false;
+  std::string only;  // non-empty: bracket only the launches of this kernel
   struct Rec {
     std::string name;
     hipEvent_t a, b;

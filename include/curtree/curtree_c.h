@@ -298,6 +298,9 @@ CURTREE_API int curtree_release_workspace(void);
  * the summed milliseconds and launch count of kernel `name` since the last
  * reset (events are resolved at the call, which synchronises). */
 CURTREE_API int curtree_prof_enable(int on);
+/* Bracket only the launches of kernel `name` (NULL or "": every kernel), so
+ * that timing one kernel adds two events per launch of it and no others. */
+CURTREE_API int curtree_prof_only(const char* name);
 CURTREE_API int curtree_prof_reset(void);
 CURTREE_API int curtree_prof_get(const char* name, double* total_ms, int64_t* launches);
 /* Elapsed milliseconds between two events recorded with curtree_event_record. */
