@@ -285,7 +285,8 @@ __device__ __forceinline__ uint32_t nth_set_bit(uint64_t m, uint32_t r) {
 }
 
 // Exclusive scan of one u64 per thread over the block.
-__device__ __forceinline__ uint64_t block_excl_scan64(uint64_t v, uint64_t* s_tmp, uint64_t* total) {
+__device__ __forceinline__ uint64_t block_excl_scan64(uint64_t v, uint64_t* s_tmp,
+                                                      uint64_t* total) {
   const int lane = lane_id(), w = threadIdx.x >> 6;
   uint64_t x = v;
 #pragma unroll
@@ -310,7 +311,8 @@ __device__ __forceinline__ uint64_t block_excl_scan64(uint64_t v, uint64_t* s_tm
 #ifndef CT_LEAF_H
 #define CT_LEAF_H 2
 #endif
-template <int G, int STEPS, bool STORE>
+// QCOUNTS: per-query counts into `counts` as well
+template <int G, int STEPS, bool STORE, bool QCOUNTS>
 __global__ __launch_bounds__(kQThreads) void k_leaf_count(
     int slot, QTree tr, QBoxes qb, QBoxes rb, QCtrl* c, const uint2* __restrict__ fin,
     int64_t r_sz, int64_t nbox, QLeaf* __restrict__ est, uint64_t* __restrict__ ttot,
@@ -327,6 +329,12 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_count(
   // between two rounds of loads would make the second wait for it (stores
   // and loads share one counter)
   __shared__ QLeaf s_est[STORE ? P : 1];
+  // per-query counts: the query and pair count of every entry of the tile,
+  // summed per run of one query (the frontier keeps a query's entries
+  // together) and added with one atomic at each end of the run
+  __shared__ uint32_t s_cq[QCOUNTS ? P : 1];
+  __shared__ uint32_t s_cn[QCOUNTS ? P : 1];
+  __shared__ uint64_t s_scan[kQWaves];
   if (c->overflow) return;
   const int64_t n_in = (int64_t)c->count[slot];
   const int64_t q0 = c->q0;
@@ -429,7 +437,10 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_count(
             x.qf = q[k] | (full[k] ? kQRange : 0u);
             s_est[e - tbase] = x;
           }
-          if (counts && nh) atomicAdd(counts + q[k], (unsigned long long)nh);
+          if (QCOUNTS) {
+            s_cq[e - tbase] = q[k];
+            s_cn[e - tbase] = (uint32_t)nh;  // (at most F, or a range of < 2^30 boxes)
+          }
         }
         uint64_t v = (c_ == 0 && valid) ? nh : 0ull;
 #pragma unroll
@@ -448,6 +459,28 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_count(
       for (int i = 0; i < kQWaves; i++) tot += s_red[i];
       if (STORE) ttot[tile] = tot;
       block_total += tot;
+    }
+    if (QCOUNTS) {
+      // thread t holds entries t*R .. t*R+R-1: their sum, a block scan, then
+      // count[q] -= (pairs before the run) at a run's first entry and
+      // += (pairs up to its end) at its last (unsigned, so the two cancel)
+      constexpr int R = (P + kQThreads - 1) / kQThreads;
+      const int nv = n_in - tbase < P ? (int)(n_in - tbase) : P;
+      uint64_t sum = 0;
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        if (t * R + r < nv) sum += s_cn[t * R + r];
+      uint64_t tile_sum;
+      uint64_t before = block_excl_scan64(sum, s_scan, &tile_sum);
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int i = t * R + r;
+        if (i >= nv) break;
+        const uint32_t qi = s_cq[i];
+        if (i == 0 || s_cq[i - 1] != qi) atomicAdd(counts + qi, 0ull - before);
+        before += s_cn[i];
+        if (i + 1 == nv || s_cq[i + 1] != qi) atomicAdd(counts + qi, before);
+      }
     }
   }
   if (threadIdx.x == 0 && block_total)
@@ -893,8 +926,10 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_write(
 
 // Long box ranges of contained subtrees: one block per piece, coalesced id
 // reads and pair writes.
-__global__ __launch_bounds__(256) void k_write_pieces(const QCtrl* c, const QPiece* __restrict__ pieces,
-                                                      int64_t piece_cap, const int* __restrict__ rid,
+__global__ __launch_bounds__(256) void k_write_pieces(const QCtrl* c,
+                                                      const QPiece* __restrict__ pieces,
+                                                      int64_t piece_cap,
+                                                      const int* __restrict__ rid,
                                                       int* __restrict__ out_fid,
                                                       int* __restrict__ out_tid, int64_t out_cap) {
   if (c->overflow) return;
@@ -1181,9 +1216,9 @@ static unsigned resident_grid(const void* kernel) {
   return (unsigned)(cus * per);
 }
 
-template <int G, int STEPS, bool STORE>
+template <int G, int STEPS, bool STORE, bool QCOUNTS>
 static unsigned leaf_count_grid() {
-  static const unsigned g = resident_grid((const void*)k_leaf_count<G, STEPS, STORE>);
+  static const unsigned g = resident_grid((const void*)k_leaf_count<G, STEPS, STORE, QCOUNTS>);
   return g;
 }
 
@@ -1210,13 +1245,15 @@ static unsigned leaf_write_grid() {
 template <int G>
 static void warm_grids_g(int steps) {
   switch (steps) {
-#define CT_WARM(st)                          \
-  case st:                                   \
-    (void)expand_count_grid<G, st>();        \
-    (void)expand_write_grid<G, st>();        \
-    (void)leaf_count_grid<G, st, true>();    \
-    (void)leaf_count_grid<G, st, false>();   \
-    (void)leaf_write_grid<G, st>();          \
+#define CT_WARM(st)                               \
+  case st:                                        \
+    (void)expand_count_grid<G, st>();             \
+    (void)expand_write_grid<G, st>();             \
+    (void)leaf_count_grid<G, st, true, false>();  \
+    (void)leaf_count_grid<G, st, true, true>();   \
+    (void)leaf_count_grid<G, st, false, false>(); \
+    (void)leaf_count_grid<G, st, false, true>();  \
+    (void)leaf_write_grid<G, st>();               \
     break;
     CT_WARM(4) CT_WARM(16)
     default: CT_WARM(8)
@@ -1283,11 +1320,16 @@ void enqueue_run_g(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
   const int leaf = a.nl - 1;
   if (a.write) {
     pf.mark(s, "k_leaf_count");
-    hipLaunchKernelGGL((k_leaf_count<G, STEPS, true>),
-                       dim3(std::min<unsigned>(a.grid, leaf_count_grid<G, STEPS, true>())),
-                       dim3(kQThreads), 0, s, leaf,
-                       a.tr, a.qb, a.rb, qe.ctrl, cur, a.r_sz, a.nbox, (QLeaf*)qe.est, qe.ttot,
-                       a.counts);
+    if (a.counts)
+      hipLaunchKernelGGL((k_leaf_count<G, STEPS, true, true>),
+                         dim3(std::min<unsigned>(a.grid, leaf_count_grid<G, STEPS, true, true>())),
+                         dim3(kQThreads), 0, s, leaf, a.tr, a.qb, a.rb, qe.ctrl, cur, a.r_sz,
+                         a.nbox, (QLeaf*)qe.est, qe.ttot, a.counts);
+    else
+      hipLaunchKernelGGL((k_leaf_count<G, STEPS, true, false>),
+                         dim3(std::min<unsigned>(a.grid, leaf_count_grid<G, STEPS, true, false>())),
+                         dim3(kQThreads), 0, s, leaf, a.tr, a.qb, a.rb, qe.ctrl, cur, a.r_sz,
+                         a.nbox, (QLeaf*)qe.est, qe.ttot, (unsigned long long*)nullptr);
     pf.mark(s, "k_tile_scan");
     hipLaunchKernelGGL(k_tile_scan, dim3(64), dim3(kQThreads), 0, s, leaf + 1, leaf, S::kPerTile,
                        qe.ctrl, qe.ttot, qe.status);
@@ -1303,11 +1345,18 @@ void enqueue_run_g(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
                        a.out_cap);
   } else {
     pf.mark(s, "k_leaf_count");
-    hipLaunchKernelGGL((k_leaf_count<G, STEPS, false>),
-                       dim3(std::min<unsigned>(a.grid, leaf_count_grid<G, STEPS, false>())),
-                       dim3(kQThreads), 0, s, leaf,
-                       a.tr, a.qb, a.rb, qe.ctrl, cur, a.r_sz, a.nbox, (QLeaf*)nullptr,
-                       (uint64_t*)nullptr, a.counts);
+    if (a.counts)
+      hipLaunchKernelGGL((k_leaf_count<G, STEPS, false, true>),
+                         dim3(std::min<unsigned>(a.grid, leaf_count_grid<G, STEPS, false, true>())),
+                         dim3(kQThreads), 0, s, leaf, a.tr, a.qb, a.rb, qe.ctrl, cur, a.r_sz,
+                         a.nbox, (QLeaf*)nullptr, (uint64_t*)nullptr, a.counts);
+    else
+      hipLaunchKernelGGL((k_leaf_count<G, STEPS, false, false>),
+                         dim3(std::min<unsigned>(a.grid,
+                                                 leaf_count_grid<G, STEPS, false, false>())),
+                         dim3(kQThreads), 0, s, leaf, a.tr, a.qb, a.rb, qe.ctrl, cur, a.r_sz,
+                         a.nbox, (QLeaf*)nullptr, (uint64_t*)nullptr,
+                         (unsigned long long*)nullptr);
   }
   pf.mark(s, nullptr);
 }
@@ -1468,7 +1517,8 @@ int query_run_f32(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curt
   for (int pass = 0; pass < 2; pass++) {
     for (int attempt = 0;; attempt++) {
       if (a.counts) enqueue_fill_counts(qo->counts + q0, nq, s);
-      if ((rc = qe.reserve_status(std::max<int64_t>(qe.cap_front, nq) / per_tile(a.G, a.steps) + 2)))
+      if ((rc = qe.reserve_status(std::max<int64_t>(qe.cap_front, nq) / per_tile(a.G, a.steps) +
+                                  2)))
         return rc;
       if (a.write && (rc = qe.reserve_pieces(pieces_needed(qe, a)))) return rc;
       enqueue_run(qe, pf, a, s);

@@ -551,7 +551,8 @@ CURTREE_API int curtree_server_add_tree(curtree_server* s, const curtree_rtree_f
     if (!rc) rc = stage_boxes(&t.box4, &cb, rb, rbox->sz, nullptr);
     if (!rc) rc = stage_boxes(&t.node4, &cn, nb, rt->sz, nullptr);
     if (!rc) rc = check_layout(&t.canon, rt, rbox->sz, nullptr);
-    if (!rc && hipDeviceSynchronize() != hipSuccess) rc = set_error(CURTREE_EHIP, "server_add_tree: staging failed");
+    if (!rc && hipDeviceSynchronize() != hipSuccess)
+      rc = set_error(CURTREE_EHIP, "server_add_tree: staging failed");
     t.safe = qe.safe;
     qe.safe = nullptr;  // ownership moves to the tree
     if (rc) {
