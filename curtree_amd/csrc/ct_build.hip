@@ -929,7 +929,8 @@ __global__ __launch_bounds__(256) void k_run_len(int64_t n, BuildCtrl* c,
   }
 }
 
-// Listed runs of up to 64 boxes, one wave each: lane l holds position i + l.
+// Listed runs of up to 64 boxes: eight runs of up to 8 per wave, longer ones
+// one per wave (lane l holds position i + l of its run).
 __global__ __launch_bounds__(256) void k_fixup_short(int64_t n, BuildCtrl* c,
                                                      float* __restrict__ xmin,
                                                      float* __restrict__ ymin,
@@ -942,17 +943,14 @@ __global__ __launch_bounds__(256) void k_fixup_short(int64_t n, BuildCtrl* c,
   if (blockIdx.x * 4u >= nr || c->fallback) return;
   const KeyParams kp = key_params<kKeyMorton64>(c);
   const int lane = lane_id();
-  const uint32_t nw = gridDim.x * 4u;
-  for (uint32_t q = (blockIdx.x * 256u + threadIdx.x) >> 6; q < nr; q += nw) {
-    const int L = (int)runs[runs_cap + q];
-    if (L > kWave) continue;
-    const int64_t p = (int64_t)runs[q] + lane;
-    const bool member = lane < L;
+  const uint32_t wave = (blockIdx.x * 256u + threadIdx.x) >> 6, nw = gridDim.x * 4u;
+  // lane `member` of a run at position p: sort the wave's runs, move what moved
+  auto settle = [&](bool member, int st, int en, int64_t p) {
     float4 b = make_float4(0.f, 0.f, 0.f, 0.f);
     if (member) b = make_float4(xmin[p], ymin[p], xmax[p], ymax[p]);
     uint32_t lo;
     (void)morton_cell(b.x, b.y, b.z, b.w, kp, &lo);
-    const int dest = local_run_dest(member, 0, L - 1, lo);
+    const int dest = local_run_dest(member, st, en, lo);
     int v = dest != lane ? id[p] : 0;
     wave_permute(dest, b, v);
     if (dest != lane) {
@@ -962,6 +960,22 @@ __global__ __launch_bounds__(256) void k_fixup_short(int64_t n, BuildCtrl* c,
       ymax[p] = b.w;
       id[p] = v;
     }
+  };
+  // runs of up to kSeg boxes (nearly all): kWave / kSeg of them per wave,
+  // lanes kSeg*j .. kSeg*j+kSeg-1 holding run j
+  constexpr int kSeg = 8, kPerWave = kWave / kSeg;
+  for (uint32_t q0 = wave * kPerWave; q0 < nr; q0 += nw * kPerWave) {
+    const int j = lane / kSeg, i = lane % kSeg;
+    const uint32_t q = q0 + j;
+    const int L = q < nr ? (int)runs[runs_cap + q] : 0;
+    const bool member = L <= kSeg && i < L;
+    settle(member, j * kSeg, j * kSeg + L - 1, member ? (int64_t)runs[q] + i : 0);
+  }
+  // runs of kSeg+1 .. kWave boxes: one per wave
+  for (uint32_t q = wave; q < nr; q += nw) {
+    const int L = (int)runs[runs_cap + q];
+    if (L <= kSeg || L > kWave) continue;
+    settle(lane < L, 0, L - 1, (int64_t)runs[q] + lane);
   }
 }
 
