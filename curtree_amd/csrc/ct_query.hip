@@ -442,12 +442,11 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_count(
             s_cn[e - tbase] = (uint32_t)nh;  // (at most F, or a range of < 2^30 boxes)
           }
         }
-        uint64_t v = (c_ == 0 && valid) ? nh : 0ull;
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-        wsum += v;
+        if (c_ == 0 && valid) wsum += nh;  // (this lane's share; summed over the wave below)
       }
     }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) wsum += __shfl_xor(wsum, o);
     if (lane == 0) s_red[w] = wsum;
     __syncthreads();
     if (STORE)
@@ -668,12 +667,11 @@ __global__ __launch_bounds__(kQThreads) void k_expand_count(int slot, QTree tr, 
           x.first = full ? (en[k].y & ~kFull) : (uint32_t)p[k];
           s_est[e - tbase] = x;
         }
-        uint64_t v = (c_ == 0 && valid) ? (uint64_t)popc64(gm) : 0ull;
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-        wsum += v;
+        if (c_ == 0 && valid) wsum += (uint64_t)popc64(gm);  // (summed over the wave below)
       }
     }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) wsum += __shfl_xor(wsum, o);
     if (lane == 0) s_red[w] = wsum;
     __syncthreads();
     for (int i = t; i < P; i += kQThreads)
