@@ -286,3 +286,21 @@ def test_expansion_paths(split, fanout, key_mode):
         run_case(bx, np.arange(n, dtype=np.int32), big, fanout, key_mode, brute_sample=4)
     finally:
         ct.set_option("expand_split", -1)
+
+
+def test_skewed_mixed_point_and_window_queries():
+    # the large/skewed workload's shape at a parity size: skewed boxes (hot
+    # cells, 20% points) and a mix of point queries and windows, half of
+    # them centred in the hot cells
+    n = 300_000
+    bx = datagen.skewed_boxes(n, seed=195)
+    r = np.random.default_rng(196)
+    nq = 4000
+    pick = r.integers(0, n, nq // 2)
+    cx = np.concatenate([(bx[0][pick] + bx[2][pick]) / 2, r.random(nq // 2).astype(np.float32)])
+    cy = np.concatenate([(bx[1][pick] + bx[3][pick]) / 2, r.random(nq // 2).astype(np.float32)])
+    side = np.where(r.random(nq) < 0.5, 0.0, 10 ** r.uniform(-5, -2, nq)).astype(np.float32)
+    qs = (cx.astype(np.float32) - side / 2, cy.astype(np.float32) - side / 2,
+          cx.astype(np.float32) + side / 2, cy.astype(np.float32) + side / 2)
+    for fanout, km in ((16, 1), (32, 1)):
+        run_case(bx, np.arange(n, dtype=np.int32), qs, fanout, km, brute_sample=400)
