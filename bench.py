@@ -516,8 +516,10 @@ def main():
     ct.Prof.enable(False)
     kernels = {}
     for name in ("k_build_init", "k_extent", "k_hist", "k_sort_plan", "k_sweep_kv", "k_final",
-                 "k_fixup_long", "k_fallback", "k_level", "k_safe", "k_run_begin", "k_expand",
-                 "k_expand_count", "k_expand_write", "k_leaf_count", "k_tile_scan",
+                 "k_run_len", "k_fixup_short", "k_fixup_long", "k_fixup_huge", "k_fixup_groups",
+                 "k_fallback", "k_level",
+                 "k_safe", "k_run_begin", "k_expand", "k_expand_count", "k_expand_write",
+                 "k_leaf_count", "k_tile_scan",
                  "k_leaf_write", "k_write_pieces"):
         tot, cnt = ct.Prof.get(name)
         if cnt:

@@ -57,6 +57,8 @@ struct BuildCtrl {
   uint32_t fallback;               // a run is too long to settle in place: full 64-bit sort
   uint32_t nruns;                  // runs listed by k_final
   uint32_t nlong;                  // of them longer than 64 (k_fixup_long)
+  uint32_t nhuge;                  // of them longer than kFixLong (k_fixup_huge)
+  uint32_t huge_next;              // k_fixup_huge: next listed run to claim
   uint32_t trivial;                // bit d: digit d is the same for every key
 };
 
@@ -84,6 +86,8 @@ __global__ __launch_bounds__(256) void k_build_init(BuildCtrl* c) {
     c->fallback = 0;
     c->nruns = 0;
     c->nlong = 0;
+    c->nhuge = 0;
+    c->huge_next = 0;
     c->trivial = 0;
   }
 }
@@ -872,14 +876,17 @@ __global__ __launch_bounds__(kSortThreads) void k_sweep_kv(
 // never change during the launch.  Every other run is listed by the chunk
 // that holds its start; k_run_len measures the listed runs (reading only),
 // k_fixup_short sorts those of up to 64 boxes (one wave each), k_fixup_long
-// the longer ones up to kFixLong (one block each, bitonic in LDS), and
-// k_fixup_groups recomputes the leaf-parent nodes over them.  A longer run, or
-// a full list, sets `fallback`: the full 8-pass LSD sort of 64-bit keys then
-// runs from the order reached so far (in which equal full keys are still in
-// input order).
+// the longer ones up to kFixLong (one block each, bitonic in LDS),
+// k_fixup_huge the longer ones still (one block each, LSD in the free key
+// buffers), and k_fixup_groups recomputes the leaf-parent nodes over them.  A
+// run too long for one block, or a full list, sets `fallback`: the full 8-pass
+// LSD sort of 64-bit keys then runs from the order reached so far (in which
+// equal full keys are still in input order).
 constexpr int kFixLong = 4096;
 constexpr int kFixSlots = 4096;      // blocks of the fixup launches (they loop over the list)
-constexpr int kFixLongSlots = 256;   // blocks of k_fixup_long (rare work)
+constexpr int kFixLongSlots = 1024;  // blocks of k_fixup_long (four per CU)
+constexpr int kFixHugeSlots = 512;   // blocks of k_fixup_huge (two per CU)
+constexpr int kHugeK = 8;            // k_fixup_huge: positions per thread in one tile
 #ifndef CT_FINAL_GRID
 #define CT_FINAL_GRID 4096
 #endif
@@ -887,7 +894,8 @@ constexpr int kFinTile = 2048;       // positions owned by one block of k_final 
 
 // Length of every listed run (one wave per run; nothing is written to the
 // boxes, so every wave reads stable data).  Runs of more than 64 boxes are
-// listed again for k_fixup_long; one longer than kFixLong sets `fallback`.
+// listed again for k_fixup_long, runs longer than kFixLong (counted only up to
+// kFixLong + 1) for k_fixup_huge, from the far end of the same list.
 __global__ __launch_bounds__(256) void k_run_len(int64_t n, BuildCtrl* c,
                                                  const float* __restrict__ xmin,
                                                  const float* __restrict__ ymin,
@@ -907,20 +915,26 @@ __global__ __launch_bounds__(256) void k_run_len(int64_t n, BuildCtrl* c,
     const int64_t i = runs[q];
     const uint32_t h0 = (uint32_t)(morton64(xmin[i], ymin[i], xmax[i], ymax[i], kp) >> 32);
     if (L == 0) L = kFixLong + 1;
-    for (int64_t b = i; L > kFixLong && b - i <= kFixLong; b += kWave) {
-      const int64_t p = b + lane;
-      const bool eq =
-          p < n && (uint32_t)(morton64(xmin[p], ymin[p], xmax[p], ymax[p], kp) >> 32) == h0;
-      const uint64_t ne = ballot(!eq);
-      if (ne) {
-        L = (int)(b - i) + __ffsll((long long)ne) - 1;
-        break;
+    // four chunks of 64 positions per round (independent loads in flight)
+    for (int64_t b = i; L > kFixLong && b - i <= kFixLong; b += 4 * kWave) {
+      bool eq[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int64_t p = b + kWave * u + lane;
+        eq[u] = p < n &&
+                (uint32_t)(morton64(xmin[p], ymin[p], xmax[p], ymax[p], kp) >> 32) == h0;
+      }
+#pragma unroll
+      for (int u = 3; u >= 0; u--) {
+        const uint64_t ne = ballot(!eq[u]);
+        if (ne) L = (int)(b - i) + kWave * u + __ffsll((long long)ne) - 1;
       }
     }
     if (lane == 0) {
       runs[runs_cap + q] = (uint32_t)L;
       if (L > kFixLong) {
-        atomicOr(&c->fallback, 1u);
+        const uint32_t at = atomicAdd(&c->nhuge, 1u);
+        runs[3 * (size_t)runs_cap - 1 - at] = q;
       } else if (L > kWave) {
         const uint32_t at = atomicAdd(&c->nlong, 1u);
         runs[2 * (size_t)runs_cap + at] = q;
@@ -979,7 +993,12 @@ __global__ __launch_bounds__(256) void k_fixup_short(int64_t n, BuildCtrl* c,
   }
 }
 
-// Listed runs of 65..kFixLong boxes, one block each.
+// Listed runs of 65..kFixLong boxes, one block each: the run's
+// (low half << 32 | index in the run) stay in registers (wave w, item k,
+// lane l holds index w * 64 * kUse + 64 * k + l; the slots past the run hold
+// ~0, which sorts after every real key) through four stable LSD passes over
+// 8-bit digits of the low half: ranks from ballots and per-wave running counts,
+// offsets from a scan of the counts, the scatter and the reload through LDS.
 __global__ __launch_bounds__(256) void k_fixup_long(int64_t n, BuildCtrl* c,
                                                     float* __restrict__ xmin,
                                                     float* __restrict__ ymin,
@@ -987,72 +1006,273 @@ __global__ __launch_bounds__(256) void k_fixup_long(int64_t n, BuildCtrl* c,
                                                     float* __restrict__ ymax,
                                                     int* __restrict__ id,
                                                     const uint32_t* __restrict__ runs,
-                                                    uint32_t runs_cap) {
+                                                    uint32_t runs_cap,
+                                                    float4* __restrict__ stage,
+                                                    uint32_t* __restrict__ stage_id) {
   constexpr int kPer = kFixLong / 256;
   __shared__ uint64_t s_net[kFixLong];
+  __shared__ uint32_t s_cnt[4][256];  // per wave, per digit: running count -> offset
+  __shared__ uint32_t s_wsum[4];
   const uint32_t nl = c->nlong;
   if (blockIdx.x >= nl || c->fallback) return;
   const KeyParams kp = key_params<kKeyMorton64>(c);
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, lane = lane_id(), w = t >> 6;
+  const uint64_t lt = lanemask_lt();
   for (uint32_t r = blockIdx.x; r < nl; r += gridDim.x) {
     const uint32_t q = runs[2 * (size_t)runs_cap + r];
     const int64_t i = runs[q];
     const int L = (int)runs[runs_cap + q];
-    int m = 1;
-    while (m < L) m <<= 1;
-    __syncthreads();  // the previous run is done with s_net
-    for (int j = t; j < m; j += 256) {
-      uint64_t v = ~0ull;
-      if (j < L) {
-        const int64_t p = i + j;
-        v = ((uint64_t)(uint32_t)morton64(xmin[p], ymin[p], xmax[p], ymax[p], kp) << 32) |
-            (uint32_t)j;
+    const int kUse = (L + 255) >> 8;  // items per lane
+    const int wbase = w * 64 * kUse + lane;
+    uint64_t x[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      x[k] = ~0ull;
+      const int e = wbase + 64 * k;
+      if (k < kUse && e < L) {
+        const int64_t p = i + e;
+        x[k] = ((uint64_t)(uint32_t)morton64(xmin[p], ymin[p], xmax[p], ymax[p], kp) << 32) |
+               (uint32_t)e;
       }
-      s_net[j] = v;
+    }
+    for (int pass = 0; pass < 4; pass++) {
+      const int sh = 32 + 8 * pass;
+#pragma unroll
+      for (int k = 0; k < 4; k++) s_cnt[w][lane + 64 * k] = 0;
+      uint32_t rank[kPer];
+#pragma unroll
+      for (int k = 0; k < kPer; k++) {
+        if (k < kUse) {
+          const uint32_t d = (uint32_t)(x[k] >> sh) & 255u;
+          uint64_t m = ~0ull;
+#pragma unroll
+          for (int bit = 0; bit < 8; bit++) {
+            const uint64_t bb = ballot((d >> bit) & 1u);
+            m &= ((d >> bit) & 1u) ? bb : ~bb;
+          }
+          const uint32_t before = s_cnt[w][d];
+          rank[k] = before + (uint32_t)popc64(m & lt);
+          if ((m & lt) == 0) s_cnt[w][d] = before + (uint32_t)popc64(m);
+        }
+      }
+      __syncthreads();
+      // digit t: offsets of the four waves' shares, after all smaller digits
+      const uint32_t c0 = s_cnt[0][t], c1 = s_cnt[1][t], c2 = s_cnt[2][t], c3 = s_cnt[3][t];
+      const uint32_t tot = c0 + c1 + c2 + c3;
+      uint32_t inc = tot;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t u = __shfl_up(inc, d);
+        if (lane >= d) inc += u;
+      }
+      if (lane == 63) s_wsum[w] = inc;
+      __syncthreads();
+      uint32_t off = inc - tot;
+      for (int k = 0; k < w; k++) off += s_wsum[k];
+      s_cnt[0][t] = off;
+      s_cnt[1][t] = off + c0;
+      s_cnt[2][t] = off + c0 + c1;
+      s_cnt[3][t] = off + c0 + c1 + c2;
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < kPer; k++)
+        if (k < kUse) s_net[s_cnt[w][(uint32_t)(x[k] >> sh) & 255u] + rank[k]] = x[k];
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < kPer; k++)
+        if (k < kUse) x[k] = s_net[wbase + 64 * k];
+    }
+    // every moved box staged (the run's span of the free staging records)
+    // before any is written back
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const int e = wbase + 64 * k;
+      if (k < kUse && e < L && (uint32_t)x[k] != (uint32_t)e) {
+        const int64_t sp = i + (uint32_t)x[k];
+        stage[i + e] = make_float4(xmin[sp], ymin[sp], xmax[sp], ymax[sp]);
+        stage_id[i + e] = (uint32_t)id[sp];
+      }
     }
     __syncthreads();
-    for (int k = 2; k <= m; k <<= 1) {
-      for (int d = k >> 1; d > 0; d >>= 1) {
-        for (int j = t; j < m; j += 256) {
-          const int p = j ^ d;
-          if (p > j) {
-            const uint64_t x = s_net[j], y = s_net[p];
-            if ((x > y) == ((j & k) == 0)) {
-              s_net[j] = y;
-              s_net[p] = x;
-            }
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const int e = wbase + 64 * k;
+      if (k < kUse && e < L && (uint32_t)x[k] != (uint32_t)e) {
+        const int64_t p = i + e;
+        const float4 b = stage[p];
+        xmin[p] = b.x;
+        ymin[p] = b.y;
+        xmax[p] = b.z;
+        ymax[p] = b.w;
+        id[p] = (int)stage_id[p];
+      }
+    }
+    __syncthreads();  // s_net and s_cnt are reused by the next run
+  }
+}
+
+// Listed runs longer than kFixLong, one block each, claimed by ticket.  The
+// block measures the run (k_run_len stopped at kFixLong + 1) and records its
+// length for k_fixup_groups, sorts (low half << 32 | index in the run) by the
+// low half with a stable LSD sort over the run's own span of the two free key
+// buffers (four 8-bit digits; tiles of 1024 * kHugeK positions: ranks from
+// ballots and per-wave running counts, wave offsets from a scan over the 16
+// waves), then moves the boxes through staging records.  A run longer than `hmax` (one block would
+// take longer than the full 64-bit sort) sets `fallback` and stays as it is;
+// every other run is settled completely, so the fallback may start from any
+// mix of the two.
+__global__ __launch_bounds__(1024) void k_fixup_huge(
+    int64_t n, BuildCtrl* c, float* __restrict__ xmin, float* __restrict__ ymin,
+    float* __restrict__ xmax, float* __restrict__ ymax, int* __restrict__ id,
+    uint32_t* __restrict__ runs, uint32_t runs_cap, uint64_t* __restrict__ ka,
+    uint64_t* __restrict__ kb, float4* __restrict__ stage, uint32_t* __restrict__ stage_id,
+    int64_t hmax) {
+  __shared__ uint32_t s_hist[2][256];  // digit counts of this pass / the next
+  __shared__ uint32_t s_base[256];     // running offset of each digit
+  __shared__ uint32_t s_wcnt[16][256]; // a tile's count per (wave, digit) -> offset
+  __shared__ uint32_t s_wsum[4];
+  __shared__ uint32_t s_claim, s_len;
+  const uint32_t nh = c->nhuge;
+  if (blockIdx.x >= nh || c->fallback) return;
+  const KeyParams kp = key_params<kKeyMorton64>(c);
+  const int t = threadIdx.x, lane = lane_id(), w = t >> 6;
+  const uint64_t lt = lanemask_lt();
+  for (;;) {
+    if (t == 0) {
+      s_claim = atomicAdd(&c->huge_next, 1u);
+      s_len = 0xFFFFFFFFu;
+    }
+    __syncthreads();
+    const uint32_t r = s_claim;
+    if (r >= nh) break;
+    const uint32_t q = runs[3 * (size_t)runs_cap - 1 - r];
+    const int64_t i = runs[q];
+    const uint32_t h0 = (uint32_t)(morton64(xmin[i], ymin[i], xmax[i], ymax[i], kp) >> 32);
+    // the run's end: the first later position with another high half
+    const int64_t lim = n - i < hmax + 1 ? n - i : hmax + 1;
+    for (int64_t b = kFixLong; b < lim; b += 1024) {
+      const int64_t p = i + b + t;
+      if (b + t < lim &&
+          (uint32_t)(morton64(xmin[p], ymin[p], xmax[p], ymax[p], kp) >> 32) != h0)
+        atomicMin(&s_len, (uint32_t)(b + t));
+      __syncthreads();
+      const bool found = s_len != 0xFFFFFFFFu;
+      __syncthreads();
+      if (found) break;
+    }
+    const uint32_t L = s_len != 0xFFFFFFFFu ? s_len : (uint32_t)lim;
+    if ((int64_t)L > hmax) {
+      if (t == 0) atomicOr(&c->fallback, 1u);
+      __syncthreads();  // s_claim / s_len are rewritten for the next run
+      continue;
+    }
+    if (t == 0) runs[runs_cap + q] = L;
+    if (t < 256) {
+      s_hist[0][t] = 0;
+      s_hist[1][t] = 0;
+    }
+    __syncthreads();
+    for (uint32_t e = t; e < L; e += 1024) {
+      const int64_t p = i + e;
+      const uint32_t lo = (uint32_t)morton64(xmin[p], ymin[p], xmax[p], ymax[p], kp);
+      ka[p] = (uint64_t)lo << 32 | e;
+      atomicAdd(&s_hist[0][lo & 255u], 1u);
+    }
+    uint64_t* src = ka;
+    uint64_t* dst = kb;
+    for (int pass = 0; pass < 4; pass++) {
+      const int sh = 32 + 8 * pass;
+      __syncthreads();  // this pass's histogram is complete
+      // exclusive offsets of the digits (waves 0..3, 64 digits each)
+      uint32_t v = 0, inc = 0;
+      if (t < 256) {
+        v = s_hist[pass & 1][t];
+        inc = v;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+          const uint32_t u = __shfl_up(inc, d);
+          if (lane >= d) inc += u;
+        }
+        if (lane == 63) s_wsum[w] = inc;
+        s_hist[(pass + 1) & 1][t] = 0;
+      }
+      __syncthreads();
+      if (t < 256) {
+        uint32_t off = 0;
+        for (int k = 0; k < w; k++) off += s_wsum[k];
+        s_base[t] = off + inc - v;
+      }
+      for (uint32_t e0 = 0; e0 < L; e0 += 1024 * kHugeK) {
+        // wave w, item k, lane l: index e0 + w * 64 * kHugeK + 64 * k + l
+        const uint32_t eb = e0 + (uint32_t)(w * 64 * kHugeK + lane);
+        uint64_t x[kHugeK];
+        uint32_t rank[kHugeK];
+#pragma unroll
+        for (int k = 0; k < kHugeK; k++) {
+          const uint32_t e = eb + 64u * k;
+          x[k] = e < L ? src[i + e] : 0ull;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) s_wcnt[w][lane + 64 * k] = 0;
+#pragma unroll
+        for (int k = 0; k < kHugeK; k++) {
+          const bool valid = eb + 64u * k < L;
+          const uint32_t d = (uint32_t)(x[k] >> sh) & 255u;
+          uint64_t m = ballot(valid);
+#pragma unroll
+          for (int bit = 0; bit < 8; bit++) {
+            const uint64_t bb = ballot((d >> bit) & 1u);
+            m &= ((d >> bit) & 1u) ? bb : ~bb;
           }
+          const uint32_t before = s_wcnt[w][d];
+          rank[k] = before + (uint32_t)popc64(m & lt);
+          if (valid && (m & lt) == 0) s_wcnt[w][d] = before + (uint32_t)popc64(m);
         }
         __syncthreads();
-      }
-    }
-    // every box of the run read (into registers) before any is written back
-    float4 rb[kPer];
-    int rid[kPer];
-    uint32_t src[kPer];
+        if (t < 256) {
+          uint32_t run = s_base[t];
 #pragma unroll
-    for (int k = 0; k < kPer; k++) {
-      const int j = t + 256 * k;
-      src[k] = j < L ? (uint32_t)s_net[j] : (uint32_t)j;
-      if (src[k] != (uint32_t)j) {
-        const int64_t sp = i + src[k];
-        rb[k] = make_float4(xmin[sp], ymin[sp], xmax[sp], ymax[sp]);
-        rid[k] = id[sp];
+          for (int k = 0; k < 16; k++) {
+            const uint32_t cnt = s_wcnt[k][t];
+            s_wcnt[k][t] = run;
+            run += cnt;
+          }
+          s_base[t] = run;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kHugeK; k++) {
+          if (eb + 64u * k < L) {
+            dst[i + s_wcnt[w][(uint32_t)(x[k] >> sh) & 255u] + rank[k]] = x[k];
+            if (pass < 3)
+              atomicAdd(&s_hist[(pass + 1) & 1][(uint32_t)(x[k] >> (sh + 8)) & 255u], 1u);
+          }
+        }
+        __syncthreads();  // s_wcnt is cleared by the next tile
       }
+      uint64_t* tmp = src;
+      src = dst;
+      dst = tmp;
+    }
+    // src == ka: the run's order; stage every box, then write back the moved
+    for (uint32_t e = t; e < L; e += 1024) {
+      const int64_t sp = i + (uint32_t)ka[i + e];
+      stage[i + e] = make_float4(xmin[sp], ymin[sp], xmax[sp], ymax[sp]);
+      stage_id[i + e] = (uint32_t)id[sp];
     }
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kPer; k++) {
-      const int j = t + 256 * k;
-      if (src[k] != (uint32_t)j) {
-        const int64_t p = i + j;
-        xmin[p] = rb[k].x;
-        ymin[p] = rb[k].y;
-        xmax[p] = rb[k].z;
-        ymax[p] = rb[k].w;
-        id[p] = rid[k];
+    for (uint32_t e = t; e < L; e += 1024) {
+      const int64_t p = i + e;
+      if ((uint32_t)ka[p] != e) {
+        const float4 b = stage[p];
+        xmin[p] = b.x;
+        ymin[p] = b.y;
+        xmax[p] = b.z;
+        ymax[p] = b.w;
+        id[p] = (int)stage_id[p];
       }
     }
+    __syncthreads();  // s_claim / s_len are rewritten for the next run
   }
 }
 
@@ -1481,16 +1701,25 @@ static void launch_tail(Prof& pf, hipStream_t s, int64_t n, BuildCtrl* c, const 
                      dim3(256), 0, s, n, c, box->xmin, box->ymin, box->xmax, box->ymax, box->id,
                      tr, l0start, r_sz, b.runs, b.runs_cap);
   if (MODE != kKeyMorton64) return;
-  pf.mark(s, "k_fixup_long");
+  pf.mark(s, "k_run_len");
   hipLaunchKernelGGL(k_run_len, dim3(kFixSlots), dim3(256), 0, s, n, c, (const float*)box->xmin,
                      (const float*)box->ymin, (const float*)box->xmax, (const float*)box->ymax,
                      b.runs, b.runs_cap);
+  pf.mark(s, "k_fixup_short");
   hipLaunchKernelGGL(k_fixup_short, dim3(kFixSlots), dim3(256), 0, s, n, c, box->xmin,
                      box->ymin, box->xmax, box->ymax, box->id, (const uint32_t*)b.runs,
                      b.runs_cap);
+  pf.mark(s, "k_fixup_long");
   hipLaunchKernelGGL(k_fixup_long, dim3(kFixLongSlots), dim3(256), 0, s, n, c, box->xmin,
                      box->ymin, box->xmax, box->ymax, box->id, (const uint32_t*)b.runs,
-                     b.runs_cap);
+                     b.runs_cap, (float4*)b.aos, b.v[0]);
+  // a run longer than n / 64 (at least 65536) takes the full 64-bit sort
+  pf.mark(s, "k_fixup_huge");
+  hipLaunchKernelGGL(k_fixup_huge, dim3(kFixHugeSlots), dim3(1024), 0, s, n, c, box->xmin,
+                     box->ymin, box->xmax, box->ymax, box->id, b.runs, b.runs_cap, b.k64[0],
+                     b.k64[1], (float4*)b.aos, b.v[0],
+                     std::max<int64_t>(int64_t(1) << 16, n / 64));
+  pf.mark(s, "k_fixup_groups");
   if constexpr (G > 0)
     hipLaunchKernelGGL(k_fixup_groups<G>, dim3(kFixSlots), dim3(256), 0, s, n,
                        (const BuildCtrl*)c, (const uint32_t*)b.runs, b.runs_cap, box->xmin,

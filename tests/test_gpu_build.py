@@ -166,18 +166,22 @@ def _clustered_points(sizes, seed, background):
     return (x, y, x.copy(), y.copy())
 
 
-@pytest.mark.parametrize("case", ["short_runs", "wave_runs", "long_runs", "too_long_runs"])
+@pytest.mark.parametrize("case", ["short_runs", "wave_runs", "long_runs", "huge_runs",
+                                  "limit_run", "too_long_runs"])
 def test_morton64_equal_high_half_runs(case):
     # the 64-bit sort orders by the high half, then sorts runs of equal high
     # halves by (low half, input position) in place: runs inside one chunk of
     # 64 positions by its wave, other runs of up to 64 by one wave each, up to
-    # 4096 by a block network; a longer run takes the full eight-pass path.
-    # Every case must equal the oracle's stable sort.
+    # 4096 by a block network, up to max(65536, n / 64) by a block LSD sort; a
+    # longer run takes the full eight-pass path (the other runs may or may not
+    # have been settled before).  Every case must equal the oracle's stable sort.
     r = np.random.default_rng(7)
     sizes = {"short_runs": r.integers(2, 33, 3000),
              "wave_runs": r.integers(48, 81, 400),
              "long_runs": r.integers(33, 1026, 40),
-             "too_long_runs": [5000, 40, 3]}[case]
+             "huge_runs": list(r.integers(4097, 20000, 12)) + [4097, 8192, 8193, 5000],
+             "limit_run": [65536, 4500, 70],
+             "too_long_runs": [65537, 9000, 5000, 40, 3]}[case]
     bx = _clustered_points(sizes, seed=len(case), background=100_000)
     n = bx[0].size
     for f in (16, 10):
