@@ -56,7 +56,8 @@ struct BuildCtrl {
   int32_t first;                   // first step that runs (-1: none)
   uint32_t fallback;               // a run is too long to settle in place: full 64-bit sort
   uint32_t nruns;                  // runs listed by k_final
-  uint32_t nlong;                  // of them longer than 64 (k_fixup_long)
+  uint32_t nmid;                   // of them 65..kWaveSortMax long (k_fixup_long, a wave each)
+  uint32_t nlong;                  // of them up to kFixLong long (k_fixup_long, a block each)
   uint32_t nhuge;                  // of them longer than kFixLong (k_fixup_huge)
   uint32_t huge_next;              // k_fixup_huge: next listed run to claim
   uint32_t trivial;                // bit d: digit d is the same for every key
@@ -85,6 +86,7 @@ __global__ __launch_bounds__(256) void k_build_init(BuildCtrl* c) {
     c->first = -1;
     c->fallback = 0;
     c->nruns = 0;
+    c->nmid = 0;
     c->nlong = 0;
     c->nhuge = 0;
     c->huge_next = 0;
@@ -883,63 +885,81 @@ __global__ __launch_bounds__(kSortThreads) void k_sweep_kv(
 // LSD sort of 64-bit keys then runs from the order reached so far (in which
 // equal full keys are still in input order).
 constexpr int kFixLong = 4096;
+constexpr int kWaveSortMax = 1024;   // runs sorted by one wave (16 items per lane)
 constexpr int kFixSlots = 4096;      // blocks of the fixup launches (they loop over the list)
 constexpr int kFixLongSlots = 1024;  // blocks of k_fixup_long (four per CU)
+constexpr int kHugeThreads = 512;    // k_fixup_huge: threads per block
+constexpr int kHugeWaves = kHugeThreads / 64;
 constexpr int kFixHugeSlots = 512;   // blocks of k_fixup_huge (two per CU)
 constexpr int kHugeK = 8;            // k_fixup_huge: positions per thread in one tile
+constexpr int kHugeLds = kHugeWaves * 64 * 16;  // k_fixup_huge: longest run sorted in LDS
 #ifndef CT_FINAL_GRID
 #define CT_FINAL_GRID 4096
 #endif
 constexpr int kFinTile = 2048;       // positions owned by one block of k_final (multiple of 64)
 
-// Length of every listed run (one wave per run; nothing is written to the
-// boxes, so every wave reads stable data).  Runs of more than 64 boxes are
-// listed again for k_fixup_long, runs longer than kFixLong (counted only up to
-// kFixLong + 1) for k_fixup_huge, from the far end of the same list.
+// Length of every listed run and the lists of the long ones.  Each wave takes
+// 64 listed runs (one per lane); the open ones (length 0: they reached the end
+// of a k_final tile) are measured one at a time by the whole wave, which reads
+// the boxes only, so every wave reads stable data.  Runs of 65..kWaveSortMax
+// boxes are listed for the waves of k_fixup_long, runs of up to kFixLong for
+// its blocks, longer ones (counted only up to kFixLong + 1) for k_fixup_huge
+// from the far end of the first list, with one atomic per wave and list.
 __global__ __launch_bounds__(256) void k_run_len(int64_t n, BuildCtrl* c,
                                                  const float* __restrict__ xmin,
                                                  const float* __restrict__ ymin,
                                                  const float* __restrict__ xmax,
                                                  const float* __restrict__ ymax, uint32_t* runs,
                                                  uint32_t runs_cap) {
-  // (no early exit on `fallback`: this launch sets it, so blocks could see
-  // it differently)
   const uint32_t nr = c->nruns < runs_cap ? c->nruns : runs_cap;
-  if (blockIdx.x * 4u >= nr) return;
+  if (blockIdx.x * 256u >= nr) return;
   const KeyParams kp = key_params<kKeyMorton64>(c);
   const int lane = lane_id();
-  const uint32_t nw = gridDim.x * 4u;
-  for (uint32_t q = (blockIdx.x * 256u + threadIdx.x) >> 6; q < nr; q += nw) {
-    int L = (int)runs[runs_cap + q];
-    if (L > 0 && L <= kWave) continue;  // (k_final measured it)
-    const int64_t i = runs[q];
-    const uint32_t h0 = (uint32_t)(morton64(xmin[i], ymin[i], xmax[i], ymax[i], kp) >> 32);
-    if (L == 0) L = kFixLong + 1;
-    // four chunks of 64 positions per round (independent loads in flight)
-    for (int64_t b = i; L > kFixLong && b - i <= kFixLong; b += 4 * kWave) {
-      bool eq[4];
+  const uint64_t lt = lanemask_lt();
+  const uint32_t wave = (blockIdx.x * 256u + threadIdx.x) >> 6, nw = gridDim.x * 4u;
+  for (uint32_t q0 = wave * 64u; q0 < nr; q0 += nw * 64u) {
+    const uint32_t q = q0 + lane;
+    const bool has = q < nr;
+    int L = has ? (int)runs[runs_cap + q] : 1;
+    const bool was_open = has && L == 0;
+    for (uint64_t open = ballot(was_open); open; open &= open - 1) {
+      const int j = __ffsll((long long)open) - 1;
+      const int64_t i = runs[q0 + j];
+      const uint32_t h0 = (uint32_t)(morton64(xmin[i], ymin[i], xmax[i], ymax[i], kp) >> 32);
+      int Lj = kFixLong + 1;
+      // four chunks of 64 positions per round (independent loads in flight)
+      for (int64_t b = i; Lj > kFixLong && b - i <= kFixLong; b += 4 * kWave) {
+        bool eq[4];
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const int64_t p = b + kWave * u + lane;
-        eq[u] = p < n &&
-                (uint32_t)(morton64(xmin[p], ymin[p], xmax[p], ymax[p], kp) >> 32) == h0;
-      }
+        for (int u = 0; u < 4; u++) {
+          const int64_t p = b + kWave * u + lane;
+          eq[u] = p < n &&
+                  (uint32_t)(morton64(xmin[p], ymin[p], xmax[p], ymax[p], kp) >> 32) == h0;
+        }
 #pragma unroll
-      for (int u = 3; u >= 0; u--) {
-        const uint64_t ne = ballot(!eq[u]);
-        if (ne) L = (int)(b - i) + kWave * u + __ffsll((long long)ne) - 1;
+        for (int u = 3; u >= 0; u--) {
+          const uint64_t ne = ballot(!eq[u]);
+          if (ne) Lj = (int)(b - i) + kWave * u + __ffsll((long long)ne) - 1;
+        }
       }
+      if (lane == j) L = Lj;
     }
+    if (was_open) runs[runs_cap + q] = (uint32_t)L;
+    const bool mid = has && L > kWave && L <= kWaveSortMax;
+    const bool lng = has && L > kWaveSortMax && L <= kFixLong, hug = has && L > kFixLong;
+    const uint64_t mm = ballot(mid), ml = ballot(lng), mh = ballot(hug);
+    uint32_t bm = 0, bl = 0, bh = 0;
     if (lane == 0) {
-      runs[runs_cap + q] = (uint32_t)L;
-      if (L > kFixLong) {
-        const uint32_t at = atomicAdd(&c->nhuge, 1u);
-        runs[3 * (size_t)runs_cap - 1 - at] = q;
-      } else if (L > kWave) {
-        const uint32_t at = atomicAdd(&c->nlong, 1u);
-        runs[2 * (size_t)runs_cap + at] = q;
-      }
+      if (mm) bm = atomicAdd(&c->nmid, (uint32_t)popc64(mm));
+      if (ml) bl = atomicAdd(&c->nlong, (uint32_t)popc64(ml));
+      if (mh) bh = atomicAdd(&c->nhuge, (uint32_t)popc64(mh));
     }
+    bm = __shfl(bm, 0);
+    bl = __shfl(bl, 0);
+    bh = __shfl(bh, 0);
+    if (mid) runs[2 * (size_t)runs_cap + bm + popc64(mm & lt)] = q;
+    if (lng) runs[3 * (size_t)runs_cap + bl + popc64(ml & lt)] = q;
+    if (hug) runs[3 * (size_t)runs_cap - 1 - (bh + popc64(mh & lt))] = q;
   }
 }
 
@@ -993,12 +1013,228 @@ __global__ __launch_bounds__(256) void k_fixup_short(int64_t n, BuildCtrl* c,
   }
 }
 
-// Listed runs of 65..kFixLong boxes, one block each: the run's
+// Sorts the run of L <= NW * 64 * kRunPer boxes at position i by (low half,
+// index in the run), in place, with a block of NW waves: the run's
 // (low half << 32 | index in the run) stay in registers (wave w, item k,
 // lane l holds index w * 64 * kUse + 64 * k + l; the slots past the run hold
 // ~0, which sorts after every real key) through four stable LSD passes over
 // 8-bit digits of the low half: ranks from ballots and per-wave running counts,
 // offsets from a scan of the counts, the scatter and the reload through LDS.
+// The moved boxes go through the run's span of the staging records.  Every
+// thread of the block calls it; it ends with a barrier.
+constexpr int kRunPer = 16;  // items per lane of lds_sort_run
+template <int NW>
+__device__ __forceinline__ void lds_sort_run(int64_t i, int L, const KeyParams& kp,
+                                             float* __restrict__ xmin, float* __restrict__ ymin,
+                                             float* __restrict__ xmax, float* __restrict__ ymax,
+                                             int* __restrict__ id, float4* __restrict__ stage,
+                                             uint32_t* __restrict__ stage_id, uint64_t* s_net,
+                                             uint32_t (*s_cnt)[256], uint32_t* s_wsum) {
+  const int t = threadIdx.x, lane = lane_id(), w = t >> 6;
+  const uint64_t lt = lanemask_lt();
+  const int kUse = (L + NW * 64 - 1) / (NW * 64);  // items per lane
+  const int wbase = w * 64 * kUse + lane;
+  uint64_t x[kRunPer];
+#pragma unroll
+  for (int k = 0; k < kRunPer; k++) {
+    x[k] = ~0ull;
+    const int e = wbase + 64 * k;
+    if (k < kUse && e < L) {
+      const int64_t p = i + e;
+      x[k] = ((uint64_t)(uint32_t)morton64(xmin[p], ymin[p], xmax[p], ymax[p], kp) << 32) |
+             (uint32_t)e;
+    }
+  }
+  for (int pass = 0; pass < 4; pass++) {
+    const int sh = 32 + 8 * pass;
+#pragma unroll
+    for (int k = 0; k < 4; k++) s_cnt[w][lane + 64 * k] = 0;
+    uint32_t rank[kRunPer];
+#pragma unroll
+    for (int k = 0; k < kRunPer; k++) {
+      if (k < kUse) {
+        const uint32_t d = (uint32_t)(x[k] >> sh) & 255u;
+        uint64_t m = ~0ull;
+#pragma unroll
+        for (int bit = 0; bit < 8; bit++) {
+          const uint64_t bb = ballot((d >> bit) & 1u);
+          m &= ((d >> bit) & 1u) ? bb : ~bb;
+        }
+        const uint32_t before = s_cnt[w][d];
+        rank[k] = before + (uint32_t)popc64(m & lt);
+        if ((m & lt) == 0) s_cnt[w][d] = before + (uint32_t)popc64(m);
+      }
+    }
+    __syncthreads();
+    // digit t (threads 0..255): offsets of the waves' shares, after all
+    // smaller digits
+    uint32_t cw[NW], tot = 0, inc = 0;
+    if (t < 256) {
+#pragma unroll
+      for (int k = 0; k < NW; k++) {
+        cw[k] = s_cnt[k][t];
+        tot += cw[k];
+      }
+      inc = tot;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t u = __shfl_up(inc, d);
+        if (lane >= d) inc += u;
+      }
+      if (lane == 63) s_wsum[w] = inc;
+    }
+    __syncthreads();
+    if (t < 256) {
+      uint32_t off = inc - tot;
+      for (int k = 0; k < w; k++) off += s_wsum[k];
+#pragma unroll
+      for (int k = 0; k < NW; k++) {
+        s_cnt[k][t] = off;
+        off += cw[k];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kRunPer; k++)
+      if (k < kUse) s_net[s_cnt[w][(uint32_t)(x[k] >> sh) & 255u] + rank[k]] = x[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kRunPer; k++)
+      if (k < kUse) x[k] = s_net[wbase + 64 * k];
+  }
+  // every moved box staged before any is written back
+#pragma unroll
+  for (int k = 0; k < kRunPer; k++) {
+    const int e = wbase + 64 * k;
+    if (k < kUse && e < L && (uint32_t)x[k] != (uint32_t)e) {
+      const int64_t sp = i + (uint32_t)x[k];
+      stage[i + e] = make_float4(xmin[sp], ymin[sp], xmax[sp], ymax[sp]);
+      stage_id[i + e] = (uint32_t)id[sp];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kRunPer; k++) {
+    const int e = wbase + 64 * k;
+    if (k < kUse && e < L && (uint32_t)x[k] != (uint32_t)e) {
+      const int64_t p = i + e;
+      const float4 b = stage[p];
+      xmin[p] = b.x;
+      ymin[p] = b.y;
+      xmax[p] = b.z;
+      ymax[p] = b.w;
+      id[p] = (int)stage_id[p];
+    }
+  }
+  __syncthreads();  // s_net and s_cnt are reused by the next run
+}
+
+// Sorts the run of 65..kWaveSortMax boxes at position i like lds_sort_run,
+// with one wave and no barriers: lane l, item k holds index 64 * k + l; the
+// wave's own LDS rows hold the counts (lane l scans digits 4l..4l+3) and the
+// scatter.  LDS written by one lane and read by another is ordered by the
+// wave's program order (LDS operations of one wave complete in order).
+__device__ __forceinline__ void wave_sort_run(int64_t i, int L, const KeyParams& kp,
+                                              float* __restrict__ xmin, float* __restrict__ ymin,
+                                              float* __restrict__ xmax, float* __restrict__ ymax,
+                                              int* __restrict__ id, float4* __restrict__ stage,
+                                              uint32_t* __restrict__ stage_id, uint64_t* s_w,
+                                              uint32_t* s_c) {
+  const int lane = lane_id();
+  const uint64_t lt = lanemask_lt();
+  const int kUse = (L + 63) >> 6;
+  uint64_t x[kRunPer];
+#pragma unroll
+  for (int k = 0; k < kRunPer; k++) {
+    x[k] = ~0ull;
+    const int e = 64 * k + lane;
+    if (k < kUse && e < L) {
+      const int64_t p = i + e;
+      x[k] = ((uint64_t)(uint32_t)morton64(xmin[p], ymin[p], xmax[p], ymax[p], kp) << 32) |
+             (uint32_t)e;
+    }
+  }
+  for (int pass = 0; pass < 4; pass++) {
+    const int sh = 32 + 8 * pass;
+#pragma unroll
+    for (int k = 0; k < 4; k++) s_c[lane + 64 * k] = 0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    uint32_t rank[kRunPer];
+#pragma unroll
+    for (int k = 0; k < kRunPer; k++) {
+      if (k < kUse) {
+        const uint32_t d = (uint32_t)(x[k] >> sh) & 255u;
+        uint64_t m = ~0ull;
+#pragma unroll
+        for (int bit = 0; bit < 8; bit++) {
+          const uint64_t bb = ballot((d >> bit) & 1u);
+          m &= ((d >> bit) & 1u) ? bb : ~bb;
+        }
+        const uint32_t before = s_c[d];
+        rank[k] = before + (uint32_t)popc64(m & lt);
+        if ((m & lt) == 0) s_c[d] = before + (uint32_t)popc64(m);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+      }
+    }
+    const uint4 cnt = *(const uint4*)&s_c[4 * lane];
+    const uint32_t tot = cnt.x + cnt.y + cnt.z + cnt.w;
+    uint32_t inc = tot;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t u = __shfl_up(inc, d);
+      if (lane >= d) inc += u;
+    }
+    const uint32_t o = inc - tot;
+    *(uint4*)&s_c[4 * lane] = make_uint4(o, o + cnt.x, o + cnt.x + cnt.y,
+                                         o + cnt.x + cnt.y + cnt.z);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int k = 0; k < kRunPer; k++)
+      if (k < kUse) s_w[s_c[(uint32_t)(x[k] >> sh) & 255u] + rank[k]] = x[k];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int k = 0; k < kRunPer; k++)
+      if (k < kUse) x[k] = s_w[64 * k + lane];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+  // the moved boxes through the run's span of the staging records: a lane
+  // stages and writes back the same positions, and every box is read (its
+  // staging store waits for it) before the first box store
+#pragma unroll
+  for (int k = 0; k < kRunPer; k++) {
+    const int e = 64 * k + lane;
+    if (k < kUse && e < L && (uint32_t)x[k] != (uint32_t)e) {
+      const int64_t sp = i + (uint32_t)x[k];
+      stage[i + e] = make_float4(xmin[sp], ymin[sp], xmax[sp], ymax[sp]);
+      stage_id[i + e] = (uint32_t)id[sp];
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int k = 0; k < kRunPer; k++) {
+    const int e = 64 * k + lane;
+    if (k < kUse && e < L && (uint32_t)x[k] != (uint32_t)e) {
+      const int64_t p = i + e;
+      const float4 b = stage[p];
+      xmin[p] = b.x;
+      ymin[p] = b.y;
+      xmax[p] = b.z;
+      ymax[p] = b.w;
+      id[p] = (int)stage_id[p];
+    }
+  }
+}
+
+// Listed runs of 1025..kFixLong boxes, one block of four waves each, then the
+// listed runs of 65..kWaveSortMax boxes, one wave each.
+static_assert(kFixLong == 4 * 64 * kRunPer, "k_fixup_long: one item slot per box");
+static_assert(kWaveSortMax == 64 * kRunPer, "wave_sort_run: one item slot per box");
 __global__ __launch_bounds__(256) void k_fixup_long(int64_t n, BuildCtrl* c,
                                                     float* __restrict__ xmin,
                                                     float* __restrict__ ymin,
@@ -1009,119 +1245,37 @@ __global__ __launch_bounds__(256) void k_fixup_long(int64_t n, BuildCtrl* c,
                                                     uint32_t runs_cap,
                                                     float4* __restrict__ stage,
                                                     uint32_t* __restrict__ stage_id) {
-  constexpr int kPer = kFixLong / 256;
   __shared__ uint64_t s_net[kFixLong];
-  __shared__ uint32_t s_cnt[4][256];  // per wave, per digit: running count -> offset
+  __shared__ __align__(16) uint32_t s_cnt[4][256];  // per wave, per digit: count -> offset
   __shared__ uint32_t s_wsum[4];
-  const uint32_t nl = c->nlong;
-  if (blockIdx.x >= nl || c->fallback) return;
+  const uint32_t nl = c->nlong, nm = c->nmid;
+  if ((blockIdx.x >= nl && blockIdx.x * 4u >= nm) || c->fallback) return;
   const KeyParams kp = key_params<kKeyMorton64>(c);
-  const int t = threadIdx.x, lane = lane_id(), w = t >> 6;
-  const uint64_t lt = lanemask_lt();
   for (uint32_t r = blockIdx.x; r < nl; r += gridDim.x) {
+    const uint32_t q = runs[3 * (size_t)runs_cap + r];
+    lds_sort_run<4>(runs[q], (int)runs[runs_cap + q], kp, xmin, ymin, xmax, ymax, id, stage,
+                    stage_id, s_net, s_cnt, s_wsum);
+  }
+  const int w = threadIdx.x >> 6;
+  for (uint32_t r = blockIdx.x * 4u + w; r < nm; r += gridDim.x * 4u) {
     const uint32_t q = runs[2 * (size_t)runs_cap + r];
-    const int64_t i = runs[q];
-    const int L = (int)runs[runs_cap + q];
-    const int kUse = (L + 255) >> 8;  // items per lane
-    const int wbase = w * 64 * kUse + lane;
-    uint64_t x[kPer];
-#pragma unroll
-    for (int k = 0; k < kPer; k++) {
-      x[k] = ~0ull;
-      const int e = wbase + 64 * k;
-      if (k < kUse && e < L) {
-        const int64_t p = i + e;
-        x[k] = ((uint64_t)(uint32_t)morton64(xmin[p], ymin[p], xmax[p], ymax[p], kp) << 32) |
-               (uint32_t)e;
-      }
-    }
-    for (int pass = 0; pass < 4; pass++) {
-      const int sh = 32 + 8 * pass;
-#pragma unroll
-      for (int k = 0; k < 4; k++) s_cnt[w][lane + 64 * k] = 0;
-      uint32_t rank[kPer];
-#pragma unroll
-      for (int k = 0; k < kPer; k++) {
-        if (k < kUse) {
-          const uint32_t d = (uint32_t)(x[k] >> sh) & 255u;
-          uint64_t m = ~0ull;
-#pragma unroll
-          for (int bit = 0; bit < 8; bit++) {
-            const uint64_t bb = ballot((d >> bit) & 1u);
-            m &= ((d >> bit) & 1u) ? bb : ~bb;
-          }
-          const uint32_t before = s_cnt[w][d];
-          rank[k] = before + (uint32_t)popc64(m & lt);
-          if ((m & lt) == 0) s_cnt[w][d] = before + (uint32_t)popc64(m);
-        }
-      }
-      __syncthreads();
-      // digit t: offsets of the four waves' shares, after all smaller digits
-      const uint32_t c0 = s_cnt[0][t], c1 = s_cnt[1][t], c2 = s_cnt[2][t], c3 = s_cnt[3][t];
-      const uint32_t tot = c0 + c1 + c2 + c3;
-      uint32_t inc = tot;
-#pragma unroll
-      for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t u = __shfl_up(inc, d);
-        if (lane >= d) inc += u;
-      }
-      if (lane == 63) s_wsum[w] = inc;
-      __syncthreads();
-      uint32_t off = inc - tot;
-      for (int k = 0; k < w; k++) off += s_wsum[k];
-      s_cnt[0][t] = off;
-      s_cnt[1][t] = off + c0;
-      s_cnt[2][t] = off + c0 + c1;
-      s_cnt[3][t] = off + c0 + c1 + c2;
-      __syncthreads();
-#pragma unroll
-      for (int k = 0; k < kPer; k++)
-        if (k < kUse) s_net[s_cnt[w][(uint32_t)(x[k] >> sh) & 255u] + rank[k]] = x[k];
-      __syncthreads();
-#pragma unroll
-      for (int k = 0; k < kPer; k++)
-        if (k < kUse) x[k] = s_net[wbase + 64 * k];
-    }
-    // every moved box staged (the run's span of the free staging records)
-    // before any is written back
-#pragma unroll
-    for (int k = 0; k < kPer; k++) {
-      const int e = wbase + 64 * k;
-      if (k < kUse && e < L && (uint32_t)x[k] != (uint32_t)e) {
-        const int64_t sp = i + (uint32_t)x[k];
-        stage[i + e] = make_float4(xmin[sp], ymin[sp], xmax[sp], ymax[sp]);
-        stage_id[i + e] = (uint32_t)id[sp];
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kPer; k++) {
-      const int e = wbase + 64 * k;
-      if (k < kUse && e < L && (uint32_t)x[k] != (uint32_t)e) {
-        const int64_t p = i + e;
-        const float4 b = stage[p];
-        xmin[p] = b.x;
-        ymin[p] = b.y;
-        xmax[p] = b.z;
-        ymax[p] = b.w;
-        id[p] = (int)stage_id[p];
-      }
-    }
-    __syncthreads();  // s_net and s_cnt are reused by the next run
+    wave_sort_run(runs[q], (int)runs[runs_cap + q], kp, xmin, ymin, xmax, ymax, id, stage,
+                  stage_id, s_net + kWaveSortMax * w, s_cnt[w]);
   }
 }
 
 // Listed runs longer than kFixLong, one block each, claimed by ticket.  The
 // block measures the run (k_run_len stopped at kFixLong + 1) and records its
-// length for k_fixup_groups, sorts (low half << 32 | index in the run) by the
-// low half with a stable LSD sort over the run's own span of the two free key
-// buffers (four 8-bit digits; tiles of 1024 * kHugeK positions: ranks from
-// ballots and per-wave running counts, wave offsets from a scan over the 16
-// waves), then moves the boxes through staging records.  A run longer than `hmax` (one block would
-// take longer than the full 64-bit sort) sets `fallback` and stays as it is;
-// every other run is settled completely, so the fallback may start from any
-// mix of the two.
-__global__ __launch_bounds__(1024) void k_fixup_huge(
+// length for k_fixup_groups.  A run of up to kHugeLds boxes is sorted in LDS
+// (lds_sort_run); a longer one sorts (low half << 32 | index in the run) by
+// the low half with a stable LSD sort over the run's own span of the two free
+// key buffers (four 8-bit digits; tiles of kHugeThreads * kHugeK positions:
+// ranks from ballots and per-wave running counts, wave offsets from a scan
+// over the block's waves), then moves the boxes through staging records.  A
+// run longer than `hmax` (one block would take longer than the full 64-bit
+// sort) sets `fallback` and stays as it is; every other run is settled
+// completely, so the fallback may start from any mix of the two.
+__global__ __launch_bounds__(kHugeThreads) void k_fixup_huge(
     int64_t n, BuildCtrl* c, float* __restrict__ xmin, float* __restrict__ ymin,
     float* __restrict__ xmax, float* __restrict__ ymax, int* __restrict__ id,
     uint32_t* __restrict__ runs, uint32_t runs_cap, uint64_t* __restrict__ ka,
@@ -1129,9 +1283,10 @@ __global__ __launch_bounds__(1024) void k_fixup_huge(
     int64_t hmax) {
   __shared__ uint32_t s_hist[2][256];  // digit counts of this pass / the next
   __shared__ uint32_t s_base[256];     // running offset of each digit
-  __shared__ uint32_t s_wcnt[16][256]; // a tile's count per (wave, digit) -> offset
+  __shared__ uint32_t s_wcnt[kHugeWaves][256];  // a tile's count per (wave, digit) -> offset
   __shared__ uint32_t s_wsum[4];
   __shared__ uint32_t s_claim, s_len;
+  __shared__ uint64_t s_net[kHugeLds];
   const uint32_t nh = c->nhuge;
   if (blockIdx.x >= nh || c->fallback) return;
   const KeyParams kp = key_params<kKeyMorton64>(c);
@@ -1150,7 +1305,7 @@ __global__ __launch_bounds__(1024) void k_fixup_huge(
     const uint32_t h0 = (uint32_t)(morton64(xmin[i], ymin[i], xmax[i], ymax[i], kp) >> 32);
     // the run's end: the first later position with another high half
     const int64_t lim = n - i < hmax + 1 ? n - i : hmax + 1;
-    for (int64_t b = kFixLong; b < lim; b += 1024) {
+    for (int64_t b = kFixLong; b < lim; b += kHugeThreads) {
       const int64_t p = i + b + t;
       if (b + t < lim &&
           (uint32_t)(morton64(xmin[p], ymin[p], xmax[p], ymax[p], kp) >> 32) != h0)
@@ -1167,12 +1322,17 @@ __global__ __launch_bounds__(1024) void k_fixup_huge(
       continue;
     }
     if (t == 0) runs[runs_cap + q] = L;
+    if ((int)L <= kHugeLds) {
+      lds_sort_run<kHugeWaves>(i, (int)L, kp, xmin, ymin, xmax, ymax, id, stage, stage_id, s_net,
+                               s_wcnt, s_wsum);
+      continue;
+    }
     if (t < 256) {
       s_hist[0][t] = 0;
       s_hist[1][t] = 0;
     }
     __syncthreads();
-    for (uint32_t e = t; e < L; e += 1024) {
+    for (uint32_t e = t; e < L; e += kHugeThreads) {
       const int64_t p = i + e;
       const uint32_t lo = (uint32_t)morton64(xmin[p], ymin[p], xmax[p], ymax[p], kp);
       ka[p] = (uint64_t)lo << 32 | e;
@@ -1202,7 +1362,7 @@ __global__ __launch_bounds__(1024) void k_fixup_huge(
         for (int k = 0; k < w; k++) off += s_wsum[k];
         s_base[t] = off + inc - v;
       }
-      for (uint32_t e0 = 0; e0 < L; e0 += 1024 * kHugeK) {
+      for (uint32_t e0 = 0; e0 < L; e0 += kHugeThreads * kHugeK) {
         // wave w, item k, lane l: index e0 + w * 64 * kHugeK + 64 * k + l
         const uint32_t eb = e0 + (uint32_t)(w * 64 * kHugeK + lane);
         uint64_t x[kHugeK];
@@ -1232,7 +1392,7 @@ __global__ __launch_bounds__(1024) void k_fixup_huge(
         if (t < 256) {
           uint32_t run = s_base[t];
 #pragma unroll
-          for (int k = 0; k < 16; k++) {
+          for (int k = 0; k < kHugeWaves; k++) {
             const uint32_t cnt = s_wcnt[k][t];
             s_wcnt[k][t] = run;
             run += cnt;
@@ -1255,13 +1415,13 @@ __global__ __launch_bounds__(1024) void k_fixup_huge(
       dst = tmp;
     }
     // src == ka: the run's order; stage every box, then write back the moved
-    for (uint32_t e = t; e < L; e += 1024) {
+    for (uint32_t e = t; e < L; e += kHugeThreads) {
       const int64_t sp = i + (uint32_t)ka[i + e];
       stage[i + e] = make_float4(xmin[sp], ymin[sp], xmax[sp], ymax[sp]);
       stage_id[i + e] = (uint32_t)id[sp];
     }
     __syncthreads();
-    for (uint32_t e = t; e < L; e += 1024) {
+    for (uint32_t e = t; e < L; e += kHugeThreads) {
       const int64_t p = i + e;
       if ((uint32_t)ka[p] != e) {
         const float4 b = stage[p];
@@ -1715,7 +1875,7 @@ static void launch_tail(Prof& pf, hipStream_t s, int64_t n, BuildCtrl* c, const 
                      b.runs_cap, (float4*)b.aos, b.v[0]);
   // a run longer than n / 64 (at least 65536) takes the full 64-bit sort
   pf.mark(s, "k_fixup_huge");
-  hipLaunchKernelGGL(k_fixup_huge, dim3(kFixHugeSlots), dim3(1024), 0, s, n, c, box->xmin,
+  hipLaunchKernelGGL(k_fixup_huge, dim3(kFixHugeSlots), dim3(kHugeThreads), 0, s, n, c, box->xmin,
                      box->ymin, box->xmax, box->ymax, box->id, b.runs, b.runs_cap, b.k64[0],
                      b.k64[1], (float4*)b.aos, b.v[0],
                      std::max<int64_t>(int64_t(1) << 16, n / 64));
@@ -1838,9 +1998,9 @@ int build_f32(Context& ctx, curtree_bbox_f32* box, curtree_rtree_f32* rt, int ke
       (rc = ws.reserve(WsBuildCtrl, sizeof(BuildCtrl))))
     return fail(rc);
   // listed runs of equal high halves (64-bit keys): start, length and the
-  // list of the long ones; at most two per chunk of 64 positions
+  // lists of the long ones; at most two per chunk of 64 positions
   const uint32_t runs_cap = m64 ? (uint32_t)(2 * ((n + kWave - 1) / kWave) + 2) : 1;
-  if ((rc = ws.reserve(WsBuildRuns, (size_t)runs_cap * 12))) return fail(rc);
+  if ((rc = ws.reserve(WsBuildRuns, (size_t)runs_cap * 16))) return fail(rc);
   BuildCtrl* c = (BuildCtrl*)ws.ptr(WsBuildCtrl);
   SortBufs sb;
   sb.kv[0] = (uint64_t*)ws.ptr(WsBuildKeys0);
