@@ -1013,38 +1013,20 @@ __global__ __launch_bounds__(256) void k_fixup_short(int64_t n, BuildCtrl* c,
   }
 }
 
-// Sorts the run of L <= NW * 64 * kRunPer boxes at position i by (low half,
-// index in the run), in place, with a block of NW waves: the run's
-// (low half << 32 | index in the run) stay in registers (wave w, item k,
-// lane l holds index w * 64 * kUse + 64 * k + l; the slots past the run hold
-// ~0, which sorts after every real key) through four stable LSD passes over
-// 8-bit digits of the low half: ranks from ballots and per-wave running counts,
-// offsets from a scan of the counts, the scatter and the reload through LDS.
-// The moved boxes go through the run's span of the staging records.  Every
-// thread of the block calls it; it ends with a barrier.
-constexpr int kRunPer = 16;  // items per lane of lds_sort_run
+// Four stable LSD passes over 8-bit digits of the high words of x (the low
+// half of the 64-bit key above the index in the run), with a block of NW
+// waves: wave w, item k, lane l holds index wbase + 64 * k with
+// wbase = w * 64 * kUse + l; slots past the run hold ~0, which sorts after
+// every real key.  Ranks from ballots and per-wave running counts, offsets
+// from a scan of the counts, the scatter and the reload through LDS.  Every
+// thread of the block calls it.
+constexpr int kRunPer = 16;  // items per lane of the LDS run sorts
 template <int NW>
-__device__ __forceinline__ void lds_sort_run(int64_t i, int L, const KeyParams& kp,
-                                             float* __restrict__ xmin, float* __restrict__ ymin,
-                                             float* __restrict__ xmax, float* __restrict__ ymax,
-                                             int* __restrict__ id, float4* __restrict__ stage,
-                                             uint32_t* __restrict__ stage_id, uint64_t* s_net,
-                                             uint32_t (*s_cnt)[256], uint32_t* s_wsum) {
+__device__ __forceinline__ void lds_lsd(uint64_t (&x)[kRunPer], int kUse, int wbase,
+                                        uint64_t* s_net, uint32_t (*s_cnt)[256],
+                                        uint32_t* s_wsum) {
   const int t = threadIdx.x, lane = lane_id(), w = t >> 6;
   const uint64_t lt = lanemask_lt();
-  const int kUse = (L + NW * 64 - 1) / (NW * 64);  // items per lane
-  const int wbase = w * 64 * kUse + lane;
-  uint64_t x[kRunPer];
-#pragma unroll
-  for (int k = 0; k < kRunPer; k++) {
-    x[k] = ~0ull;
-    const int e = wbase + 64 * k;
-    if (k < kUse && e < L) {
-      const int64_t p = i + e;
-      x[k] = ((uint64_t)(uint32_t)morton64(xmin[p], ymin[p], xmax[p], ymax[p], kp) << 32) |
-             (uint32_t)e;
-    }
-  }
   for (int pass = 0; pass < 4; pass++) {
     const int sh = 32 + 8 * pass;
 #pragma unroll
@@ -1102,6 +1084,34 @@ __device__ __forceinline__ void lds_sort_run(int64_t i, int L, const KeyParams& 
     for (int k = 0; k < kRunPer; k++)
       if (k < kUse) x[k] = s_net[wbase + 64 * k];
   }
+}
+
+// Sorts the run of L <= NW * 64 * kRunPer boxes at position i by (low half,
+// index in the run), in place, with a block of NW waves (lds_lsd over the
+// keys computed from the boxes); the moved boxes go through the run's span of
+// the staging records.  Every thread of the block calls it; it ends with a
+// barrier.
+template <int NW>
+__device__ __forceinline__ void lds_sort_run(int64_t i, int L, const KeyParams& kp,
+                                             float* __restrict__ xmin, float* __restrict__ ymin,
+                                             float* __restrict__ xmax, float* __restrict__ ymax,
+                                             int* __restrict__ id, float4* __restrict__ stage,
+                                             uint32_t* __restrict__ stage_id, uint64_t* s_net,
+                                             uint32_t (*s_cnt)[256], uint32_t* s_wsum) {
+  const int kUse = (L + NW * 64 - 1) / (NW * 64);  // items per lane
+  const int wbase = (threadIdx.x >> 6) * 64 * kUse + lane_id();
+  uint64_t x[kRunPer];
+#pragma unroll
+  for (int k = 0; k < kRunPer; k++) {
+    x[k] = ~0ull;
+    const int e = wbase + 64 * k;
+    if (k < kUse && e < L) {
+      const int64_t p = i + e;
+      x[k] = ((uint64_t)(uint32_t)morton64(xmin[p], ymin[p], xmax[p], ymax[p], kp) << 32) |
+             (uint32_t)e;
+    }
+  }
+  lds_lsd<NW>(x, kUse, wbase, s_net, s_cnt, s_wsum);
   // every moved box staged before any is written back
 #pragma unroll
   for (int k = 0; k < kRunPer; k++) {
@@ -1127,6 +1137,31 @@ __device__ __forceinline__ void lds_sort_run(int64_t i, int L, const KeyParams& 
     }
   }
   __syncthreads();  // s_net and s_cnt are reused by the next run
+}
+
+// Sorts L <= NW * 64 * kRunPer keys (low half << 32 | index) from `src` into
+// `dst` with a block of NW waves (lds_lsd).  Every thread of the block calls
+// it; it ends with a barrier.
+template <int NW>
+__device__ __forceinline__ void lds_sort_keys(const uint64_t* __restrict__ src,
+                                              uint64_t* __restrict__ dst, int L,
+                                              uint64_t* s_net, uint32_t (*s_cnt)[256],
+                                              uint32_t* s_wsum) {
+  const int kUse = (L + NW * 64 - 1) / (NW * 64);
+  const int wbase = (threadIdx.x >> 6) * 64 * kUse + lane_id();
+  uint64_t x[kRunPer];
+#pragma unroll
+  for (int k = 0; k < kRunPer; k++) {
+    const int e = wbase + 64 * k;
+    x[k] = k < kUse && e < L ? src[e] : ~0ull;
+  }
+  lds_lsd<NW>(x, kUse, wbase, s_net, s_cnt, s_wsum);
+#pragma unroll
+  for (int k = 0; k < kRunPer; k++) {
+    const int e = wbase + 64 * k;
+    if (k < kUse && e < L) dst[e] = x[k];
+  }
+  __syncthreads();  // s_net and s_cnt are reused
 }
 
 // Sorts the run of 65..kWaveSortMax boxes at position i like lds_sort_run,
@@ -1267,15 +1302,17 @@ __global__ __launch_bounds__(256) void k_fixup_long(int64_t n, BuildCtrl* c,
 // Listed runs longer than kFixLong, one block each, claimed by ticket.  The
 // block measures the run (k_run_len stopped at kFixLong + 1) and records its
 // length for k_fixup_groups.  A run of up to kHugeLds boxes is sorted in LDS
-// (lds_sort_run); a longer one sorts (low half << 32 | index in the run) by
-// the low half with a stable LSD sort over the run's own span of the two free
-// key buffers (four 8-bit digits; tiles of kHugeThreads * kHugeK positions:
-// ranks from ballots and per-wave running counts, wave offsets from a scan
-// over the block's waves), then moves the boxes through staging records.  A
-// run longer than `hmax` (one block would take longer than the full 64-bit
-// sort) sets `fallback` and stays as it is; every other run is settled
-// completely, so the fallback may start from any mix of the two.
-__global__ __launch_bounds__(kHugeThreads) void k_fixup_huge(
+// (lds_sort_run).  A longer one is split stably by the top 8 bits of the low
+// half (one LSD-style scatter of (low half << 32 | index in the run) over the
+// run's span of the free key buffers), groups of whole digits of up to
+// kHugeLds keys are sorted in LDS (lds_sort_keys), and the boxes are moved
+// once, through staging records; if one digit alone holds more, four LSD
+// passes over the whole run in the key buffers sort it.  A run longer than `hmax` (one block would take
+// longer than the full 64-bit sort) sets `fallback` and stays as it is; every
+// other run is settled completely, so the fallback may start from any mix of
+// the two.
+// (two blocks per CU: at most 128 VGPRs)
+__global__ __launch_bounds__(kHugeThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_fixup_huge(
     int64_t n, BuildCtrl* c, float* __restrict__ xmin, float* __restrict__ ymin,
     float* __restrict__ xmax, float* __restrict__ ymax, int* __restrict__ id,
     uint32_t* __restrict__ runs, uint32_t runs_cap, uint64_t* __restrict__ ka,
@@ -1283,15 +1320,180 @@ __global__ __launch_bounds__(kHugeThreads) void k_fixup_huge(
     int64_t hmax) {
   __shared__ uint32_t s_hist[2][256];  // digit counts of this pass / the next
   __shared__ uint32_t s_base[256];     // running offset of each digit
+  __shared__ uint32_t s_bstart[256];   // first offset of each digit (the split pass)
   __shared__ uint32_t s_wcnt[kHugeWaves][256];  // a tile's count per (wave, digit) -> offset
   __shared__ uint32_t s_wsum[4];
-  __shared__ uint32_t s_claim, s_len;
+  __shared__ uint32_t s_claim, s_len, s_nchunk, s_big;
+  __shared__ uint32_t s_chunk[258];
   __shared__ uint64_t s_net[kHugeLds];
   const uint32_t nh = c->nhuge;
   if (blockIdx.x >= nh || c->fallback) return;
   const KeyParams kp = key_params<kKeyMorton64>(c);
   const int t = threadIdx.x, lane = lane_id(), w = t >> 6;
   const uint64_t lt = lanemask_lt();
+
+  // (low half << 32 | index) of the run's boxes into ka, and the counts of
+  // the digit at `hsh` of the low half into s_hist[0]
+  auto load_keys = [&](int64_t i, uint32_t L, int hsh) {
+    if (t < 256) {
+      s_hist[0][t] = 0;
+      s_hist[1][t] = 0;
+    }
+    __syncthreads();
+    // four positions per thread and round: the loads of a round in flight together
+    for (uint32_t e0 = t; e0 < L; e0 += 4 * kHugeThreads) {
+      uint32_t lo[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const uint32_t e = e0 + kHugeThreads * u;
+        const int64_t p = i + e;
+        lo[u] = e < L ? (uint32_t)morton64(xmin[p], ymin[p], xmax[p], ymax[p], kp) : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const uint32_t e = e0 + kHugeThreads * u;
+        if (e < L) {
+          ka[i + e] = (uint64_t)lo[u] << 32 | e;
+          atomicAdd(&s_hist[0][(lo[u] >> hsh) & 255u], 1u);
+        }
+      }
+    }
+  };
+  // one stable scatter of the run's keys by the digit at bit `sh`, src -> dst,
+  // from the counts in s_hist[h]; with `next`, the counts of the following
+  // digit go to s_hist[h ^ 1]; with `starts`, the digits' first offsets to
+  // s_bstart
+  auto lsd_pass = [&](int64_t i, uint32_t L, const uint64_t* src, uint64_t* dst, int sh, int h,
+                      bool next, bool starts) {
+    __syncthreads();  // s_hist[h] is complete
+    uint32_t v = 0, inc = 0;
+    if (t < 256) {
+      v = s_hist[h][t];
+      inc = v;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t u = __shfl_up(inc, d);
+        if (lane >= d) inc += u;
+      }
+      if (lane == 63) s_wsum[w] = inc;
+      s_hist[h ^ 1][t] = 0;
+    }
+    __syncthreads();
+    if (t < 256) {
+      uint32_t off = 0;
+      for (int k = 0; k < w; k++) off += s_wsum[k];
+      s_base[t] = off + inc - v;
+      if (starts) s_bstart[t] = off + inc - v;
+    }
+    for (uint32_t e0 = 0; e0 < L; e0 += kHugeThreads * kHugeK) {
+      // wave w, item k, lane l: index e0 + w * 64 * kHugeK + 64 * k + l
+      const uint32_t eb = e0 + (uint32_t)(w * 64 * kHugeK + lane);
+      uint64_t x[kHugeK];
+      uint32_t rank[kHugeK];
+#pragma unroll
+      for (int k = 0; k < kHugeK; k++) {
+        const uint32_t e = eb + 64u * k;
+        x[k] = e < L ? src[i + e] : 0ull;
+      }
+#pragma unroll
+      for (int k = 0; k < 4; k++) s_wcnt[w][lane + 64 * k] = 0;
+#pragma unroll
+      for (int k = 0; k < kHugeK; k++) {
+        const bool valid = eb + 64u * k < L;
+        const uint32_t d = (uint32_t)(x[k] >> sh) & 255u;
+        uint64_t m = ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; bit++) {
+          const uint64_t bb = ballot((d >> bit) & 1u);
+          m &= ((d >> bit) & 1u) ? bb : ~bb;
+        }
+        const uint32_t before = s_wcnt[w][d];
+        rank[k] = before + (uint32_t)popc64(m & lt);
+        if (valid && (m & lt) == 0) s_wcnt[w][d] = before + (uint32_t)popc64(m);
+      }
+      __syncthreads();
+      if (t < 256) {
+        uint32_t run = s_base[t];
+#pragma unroll
+        for (int k = 0; k < kHugeWaves; k++) {
+          const uint32_t cnt = s_wcnt[k][t];
+          s_wcnt[k][t] = run;
+          run += cnt;
+        }
+        s_base[t] = run;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < kHugeK; k++) {
+        if (eb + 64u * k < L) {
+          dst[i + s_wcnt[w][(uint32_t)(x[k] >> sh) & 255u] + rank[k]] = x[k];
+          if (next)
+            atomicAdd(&s_hist[h ^ 1][(uint32_t)(x[k] >> (sh + 8)) & 255u], 1u);
+        }
+      }
+      __syncthreads();  // s_wcnt is cleared by the next tile
+    }
+  };
+  // the run's boxes put in the order of `k` (the index halves) through the
+  // run's span of the staging records
+  auto gather = [&](int64_t i, uint32_t L, const uint64_t* k) {
+    // four positions per thread and round (loads in flight together)
+    for (uint32_t e0 = t; e0 < L; e0 += 4 * kHugeThreads) {
+      uint32_t src[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const uint32_t e = e0 + kHugeThreads * u;
+        src[u] = e < L ? (uint32_t)k[i + e] : 0u;
+      }
+      float4 b[4];
+      int v[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const uint32_t e = e0 + kHugeThreads * u;
+        if (e < L) {
+          const int64_t sp = i + src[u];
+          b[u] = make_float4(xmin[sp], ymin[sp], xmax[sp], ymax[sp]);
+          v[u] = id[sp];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const uint32_t e = e0 + kHugeThreads * u;
+        if (e < L) {
+          stage[i + e] = b[u];
+          stage_id[i + e] = (uint32_t)v[u];
+        }
+      }
+    }
+    __syncthreads();
+    for (uint32_t e0 = t; e0 < L; e0 += 4 * kHugeThreads) {
+      float4 b[4];
+      uint32_t v[4];
+      bool mv[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const uint32_t e = e0 + kHugeThreads * u;
+        mv[u] = e < L && (uint32_t)k[i + e] != e;
+        if (mv[u]) {
+          b[u] = stage[i + e];
+          v[u] = stage_id[i + e];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        if (mv[u]) {
+          const int64_t p = i + e0 + kHugeThreads * u;
+          xmin[p] = b[u].x;
+          ymin[p] = b[u].y;
+          xmax[p] = b[u].z;
+          ymax[p] = b[u].w;
+          id[p] = (int)v[u];
+        }
+      }
+    }
+    __syncthreads();
+  };
+
   for (;;) {
     if (t == 0) {
       s_claim = atomicAdd(&c->huge_next, 1u);
@@ -1305,11 +1507,14 @@ __global__ __launch_bounds__(kHugeThreads) void k_fixup_huge(
     const uint32_t h0 = (uint32_t)(morton64(xmin[i], ymin[i], xmax[i], ymax[i], kp) >> 32);
     // the run's end: the first later position with another high half
     const int64_t lim = n - i < hmax + 1 ? n - i : hmax + 1;
-    for (int64_t b = kFixLong; b < lim; b += kHugeThreads) {
-      const int64_t p = i + b + t;
-      if (b + t < lim &&
-          (uint32_t)(morton64(xmin[p], ymin[p], xmax[p], ymax[p], kp) >> 32) != h0)
-        atomicMin(&s_len, (uint32_t)(b + t));
+    for (int64_t b = kFixLong; b < lim; b += 4 * kHugeThreads) {
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int64_t rel = b + kHugeThreads * u + t, p = i + rel;
+        if (rel < lim &&
+            (uint32_t)(morton64(xmin[p], ymin[p], xmax[p], ymax[p], kp) >> 32) != h0)
+          atomicMin(&s_len, (uint32_t)rel);
+      }
       __syncthreads();
       const bool found = s_len != 0xFFFFFFFFu;
       __syncthreads();
@@ -1327,112 +1532,44 @@ __global__ __launch_bounds__(kHugeThreads) void k_fixup_huge(
                                s_wcnt, s_wsum);
       continue;
     }
-    if (t < 256) {
-      s_hist[0][t] = 0;
-      s_hist[1][t] = 0;
+    // a stable split by the top 8 bits of the low half, then groups of whole
+    // digits of up to kHugeLds boxes sorted in LDS (within a group the top
+    // digits are already in order, and equal keys keep their input order)
+    load_keys(i, L, 24);
+    lsd_pass(i, L, ka, kb, 56, 0, false, true);
+    if (t == 0) {
+      uint32_t nc = 0, cs = 0, big = 0;
+      for (int d = 0; d < 256; d++) {
+        const uint32_t bs = s_bstart[d], be = d < 255 ? s_bstart[d + 1] : L;
+        if (be - bs > (uint32_t)kHugeLds) big = 1;
+        if (be - cs > (uint32_t)kHugeLds) {
+          s_chunk[nc++] = cs;
+          cs = bs;
+        }
+      }
+      s_chunk[nc++] = cs;
+      s_chunk[nc] = L;
+      s_nchunk = nc;
+      s_big = big;
     }
     __syncthreads();
-    for (uint32_t e = t; e < L; e += kHugeThreads) {
-      const int64_t p = i + e;
-      const uint32_t lo = (uint32_t)morton64(xmin[p], ymin[p], xmax[p], ymax[p], kp);
-      ka[p] = (uint64_t)lo << 32 | e;
-      atomicAdd(&s_hist[0][lo & 255u], 1u);
-    }
-    uint64_t* src = ka;
-    uint64_t* dst = kb;
-    for (int pass = 0; pass < 4; pass++) {
-      const int sh = 32 + 8 * pass;
-      __syncthreads();  // this pass's histogram is complete
-      // exclusive offsets of the digits (waves 0..3, 64 digits each)
-      uint32_t v = 0, inc = 0;
-      if (t < 256) {
-        v = s_hist[pass & 1][t];
-        inc = v;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-          const uint32_t u = __shfl_up(inc, d);
-          if (lane >= d) inc += u;
-        }
-        if (lane == 63) s_wsum[w] = inc;
-        s_hist[(pass + 1) & 1][t] = 0;
+    if (!s_big) {
+      for (uint32_t g = 0; g < s_nchunk; g++) {
+        const uint32_t cs = s_chunk[g], ce = s_chunk[g + 1];
+        lds_sort_keys<kHugeWaves>(kb + i + cs, ka + i + cs, (int)(ce - cs), s_net, s_wcnt,
+                                  s_wsum);
       }
-      __syncthreads();
-      if (t < 256) {
-        uint32_t off = 0;
-        for (int k = 0; k < w; k++) off += s_wsum[k];
-        s_base[t] = off + inc - v;
-      }
-      for (uint32_t e0 = 0; e0 < L; e0 += kHugeThreads * kHugeK) {
-        // wave w, item k, lane l: index e0 + w * 64 * kHugeK + 64 * k + l
-        const uint32_t eb = e0 + (uint32_t)(w * 64 * kHugeK + lane);
-        uint64_t x[kHugeK];
-        uint32_t rank[kHugeK];
-#pragma unroll
-        for (int k = 0; k < kHugeK; k++) {
-          const uint32_t e = eb + 64u * k;
-          x[k] = e < L ? src[i + e] : 0ull;
-        }
-#pragma unroll
-        for (int k = 0; k < 4; k++) s_wcnt[w][lane + 64 * k] = 0;
-#pragma unroll
-        for (int k = 0; k < kHugeK; k++) {
-          const bool valid = eb + 64u * k < L;
-          const uint32_t d = (uint32_t)(x[k] >> sh) & 255u;
-          uint64_t m = ballot(valid);
-#pragma unroll
-          for (int bit = 0; bit < 8; bit++) {
-            const uint64_t bb = ballot((d >> bit) & 1u);
-            m &= ((d >> bit) & 1u) ? bb : ~bb;
-          }
-          const uint32_t before = s_wcnt[w][d];
-          rank[k] = before + (uint32_t)popc64(m & lt);
-          if (valid && (m & lt) == 0) s_wcnt[w][d] = before + (uint32_t)popc64(m);
-        }
-        __syncthreads();
-        if (t < 256) {
-          uint32_t run = s_base[t];
-#pragma unroll
-          for (int k = 0; k < kHugeWaves; k++) {
-            const uint32_t cnt = s_wcnt[k][t];
-            s_wcnt[k][t] = run;
-            run += cnt;
-          }
-          s_base[t] = run;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < kHugeK; k++) {
-          if (eb + 64u * k < L) {
-            dst[i + s_wcnt[w][(uint32_t)(x[k] >> sh) & 255u] + rank[k]] = x[k];
-            if (pass < 3)
-              atomicAdd(&s_hist[(pass + 1) & 1][(uint32_t)(x[k] >> (sh + 8)) & 255u], 1u);
-          }
-        }
-        __syncthreads();  // s_wcnt is cleared by the next tile
-      }
-      uint64_t* tmp = src;
-      src = dst;
-      dst = tmp;
+      gather(i, L, ka);
+    } else {
+      // one digit holds more than kHugeLds boxes: four LSD passes over the
+      // whole run in the key buffers
+      load_keys(i, L, 0);
+      for (int pass = 0; pass < 4; pass++)
+        lsd_pass(i, L, pass & 1 ? kb : ka, pass & 1 ? ka : kb, 32 + 8 * pass, pass & 1, pass < 3,
+                 false);
+      gather(i, L, ka);
     }
-    // src == ka: the run's order; stage every box, then write back the moved
-    for (uint32_t e = t; e < L; e += kHugeThreads) {
-      const int64_t sp = i + (uint32_t)ka[i + e];
-      stage[i + e] = make_float4(xmin[sp], ymin[sp], xmax[sp], ymax[sp]);
-      stage_id[i + e] = (uint32_t)id[sp];
-    }
-    __syncthreads();
-    for (uint32_t e = t; e < L; e += kHugeThreads) {
-      const int64_t p = i + e;
-      if ((uint32_t)ka[p] != e) {
-        const float4 b = stage[p];
-        xmin[p] = b.x;
-        ymin[p] = b.y;
-        xmax[p] = b.z;
-        ymax[p] = b.w;
-        id[p] = (int)stage_id[p];
-      }
-    }
-    __syncthreads();  // s_claim / s_len are rewritten for the next run
+    __syncthreads();  // s_claim / s_len / s_chunk are rewritten for the next run
   }
 }
 
@@ -1720,17 +1857,26 @@ __global__ __launch_bounds__(256) void k_fixup_groups(
     const int64_t p0 = (i / G) * G;
     int64_t p1 = ((i + L - 1) / G + 1) * G;
     if (p1 > n) p1 = n;
-    for (int64_t base = p0; base < p1; base += kWave) {
-      const int64_t p = base + lane_id();
-      const bool has = p < p1;
-      float a = 0.f, b = 0.f, cc = 0.f, d = 0.f;
-      if (has) {
-        a = xmin[p];
-        b = ymin[p];
-        cc = xmax[p];
-        d = ymax[p];
+    // four chunks of 64 positions per round (the loads in flight together)
+    for (int64_t base = p0; base < p1; base += 4 * kWave) {
+      float a[4], b[4], cc[4], d[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int64_t p = base + kWave * u + lane_id();
+        a[u] = b[u] = cc[u] = d[u] = 0.f;
+        if (p < p1) {
+          a[u] = xmin[p];
+          b[u] = ymin[p];
+          cc[u] = xmax[p];
+          d[u] = ymax[p];
+        }
       }
-      leaf_parent<G>(p, n, a, b, cc, d, has, tr, l0start, r_sz, true);
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int64_t p = base + kWave * u + lane_id();
+        if (base + kWave * u < p1)
+          leaf_parent<G>(p, n, a[u], b[u], cc[u], d[u], p < p1, tr, l0start, r_sz, true);
+      }
     }
   }
 }

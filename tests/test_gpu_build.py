@@ -188,6 +188,25 @@ def test_morton64_equal_high_half_runs(case):
         check_against_oracle(bx, np.arange(n, dtype=np.int32), f, 1)
 
 
+@pytest.mark.parametrize("dups", [9000, 20000])
+def test_morton64_duplicate_heavy_run(dups):
+    # a run of equal high halves longer than 8192 whose low halves are mostly
+    # one value: after the split by the top digit of the low half one digit
+    # still holds more than 8192 boxes (the run then takes four LSD passes)
+    x, y, _, _ = _clustered_points([3000], seed=dups, background=50_000)
+    # the cluster's points are the ones sharing the most common cell
+    cell = (np.floor(x * 65536).astype(np.int64) << 20) + np.floor(y * 65536).astype(np.int64)
+    vals, cnt = np.unique(cell, return_counts=True)
+    j = np.nonzero(cell == vals[np.argmax(cnt)])[0][0]
+    x = np.concatenate([x, np.full(dups, x[j], np.float32)])
+    y = np.concatenate([y, np.full(dups, y[j], np.float32)])
+    p = np.random.default_rng(dups).permutation(x.size)
+    bx = (x[p], y[p], x[p].copy(), y[p].copy())
+    n = x.size
+    for f in (16, 10):
+        check_against_oracle(bx, np.arange(n, dtype=np.int32), f, 1)
+
+
 @pytest.mark.parametrize("fanout,key_mode", [(16, 1), (32, 1), (8, 0)])
 def test_skewed_hot_cells(fanout, key_mode):
     # the large/skewed workload's shape at a parity size: 70% of the boxes in
