@@ -260,6 +260,83 @@ def _at_p99(points, p99):
     return max(ok, key=lambda p: p["pairs_per_s"]) if ok else None
 
 
+def bench_skewed(args, ct, datagen):
+    """Bulk load of the large/skewed workload: 70% of the boxes in 1,000
+    Zipf(1.2)-weighted hot cells of side 1e-3, 20% points, 64-bit Morton keys
+    (long runs of equal high key halves).  Build only: windows in the hot
+    cells would return ~1e4..1e7 pairs each at this size."""
+    n = args.boxes if args.boxes_set else 1_000_000_000
+    F = args.fanout or 16
+    t0 = time.time()
+
+    def progress(done):
+        print(f"[bench] generated {done:,} of {n:,} boxes", file=sys.stderr, flush=True)
+
+    boxes = datagen.skewed_boxes_large(n, seed=9, progress=progress)
+    pristine = ct.DeviceBoxes.from_host(*boxes, np.arange(n, dtype=np.int32))
+    del boxes
+    work = ct.DeviceBoxes(n)
+    tree = ct.alloc_tree(n, F)
+    print(f"[bench] {n:,} skewed boxes resident in {time.time() - t0:.1f} s", file=sys.stderr,
+          flush=True)
+    ev = [ct.Event() for _ in range(2)]
+
+    def step():
+        for f in ("xmin", "ymin", "xmax", "ymax", "id"):
+            ct._check(ct.lib.curtree_memcpy_d2d(getattr(work.s, f), getattr(pristine.s, f),
+                                                4 * n, None))
+        ct.synchronize()
+        ev[0].record()
+        ct.build_into(work, tree, ct.KEY_MORTON64)
+        ev[1].record()
+        ct.synchronize()
+        return ev[0].elapsed_ms(ev[1])
+
+    for _ in range(args.warmup):
+        step()
+    ms = []
+    for i in range(args.steps):
+        ms.append(step())
+        print(f"[bench] step {i}: build {ms[-1]:.2f} ms", file=sys.stderr, flush=True)
+    ct.Prof.enable(True)
+    ct.Prof.reset()
+    for _ in range(args.prof_steps):
+        step()
+    ct.Prof.enable(False)
+    kernels = {}
+    for name in ("k_build_init", "k_extent", "k_hist", "k_sort_plan", "k_sweep_kv", "k_final",
+                 "k_run_len", "k_fixup_short", "k_fixup_long", "k_fixup_huge", "k_fixup_groups",
+                 "k_fallback", "k_level"):
+        tot, cnt = ct.Prof.get(name)
+        if cnt:
+            kernels[name] = {"avg_ms": tot / cnt, "ms_per_step": tot / max(args.prof_steps, 1)}
+    ms = np.array(ms)
+    out = {
+        "metric": "bulk-load throughput (large/skewed workload)",
+        "value": n / (ms.mean() * 1e-3),
+        "unit": "boxes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": float(ms.mean()),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"bulk load: {n:,} FP32 boxes, 70% in 1,000 Zipf(1.2) hot cells "
+                               f"of side 1e-3, 20% points, fanout {F}, 64-bit Morton keys",
+                   "boxes": n, "fanout": F, "key_bits": 64},
+        "build": {"ms": float(ms.mean()), "ms_p50": float(np.percentile(ms, 50)),
+                  "ms_max": float(ms.max())},
+        "kernels": kernels,
+    }
+    tree.free()
+    work.free()
+    pristine.free()
+    print(json.dumps(out), flush=True)
+
+
 def bench_serve(args, ct, datagen):
     scale = args.serve_scale
     sizes = [int(x * scale) for x in (1e7, 2e7, 4e7, 6e7, 8e7, 1.2e8, 1.6e8, 2e8)]
@@ -369,7 +446,8 @@ def main():
                     help="untimed steps (after the warm-up) with per-kernel events for the "
                          "kernel table")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--mode", choices=["plain", "batched", "serve"], default="plain")
+    ap.add_argument("--mode", choices=["plain", "batched", "serve", "skewed"], default="plain",
+                    help="skewed: bulk load of the large/skewed workload only (1e9 boxes)")
     ap.add_argument("--boxes", type=int, default=None)
     ap.add_argument("--windows", type=int, default=None)
     ap.add_argument("--budget-pairs", type=int, default=256 << 20,
@@ -438,6 +516,8 @@ def main():
         return bench_batched(args, ct, datagen)
     if args.mode == "serve":
         return bench_serve(args, ct, datagen)
+    if args.mode == "skewed":
+        return bench_skewed(args, ct, datagen)
     if not args.boxes_set:
         args.boxes = 100_000_000
     if not args.windows_set:

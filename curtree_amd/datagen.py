@@ -14,6 +14,7 @@ center -/+ fl(size/2).  Sizes and distributions:
                      (sigma 1e-3); widths/heights U[0, wmax]
   skewed_boxes       frac_hot in hot cells of side 1e-3 chosen with
                      Zipf(1.2) weights, rest uniform; a share are points
+  skewed_boxes_large the same distribution, generated in chunks (1e9 boxes)
   roadlike_boxes     line-segment MBRs: uniform start, Pareto length,
                      random angle
 """
@@ -123,6 +124,41 @@ def skewed_boxes(n, seed=9, cells=1000, frac_hot=0.7, cell=1e-3, zipf=1.2, point
     w[pt] = 0
     h[pt] = 0
     return _boxes(cx, cy, w, h)
+
+
+def skewed_boxes_large(n, seed=9, chunk=100_000_000, progress=None, cells=1000, frac_hot=0.7,
+                       cell=1e-3, zipf=1.2, points=0.2, wmax=1e-4):
+    """skewed_boxes' distribution (same hot cells for every chunk) generated in
+    chunks of `chunk` boxes, each with its own stream, so that 1e9 boxes fit
+    in host memory with few temporaries; `progress(done)` after each chunk.
+    The values differ from skewed_boxes(n, seed)."""
+    r0 = _rng(seed)
+    corner = r0.random((cells, 2)) * (1 - cell)
+    wts = 1.0 / np.arange(1, cells + 1) ** zipf
+    wts /= wts.sum()
+    out = tuple(np.empty(n, dtype=np.float32) for _ in range(4))
+    for k, s in enumerate(range(0, n, chunk)):
+        m = min(chunk, n - s)
+        r = _rng(seed * 1000 + k + 1)
+        nh = int(m * frac_hot)
+        cx = np.empty(m, dtype=np.float32)
+        cy = np.empty(m, dtype=np.float32)
+        c = r.choice(cells, nh, p=wts)
+        cx[:nh] = corner[c, 0] + cell * r.random(nh)
+        cy[:nh] = corner[c, 1] + cell * r.random(nh)
+        cx[nh:] = r.random(m - nh, dtype=np.float32)
+        cy[nh:] = r.random(m - nh, dtype=np.float32)
+        perm = r.permutation(m)
+        w = r.random(m, dtype=np.float32) * np.float32(wmax)
+        h = r.random(m, dtype=np.float32) * np.float32(wmax)
+        pt = r.random(m) < points
+        w[pt] = 0
+        h[pt] = 0
+        for o, v in zip(out, _boxes(cx[perm], cy[perm], w, h)):
+            o[s:s + m] = v
+        if progress:
+            progress(s + m)
+    return out
 
 
 def roadlike_boxes(n, seed=7, lmin=1e-5, alpha=1.5):

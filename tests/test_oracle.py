@@ -287,3 +287,24 @@ def test_bfs_edge_cases():
     # zero queries
     fid, tid, c = orc.bfs(t, sb, tuple(np.zeros(0, np.float32) for _ in range(4)))
     assert fid.size == 0 and c == (0, 0, 0)
+
+
+def test_skewed_large_generator():
+    # chunked generation: deterministic, independent of nothing but (n, seed,
+    # chunk), 70% of the boxes inside the 1,000 hot cells of side 1e-3,
+    # about 20% points, every box inside [0, 1]^2 up to its half-size
+    a = datagen.skewed_boxes_large(30_000, seed=3, chunk=7_000)
+    b = datagen.skewed_boxes_large(30_000, seed=3, chunk=7_000)
+    for u, v in zip(a, b):
+        assert u.dtype == np.float32 and np.array_equal(u, v)
+    xmin, ymin, xmax, ymax = (x.astype(np.float64) for x in a)
+    assert np.all(xmax >= xmin) and np.all(ymax >= ymin)
+    pts = np.mean((xmax == xmin) & (ymax == ymin))
+    assert 0.17 < pts < 0.23
+    cx, cy = (xmin + xmax) / 2, (ymin + ymax) / 2
+    assert cx.min() > -1e-4 and cx.max() < 1 + 1e-4 and cy.min() > -1e-4 and cy.max() < 1 + 1e-4
+    # hot-cell occupancy: the 1,000 most occupied cells of side 1e-3 (on a
+    # 1e-3 grid a hot cell spans up to four) hold well over half the boxes
+    g = np.floor(cx * 1000).astype(np.int64) * 1001 + np.floor(cy * 1000).astype(np.int64)
+    _, cnt = np.unique(g, return_counts=True)
+    assert np.sort(cnt)[::-1][:4000].sum() / cx.size > 0.6
