@@ -878,12 +878,13 @@ __global__ __launch_bounds__(kSortThreads) void k_sweep_kv(
 // never change during the launch.  Every other run is listed by the chunk
 // that holds its start; k_run_len measures the listed runs (reading only),
 // k_fixup_short sorts those of up to 64 boxes (one wave each), k_fixup_long
-// the longer ones up to kFixLong (one block each, bitonic in LDS),
-// k_fixup_huge the longer ones still (one block each, LSD in the free key
-// buffers), and k_fixup_groups recomputes the leaf-parent nodes over them.  A
-// run too long for one block, or a full list, sets `fallback`: the full 8-pass
-// LSD sort of 64-bit keys then runs from the order reached so far (in which
-// equal full keys are still in input order).
+// those of up to kWaveSortMax (one wave each) and kFixLong (one block each),
+// with an LSD sort in registers and LDS, k_fixup_huge the longer ones still
+// (one block each, in LDS after a split by the top digit), and
+// k_fixup_groups recomputes the leaf-parent nodes over them.  A run too long
+// for one block, or a full list, sets `fallback`: the full 8-pass LSD sort of
+// 64-bit keys then runs from the order reached so far (in which equal full
+// keys are still in input order).
 constexpr int kFixLong = 4096;
 constexpr int kWaveSortMax = 1024;   // runs sorted by one wave (16 items per lane)
 constexpr int kFixSlots = 4096;      // blocks of the fixup launches (they loop over the list)
@@ -892,7 +893,7 @@ constexpr int kHugeThreads = 512;    // k_fixup_huge: threads per block
 constexpr int kHugeWaves = kHugeThreads / 64;
 constexpr int kFixHugeSlots = 512;   // blocks of k_fixup_huge (two per CU)
 constexpr int kHugeK = 8;            // k_fixup_huge: positions per thread in one tile
-constexpr int kHugeLds = kHugeWaves * 64 * 16;  // k_fixup_huge: longest run sorted in LDS
+constexpr int kHugeLds = kHugeWaves * 64 * 16;  // k_fixup_huge: longest run sorted whole in LDS
 #ifndef CT_FINAL_GRID
 #define CT_FINAL_GRID 4096
 #endif
