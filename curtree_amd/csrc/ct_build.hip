@@ -1089,8 +1089,8 @@ __device__ __forceinline__ void lds_lsd(uint64_t (&x)[kRunPer], int kUse, int wb
 
 // Sorts the run of L <= NW * 64 * kRunPer boxes at position i by (low half,
 // index in the run), in place, with a block of NW waves (lds_lsd over the
-// keys computed from the boxes); the moved boxes go through the run's span of
-// the staging records.  Every thread of the block calls it; it ends with a
+// keys computed from the boxes); the moved boxes come from a copy of the run
+// in its span of the staging records.  Every thread of the block calls it; it ends with a
 // barrier.
 template <int NW>
 __device__ __forceinline__ void lds_sort_run(int64_t i, int L, const KeyParams& kp,
@@ -1102,39 +1102,32 @@ __device__ __forceinline__ void lds_sort_run(int64_t i, int L, const KeyParams& 
   const int kUse = (L + NW * 64 - 1) / (NW * 64);  // items per lane
   const int wbase = (threadIdx.x >> 6) * 64 * kUse + lane_id();
   uint64_t x[kRunPer];
+  // keys from the boxes, and the run copied (coalesced) into its span of the
+  // staging records, from which the moved positions read their source
 #pragma unroll
   for (int k = 0; k < kRunPer; k++) {
     x[k] = ~0ull;
     const int e = wbase + 64 * k;
     if (k < kUse && e < L) {
       const int64_t p = i + e;
-      x[k] = ((uint64_t)(uint32_t)morton64(xmin[p], ymin[p], xmax[p], ymax[p], kp) << 32) |
-             (uint32_t)e;
+      const float4 b = make_float4(xmin[p], ymin[p], xmax[p], ymax[p]);
+      stage[p] = b;
+      stage_id[p] = (uint32_t)id[p];
+      x[k] = ((uint64_t)(uint32_t)morton64(b.x, b.y, b.z, b.w, kp) << 32) | (uint32_t)e;
     }
   }
-  lds_lsd<NW>(x, kUse, wbase, s_net, s_cnt, s_wsum);
-  // every moved box staged before any is written back
+  lds_lsd<NW>(x, kUse, wbase, s_net, s_cnt, s_wsum);  // (its barriers order the copy)
 #pragma unroll
   for (int k = 0; k < kRunPer; k++) {
     const int e = wbase + 64 * k;
     if (k < kUse && e < L && (uint32_t)x[k] != (uint32_t)e) {
-      const int64_t sp = i + (uint32_t)x[k];
-      stage[i + e] = make_float4(xmin[sp], ymin[sp], xmax[sp], ymax[sp]);
-      stage_id[i + e] = (uint32_t)id[sp];
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < kRunPer; k++) {
-    const int e = wbase + 64 * k;
-    if (k < kUse && e < L && (uint32_t)x[k] != (uint32_t)e) {
-      const int64_t p = i + e;
-      const float4 b = stage[p];
+      const int64_t p = i + e, sp = i + (uint32_t)x[k];
+      const float4 b = stage[sp];
       xmin[p] = b.x;
       ymin[p] = b.y;
       xmax[p] = b.z;
       ymax[p] = b.w;
-      id[p] = (int)stage_id[p];
+      id[p] = (int)stage_id[sp];
     }
   }
   __syncthreads();  // s_net and s_cnt are reused by the next run
@@ -1180,14 +1173,18 @@ __device__ __forceinline__ void wave_sort_run(int64_t i, int L, const KeyParams&
   const uint64_t lt = lanemask_lt();
   const int kUse = (L + 63) >> 6;
   uint64_t x[kRunPer];
+  // keys from the boxes, and the run copied (coalesced) into its span of the
+  // staging records, from which the moved positions read their source
 #pragma unroll
   for (int k = 0; k < kRunPer; k++) {
     x[k] = ~0ull;
     const int e = 64 * k + lane;
     if (k < kUse && e < L) {
       const int64_t p = i + e;
-      x[k] = ((uint64_t)(uint32_t)morton64(xmin[p], ymin[p], xmax[p], ymax[p], kp) << 32) |
-             (uint32_t)e;
+      const float4 b = make_float4(xmin[p], ymin[p], xmax[p], ymax[p]);
+      stage[p] = b;
+      stage_id[p] = (uint32_t)id[p];
+      x[k] = ((uint64_t)(uint32_t)morton64(b.x, b.y, b.z, b.w, kp) << 32) | (uint32_t)e;
     }
   }
   for (int pass = 0; pass < 4; pass++) {
@@ -1238,31 +1235,21 @@ __device__ __forceinline__ void wave_sort_run(int64_t i, int L, const KeyParams&
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
   }
-  // the moved boxes through the run's span of the staging records: a lane
-  // stages and writes back the same positions, and every box is read (its
-  // staging store waits for it) before the first box store
-#pragma unroll
-  for (int k = 0; k < kRunPer; k++) {
-    const int e = 64 * k + lane;
-    if (k < kUse && e < L && (uint32_t)x[k] != (uint32_t)e) {
-      const int64_t sp = i + (uint32_t)x[k];
-      stage[i + e] = make_float4(xmin[sp], ymin[sp], xmax[sp], ymax[sp]);
-      stage_id[i + e] = (uint32_t)id[sp];
-    }
-  }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  // the copy's stores complete before the moved positions read other lanes'
+  // records (a fence at work-group scope waits for them)
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
   __builtin_amdgcn_wave_barrier();
 #pragma unroll
   for (int k = 0; k < kRunPer; k++) {
     const int e = 64 * k + lane;
     if (k < kUse && e < L && (uint32_t)x[k] != (uint32_t)e) {
-      const int64_t p = i + e;
-      const float4 b = stage[p];
+      const int64_t p = i + e, sp = i + (uint32_t)x[k];
+      const float4 b = stage[sp];
       xmin[p] = b.x;
       ymin[p] = b.y;
       xmax[p] = b.z;
       ymax[p] = b.w;
-      id[p] = (int)stage_id[p];
+      id[p] = (int)stage_id[sp];
     }
   }
 }
@@ -1435,26 +1422,20 @@ __global__ __launch_bounds__(kHugeThreads) __attribute__((amdgpu_waves_per_eu(4)
       __syncthreads();  // s_wcnt is cleared by the next tile
     }
   };
-  // the run's boxes put in the order of `k` (the index halves) through the
-  // run's span of the staging records
+  // the run's boxes put in the order of `k` (the index halves): a coalesced
+  // copy of the run into its span of the staging records, then each position
+  // reads its source record (two random reads instead of one per array)
   auto gather = [&](int64_t i, uint32_t L, const uint64_t* k) {
-    // four positions per thread and round (loads in flight together)
     for (uint32_t e0 = t; e0 < L; e0 += 4 * kHugeThreads) {
-      uint32_t src[4];
-#pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const uint32_t e = e0 + kHugeThreads * u;
-        src[u] = e < L ? (uint32_t)k[i + e] : 0u;
-      }
       float4 b[4];
       int v[4];
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         const uint32_t e = e0 + kHugeThreads * u;
         if (e < L) {
-          const int64_t sp = i + src[u];
-          b[u] = make_float4(xmin[sp], ymin[sp], xmax[sp], ymax[sp]);
-          v[u] = id[sp];
+          const int64_t p = i + e;
+          b[u] = make_float4(xmin[p], ymin[p], xmax[p], ymax[p]);
+          v[u] = id[p];
         }
       }
 #pragma unroll
@@ -1468,22 +1449,26 @@ __global__ __launch_bounds__(kHugeThreads) __attribute__((amdgpu_waves_per_eu(4)
     }
     __syncthreads();
     for (uint32_t e0 = t; e0 < L; e0 += 4 * kHugeThreads) {
-      float4 b[4];
-      uint32_t v[4];
-      bool mv[4];
+      uint32_t src[4];
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         const uint32_t e = e0 + kHugeThreads * u;
-        mv[u] = e < L && (uint32_t)k[i + e] != e;
-        if (mv[u]) {
-          b[u] = stage[i + e];
-          v[u] = stage_id[i + e];
+        src[u] = e < L ? (uint32_t)k[i + e] : e;
+      }
+      float4 b[4];
+      uint32_t v[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        if (src[u] != e0 + kHugeThreads * u) {
+          b[u] = stage[i + src[u]];
+          v[u] = stage_id[i + src[u]];
         }
       }
 #pragma unroll
       for (int u = 0; u < 4; u++) {
-        if (mv[u]) {
-          const int64_t p = i + e0 + kHugeThreads * u;
+        const uint32_t e = e0 + kHugeThreads * u;
+        if (src[u] != e) {
+          const int64_t p = i + e;
           xmin[p] = b[u].x;
           ymin[p] = b[u].y;
           xmax[p] = b[u].z;
