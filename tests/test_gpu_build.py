@@ -244,3 +244,38 @@ def test_skewed_large_properties():
     o = np.argsort(sb[4])
     for i in range(4):
         assert np.array_equal(sb[i][o], bx[i])
+
+
+def test_over_a_million_listed_runs():
+    # 1.1M runs of 65 equal high halves (each crosses a chunk of 64, so every
+    # run is listed and sorted by one wave): the run-length pass takes more
+    # than one round over the list (1,048,576 runs per round); keys sorted,
+    # ties in input order, leaf-parents the min/max of their boxes
+    r = np.random.default_rng(77)
+    m, L = 1_100_000, 65
+    cells = np.unique(r.integers(0, 65536, (m + m // 8, 2)), axis=0)[:m]
+    r.shuffle(cells)
+    cx = np.repeat(cells[:, 0], L) + 0.5 + r.uniform(-0.45, 0.45, m * L)
+    cy = np.repeat(cells[:, 1], L) + 0.5 + r.uniform(-0.45, 0.45, m * L)
+    x = np.concatenate([cx / 65536.0, [0.0, 1.0]]).astype(np.float32)
+    y = np.concatenate([cy / 65536.0, [0.0, 1.0]]).astype(np.float32)
+    p = r.permutation(x.size)
+    bx = (x[p], y[p], x[p].copy(), y[p].copy())
+    n = x.size
+    ids = np.arange(n, dtype=np.int32)
+    sb, t, sz = gpu_build(bx, ids, 16, 1)
+    lv, rs = orc.level_plan(n, 16)
+    assert sz == rs
+    k = orc.keys_morton64(*sb[:4], ext=orc.centroid_extent(*bx))
+    assert np.all(k[1:] >= k[:-1])
+    same = k[1:] == k[:-1]
+    assert np.all(sb[4][1:][same] > sb[4][:-1][same])
+    hi = k >> np.uint64(32)
+    assert np.count_nonzero(hi[1:] != hi[:-1]) >= m - 1  # the runs are there
+    start = rs - lv[0]
+    g = np.arange(0, n, 16)
+    assert np.array_equal(t["xmin"][start:], np.minimum.reduceat(sb[0], g))
+    assert np.array_equal(t["ymax"][start:], np.maximum.reduceat(sb[3], g))
+    o = np.argsort(sb[4])
+    for i in range(4):
+        assert np.array_equal(sb[i][o], bx[i])
