@@ -263,9 +263,11 @@ def _at_p99(points, p99):
 def bench_skewed(args, ct, datagen):
     """Bulk load of the large/skewed workload: 70% of the boxes in 1,000
     Zipf(1.2)-weighted hot cells of side 1e-3, 20% points, 64-bit Morton keys
-    (long runs of equal high key halves).  Build only: windows in the hot
-    cells would return ~1e4..1e7 pairs each at this size."""
+    (long runs of equal high key halves); then 1e7 queries, half points and
+    half windows of side log-uniform [1e-5, 1e-3], centers uniform, against
+    the built tree (`query` in the JSON line; `value` is the bulk load)."""
     n = args.boxes if args.boxes_set else 1_000_000_000
+    nq = args.windows if args.windows_set else 10_000_000
     F = args.fanout or 16
     t0 = time.time()
 
@@ -311,6 +313,30 @@ def bench_skewed(args, ct, datagen):
         if cnt:
             kernels[name] = {"avg_ms": tot / cnt, "ms_per_step": tot / max(args.prof_steps, 1)}
     ms = np.array(ms)
+    # the queries: count pass to size the output, then timed query steps
+    # into a caller buffer (the tree built by the last step)
+    q = ct.DeviceBoxes.from_host(*datagen.mixed_queries(nq, seed=10))
+    H = ct.query_count(tree, work, q)
+    print(f"[bench] {H:,} result pairs per query batch", file=sys.stderr, flush=True)
+    fbuf, tbuf = ctypes.c_void_p(), ctypes.c_void_p()
+    ct._check(ct.lib.curtree_malloc(ctypes.byref(fbuf), 4 * max(H, 1)))
+    ct._check(ct.lib.curtree_malloc(ctypes.byref(tbuf), 4 * max(H, 1)))
+    qms = []
+    for i in range(args.warmup + args.steps):
+        ct.synchronize()
+        ev[0].record()
+        got = ct.query_into(tree, work, q, fbuf.value, tbuf.value, H)
+        ev[1].record()
+        ct.synchronize()
+        assert got == H, (got, H)
+        if i >= args.warmup:
+            qms.append(ev[0].elapsed_ms(ev[1]))
+            print(f"[bench] query step {i - args.warmup}: {qms[-1]:.2f} ms", file=sys.stderr,
+                  flush=True)
+    ct.lib.curtree_free(fbuf)
+    ct.lib.curtree_free(tbuf)
+    q.free()
+    qms = np.array(qms)
     out = {
         "metric": "bulk-load throughput (large/skewed workload)",
         "value": n / (ms.mean() * 1e-3),
@@ -325,10 +351,15 @@ def bench_skewed(args, ct, datagen):
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"bulk load: {n:,} FP32 boxes, 70% in 1,000 Zipf(1.2) hot cells "
-                               f"of side 1e-3, 20% points, fanout {F}, 64-bit Morton keys",
-                   "boxes": n, "fanout": F, "key_bits": 64},
+                               f"of side 1e-3, 20% points, fanout {F}, 64-bit Morton keys; "
+                               f"{nq:,} queries, half points, half windows of side log-uniform "
+                               f"[1e-5,1e-3], centers uniform",
+                   "boxes": n, "queries": nq, "fanout": F, "key_bits": 64},
         "build": {"ms": float(ms.mean()), "ms_p50": float(np.percentile(ms, 50)),
                   "ms_max": float(ms.max())},
+        "query": {"ms": float(qms.mean()), "pairs": int(H),
+                  "pairs_per_s": H / (qms.mean() * 1e-3),
+                  "queries_per_s": nq / (qms.mean() * 1e-3)},
         "kernels": kernels,
     }
     tree.free()

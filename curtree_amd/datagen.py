@@ -15,6 +15,8 @@ center -/+ fl(size/2).  Sizes and distributions:
   skewed_boxes       frac_hot in hot cells of side 1e-3 chosen with
                      Zipf(1.2) weights, rest uniform; a share are points
   skewed_boxes_large the same distribution, generated in chunks (1e9 boxes)
+  mixed_queries      centers U[0,1)^2; half points, half squares of side
+                     log-uniform [lo, hi]
   roadlike_boxes     line-segment MBRs: uniform start, Pareto length,
                      random angle
 """
@@ -124,6 +126,17 @@ def skewed_boxes(n, seed=9, cells=1000, frac_hot=0.7, cell=1e-3, zipf=1.2, point
     w[pt] = 0
     h[pt] = 0
     return _boxes(cx, cy, w, h)
+
+
+def mixed_queries(q, seed=10, lo=1e-5, hi=1e-3, points=0.5):
+    """Query boxes: centers U[0,1)^2; a share `points` are points (side 0),
+    the others squares of side log-uniform [lo, hi]."""
+    r = _rng(seed)
+    cx = r.random(q, dtype=np.float32)
+    cy = r.random(q, dtype=np.float32)
+    s = np.exp(r.uniform(np.log(lo), np.log(hi), q)).astype(np.float32)
+    s[r.random(q) < points] = 0
+    return _boxes(cx, cy, s, s)
 
 
 def skewed_boxes_large(n, seed=9, chunk=100_000_000, progress=None, cells=1000, frac_hot=0.7,
