@@ -30,6 +30,12 @@ Other modes (same JSON contract, one line):
                   at the plain entry point's p99 latency (at its highest
                   sustained rate); the comparison with the unbatched server is
                   reported beside it.
+  --mode skewed   1,000,000,000 boxes, 70% in 1,000 Zipf(1.2)-weighted hot
+                  cells of side 1e-3, 20% points, fanout 16, 64-bit Morton
+                  keys (long runs of equal high key halves); value = bulk-load
+                  boxes/s; then 10,000,000 queries (half points, half windows
+                  of side log-uniform [1e-5, 1e-3], centers uniform) against
+                  the tree, in `query`.
 """
 import argparse
 import ctypes
