@@ -1258,6 +1258,7 @@ __device__ __forceinline__ void wave_sort_run(int64_t i, int L, const KeyParams&
 // listed runs of 65..kWaveSortMax boxes, one wave each.
 static_assert(kFixLong == 4 * 64 * kRunPer, "k_fixup_long: one item slot per box");
 static_assert(kWaveSortMax == 64 * kRunPer, "wave_sort_run: one item slot per box");
+static_assert(kHugeLds == kHugeWaves * 64 * kRunPer, "k_fixup_huge: one item slot per box");
 __global__ __launch_bounds__(256) void k_fixup_long(int64_t n, BuildCtrl* c,
                                                     float* __restrict__ xmin,
                                                     float* __restrict__ ymin,
