@@ -175,7 +175,7 @@ LEVELS = {
 }
 
 
-def test_level_plan_closed_form_and_survey_sizes():
+def test_level_plan_closed_form_and_workload_sizes():
     assert orc.level_plan(1_000_000, 16) == LEVELS[(1_000_000, 16)]
     assert orc.level_plan(100_000_000, 16)[1] == 6_666_669
     assert orc.level_plan(100_000_000, 32)[1] == 3_225_809
