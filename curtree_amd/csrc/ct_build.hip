@@ -2122,6 +2122,16 @@ int build_f32(Context& ctx, curtree_bbox_f32* box, curtree_rtree_f32* rt, int ke
   const size_t step_words = (size_t)std::max(ntiles, ntiles_lsd) * 256;
   const size_t status_words = (size_t)nsteps * step_words;
   Workspace& ws = ctx.ws;
+  const uint32_t runs_cap = m64 ? (uint32_t)(2 * ((n + kWave - 1) / kWave) + 2) : 1;
+  // a buffer that grows is freed first: the previous build (possibly on
+  // another stream) must be done with it
+  if (ctx.build_done &&
+      (!ws.fits(WsBuildKeys0, n * 8) || !ws.fits(WsBuildKeys1, n * 8) ||
+       !ws.fits(WsBuildAos, n * 32) || !ws.fits(WsBuildVals0, n * 4) ||
+       !ws.fits(WsBuildVals1, n * 4) || !ws.fits(WsBuildStatus, status_words * 4) ||
+       !ws.fits(WsBuildRuns, (size_t)runs_cap * 16)) &&
+      hipEventSynchronize(ctx.build_done) != hipSuccess)
+    return fail(set_error(CURTREE_EHIP, "build: cannot wait for the previous build"));
   // payload sort: (key32, id) and the box halves, double-buffered (48 B/box);
   // the fallback reuses them for 64-bit keys and 32-B staging records
   if ((rc = ws.reserve(WsBuildKeys0, n * 8)) || (rc = ws.reserve(WsBuildKeys1, n * 8)) ||
@@ -2132,7 +2142,6 @@ int build_f32(Context& ctx, curtree_bbox_f32* box, curtree_rtree_f32* rt, int ke
     return fail(rc);
   // listed runs of equal high halves (64-bit keys): start, length and the
   // lists of the long ones; at most two per chunk of 64 positions
-  const uint32_t runs_cap = m64 ? (uint32_t)(2 * ((n + kWave - 1) / kWave) + 2) : 1;
   if ((rc = ws.reserve(WsBuildRuns, (size_t)runs_cap * 16))) return fail(rc);
   BuildCtrl* c = (BuildCtrl*)ws.ptr(WsBuildCtrl);
   SortBufs sb;
@@ -2154,6 +2163,21 @@ int build_f32(Context& ctx, curtree_bbox_f32* box, curtree_rtree_f32* rt, int ke
   sb.runs = (uint32_t*)ws.ptr(WsBuildRuns);
   sb.runs_cap = runs_cap;
 
+  // every build shares the work buffers: a build on another stream waits for
+  // the previous build's kernels (asynchronous builds may overlap on the host)
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cap) != hipSuccess) {
+    (void)hipGetLastError();
+    cap = hipStreamCaptureStatusNone;
+  }
+  const bool order = cap == hipStreamCaptureStatusNone;
+  if (order && !ctx.build_done &&
+      hipEventCreateWithFlags(&ctx.build_done, hipEventDisableTiming) != hipSuccess) {
+    ctx.build_done = nullptr;
+    return fail(set_error(CURTREE_EHIP, "build: cannot create an event"));
+  }
+  if (order && hipStreamWaitEvent(s, ctx.build_done, 0) != hipSuccess)
+    return fail(set_error(CURTREE_EHIP, "build: cannot order after the previous build"));
   Prof& pf = ctx.prof;
   pf.mark(s, "k_build_init");
   hipLaunchKernelGGL(k_build_init, dim3(1), dim3(256), 0, s, c);
@@ -2207,6 +2231,7 @@ int build_f32(Context& ctx, curtree_bbox_f32* box, curtree_rtree_f32* rt, int ke
     child = pstart;
   }
   pf.mark(s, nullptr);
+  if (order) (void)hipEventRecord(ctx.build_done, s);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess)
     return fail(set_error(CURTREE_EHIP, "build: launch failed: %s", hipGetErrorString(e)));

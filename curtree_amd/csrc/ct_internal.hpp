@@ -78,6 +78,7 @@ struct Workspace {
   void* p[WsSlots] = {};
   size_t cap[WsSlots] = {};
   int reserve(WsSlot s, size_t bytes);  // keeps contents only if no growth
+  bool fits(WsSlot s, size_t bytes) const { return cap[s] >= (bytes ? bytes : 256); }
   void* ptr(WsSlot s) const { return p[s]; }
   void release();
 };
@@ -198,6 +199,9 @@ struct Context {
   QEngine qe;            // engine of the plain (synchronous) query entry points
   BatchState batch;
   int opt_contain = -1;  // containment shortcut: -1 auto, 0 off, 1 on
+  // recorded after every build's kernels: the next build, on any stream,
+  // waits for it before it touches the shared build buffers
+  hipEvent_t build_done = nullptr;
 };
 
 Context& context();

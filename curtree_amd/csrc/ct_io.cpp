@@ -54,6 +54,33 @@ int read_dev(FILE* f, void* dev, size_t bytes, Pinned& buf) {
   return CURTREE_OK;
 }
 
+// Every node's children must lie inside the tree: node children
+// [pos, pos+len) below r_sz, or box children [pos-r_sz, pos-r_sz+len) inside
+// the n boxes.  The query never follows a range outside them, but a file that
+// holds one is corrupt and is refused.
+int check_file_layout(FILE* f, int64_t n, int64_t r_sz) {
+  const long off = (long)(sizeof(FileHeader) + (size_t)n * 20 + (size_t)r_sz * 16);
+  const long want = off + (long)r_sz * 8;
+  if (fseek(f, 0, SEEK_END) != 0 || ftell(f) != want)
+    return set_error(CURTREE_EINVAL, "tree_load: the file size does not match its header");
+  std::vector<int32_t> pos((size_t)r_sz), len((size_t)r_sz);
+  if (fseek(f, off, SEEK_SET) != 0 || fread(pos.data(), 4, (size_t)r_sz, f) != (size_t)r_sz ||
+      fread(len.data(), 4, (size_t)r_sz, f) != (size_t)r_sz)
+    return set_error(CURTREE_EINVAL, "tree_load: short file");
+  for (int64_t j = 0; j < r_sz; j++) {
+    const int64_t p = pos[(size_t)j], l = len[(size_t)j];
+    const bool nodes = p >= 0 && l >= 0 && p + l <= r_sz;
+    const bool boxes = p >= r_sz && l >= 0 && p - r_sz + l <= n;
+    if (!nodes && !boxes)
+      return set_error(CURTREE_EINVAL,
+                       "tree_load: node %lld points outside the tree (pos %lld, len %lld)",
+                       (long long)j, (long long)p, (long long)l);
+  }
+  if (fseek(f, (long)sizeof(FileHeader), SEEK_SET) != 0)
+    return set_error(CURTREE_EINVAL, "tree_load: seek failed");
+  return CURTREE_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -125,6 +152,8 @@ CURTREE_API int curtree_tree_load(const char* path, curtree_rtree_f32* rt,
     if (h.n_boxes < 0 || h.n_boxes > INT32_MAX || h.fanout < 2 || lp.r_sz != h.r_sz)
       rc = set_error(CURTREE_EINVAL, "tree_load: inconsistent header");
   }
+  // the node ranges are checked on the host before any device memory is touched
+  if (!rc) rc = check_file_layout(f, h.n_boxes, h.r_sz);
   if (rc) {
     fclose(f);
     return rc;

@@ -57,7 +57,12 @@ int self_join_f32(Context& ctx, const curtree_rtree_f32* rt, const curtree_bbox_
     const unsigned g2 = (unsigned)std::min<int64_t>((qo.n + 255) / 256, 1 << 16);
     hipLaunchKernelGGL(k_map_query_ids, dim3(g2), dim3(256), 0, s, qo.n, qo.out_tid, rbox->id);
   }
-  CT_HIP(hipGetLastError());
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    if (qo.out_fid) (void)hipFree(qo.out_fid);
+    if (qo.out_tid) (void)hipFree(qo.out_tid);
+    return set_error(CURTREE_EHIP, "self_join: launch failed: %s", hipGetErrorString(e));
+  }
   out->sz = qo.n;
   out->fid = qo.out_fid;
   out->tid = qo.out_tid;

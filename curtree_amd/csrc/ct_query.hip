@@ -186,7 +186,7 @@ __global__ __launch_bounds__(kQThreads) void k_expand(int slot, QTree tr, QBoxes
             p = tr.pos[node];
             L = tr.len[node];
           }
-          if (c_ < L && p + L <= r_sz) {  // children must be nodes
+          if (p >= 0 && c_ < L && p + L <= r_sz) {  // children must be nodes
             const uint32_t child = (uint32_t)(p + c_);
             const float4 wq = ld_box(qb, q);
             const float4 mb = ld_box(tr.mbr, child);
@@ -632,7 +632,8 @@ __global__ __launch_bounds__(kQThreads) void k_expand_count(int slot, QTree tr, 
 #pragma unroll
       for (int k = 0; k < H; k++) {
         const bool full = (en[k].y & kFull) != 0;
-        test[k] = !full && c_ < L[k] && (int64_t)p[k] + L[k] <= r_sz;  // children must be nodes
+        // children must be nodes (a corrupt pos/len is never followed)
+        test[k] = !full && p[k] >= 0 && c_ < L[k] && (int64_t)p[k] + L[k] <= r_sz;
         const uint32_t child = test[k] ? (uint32_t)(p[k] + c_) : 0u;
         wq[k] = ld_box(qb, en[k].x);
         mb[k] = ld_box(tr.mbr, child);
@@ -1512,6 +1513,15 @@ int query_run_f32(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curt
   a.out_tid = qo->tid;
   a.out_cap = direct ? qo->cap : 0;
   if ((rc = qe.reserve_front(std::max<int64_t>(2 * nq, 1 << 20)))) return rc;
+  // the exact output of an allocating query: freed on every error return
+  struct Owned {
+    int* fid = nullptr;
+    int* tid = nullptr;
+    ~Owned() {
+      if (fid) (void)hipFree(fid);
+      if (tid) (void)hipFree(tid);
+    }
+  } owned;
   for (int pass = 0; pass < 2; pass++) {
     for (int attempt = 0;; attempt++) {
       if (a.counts) enqueue_fill_counts(qo->counts + q0, nq, s);
@@ -1552,26 +1562,25 @@ int query_run_f32(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curt
     const uint64_t H = qe.host_ctrl->total;
     qo->n = (int64_t)H;
     if (a.write) {
-      qo->out_fid = a.out_fid;
-      qo->out_tid = a.out_tid;
       if ((int64_t)H > a.out_cap)
         return set_error(CURTREE_ERANGE, "query: %llu pairs exceed the buffer of %lld",
                          (unsigned long long)H, (long long)a.out_cap);
+      qo->out_fid = a.out_fid;
+      qo->out_tid = a.out_tid;
+      owned.fid = owned.tid = nullptr;  // handed to the caller
       return CURTREE_OK;
     }
     if (qo->count_only || H == 0) return CURTREE_OK;
     // second pass: write into an exact allocation (or the caller's buffer)
     if (qo->alloc) {
-      int* fid = nullptr;
-      int* tid = nullptr;
-      CT_HIP(hipMalloc((void**)&fid, H * sizeof(int)));
-      if (hipMalloc((void**)&tid, H * sizeof(int)) != hipSuccess) {
-        (void)hipFree(fid);
+      if (hipMalloc((void**)&owned.fid, H * sizeof(int)) != hipSuccess ||
+          hipMalloc((void**)&owned.tid, H * sizeof(int)) != hipSuccess) {
+        (void)hipGetLastError();
         return set_error(CURTREE_ENOMEM, "query: cannot allocate %llu result pairs",
                          (unsigned long long)H);
       }
-      a.out_fid = fid;
-      a.out_tid = tid;
+      a.out_fid = owned.fid;
+      a.out_tid = owned.tid;
       a.out_cap = (int64_t)H;
     } else if ((int64_t)H > qo->cap) {
       return set_error(CURTREE_ERANGE, "query: %llu pairs exceed the buffer of %lld",

@@ -279,3 +279,36 @@ def test_over_a_million_listed_runs():
     o = np.argsort(sb[4])
     for i in range(4):
         assert np.array_equal(sb[i][o], bx[i])
+
+
+def test_async_builds_on_two_streams():
+    # two asynchronous builds share the library's work buffers; enqueued on
+    # two streams back to back, both trees must still equal the oracle's
+    import curtree_amd as ct
+
+    sets = [(datagen.mixture_boxes(300_001, seed=41), 16),
+            (datagen.uniform_boxes(250_000, seed=42, wmax=1e-3), 8)]
+    streams = [ct.Stream(), ct.Stream()]
+    devs, trees = [], []
+    for (bx, F), st in zip(sets, streams):
+        ids = np.arange(bx[0].shape[0], dtype=np.int32)
+        d = ct.DeviceBoxes.from_host(*bx, ids)
+        ct.synchronize()
+        devs.append(d)
+    for d, (bx, F), st in zip(devs, sets, streams):
+        trees.append(ct.build_rtree(d, F, ct.KEY_MORTON64, stream=st.p, sync=False))
+    for st in streams:
+        ct.synchronize(st.p)
+    for d, rt, (bx, F) in zip(devs, trees, sets):
+        ids = np.arange(bx[0].shape[0], dtype=np.int32)
+        osb, ot = orc.build_rtree(*bx, ids, F, 1)
+        sb = d.to_host()
+        t = rt.to_host()
+        for i, name in enumerate(("xmin", "ymin", "xmax", "ymax", "id")):
+            assert_bits_equal(sb[i], osb[i], f"sorted {name}")
+        for name in ("xmin", "ymin", "xmax", "ymax", "pos", "len"):
+            assert_bits_equal(t[name], ot[name], f"node {name}")
+        rt.free()
+        d.free()
+    for st in streams:
+        st.destroy()
