@@ -20,7 +20,12 @@
  *   or_level_plan     closed form and a brute-force loop (float plan for N<=2^24)
  *   or_build_levels   invariants (parent MBR = min/max of children, pos/len
  *                     tile each level) + numpy reduceat over groups
- *   or_bfs            equal, after canonical ordering, to or_brute_force
+ *   or_bfs            equal, after canonical ordering, to or_brute_force; its
+ *                     work counters (X, T) and per-depth frontier sizes equal
+ *                     a numpy closed form (the depth-d frontier of a tree whose
+ *                     parent MBRs contain their children is every (window, node
+ *                     at depth d) pair that intersects)
+ *   or_canonical_sort numpy lexsort by (query id, box id)
  *   or_brute_force    src/utility.h:22-45 semantics; checked against a numpy
  *                     vectorised brute force and, where it builds, against the
  *                     reference's own nested_loop_query (oracle/_ref)
@@ -341,8 +346,11 @@ OR_API int64_t or_bfs(int64_t r_sz, const float* rxmin, const float* rymin, cons
                       const float* bxmin, const float* bymin, const float* bxmax,
                       const float* bymax, const int32_t* bid, int64_t nq, const float* qxmin,
                       const float* qymin, const float* qxmax, const float* qymax,
-                      int32_t* out_fid, int32_t* out_tid, int64_t out_cap, int64_t* counters) {
+                      int32_t* out_fid, int32_t* out_tid, int64_t out_cap, int64_t* counters,
+                      int64_t* depth_sizes, int depth_cap) {
   int64_t X = 0, T = 0, H = 0;
+  int depth = 0;
+  for (int d = 0; depth_sizes && d < depth_cap; d++) depth_sizes[d] = 0;
   if (r_sz <= 0 || nq <= 0) {
     if (counters) { counters[0] = 0; counters[1] = 0; counters[2] = 0; }
     return 0;
@@ -354,6 +362,8 @@ OR_API int64_t or_bfs(int64_t r_sz, const float* rxmin, const float* rymin, cons
   while (n > 0) {
     /* all entries of one frontier sit on the same level of a balanced tree */
     int leaf = rpos[cur[0].node] >= r_sz;
+    if (depth_sizes && depth < depth_cap) depth_sizes[depth] = n;
+    depth++;
     if (!root) X += n;
     root = 0;
     if (leaf) {

@@ -140,9 +140,10 @@ def build_levels(sorted_boxes, fanout):
     return t
 
 
-def bfs(tree, sorted_boxes, windows, cap=None):
+def bfs(tree, sorted_boxes, windows, cap=None, depths=False):
     """Oracle window query: (fid, tid) in (query, sorted position) order and
-    counters (X, T, H)."""
+    counters (X, T, H); with depths=True also the frontier size at every depth
+    (depth 0: one entry per window at the root)."""
     xmin, ymin, xmax, ymax = map(f32, sorted_boxes[:4])
     ids = i32(sorted_boxes[4])
     qx0, qy0, qx1, qy1 = map(f32, windows[:4])
@@ -154,14 +155,20 @@ def bfs(tree, sorted_boxes, windows, cap=None):
     args = [ctypes.c_int64(r_sz), _p(tx0), _p(ty0), _p(tx1), _p(ty1), _p(tpos),
             _p(tlen), _p(xmin), _p(ymin), _p(xmax), _p(ymax), _p(ids),
             ctypes.c_int64(qx0.shape[0]), _p(qx0), _p(qy0), _p(qx1), _p(qy1)]
+    dsz = np.zeros(64, np.int64)
     if cap is None:
-        L.or_bfs(*args, None, None, ctypes.c_int64(0), _p(cnt))
+        L.or_bfs(*args, None, None, ctypes.c_int64(0), _p(cnt), None, ctypes.c_int(0))
         cap = int(cnt[2])
     fid = np.empty(max(cap, 1), np.int32)
     tid = np.empty(max(cap, 1), np.int32)
-    h = L.or_bfs(*args, _p(fid), _p(tid), ctypes.c_int64(cap), _p(cnt))
+    h = L.or_bfs(*args, _p(fid), _p(tid), ctypes.c_int64(cap), _p(cnt), _p(dsz),
+                 ctypes.c_int(64))
     assert h >= 0
-    return fid[:h], tid[:h], tuple(int(c) for c in cnt)
+    res = (fid[:h], tid[:h], tuple(int(c) for c in cnt))
+    if depths:
+        nz = np.nonzero(dsz)[0]
+        res = res + ([int(x) for x in dsz[: nz[-1] + 1]] if nz.size else [],)
+    return res
 
 
 def brute_force(boxes, windows):
@@ -188,6 +195,14 @@ def brute_count(boxes, windows, q0=0, q1=None, threads=1):
     return int(lib().or_brute_count(ctypes.c_int64(q0), ctypes.c_int64(q1), _p(qx0), _p(qy0),
                                     _p(qx1), _p(qy1), ctypes.c_int64(xmin.shape[0]), _p(xmin),
                                     _p(ymin), _p(xmax), _p(ymax), ctypes.c_int(threads)))
+
+
+def canonical_c(fid, tid):
+    """The oracle's canonical ordering (or_canonical_sort) on copies."""
+    fid = i32(fid).copy()
+    tid = i32(tid).copy()
+    lib().or_canonical_sort(ctypes.c_int64(fid.shape[0]), _p(fid), _p(tid))
+    return fid, tid
 
 
 def canonical(fid, tid):

@@ -308,3 +308,63 @@ def test_skewed_large_generator():
     g = np.floor(cx * 1000).astype(np.int64) * 1001 + np.floor(cy * 1000).astype(np.int64)
     _, cnt = np.unique(g, return_counts=True)
     assert np.sort(cnt)[::-1][:4000].sum() / cx.size > 0.6
+
+
+def closed_form_work(t, windows, n_levels_nodes):
+    """Frontier sizes of the BFS from its closed form: in a bulk-loaded tree
+    every parent MBR contains its children's (min/max of the children), so a
+    window that meets a node meets all its ancestors, and the depth-d frontier
+    is exactly the set of (window, node at depth d) pairs that intersect.
+    Returns (sizes per depth, X, T): X sums the depths >= 1, T the child
+    counts of every frontier entry (the tests the traversal makes)."""
+    q0, r0, q1, r1 = (np.asarray(a, np.float32) for a in windows[:4])
+    nq = q0.shape[0]
+    sizes = [nq]
+    T = nq * int(t["len"][0])
+    start = 0
+    for d, cnt in enumerate(n_levels_nodes):  # root level first
+        if d == 0:
+            start += cnt
+            continue
+        sl = slice(start, start + cnt)
+        x0, y0, x1, y1 = t["xmin"][sl], t["ymin"][sl], t["xmax"][sl], t["ymax"][sl]
+        ln = t["len"][sl].astype(np.int64)
+        hits = 0
+        tests = 0
+        for i in range(0, nq, 256):  # windows in blocks: (256 x nodes) masks
+            a = slice(i, min(i + 256, nq))
+            m = ~((q0[a, None] > x1[None]) | (q1[a, None] < x0[None]) |
+                  (r0[a, None] > y1[None]) | (r1[a, None] < y0[None]))
+            hits += int(m.sum())
+            tests += int((m * ln[None]).sum())
+        sizes.append(hits)
+        T += tests
+        start += cnt
+    return sizes, sum(sizes[1:]), T
+
+
+@pytest.mark.parametrize("n,fanout,nq", [(200_000, 16, 2000), (50_000, 5, 700), (30_000, 64, 500)])
+def test_bfs_work_counters_closed_form(n, fanout, nq):
+    # the workload shape of the plain CPU-runnable case, scaled down
+    bx = datagen.uniform_boxes(n, seed=21, wmax=1e-3)
+    qs = datagen.uniform_windows(nq, seed=22, smax=0.01)
+    ids = np.arange(n, dtype=np.int32)
+    sb, t = orc.build_rtree(*bx, ids, fanout, 1)
+    fid, tid, (X, T, H), depths = orc.bfs(t, sb, qs, depths=True)
+    lv, _ = orc.level_plan(n, fanout)
+    sizes, Xc, Tc = closed_form_work(t, qs, lv[::-1])
+    assert depths == sizes
+    assert (X, T) == (Xc, Tc)
+    assert H == orc.brute_count(bx, qs) == fid.shape[0]
+
+
+def test_canonical_sort_c_matches_lexsort():
+    r = np.random.default_rng(23)
+    n = 50_000
+    tid = np.sort(r.integers(-5, 300, n)).astype(np.int32)[r.permutation(n)]
+    fid = r.integers(-2**31, 2**31 - 1, n, dtype=np.int64).astype(np.int32)
+    fid[: n // 4] = fid[n // 4: n // 2]  # duplicate (query, box) pairs too
+    tid[: n // 4] = tid[n // 4: n // 2]
+    a = orc.canonical_c(fid, tid)
+    o = np.lexsort((fid, tid))
+    assert np.array_equal(a[0], fid[o]) and np.array_equal(a[1], tid[o])
