@@ -99,6 +99,8 @@ def _load():
         "curtree_query_f32": (I, [P(RTreeStruct), P(BBox), P(BBox), P(IdPair64), I, V]),
         "curtree_query_into_f32": (I, [P(RTreeStruct), P(BBox), P(BBox), V, V, I64, P(I64), I, V]),
         "curtree_query_count_f32": (I, [P(RTreeStruct), P(BBox), P(BBox), V, P(I64), V]),
+        "curtree_query_into_noalloc_f32": (I, [P(RTreeStruct), P(BBox), P(BBox), V, V, I64,
+                                               P(I64), V]),
         "curtree_pairs_canonical_sort": (I, [V, V, I64, V]),
         "curtree_self_join_f32": (I, [P(RTreeStruct), P(BBox), ctypes.c_float, P(IdPair64), V]),
         "curtree_query_batched_f32": (I, [P(RTreeStruct), P(BBox), P(BBox), I64, V, V,
@@ -359,6 +361,19 @@ def query_into(rt, rbox, qbox, fid_ptr, tid_ptr, cap, algo=ALGO_FRONTIER, stream
     rc = lib.curtree_query_into_f32(ctypes.byref(rt.s), ctypes.byref(rbox.s), ctypes.byref(qbox.s),
                                     fid_ptr, tid_ptr, int(cap), ctypes.byref(n), int(algo), stream)
     if rc == ERANGE:
+        return -n.value
+    _check(rc)
+    return n.value
+
+
+def query_into_noalloc(rt, rbox, qbox, fid_ptr, tid_ptr, cap, stream=None):
+    """query_into without any device allocation (the library's fixed scratch);
+    returns the pair count, negated when it exceeds `cap`."""
+    n = ctypes.c_int64()
+    rc = lib.curtree_query_into_noalloc_f32(ctypes.byref(rt.s), ctypes.byref(rbox.s),
+                                            ctypes.byref(qbox.s), fid_ptr, tid_ptr, int(cap),
+                                            ctypes.byref(n), stream)
+    if rc == ERANGE and n.value > cap:
         return -n.value
     _check(rc)
     return n.value

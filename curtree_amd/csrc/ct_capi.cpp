@@ -228,6 +228,13 @@ CURTREE_API int curtree_query_into_f32(const curtree_rtree_f32* rt, const curtre
   return CURTREE_OK;
 }
 
+CURTREE_API int curtree_query_into_noalloc_f32(const curtree_rtree_f32* rt,
+                                               const curtree_bbox_f32* rbox,
+                                               const curtree_bbox_f32* qbox, int* fid, int* tid,
+                                               int64_t cap, int64_t* n_out, void* stream) {
+  return query_into_noalloc_f32(rt, rbox, qbox, fid, tid, cap, n_out, as_stream(stream));
+}
+
 CURTREE_API int curtree_query_count_f32(const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox,
                                         const curtree_bbox_f32* qbox, int64_t* counts,
                                         int64_t* n_out, void* stream) {

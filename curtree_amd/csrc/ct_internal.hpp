@@ -170,6 +170,10 @@ struct QEngine {
   float4* win4 = nullptr;
   int64_t cap_win4 = 0;
   uint32_t* canon = nullptr;            // the tree has the bulk-load layout (device word)
+  // fixed: every buffer above is a slice of memory the engine does not own
+  // (the allocation-free entry point's scratch); a request beyond a buffer's
+  // capacity fails with CURTREE_ERANGE instead of allocating
+  bool fixed = false;
   bool needs_clear = true;              // status/ctrl are fresh: clear on the next run
   int last_nl = 0;                      // depths of the last run read back (host_ctrl)
   int reserve_front(int64_t entries);
@@ -234,6 +238,11 @@ int query_run_f32(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curt
                   hipStream_t s);
 
 int pairs_canonical_sort(Context& ctx, int* fid, int* tid, int64_t n, hipStream_t s);
+
+// query_into on the fixed scratch engine (no allocation; see ct_query.hip).
+int query_into_noalloc_f32(const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox,
+                           const curtree_bbox_f32* qbox, int* fid, int* tid, int64_t cap,
+                           int64_t* n_out, hipStream_t s);
 
 int query_batched_f32(Context& ctx, const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox,
                       const curtree_bbox_f32* qbox, int64_t budget, curtree_chunk_fn fn,

@@ -998,8 +998,15 @@ static int dev_alloc(void** p, size_t bytes) {
   return CURTREE_OK;
 }
 
+static int fixed_full(const char* what, int64_t want, int64_t cap) {
+  return set_error(CURTREE_ERANGE,
+                   "query: %s needs %lld entries, the fixed scratch holds %lld", what,
+                   (long long)want, (long long)cap);
+}
+
 int QEngine::reserve_front(int64_t entries) {
   if (entries <= cap_front) return CURTREE_OK;
+  if (fixed) return fixed_full("the frontier", entries, cap_front);
   int rc;
   // per-entry state (a QLeaf or QExp each) and tile counts (tiles hold >= 16
   // entries)
@@ -1016,6 +1023,7 @@ int QEngine::reserve_front(int64_t entries) {
 
 int QEngine::reserve_pieces(int64_t n) {
   if (n <= cap_pieces) return CURTREE_OK;
+  if (fixed) return fixed_full("the piece list", n, cap_pieces);
   n = std::max<int64_t>(n + n / 4, 4096);
   int rc = dev_alloc(&pieces, (size_t)n * sizeof(QPiece));
   cap_pieces = rc ? 0 : n;
@@ -1024,6 +1032,7 @@ int QEngine::reserve_pieces(int64_t n) {
 
 int QEngine::reserve_status(int64_t words) {
   if (words <= status_words) return CURTREE_OK;
+  if (fixed) return fixed_full("the scan status", words, status_words);
   words = std::max<int64_t>(words + words / 2, 1 << 16);
   int rc = dev_alloc((void**)&status, (size_t)words * 8);
   status_words = rc ? 0 : words;
@@ -1033,6 +1042,7 @@ int QEngine::reserve_status(int64_t words) {
 
 int QEngine::reserve_safe(int64_t nodes) {
   if (nodes <= cap_safe) return CURTREE_OK;
+  if (fixed) return fixed_full("the node flags", nodes, cap_safe);
   int rc = dev_alloc((void**)&safe, (size_t)std::max<int64_t>(nodes, 256));
   cap_safe = rc ? 0 : nodes;
   return rc;
@@ -1040,6 +1050,7 @@ int QEngine::reserve_safe(int64_t nodes) {
 
 int QEngine::reserve_plan(int64_t chunks) {
   if (chunks <= cap_plan) return CURTREE_OK;
+  if (fixed) return fixed_full("the chunk plan", chunks, cap_plan);
   int rc = dev_alloc((void**)&plan, (size_t)chunks * 16);
   cap_plan = rc ? 0 : chunks;
   return rc;
@@ -1047,6 +1058,7 @@ int QEngine::reserve_plan(int64_t chunks) {
 
 int QEngine::ensure_ctrl() {
   if (ctrl) return CURTREE_OK;
+  if (fixed) return set_error(CURTREE_EINVAL, "query: fixed engine without a control block");
   int rc = dev_alloc((void**)&ctrl, sizeof(QCtrl));
   if (rc) return rc;
   if (hipHostMalloc((void**)&host_ctrl, sizeof(QCtrl), hipHostMallocDefault) != hipSuccess) {
@@ -1058,6 +1070,7 @@ int QEngine::ensure_ctrl() {
 }
 
 void QEngine::release() {
+  if (fixed) return;  // slices of memory the engine does not own
   void* ps[] = {front[0], front[1], pieces, status, ctrl,  safe, plan,
                 box4,     node4,    win4,   canon,  est,   ttot};
   for (void* p : ps)
@@ -1381,6 +1394,8 @@ void enqueue_run(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
 }
 
 int64_t pieces_needed(const QEngine& qe, const QRunArgs& a) {
+  // contained ranges exist only with the containment shortcut (node flags)
+  if (!a.tr.safe) return 0;
   // every long range is at most one leaf entry and cut into kPiece slots
   return qe.cap_front + a.out_cap / (int64_t)kPiece + 1;
 }
@@ -1458,12 +1473,14 @@ int prepare_query(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curt
     }
   }
   // the flags cost one pass over the boxes; worth it for larger batches
-  const bool use = contain == 1 || (contain < 0 && nq_hint >= 256);
+  // (a fixed engine holds no tree-sized buffers: no flags, no staging)
+  const bool use = !qe.fixed && (contain == 1 || (contain < 0 && nq_hint >= 256));
   // AoS staging costs a read and a write of every box: worth it when the
   // batch is large against the tree (callers that keep trees resident
   // (contain == 0 here) stage once themselves)
   const int st = qtuning().staging;
-  const bool stage = st == 1 || (st < 0 && contain != 0 && nq_hint >= 256 && nq_hint * 256 >= N);
+  const bool stage = !qe.fixed &&
+                     (st == 1 || (st < 0 && contain != 0 && nq_hint >= 256 && nq_hint * 256 >= N));
   bool staged = false;
   if (stage && (rc = reserve_f4(&qe.box4, &qe.cap_box4, N))) return rc;
   if (use) {  // the flag pass also writes the AoS boxes when both are wanted
@@ -1482,7 +1499,7 @@ int prepare_query(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curt
     a->tr.mbr.b4 = qe.node4;
     a->qb.b4 = qe.win4;
   }
-  if (use || stage) {
+  if (use || stage || qe.fixed) {
     if ((rc = check_layout(&qe.canon, rt, N, s))) return rc;
     a->tr.canon = qe.canon;
   }
@@ -1598,6 +1615,103 @@ int query_frontier_f32(Context& ctx, const curtree_rtree_f32* rt, const curtree_
                        const curtree_bbox_f32* qbox, QueryOut* qo, hipStream_t s) {
   if (!qbox) return set_error(CURTREE_EINVAL, "query: null argument");
   return query_run_f32(ctx.qe, ctx.prof, rt, rbox, qbox, 0, qbox->sz, ctx.opt_contain, qo, s);
+}
+
+// ---------------------------------------------------------------------------
+// The allocation-free query (curtree_query_into_noalloc_f32): the plain
+// query's kernels on an engine whose buffers are slices of `scratch`, a
+// file-scope device array that every GPU holds until the process ends.  The
+// containment shortcut and the AoS staging are off (they need buffers the size
+// of the tree), so the run needs only the control block, the layout flag, the
+// scan status words, two frontier buffers, the per-entry state and the tile
+// counts, all sized for kScratchEntries frontier entries.
+constexpr int64_t kScratchEntries = int64_t(3) << 20;
+constexpr int64_t kScratchMaxWindows = kScratchEntries / 2;  // reserve_front asks for 2 per window
+constexpr int64_t kScratchTileWords = kScratchEntries / 16 + 2;  // tiles hold >= 16 entries
+constexpr size_t scr_up(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t kScrCtrl = 0;
+constexpr size_t kScrCanon = 4096;
+constexpr size_t kScrStatus = 8192;
+constexpr size_t kScrFront0 = kScrStatus + scr_up((size_t)kScratchTileWords * 8);
+constexpr size_t kScrFront1 = kScrFront0 + scr_up((size_t)kScratchEntries * 8);
+constexpr size_t kScrEst = kScrFront1 + scr_up((size_t)kScratchEntries * 8);
+constexpr size_t kScrTtot = kScrEst + scr_up((size_t)kScratchEntries * kEntryBytes);
+constexpr size_t kScrEnd = kScrTtot + scr_up((size_t)kScratchTileWords * 8);
+constexpr size_t kScratchBytes = size_t(128) << 20;
+static_assert(sizeof(QCtrl) <= kScrCanon, "control block slice");
+static_assert(kScrEnd <= kScratchBytes, "scratch layout");
+static_assert(kScrFront0 % 256 == 0 && kScrEst % 256 == 0 && kScrTtot % 256 == 0,
+              "slices stay aligned");
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kScratchDevices = 64;
+struct ScratchSlot {
+  bool ready = false;
+  QEngine qe;
+  QCtrl host_ctrl;  // read-back copy (plain host memory: nothing is allocated)
+};
+std::mutex g_scratch_mu;  // one allocation-free query at a time in the process
+ScratchSlot g_scratch[kScratchDevices];
+
+// The engine of the current device over that device's copy of `scratch`; its
+// address is looked up once per device, with the device current.
+int scratch_engine(QEngine** out) {
+  int dev = 0;
+  CT_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kScratchDevices)
+    return set_error(CURTREE_EINVAL, "query_noalloc: device %d beyond %d", dev, kScratchDevices);
+  ScratchSlot& sl = g_scratch[dev];
+  if (!sl.ready) {
+    unsigned char* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return set_error(CURTREE_EHIP, "query_noalloc: no address for the scratch array: %s",
+                       hipGetErrorString(e));
+    QEngine& q = sl.qe;
+    q.fixed = true;
+    q.ctrl = (QCtrl*)(ptr + kScrCtrl);
+    q.host_ctrl = &sl.host_ctrl;
+    q.canon = (uint32_t*)(ptr + kScrCanon);
+    q.status = (uint64_t*)(ptr + kScrStatus);
+    q.status_words = kScratchTileWords;
+    q.front[0] = ptr + kScrFront0;
+    q.front[1] = ptr + kScrFront1;
+    q.est = ptr + kScrEst;
+    q.ttot = (uint64_t*)(ptr + kScrTtot);
+    q.cap_front = kScratchEntries;
+    q.needs_clear = true;  // the status words start at zero on the first run
+    sl.ready = true;
+  }
+  *out = &sl.qe;
+  return CURTREE_OK;
+}
+}  // namespace
+
+int query_into_noalloc_f32(const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox,
+                           const curtree_bbox_f32* qbox, int* fid, int* tid, int64_t cap,
+                           int64_t* n_out, hipStream_t s) {
+  if (n_out) *n_out = 0;
+  if (!qbox) return set_error(CURTREE_EINVAL, "query_noalloc: null argument");
+  if (cap > 0 && (!fid || !tid)) return set_error(CURTREE_EINVAL, "query_noalloc: null output");
+  if (qbox->sz > kScratchMaxWindows)
+    return set_error(CURTREE_EINVAL, "query_noalloc: %d windows, at most %lld per call",
+                     qbox->sz, (long long)kScratchMaxWindows);
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  QEngine* qe = nullptr;
+  int rc;
+  if ((rc = scratch_engine(&qe))) return rc;
+  Prof pf;  // no per-kernel events (they would be allocated)
+  QueryOut qo;
+  qo.fid = fid;
+  qo.tid = tid;
+  qo.cap = cap;
+  rc = query_run_f32(*qe, pf, rt, rbox, qbox, 0, qbox->sz, 0, &qo, s);
+  if (n_out) *n_out = qo.n;
+  if (rc) return rc;
+  CT_HIP(hipStreamSynchronize(s));
+  return CURTREE_OK;
 }
 
 }  // namespace ct

@@ -119,6 +119,23 @@ CURTREE_API int curtree_query_into_f32(const curtree_rtree_f32* rt, const curtre
                                        const curtree_bbox_f32* qbox, int* fid, int* tid,
                                        int64_t cap, int64_t* n_out, int algo, void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc): the result of
+ * curtree_query_into_f32, computed with no allocation at all.  The work
+ * buffers are one file-scope device array of 128 MiB that every GPU the
+ * library runs on holds until the process ends.  Limits: at most 1,572,864
+ * windows per call (else CURTREE_EINVAL) and at most 3,145,728 (window, node)
+ * entries on any tree level (else CURTREE_ERANGE, the pairs are then not
+ * written); the containment shortcut is off, which never changes the pairs.
+ * Calls in one process run one at a time (a lock); the array keeps the last
+ * call's contents and is shared by every call on the GPU, so calls that
+ * overlap on one GPU from several processes or libraries race.  Synchronises
+ * `stream`. */
+CURTREE_API int curtree_query_into_noalloc_f32(const curtree_rtree_f32* rt,
+                                               const curtree_bbox_f32* rbox,
+                                               const curtree_bbox_f32* qbox, int* fid, int* tid,
+                                               int64_t cap, int64_t* n_out, void* stream);
+
 /* Count pass only: number of pairs per query (`counts`, Q int64 in device
  * memory, may be NULL) and in total (*n_out). */
 CURTREE_API int curtree_query_count_f32(const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox,
