@@ -8,6 +8,7 @@
  *   box_h2d, box_d2h, box_d_free, box_h_free, rt_d_alloc, rt_d_free,
  *   idpair_d_alloc, idpair_d2h, idpair_d_free, idpair_h_free
  *                        src/utility.cuh:16-205
+ *   readMBR              src/utility.h:7-14, src/utility.cpp:5-40
  *
  * Same signatures, ownership and synchronous semantics (null stream, complete
  * on return).  Errors print the library's message and exit(EXIT_FAILURE), as
@@ -159,4 +160,20 @@ inline void idpair_h_free(IDPAIR& ft) {
   delete[] ft.fid;
   delete[] ft.tid;
   ft.fid = ft.tid = nullptr;
+}
+
+/* ---- src/utility.h: readMBR ---------------------------------------------- */
+
+/* Reads `size` lines "x1,y1,x2,y2,id,pid,nume" into host arrays allocated
+ * here with new[] (free with box_h_free); box = [min(x1,x2)-r, max(x1,x2)+r] x
+ * [min(y1,y2)-r, max(y1,y2)+r].  A missing file or a bad line is an error. */
+inline void readMBR(const char* file, const int size, const float r, float*& xmin, float*& ymin,
+                    float*& xmax, float*& ymax, int*& id) {
+  const int n = size > 0 ? size : 0;
+  xmin = new float[n > 0 ? n : 1];
+  ymin = new float[n > 0 ? n : 1];
+  xmax = new float[n > 0 ? n : 1];
+  ymax = new float[n > 0 ? n : 1];
+  id = new int[n > 0 ? n : 1];
+  curtree_detail::check(curtree_read_mbr_csv(file, n, r, xmin, ymin, xmax, ymax, id), "readMBR");
 }

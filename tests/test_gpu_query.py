@@ -304,3 +304,52 @@ def test_skewed_mixed_point_and_window_queries():
           cx.astype(np.float32) + side / 2, cy.astype(np.float32) + side / 2)
     for fanout, km in ((16, 1), (32, 1)):
         run_case(bx, np.arange(n, dtype=np.int32), qs, fanout, km, brute_sample=400)
+
+
+@pytest.mark.parametrize("fanout", [16, 32])
+def test_mixture_loguniform_query_shape(fanout):
+    # the query shape of the 100M-box plain workload at a size the oracle
+    # finishes in seconds: 64-component Gaussian-mixture boxes with log-normal
+    # sizes, windows of side log-uniform [1e-4, 1e-2], 64-bit Morton keys; the
+    # raw pairs equal the oracle's traversal element for element, the
+    # canonical pairs of 1,000 sampled windows equal brute force
+    n, nq = 1_500_000, 40_000
+    bx = datagen.mixture_boxes(n, seed=401)
+    ws = datagen.loguniform_windows(nq, seed=402, lo=1e-4, hi=1e-2)
+    ids = np.random.default_rng(403).permutation(n).astype(np.int32)
+    fid, tid, (X, T, H) = run_case(bx, ids, ws, fanout, 1, brute_sample=1000)
+    print(f"[case] F={fanout}: {H} pairs, X={X}, T={T}", flush=True)
+    assert H == fid.shape[0] and H > 10 * nq
+
+
+def test_work_counts_match_oracle_plain_1e6():
+    # the CPU-runnable plain workload in full (1e6 boxes, fanout 16, 1e4
+    # windows): with the containment shortcut off the GPU frontier holds the
+    # same entries at every depth as the oracle's traversal, whose counts are
+    # pinned by a closed form; X and T (the query's algorithmic bytes) follow
+    import curtree_amd as ct
+
+    n, nq, F = 1_000_000, 10_000, 16
+    bx = datagen.uniform_boxes(n, seed=1, wmax=1e-3)
+    ws = datagen.uniform_windows(nq, seed=2, smax=0.01)
+    ids = np.arange(n, dtype=np.int32)
+    d, rt, q = setup(bx, ids, ws, F, 1)
+    ct.set_option("containment", 0)
+    try:
+        H = ct.query_count(rt, d, q)
+        st = ct.query_last_stats()
+    finally:
+        ct.set_option("containment", -1)
+    sb = d.to_host()
+    t = rt.to_host()
+    _, _, (X, T, Ho), depths = orc.bfs(t, sb, ws, depths=True)
+    print(f"[case] depths {depths}, X={X}, T={T}, H={Ho}", flush=True)
+    assert st["entries"] == depths and H == Ho == st["pairs"]
+    assert X == sum(st["entries"][1:])
+    # T: every entry tests all of its node's children, F but for the last
+    # node of a level (the bench prices T as F per entry)
+    Tf = F * sum(st["entries"])
+    assert T <= Tf and (Tf - T) < 1e-3 * T
+    rt.free()
+    d.free()
+    q.free()

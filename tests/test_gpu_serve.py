@@ -1,13 +1,16 @@
 """GPU: the serving runtime.  Requests of 1..4096 windows on three resident
-trees are submitted while others run; every request's result must equal the
-same windows queried alone (plain entry point), pair for pair, with batching
-on (4 slots, continuous batching) and off (one request at a time), under a
-result budget small enough that batches wait for released results."""
+trees (uniform, clustered and road-like line-segment MBRs) are submitted while
+others run; every request's result must equal the same windows queried alone
+(plain entry point) and the oracle's traversal of the same sorted boxes and
+tree (src/rtree_bfs_frontier.cuh:60-289 semantics), pair for pair, with
+batching on (4 slots, continuous batching) and off (one request at a time),
+under a result budget small enough that batches wait for released results."""
 import time
 
 import numpy as np
 import pytest
 
+import oracle_py as orc
 from curtree_amd import datagen
 
 pytestmark = pytest.mark.gpu
@@ -35,8 +38,10 @@ def test_results_identical_alone_and_in_the_mix(batching, slots):
     import curtree_amd as ct
 
     trees = make_trees(ct)
-    pool = ct.DeviceBoxes.from_host(*datagen.loguniform_windows(200_000, seed=133, lo=1e-4,
-                                                                  hi=3e-2))
+    hpool = datagen.loguniform_windows(200_000, seed=133, lo=1e-4, hi=3e-2)
+    pool = ct.DeviceBoxes.from_host(*hpool)
+    # host copies of every tree as built (sorted boxes and node arrays)
+    host = [(d.to_host(), rt.to_host()) for rt, d in trees]
     r = np.random.default_rng(134)
     nreq = 240
     sizes = np.exp(r.uniform(0, np.log(4096), nreq)).astype(np.int64)
@@ -79,3 +84,63 @@ def test_results_identical_alone_and_in_the_mix(batching, slots):
         fid, tid = ct.bfs_frontier(rt, d, _View(ct.window_view(pool, starts[i], sizes[i])))
         gf, gt, _ = got[i]
         assert np.array_equal(gf, fid) and np.array_equal(gt, tid), i
+        sb, t = host[which[i]]
+        win = tuple(w[starts[i]:starts[i] + sizes[i]] for w in hpool)
+        ofid, otid, _ = orc.bfs(t, sb, win)
+        assert np.array_equal(gf, ofid) and np.array_equal(gt, otid), ("oracle", i)
+
+
+def test_served_roadlike_tree_matches_oracle():
+    # the serving workload's tree kinds and window sizes at oracle-feasible
+    # sizes: a 2M-box road-like tree and a 1M-box clustered tree at fanout 32,
+    # windows of side log-uniform [1e-5, 2e-3]; 150 requests of 1..65,536
+    # windows (log-uniform count) submitted at once, each checked against the
+    # oracle's traversal
+    import curtree_amd as ct
+
+    sets = [datagen.roadlike_boxes(2_000_000, seed=140),
+            datagen.clustered_boxes(1_000_000, seed=141, frac=0.3, k=20, sigma=5e-3, wmax=1e-3)]
+    trees, host = [], []
+    for bx in sets:
+        d = ct.DeviceBoxes.from_host(*bx)
+        rt = ct.build_rtree(d, 32, ct.KEY_MORTON64)
+        trees.append((rt, d))
+        host.append((d.to_host(), rt.to_host()))
+    npool = 400_000
+    hpool = datagen.loguniform_windows(npool, seed=142, lo=1e-5, hi=2e-3)
+    pool = ct.DeviceBoxes.from_host(*hpool)
+    r = np.random.default_rng(143)
+    nreq = 150
+    sizes = np.exp(r.uniform(0, np.log(65536), nreq)).astype(np.int64)
+    starts = np.array([r.integers(0, npool - k) for k in sizes])
+    which = r.integers(0, 2, nreq)
+    sv = ct.Server(slots=4, budget_bytes=64 << 20, max_batch_windows=1 << 17, batching=True)
+    ids = [sv.add_tree(rt, d) for rt, d in trees]
+    req = {sv.submit(ids[which[k]], ct.window_view(pool, starts[k], sizes[k])): k
+           for k in range(nreq)}
+    got = {}
+    t_end = time.time() + 120
+    while len(got) < nreq and time.time() < t_end:
+        for rid in sv.completed():
+            st, res, _ = sv.poll(rid)
+            assert st == ct.REQ_DONE
+            n, fp, tp = res
+            got[req[rid]] = (ct._host_from_dev(fp, n, np.int32), ct._host_from_dev(tp, n, np.int32))
+            sv.release(rid)
+        time.sleep(0.0005)
+    sv.close()
+    assert len(got) == nreq
+    total = 0
+    for k in range(nreq):
+        sb, t = host[which[k]]
+        win = tuple(w[starts[k]:starts[k] + sizes[k]] for w in hpool)
+        ofid, otid, _ = orc.bfs(t, sb, win)
+        gf, gt = got[k]
+        assert np.array_equal(gf, ofid) and np.array_equal(gt, otid), k
+        total += gf.shape[0]
+    print(f"[case] {nreq} requests, {int(sizes.sum())} windows, {total} pairs", flush=True)
+    assert total > 0
+    pool.free()
+    for rt, d in trees:
+        rt.free()
+        d.free()
