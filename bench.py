@@ -11,7 +11,19 @@ region; each step first restores the unsorted boxes with a device copy that
 is outside the timed events.
 
 Prints one JSON line.  `value` is bulk-load boxes/s; the query's pairs/s and
-per-step latency percentiles are in `query`.
+per-step latency percentiles are in `query`.  The default run then measures
+the rest of the metric, each on its own workload at full size:
+  canonical_sort  the device (query id, box id) order of the query's pairs;
+  plain_f32  the same boxes and windows at fanout 32 (build + query steps);
+  plain_1e6  the 1,000,000-box CPU-runnable workload (build, query, sort);
+  batched    the batched entry point on the uneven-window workload below;
+  join       the distance self-join of 10,000,000 road-like segment MBRs
+             (r = 3e-4);
+  serve      served vs plain per-request launches on the 8 resident trees
+             below, a shorter request-rate sweep (0.75 s per rate, x1.25 steps);
+and `cpu_baseline` times the oracle's host path on this machine (bulk load,
+window query, brute force, self-join; 1 thread and every core).
+--plain-only skips the legs.
 
 Other modes (same JSON contract, one line):
   --mode batched  50,000,000 boxes (90% uniform + 10% in 5 Gaussian clusters,
@@ -50,6 +62,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "result pairs/s + p99 latency, served vs plain batch; bulk-load boxes/s"
+T_START = time.time()
 HBM_PEAK_GBS = 8000.0
 
 
@@ -81,86 +94,196 @@ def kernel_bytes(name, n, kbits, fanout):
     }.get(name)
 
 
-def cpu_baseline(boxes, fanout, key_mode, sample):
-    """The oracle's host bulk load (the reference algorithm restated in C),
-    single thread, on the first `sample` boxes of the same workload."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _parallel(fn, nq, threads):
+    """fn(q0, q1) over `threads` slices of [0, nq), in threads (the oracle's C
+    calls release the GIL); returns the list of results."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    cuts = np.linspace(0, nq, threads + 1).astype(np.int64)
+    with ThreadPoolExecutor(threads) as ex:
+        return list(ex.map(lambda k: fn(int(cuts[k]), int(cuts[k + 1])), range(threads)))
+
+
+def cpu_baseline(boxes, wins, fanout, key_mode, sample, join_boxes=None, join_r=0.0):
+    """The reference's CPU path restated in the oracle (oracle/oracle.c), timed
+    on this host: the host R-tree bulk load (1 thread) of the workload, which is
+    the headline row; then the host R-tree window query on the same tree and
+    windows, the brute-force nested loop (src/utility.h:22-45) on the 1e6-box
+    plain workload, and the distance self-join -- each with 1 thread and with
+    every core, on bounded samples whose rates extrapolate linearly."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_py as orc
 
+    nproc = os.cpu_count() or 1
+    rows = []
     sub = tuple(np.ascontiguousarray(a[:sample]) for a in boxes)
     ids = np.arange(sample, dtype=np.int32)
     t0 = time.perf_counter()
-    orc.build_rtree(*sub, ids, fanout, key_mode)
+    sb, tree = orc.build_rtree(*sub, ids, fanout, key_mode)
     dt = time.perf_counter() - t0
-    return {"value": sample / dt, "unit": "boxes/s", "cores": 1, "kind": "port",
+    head = {"value": sample / dt, "unit": "boxes/s", "cores": 1, "kind": "port",
             "sample": f"oracle host bulk load (Morton key, LSD sort, level reduce) of the first "
-                      f"{sample:,} boxes of the same workload, 1 thread, {dt:.2f} s"}
+                      f"{sample:,} boxes of the same workload, 1 thread, {dt:.2f} s",
+            "nproc": nproc, "cpu_model": _cpu_model()}
+    rows.append({"what": "host R-tree bulk load", "threads": 1, "items": sample, "wall_s": dt,
+                 "rate": sample / dt, "unit": "boxes/s"})
+    del sub
+    # host R-tree query (the oracle's traversal) on the same tree
+    if sample == boxes[0].shape[0]:
+        nq = wins[0].shape[0]
+        for threads, cnt in ((1, min(nq, 100_000)), (nproc, nq)):
+            w = tuple(a[:cnt] for a in wins[:4])
+            t0 = time.perf_counter()
+            r = _parallel(lambda q0, q1: orc.bfs_count(tree, sb, tuple(a[q0:q1] for a in w)),
+                          cnt, threads)
+            dt = time.perf_counter() - t0
+            H = sum(x[2] for x in r)
+            rows.append({"what": "host R-tree window query (BFS)", "threads": threads,
+                         "windows": cnt, "pairs": H, "wall_s": dt, "rate": H / dt,
+                         "unit": "pairs/s"})
+    del sb, tree
+    # brute force on the plain CPU-runnable workload: 1e6 boxes x 1e4 windows
+    from curtree_amd import datagen
+
+    b1 = datagen.uniform_boxes(1_000_000, seed=1, wmax=1e-3)
+    w1 = datagen.uniform_windows(10_000, seed=2, smax=0.01)
+    for threads, cnt in ((1, 2_000), (nproc, 10_000)):
+        t0 = time.perf_counter()
+        H = orc.brute_count(b1, w1, 0, cnt, threads=threads)
+        dt = time.perf_counter() - t0
+        rows.append({"what": "brute force nested loop, 1e6 boxes", "threads": threads,
+                     "windows": cnt, "tests": cnt * 1_000_000, "pairs": H, "wall_s": dt,
+                     "rate": H / dt, "unit": "pairs/s", "tests_per_s": cnt * 1e6 / dt})
+    # brute force on the workload's own boxes, a sample of windows, all cores
+    if sample == boxes[0].shape[0]:
+        cnt = 64
+        t0 = time.perf_counter()
+        H = orc.brute_count(boxes, wins, 0, cnt, threads=nproc)
+        dt = time.perf_counter() - t0
+        rows.append({"what": f"brute force nested loop, {sample:,} boxes", "threads": nproc,
+                     "windows": cnt, "tests": cnt * sample, "pairs": H, "wall_s": dt,
+                     "rate": H / dt, "unit": "pairs/s", "tests_per_s": cnt * sample / dt})
+    # distance self-join: host R-tree of the join boxes, queried with every
+    # box expanded by r (a sample of the boxes as queries)
+    if join_boxes is not None:
+        nj = join_boxes[0].shape[0]
+        t0 = time.perf_counter()
+        jsb, jt = orc.build_rtree(*join_boxes, np.arange(nj, dtype=np.int32), fanout, 1)
+        tb = time.perf_counter() - t0
+        r32 = np.float32(join_r)
+        for threads, cnt in ((1, min(nj, 200_000)), (nproc, nj)):
+            jw = (jsb[0][:cnt] - r32, jsb[1][:cnt] - r32, jsb[2][:cnt] + r32, jsb[3][:cnt] + r32)
+            t0 = time.perf_counter()
+            r = _parallel(lambda q0, q1: orc.bfs_count(jt, jsb, tuple(a[q0:q1] for a in jw)),
+                          cnt, threads)
+            dt = time.perf_counter() - t0
+            H = sum(x[2] for x in r)
+            rows.append({"what": f"host R-tree distance self-join, {nj:,} boxes, r={join_r:g}",
+                         "threads": threads, "query_boxes": cnt, "pairs": H, "wall_s": dt,
+                         "build_s_1thread": tb, "rate": H / dt, "unit": "pairs/s"})
+    head["rows"] = rows
+    return head
 
 
-def bench_batched(args, ct, datagen):
-    n = args.boxes if args.boxes_set else 50_000_000
-    nq = args.windows if args.windows_set else 4_000_000
-    tuning = None
+def _batched_tuning(args, ct):
     tf = os.path.join(ROOT, "curtree_amd", "tuning.json")
-    if not args.no_tuning and os.path.exists(tf):
-        with open(tf) as f:
-            tuning = json.load(f)["tuned"]["batched"]
+    if args.no_tuning or not os.path.exists(tf):
+        return None
+    with open(tf) as f:
+        return json.load(f)["tuned"]["batched"]
+
+
+def batched_leg(args, ct, datagen, n, nq, steps, warmup):
+    """The batched entry point on the uneven-window workload: n boxes (90%
+    uniform + 10% in 5 Gaussian clusters, sigma 1e-3, sides U[0,1e-3]), nq
+    windows of side log-uniform [1e-5, 0.05]; one step = count pass + one
+    graph replay per chunk of at most budget_pairs pairs.  Returns a dict."""
+    tuning = _batched_tuning(args, ct)
+    saved = None
+    if tuning:
         ct.set_option("query_tile_steps", tuning["tile_steps"])
         ct.set_option("query_grid", tuning["grid"])
-    F = args.fanout or (tuning["fanout"] if tuning else 16)
+    F = args.fanout if (args.fanout and args.mode == "batched") else (
+        tuning["fanout"] if tuning else 16)
     t0 = time.time()
     boxes = datagen.clustered_boxes(n, seed=5, frac=0.1, k=5, sigma=1e-3, wmax=1e-3)
     wins = datagen.loguniform_windows(nq, seed=6, lo=1e-5, hi=0.05)
-    print(f"[bench] generated {n:,} boxes, {nq:,} windows in {time.time() - t0:.1f} s",
+    print(f"[bench] batched: generated {n:,} boxes, {nq:,} windows in {time.time() - t0:.1f} s",
           file=sys.stderr, flush=True)
     d = ct.DeviceBoxes.from_host(*boxes, np.arange(n, dtype=np.int32))
     del boxes
     rt = ct.build_rtree(d, F, ct.KEY_MORTON64)
     q = ct.DeviceBoxes.from_host(*wins)
-    chunk_pairs = []
+    del wins
 
     def on_chunk(fid, tid, npairs, q0, q1):
-        chunk_pairs.append(npairs)  # pairs stay on the device; a consumer would read them here
-        return True
+        return True  # pairs stay on the device; a consumer would read them here
 
     def step():
-        chunk_pairs.clear()
         ct.synchronize()
         t = time.perf_counter()
         st = ct.query_batched(rt, d, q, args.budget_pairs, on_chunk)
         ct.synchronize()
         return time.perf_counter() - t, st
 
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         step()
-    print("[bench] warmup done", file=sys.stderr, flush=True)
     rec = []
-    for i in range(args.steps):
+    for i in range(steps):
         dt, st = step()
         rec.append((dt, st))
-        print(f"[bench] step {i}: {dt * 1e3:.1f} ms, {st['pairs']:,} pairs in {st['chunks']} "
-              f"chunks (count {st['count_ms']:.1f} ms, write {st['write_ms']:.1f} ms)",
-              file=sys.stderr, flush=True)
+        print(f"[bench] batched step {i}: {dt * 1e3:.1f} ms, {st['pairs']:,} pairs in "
+              f"{st['chunks']} chunks (count {st['count_ms']:.1f} ms, write "
+              f"{st['write_ms']:.1f} ms)", file=sys.stderr, flush=True)
     ms = np.array([r[0] for r in rec]) * 1e3
     st = rec[-1][1]
     total, counts = ct.query_count(rt, d, q, per_query=True)
+    assert total == st["pairs"], (total, st["pairs"])
+    rt.free()
+    d.free()
+    q.free()
+    if tuning:  # back to the plain workload's knobs
+        ct.set_option("query_tile_steps", args.plain_tuning["tile_steps"] if args.plain_tuning
+                      else 8)
+        ct.set_option("query_grid", args.plain_tuning["grid"] if args.plain_tuning else 0)
+    return {"workload": f"batched uneven windows: {n:,} boxes (90% uniform + 10% in 5 Gaussian "
+                        f"clusters), {nq:,} windows side log-uniform [1e-5,0.05], fanout {F}, "
+                        f"budget {args.budget_pairs:,} pairs per chunk",
+            "boxes": n, "windows": nq, "fanout": F, "budget_pairs": args.budget_pairs,
+            "tuning": tuning and {k: tuning[k] for k in ("fanout", "tile_steps", "grid")},
+            "pairs": int(st["pairs"]), "pairs_per_s": st["pairs"] / (ms.mean() * 1e-3),
+            "ms": float(ms.mean()), "chunks": int(st["chunks"]),
+            "max_chunk_pairs": int(st["max_chunk_pairs"]),
+            "count_ms": float(st["count_ms"]), "write_ms": float(st["write_ms"]),
+            "hits_per_window": {"max": int(counts.max()), "mean": float(counts.mean()),
+                                "zero_frac": float((counts == 0).mean())},
+            "step_p50_ms": float(np.percentile(ms, 50)),
+            "step_p99_ms": float(np.percentile(ms, 99)), "steps": steps}
+
+
+def bench_batched(args, ct, datagen):
+    n = args.boxes if args.boxes_set else 50_000_000
+    nq = args.windows if args.windows_set else 4_000_000
+    b = batched_leg(args, ct, datagen, n, nq, args.steps, args.warmup)
     out = {
-        "metric": METRIC, "value": st["pairs"] / (ms.mean() * 1e-3), "unit": "pairs/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": float(ms.mean()),
+        "metric": METRIC, "value": b["pairs_per_s"], "unit": "pairs/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": b["ms"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"batched uneven windows: {n:,} boxes (90% uniform + 10% in 5 "
-                               f"Gaussian clusters), {nq:,} windows side log-uniform [1e-5,0.05], "
-                               f"fanout {F}, budget {args.budget_pairs:,} pairs per chunk",
-                   "boxes": n, "windows": nq, "fanout": F, "budget_pairs": args.budget_pairs,
-                   "tuning": tuning and {k: tuning[k] for k in ("fanout", "tile_steps", "grid")}},
-        "batched": {"pairs": int(st["pairs"]), "chunks": int(st["chunks"]),
-                    "max_chunk_pairs": int(st["max_chunk_pairs"]),
-                    "count_ms": float(st["count_ms"]), "write_ms": float(st["write_ms"]),
-                    "hits_per_window": {"max": int(counts.max()), "mean": float(counts.mean()),
-                                        "zero_frac": float((counts == 0).mean())},
-                    "step_p50_ms": float(np.percentile(ms, 50)),
-                    "step_p99_ms": float(np.percentile(ms, 99))},
+        "config": {"workload": b["workload"], "boxes": n, "windows": nq, "fanout": b["fanout"],
+                   "budget_pairs": args.budget_pairs, "tuning": b["tuning"]},
+        "batched": b,
     }
     print(json.dumps(out), flush=True)
 
@@ -374,8 +497,9 @@ def bench_skewed(args, ct, datagen):
     print(json.dumps(out), flush=True)
 
 
-def bench_serve(args, ct, datagen):
-    scale = args.serve_scale
+def serve_trees(args, ct, datagen, scale):
+    """The 8 resident trees of the serving workload: 1e7..2e8 boxes (times
+    `scale`), uniform, clustered and road-like line-segment MBRs in turn."""
     sizes = [int(x * scale) for x in (1e7, 2e7, 4e7, 6e7, 8e7, 1.2e8, 1.6e8, 2e8)]
     kinds = ["uniform", "clustered", "roadlike"]
     trees = []
@@ -390,10 +514,20 @@ def bench_serve(args, ct, datagen):
             bx = datagen.roadlike_boxes(n, seed=70 + i)
         d = ct.DeviceBoxes.from_host(*bx, np.arange(n, dtype=np.int32))
         del bx
-        rt = ct.build_rtree(d, args.fanout, ct.KEY_MORTON64)
+        rt = ct.build_rtree(d, args.serve_fanout, ct.KEY_MORTON64)
         trees.append((rt, d, kind))
-        print(f"[bench] tree {i}: {n:,} {kind} boxes ({time.time() - t0:.0f} s)", file=sys.stderr,
-              flush=True)
+        print(f"[bench] serve tree {i}: {n:,} {kind} boxes ({time.time() - t0:.0f} s)",
+              file=sys.stderr, flush=True)
+    return sizes, trees
+
+
+def serve_leg(args, ct, datagen, scale, seconds, rate_step, drain_s):
+    """Served vs plain per-request launches on the 8 resident trees: Poisson
+    arrivals at a swept rate, requests of 1..65,536 windows (log-uniform
+    count), side log-uniform [1e-5, 2e-3], result budget serve_budget_gb.
+    Each mode's rate rises by rate_step until it no longer keeps up; the
+    served pairs/s are compared at each baseline's knee p99."""
+    sizes, trees = serve_trees(args, ct, datagen, scale)
     npool = 4_000_000
     pool = ct.DeviceBoxes.from_host(*datagen.loguniform_windows(npool, seed=80, lo=1e-5, hi=2e-3))
     budget = args.serve_budget_gb << 30
@@ -404,10 +538,12 @@ def bench_serve(args, ct, datagen):
         sv = ct.Server(budget_bytes=budget, max_batch_windows=args.serve_batch_windows, **kw)
         ids = [sv.add_tree(rt, d) for rt, d, _ in trees]
         pts = []
-        _poisson_run(ct, sv, ids, pool, npool, 200.0, 0.5, seed=1)  # warm-up
-        lam = args.serve_rate0
+        _poisson_run(ct, sv, ids, pool, npool, 200.0, 0.5, seed=1, drain_s=drain_s)  # warm-up
+        # the baseline's sweep starts lower, so that its knee lies inside it
+        lam = args.serve_rate0 / (4 if name == "plain" else 1)
         while lam <= args.serve_rate_max:
-            p = _poisson_run(ct, sv, ids, pool, npool, lam, args.serve_seconds, seed=int(lam))
+            p = _poisson_run(ct, sv, ids, pool, npool, lam, seconds, seed=int(lam),
+                             drain_s=drain_s)
             pts.append(p)
             print(f"[bench] {name} {lam:.0f} req/s: {p['pairs_per_s'] / 1e9:.2f} Gpairs/s, "
                   f"p50 {p['p50_ms']:.2f} ms, p99 {p['p99_ms']:.2f} ms, kept up {p['kept_up']}, "
@@ -415,7 +551,7 @@ def bench_serve(args, ct, datagen):
                   file=sys.stderr, flush=True)
             if not p["kept_up"]:
                 break
-            lam *= args.serve_rate_step
+            lam *= rate_step
         sv.close()
         res[name] = pts
     # the plain entry point per request (what a caller does without the server)
@@ -424,55 +560,204 @@ def bench_serve(args, ct, datagen):
     ct._check(ct.lib.curtree_malloc(ctypes.byref(fb), 4 * cap))
     ct._check(ct.lib.curtree_malloc(ctypes.byref(tb), 4 * cap))
     pts = []
-    _plain_entry_run(ct, trees, pool, npool, 200.0, 0.5, 1, (fb.value, tb.value), cap)
+    _plain_entry_run(ct, trees, pool, npool, 200.0, 0.5, 1, (fb.value, tb.value), cap,
+                     drain_s=drain_s)
     lam = args.serve_rate0 / 4
     while lam <= args.serve_rate_max:
-        p = _plain_entry_run(ct, trees, pool, npool, lam, args.serve_seconds, int(lam),
-                             (fb.value, tb.value), cap)
+        p = _plain_entry_run(ct, trees, pool, npool, lam, seconds, int(lam),
+                             (fb.value, tb.value), cap, drain_s=drain_s)
         pts.append(p)
-        print(f"[bench] plain entry point {lam:.0f} req/s: {p['pairs_per_s'] / 1e9:.2f} Gpairs/s, "
-              f"p50 {p['p50_ms']:.2f} ms, p99 {p['p99_ms']:.2f} ms, kept up {p['kept_up']}",
-              file=sys.stderr, flush=True)
+        print(f"[bench] plain entry point {lam:.0f} req/s: {p['pairs_per_s'] / 1e9:.2f} "
+              f"Gpairs/s, p50 {p['p50_ms']:.2f} ms, p99 {p['p99_ms']:.2f} ms, kept up "
+              f"{p['kept_up']}", file=sys.stderr, flush=True)
         if not p["kept_up"]:
             break
-        lam *= args.serve_rate_step
+        lam *= rate_step
     ct.lib.curtree_free(fb)
     ct.lib.curtree_free(tb)
     res["plain_entry"] = pts
+    pool.free()
+    for rt, d, _ in trees:
+        rt.free()
+        d.free()
     # equal-p99 comparisons: served pairs/s at each baseline's knee p99
     ref = _knee(res["plain"])
     ref_e = _knee(res["plain_entry"])
     best = _at_p99(res["served"], ref["p99_ms"])
     best_e = _at_p99(res["served"], ref_e["p99_ms"])
     value = best_e["pairs_per_s"] if best_e else 0.0
+    return {
+        "workload": "serving: 8 resident trees of " + "/".join(f"{n / 1e6:g}M" for n in sizes)
+                    + " boxes (uniform, clustered, road-like), Poisson arrivals, 1..65536 "
+                    "windows per request (log-uniform), side log-uniform [1e-5,2e-3], "
+                    f"{args.serve_budget_gb} GB result budget",
+        "trees": sizes, "fanout": args.serve_fanout, "seconds_per_rate": seconds,
+        "rate_step": rate_step,
+        "served_pairs_per_s": value,
+        "plain": res["plain"], "served": res["served"], "plain_entry": res["plain_entry"],
+        "vs_plain_entry_point": {
+            "baseline": "curtree_query_into per request, one at a time",
+            "knee_rate": ref_e["rate"], "p99_ref_ms": ref_e["p99_ms"],
+            "plain_pairs_per_s": ref_e["pairs_per_s"],
+            "served_pairs_per_s_at_equal_p99": value,
+            "served_p99_ms": best_e["p99_ms"] if best_e else None,
+            "ratio": value / ref_e["pairs_per_s"] if ref_e["pairs_per_s"] else None},
+        "vs_unbatched_server": {
+            "baseline": "the server with batching off and one slot",
+            "knee_rate": ref["rate"], "p99_ref_ms": ref["p99_ms"],
+            "plain_pairs_per_s": ref["pairs_per_s"],
+            "served_pairs_per_s_at_equal_p99": best["pairs_per_s"] if best else 0.0,
+            "served_p99_ms": best["p99_ms"] if best else None,
+            "ratio": (best["pairs_per_s"] / ref["pairs_per_s"])
+                     if best and ref["pairs_per_s"] else None}}
+
+
+def bench_serve(args, ct, datagen):
+    sv = serve_leg(args, ct, datagen, args.serve_scale, args.serve_seconds, args.serve_rate_step,
+                   20.0)
     out = {
-        "metric": METRIC, "value": value, "unit": "pairs/s", "n_gpus": 1,
-        "steps": len(res["plain"]) + len(res["served"]), "warmup": 1,
+        "metric": METRIC, "value": sv["served_pairs_per_s"], "unit": "pairs/s", "n_gpus": 1,
+        "steps": len(sv["plain"]) + len(sv["served"]), "warmup": 1,
         "ms_per_step": args.serve_seconds * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "serving: 8 resident trees of "
-                               + "/".join(f"{n / 1e6:g}M" for n in sizes)
-                               + " boxes (uniform, clustered, road-like), Poisson arrivals, "
-                               "1..65536 windows per request (log-uniform), side log-uniform "
-                               f"[1e-5,2e-3], {args.serve_budget_gb} GB result budget",
-                   "trees": sizes, "fanout": args.fanout},
-        "serve": {"plain": res["plain"], "served": res["served"],
-                  "plain_entry": res["plain_entry"],
-                  "vs_plain_entry_point": {
-                      "baseline": "curtree_query_into per request, one at a time",
-                      "knee_rate": ref_e["rate"], "p99_ref_ms": ref_e["p99_ms"],
-                      "plain_pairs_per_s": ref_e["pairs_per_s"],
-                      "served_pairs_per_s_at_equal_p99": value,
-                      "ratio": value / ref_e["pairs_per_s"] if ref_e["pairs_per_s"] else None},
-                  "vs_unbatched_server": {
-                      "baseline": "the server with batching off and one slot",
-                      "knee_rate": ref["rate"], "p99_ref_ms": ref["p99_ms"],
-                      "plain_pairs_per_s": ref["pairs_per_s"],
-                      "served_pairs_per_s_at_equal_p99": best["pairs_per_s"] if best else 0.0,
-                      "ratio": (best["pairs_per_s"] / ref["pairs_per_s"])
-                               if best and ref["pairs_per_s"] else None}},
+        "config": {"workload": sv["workload"], "trees": sv["trees"], "fanout": sv["fanout"]},
+        "serve": sv,
     }
     print(json.dumps(out), flush=True)
+
+
+def fanout_leg(ct, pristine, work, q, n, F, key_mode, kbits, H, steps, warmup, fbuf, tbuf):
+    """Build + query steps of the plain workload at another fanout (same boxes
+    and windows, so the same H pairs)."""
+    tree = ct.alloc_tree(n, F)
+    ev = [ct.Event() for _ in range(3)]
+    rec = []
+    for i in range(warmup + steps):
+        for f in ("xmin", "ymin", "xmax", "ymax", "id"):
+            ct._check(ct.lib.curtree_memcpy_d2d(getattr(work.s, f), getattr(pristine.s, f),
+                                                4 * n, None))
+        ct.synchronize()
+        ev[0].record()
+        ct.build_into(work, tree, key_mode)
+        ev[1].record()
+        got = ct.query_into(tree, work, q, fbuf.value, tbuf.value, H)
+        ev[2].record()
+        ct.synchronize()
+        assert got == H, (got, H)
+        if i >= warmup:
+            rec.append((ev[0].elapsed_ms(ev[1]), ev[1].elapsed_ms(ev[2])))
+    tree.free()
+    b = np.array([r[0] for r in rec])
+    qm = np.array([r[1] for r in rec])
+    bpb = build_bytes_per_box(kbits, F)
+    gbs = bpb * n / (b.mean() * 1e-3) / 1e9
+    print(f"[bench] fanout {F}: build {b.mean():.3f} ms, query {qm.mean():.3f} ms",
+          file=sys.stderr, flush=True)
+    return {"fanout": F, "steps": steps, "build_ms": float(b.mean()),
+            "boxes_per_s": n / (b.mean() * 1e-3), "algorithmic_bytes_per_box": bpb,
+            "achieved_GBs": gbs, "frac_of_8TBs": gbs / HBM_PEAK_GBS,
+            "query_ms": float(qm.mean()), "pairs": int(H),
+            "pairs_per_s": H / (qm.mean() * 1e-3)}
+
+
+def canonical_leg(ct, tree, work, q, H, fbuf, tbuf, steps):
+    """The device canonical (query id, box id) order of a query's H pairs:
+    each step writes the pairs again (untimed) and sorts them in place."""
+    ev = [ct.Event() for _ in range(2)]
+    ms = []
+    for _ in range(steps):
+        assert ct.query_into(tree, work, q, fbuf.value, tbuf.value, H) == H
+        ct.synchronize()
+        ev[0].record()
+        ct._check(ct.lib.curtree_pairs_canonical_sort(fbuf, tbuf, int(H), None))
+        ev[1].record()
+        ct.synchronize()
+        ms.append(ev[0].elapsed_ms(ev[1]))
+    ms = np.array(ms)
+    print(f"[bench] canonical sort of {H:,} pairs: {ms.mean():.2f} ms", file=sys.stderr,
+          flush=True)
+    return {"pairs": int(H), "ms": float(ms.mean()), "pairs_per_s": H / (ms.mean() * 1e-3),
+            "steps": steps}
+
+
+def small_leg(ct, datagen, steps):
+    """The CPU-runnable plain workload on the GPU: 1,000,000 boxes (centers
+    uniform, sides U[0,1e-3]), fanout 16, 1e4 windows of side U[0,0.01];
+    bulk load, query and canonical sort timed separately."""
+    n, nq, F = 1_000_000, 10_000, 16
+    bx = datagen.uniform_boxes(n, seed=1, wmax=1e-3)
+    pristine = ct.DeviceBoxes.from_host(*bx, np.arange(n, dtype=np.int32))
+    work = ct.DeviceBoxes(n)
+    q = ct.DeviceBoxes.from_host(*datagen.uniform_windows(nq, seed=2, smax=0.01))
+    tree = ct.alloc_tree(n, F)
+    ev = [ct.Event() for _ in range(4)]
+    fb, tb = ctypes.c_void_p(), ctypes.c_void_p()
+    H = None
+    rec = []
+    for i in range(steps + 1):
+        for f in ("xmin", "ymin", "xmax", "ymax", "id"):
+            ct._check(ct.lib.curtree_memcpy_d2d(getattr(work.s, f), getattr(pristine.s, f),
+                                                4 * n, None))
+        ct.synchronize()
+        ev[0].record()
+        ct.build_into(work, tree, ct.KEY_MORTON64)
+        ev[1].record()
+        if H is None:
+            H = ct.query_count(tree, work, q)
+            ct._check(ct.lib.curtree_malloc(ctypes.byref(fb), 4 * max(H, 1)))
+            ct._check(ct.lib.curtree_malloc(ctypes.byref(tb), 4 * max(H, 1)))
+            ev[1].record()
+        got = ct.query_into(tree, work, q, fb.value, tb.value, H)
+        ev[2].record()
+        ct._check(ct.lib.curtree_pairs_canonical_sort(fb, tb, int(H), None))
+        ev[3].record()
+        ct.synchronize()
+        assert got == H
+        if i:
+            rec.append([ev[j].elapsed_ms(ev[j + 1]) for j in range(3)])
+    for p_ in (fb, tb):
+        ct.lib.curtree_free(p_)
+    tree.free()
+    work.free()
+    pristine.free()
+    q.free()
+    r = np.array(rec).mean(axis=0)
+    return {"workload": f"{n:,} FP32 boxes (centers uniform, sides U[0,1e-3]), fanout {F}, "
+                        f"64-bit Morton keys; {nq:,} windows side U[0,0.01]",
+            "boxes": n, "windows": nq, "pairs": int(H), "build_ms": float(r[0]),
+            "query_ms": float(r[1]), "canonical_sort_ms": float(r[2]),
+            "boxes_per_s": n / (r[0] * 1e-3), "pairs_per_s": H / (r[1] * 1e-3), "steps": steps}
+
+
+def join_leg(ct, datagen, n, r, fanout, steps):
+    """Distance self-join on road-like segment MBRs (the reference drivers'
+    workload shape: every data box expanded by r queries the tree of the same
+    boxes): GPU bulk load once, then timed self-join calls (allocating their
+    pairs, as curtree_self_join_f32 does).  Returns the dict and the host boxes
+    (for the CPU row)."""
+    bx = datagen.roadlike_boxes(n, seed=90)
+    d = ct.DeviceBoxes.from_host(*bx, np.arange(n, dtype=np.int32))
+    rt = ct.build_rtree(d, fanout, ct.KEY_MORTON64)
+    ev = [ct.Event() for _ in range(2)]
+    ms, H = [], 0
+    for i in range(steps + 1):
+        ct.synchronize()
+        ev[0].record()
+        p = ct.self_join(rt, d, r)
+        ev[1].record()
+        ct.synchronize()
+        H = p.n
+        p.free()
+        if i:
+            ms.append(ev[0].elapsed_ms(ev[1]))
+    rt.free()
+    d.free()
+    ms = np.array(ms)
+    print(f"[bench] self-join: {n:,} boxes, r={r:g}: {H:,} pairs in {ms.mean():.2f} ms",
+          file=sys.stderr, flush=True)
+    return {"workload": f"distance self-join: {n:,} road-like segment MBRs, r = {r:g}, fanout "
+                        f"{fanout}", "boxes": n, "r": r, "pairs": int(H), "ms": float(ms.mean()),
+            "pairs_per_s": H / (ms.mean() * 1e-3), "steps": steps}, bx
 
 
 def main():
@@ -507,10 +792,22 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=0,
                     help="boxes timed in the CPU baseline (0: the whole workload)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--plain-only", action="store_true",
+                    help="plain mode: only the 100M-box plain workload (no fanout-32, batched, "
+                         "self-join or serving legs)")
+    ap.add_argument("--leg-steps", type=int, default=3,
+                    help="plain mode: timed steps of the fanout-32, batched and self-join legs")
+    ap.add_argument("--leg-serve-seconds", type=float, default=0.75,
+                    help="plain mode: seconds per request rate in the serving leg")
+    ap.add_argument("--leg-serve-rate-step", type=float, default=1.25,
+                    help="plain mode: request rate factor between points of the serving leg")
     ap.add_argument("--query-tile-steps", type=int, default=0,
                     help="wave-steps per query frontier tile (4, 8, 16; 0: library default)")
     ap.add_argument("--query-grid", type=int, default=0,
                     help="blocks per chained query launch (0: library default)")
+    ap.add_argument("--leaf-fused", type=int, default=None, choices=[-1, 0, 1],
+                    help="writing leaf stage: -1 by run size (library default), 0 count/scan/"
+                         "write, 1 one chained pass")
     ap.add_argument("--expand-split", type=int, default=-1, choices=[-1, 0, 1],
                     help="internal levels: -1 by run size (library default), 0 chained scan, "
                          "1 count/scan/write")
@@ -531,6 +828,8 @@ def main():
         ct.set_option("query_grid", args.query_grid)
     if args.expand_split != -1:
         ct.set_option("expand_split", args.expand_split)
+    if args.leaf_fused is not None:
+        ct.set_option("leaf_fused", args.leaf_fused)
     args.boxes_set = args.boxes is not None
     args.windows_set = args.windows is not None
     if args.mode == "plain" and args.fanout is None:
@@ -547,8 +846,7 @@ def main():
                 ct.set_option("query_tile_steps", tp["tile_steps"])
             if not args.query_grid:
                 ct.set_option("query_grid", tp["grid"])
-    if args.mode == "serve" and args.fanout is None:
-        args.fanout = 32
+    args.serve_fanout = args.fanout if (args.mode == "serve" and args.fanout) else 32
     if args.mode == "batched":
         return bench_batched(args, ct, datagen)
     if args.mode == "serve":
@@ -635,8 +933,8 @@ def main():
     for name in ("k_build_init", "k_extent", "k_hist", "k_sort_plan", "k_sweep_kv", "k_final",
                  "k_run_len", "k_fixup_short", "k_fixup_long", "k_fixup_huge", "k_fixup_groups",
                  "k_fallback", "k_level",
-                 "k_safe", "k_run_begin", "k_expand", "k_expand_count", "k_expand_write",
-                 "k_leaf_count", "k_tile_scan",
+                 "k_safe", "k_stage4", "k_run_begin", "k_expand", "k_expand_count", "k_expand_write",
+                 "k_leaf_count", "k_tile_scan", "k_leaf_fused",
                  "k_leaf_write", "k_write_pieces"):
         tot, cnt = ct.Prof.get(name)
         if cnt:
@@ -721,11 +1019,36 @@ def main():
                      "algorithmic_bytes_per_launch": dom_bytes,
                      "avg_launch_ms": dom_ms, "launches_timed": dom_cnt},
     }
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(boxes, F, key_mode,
-                                           min(args.cpu_sample, n) if args.cpu_sample else n)
+    join_boxes = None
+    join_r = 3e-4
+    out["canonical_sort"] = canonical_leg(ct, tree, work, q, H, fbuf, tbuf, 2)
+    if not args.plain_only:
+        # the rest of the metric, each on its own workload at full size
+        out["plain_f32"] = fanout_leg(ct, pristine, work, q, n, 32, key_mode, kbits, H,
+                                      args.leg_steps, 1, fbuf, tbuf)
     ct.lib.curtree_free(fbuf)
     ct.lib.curtree_free(tbuf)
+    tree.free()
+    work.free()
+    pristine.free()
+    q.free()
+    if not args.plain_only:
+        out["plain_1e6"] = small_leg(ct, datagen, args.leg_steps)
+        out["batched"] = batched_leg(args, ct, datagen, 50_000_000, 4_000_000, args.leg_steps, 1)
+        out["join"], join_boxes = join_leg(ct, datagen, 10_000_000, join_r, F, args.leg_steps)
+        out["serve"] = serve_leg(args, ct, datagen, 1.0, args.leg_serve_seconds,
+                                 args.leg_serve_rate_step, 3.0)
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(boxes, wins, F, key_mode,
+                                           min(args.cpu_sample, n) if args.cpu_sample else n,
+                                           join_boxes, join_r)
+        if out.get("join"):
+            allc = [r for r in out["cpu_baseline"]["rows"]
+                    if r["what"].startswith("host R-tree distance self-join")
+                    and r["query_boxes"] == out["join"]["boxes"]]
+            if allc:  # the host path over every box finds the GPU's pairs
+                out["join"]["cpu_pairs_equal"] = allc[-1]["pairs"] == out["join"]["pairs"]
+    out["wall_s_total"] = time.time() - T_START
     print(json.dumps(out), flush=True)
 
 

@@ -1022,69 +1022,79 @@ __global__ __launch_bounds__(256) void k_fixup_short(int64_t n, BuildCtrl* c,
 // from a scan of the counts, the scatter and the reload through LDS.  Every
 // thread of the block calls it.
 constexpr int kRunPer = 16;  // items per lane of the LDS run sorts
+// One stable pass of lds_lsd over the 8-bit digit at bit `sh` of the items.
+// On return s_cnt[0][d] is the first position of digit d in the reordered
+// items, and x[k] the item at position wbase + 64 * k.
+template <int NW>
+__device__ __forceinline__ void lds_pass(uint64_t (&x)[kRunPer], int kUse, int wbase, int sh,
+                                         uint64_t* s_net, uint32_t (*s_cnt)[256],
+                                         uint32_t* s_wsum) {
+  const int t = threadIdx.x, lane = lane_id(), w = t >> 6;
+  const uint64_t lt = lanemask_lt();
+#pragma unroll
+  for (int k = 0; k < 4; k++) s_cnt[w][lane + 64 * k] = 0;
+  uint32_t rank[kRunPer];
+#pragma unroll
+  for (int k = 0; k < kRunPer; k++) {
+    if (k < kUse) {
+      const uint32_t d = (uint32_t)(x[k] >> sh) & 255u;
+      uint64_t m = ~0ull;
+#pragma unroll
+      for (int bit = 0; bit < 8; bit++) {
+        const uint64_t bb = ballot((d >> bit) & 1u);
+        m &= ((d >> bit) & 1u) ? bb : ~bb;
+      }
+      const uint32_t before = s_cnt[w][d];
+      rank[k] = before + (uint32_t)popc64(m & lt);
+      if ((m & lt) == 0) s_cnt[w][d] = before + (uint32_t)popc64(m);
+    }
+  }
+  __syncthreads();
+  // digit t (threads 0..255): offsets of the waves' shares, after all
+  // smaller digits
+  uint32_t cw[NW], tot = 0, inc = 0;
+  if (t < 256) {
+#pragma unroll
+    for (int k = 0; k < NW; k++) {
+      cw[k] = s_cnt[k][t];
+      tot += cw[k];
+    }
+    inc = tot;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t u = __shfl_up(inc, d);
+      if (lane >= d) inc += u;
+    }
+    if (lane == 63) s_wsum[w] = inc;
+  }
+  __syncthreads();
+  if (t < 256) {
+    uint32_t off = inc - tot;
+    for (int k = 0; k < w; k++) off += s_wsum[k];
+#pragma unroll
+    for (int k = 0; k < NW; k++) {
+      s_cnt[k][t] = off;
+      off += cw[k];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kRunPer; k++)
+    if (k < kUse) s_net[s_cnt[w][(uint32_t)(x[k] >> sh) & 255u] + rank[k]] = x[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kRunPer; k++)
+    if (k < kUse) x[k] = s_net[wbase + 64 * k];
+}
+
 template <int NW>
 __device__ __forceinline__ void lds_lsd(uint64_t (&x)[kRunPer], int kUse, int wbase,
                                         uint64_t* s_net, uint32_t (*s_cnt)[256],
                                         uint32_t* s_wsum) {
-  const int t = threadIdx.x, lane = lane_id(), w = t >> 6;
-  const uint64_t lt = lanemask_lt();
-  for (int pass = 0; pass < 4; pass++) {
-    const int sh = 32 + 8 * pass;
-#pragma unroll
-    for (int k = 0; k < 4; k++) s_cnt[w][lane + 64 * k] = 0;
-    uint32_t rank[kRunPer];
-#pragma unroll
-    for (int k = 0; k < kRunPer; k++) {
-      if (k < kUse) {
-        const uint32_t d = (uint32_t)(x[k] >> sh) & 255u;
-        uint64_t m = ~0ull;
-#pragma unroll
-        for (int bit = 0; bit < 8; bit++) {
-          const uint64_t bb = ballot((d >> bit) & 1u);
-          m &= ((d >> bit) & 1u) ? bb : ~bb;
-        }
-        const uint32_t before = s_cnt[w][d];
-        rank[k] = before + (uint32_t)popc64(m & lt);
-        if ((m & lt) == 0) s_cnt[w][d] = before + (uint32_t)popc64(m);
-      }
-    }
-    __syncthreads();
-    // digit t (threads 0..255): offsets of the waves' shares, after all
-    // smaller digits
-    uint32_t cw[NW], tot = 0, inc = 0;
-    if (t < 256) {
-#pragma unroll
-      for (int k = 0; k < NW; k++) {
-        cw[k] = s_cnt[k][t];
-        tot += cw[k];
-      }
-      inc = tot;
-#pragma unroll
-      for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t u = __shfl_up(inc, d);
-        if (lane >= d) inc += u;
-      }
-      if (lane == 63) s_wsum[w] = inc;
-    }
-    __syncthreads();
-    if (t < 256) {
-      uint32_t off = inc - tot;
-      for (int k = 0; k < w; k++) off += s_wsum[k];
-#pragma unroll
-      for (int k = 0; k < NW; k++) {
-        s_cnt[k][t] = off;
-        off += cw[k];
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kRunPer; k++)
-      if (k < kUse) s_net[s_cnt[w][(uint32_t)(x[k] >> sh) & 255u] + rank[k]] = x[k];
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kRunPer; k++)
-      if (k < kUse) x[k] = s_net[wbase + 64 * k];
-  }
+  // (a pass reuses only its own wave's row of s_cnt before its first barrier,
+  // and writes s_net only after two, so passes need no barrier between them)
+  for (int pass = 0; pass < 4; pass++)
+    lds_pass<NW>(x, kUse, wbase, 32 + 8 * pass, s_net, s_cnt, s_wsum);
 }
 
 // Sorts the run of L <= NW * 64 * kRunPer boxes at position i by (low half,
@@ -2239,6 +2249,337 @@ int build_f32(Context& ctx, curtree_bbox_f32* box, curtree_rtree_f32* rt, int ke
 }
 
 // ---------------------------------------------------------------------------
+// Canonical pair order by segments.  Query results arrive in (query, sorted
+// box position) order: query ids never decrease, so the pairs of one query
+// are one contiguous segment and the canonical (query id, box id) order is
+// each segment's box ids sorted in place (the query ids stay where they are).
+//   k_seg_find    one pass over the query ids: segment heads, each segment's
+//                 length (a galloping search for its end), a flag if a query
+//                 id ever decreases, and the segments of 2 or more pairs listed
+//                 by length class;
+//   k_seg_wave    segments of 2..1024 pairs, one wave each (up to 64 a bitonic
+//                 network over the lanes, longer ones four LSD passes in
+//                 registers and the wave's LDS);
+//   k_seg_block   1025..4096 pairs, one block each (lds_lsd);
+//   k_seg_huge    longer segments, one block each: the four digit histograms
+//                 in one read, then four stable LSD passes between the box ids
+//                 and a work array, in chunks of 4096 ranked in LDS.
+// Box ids are sorted as unsigned keys (id ^ 0x80000000), i.e. in signed order.
+constexpr int kSegWaveMax = 1024;
+constexpr int kSegBlockMax = 4096;
+struct SegCtrl {
+  unsigned long long nseg[3];  // listed segments: wave, block, huge class
+  uint32_t unordered;          // a query id is below the one before it
+  uint32_t pad;
+};
+struct SegBufs {
+  uint64_t* small;             // wave class: start << 11 | length (length <= 1024)
+  uint64_t* big;               // block and huge classes: (start, length) pairs
+  uint64_t cap_small, cap_big;  // entries of each list (the huge class from the back)
+};
+
+__global__ void k_seg_init(SegCtrl* c) {
+  if (threadIdx.x < 3) c->nseg[threadIdx.x] = 0;
+  if (threadIdx.x == 0) c->unordered = 0;
+}
+
+// End of the segment of query id v that starts at i: the first position
+// after i whose id differs (ids of one segment are contiguous), by doubling
+// steps and a binary search between the last equal and the first different.
+__device__ __forceinline__ int64_t seg_end(const int* __restrict__ tid, int64_t n, int64_t i,
+                                           int v) {
+  int64_t lo = i, step = 1, hi;
+  for (;;) {
+    hi = lo + step;
+    if (hi >= n) {
+      hi = n;
+      break;
+    }
+    if (tid[hi] != v) break;
+    lo = hi;
+    step <<= 1;
+  }
+  // tid[lo] == v, and hi == n or tid[hi] != v
+  while (hi - lo > 1) {
+    const int64_t mid = lo + ((hi - lo) >> 1);
+    if (tid[mid] == v) lo = mid;
+    else hi = mid;
+  }
+  return hi;
+}
+
+__global__ __launch_bounds__(256) void k_seg_find(int64_t n, const int* __restrict__ tid,
+                                                  SegCtrl* c, SegBufs b) {
+  const int lane = lane_id();
+  const uint64_t lt = lanemask_lt();
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  // (the loop bound is uniform across the wave: every lane takes part in the
+  // ballots)
+  for (int64_t base = blockIdx.x * 256ll; base < n; base += stride) {
+    const int64_t i = base + threadIdx.x;
+    int cls = -1;
+    int64_t len = 0;
+    if (i < n) {
+      const int v = tid[i];
+      const int prev = i > 0 ? tid[i - 1] : v;
+      if (i == 0 || prev != v) {
+        if (prev > v) c->unordered = 1u;
+        len = seg_end(tid, n, i, v) - i;
+        cls = len < 2 ? -1 : len <= kSegWaveMax ? 0 : len <= kSegBlockMax ? 1 : 2;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const uint64_t m = ballot(cls == k);
+      if (!m) continue;
+      const int leader = __ffsll((unsigned long long)m) - 1;
+      unsigned long long at = 0;
+      if (lane == leader) at = atomicAdd(&c->nseg[k], (unsigned long long)popc64(m));
+      at = __shfl(at, leader);
+      if (cls == k) {
+        const uint64_t r = at + (uint64_t)popc64(m & lt);
+        if (k == 0) {
+          if (r < b.cap_small) b.small[r] = ((uint64_t)i << 11) | (uint64_t)len;
+        } else {
+          // the block class from the front, the huge class from the back
+          const uint64_t slot = k == 1 ? r : b.cap_big - 1 - r;
+          if (r < b.cap_big) {
+            b.big[2 * slot] = (uint64_t)i;
+            b.big[2 * slot + 1] = (uint64_t)len;
+          }
+        }
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ uint32_t id_key(int v) { return (uint32_t)v ^ 0x80000000u; }
+__device__ __forceinline__ int key_id(uint32_t k) { return (int)(k ^ 0x80000000u); }
+
+// Segments of 2..1024 box ids, one wave each.
+__global__ __launch_bounds__(256) void k_seg_wave(const SegCtrl* c, SegBufs b,
+                                                  int* __restrict__ fid) {
+  __shared__ uint32_t s_w[4][kSegWaveMax];
+  __shared__ __align__(16) uint32_t s_c[4][256];
+  const uint64_t nseg = c->nseg[0] < b.cap_small ? c->nseg[0] : b.cap_small;
+  const int w = threadIdx.x >> 6, lane = lane_id();
+  const uint64_t lt = lanemask_lt();
+  for (uint64_t sg = (uint64_t)blockIdx.x * 4 + w; sg < nseg; sg += (uint64_t)gridDim.x * 4) {
+    const uint64_t e = b.small[sg];
+    const int64_t st = (int64_t)(e >> 11);
+    const int L = (int)(e & 2047u);
+    if (L <= 64) {
+      // bitonic network over the lanes (lanes past the segment hold the
+      // largest key and stay at the end)
+      uint32_t x = lane < L ? id_key(fid[st + lane]) : 0xFFFFFFFFu;
+#pragma unroll
+      for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+          const uint32_t y = (uint32_t)__shfl_xor((int)x, j);
+          const bool up = (lane & k) == 0;   // this block of k sorts ascending
+          const bool low = (lane & j) == 0;  // this lane keeps the smaller one
+          const uint32_t mn = x < y ? x : y, mx = x < y ? y : x;
+          x = (up == low) ? mn : mx;
+        }
+      }
+      if (lane < L) fid[st + lane] = key_id(x);
+      continue;
+    }
+    // four LSD passes: item k of this lane is position 64 * k + lane
+    const int kUse = (L + 63) >> 6;
+    uint32_t x[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      const int p = 64 * k + lane;
+      x[k] = (k < kUse && p < L) ? id_key(fid[st + p]) : 0xFFFFFFFFu;
+    }
+    for (int pass = 0; pass < 4; pass++) {
+      const int sh = 8 * pass;
+#pragma unroll
+      for (int k = 0; k < 4; k++) s_c[w][lane + 64 * k] = 0;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      uint32_t rank[16];
+#pragma unroll
+      for (int k = 0; k < 16; k++) {
+        if (k < kUse) {
+          const uint32_t d = (x[k] >> sh) & 255u;
+          uint64_t m = ~0ull;
+#pragma unroll
+          for (int bit = 0; bit < 8; bit++) {
+            const uint64_t bb = ballot((d >> bit) & 1u);
+            m &= ((d >> bit) & 1u) ? bb : ~bb;
+          }
+          const uint32_t before = s_c[w][d];
+          rank[k] = before + (uint32_t)popc64(m & lt);
+          if ((m & lt) == 0) s_c[w][d] = before + (uint32_t)popc64(m);
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+        }
+      }
+      const uint4 cnt = *(const uint4*)&s_c[w][4 * lane];
+      const uint32_t tot = cnt.x + cnt.y + cnt.z + cnt.w;
+      uint32_t inc = tot;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t u = __shfl_up(inc, d);
+        if (lane >= d) inc += u;
+      }
+      const uint32_t o = inc - tot;
+      *(uint4*)&s_c[w][4 * lane] =
+          make_uint4(o, o + cnt.x, o + cnt.x + cnt.y, o + cnt.x + cnt.y + cnt.z);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int k = 0; k < 16; k++)
+        if (k < kUse) s_w[w][s_c[w][(x[k] >> sh) & 255u] + rank[k]] = x[k];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int k = 0; k < 16; k++)
+        if (k < kUse) x[k] = s_w[w][64 * k + lane];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      const int p = 64 * k + lane;
+      if (k < kUse && p < L) fid[st + p] = key_id(x[k]);
+    }
+  }
+}
+
+// Segments of 1025..4096 box ids, one block of four waves each.
+__global__ __launch_bounds__(256) void k_seg_block(const SegCtrl* c, SegBufs b,
+                                                   int* __restrict__ fid) {
+  __shared__ uint64_t s_net[kSegBlockMax];
+  __shared__ __align__(16) uint32_t s_cnt[4][256];
+  __shared__ uint32_t s_wsum[4];
+  const uint64_t nseg = c->nseg[1] < b.cap_big ? c->nseg[1] : b.cap_big;
+  for (uint64_t sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
+    const int64_t st = (int64_t)b.big[2 * sg];
+    const int L = (int)b.big[2 * sg + 1];
+    const int kUse = (L + 255) / 256;
+    const int wbase = (threadIdx.x >> 6) * 64 * kUse + lane_id();
+    uint64_t x[kRunPer];
+#pragma unroll
+    for (int k = 0; k < kRunPer; k++) {
+      const int p = wbase + 64 * k;
+      x[k] = (k < kUse && p < L) ? ((uint64_t)id_key(fid[st + p]) << 32) | (uint32_t)p : ~0ull;
+    }
+    lds_lsd<4>(x, kUse, wbase, s_net, s_cnt, s_wsum);
+#pragma unroll
+    for (int k = 0; k < kRunPer; k++) {
+      const int p = wbase + 64 * k;
+      if (k < kUse && p < L) fid[st + p] = key_id((uint32_t)(x[k] >> 32));
+    }
+    __syncthreads();  // s_net and s_cnt are reused by the next segment
+  }
+}
+
+// Segments longer than 4096 box ids, one block each: stable LSD passes
+// between fid and the work array `tmp` (same positions).  A digit that every
+// id of the segment shares is skipped; an odd number of passes ends with a
+// copy back.
+__global__ __launch_bounds__(256) void k_seg_huge(const SegCtrl* c, SegBufs b,
+                                                  int* __restrict__ fid,
+                                                  uint32_t* __restrict__ tmp) {
+  __shared__ uint64_t s_net[kSegBlockMax];
+  __shared__ __align__(16) uint32_t s_cnt[4][256];
+  __shared__ uint32_t s_wsum[4];
+  __shared__ uint32_t s_hist[4][256];
+  __shared__ uint32_t s_base[256];
+  __shared__ int s_skip[4];
+  const int t = threadIdx.x;
+  const uint64_t nseg = c->nseg[2] < b.cap_big ? c->nseg[2] : b.cap_big;
+  for (uint64_t sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
+    const uint64_t slot = b.cap_big - 1 - sg;
+    const int64_t st = (int64_t)b.big[2 * slot];
+    const int64_t L = (int64_t)b.big[2 * slot + 1];
+#pragma unroll
+    for (int p = 0; p < 4; p++) s_hist[p][t] = 0;
+    __syncthreads();
+    for (int64_t i = t; i < L; i += 256) {
+      const uint32_t k = id_key(fid[st + i]);
+#pragma unroll
+      for (int p = 0; p < 4; p++) atomicAdd(&s_hist[p][(k >> (8 * p)) & 255u], 1u);
+    }
+    __syncthreads();
+    if (t < 4) s_skip[t] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < 4; p++)
+      if (s_hist[p][t] == (uint32_t)L) s_skip[p] = 1;  // digit p is the same for all
+    __syncthreads();
+    // the keys: in fid as raw ids before the first pass that runs, then as
+    // keys in the buffer the last pass wrote
+    bool in_tmp = false, fid_raw = true;
+    for (int p = 0; p < 4; p++) {
+      if (s_skip[p]) continue;
+      // exclusive offsets of digit p within the segment
+      {
+        const uint32_t v = s_hist[p][t];
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan<4>(v, s_wsum, &tot);
+        s_base[t] = ex;
+      }
+      __syncthreads();
+      for (int64_t c0 = 0; c0 < L; c0 += kSegBlockMax) {
+        const int Lc = (int)(L - c0 < kSegBlockMax ? L - c0 : kSegBlockMax);
+        const int kUse = (Lc + 255) / 256;
+        const int wbase = (t >> 6) * 64 * kUse + lane_id();
+        uint64_t x[kRunPer];
+#pragma unroll
+        for (int k = 0; k < kRunPer; k++) {
+          const int q = wbase + 64 * k;
+          uint32_t key = 0;
+          if (k < kUse && q < Lc)
+            key = in_tmp    ? tmp[st + c0 + q]
+                  : fid_raw ? id_key(fid[st + c0 + q])
+                            : (uint32_t)fid[st + c0 + q];
+          x[k] = (k < kUse && q < Lc) ? ((uint64_t)key << 32) | (uint32_t)q : ~0ull;
+        }
+        lds_pass<4>(x, kUse, wbase, 32 + 8 * p, s_net, s_cnt, s_wsum);
+        // item at chunk position q: digit d, output base[d] + q - start of d
+        // in the chunk (the padding sorts after every item of the chunk)
+#pragma unroll
+        for (int k = 0; k < kRunPer; k++) {
+          const int q = wbase + 64 * k;
+          if (k < kUse && q < Lc) {
+            const uint32_t key = (uint32_t)(x[k] >> 32);
+            const uint32_t d = (key >> (8 * p)) & 255u;
+            const int64_t o = st + (int64_t)s_base[d] + (q - (int)s_cnt[0][d]);
+            if (in_tmp) fid[o] = (int)key;  // (a key; converted after the last pass)
+            else tmp[o] = key;
+          }
+        }
+        __syncthreads();
+        // the digits' next offsets: this chunk's items of digit t
+        {
+          const uint32_t s0 = s_cnt[0][t];
+          const uint32_t s1 = t < 255 ? s_cnt[0][t + 1] : (uint32_t)Lc;
+          // (digit 255 of the padding sorts after the real ones)
+          const uint32_t cnt = (t < 255 ? s1 : (s1 < s0 ? s0 : s1)) - s0;
+          s_base[t] += cnt < (uint32_t)Lc ? cnt : (uint32_t)Lc;
+        }
+        __syncthreads();
+      }
+      if (in_tmp) fid_raw = false;  // this pass wrote keys into fid
+      in_tmp = !in_tmp;
+    }
+    // keys -> ids: the last pass left keys in tmp (copy back) or in fid
+    if (in_tmp || !fid_raw) {
+      for (int64_t i = t; i < L; i += 256) {
+        const uint32_t key = in_tmp ? tmp[st + i] : (uint32_t)fid[st + i];
+        fid[st + i] = key_id(key);
+      }
+    }
+    __syncthreads();  // the shared arrays are reused by the next segment
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Canonical pair order: (query id, box id) lexicographic, as the reference's
 // two host stable sorts leave it (src/rtree_test_frontier.cu:102-105).  Pairs
 // are packed into one 64-bit key (query id high, box id with its sign bit
@@ -2277,12 +2618,11 @@ __global__ __launch_bounds__(256) void k_pair_unpack(int64_t n, const BuildCtrl*
   fid[i] = (int)((uint32_t)k ^ 0x80000000u);
 }
 
-int pairs_canonical_sort(Context& ctx, int* fid, int* tid, int64_t n, hipStream_t s) {
-  if (n < 0) return set_error(CURTREE_EINVAL, "canonical sort: negative count");
-  if (n <= 1) return CURTREE_OK;
-  if (!fid || !tid) return set_error(CURTREE_EINVAL, "canonical sort: null array");
+// Any order of pairs: a full radix sort of (query id, box id) keys.
+static int pairs_full_sort(Context& ctx, int* fid, int* tid, int64_t n, hipStream_t s) {
   if (n >= (int64_t)1 << 30)
-    return set_error(CURTREE_ERANGE, "canonical sort: at most 2^30-1 pairs per call");
+    return set_error(CURTREE_ERANGE,
+                     "canonical sort: pairs not in query order take at most 2^30-1 per call");
   const int64_t ntiles = (n + kSortTile - 1) / kSortTile;
   const size_t status_words = (size_t)8 * ntiles * 256;
   Workspace& ws = ctx.ws;
@@ -2314,6 +2654,58 @@ int pairs_canonical_sort(Context& ctx, int* fid, int* tid, int64_t n, hipStream_
   pf.mark(s, "k_pair_unpack");
   hipLaunchKernelGGL(k_pair_unpack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, c, k0,
                      k1, fid, tid);
+  pf.mark(s, nullptr);
+  CT_HIP(hipGetLastError());
+  return CURTREE_OK;
+}
+
+int pairs_canonical_sort(Context& ctx, int* fid, int* tid, int64_t n, hipStream_t s) {
+  if (n < 0) return set_error(CURTREE_EINVAL, "canonical sort: negative count");
+  if (n <= 1) return CURTREE_OK;
+  if (!fid || !tid) return set_error(CURTREE_EINVAL, "canonical sort: null array");
+  Workspace& ws = ctx.ws;
+  int rc;
+  // lists: wave-class segments (8 B, at most n/2 of them), longer ones (16 B,
+  // at most n/1025); the huge class's work array (4 B per pair)
+  SegBufs b;
+  b.cap_small = (uint64_t)(n / 2 + 1);
+  b.cap_big = (uint64_t)(n / (kSegWaveMax + 1) + 2);
+  if ((rc = ws.reserve(WsSortKeys0, (size_t)n * 4)) ||
+      (rc = ws.reserve(WsSortKeys1, (size_t)(b.cap_small * 8 + b.cap_big * 16))) ||
+      (rc = ws.reserve(WsSortCtrl, sizeof(SegCtrl) > sizeof(BuildCtrl) ? sizeof(SegCtrl)
+                                                                         : sizeof(BuildCtrl))))
+    return rc;
+  SegCtrl* c = (SegCtrl*)ws.ptr(WsSortCtrl);
+  b.small = (uint64_t*)ws.ptr(WsSortKeys1);
+  b.big = b.small + b.cap_small;
+  Prof& pf = ctx.prof;
+  pf.mark(s, "k_seg_find");
+  hipLaunchKernelGGL(k_seg_init, dim3(1), dim3(64), 0, s, c);
+  hipLaunchKernelGGL(k_seg_find, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 8192)),
+                     dim3(256), 0, s, n, (const int*)tid, c, b);
+  pf.mark(s, nullptr);
+  CT_HIP(hipGetLastError());
+  SegCtrl hc;
+  CT_HIP(hipMemcpyAsync(&hc, c, sizeof hc, hipMemcpyDeviceToHost, s));
+  CT_HIP(hipStreamSynchronize(s));
+  // query ids that ever decrease (e.g. the self-join's box ids): the full sort
+  if (hc.unordered) return pairs_full_sort(ctx, fid, tid, n, s);
+  if (hc.nseg[0]) {
+    pf.mark(s, "k_seg_wave");
+    hipLaunchKernelGGL(k_seg_wave, dim3((unsigned)std::min<uint64_t>((hc.nseg[0] + 3) / 4, 2048)),
+                       dim3(256), 0, s, (const SegCtrl*)c, b, fid);
+  }
+  if (hc.nseg[1]) {
+    pf.mark(s, "k_seg_block");
+    hipLaunchKernelGGL(k_seg_block, dim3((unsigned)std::min<uint64_t>(hc.nseg[1], 1024)),
+                       dim3(256), 0, s, (const SegCtrl*)c, b, fid);
+  }
+  if (hc.nseg[2]) {
+    pf.mark(s, "k_seg_huge");
+    hipLaunchKernelGGL(k_seg_huge, dim3((unsigned)std::min<uint64_t>(hc.nseg[2], 768)),
+                       dim3(256), 0, s, (const SegCtrl*)c, b, fid,
+                       (uint32_t*)ws.ptr(WsSortKeys0));
+  }
   pf.mark(s, nullptr);
   CT_HIP(hipGetLastError());
   return CURTREE_OK;

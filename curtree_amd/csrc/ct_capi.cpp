@@ -201,6 +201,7 @@ CURTREE_API int curtree_query_f32(const curtree_rtree_f32* rt, const curtree_bbo
   std::lock_guard<std::mutex> g(c.mu);
   QueryOut qo;
   qo.alloc = true;
+  qo.algo = algo;
   int rc = query_frontier_f32(c, rt, rbox, qbox, &qo, as_stream(stream));
   if (rc) return rc;
   CT_HIP(hipStreamSynchronize(as_stream(stream)));
@@ -221,6 +222,7 @@ CURTREE_API int curtree_query_into_f32(const curtree_rtree_f32* rt, const curtre
   qo.fid = fid;
   qo.tid = tid;
   qo.cap = cap;
+  qo.algo = algo;
   int rc = query_frontier_f32(c, rt, rbox, qbox, &qo, as_stream(stream));
   if (n_out) *n_out = qo.n;
   if (rc) return rc;
@@ -497,6 +499,11 @@ CURTREE_API int curtree_set_option(const char* name, int64_t value) {
   if (!strcmp(name, "expand_split")) {
     if (value < -1 || value > 1) return set_error(CURTREE_EINVAL, "expand_split: -1, 0 or 1");
     qtuning().expand = (int)value;
+    return CURTREE_OK;
+  }
+  if (!strcmp(name, "leaf_fused")) {
+    if (value < -1 || value > 1) return set_error(CURTREE_EINVAL, "leaf_fused: -1, 0 or 1");
+    qtuning().fused = (int)value;
     return CURTREE_OK;
   }
   if (!strcmp(name, "query_grid")) {

@@ -174,6 +174,7 @@ struct QEngine {
   // (the allocation-free entry point's scratch); a request beyond a buffer's
   // capacity fails with CURTREE_ERANGE instead of allocating
   bool fixed = false;
+  bool wide = false;                    // tile counts hold one word per entry (fanout > 64)
   bool needs_clear = true;              // status/ctrl are fresh: clear on the next run
   int last_nl = 0;                      // depths of the last run read back (host_ctrl)
   int reserve_front(int64_t entries);
@@ -223,6 +224,7 @@ struct QueryOut {
   int64_t* counts = nullptr;  // optional per-query counts (device)
   bool count_only = false;
   bool contain = true;        // use the containment shortcut (set by the entry point)
+  int algo = CURTREE_ALGO_FRONTIER;  // CURTREE_ALGO_PERSISTENT: chained launches only
   // results
   int64_t n = 0;
   int* out_fid = nullptr;

@@ -308,6 +308,74 @@ __device__ __forceinline__ uint64_t block_excl_scan64(uint64_t v, uint64_t* s_tm
   return x - v + off;
 }
 
+// The entries of one lane group for STEPS wave-steps (entry e0 + k*E at step
+// k; past the frontier: its last entry, unused; fin == null: the implicit
+// first frontier, window q0 + e at the root).  Groups of 16 lanes or more:
+// the first lane of each row of 16 loads and DPP broadcasts to the row (every
+// row of a group loads the same entry), so a wave issues a quarter-row of
+// loads instead of 64 identical ones per group.
+template <int G, int STEPS, int PERWAVE, int E>
+__device__ __forceinline__ void load_entries(uint2 (&ent)[STEPS], const uint2* __restrict__ fin,
+                                             int64_t e0, int64_t n_in, int64_t q0) {
+  if (!fin) {
+#pragma unroll
+    for (int k = 0; k < STEPS; k++) {
+      const int64_t e = e0 + k * E;
+      ent[k] = make_uint2((uint32_t)(q0 + (e < n_in ? e : n_in - 1)), 0u);
+    }
+    return;
+  }
+  if constexpr (G >= 16) {
+    uint2 raw[STEPS];
+#pragma unroll
+    for (int k = 0; k < STEPS; k++) raw[k] = make_uint2(0u, 0u);
+    if ((lane_id() & 15) == 0) {
+#pragma unroll
+      for (int k = 0; k < STEPS; k++) {
+        const int64_t e = e0 + k * E;
+        raw[k] = fin[e < n_in ? e : n_in - 1];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < STEPS; k++) ent[k] = make_uint2(row_first(raw[k].x), row_first(raw[k].y));
+  } else {
+#pragma unroll
+    for (int k = 0; k < STEPS; k++) {
+      const int64_t e = e0 + k * E;
+      ent[k] = fin[e < n_in ? e : n_in - 1];
+    }
+  }
+}
+
+// The windows of H entries (queries q[k], the same in every lane of a group):
+// load_windows issues the loads (groups of 16 lanes or more: the first lane of
+// each row only), bcast_windows completes them (the row broadcast), so other
+// loads can be issued between the two.
+template <int G, int H>
+__device__ __forceinline__ void load_windows(float4 (&wq)[H], const QBoxes& qb,
+                                             const uint32_t (&q)[H]) {
+  if constexpr (G >= 16) {
+#pragma unroll
+    for (int k = 0; k < H; k++) wq[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if ((lane_id() & 15) == 0) {
+#pragma unroll
+      for (int k = 0; k < H; k++) wq[k] = ld_box(qb, q[k]);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < H; k++) wq[k] = ld_box(qb, q[k]);
+  }
+}
+template <int G, int H>
+__device__ __forceinline__ void bcast_windows(float4 (&wq)[H]) {
+  if constexpr (G >= 16) {
+#pragma unroll
+    for (int k = 0; k < H; k++)
+      wq[k] = make_float4(row_first(wq[k].x), row_first(wq[k].y), row_first(wq[k].z),
+                          row_first(wq[k].w));
+  }
+}
+
 #ifndef CT_LEAF_H
 #define CT_LEAF_H 2
 #endif
@@ -585,19 +653,7 @@ __global__ __launch_bounds__(kQThreads) void k_expand_count(int slot, QTree tr, 
     // the tile's entries (a position past the frontier reads its last entry,
     // unused)
     uint2 ent[STEPS];
-    if (fin) {
-#pragma unroll
-      for (int k = 0; k < STEPS; k++) {
-        const int64_t e = tbase + w * S::kPerWave + k * S::E + sub;
-        ent[k] = fin[e < n_in ? e : n_in - 1];
-      }
-    } else {
-#pragma unroll
-      for (int k = 0; k < STEPS; k++) {
-        const int64_t e = tbase + w * S::kPerWave + k * S::E + sub;
-        ent[k] = make_uint2((uint32_t)(q0 + (e < n_in ? e : n_in - 1)), 0u);
-      }
-    }
+    load_entries<G, STEPS, S::kPerWave, S::E>(ent, fin, tbase + w * S::kPerWave + sub, n_in, q0);
 #pragma unroll
     for (int k0 = 0; k0 < STEPS; k0 += H) {
       const uint2* en = ent + k0;
@@ -629,16 +685,20 @@ __global__ __launch_bounds__(kQThreads) void k_expand_count(int slot, QTree tr, 
       float4 wq[H], mb[H];
       uint8_t sf[H];
       bool test[H];
+      uint32_t qv[H];
+#pragma unroll
+      for (int k = 0; k < H; k++) qv[k] = en[k].x;
+      load_windows<G, H>(wq, qb, qv);
 #pragma unroll
       for (int k = 0; k < H; k++) {
         const bool full = (en[k].y & kFull) != 0;
         // children must be nodes (a corrupt pos/len is never followed)
         test[k] = !full && p[k] >= 0 && c_ < L[k] && (int64_t)p[k] + L[k] <= r_sz;
         const uint32_t child = test[k] ? (uint32_t)(p[k] + c_) : 0u;
-        wq[k] = ld_box(qb, en[k].x);
         mb[k] = ld_box(tr.mbr, child);
         sf[k] = shortcut ? tr.safe[child] : 0;
       }
+      bcast_windows<G, H>(wq);
 #pragma unroll
       for (int k = 0; k < H; k++) {
         const int64_t e = tbase + w * S::kPerWave + (k0 + k) * S::E + sub;
@@ -923,6 +983,239 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_write(
   }
 }
 
+// Leaf stage in one pass (writing runs): the boxes of every entry are tested
+// once, the tile's pair total is published through a decoupled look-back
+// (tiles taken in order from a ticket), and every hit is stored straight from
+// the registers that tested it -- the box id is loaded beside the box, and the
+// hits of one wave-step are consecutive output slots, so the stores of a step
+// are one contiguous run.  Contained ranges are written by the block after the
+// partial entries (short ones flat and load-balanced, long ones cut into
+// pieces for k_write_pieces).  QCOUNTS: per-query counts as k_leaf_count.
+// The last tile leaves the run's pair total in c->total.
+template <int G, int STEPS, bool QCOUNTS>
+__global__ __launch_bounds__(kQThreads) void k_leaf_fused(
+    int slot, QTree tr, QBoxes qb, QBoxes rb, QCtrl* c, const uint2* __restrict__ fin,
+    int64_t r_sz, int64_t nbox, const int* __restrict__ rid, int* __restrict__ out_fid,
+    int* __restrict__ out_tid, int64_t out_cap, uint64_t* status, QPiece* __restrict__ pieces,
+    int64_t piece_cap, unsigned long long* counts) {
+  using S = QShape<G, STEPS>;
+  constexpr int P = S::kPerTile;
+  constexpr int H = STEPS < CT_LEAF_H ? STEPS : CT_LEAF_H;
+  static_assert(STEPS % H == 0, "steps in whole groups");
+  constexpr int R = (P + kQThreads - 1) / kQThreads;  // entries per thread in the scans
+  __shared__ uint32_t s_tile;
+  __shared__ uint64_t s_base;
+  __shared__ uint64_t s_scan[kQWaves];
+  __shared__ uint64_t s_off[P];    // tile-relative first output slot of each entry
+  __shared__ uint32_t s_cnt[P];    // pairs of each entry (ranges: their length)
+  __shared__ uint32_t s_q[P];      // query | kQRange for ranges, kQNone past the end
+  __shared__ uint32_t s_first[P];  // ranges: first sorted box
+  __shared__ uint32_t s_foff[P];   // ranges up to kFlatMax: tile-relative flat offset
+  __shared__ uint32_t s_ftot;
+  __shared__ QLevels s_lv;
+  if (c->overflow) return;
+  const int64_t n_in = (int64_t)c->count[slot];
+  const uint32_t epoch = c->epoch_base + slot;
+  const int64_t q0 = c->q0;
+  const int64_t ntiles = (n_in + P - 1) / P;
+  const int t = threadIdx.x, w = t >> 6, lane = lane_id();
+  if (ntiles == 0) return;  // the run's total stays 0
+  for (int i = t; i < (int)(sizeof(QLevels) / 4); i += kQThreads)
+    ((uint32_t*)&s_lv)[i] = ((const uint32_t*)&c->lv)[i];
+  const int sub = lane / G, c_ = lane % G;
+  const int lead = sub * G;
+  const uint64_t below = (1ull << c_) - 1ull;  // group lanes before this one
+  const bool canon = tr.canon && *tr.canon;
+  const int64_t ls = c->lv.lstart[0];
+  int64_t tile;
+  while ((tile = next_tile(&c->ticket[slot], ntiles, &s_tile)) >= 0) {
+    const int64_t tbase = tile * P;
+    uint2 ent[STEPS];
+    load_entries<G, STEPS, S::kPerWave, S::E>(ent, fin, tbase + w * S::kPerWave + sub, n_in, q0);
+    // kept for the stores: this lane's group mask and box id per step
+    uint64_t gmk[STEPS];
+    int idk[STEPS];
+#pragma unroll
+    for (int k0 = 0; k0 < STEPS; k0 += H) {
+      uint32_t q[H];
+      int first[H], cnt[H];  // contained: range length; partial: children
+      bool full[H];
+      const uint2* en = ent + k0;
+#pragma unroll
+      for (int k = 0; k < H; k++) {
+        q[k] = en[k].x;
+        const uint32_t node = en[k].y & ~kFull;
+        full[k] = (en[k].y & kFull) != 0;
+        if (full[k]) {
+          int64_t f, n;
+          node_range(node, &s_lv, nbox, &f, &n);
+          first[k] = (int)f;
+          cnt[k] = (int)n;
+        } else {
+          const int64_t f = ((int64_t)node - ls) * tr.F;
+          first[k] = (int)f;
+          cnt[k] = (int)(nbox - f < tr.F ? nbox - f : tr.F);
+        }
+      }
+      if (!canon) {  // any other layout: the node's pos/len
+        int pos[H], len[H];
+#pragma unroll
+        for (int k = 0; k < H; k++) {
+          const uint32_t node = en[k].y & ~kFull;
+          pos[k] = tr.pos[node];
+          len[k] = tr.len[node];
+        }
+#pragma unroll
+        for (int k = 0; k < H; k++)
+          if (!full[k]) {
+            first[k] = (int)((int64_t)pos[k] - r_sz);
+            cnt[k] = len[k];
+          }
+      }
+      float4 wq[H], bx[H];
+      int id[H];
+      bool test[H];
+      load_windows<G, H>(wq, qb, q);
+#pragma unroll
+      for (int k = 0; k < H; k++) {
+        const int bi = first[k] + c_;
+        test[k] = !full[k] && c_ < cnt[k] && bi >= 0 && bi < nbox;  // children must be boxes
+        bx[k] = ld_box(rb, test[k] ? bi : 0);
+        id[k] = rid[test[k] ? bi : 0];
+      }
+      bcast_windows<G, H>(wq);
+#pragma unroll
+      for (int k = 0; k < H; k++) {
+        const int el = w * S::kPerWave + (k0 + k) * S::E + sub;  // entry in the tile
+        const bool valid = tbase + el < n_in;
+        const bool hit = valid && test[k] &&
+                         intersect(wq[k].x, wq[k].y, wq[k].z, wq[k].w, bx[k].x, bx[k].y, bx[k].z,
+                                   bx[k].w);
+        const uint64_t gm = (ballot(hit) >> lead) & S::kGroupMask;
+        gmk[k0 + k] = gm;
+        idk[k0 + k] = id[k];
+        if (c_ == 0) {
+          s_cnt[el] = !valid ? 0u : full[k] ? (uint32_t)cnt[k] : (uint32_t)popc64(gm);
+          s_q[el] = !valid ? kQNone : (q[k] | (full[k] ? kQRange : 0u));
+          s_first[el] = (uint32_t)first[k];
+        }
+      }
+    }
+    __syncthreads();
+    // tile-relative offsets of every entry and flat offsets of short ranges
+    uint64_t loc[R];
+    uint64_t sum = 0, fsum = 0;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int li = t * R + r;
+      loc[r] = li < P ? s_cnt[li] : 0;
+      sum += loc[r];
+      if (li < P && (s_q[li] & kQRange) && s_q[li] != kQNone && loc[r] <= kFlatMax)
+        fsum += loc[r];
+    }
+    uint64_t ttot, ftot;
+    uint64_t ex = block_excl_scan64(sum, s_scan, &ttot);
+    uint64_t fex = block_excl_scan64(fsum, s_scan, &ftot);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int li = t * R + r;
+      if (li < P) {
+        s_off[li] = ex;
+        s_foff[li] = (uint32_t)fex;
+        if ((s_q[li] & kQRange) && s_q[li] != kQNone && loc[r] <= kFlatMax) fex += loc[r];
+      }
+      ex += loc[r];
+    }
+    // the tile's base: decoupled look-back over the tiles before it
+    if (w == 0) {
+      uint64_t excl = 0;
+      if (tile == 0) {
+        if (lane == 0) st_agent(status, st_pack(2u * epoch + 2u, ttot));
+      } else {
+        if (lane == 0) st_agent(status + tile, st_pack(2u * epoch + 1u, ttot));
+        excl = lookback64(status, tile, epoch);
+        if (lane == 0) st_agent(status + tile, st_pack(2u * epoch + 2u, excl + ttot));
+      }
+      if (lane == 0) {
+        s_base = excl;
+        s_ftot = (uint32_t)ftot;
+        if (tile == ntiles - 1) c->total = excl + ttot;
+      }
+    }
+    __syncthreads();
+    const uint64_t base = s_base;
+    // partial entries: each hit from its own lane (ranges have no hit bits)
+#pragma unroll
+    for (int k = 0; k < STEPS; k++) {
+      const uint64_t gm = gmk[k];
+      if ((gm >> c_) & 1ull) {
+        const int el = w * S::kPerWave + k * S::E + sub;
+        const uint64_t o = base + s_off[el] + (uint64_t)popc64(gm & below);
+        if (o < (uint64_t)out_cap) {
+          out_fid[o] = idk[k];
+          out_tid[o] = (int)(s_q[el]);
+        }
+      }
+    }
+    // contained ranges: long ones as pieces, short ones flat over the block
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int li = t * R + r;
+      if (li >= P) break;
+      const uint32_t qf = s_q[li];
+      if (qf == kQNone || !(qf & kQRange) || s_cnt[li] <= kFlatMax) continue;
+      const uint64_t ev = s_cnt[li];
+      const uint64_t off = base + s_off[li];
+      const uint64_t np = (ev + kPiece - 1) / kPiece;
+      const uint64_t at = atomicAdd(&c->npieces, (unsigned long long)np);
+      for (uint64_t p = 0; p < np; p++) {
+        if (at + p >= (uint64_t)piece_cap) break;  // only when the total exceeds out_cap
+        const uint64_t rest = ev - p * kPiece;
+        QPiece pc;
+        pc.off = off + p * kPiece;
+        pc.first = (int64_t)s_first[li] + (int64_t)(p * kPiece);
+        pc.len = (uint32_t)(rest < kPiece ? rest : kPiece);
+        pc.q = qf & ~kQRange;
+        pieces[at + p] = pc;
+      }
+    }
+    const uint32_t fn = s_ftot;
+    for (uint32_t j = t; j < fn; j += kQThreads) {
+      int lo = 0;
+#pragma unroll
+      for (int step = P / 2; step > 0; step >>= 1)
+        if (s_foff[lo + step] <= j) lo += step;
+      // (entries that are not short ranges hold no flat slots: the search
+      // lands on the short range that holds slot j)
+      const uint32_t r = j - s_foff[lo];
+      const uint64_t o = base + s_off[lo] + r;
+      if (o < (uint64_t)out_cap) {
+        out_fid[o] = rid[(int64_t)s_first[lo] + r];
+        out_tid[o] = (int)(s_q[lo] & ~kQRange);
+      }
+    }
+    if (QCOUNTS) {
+      // per-query counts: sums over runs of one query's entries in the tile
+      const int nv = n_in - tbase < P ? (int)(n_in - tbase) : P;
+      uint64_t csum = 0;
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        if (t * R + r < nv) csum += s_cnt[t * R + r];
+      uint64_t tile_sum;
+      uint64_t before = block_excl_scan64(csum, s_scan, &tile_sum);
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int i = t * R + r;
+        if (i >= nv) break;
+        const uint32_t qi = s_q[i] & ~kQRange;
+        if (i == 0 || (s_q[i - 1] & ~kQRange) != qi) atomicAdd(counts + qi, 0ull - before);
+        before += s_cnt[i];
+        if (i + 1 == nv || (s_q[i + 1] & ~kQRange) != qi) atomicAdd(counts + qi, before);
+      }
+    }
+  }
+}
+
 // Long box ranges of contained subtrees: one block per piece, coalesced id
 // reads and pair writes.
 __global__ __launch_bounds__(256) void k_write_pieces(const QCtrl* c,
@@ -1013,7 +1306,7 @@ int QEngine::reserve_front(int64_t entries) {
   if ((rc = dev_alloc(&front[0], (size_t)entries * 8)) ||
       (rc = dev_alloc(&front[1], (size_t)entries * 8)) ||
       (rc = dev_alloc(&est, (size_t)entries * kEntryBytes)) ||
-      (rc = dev_alloc((void**)&ttot, (size_t)(entries / 16 + 2) * 8))) {
+      (rc = dev_alloc((void**)&ttot, (size_t)((wide ? entries : entries / 16) + 2) * 8))) {
     cap_front = 0;
     return rc;
   }
@@ -1291,8 +1584,11 @@ void enqueue_run_g(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
   // expansions' tile scans nl+1..2nl-1
   const unsigned nslots = 2u * (unsigned)a.nl;
   const int ex = qtuning().expand;
-  const bool split =
-      ex == 1 || (ex < 0 && (a.plan != nullptr || a.q1 - a.q0 >= kSplitMinQueries));
+  const bool small = a.plan == nullptr && a.q1 - a.q0 < kSplitMinQueries;
+  // the persistent algorithm: one chained launch per level, leaf stage included
+  const bool split = !a.chained && (ex == 1 || (ex < 0 && !small));
+  const int fz = qtuning().fused;
+  const bool fused = a.chained || fz == 1 || (fz < 0 && small);
   if (qe.needs_clear) {
     hipLaunchKernelGGL(k_ctrl_init, dim3(1), dim3(64), 0, s, qe.ctrl);
     qe.needs_clear = false;
@@ -1330,7 +1626,23 @@ void enqueue_run_g(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
     which ^= 1;
   }
   const int leaf = a.nl - 1;
-  if (a.write) {
+  if (a.write && fused) {
+    pf.mark(s, "k_leaf_fused");
+    if (a.counts)
+      hipLaunchKernelGGL((k_leaf_fused<G, STEPS, true>), dim3(a.grid), dim3(kQThreads), 0, s,
+                         leaf, a.tr, a.qb, a.rb, qe.ctrl, cur, a.r_sz, a.nbox, a.rid, a.out_fid,
+                         a.out_tid, a.out_cap, qe.status, (QPiece*)qe.pieces, qe.cap_pieces,
+                         a.counts);
+    else
+      hipLaunchKernelGGL((k_leaf_fused<G, STEPS, false>), dim3(a.grid), dim3(kQThreads), 0, s,
+                         leaf, a.tr, a.qb, a.rb, qe.ctrl, cur, a.r_sz, a.nbox, a.rid, a.out_fid,
+                         a.out_tid, a.out_cap, qe.status, (QPiece*)qe.pieces, qe.cap_pieces,
+                         (unsigned long long*)nullptr);
+    pf.mark(s, "k_write_pieces");
+    hipLaunchKernelGGL(k_write_pieces, dim3(4096), dim3(256), 0, s, (const QCtrl*)qe.ctrl,
+                       (const QPiece*)qe.pieces, qe.cap_pieces, a.rid, a.out_fid, a.out_tid,
+                       a.out_cap);
+  } else if (a.write) {
     pf.mark(s, "k_leaf_count");
     if (a.counts)
       hipLaunchKernelGGL((k_leaf_count<G, STEPS, true, true>),
@@ -1382,7 +1694,175 @@ static void enqueue_run_steps(QEngine& qe, Prof& pf, const QRunArgs& a, hipStrea
   }
 }
 
+// ---------------------------------------------------------------------------
+// Trees of fanout above 64 (one lane per child does not fit a wave): one wave
+// per frontier entry visits the entry's children 64 at a time.  Per level:
+//   k_gen_count  the children that intersect the entry's window (internal
+//                levels: child nodes, the leaf level: boxes), counted into
+//                cnt[entry]; the level's total by one atomic per wave, and
+//                per-query counts at the leaf level when asked;
+//   k_tile_scan  the counts, scanned in place (one count per "tile");
+//   k_gen_write  the same children again, stored compacted at the entry's
+//                offset: (query, child) entries of the next frontier, or
+//                (box id, query) pairs.
+// Output order is the entries' order, then child order, as on the other
+// paths; there is no containment shortcut here.
+template <bool LEAF>
+__device__ __forceinline__ void gen_children(const QTree& tr, const QCtrl* c, int slot,
+                                             uint32_t node, int64_t r_sz, int64_t nbox,
+                                             bool canon, int64_t* first, int64_t* len) {
+  const int lvl = LEAF ? 0 : c->lv.nl - 1 - slot;
+  const int64_t ls = c->lv.lstart[lvl];
+  if (canon) {
+    const int64_t f = ((int64_t)node - ls) * tr.F;
+    const int64_t cnt = LEAF ? nbox : c->lv.lcount[lvl - 1];
+    *first = (LEAF ? 0 : c->lv.lstart[lvl - 1]) + f;
+    *len = cnt - f < tr.F ? cnt - f : tr.F;
+  } else {
+    *first = LEAF ? (int64_t)tr.pos[node] - r_sz : (int64_t)tr.pos[node];
+    *len = tr.len[node];
+  }
+  // children must be boxes (leaf level) or nodes (a corrupt pos/len is never
+  // followed)
+  const int64_t lim = LEAF ? nbox : r_sz;
+  if (*first < 0 || *len < 0 || *first + *len > lim) *len = 0;
+}
+
+template <bool LEAF>
+__global__ __launch_bounds__(256) void k_gen_count(int slot, QTree tr, QBoxes qb, QBoxes rb,
+                                                   QCtrl* c, const uint2* __restrict__ fin,
+                                                   int64_t r_sz, int64_t nbox,
+                                                   uint64_t* __restrict__ cnt,
+                                                   unsigned long long* counts) {
+  if (c->overflow) return;
+  const int64_t n_in = (int64_t)c->count[slot];
+  const int64_t q0 = c->q0;
+  const int lane = lane_id();
+  const bool canon = tr.canon && *tr.canon;
+  const QBoxes& kids = LEAF ? rb : tr.mbr;
+  uint64_t wsum = 0;
+  for (int64_t e = blockIdx.x * 4ll + (threadIdx.x >> 6); e < n_in;
+       e += (int64_t)gridDim.x * 4) {
+    const uint2 en = fin ? fin[e] : make_uint2((uint32_t)(q0 + e), 0u);
+    int64_t first, L;
+    gen_children<LEAF>(tr, c, slot, en.y & ~kFull, r_sz, nbox, canon, &first, &L);
+    const float4 wq = ld_box(qb, en.x);
+    uint64_t hits = 0;
+    for (int64_t j = 0; j < L; j += kWave) {
+      const bool ok = j + lane < L;
+      const float4 b = ld_box(kids, ok ? first + j + lane : 0);
+      hits += (uint64_t)popc64(ballot(ok && intersect(wq.x, wq.y, wq.z, wq.w, b.x, b.y, b.z, b.w)));
+    }
+    if (lane == 0) {
+      cnt[e] = hits;
+      wsum += hits;
+      if (LEAF && counts && hits) atomicAdd(counts + en.x, (unsigned long long)hits);
+    }
+  }
+  if (lane == 0 && wsum)
+    atomicAdd((unsigned long long*)(LEAF ? &c->total : &c->count[slot + 1]),
+              (unsigned long long)wsum);
+}
+
+template <bool LEAF>
+__global__ __launch_bounds__(256) void k_gen_write(int slot, QTree tr, QBoxes qb, QBoxes rb,
+                                                   QCtrl* c, const uint2* __restrict__ fin,
+                                                   int64_t r_sz, int64_t nbox,
+                                                   const uint64_t* __restrict__ off,
+                                                   uint2* __restrict__ fout, int64_t cap,
+                                                   const int* __restrict__ rid,
+                                                   int* __restrict__ out_fid,
+                                                   int* __restrict__ out_tid, int64_t out_cap) {
+  if (c->overflow) return;
+  if (!LEAF && (int64_t)c->count[slot + 1] > cap) {  // the next frontier does not fit
+    if (blockIdx.x == 0 && threadIdx.x == 0) c->overflow = 1;
+    return;
+  }
+  const int64_t n_in = (int64_t)c->count[slot];
+  const int64_t q0 = c->q0;
+  const int lane = lane_id();
+  const uint64_t lt = lanemask_lt();
+  const bool canon = tr.canon && *tr.canon;
+  const QBoxes& kids = LEAF ? rb : tr.mbr;
+  for (int64_t e = blockIdx.x * 4ll + (threadIdx.x >> 6); e < n_in;
+       e += (int64_t)gridDim.x * 4) {
+    const uint2 en = fin ? fin[e] : make_uint2((uint32_t)(q0 + e), 0u);
+    int64_t first, L;
+    gen_children<LEAF>(tr, c, slot, en.y & ~kFull, r_sz, nbox, canon, &first, &L);
+    const float4 wq = ld_box(qb, en.x);
+    uint64_t o = off[e];
+    for (int64_t j = 0; j < L; j += kWave) {
+      const bool ok = j + lane < L;
+      const int64_t ci = ok ? first + j + lane : 0;
+      const float4 b = ld_box(kids, ci);
+      const bool hit = ok && intersect(wq.x, wq.y, wq.z, wq.w, b.x, b.y, b.z, b.w);
+      const uint64_t m = ballot(hit);
+      if (hit) {
+        const uint64_t slot_o = o + (uint64_t)popc64(m & lt);
+        if (LEAF) {
+          if (slot_o < (uint64_t)out_cap) {
+            out_fid[slot_o] = rid[ci];
+            out_tid[slot_o] = (int)en.x;
+          }
+        } else if (slot_o < (uint64_t)cap) {
+          fout[slot_o] = make_uint2(en.x, (uint32_t)ci);
+        }
+      }
+      o += (uint64_t)popc64(m);
+    }
+  }
+}
+
+static void enqueue_run_generic(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
+  const unsigned nslots = 2u * (unsigned)a.nl;
+  if (qe.needs_clear) {
+    hipLaunchKernelGGL(k_ctrl_init, dim3(1), dim3(64), 0, s, qe.ctrl);
+    qe.needs_clear = false;
+  }
+  pf.mark(s, "k_run_begin");
+  hipLaunchKernelGGL(k_run_begin, dim3(1), dim3(64), 0, s, qe.ctrl, a.q0, a.q1, a.plan, nslots,
+                     a.lv);
+  hipLaunchKernelGGL(k_status_zero, dim3(256), dim3(256), 0, s, (const QCtrl*)qe.ctrl, qe.status,
+                     qe.status_words);
+  const dim3 grid((unsigned)std::max(a.grid, 1024));
+  const uint2* cur = nullptr;
+  int which = 0;
+  for (int l = 0; l + 1 < a.nl; l++) {
+    uint2* out = (uint2*)qe.front[which];
+    pf.mark(s, "k_gen_count");
+    hipLaunchKernelGGL(k_gen_count<false>, grid, dim3(256), 0, s, l, a.tr, a.qb, a.rb, qe.ctrl,
+                       cur, a.r_sz, a.nbox, qe.ttot, (unsigned long long*)nullptr);
+    pf.mark(s, "k_tile_scan");
+    hipLaunchKernelGGL(k_tile_scan, dim3(64), dim3(kQThreads), 0, s, a.nl + 1 + l, l, 1, qe.ctrl,
+                       qe.ttot, qe.status);
+    pf.mark(s, "k_gen_write");
+    hipLaunchKernelGGL(k_gen_write<false>, grid, dim3(256), 0, s, l, a.tr, a.qb, a.rb, qe.ctrl,
+                       cur, a.r_sz, a.nbox, (const uint64_t*)qe.ttot, out, qe.cap_front,
+                       (const int*)nullptr, (int*)nullptr, (int*)nullptr, (int64_t)0);
+    cur = out;
+    which ^= 1;
+  }
+  const int leaf = a.nl - 1;
+  pf.mark(s, "k_gen_count");
+  hipLaunchKernelGGL(k_gen_count<true>, grid, dim3(256), 0, s, leaf, a.tr, a.qb, a.rb, qe.ctrl,
+                     cur, a.r_sz, a.nbox, qe.ttot, a.counts);
+  if (a.write) {
+    pf.mark(s, "k_tile_scan");
+    hipLaunchKernelGGL(k_tile_scan, dim3(64), dim3(kQThreads), 0, s, leaf + 1, leaf, 1, qe.ctrl,
+                       qe.ttot, qe.status);
+    pf.mark(s, "k_gen_write");
+    hipLaunchKernelGGL(k_gen_write<true>, grid, dim3(256), 0, s, leaf, a.tr, a.qb, a.rb, qe.ctrl,
+                       cur, a.r_sz, a.nbox, (const uint64_t*)qe.ttot, (uint2*)nullptr,
+                       (int64_t)0, a.rid, a.out_fid, a.out_tid, a.out_cap);
+  }
+  pf.mark(s, nullptr);
+}
+
 void enqueue_run(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
+  if (a.generic) {
+    enqueue_run_generic(qe, pf, a, s);
+    return;
+  }
   switch (a.G) {
     case 2: enqueue_run_steps<2>(qe, pf, a, s); break;
     case 4: enqueue_run_steps<4>(qe, pf, a, s); break;
@@ -1424,20 +1904,28 @@ int prepare_query(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curt
   const int F = rt->fanout;
   if (N < 0 || Q < 0) return set_error(CURTREE_EINVAL, "query: negative size");
   if (F < 2) return set_error(CURTREE_EINVAL, "query: fanout must be >= 2");
-  if (F > 64)
-    return set_error(CURTREE_EINVAL, "query: fanout %d > 64 is not supported by the GPU query", F);
+  if (F > 64 && qe.fixed)
+    return set_error(CURTREE_EINVAL,
+                     "query_noalloc: fanout %d above 64 needs per-entry counts beyond the scratch",
+                     F);
   if (Q > (int64_t)0xFFFFFFFF) return set_error(CURTREE_EINVAL, "query: too many windows");
   *a = QRunArgs();
   a->steps = qtuning().steps;
   // a block loops over tiles, so any grid is correct; small batches get a
-  // small grid (cheaper launches that leave room for concurrent runs)
-  a->grid = qtuning().grid > 0
-                ? qtuning().grid
-                : (int)std::min<int64_t>(kDefaultChainGrid, std::max<int64_t>(512, nq_hint / 8));
+  // small grid (cheaper launches that leave room for concurrent runs), also
+  // under a tuned grid (tuned on large batches)
+  const int64_t by_size = std::max<int64_t>(512, nq_hint / 8);
+  a->grid = (int)std::min<int64_t>(qtuning().grid > 0 ? qtuning().grid : kDefaultChainGrid,
+                                   by_size);
   a->nbox = N;
   a->F = F;
-  a->G = next_pow2_group(F);
+  a->generic = F > 64;  // one wave per entry, children 64 at a time
+  a->G = a->generic ? 64 : next_pow2_group(F);
   warm_grids(a->G, a->steps);
+  if (a->generic && !qe.wide) {  // per-entry counts: the buffers are sized again
+    qe.wide = true;
+    qe.cap_front = 0;
+  }
   if (N == 0) return CURTREE_OK;
   LevelPlan lp;
   level_plan(N, F, &lp);
@@ -1474,7 +1962,7 @@ int prepare_query(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curt
   }
   // the flags cost one pass over the boxes; worth it for larger batches
   // (a fixed engine holds no tree-sized buffers: no flags, no staging)
-  const bool use = !qe.fixed && (contain == 1 || (contain < 0 && nq_hint >= 256));
+  const bool use = !qe.fixed && !a->generic && (contain == 1 || (contain < 0 && nq_hint >= 256));
   // AoS staging costs a read and a write of every box: worth it when the
   // batch is large against the tree (callers that keep trees resident
   // (contain == 0 here) stage once themselves)
@@ -1523,6 +2011,7 @@ int query_run_f32(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curt
   a.q0 = q0;
   a.q1 = q1;
   a.counts = (unsigned long long*)qo->counts;
+  a.chained = qo->algo == CURTREE_ALGO_PERSISTENT;
   // caller buffer: one writing run; otherwise count first
   const bool direct = !qo->alloc && !qo->count_only && !qo->counts && qo->fid;
   a.write = direct;

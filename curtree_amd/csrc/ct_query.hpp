@@ -2,6 +2,8 @@
 // serving paths.
 #pragma once
 
+#include <type_traits>
+
 #include "ct_internal.hpp"
 
 namespace ct {
@@ -22,6 +24,8 @@ struct QTuning {
   int grid = 0;      // blocks per chained launch; 0: by batch size, up to kDefaultChainGrid
   int staging = -1;  // AoS box staging: -1 auto, 0 off, 1 on
   int expand = -1;   // internal levels: -1 by run size, 0 chained scan, 1 count/scan/write
+  int fused = 0;     // writing leaf stage: 1 one chained pass, 0 count / tile scan / write,
+                     // -1 the chained pass for runs below kSplitMinQueries windows
 };
 QTuning& qtuning();
 
@@ -51,6 +55,15 @@ struct QTree {
 __device__ __forceinline__ float4 ld_box(const QBoxes& b, int64_t i) {
   if (b.b4) return b.b4[i];
   return make_float4(b.x0[i], b.y0[i], b.x1[i], b.y1[i]);
+}
+
+// Row broadcast: every lane receives the value of the first lane of its row of
+// 16 lanes (DPP row_newbcast:0).  Called with every lane of the wave active.
+__device__ __forceinline__ uint32_t row_first(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x150, 0xF, 0xF, false);
+}
+__device__ __forceinline__ float row_first(float v) {
+  return __uint_as_float(row_first(__float_as_uint(v)));
 }
 
 // Frontier entries are (query, node | kFull).  kFull marks a node whose MBR
@@ -98,6 +111,8 @@ struct QRunArgs {
   const int64_t* plan = nullptr;  // take (q0, q1) from the next plan chunk instead
   bool write = false;             // leaf stage writes pairs (else counts only)
   bool split = false;             // internal levels as count/scan/write (no tile chain)
+  bool chained = false;           // every level one chained launch (CURTREE_ALGO_PERSISTENT)
+  bool generic = false;           // fanout above 64: one wave per entry (k_gen_*)
   int* out_fid = nullptr;
   int* out_tid = nullptr;
   int64_t out_cap = 0;

@@ -32,8 +32,13 @@ extern "C" {
 #define CURTREE_KEY_MORTON64 1 /* 32 bits per axis over the centroid extent, uint64 order */
 
 /* query algorithms */
-#define CURTREE_ALGO_FRONTIER 0   /* level-synchronous BFS frontier in HBM */
-#define CURTREE_ALGO_PERSISTENT 1 /* persistent work-stealing frontier kernel */
+/* level-synchronous BFS frontier in HBM; each level by the launches that suit
+ * the run's size (chained or count / scan / write) */
+#define CURTREE_ALGO_FRONTIER 0
+/* the same frontier, every level (leaf stage included) one launch of
+ * persistent blocks that take tiles from a ticket counter and chain them by a
+ * decoupled look-back (fewest launches; the same pairs) */
+#define CURTREE_ALGO_PERSISTENT 1
 
 /* == BBOX<float>, src/rtree.h:3-11 */
 typedef struct curtree_bbox_f32 {
@@ -292,8 +297,9 @@ CURTREE_API int curtree_query_last_stats(int64_t* entries, int cap, int* levels,
  *   "query_tile_steps"  4, 8 (default) or 16: wave-steps per frontier tile
  *                  (tile = 4 waves x steps x 64/next_pow2(F) entries).
  *   "query_grid"   blocks of each chained query launch (they loop over
- *                  tiles), i.e. waves per CU; 0 (default): by batch size,
- *                  windows/8 clamped to [512, 2048].
+ *                  tiles), i.e. waves per CU, for batches of at least 8 x that
+ *                  many windows (smaller batches: windows/8, at least 512);
+ *                  0 (default): windows/8 clamped to [512, 2048].
  *   "box_staging"  -1 (default: auto), 0 off, 1 on: a query copies boxes,
  *                  node MBRs and windows into AoS float4 arrays first (one
  *                  extra pass over the boxes; auto when the batch has at
@@ -304,6 +310,11 @@ CURTREE_API int curtree_query_last_stats(int64_t* entries, int cap, int* levels,
  *                  scan and write kernels with no chain between tiles (1;
  *                  auto for runs of at least 32768 windows and for batched
  *                  chunks).  Results are identical either way.
+ *   "leaf_fused"   0 (default) never, 1 always, -1 for runs below 32768
+ *                  windows: a writing query tests the leaf entries' boxes and
+ *                  stores their pairs in one chained pass (as
+ *                  CURTREE_ALGO_PERSISTENT does), instead of a count pass, a
+ *                  tile scan and a write pass.  Results are identical.
  * Returns CURTREE_EINVAL for an unknown name or value. */
 CURTREE_API int curtree_set_option(const char* name, int64_t value);
 

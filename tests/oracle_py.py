@@ -171,6 +171,22 @@ def bfs(tree, sorted_boxes, windows, cap=None, depths=False):
     return res
 
 
+def bfs_count(tree, sorted_boxes, windows):
+    """Oracle window query, counters only (X, T, H); releases the GIL, so
+    calls on disjoint window slices run in parallel threads."""
+    xmin, ymin, xmax, ymax = map(f32, sorted_boxes[:4])
+    ids = i32(sorted_boxes[4])
+    qx0, qy0, qx1, qy1 = map(f32, windows[:4])
+    tx0, ty0, tx1, ty1 = (f32(tree[k]) for k in ("xmin", "ymin", "xmax", "ymax"))
+    tpos, tlen = i32(tree["pos"]), i32(tree["len"])
+    cnt = np.zeros(3, np.int64)
+    lib().or_bfs(ctypes.c_int64(tpos.shape[0]), _p(tx0), _p(ty0), _p(tx1), _p(ty1), _p(tpos),
+                 _p(tlen), _p(xmin), _p(ymin), _p(xmax), _p(ymax), _p(ids),
+                 ctypes.c_int64(qx0.shape[0]), _p(qx0), _p(qy0), _p(qx1), _p(qy1), None, None,
+                 ctypes.c_int64(0), _p(cnt), None, ctypes.c_int(0))
+    return tuple(int(c) for c in cnt)
+
+
 def brute_force(boxes, windows):
     """nested_loop_query semantics: (fid, tid) in (query, input order) order."""
     xmin, ymin, xmax, ymax = map(f32, boxes[:4])

@@ -353,3 +353,101 @@ def test_work_counts_match_oracle_plain_1e6():
     rt.free()
     d.free()
     q.free()
+
+
+@pytest.mark.parametrize("fanout", [8, 16, 64])
+def test_persistent_algorithm_same_pairs(fanout):
+    # CURTREE_ALGO_PERSISTENT: every level one chained launch of ticketed
+    # blocks (the leaf stage included); windows large enough for contained
+    # ranges longer than a flat tile (pieces) and for frontier regrowth
+    import curtree_amd as ct
+
+    n, nq = 400_000, 3_000
+    bx = datagen.mixture_boxes(n, seed=410 + fanout)
+    ws = datagen.loguniform_windows(nq, seed=420 + fanout, lo=1e-4, hi=0.2)
+    ids = np.random.default_rng(fanout).permutation(n).astype(np.int32)
+    d, rt, q = setup(bx, ids, ws, fanout, 1)
+    p = ct.query(rt, d, q, algo=ct.ALGO_PERSISTENT)
+    fid, tid = p.to_host()
+    p.free()
+    sb = d.to_host()
+    ofid, otid, _ = orc.bfs(rt.to_host(), sb, ws)
+    print(f"[case] F={fanout}: {ofid.shape[0]} pairs", flush=True)
+    assert np.array_equal(fid, ofid) and np.array_equal(tid, otid)
+    rt.free()
+    d.free()
+    q.free()
+
+
+@pytest.mark.parametrize("fused", [0, 1])
+def test_leaf_stage_paths_large_batch(fused):
+    # the writing leaf stage as one chained pass (1) or count / scan / write
+    # (0), forced on a batch above the automatic cut (32,768 windows), with
+    # contained ranges of every length and a caller buffer
+    import curtree_amd as ct
+
+    n, nq = 600_000, 40_000
+    bx = datagen.clustered_boxes(n, seed=430, frac=0.3, sigma=2e-3, wmax=1e-3)
+    ws = datagen.loguniform_windows(nq, seed=431, lo=1e-4, hi=0.1)
+    d, rt, q = setup(bx, np.arange(n, dtype=np.int32), ws, 16, 1)
+    ct.set_option("leaf_fused", fused)
+    try:
+        H = ct.query_count(rt, d, q)
+        f = ctypes.c_void_p()
+        t = ctypes.c_void_p()
+        ct._check(ct.lib.curtree_malloc(ctypes.byref(f), 4 * H))
+        ct._check(ct.lib.curtree_malloc(ctypes.byref(t), 4 * H))
+        assert ct.query_into(rt, d, q, f.value, t.value, H) == H
+        st = ct.query_last_stats()
+        fid = ct._host_from_dev(f.value, H, np.int32)
+        tid = ct._host_from_dev(t.value, H, np.int32)
+        ct.lib.curtree_free(f)
+        ct.lib.curtree_free(t)
+    finally:
+        ct.set_option("leaf_fused", -1)
+    print(f"[case] fused={fused}: {H} pairs, {st['pieces']} pieces", flush=True)
+    assert st["pieces"] > 0
+    ofid, otid, _ = orc.bfs(rt.to_host(), d.to_host(), ws)
+    assert np.array_equal(fid, ofid) and np.array_equal(tid, otid)
+    rt.free()
+    d.free()
+    q.free()
+
+
+@pytest.mark.parametrize("fanout", [65, 100, 257])
+def test_fanout_above_64(fanout):
+    # the reference takes any fanout (src/rtree_test_frontier.cu:57 reads it
+    # from argv): trees of fanout above 64 are queried one wave per entry,
+    # children 64 at a time -- raw pairs equal the oracle's traversal, the
+    # caller-buffer and count entry points agree, and so does the batched one
+    import curtree_amd as ct
+
+    n, nq = 300_000, 4_000
+    bx = datagen.mixture_boxes(n, seed=440 + fanout)
+    ws = datagen.loguniform_windows(nq, seed=450 + fanout, lo=1e-4, hi=0.05)
+    ids = np.random.default_rng(fanout).permutation(n).astype(np.int32)
+    fid, tid, (X, T, H) = run_case(bx, ids, ws, fanout, 1, brute_sample=300)
+    print(f"[case] F={fanout}: {H} pairs", flush=True)
+    d, rt, q = setup(bx, ids, ws, fanout, 1)
+    total, counts = ct.query_count(rt, d, q, per_query=True)
+    assert total == H and np.array_equal(counts, np.bincount(tid, minlength=nq))
+    f = ctypes.c_void_p()
+    t = ctypes.c_void_p()
+    ct._check(ct.lib.curtree_malloc(ctypes.byref(f), 4 * H))
+    ct._check(ct.lib.curtree_malloc(ctypes.byref(t), 4 * H))
+    assert ct.query_into(rt, d, q, f.value, t.value, H) == H
+    assert np.array_equal(ct._host_from_dev(f.value, H, np.int32), fid)
+    assert np.array_equal(ct._host_from_dev(t.value, H, np.int32), tid)
+    ct.lib.curtree_free(f)
+    ct.lib.curtree_free(t)
+    got = []
+    st = ct.query_batched(rt, d, q, max(H // 5, 1),
+                          lambda fp, tp, m, q0, q1: got.append(
+                              (ct._host_from_dev(fp, m, np.int32),
+                               ct._host_from_dev(tp, m, np.int32))))
+    assert st["pairs"] == H and st["chunks"] > 1
+    assert np.array_equal(np.concatenate([g[0] for g in got]), fid)
+    assert np.array_equal(np.concatenate([g[1] for g in got]), tid)
+    rt.free()
+    d.free()
+    q.free()
