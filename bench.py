@@ -674,10 +674,21 @@ def canonical_leg(ct, tree, work, q, H, fbuf, tbuf, steps):
         ct.synchronize()
         ms.append(ev[0].elapsed_ms(ev[1]))
     ms = np.array(ms)
-    print(f"[bench] canonical sort of {H:,} pairs: {ms.mean():.2f} ms", file=sys.stderr,
+    # one more, untimed, with an event pair around every launch
+    assert ct.query_into(tree, work, q, fbuf.value, tbuf.value, H) == H
+    ct.Prof.enable(True)
+    ct.Prof.reset()
+    ct._check(ct.lib.curtree_pairs_canonical_sort(fbuf, tbuf, int(H), None))
+    ct.Prof.enable(False)
+    kernels = {}
+    for name in ("k_seg_find", "k_seg_wave", "k_seg_block", "k_seg_huge"):
+        tot, cnt = ct.Prof.get(name)
+        if cnt:
+            kernels[name] = tot
+    print(f"[bench] canonical sort of {H:,} pairs: {ms.mean():.2f} ms {kernels}", file=sys.stderr,
           flush=True)
     return {"pairs": int(H), "ms": float(ms.mean()), "pairs_per_s": H / (ms.mean() * 1e-3),
-            "steps": steps}
+            "steps": steps, "kernels_ms": kernels}
 
 
 def small_leg(ct, datagen, steps):
