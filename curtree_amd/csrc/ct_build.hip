@@ -2260,94 +2260,194 @@ int build_f32(Context& ctx, curtree_bbox_f32* box, curtree_rtree_f32* rt, int ke
 //   k_seg_wave    segments of 2..1024 pairs, one wave each (up to 64 a bitonic
 //                 network over the lanes, longer ones four LSD passes in
 //                 registers and the wave's LDS);
-//   k_seg_block   1025..4096 pairs, one block each (lds_lsd);
+//   k_seg_block   1025..4096 pairs, one block of four waves each, and
+//                 4097..16384, one block of sixteen waves each (lds_lsd);
 //   k_seg_huge    longer segments, one block each: the four digit histograms
 //                 in one read, then four stable LSD passes between the box ids
 //                 and a work array, in chunks of 4096 ranked in LDS.
 // Box ids are sorted as unsigned keys (id ^ 0x80000000), i.e. in signed order.
 constexpr int kSegWaveMax = 1024;
 constexpr int kSegBlockMax = 4096;
+constexpr int kSegBigMax = 16384;
 struct SegCtrl {
-  unsigned long long nseg[3];  // listed segments: wave, block, huge class
+  unsigned long long nseg[4];  // listed segments: wave, block, big-block and huge class
   uint32_t unordered;          // a query id is below the one before it
   uint32_t pad;
 };
 struct SegBufs {
-  uint64_t* small;             // wave class: start << 11 | length (length <= 1024)
-  uint64_t* big;               // block and huge classes: (start, length) pairs
-  uint64_t cap_small, cap_big;  // entries of each list (the huge class from the back)
+  uint64_t* small;  // wave class: start << 11 | length (length <= 1024)
+  uint64_t* big;    // the other classes: (start, length) pairs, a region each
+  uint64_t off[4];  // first entry of each class's region in `big` (classes 1..3)
+  uint64_t cap[4];  // entries each class's list holds
 };
 
 __global__ void k_seg_init(SegCtrl* c) {
-  if (threadIdx.x < 3) c->nseg[threadIdx.x] = 0;
+  if (threadIdx.x < 4) c->nseg[threadIdx.x] = 0;
   if (threadIdx.x == 0) c->unordered = 0;
 }
 
-// End of the segment of query id v that starts at i: the first position
-// after i whose id differs (ids of one segment are contiguous), by doubling
-// steps and a binary search between the last equal and the first different.
-__device__ __forceinline__ int64_t seg_end(const int* __restrict__ tid, int64_t n, int64_t i,
-                                           int v) {
-  int64_t lo = i, step = 1, hi;
-  for (;;) {
-    hi = lo + step;
-    if (hi >= n) {
-      hi = n;
-      break;
+// End of the segment of query id v that starts at (or continues past) `from`
+// (tid[from - 1] == v): the first position at or after `from` whose id
+// differs, or n (ids of one segment are contiguous).  The whole wave searches:
+// lane k probes from - 1 + 2^k, the first probe that differs brackets the end,
+// and rounds of 64 evenly spaced probes narrow the bracket.  Every lane calls
+// it with the same arguments and gets the same result.
+__device__ __forceinline__ int64_t seg_end_wave(const int* __restrict__ tid, int64_t n,
+                                                int64_t from, int v) {
+  const int lane = lane_id();
+  // lo: a position known to hold v; hi: a position known to differ (or n)
+  int64_t lo = from - 1, hi = n;
+  {
+    const int64_t d = lane < 62 ? (int64_t)1 << lane : ((int64_t)1 << 62);
+    const int64_t p = lo + d;
+    const bool differ = p >= n || tid[p] != v;
+    const uint64_t m = ballot(differ);
+    if (m) {
+      const int k = __ffsll((unsigned long long)m) - 1;  // differs first at distance 2^k
+      hi = lo + ((int64_t)1 << k);
+      if (hi > n) hi = n;
+      if (k > 0) lo += (int64_t)1 << (k - 1);
     }
-    if (tid[hi] != v) break;
-    lo = hi;
-    step <<= 1;
   }
-  // tid[lo] == v, and hi == n or tid[hi] != v
   while (hi - lo > 1) {
-    const int64_t mid = lo + ((hi - lo) >> 1);
-    if (tid[mid] == v) lo = mid;
-    else hi = mid;
+    // probes lo + step*(lane+1), lane 0..63, covering (lo, hi)
+    const int64_t span = hi - lo;
+    const int64_t step = (span + 63) / 64;
+    const int64_t p = lo + step * (lane + 1);
+    const bool differ = p >= hi || tid[p] != v;
+    const uint64_t m = ballot(differ);
+    const int k = __ffsll((unsigned long long)m) - 1;  // (lane 63's probe reaches hi)
+    const int64_t nhi = lo + step * (k + 1) < hi ? lo + step * (k + 1) : hi;
+    lo = lo + step * k;
+    hi = nhi;
   }
   return hi;
 }
 
+// Each lane holds 8 consecutive ids (a wave: a chunk of 512), so the ids are
+// read with wide loads and a segment's end is, nearly always, the next head
+// inside the chunk: the next head bit of the same lane, else the first head
+// of a later lane (a suffix minimum over the lanes).  Only the chunk's last
+// segment can go on past it; the wave measures that one together.
+constexpr int kSegPer = 8;
 __global__ __launch_bounds__(256) void k_seg_find(int64_t n, const int* __restrict__ tid,
                                                   SegCtrl* c, SegBufs b) {
   const int lane = lane_id();
-  const uint64_t lt = lanemask_lt();
-  const int64_t stride = (int64_t)gridDim.x * 256;
-  // (the loop bound is uniform across the wave: every lane takes part in the
-  // ballots)
-  for (int64_t base = blockIdx.x * 256ll; base < n; base += stride) {
-    const int64_t i = base + threadIdx.x;
-    int cls = -1;
-    int64_t len = 0;
-    if (i < n) {
-      const int v = tid[i];
-      const int prev = i > 0 ? tid[i - 1] : v;
-      if (i == 0 || prev != v) {
-        if (prev > v) c->unordered = 1u;
-        len = seg_end(tid, n, i, v) - i;
-        cls = len < 2 ? -1 : len <= kSegWaveMax ? 0 : len <= kSegBlockMax ? 1 : 2;
+  constexpr int64_t kChunk = (int64_t)kWave * kSegPer;
+  const int64_t nchunks = (n + kChunk - 1) / kChunk;
+  const int64_t wstride = (int64_t)gridDim.x * 4;
+  for (int64_t ch = blockIdx.x * 4ll + (threadIdx.x >> 6); ch < nchunks; ch += wstride) {
+    const int64_t c0 = ch * kChunk;
+    const int64_t p0 = c0 + (int64_t)lane * kSegPer;  // this lane's first position
+    int v[kSegPer];
+    // (wide loads only from a 16-byte aligned array: p0 is a multiple of 8)
+    if (p0 + kSegPer <= n && ((uintptr_t)tid & 15u) == 0) {
+      const int4 a = *(const int4*)(tid + p0);
+      const int4 d = *(const int4*)(tid + p0 + 4);
+      v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+      v[4] = d.x; v[5] = d.y; v[6] = d.z; v[7] = d.w;
+    } else {
+#pragma unroll
+      for (int k = 0; k < kSegPer; k++) v[k] = p0 + k < n ? tid[p0 + k] : 0;
+    }
+    // the id before this lane's first: the previous lane's last (lane 0: a load)
+    int before = __shfl_up(v[kSegPer - 1], 1);
+    if (lane == 0) before = c0 > 0 ? tid[c0 - 1] : v[0];
+    uint32_t hb = 0;  // head bits of this lane's positions
+    bool down = false;
+#pragma unroll
+    for (int k = 0; k < kSegPer; k++) {
+      const int prev = k ? v[k - 1] : before;
+      const bool valid = p0 + k < n;
+      const bool head = valid && (p0 + k == 0 || prev != v[k]);
+      if (head) hb |= 1u << k;
+      down |= head && prev > v[k];
+    }
+    if (ballot(down)) {
+      if (lane == 0) c->unordered = 1u;
+    }
+    // the first head of any later lane (n past the chunk: not in it)
+    const int64_t kNone = INT64_MAX;
+    int64_t nxt = hb ? p0 + (__ffs(hb) - 1) : kNone;
+    // suffix minimum over lanes > this one
+    int64_t after = kNone;
+    {
+      int64_t x = nxt;
+#pragma unroll
+      for (int o = 1; o < kWave; o <<= 1) {
+        const int64_t y = __shfl_down(x, o);
+        if (lane + o < kWave && y < x) x = y;
+      }
+      // x: min over lanes >= this one; after: min over lanes > this one
+      after = __shfl_down(x, 1);
+      if (lane == kWave - 1) after = kNone;
+    }
+    const int64_t cend = c0 + kChunk < n ? c0 + kChunk : n;
+    // the chunk's last head, if its segment reaches the chunk's end: measured
+    // by the whole wave (kNone when the chunk holds no head)
+    const int64_t last_head = [&] {
+      const uint64_t lm = ballot(hb != 0);
+      if (!lm) return kNone;
+      const int L = 63 - __clzll((long long)lm);
+      const uint32_t lb = (uint32_t)__shfl((int)hb, L);
+      return c0 + (int64_t)L * kSegPer + (31 - __clz(lb));
+    }();
+    int64_t last_end = 0;
+    if (last_head != kNone) {
+      // no head follows the last one in the chunk, so its segment holds the
+      // chunk's last id; a full chunk's segment may go on past it
+      const int lv = __shfl(v[kSegPer - 1], 63);
+      last_end = cend >= n ? n : seg_end_wave(tid, n, cend, lv);
+    }
+    // lengths and classes of this lane's heads; appends aggregated per class
+    int64_t hstart[kSegPer];
+    int64_t hlen[kSegPer];
+    int cls[kSegPer];
+#pragma unroll
+    for (int k = 0; k < kSegPer; k++) {
+      cls[k] = -1;
+      hstart[k] = 0;
+      hlen[k] = 0;
+      if ((hb >> k) & 1u) {
+        const uint32_t later = hb & ~((2u << k) - 1u);  // this lane's heads after k
+        int64_t end = later ? p0 + (__ffs(later) - 1) : after;
+        if (end == kNone) end = last_end;  // the chunk's last segment
+        const int64_t len = end - (p0 + k);
+        hstart[k] = p0 + k;
+        hlen[k] = len;
+        cls[k] = len < 2 ? -1 : len <= kSegWaveMax ? 0 : len <= kSegBlockMax ? 1
+               : len <= kSegBigMax ? 2 : 3;
       }
     }
 #pragma unroll
-    for (int k = 0; k < 3; k++) {
-      const uint64_t m = ballot(cls == k);
-      if (!m) continue;
-      const int leader = __ffsll((unsigned long long)m) - 1;
+    for (int q = 0; q < 4; q++) {
+      uint32_t mine = 0;
+#pragma unroll
+      for (int k = 0; k < kSegPer; k++) mine += cls[k] == q ? 1u : 0u;
+      if (!ballot(mine != 0)) continue;
+      uint32_t inc = mine;
+#pragma unroll
+      for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t y = __shfl_up(inc, o);
+        if (lane >= o) inc += y;
+      }
+      const uint32_t wtot = __shfl(inc, kWave - 1);
       unsigned long long at = 0;
-      if (lane == leader) at = atomicAdd(&c->nseg[k], (unsigned long long)popc64(m));
-      at = __shfl(at, leader);
-      if (cls == k) {
-        const uint64_t r = at + (uint64_t)popc64(m & lt);
-        if (k == 0) {
-          if (r < b.cap_small) b.small[r] = ((uint64_t)i << 11) | (uint64_t)len;
-        } else {
-          // the block class from the front, the huge class from the back
-          const uint64_t slot = k == 1 ? r : b.cap_big - 1 - r;
-          if (r < b.cap_big) {
-            b.big[2 * slot] = (uint64_t)i;
-            b.big[2 * slot + 1] = (uint64_t)len;
+      if (lane == 0) at = atomicAdd(&c->nseg[q], (unsigned long long)wtot);
+      at = __shfl(at, 0);
+      uint64_t r = at + (inc - mine);
+#pragma unroll
+      for (int k = 0; k < kSegPer; k++) {
+        if (cls[k] != q) continue;
+        if (r < b.cap[q]) {
+          if (q == 0) {
+            b.small[r] = ((uint64_t)hstart[k] << 11) | (uint64_t)hlen[k];
+          } else {
+            b.big[2 * (b.off[q] + r)] = (uint64_t)hstart[k];
+            b.big[2 * (b.off[q] + r) + 1] = (uint64_t)hlen[k];
           }
         }
+        r++;
       }
     }
   }
@@ -2361,7 +2461,7 @@ __global__ __launch_bounds__(256) void k_seg_wave(const SegCtrl* c, SegBufs b,
                                                   int* __restrict__ fid) {
   __shared__ uint32_t s_w[4][kSegWaveMax];
   __shared__ __align__(16) uint32_t s_c[4][256];
-  const uint64_t nseg = c->nseg[0] < b.cap_small ? c->nseg[0] : b.cap_small;
+  const uint64_t nseg = c->nseg[0] < b.cap[0] ? c->nseg[0] : b.cap[0];
   const int w = threadIdx.x >> 6, lane = lane_id();
   const uint64_t lt = lanemask_lt();
   for (uint64_t sg = (uint64_t)blockIdx.x * 4 + w; sg < nseg; sg += (uint64_t)gridDim.x * 4) {
@@ -2450,17 +2550,19 @@ __global__ __launch_bounds__(256) void k_seg_wave(const SegCtrl* c, SegBufs b,
   }
 }
 
-// Segments of 1025..4096 box ids, one block of four waves each.
-__global__ __launch_bounds__(256) void k_seg_block(const SegCtrl* c, SegBufs b,
-                                                   int* __restrict__ fid) {
-  __shared__ uint64_t s_net[kSegBlockMax];
-  __shared__ __align__(16) uint32_t s_cnt[4][256];
-  __shared__ uint32_t s_wsum[4];
-  const uint64_t nseg = c->nseg[1] < b.cap_big ? c->nseg[1] : b.cap_big;
+// Segments of 1025..4096 box ids (NW = 4, class 1) or 4097..16384 (NW = 16,
+// class 2), one block of NW waves each, sorted whole in LDS.
+template <int NW, int CLS>
+__global__ __launch_bounds__(NW * 64) void k_seg_block(const SegCtrl* c, SegBufs b,
+                                                       int* __restrict__ fid) {
+  __shared__ uint64_t s_net[NW * 64 * kRunPer];
+  __shared__ __align__(16) uint32_t s_cnt[NW][256];
+  __shared__ uint32_t s_wsum[NW];
+  const uint64_t nseg = c->nseg[CLS] < b.cap[CLS] ? c->nseg[CLS] : b.cap[CLS];
   for (uint64_t sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
-    const int64_t st = (int64_t)b.big[2 * sg];
-    const int L = (int)b.big[2 * sg + 1];
-    const int kUse = (L + 255) / 256;
+    const int64_t st = (int64_t)b.big[2 * (b.off[CLS] + sg)];
+    const int L = (int)b.big[2 * (b.off[CLS] + sg) + 1];
+    const int kUse = (L + NW * 64 - 1) / (NW * 64);
     const int wbase = (threadIdx.x >> 6) * 64 * kUse + lane_id();
     uint64_t x[kRunPer];
 #pragma unroll
@@ -2468,7 +2570,7 @@ __global__ __launch_bounds__(256) void k_seg_block(const SegCtrl* c, SegBufs b,
       const int p = wbase + 64 * k;
       x[k] = (k < kUse && p < L) ? ((uint64_t)id_key(fid[st + p]) << 32) | (uint32_t)p : ~0ull;
     }
-    lds_lsd<4>(x, kUse, wbase, s_net, s_cnt, s_wsum);
+    lds_lsd<NW>(x, kUse, wbase, s_net, s_cnt, s_wsum);
 #pragma unroll
     for (int k = 0; k < kRunPer; k++) {
       const int p = wbase + 64 * k;
@@ -2478,7 +2580,7 @@ __global__ __launch_bounds__(256) void k_seg_block(const SegCtrl* c, SegBufs b,
   }
 }
 
-// Segments longer than 4096 box ids, one block each: stable LSD passes
+// Segments longer than 16384 box ids, one block each: stable LSD passes
 // between fid and the work array `tmp` (same positions).  A digit that every
 // id of the segment shares is skipped; an odd number of passes ends with a
 // copy back.
@@ -2492,9 +2594,9 @@ __global__ __launch_bounds__(256) void k_seg_huge(const SegCtrl* c, SegBufs b,
   __shared__ uint32_t s_base[256];
   __shared__ int s_skip[4];
   const int t = threadIdx.x;
-  const uint64_t nseg = c->nseg[2] < b.cap_big ? c->nseg[2] : b.cap_big;
+  const uint64_t nseg = c->nseg[3] < b.cap[3] ? c->nseg[3] : b.cap[3];
   for (uint64_t sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
-    const uint64_t slot = b.cap_big - 1 - sg;
+    const uint64_t slot = b.off[3] + sg;
     const int64_t st = (int64_t)b.big[2 * slot];
     const int64_t L = (int64_t)b.big[2 * slot + 1];
 #pragma unroll
@@ -2667,21 +2769,29 @@ int pairs_canonical_sort(Context& ctx, int* fid, int* tid, int64_t n, hipStream_
   int rc;
   // lists: wave-class segments (8 B, at most n/2 of them), longer ones (16 B,
   // at most n/1025); the huge class's work array (4 B per pair)
+  // (a class of segments longer than m holds at most n / (m + 1) of them)
   SegBufs b;
-  b.cap_small = (uint64_t)(n / 2 + 1);
-  b.cap_big = (uint64_t)(n / (kSegWaveMax + 1) + 2);
+  b.cap[0] = (uint64_t)(n / 2 + 1);
+  b.cap[1] = (uint64_t)(n / (kSegWaveMax + 1) + 1);
+  b.cap[2] = (uint64_t)(n / (kSegBlockMax + 1) + 1);
+  b.cap[3] = (uint64_t)(n / (kSegBigMax + 1) + 1);
+  b.off[0] = 0;
+  b.off[1] = 0;
+  b.off[2] = b.cap[1];
+  b.off[3] = b.cap[1] + b.cap[2];
+  const uint64_t nbig = b.cap[1] + b.cap[2] + b.cap[3];
   if ((rc = ws.reserve(WsSortKeys0, (size_t)n * 4)) ||
-      (rc = ws.reserve(WsSortKeys1, (size_t)(b.cap_small * 8 + b.cap_big * 16))) ||
+      (rc = ws.reserve(WsSortKeys1, (size_t)(b.cap[0] * 8 + nbig * 16))) ||
       (rc = ws.reserve(WsSortCtrl, sizeof(SegCtrl) > sizeof(BuildCtrl) ? sizeof(SegCtrl)
                                                                          : sizeof(BuildCtrl))))
     return rc;
   SegCtrl* c = (SegCtrl*)ws.ptr(WsSortCtrl);
   b.small = (uint64_t*)ws.ptr(WsSortKeys1);
-  b.big = b.small + b.cap_small;
+  b.big = b.small + b.cap[0];
   Prof& pf = ctx.prof;
   pf.mark(s, "k_seg_find");
   hipLaunchKernelGGL(k_seg_init, dim3(1), dim3(64), 0, s, c);
-  hipLaunchKernelGGL(k_seg_find, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 8192)),
+  hipLaunchKernelGGL(k_seg_find, dim3((unsigned)std::min<int64_t>((n + 2047) / 2048, 8192)),
                      dim3(256), 0, s, n, (const int*)tid, c, b);
   pf.mark(s, nullptr);
   CT_HIP(hipGetLastError());
@@ -2697,12 +2807,17 @@ int pairs_canonical_sort(Context& ctx, int* fid, int* tid, int64_t n, hipStream_
   }
   if (hc.nseg[1]) {
     pf.mark(s, "k_seg_block");
-    hipLaunchKernelGGL(k_seg_block, dim3((unsigned)std::min<uint64_t>(hc.nseg[1], 1024)),
+    hipLaunchKernelGGL((k_seg_block<4, 1>), dim3((unsigned)std::min<uint64_t>(hc.nseg[1], 1024)),
                        dim3(256), 0, s, (const SegCtrl*)c, b, fid);
   }
   if (hc.nseg[2]) {
+    pf.mark(s, "k_seg_block16");
+    hipLaunchKernelGGL((k_seg_block<16, 2>), dim3((unsigned)std::min<uint64_t>(hc.nseg[2], 256)),
+                       dim3(1024), 0, s, (const SegCtrl*)c, b, fid);
+  }
+  if (hc.nseg[3]) {
     pf.mark(s, "k_seg_huge");
-    hipLaunchKernelGGL(k_seg_huge, dim3((unsigned)std::min<uint64_t>(hc.nseg[2], 768)),
+    hipLaunchKernelGGL(k_seg_huge, dim3((unsigned)std::min<uint64_t>(hc.nseg[3], 768)),
                        dim3(256), 0, s, (const SegCtrl*)c, b, fid,
                        (uint32_t*)ws.ptr(WsSortKeys0));
   }

@@ -116,3 +116,26 @@ def test_more_than_2_30_pairs_properties():
     assert np.array_equal(t, tid)
     pos = np.arange(n, dtype=np.int64) - np.repeat(starts, lengths)
     assert np.array_equal(f, (np.repeat(base, lengths) + pos).astype(np.int32))
+
+
+def test_unaligned_arrays_and_chunk_edges():
+    # pairs that start one element into their buffers (no wide loads), with
+    # segment lengths around the scan's 512-position chunks
+    import curtree_amd as ct
+
+    rng = np.random.default_rng(9)
+    lengths = [511, 1, 513, 512, 2, 1023, 1025, 4095, 4097, 16383, 16385, 3, 70_000]
+    fid, tid = _segments(lengths, rng)
+    n = fid.shape[0]
+    fp = _dev(ct, np.concatenate([[0], fid]))
+    tp = _dev(ct, np.concatenate([[0], tid]))
+    try:
+        ct._check(ct.lib.curtree_pairs_canonical_sort(ctypes.c_void_p(fp + 4),
+                                                      ctypes.c_void_p(tp + 4), n, None))
+        f = ct._host_from_dev(fp + 4, n, np.int32)
+        t = ct._host_from_dev(tp + 4, n, np.int32)
+    finally:
+        ct.lib.curtree_free(ctypes.c_void_p(fp))
+        ct.lib.curtree_free(ctypes.c_void_p(tp))
+    cf, ctt = orc.canonical(fid, tid)
+    assert np.array_equal(t, ctt) and np.array_equal(f, cf)
