@@ -816,6 +816,11 @@ def main():
                     help="wave-steps per query frontier tile (4, 8, 16; 0: library default)")
     ap.add_argument("--query-grid", type=int, default=0,
                     help="blocks per chained query launch (0: library default)")
+    ap.add_argument("--box-staging", type=int, default=None, choices=[-1, 0, 1],
+                    help="AoS copies of boxes, node MBRs and windows for a query: -1 auto "
+                         "(library default), 0 off, 1 on")
+    ap.add_argument("--containment", type=int, default=None, choices=[-1, 0, 1],
+                    help="the containment shortcut: -1 auto (library default), 0 off, 1 on")
     ap.add_argument("--leaf-fused", type=int, default=None, choices=[-1, 0, 1],
                     help="writing leaf stage: -1 by run size (library default), 0 count/scan/"
                          "write, 1 one chained pass")
@@ -841,6 +846,8 @@ def main():
         ct.set_option("expand_split", args.expand_split)
     if args.leaf_fused is not None:
         ct.set_option("leaf_fused", args.leaf_fused)
+    if args.box_staging is not None:
+        ct.set_option("box_staging", args.box_staging)
     args.boxes_set = args.boxes is not None
     args.windows_set = args.windows is not None
     if args.mode == "plain" and args.fanout is None:
@@ -897,7 +904,7 @@ def main():
     ct.set_option("containment", 0)
     H = ct.query_count(rt, work, q)
     ref_stats = ct.query_last_stats()
-    ct.set_option("containment", -1)
+    ct.set_option("containment", args.containment if args.containment is not None else -1)
     assert ct.query_count(rt, work, q) == H
     run_stats = ct.query_last_stats()
     rt.free()
@@ -1030,6 +1037,25 @@ def main():
                      "algorithmic_bytes_per_launch": dom_bytes,
                      "avg_launch_ms": dom_ms, "launches_timed": dom_cnt},
     }
+    # the query's dominant kernel, priced at the reference traversal's leaf
+    # level: 8 B per (window, leaf-parent) entry and 16 B per box test (F per
+    # entry), over its average duration in the per-kernel table
+    lc = kernels.get("k_leaf_count")
+    if lc:
+        leaf_entries = ref_stats["entries"][-1]
+        lbytes = 8 * leaf_entries + 16 * F * leaf_entries
+        lgbs = lbytes / (lc["avg_ms"] * 1e-3) / 1e9
+        ltraffic = None
+        if os.path.exists(os.path.join(ROOT, "profiles", "traffic.json")):
+            with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+                ent = json.load(f).get("k_leaf_count", {})
+            if ent.get("boxes") == n and ent.get("bytes_per_launch"):
+                ltraffic = ent["bytes_per_launch"]
+        out["roofline_query"] = {"bound": "hbm", "kernel": "k_leaf_count", "achieved": lgbs,
+                                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "frac": lgbs / HBM_PEAK_GBS, "traffic": ltraffic,
+                                 "algorithmic_bytes_per_launch": int(lbytes),
+                                 "avg_launch_ms": lc["avg_ms"]}
     join_boxes = None
     join_r = 3e-4
     out["canonical_sort"] = canonical_leg(ct, tree, work, q, H, fbuf, tbuf, 2)

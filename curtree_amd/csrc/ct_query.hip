@@ -1963,12 +1963,11 @@ int prepare_query(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curt
   // the flags cost one pass over the boxes; worth it for larger batches
   // (a fixed engine holds no tree-sized buffers: no flags, no staging)
   const bool use = !qe.fixed && !a->generic && (contain == 1 || (contain < 0 && nq_hint >= 256));
-  // AoS staging costs a read and a write of every box: worth it when the
-  // batch is large against the tree (callers that keep trees resident
-  // (contain == 0 here) stage once themselves)
+  // AoS staging (a read and a write of every box) only when asked for: the
+  // leaf and expansion kernels read the caller's SoA arrays faster (on the
+  // 100M-box workload 4.73 ms against 5.71 ms for the leaf count)
   const int st = qtuning().staging;
-  const bool stage = !qe.fixed &&
-                     (st == 1 || (st < 0 && contain != 0 && nq_hint >= 256 && nq_hint * 256 >= N));
+  const bool stage = !qe.fixed && st == 1;
   bool staged = false;
   if (stage && (rc = reserve_f4(&qe.box4, &qe.cap_box4, N))) return rc;
   if (use) {  // the flag pass also writes the AoS boxes when both are wanted

@@ -256,7 +256,7 @@ int Server::begin_batch(Slot& sl) {
                      (const ServeJob*)sl.jobs, nj, sl.win, sl.win_cap);
   curtree_bbox_f32 q{(int)nwin, sl.win, sl.win + sl.win_cap, sl.win + 2 * sl.win_cap,
                      sl.win + 3 * sl.win_cap, nullptr};
-  // the tree's cached containment flags, AoS copies and layout check
+  // the tree's cached containment flags, AoS copies (if any) and layout check
   int rc;
   if ((rc = prepare_query(sl.qe, prof, &tr.rt, &tr.rbox, &q, 0, nwin, &sl.a, sl.s))) return rc;
   sl.a.tr.safe = tr.safe;
@@ -492,7 +492,7 @@ CURTREE_API int curtree_server_add_tree(curtree_server* s, const curtree_rtree_f
   if (!sv || !rt || !rbox || !tree_id) return set_error(CURTREE_EINVAL, "server_add_tree: null");
   LevelPlan lp;
   level_plan(rbox->sz, rt->fanout, &lp);
-  if (rt->fanout < 2 || rt->fanout > 64 || lp.r_sz != rt->sz)
+  if (rt->fanout < 2 || lp.r_sz != rt->sz)
     return set_error(CURTREE_EINVAL, "server_add_tree: tree does not match its boxes");
   Tree t;
   t.rt = *rt;
@@ -505,8 +505,10 @@ CURTREE_API int curtree_server_add_tree(curtree_server* s, const curtree_rtree_f
     QBoxes nb{rt->xmin, rt->ymin, rt->xmax, rt->ymax, nullptr};
     int64_t cb = 0, cn = 0;
     int rc = compute_safe(qe, pf, rb, rt->fanout, rt->sz, rbox->sz, nullptr);
-    if (!rc) rc = stage_boxes(&t.box4, &cb, rb, rbox->sz, nullptr);
-    if (!rc) rc = stage_boxes(&t.node4, &cn, nb, rt->sz, nullptr);
+    // AoS copies of the boxes and node MBRs only when asked for ("box_staging"
+    // 1): the query reads the caller's SoA arrays faster
+    if (!rc && qtuning().staging == 1) rc = stage_boxes(&t.box4, &cb, rb, rbox->sz, nullptr);
+    if (!rc && qtuning().staging == 1) rc = stage_boxes(&t.node4, &cn, nb, rt->sz, nullptr);
     if (!rc) rc = check_layout(&t.canon, rt, rbox->sz, nullptr);
     if (!rc && hipDeviceSynchronize() != hipSuccess)
       rc = set_error(CURTREE_EHIP, "server_add_tree: staging failed");

@@ -300,11 +300,11 @@ CURTREE_API int curtree_query_last_stats(int64_t* entries, int cap, int* levels,
  *                  tiles), i.e. waves per CU, for batches of at least 8 x that
  *                  many windows (smaller batches: windows/8, at least 512);
  *                  0 (default): windows/8 clamped to [512, 2048].
- *   "box_staging"  -1 (default: auto), 0 off, 1 on: a query copies boxes,
+ *   "box_staging"  -1 or 0 (default: off), 1 on: a query copies boxes,
  *                  node MBRs and windows into AoS float4 arrays first (one
- *                  extra pass over the boxes; auto when the batch has at
- *                  least 256 windows and at least 1/256 as many windows as
- *                  boxes).  Results are identical either way.
+ *                  extra pass over the boxes; a server made while it is on
+ *                  keeps such copies of its trees).  Results are identical
+ *                  either way; the SoA arrays measured faster.
  *   "expand_split" -1 (default: by run size), 0, 1: internal tree levels are
  *                  expanded by one chained-scan kernel (0) or by count, tile
  *                  scan and write kernels with no chain between tiles (1;

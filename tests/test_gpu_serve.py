@@ -92,18 +92,18 @@ def test_results_identical_alone_and_in_the_mix(batching, slots):
 
 def test_served_roadlike_tree_matches_oracle():
     # the serving workload's tree kinds and window sizes at oracle-feasible
-    # sizes: a 2M-box road-like tree and a 1M-box clustered tree at fanout 32,
-    # windows of side log-uniform [1e-5, 2e-3]; 150 requests of 1..65,536
-    # windows (log-uniform count) submitted at once, each checked against the
-    # oracle's traversal
+    # sizes: a 2M-box road-like tree at fanout 32 and a 1M-box clustered tree
+    # at fanout 100 (one wave per frontier entry), windows of side
+    # log-uniform [1e-5, 2e-3]; 150 requests of 1..65,536 windows (log-uniform
+    # count) submitted at once, each checked against the oracle's traversal
     import curtree_amd as ct
 
     sets = [datagen.roadlike_boxes(2_000_000, seed=140),
             datagen.clustered_boxes(1_000_000, seed=141, frac=0.3, k=20, sigma=5e-3, wmax=1e-3)]
     trees, host = [], []
-    for bx in sets:
+    for bx, F in zip(sets, (32, 100)):
         d = ct.DeviceBoxes.from_host(*bx)
-        rt = ct.build_rtree(d, 32, ct.KEY_MORTON64)
+        rt = ct.build_rtree(d, F, ct.KEY_MORTON64)
         trees.append((rt, d))
         host.append((d.to_host(), rt.to_host()))
     npool = 400_000
