@@ -63,7 +63,10 @@ struct BuildCtrl {
   uint32_t trivial;                // bit d: digit d is the same for every key
 };
 
-__global__ __launch_bounds__(256) void k_build_init(BuildCtrl* c) {
+// wf: the tree's well-formed word (or null), see tree_state_register: set
+// here, cleared by k_hist if a box is not well-formed.
+__global__ __launch_bounds__(256) void k_build_init(BuildCtrl* c, uint32_t* wf) {
+  if (wf && threadIdx.x == 0) *wf = 1u;
   for (int i = threadIdx.x; i < kMaxDigits * 256; i += 256) {
     c->hist[i] = 0;
     c->base[i] = 0;
@@ -243,14 +246,16 @@ __device__ __forceinline__ uint32_t sort_key32(float a, float b, float cc, float
 
 // The histograms of the four digits of the 32-bit sort keys (the first sort
 // pass computes the keys again from the boxes it reads), and the status words
-// of the sort steps cleared.
+// of the sort steps cleared.  wf (or null): cleared when a box is not
+// well-formed (xmin <= xmax and ymin <= ymax fails, NaN included); the sort
+// only permutes the boxes, so this holds for the sorted boxes as well.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_hist(int64_t n, const float* __restrict__ xmin,
                                               const float* __restrict__ ymin,
                                               const float* __restrict__ xmax,
                                               const float* __restrict__ ymax, BuildCtrl* c,
                                               uint32_t* __restrict__ zero_ptr,
-                                              int64_t zero_words) {
+                                              int64_t zero_words, uint32_t* wf) {
   __shared__ uint32_t sh[4 * 256];
   for (int t = threadIdx.x; t < 4 * 256; t += 256) sh[t] = 0;
   const KeyParams kp = key_params<MODE>(c);
@@ -264,6 +269,13 @@ __global__ __launch_bounds__(256) void k_hist(int64_t n, const float* __restrict
       const int64_t i = i0 + 256 * u;
       b[u] = i < n ? make_float4(xmin[i], ymin[i], xmax[i], ymax[i])
                    : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    if (wf) {
+      bool bad = false;
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        bad |= i0 + 256 * u < n && !(b[u].x <= b[u].z && b[u].y <= b[u].w);
+      if (ballot(bad) && lane_id() == 0) *wf = 0u;
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
@@ -2121,6 +2133,8 @@ int build_f32(Context& ctx, curtree_bbox_f32* box, curtree_rtree_f32* rt, int ke
       return set_error(CURTREE_EINVAL, "build: null node array");
   }
   auto fail = [&](int code) {
+    const void* ps[6] = {rt->xmin, rt->ymin, rt->xmax, rt->ymax, rt->pos, rt->len};
+    tree_state_forget(ps, 6);
     if (alloc) rt_free(rt);
     return code;
   };
@@ -2188,9 +2202,12 @@ int build_f32(Context& ctx, curtree_bbox_f32* box, curtree_rtree_f32* rt, int ke
   }
   if (order && hipStreamWaitEvent(s, ctx.build_done, 0) != hipSuccess)
     return fail(set_error(CURTREE_EHIP, "build: cannot order after the previous build"));
+  // the tree's well-formed word, kept for its queries (no allocation under
+  // capture)
+  uint32_t* wf = tree_state_register(rt, box, order);
   Prof& pf = ctx.prof;
   pf.mark(s, "k_build_init");
-  hipLaunchKernelGGL(k_build_init, dim3(1), dim3(256), 0, s, c);
+  hipLaunchKernelGGL(k_build_init, dim3(1), dim3(256), 0, s, c, wf);
   const unsigned sgrid = (unsigned)std::min<int64_t>((n + 1023) / 1024, 2048);
   if (m64) {
     pf.mark(s, "k_extent");
@@ -2201,10 +2218,10 @@ int build_f32(Context& ctx, curtree_bbox_f32* box, curtree_rtree_f32* rt, int ke
   pf.mark(s, "k_hist");
   if (m64)
     hipLaunchKernelGGL(k_hist<kKeyMorton64>, dim3(sgrid), dim3(256), 0, s, n, box->xmin,
-                       box->ymin, box->xmax, box->ymax, c, sb.status, (int64_t)(4 * step_words));
+                       box->ymin, box->xmax, box->ymax, c, sb.status, (int64_t)(4 * step_words), wf);
   else
     hipLaunchKernelGGL(k_hist<kKeyRef32>, dim3(sgrid), dim3(256), 0, s, n, box->xmin, box->ymin,
-                       box->xmax, box->ymax, c, sb.status, (int64_t)(4 * step_words));
+                       box->xmax, box->ymax, c, sb.status, (int64_t)(4 * step_words), wf);
   // (the last step that runs writes the caller's arrays)
   pf.mark(s, "k_sort_plan");
   hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(256), 0, s, n, 4, c, 0, 4, 0x3210ull, 0, 0, 2,
@@ -2739,7 +2756,7 @@ static int pairs_full_sort(Context& ctx, int* fid, int* tid, int64_t n, hipStrea
   uint32_t* status = (uint32_t*)ws.ptr(WsSortStatus);
   Prof& pf = ctx.prof;
   pf.mark(s, "k_build_init");
-  hipLaunchKernelGGL(k_build_init, dim3(1), dim3(256), 0, s, c);
+  hipLaunchKernelGGL(k_build_init, dim3(1), dim3(256), 0, s, c, (uint32_t*)nullptr);
   const unsigned sgrid = (unsigned)std::min<int64_t>((n + 1023) / 1024, 2048);
   pf.mark(s, "k_pair_pack");
   hipLaunchKernelGGL(k_pair_pack, dim3(sgrid), dim3(256), 0, s, n, fid, tid, c, k0, status,

@@ -57,11 +57,116 @@ int rt_alloc(curtree_rtree_f32* rt, int64_t sz) {
     rt_free(rt);
     return set_error(CURTREE_ENOMEM, "cannot allocate %lld tree nodes", (long long)sz);
   }
+  // (fresh arrays: an entry naming their addresses outlived arrays freed elsewhere)
+  const void* ps[6] = {rt->xmin, rt->ymin, rt->xmax, rt->ymax, rt->pos, rt->len};
+  tree_state_forget(ps, 6);
   return CURTREE_OK;
+}
+
+// ---- state of the trees built here ------------------------------------------
+namespace {
+struct TreeState {
+  const void* arr[11];  // node arrays (6), then the sorted boxes (5)
+  int64_t sz = 0, nbox = 0;
+  int fanout = 0;
+  uint32_t* words = nullptr;  // device: [0] every box is well-formed
+  bool overlaps(const char* p, int64_t bytes) const {
+    for (int j = 0; j < 11; j++) {
+      const char* a = (const char*)arr[j];
+      const int64_t len = 4 * (j < 6 ? sz : nbox);
+      if (a && p < a + len && a < p + bytes) return true;
+    }
+    return false;
+  }
+};
+std::mutex g_state_mu;
+std::vector<TreeState> g_states;
+// words of forgotten entries, reused (never freed: 256 B each, and a free
+// could stall an asynchronous build or break a graph capture)
+std::vector<uint32_t*> g_spare_words;
+
+TreeState state_key(const curtree_rtree_f32* rt, const curtree_bbox_f32* box) {
+  TreeState k;
+  const void* a[11] = {rt->xmin, rt->ymin, rt->xmax, rt->ymax, rt->pos, rt->len,
+                       box->xmin, box->ymin, box->xmax, box->ymax, box->id};
+  for (int i = 0; i < 11; i++) k.arr[i] = a[i];
+  k.sz = rt->sz;
+  k.nbox = box->sz;
+  k.fanout = rt->fanout;
+  return k;
+}
+
+// Drops (under g_state_mu) every entry that shares an array with `a`.
+void forget_locked(const void* const* a, int n) {
+  for (size_t i = 0; i < g_states.size();) {
+    bool hit = false;
+    for (int j = 0; j < 11 && !hit; j++)
+      for (int k = 0; k < n && !hit; k++) hit = a[k] && g_states[i].arr[j] == a[k];
+    if (hit) {
+      g_spare_words.push_back(g_states[i].words);
+      g_states[i] = g_states.back();
+      g_states.pop_back();
+    } else {
+      i++;
+    }
+  }
+}
+}  // namespace
+
+uint32_t* tree_state_register(const curtree_rtree_f32* rt, const curtree_bbox_f32* box,
+                              bool may_alloc) {
+  std::lock_guard<std::mutex> g(g_state_mu);
+  TreeState k = state_key(rt, box);
+  forget_locked(k.arr, 11);  // trees over these arrays are rebuilt or re-sorted
+  if (g_spare_words.empty()) {
+    if (!may_alloc) return nullptr;
+    uint32_t* w = nullptr;
+    if (hipMalloc((void**)&w, 256) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;  // not kept: queries compute the flags themselves
+    }
+    g_spare_words.push_back(w);
+  }
+  k.words = g_spare_words.back();
+  g_spare_words.pop_back();
+  g_states.push_back(k);
+  return k.words;
+}
+
+const uint32_t* tree_state_find(const curtree_rtree_f32* rt, const curtree_bbox_f32* box) {
+  std::lock_guard<std::mutex> g(g_state_mu);
+  const TreeState k = state_key(rt, box);
+  for (const TreeState& e : g_states) {
+    if (e.sz != k.sz || e.nbox != k.nbox || e.fanout != k.fanout) continue;
+    bool same = true;
+    for (int j = 0; j < 11 && same; j++) same = e.arr[j] == k.arr[j];
+    if (same) return e.words;
+  }
+  return nullptr;
+}
+
+void tree_state_forget(const void* const* arrays, int n) {
+  std::lock_guard<std::mutex> g(g_state_mu);
+  forget_locked(arrays, n);
+}
+
+void tree_state_forget_range(const void* p, int64_t bytes) {
+  if (!p || bytes <= 0) return;
+  std::lock_guard<std::mutex> g(g_state_mu);
+  for (size_t i = 0; i < g_states.size();) {
+    if (g_states[i].overlaps((const char*)p, bytes)) {
+      g_spare_words.push_back(g_states[i].words);
+      g_states[i] = g_states.back();
+      g_states.pop_back();
+    } else {
+      i++;
+    }
+  }
 }
 
 void rt_free(curtree_rtree_f32* rt) {
   void* ps[6] = {rt->xmin, rt->ymin, rt->xmax, rt->ymax, rt->pos, rt->len};
+  tree_state_forget(ps, 6);
   for (void* p : ps)
     if (p) (void)hipFree(p);
   rt->xmin = rt->ymin = rt->xmax = rt->ymax = nullptr;
@@ -300,12 +405,15 @@ CURTREE_API int curtree_box_d_alloc(curtree_bbox_f32* d, int sz) {
     curtree_box_d_free(d);
     return set_error(CURTREE_ENOMEM, "box_d_alloc: cannot allocate %d boxes", sz);
   }
+  const void* ps[5] = {d->xmin, d->ymin, d->xmax, d->ymax, d->id};
+  tree_state_forget(ps, 5);
   return CURTREE_OK;
 }
 
 CURTREE_API int curtree_box_d_free(curtree_bbox_f32* d) {
   if (!d) return CURTREE_OK;
   void* ps[5] = {d->xmin, d->ymin, d->xmax, d->ymax, d->id};
+  tree_state_forget(ps, 5);
   for (void* p : ps)
     if (p) (void)hipFree(p);
   d->xmin = d->ymin = d->xmax = d->ymax = nullptr;
@@ -388,15 +496,20 @@ CURTREE_API int curtree_malloc(void** p, int64_t bytes) {
   *p = nullptr;
   if (!bytes) return CURTREE_OK;
   CT_HIP(hipMalloc(p, (size_t)bytes));
+  const void* fresh[1] = {*p};
+  tree_state_forget(fresh, 1);
   return CURTREE_OK;
 }
 
 CURTREE_API int curtree_free(void* p) {
+  const void* gone[1] = {p};
+  tree_state_forget(gone, 1);
   if (p) CT_HIP(hipFree(p));
   return CURTREE_OK;
 }
 
 CURTREE_API int curtree_memcpy_h2d(void* dst, const void* src, int64_t bytes) {
+  tree_state_forget_range(dst, bytes);  // a kept tree's boxes may change
   if (bytes > 0) CT_HIP(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice));
   return CURTREE_OK;
 }
@@ -407,12 +520,14 @@ CURTREE_API int curtree_memcpy_d2h(void* dst, const void* src, int64_t bytes) {
 }
 
 CURTREE_API int curtree_memcpy_d2d(void* dst, const void* src, int64_t bytes, void* stream) {
+  tree_state_forget_range(dst, bytes);
   if (bytes > 0)
     CT_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToDevice, as_stream(stream)));
   return CURTREE_OK;
 }
 
 CURTREE_API int curtree_memset(void* dst, int value, int64_t bytes) {
+  tree_state_forget_range(dst, bytes);
   if (bytes > 0) CT_HIP(hipMemset(dst, value, (size_t)bytes));
   return CURTREE_OK;
 }
@@ -504,6 +619,11 @@ CURTREE_API int curtree_set_option(const char* name, int64_t value) {
   if (!strcmp(name, "leaf_fused")) {
     if (value < -1 || value > 1) return set_error(CURTREE_EINVAL, "leaf_fused: -1, 0 or 1");
     qtuning().fused = (int)value;
+    return CURTREE_OK;
+  }
+  if (!strcmp(name, "tile_xcd")) {
+    if (value < 0 || value > 1) return set_error(CURTREE_EINVAL, "tile_xcd: 0 or 1");
+    qtuning().xcd = (int)value;
     return CURTREE_OK;
   }
   if (!strcmp(name, "query_grid")) {

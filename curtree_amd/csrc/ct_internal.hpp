@@ -45,6 +45,24 @@ void level_plan(int64_t n, int fanout, LevelPlan* lp);
 int rt_alloc(curtree_rtree_f32* rt, int64_t sz);
 void rt_free(curtree_rtree_f32* rt);
 
+// State of the trees this library built, kept with the tree (ct_capi.cpp):
+// one device word per tree, written by its build, nonzero when every box is
+// well-formed (xmin <= xmax and ymin <= ymax, no NaN).  A query on that tree
+// with the same sorted boxes reads it instead of computing per-node flags
+// (the containment shortcut needs them).  An entry is keyed by the six node
+// arrays, the five box arrays and the sizes.  Building into, freeing, loading
+// over or writing (through the library's memory helpers) any of those arrays
+// forgets it; boxes changed by other means after the build need a rebuild.
+// register: the word of the tree about to be built (nullptr: not kept, e.g.
+// under graph capture); find: the word of a kept tree or nullptr.
+uint32_t* tree_state_register(const curtree_rtree_f32* rt, const curtree_bbox_f32* box,
+                              bool may_alloc);
+const uint32_t* tree_state_find(const curtree_rtree_f32* rt, const curtree_bbox_f32* box);
+// Forget every entry that names one of these arrays (null entries ignored).
+void tree_state_forget(const void* const* arrays, int n);
+// Forget every entry with an array that overlaps [p, p + bytes).
+void tree_state_forget_range(const void* p, int64_t bytes);
+
 // Cached device work buffers, grown on demand and reused across calls.
 enum WsSlot {
   WsBuildKeys0,

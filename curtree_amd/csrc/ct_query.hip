@@ -147,6 +147,7 @@ __global__ __launch_bounds__(kQThreads) void k_expand(int slot, QTree tr, QBoxes
   // bulk-load layout: the entries of this launch sit on level nl-1-slot and
   // node i's children are nodes cs + i*F .. of the level below
   const bool canon = tr.canon && *tr.canon;
+  const bool shortcut = shortcut_on(tr, canon);
   const int lvl = c->lv.nl - 1 - slot;
   const int64_t ls = c->lv.lstart[lvl];
   const int64_t cs = lvl > 0 ? c->lv.lstart[lvl - 1] : 0;
@@ -194,8 +195,8 @@ __global__ __launch_bounds__(kQThreads) void k_expand(int slot, QTree tr, QBoxes
             ci = child;
             // (a contained subtree's box range comes from the level table,
             // so the shortcut needs the bulk-load layout)
-            if (hit && canon && tr.safe &&
-                contains(wq.x, wq.y, wq.z, wq.w, mb.x, mb.y, mb.z, mb.w) && tr.safe[child])
+            if (hit && shortcut &&
+                contains(wq.x, wq.y, wq.z, wq.w, mb.x, mb.y, mb.z, mb.w) && node_safe(tr, child))
               ci = child | kFull;
           }
         }
@@ -384,7 +385,7 @@ template <int G, int STEPS, bool STORE, bool QCOUNTS>
 __global__ __launch_bounds__(kQThreads) void k_leaf_count(
     int slot, QTree tr, QBoxes qb, QBoxes rb, QCtrl* c, const uint2* __restrict__ fin,
     int64_t r_sz, int64_t nbox, QLeaf* __restrict__ est, uint64_t* __restrict__ ttot,
-    unsigned long long* counts) {
+    unsigned long long* counts, bool xcd) {
   using S = QShape<G, STEPS>;
   constexpr int P = S::kPerTile;
   // memory round trips per tile: the entries of all steps, then the windows
@@ -418,7 +419,8 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_count(
   const int64_t ls = c->lv.lstart[0];
   uint64_t block_total = 0;  // thread 0: one atomic per block, not per tile
   // tiles are independent (no chain): a fixed stride over them, no ticket
-  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  const TileWalk tw = tile_walk(ntiles, xcd);
+  for (int64_t tile = tw.first; tile < tw.end; tile += tw.step) {
     __syncthreads();  // the previous tile is done with the shared words (and s_lv is in)
     uint64_t wsum = 0;
     const int64_t tbase = tile * P;
@@ -617,7 +619,7 @@ __global__ __launch_bounds__(kQThreads) void k_expand_count(int slot, QTree tr, 
                                                             QCtrl* c, const uint2* __restrict__ fin,
                                                             QExp* __restrict__ est,
                                                             uint64_t* __restrict__ ttot,
-                                                            int64_t r_sz) {
+                                                            int64_t r_sz, bool xcd) {
   using S = QShape<G, STEPS>;
   constexpr int P = S::kPerTile;
   // memory round trips per tile: the entries of all steps, then the windows,
@@ -639,14 +641,15 @@ __global__ __launch_bounds__(kQThreads) void k_expand_count(int slot, QTree tr, 
   // bulk-load layout: the entries of this launch sit on level nl-1-slot and
   // node i's children are nodes cs + i*F .. of the level below
   const bool canon = tr.canon && *tr.canon;
-  const bool shortcut = canon && tr.safe != nullptr;
+  const bool shortcut = shortcut_on(tr, canon);
   const int lvl = c->lv.nl - 1 - slot;
   const int64_t ls = c->lv.lstart[lvl];
   const int64_t cs = lvl > 0 ? c->lv.lstart[lvl - 1] : 0;
   const int64_t cc = lvl > 0 ? c->lv.lcount[lvl - 1] : 0;
   uint64_t block_total = 0;
   // tiles are independent (no chain): a fixed stride over them, no ticket
-  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  const TileWalk tw = tile_walk(ntiles, xcd);
+  for (int64_t tile = tw.first; tile < tw.end; tile += tw.step) {
     __syncthreads();  // the previous tile is done with the shared words
     uint64_t wsum = 0;
     const int64_t tbase = tile * P;
@@ -696,7 +699,7 @@ __global__ __launch_bounds__(kQThreads) void k_expand_count(int slot, QTree tr, 
         test[k] = !full && p[k] >= 0 && c_ < L[k] && (int64_t)p[k] + L[k] <= r_sz;
         const uint32_t child = test[k] ? (uint32_t)(p[k] + c_) : 0u;
         mb[k] = ld_box(tr.mbr, child);
-        sf[k] = shortcut ? tr.safe[child] : 0;
+        sf[k] = shortcut && node_safe(tr, child) ? 1 : 0;
       }
       bcast_windows<G, H>(wq);
 #pragma unroll
@@ -753,7 +756,8 @@ template <int G, int STEPS>
 __global__ __launch_bounds__(kQThreads) void k_expand_write(int slot, QCtrl* c,
                                                             const QExp* __restrict__ est,
                                                             const uint64_t* __restrict__ tbase_arr,
-                                                            uint2* __restrict__ fout, int64_t cap) {
+                                                            uint2* __restrict__ fout, int64_t cap,
+                                                            bool xcd) {
   using S = QShape<G, STEPS>;
   // a write tile is M count tiles (at least one entry per thread): its base is
   // the first count tile's scanned total
@@ -776,7 +780,8 @@ __global__ __launch_bounds__(kQThreads) void k_expand_write(int slot, QCtrl* c,
   const int64_t n_in = (int64_t)c->count[slot];
   const int64_t ntiles = (n_in + P - 1) / P;
   const int t = threadIdx.x;
-  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  const TileWalk tw = tile_walk(ntiles, xcd);
+  for (int64_t tile = tw.first; tile < tw.end; tile += tw.step) {
     __syncthreads();  // the previous tile is done with the shared arrays
     uint64_t loc[R];
     uint64_t sum = 0;
@@ -838,7 +843,7 @@ template <int G, int STEPS>
 __global__ __launch_bounds__(kQThreads) void k_leaf_write(
     int slot, QCtrl* c, const QLeaf* __restrict__ est, const uint64_t* __restrict__ tbase_arr,
     const int* __restrict__ rid, int* __restrict__ out_fid, int* __restrict__ out_tid,
-    int64_t out_cap, QPiece* __restrict__ pieces, int64_t piece_cap) {
+    int64_t out_cap, QPiece* __restrict__ pieces, int64_t piece_cap, bool xcd) {
   using S = QShape<G, STEPS>;
   // a write tile is M count tiles (at least one entry per thread): its base is
   // the first count tile's scanned total
@@ -857,7 +862,8 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_write(
   const int64_t n_in = (int64_t)c->count[slot];
   const int64_t ntiles = (n_in + P - 1) / P;
   const int t = threadIdx.x;
-  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  const TileWalk tw = tile_walk(ntiles, xcd);
+  for (int64_t tile = tw.first; tile < tw.end; tile += tw.step) {
     __syncthreads();  // the previous tile is done with the shared arrays
     // the tile's entries, and scans of all pairs and of the flat ones
     // (partial entries and ranges up to kFlatMax, written by the block)
@@ -1589,6 +1595,7 @@ void enqueue_run_g(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
   const bool split = !a.chained && (ex == 1 || (ex < 0 && !small));
   const int fz = qtuning().fused;
   const bool fused = a.chained || fz == 1 || (fz < 0 && small);
+  const bool xcd = qtuning().xcd == 1;
   if (qe.needs_clear) {
     hipLaunchKernelGGL(k_ctrl_init, dim3(1), dim3(64), 0, s, qe.ctrl);
     qe.needs_clear = false;
@@ -1607,7 +1614,7 @@ void enqueue_run_g(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
       hipLaunchKernelGGL((k_expand_count<G, STEPS>),
                          dim3(std::min<unsigned>(a.grid, expand_count_grid<G, STEPS>())),
                          dim3(kQThreads), 0, s, l,
-                         a.tr, a.qb, qe.ctrl, cur, (QExp*)qe.est, qe.ttot, a.r_sz);
+                         a.tr, a.qb, qe.ctrl, cur, (QExp*)qe.est, qe.ttot, a.r_sz, xcd);
       pf.mark(s, "k_tile_scan");
       hipLaunchKernelGGL(k_tile_scan, dim3(64), dim3(kQThreads), 0, s, a.nl + 1 + l, l,
                          S::kPerTile, qe.ctrl, qe.ttot, qe.status);
@@ -1616,7 +1623,7 @@ void enqueue_run_g(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
                          dim3(std::min<unsigned>(a.grid, expand_write_grid<G, STEPS>())),
                          dim3(kQThreads), 0, s, l,
                          qe.ctrl, (const QExp*)qe.est, (const uint64_t*)qe.ttot, out,
-                         qe.cap_front);
+                         qe.cap_front, xcd);
     } else {
       pf.mark(s, "k_expand");
       hipLaunchKernelGGL((k_expand<G, STEPS>), dim3(a.grid), dim3(kQThreads), 0, s, l, a.tr,
@@ -1648,12 +1655,12 @@ void enqueue_run_g(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
       hipLaunchKernelGGL((k_leaf_count<G, STEPS, true, true>),
                          dim3(std::min<unsigned>(a.grid, leaf_count_grid<G, STEPS, true, true>())),
                          dim3(kQThreads), 0, s, leaf, a.tr, a.qb, a.rb, qe.ctrl, cur, a.r_sz,
-                         a.nbox, (QLeaf*)qe.est, qe.ttot, a.counts);
+                         a.nbox, (QLeaf*)qe.est, qe.ttot, a.counts, xcd);
     else
       hipLaunchKernelGGL((k_leaf_count<G, STEPS, true, false>),
                          dim3(std::min<unsigned>(a.grid, leaf_count_grid<G, STEPS, true, false>())),
                          dim3(kQThreads), 0, s, leaf, a.tr, a.qb, a.rb, qe.ctrl, cur, a.r_sz,
-                         a.nbox, (QLeaf*)qe.est, qe.ttot, (unsigned long long*)nullptr);
+                         a.nbox, (QLeaf*)qe.est, qe.ttot, (unsigned long long*)nullptr, xcd);
     pf.mark(s, "k_tile_scan");
     hipLaunchKernelGGL(k_tile_scan, dim3(64), dim3(kQThreads), 0, s, leaf + 1, leaf, S::kPerTile,
                        qe.ctrl, qe.ttot, qe.status);
@@ -1662,7 +1669,7 @@ void enqueue_run_g(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
                        dim3(std::min<unsigned>(a.grid, leaf_write_grid<G, STEPS>())),
                        dim3(kQThreads), 0, s, leaf,
                        qe.ctrl, (const QLeaf*)qe.est, (const uint64_t*)qe.ttot, a.rid, a.out_fid,
-                       a.out_tid, a.out_cap, (QPiece*)qe.pieces, qe.cap_pieces);
+                       a.out_tid, a.out_cap, (QPiece*)qe.pieces, qe.cap_pieces, xcd);
     pf.mark(s, "k_write_pieces");
     hipLaunchKernelGGL(k_write_pieces, dim3(4096), dim3(256), 0, s, (const QCtrl*)qe.ctrl,
                        (const QPiece*)qe.pieces, qe.cap_pieces, a.rid, a.out_fid, a.out_tid,
@@ -1673,14 +1680,14 @@ void enqueue_run_g(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
       hipLaunchKernelGGL((k_leaf_count<G, STEPS, false, true>),
                          dim3(std::min<unsigned>(a.grid, leaf_count_grid<G, STEPS, false, true>())),
                          dim3(kQThreads), 0, s, leaf, a.tr, a.qb, a.rb, qe.ctrl, cur, a.r_sz,
-                         a.nbox, (QLeaf*)nullptr, (uint64_t*)nullptr, a.counts);
+                         a.nbox, (QLeaf*)nullptr, (uint64_t*)nullptr, a.counts, xcd);
     else
       hipLaunchKernelGGL((k_leaf_count<G, STEPS, false, false>),
                          dim3(std::min<unsigned>(a.grid,
                                                  leaf_count_grid<G, STEPS, false, false>())),
                          dim3(kQThreads), 0, s, leaf, a.tr, a.qb, a.rb, qe.ctrl, cur, a.r_sz,
                          a.nbox, (QLeaf*)nullptr, (uint64_t*)nullptr,
-                         (unsigned long long*)nullptr);
+                         (unsigned long long*)nullptr, xcd);
   }
   pf.mark(s, nullptr);
 }
@@ -1874,8 +1881,9 @@ void enqueue_run(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
 }
 
 int64_t pieces_needed(const QEngine& qe, const QRunArgs& a) {
-  // contained ranges exist only with the containment shortcut (node flags)
-  if (!a.tr.safe) return 0;
+  // contained ranges exist only with the containment shortcut (node flags or
+  // the tree's well-formed word)
+  if (!a.tr.safe && !a.tr.allwf) return 0;
   // every long range is at most one leaf entry and cut into kPiece slots
   return qe.cap_front + a.out_cap / (int64_t)kPiece + 1;
 }
@@ -1970,7 +1978,13 @@ int prepare_query(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curt
   const bool stage = !qe.fixed && st == 1;
   bool staged = false;
   if (stage && (rc = reserve_f4(&qe.box4, &qe.cap_box4, N))) return rc;
-  if (use) {  // the flag pass also writes the AoS boxes when both are wanted
+  // a tree built by this library over these boxes: its build recorded whether
+  // every box is well-formed, so no per-node flags are computed (the layout is
+  // still checked: two small launches over pos/len)
+  const uint32_t* kept = use ? tree_state_find(rt, rbox) : nullptr;
+  if (kept) {
+    a->tr.allwf = kept;
+  } else if (use) {  // the flag pass also writes the AoS boxes when both are wanted
     if ((rc = compute_safe(qe, pf, a->rb, F, lp.r_sz, N, s, stage ? qe.box4 : nullptr, &staged)))
       return rc;
     a->tr.safe = qe.safe;

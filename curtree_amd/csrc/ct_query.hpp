@@ -26,6 +26,7 @@ struct QTuning {
   int expand = -1;   // internal levels: -1 by run size, 0 chained scan, 1 count/scan/write
   int fused = 0;     // writing leaf stage: 1 one chained pass, 0 count / tile scan / write,
                      // -1 the chained pass for runs below kSplitMinQueries windows
+  int xcd = 0;       // chain-free kernels: 1 each XCD walks a contiguous eighth of the tiles
 };
 QTuning& qtuning();
 
@@ -44,13 +45,31 @@ struct QTree {
   QBoxes mbr;                   // node MBRs
   const int* pos = nullptr;
   const int* len = nullptr;
-  const uint8_t* safe = nullptr;  // per node: every box below is well-formed (or null: no shortcut)
+  const uint8_t* safe = nullptr;  // per node: every box below is well-formed
+  // or, for a tree built by this library, one device word: every box of the
+  // tree is well-formed (both null: no containment shortcut)
+  const uint32_t* allwf = nullptr;
   // Device word, nonzero when every node's pos/len is the bulk-load layout
   // (children of node i of a level are i*F .. i*F+F-1 of the level below):
   // child ranges are then computed, not loaded.  Null: always load.
   const uint32_t* canon = nullptr;
   int F = 0;
 };
+
+// Tiles of block blockIdx.x in a chain-free launch: first, first + step, ...
+// below end.  xcd: blocks are dispatched to the 8 XCDs round-robin (block b
+// on XCD b % 8), and XCD x walks the x-th contiguous eighth of the tiles, so
+// the tiles in flight on one XCD are neighbours in the frontier (nearby
+// windows when the windows are in spatial order) and share that XCD's L2.
+struct TileWalk {
+  int64_t first, end, step;
+};
+__device__ __forceinline__ TileWalk tile_walk(int64_t ntiles, bool xcd) {
+  if (!xcd || (gridDim.x & 7u) != 0) return TileWalk{(int64_t)blockIdx.x, ntiles, gridDim.x};
+  const int64_t slab = (ntiles + 7) / 8, x = blockIdx.x & 7u;
+  const int64_t end = (x + 1) * slab < ntiles ? (x + 1) * slab : ntiles;
+  return TileWalk{x * slab + (blockIdx.x >> 3), end, (int64_t)(gridDim.x >> 3)};
+}
 
 __device__ __forceinline__ float4 ld_box(const QBoxes& b, int64_t i) {
   if (b.b4) return b.b4[i];
@@ -73,6 +92,15 @@ __device__ __forceinline__ float row_first(float v) {
 // expanded without reading child MBRs or boxes.  Results are identical to
 // testing every child, as the reference does.
 constexpr uint32_t kFull = 0x80000000u;
+
+// The containment shortcut is allowed on this tree (call once per kernel),
+// and the subtree of `node` holds only well-formed boxes.
+__device__ __forceinline__ bool shortcut_on(const QTree& tr, bool canon) {
+  return canon && (tr.safe != nullptr || (tr.allwf != nullptr && *tr.allwf != 0u));
+}
+__device__ __forceinline__ bool node_safe(const QTree& tr, uint32_t node) {
+  return tr.safe == nullptr || tr.safe[node] != 0;
+}
 
 __device__ __forceinline__ bool contains(float x1, float y1, float x2, float y2, float m1,
                                          float n1, float m2, float n2) {

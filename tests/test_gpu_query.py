@@ -451,3 +451,63 @@ def test_fanout_above_64(fanout):
     rt.free()
     d.free()
     q.free()
+
+
+def test_kept_tree_state_skips_flag_pass_and_forgets_changed_boxes():
+    # A tree built by the library records whether all its boxes are
+    # well-formed, so a large batch on it runs no per-node flag pass.  Boxes
+    # rewritten through the library afterwards drop that record: the query
+    # then computes the flags again.  The rewrite inverts one box of a
+    # leaf-parent that a window contains, so a stale record would report it
+    # through the containment shortcut although it does not intersect.
+    import curtree_amd as ct
+
+    n = 50_000
+    bx = datagen.uniform_boxes(n, seed=151, wmax=2e-3)
+    ids = np.arange(n, dtype=np.int32)
+    d = ct.DeviceBoxes.from_host(*bx, ids)
+    rt = ct.build_rtree(d, 16, 1)
+    h = rt.to_host()
+    sb = list(d.to_host())
+    l0 = rt.sz - (n + 15) // 16
+    g = 7  # leaf-parent whose MBR the first window contains
+    gx0, gy0, gx1, gy1 = (float(h[k][l0 + g]) for k in ("xmin", "ymin", "xmax", "ymax"))
+    ws = datagen.loguniform_windows(2000, seed=152, lo=1e-3, hi=0.2)
+    ws = [a.copy() for a in ws[:4]]
+    ws[0][0], ws[1][0], ws[2][0], ws[3][0] = gx0 - 1e-4, gy0 - 1e-4, gx1 + 1e-4, gy1 + 1e-4
+    q = ct.DeviceBoxes.from_host(*ws)
+    ct.set_option("containment", 1)
+    ct.Prof.enable(True)
+    try:
+        ct.Prof.reset()
+        fid, tid = ct.bfs_frontier(rt, d, q)
+        assert ct.Prof.get("k_safe")[1] == 0  # the build's record was used
+        ofid, otid, _ = orc.bfs(h, tuple(sb), ws)
+        assert np.array_equal(fid, ofid) and np.array_equal(tid, otid)
+        # invert box g*16+3 so that it misses the first window: xmax left of it
+        b = g * 16 + 3
+        sb[0][b], sb[2][b] = np.float32(gx1), np.float32(gx0 - 0.01)
+        for k, a in ((0, "xmin"), (2, "xmax")):
+            ct._check(ct.lib.curtree_memcpy_h2d(ctypes.c_void_p(getattr(d.s, a)),
+                                                sb[k].ctypes.data, sb[k].nbytes))
+        ct.Prof.reset()
+        fid2, tid2 = ct.bfs_frontier(rt, d, q)
+        assert ct.Prof.get("k_safe")[1] == 1  # flags computed again
+        ofid2, otid2, _ = orc.bfs(h, tuple(sb), ws)
+        assert np.array_equal(fid2, ofid2) and np.array_equal(tid2, otid2)
+        assert not np.any((tid2 == 0) & (fid2 == sb[4][b]))
+        # a rebuild over the changed boxes records them as not all well-formed
+        rt.free()
+        rt = ct.build_rtree(d, 16, 1)
+        ct.Prof.reset()
+        fid3, tid3 = ct.bfs_frontier(rt, d, q)
+        assert ct.Prof.get("k_safe")[1] == 0
+        sb3 = d.to_host()
+        ofid3, otid3, _ = orc.bfs(rt.to_host(), sb3, ws)
+        assert np.array_equal(fid3, ofid3) and np.array_equal(tid3, otid3)
+    finally:
+        ct.Prof.enable(False)
+        ct.set_option("containment", -1)
+    rt.free()
+    d.free()
+    q.free()
