@@ -77,10 +77,10 @@ def build_bytes_per_box(kbits, fanout):
 def kernel_bytes(name, n, kbits, fanout):
     """Algorithmic bytes of one launch of build kernel `name` over n boxes.
 
-    The sort passes carry the payload: each moves a (32-bit key, id) item and
-    the 16-byte box, read and written (48 B/box).  The first pass reads the
-    caller's id and box and computes the key (44); the last writes the id and
-    the box into the caller's arrays (44).  The histogram pass reads the boxes.
+    The sort passes carry the payload: each reads and writes the box id and
+    the 16-byte box (40 B/box) and computes the sort key from the box it
+    read.  The first pass reads the caller's arrays, the last writes them.
+    The histogram pass reads the boxes.
     The final pass works in place on the caller's arrays: it reads the sorted
     boxes and ids and writes the leaf-parent nodes (the few boxes that its tie
     sort moves are not counted).  The whole build is also reported against the
@@ -88,8 +88,7 @@ def kernel_bytes(name, n, kbits, fanout):
     return {
         "k_extent": 16 * n,
         "k_hist": 16 * n,
-        # a first and a last pass of 44 B/box and two of 48
-        "k_sweep_kv": (2 * 44 + 2 * 48) / 4 * n,
+        "k_sweep_kv": 40 * n,
         "k_final": (16 + 4 + 24 / fanout) * n,
     }.get(name)
 

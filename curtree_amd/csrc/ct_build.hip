@@ -702,68 +702,55 @@ __device__ __forceinline__ void wave_permute(int dest, float4& b, int& v) {
   v = __builtin_amdgcn_ds_permute(a, v);
 }
 
-// Moves the ranked tile of the payload sort: keys, then the two box halves,
-// each reordered through LDS and scattered in runs of equal digits.  FULL:
-// every item of the tile exists (no predicated memory operations, so nothing
-// waits on earlier stores at block joins).
+// Moves the ranked tile of the payload sort: ids, then the two box halves,
+// each reordered through LDS and scattered in runs of equal digits.  On entry
+// the low halves wait in LDS at each item's tile index (s_buf as float2) and
+// the high halves in vhi.  The ids are loaded here, after the ranking (FIRST:
+// the caller's array, later passes the work array), so that their loads do not
+// hold registers through the ranking.  FULL: every item of the tile exists (no
+// predicated memory operations, so nothing waits on earlier stores at block
+// joins).
 template <bool FULL, bool FIRST>
 __device__ __forceinline__ void sweep_move(
-    uint64_t (&kv)[kSweepItems], const uint32_t (&pos)[kSweepItems],
-    float2 (&vlo)[kSweepItems], float2 (&vhi)[kSweepItems], int64_t wbase, int64_t n, int tn,
-    int shift, bool last, float* __restrict__ xmin, float* __restrict__ ymin,
-    float* __restrict__ xmax, float* __restrict__ ymax, int* __restrict__ id,
-    const float2* __restrict__ lin, const float2* __restrict__ hin, uint64_t* __restrict__ kout,
-    float2* __restrict__ lout, float2* __restrict__ hout, SweepShared& sh, uint64_t* s_buf,
-    uint8_t* s_dig) {
+    const uint32_t (&pos)[kSweepItems], const uint32_t (&dig)[kSweepItems],
+    float2 (&vhi)[kSweepItems], int64_t wbase, int64_t n, int tn, bool last,
+    float* __restrict__ xmin, float* __restrict__ ymin, float* __restrict__ xmax,
+    float* __restrict__ ymax, int* __restrict__ id, const uint32_t* __restrict__ idin,
+    uint32_t* __restrict__ idout, float2* __restrict__ lout, float2* __restrict__ hout,
+    SweepShared& sh, uint64_t* s_buf, uint8_t* s_dig) {
   const int t = threadIdx.x, lane = lane_id();
-  // later passes load the box halves here, first: their loads are then older
-  // than every store of this tile, and waiting for them never waits for a
-  // store (the first pass read them to compute the keys, and left the low
-  // halves in LDS at the item's tile index)
-  if (FIRST) {
-    const int64_t tbase = wbase - (threadIdx.x >> 6) * (kWave * kSweepItems);
-    const int* pid = id + wbase + lane;
+  const int64_t tbase = wbase - (threadIdx.x >> 6) * (kWave * kSweepItems);
+  float2 vlo[kSweepItems];
+  uint32_t vid[kSweepItems];
+  {
+    const int64_t b0 = wbase + lane;
 #pragma unroll
     for (int i = 0; i < kSweepItems; i++) {
-      const bool ok = FULL || wbase + i * kWave + lane < n;
-      kv[i] |= (uint32_t)(ok ? pid[i * kWave] : 0);
+      const bool ok = FULL || b0 + i * kWave < n;
+      vid[i] = ok ? (FIRST ? (uint32_t)id[b0 + i * kWave] : idin[b0 + i * kWave]) : 0u;
       vlo[i] = ((const float2*)s_buf)[(wbase - tbase) + i * kWave + lane];
     }
-    __syncthreads();  // every stashed half is read before the keys overwrite LDS
-  } else {
-#pragma unroll
-    for (int i = 0; i < kSweepItems; i++) {
-      const int64_t idx = wbase + i * kWave + lane;
-      const bool ok = FULL || idx < n;
-      vlo[i] = ok ? lin[idx] : make_float2(0.f, 0.f);
-      vhi[i] = ok ? hin[idx] : make_float2(0.f, 0.f);
-    }
   }
+  __syncthreads();  // every stashed half is read before the ids overwrite LDS
+  uint32_t* s_id = (uint32_t*)s_buf;
 #pragma unroll
   for (int i = 0; i < kSweepItems; i++)
-    if (FULL || wbase + i * kWave + lane < n) s_buf[pos[i]] = kv[i];
+    if (FULL || wbase + i * kWave + lane < n) {
+      s_id[pos[i]] = vid[i];
+      s_dig[pos[i]] = (uint8_t)dig[i];
+    }
   __syncthreads();
   if (last) {  // the ids, into the caller's array
 #pragma unroll
     for (int i = 0; i < kSweepItems; i++) {
       const int j = t + i * kSortThreads;
-      if (FULL || j < tn) {
-        const uint64_t kk = s_buf[j];
-        const uint32_t d = (uint32_t)(kk >> shift) & 255u;
-        s_dig[j] = (uint8_t)d;
-        id[sh.adj[d] + (uint32_t)j] = (int)(uint32_t)kk;
-      }
+      if (FULL || j < tn) id[sh.adj[s_dig[j]] + (uint32_t)j] = (int)s_id[j];
     }
   } else {
 #pragma unroll
     for (int i = 0; i < kSweepItems; i++) {
       const int j = t + i * kSortThreads;
-      if (FULL || j < tn) {
-        const uint64_t kk = s_buf[j];
-        const uint32_t d = (uint32_t)(kk >> shift) & 255u;
-        s_dig[j] = (uint8_t)d;
-        kout[sh.adj[d] + (uint32_t)j] = kk;
-      }
+      if (FULL || j < tn) idout[sh.adj[s_dig[j]] + (uint32_t)j] = s_id[j];
     }
   }
   float2* s_f2 = (float2*)s_buf;
@@ -798,16 +785,17 @@ __device__ __forceinline__ void sweep_move(
   }
 }
 
-// One 8-bit pass of the payload sort.  FIRST (step 0, which the plan always
-// runs): reads the caller's arrays and computes the keys of MODE from the
-// boxes; the last pass writes the ids and boxes into the caller's arrays in
-// the order of the 32-bit key (the plan runs at least two passes, so the last
-// never reads what it writes).
+// One 8-bit pass of the payload sort over the 32-bit sort key of MODE.  An
+// item is the box and its id (40 B moved per box); the key is computed again
+// from the box in every pass instead of being carried.  FIRST (step 0, which
+// the plan always runs) reads the caller's arrays; the last pass writes the
+// ids and boxes into the caller's arrays in the order of the key (the plan
+// runs at least two passes, so the last never reads what it writes).
 template <bool FIRST, int MODE>
 __global__ __launch_bounds__(kSortThreads) void k_sweep_kv(
     int64_t n, int step, float* __restrict__ xmin, float* __restrict__ ymin,
     float* __restrict__ xmax, float* __restrict__ ymax, int* __restrict__ id,
-    uint64_t* __restrict__ kv0, uint64_t* __restrict__ kv1, float2* __restrict__ lo0,
+    uint32_t* __restrict__ id0, uint32_t* __restrict__ id1, float2* __restrict__ lo0,
     float2* __restrict__ lo1, float2* __restrict__ hi0, float2* __restrict__ hi1, BuildCtrl* c,
     uint32_t* __restrict__ status) {
   __shared__ SweepShared sh;
@@ -817,17 +805,16 @@ __global__ __launch_bounds__(kSortThreads) void k_sweep_kv(
   if (!c->active[step]) return;
   const int t = threadIdx.x, w = t >> 6, lane = lane_id();
   const int pass = c->digit[step];
-  const int shift = 32 + 8 * pass;
+  const int shift = 8 * pass;
   const bool last = c->write_keys[step] == 0;
-  KeyParams kp{0.f, 0.f, 0.f, 0.f};
-  if (FIRST) kp = key_params<MODE>(c);
+  const KeyParams kp = key_params<MODE>(c);
   if (t == 0) sh.tile = atomicAdd(&c->ticket[step], 1u);
   for (int i = t; i < kSortWaves * 256; i += kSortThreads) (&sh.wh[0][0])[i] = 0;
   __syncthreads();
   const int64_t tile = sh.tile;
   const int src = c->src[step];
-  const uint64_t* kin = src ? kv1 : kv0;
-  uint64_t* kout = src ? kv0 : kv1;
+  const uint32_t* idin = src ? id1 : id0;
+  uint32_t* idout = src ? id0 : id1;
   const float2* lin = src ? lo1 : lo0;
   float2* lout = src ? lo0 : lo1;
   const float2* hin = src ? hi1 : hi0;
@@ -835,14 +822,12 @@ __global__ __launch_bounds__(kSortThreads) void k_sweep_kv(
 
   const int64_t tbase = tile * kSweepTile;
   const int64_t wbase = tbase + (int64_t)w * (kWave * kSweepItems);
-  uint64_t kv[kSweepItems];
   uint32_t dig[kSweepItems], pos[kSweepItems];
   float2 vlo[kSweepItems], vhi[kSweepItems];
-  // items (each load loop issues all of its loads before the first use)
+  // the boxes (each load loop issues all of its loads before the first use;
+  // one base per array and constant offsets: few address registers)
+  const int64_t b0 = wbase + lane;
   if (FIRST) {
-    // (one base per array and constant offsets: few address registers; the
-    // ids join the keys after the ranking)
-    const int64_t b0 = wbase + lane;
     const float *px0 = xmin + b0, *py0 = ymin + b0, *px1 = xmax + b0, *py1 = ymax + b0;
 #pragma unroll
     for (int i = 0; i < kSweepItems; i++) {
@@ -850,31 +835,31 @@ __global__ __launch_bounds__(kSortThreads) void k_sweep_kv(
       vlo[i] = ok ? make_float2(px0[i * kWave], py0[i * kWave]) : make_float2(0.f, 0.f);
       vhi[i] = ok ? make_float2(px1[i * kWave], py1[i * kWave]) : make_float2(0.f, 0.f);
     }
-#pragma unroll
-    for (int i = 0; i < kSweepItems; i++) {
-      kv[i] = (uint64_t)sort_key32<MODE>(vlo[i].x, vlo[i].y, vhi[i].x, vhi[i].y, kp) << 32;
-      // the low halves wait in LDS (unused until the moves) while the tile
-      // is ranked: registers for all of the box would not fit
-      ((float2*)s_buf)[(wbase - tbase) + i * kWave + lane] = vlo[i];
-    }
   } else {
+    const float2 *pl = lin + b0, *ph = hin + b0;
 #pragma unroll
     for (int i = 0; i < kSweepItems; i++) {
-      const int64_t idx = wbase + i * kWave + lane;
-      kv[i] = idx < n ? kin[idx] : 0ull;
+      const bool ok = b0 + i * kWave < n;
+      vlo[i] = ok ? pl[i * kWave] : make_float2(0.f, 0.f);
+      vhi[i] = ok ? ph[i * kWave] : make_float2(0.f, 0.f);
     }
   }
 #pragma unroll
-  for (int i = 0; i < kSweepItems; i++) dig[i] = (uint32_t)(kv[i] >> shift) & 255u;
+  for (int i = 0; i < kSweepItems; i++) {
+    dig[i] = (sort_key32<MODE>(vlo[i].x, vlo[i].y, vhi[i].x, vhi[i].y, kp) >> shift) & 255u;
+    // the low halves wait in LDS (unused until the moves) while the tile is
+    // ranked: registers for all of the box would not fit
+    ((float2*)s_buf)[(wbase - tbase) + i * kWave + lane] = vlo[i];
+  }
   sweep_rank(dig, pos, wbase, n, tile, c->base + pass * 256, status, sh);
 
   const int64_t rem = n - tbase;
   if (rem >= kSweepTile)  // full tile: no predicated loads or stores
-    sweep_move<true, FIRST>(kv, pos, vlo, vhi, wbase, n, kSweepTile, shift, last, xmin, ymin,
-                            xmax, ymax, id, lin, hin, kout, lout, hout, sh, s_buf, s_dig);
+    sweep_move<true, FIRST>(pos, dig, vhi, wbase, n, kSweepTile, last, xmin, ymin, xmax, ymax,
+                            id, idin, idout, lout, hout, sh, s_buf, s_dig);
   else
-    sweep_move<false, FIRST>(kv, pos, vlo, vhi, wbase, n, (int)rem, shift, last, xmin, ymin,
-                             xmax, ymax, id, lin, hin, kout, lout, hout, sh, s_buf, s_dig);
+    sweep_move<false, FIRST>(pos, dig, vhi, wbase, n, (int)rem, last, xmin, ymin, xmax, ymax, id,
+                             idin, idout, lout, hout, sh, s_buf, s_dig);
 }
 
 // ---------------------------------------------------------------------------
@@ -2156,8 +2141,9 @@ int build_f32(Context& ctx, curtree_bbox_f32* box, curtree_rtree_f32* rt, int ke
        !ws.fits(WsBuildRuns, (size_t)runs_cap * 16)) &&
       hipEventSynchronize(ctx.build_done) != hipSuccess)
     return fail(set_error(CURTREE_EHIP, "build: cannot wait for the previous build"));
-  // payload sort: (key32, id) and the box halves, double-buffered (48 B/box);
-  // the fallback reuses them for 64-bit keys and 32-B staging records
+  // payload sort: the ids and the box halves, double-buffered (40 B/box); the
+  // fallback uses the key buffers for 64-bit keys and the box halves for 32-B
+  // staging records
   if ((rc = ws.reserve(WsBuildKeys0, n * 8)) || (rc = ws.reserve(WsBuildKeys1, n * 8)) ||
       (rc = ws.reserve(WsBuildAos, n * 32)) ||
       (rc = ws.reserve(WsBuildVals0, n * 4)) || (rc = ws.reserve(WsBuildVals1, n * 4)) ||
@@ -2228,11 +2214,10 @@ int build_f32(Context& ctx, curtree_bbox_f32* box, curtree_rtree_f32* rt, int ke
                      1);
   for (int st = 0; st < 4; st++) {
     pf.mark(s, "k_sweep_kv");
-    auto kern = st > 0 ? k_sweep_kv<false, 0>
-                : m64     ? k_sweep_kv<true, kKeyMorton64>
-                          : k_sweep_kv<true, kKeyRef32>;
+    auto kern = m64 ? (st > 0 ? k_sweep_kv<false, kKeyMorton64> : k_sweep_kv<true, kKeyMorton64>)
+                    : (st > 0 ? k_sweep_kv<false, kKeyRef32> : k_sweep_kv<true, kKeyRef32>);
     hipLaunchKernelGGL(kern, dim3((unsigned)ntiles), dim3(kSortThreads), 0, s, n, st, box->xmin,
-                       box->ymin, box->xmax, box->ymax, box->id, sb.kv[0], sb.kv[1], sb.lo[0],
+                       box->ymin, box->xmax, box->ymax, box->id, sb.v[0], sb.v[1], sb.lo[0],
                        sb.lo[1], sb.hi[0], sb.hi[1], c, sb.status + (size_t)st * step_words);
   }
 
