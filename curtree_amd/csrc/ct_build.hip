@@ -2255,10 +2255,13 @@ int build_f32(Context& ctx, curtree_bbox_f32* box, curtree_rtree_f32* rt, int ke
 // box position) order: query ids never decrease, so the pairs of one query
 // are one contiguous segment and the canonical (query id, box id) order is
 // each segment's box ids sorted in place (the query ids stay where they are).
-//   k_seg_find    one pass over the query ids: segment heads, each segment's
-//                 length (a galloping search for its end), a flag if a query
-//                 id ever decreases, and the segments of 2 or more pairs listed
-//                 by length class;
+//   k_seg_count   per chunk of kSegChunk pairs: its segment heads (positions
+//                 whose query id differs from the one before), and a flag if
+//                 a query id ever decreases (wide loads);
+//   k_seg_scan    exclusive scan of the chunks' head counts (one block);
+//   k_seg_heads   the head positions, written in order (the head list);
+//   k_seg_classify  the segments of 2 or more pairs listed by length class
+//                 (a segment ends at the next head);
 //   k_seg_wave    segments of 2..1024 pairs, one wave each (up to 64 a bitonic
 //                 network over the lanes, longer ones four LSD passes in
 //                 registers and the wave's LDS);
@@ -2271,8 +2274,11 @@ int build_f32(Context& ctx, curtree_bbox_f32* box, curtree_rtree_f32* rt, int ke
 constexpr int kSegWaveMax = 1024;
 constexpr int kSegBlockMax = 4096;
 constexpr int kSegBigMax = 16384;
+constexpr int kSegPer = 16;               // query ids per thread of a chunk
+constexpr int kSegChunk = 256 * kSegPer;  // pairs per chunk (one block step)
 struct SegCtrl {
   unsigned long long nseg[4];  // listed segments: wave, block, big-block and huge class
+  unsigned long long nheads;   // length of the head list
   uint32_t unordered;          // a query id is below the one before it
   uint32_t pad;
 };
@@ -2285,171 +2291,161 @@ struct SegBufs {
 
 __global__ void k_seg_init(SegCtrl* c) {
   if (threadIdx.x < 4) c->nseg[threadIdx.x] = 0;
-  if (threadIdx.x == 0) c->unordered = 0;
+  if (threadIdx.x == 0) {
+    c->nheads = 0;
+    c->unordered = 0;
+  }
 }
 
-// End of the segment of query id v that starts at (or continues past) `from`
-// (tid[from - 1] == v): the first position at or after `from` whose id
-// differs, or n (ids of one segment are contiguous).  The whole wave searches:
-// lane k probes from - 1 + 2^k, the first probe that differs brackets the end,
-// and rounds of 64 evenly spaced probes narrow the bracket.  Every lane calls
-// it with the same arguments and gets the same result.
-__device__ __forceinline__ int64_t seg_end_wave(const int* __restrict__ tid, int64_t n,
-                                                int64_t from, int v) {
-  const int lane = lane_id();
-  // lo: a position known to hold v; hi: a position known to differ (or n)
-  int64_t lo = from - 1, hi = n;
-  {
-    const int64_t d = lane < 62 ? (int64_t)1 << lane : ((int64_t)1 << 62);
-    const int64_t p = lo + d;
-    const bool differ = p >= n || tid[p] != v;
-    const uint64_t m = ballot(differ);
-    if (m) {
-      const int k = __ffsll((unsigned long long)m) - 1;  // differs first at distance 2^k
-      hi = lo + ((int64_t)1 << k);
-      if (hi > n) hi = n;
-      if (k > 0) lo += (int64_t)1 << (k - 1);
+// Head bits of this thread's kSegPer positions p0 .. p0 + kSegPer - 1 of the
+// chunk at c0 (bit k: position p0 + k starts a segment); *down: a query id
+// below the one before it.  s_last: kSegChunk / kSegPer words of LDS.  Every
+// thread of the block calls it.
+__device__ __forceinline__ uint32_t seg_head_bits(int64_t n, const int* __restrict__ tid,
+                                                  int64_t c0, uint32_t* s_last, bool* down) {
+  const int t = threadIdx.x;
+  const int64_t p0 = c0 + (int64_t)t * kSegPer;
+  int v[kSegPer];
+  // (wide loads only from a 16-byte aligned array: p0 is a multiple of 16)
+  if (p0 + kSegPer <= n && ((uintptr_t)tid & 15u) == 0) {
+#pragma unroll
+    for (int q = 0; q < kSegPer / 4; q++) {
+      const int4 a = *(const int4*)(tid + p0 + 4 * q);
+      v[4 * q] = a.x;
+      v[4 * q + 1] = a.y;
+      v[4 * q + 2] = a.z;
+      v[4 * q + 3] = a.w;
     }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kSegPer; k++) v[k] = p0 + k < n ? tid[p0 + k] : 0;
   }
-  while (hi - lo > 1) {
-    // probes lo + step*(lane+1), lane 0..63, covering (lo, hi)
-    const int64_t span = hi - lo;
-    const int64_t step = (span + 63) / 64;
-    const int64_t p = lo + step * (lane + 1);
-    const bool differ = p >= hi || tid[p] != v;
-    const uint64_t m = ballot(differ);
-    const int k = __ffsll((unsigned long long)m) - 1;  // (lane 63's probe reaches hi)
-    const int64_t nhi = lo + step * (k + 1) < hi ? lo + step * (k + 1) : hi;
-    lo = lo + step * k;
-    hi = nhi;
+  __syncthreads();  // the previous chunk is done with s_last
+  s_last[t] = (uint32_t)v[kSegPer - 1];
+  __syncthreads();
+  // the id before this thread's first: the previous thread's last (thread 0:
+  // a load; position 0 always starts a segment)
+  int prev = t ? (int)s_last[t - 1] : (c0 > 0 ? tid[c0 - 1] : ~v[0]);
+  uint32_t hb = 0;
+  bool dn = false;
+#pragma unroll
+  for (int k = 0; k < kSegPer; k++) {
+    const bool valid = p0 + k < n;
+    if (valid && v[k] != prev) hb |= 1u << k;
+    dn |= valid && p0 + k > 0 && v[k] < prev;
+    prev = v[k];
   }
-  return hi;
+  *down = dn;
+  return hb;
 }
 
-// Each lane holds 8 consecutive ids (a wave: a chunk of 512), so the ids are
-// read with wide loads and a segment's end is, nearly always, the next head
-// inside the chunk: the next head bit of the same lane, else the first head
-// of a later lane (a suffix minimum over the lanes).  Only the chunk's last
-// segment can go on past it; the wave measures that one together.
-constexpr int kSegPer = 8;
-__global__ __launch_bounds__(256) void k_seg_find(int64_t n, const int* __restrict__ tid,
-                                                  SegCtrl* c, SegBufs b) {
-  const int lane = lane_id();
-  constexpr int64_t kChunk = (int64_t)kWave * kSegPer;
-  const int64_t nchunks = (n + kChunk - 1) / kChunk;
-  const int64_t wstride = (int64_t)gridDim.x * 4;
-  for (int64_t ch = blockIdx.x * 4ll + (threadIdx.x >> 6); ch < nchunks; ch += wstride) {
-    const int64_t c0 = ch * kChunk;
-    const int64_t p0 = c0 + (int64_t)lane * kSegPer;  // this lane's first position
-    int v[kSegPer];
-    // (wide loads only from a 16-byte aligned array: p0 is a multiple of 8)
-    if (p0 + kSegPer <= n && ((uintptr_t)tid & 15u) == 0) {
-      const int4 a = *(const int4*)(tid + p0);
-      const int4 d = *(const int4*)(tid + p0 + 4);
-      v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
-      v[4] = d.x; v[5] = d.y; v[6] = d.z; v[7] = d.w;
-    } else {
+__global__ __launch_bounds__(256) void k_seg_count(int64_t n, const int* __restrict__ tid,
+                                                   uint32_t* __restrict__ cnt, SegCtrl* c) {
+  __shared__ uint32_t s_last[256];
+  __shared__ uint32_t s_tmp[4];
+  const int64_t nchunks = (n + kSegChunk - 1) / kSegChunk;
+  for (int64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    bool dn;
+    const uint32_t hb = seg_head_bits(n, tid, ch * kSegChunk, s_last, &dn);
+    if (ballot(dn) && lane_id() == 0) c->unordered = 1u;
+    uint32_t tot;
+    (void)block_excl_scan<4>((uint32_t)__popc(hb), s_tmp, &tot);
+    if (threadIdx.x == 0) cnt[ch] = tot;
+  }
+}
+
+// Exclusive scan of the chunk head counts into 64-bit offsets, by one block
+// of 1024 threads (each a contiguous slice); the total is the head list's
+// length.
+__global__ __launch_bounds__(1024) void k_seg_scan(int64_t nchunks,
+                                                   const uint32_t* __restrict__ cnt,
+                                                   uint64_t* __restrict__ off, SegCtrl* c) {
+  __shared__ uint64_t s_w[16];
+  const int t = threadIdx.x, lane = lane_id(), w = t >> 6;
+  const int64_t per = (nchunks + 1023) / 1024;
+  const int64_t b0 = (int64_t)t * per, b1 = b0 + per < nchunks ? b0 + per : nchunks;
+  uint64_t sum = 0;
+  for (int64_t i = b0; i < b1; i++) sum += cnt[i];
+  uint64_t x = sum;
 #pragma unroll
-      for (int k = 0; k < kSegPer; k++) v[k] = p0 + k < n ? tid[p0 + k] : 0;
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint64_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == kWave - 1) s_w[w] = x;
+  __syncthreads();
+  uint64_t before = x - sum, total = 0;
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    if (i < w) before += s_w[i];
+    total += s_w[i];
+  }
+  for (int64_t i = b0; i < b1; i++) {
+    off[i] = before;
+    before += cnt[i];
+  }
+  if (t == 0) c->nheads = total;
+}
+
+__global__ __launch_bounds__(256) void k_seg_heads(int64_t n, const int* __restrict__ tid,
+                                                   const uint64_t* __restrict__ off,
+                                                   uint64_t* __restrict__ heads) {
+  __shared__ uint32_t s_last[256];
+  __shared__ uint32_t s_tmp[4];
+  const int64_t nchunks = (n + kSegChunk - 1) / kSegChunk;
+  for (int64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    bool dn;
+    const int64_t c0 = ch * kSegChunk;
+    uint32_t hb = seg_head_bits(n, tid, c0, s_last, &dn);
+    uint32_t tot;
+    uint64_t at = off[ch] + block_excl_scan<4>((uint32_t)__popc(hb), s_tmp, &tot);
+    const int64_t p0 = c0 + (int64_t)threadIdx.x * kSegPer;
+    while (hb) {
+      const int k = __ffs(hb) - 1;
+      hb &= hb - 1;
+      heads[at++] = (uint64_t)(p0 + k);
     }
-    // the id before this lane's first: the previous lane's last (lane 0: a load)
-    int before = __shfl_up(v[kSegPer - 1], 1);
-    if (lane == 0) before = c0 > 0 ? tid[c0 - 1] : v[0];
-    uint32_t hb = 0;  // head bits of this lane's positions
-    bool down = false;
-#pragma unroll
-    for (int k = 0; k < kSegPer; k++) {
-      const int prev = k ? v[k - 1] : before;
-      const bool valid = p0 + k < n;
-      const bool head = valid && (p0 + k == 0 || prev != v[k]);
-      if (head) hb |= 1u << k;
-      down |= head && prev > v[k];
+  }
+}
+
+// Sorts the head list into the class lists: segment lengths 2..1024 (wave
+// class, start << 11 | length), 1025..4096, 4097..16384 and longer ((start,
+// length) pairs); one-pair segments need nothing.  One thread per head; each
+// block reserves its entries of a class with one atomic.
+__global__ __launch_bounds__(256) void k_seg_classify(int64_t n, const uint64_t* __restrict__ heads,
+                                                      SegCtrl* c, SegBufs b) {
+  __shared__ uint32_t s_n[4];
+  __shared__ unsigned long long s_at[4];
+  const uint64_t nh = c->nheads;
+  for (uint64_t i0 = (uint64_t)blockIdx.x * 256; i0 < nh; i0 += (uint64_t)gridDim.x * 256) {
+    __syncthreads();  // the previous batch is done with the shared words
+    if (threadIdx.x < 4) s_n[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t i = i0 + threadIdx.x;
+    int64_t st = 0, len = 0;
+    int q = -1;
+    uint32_t r = 0;
+    if (i < nh) {
+      st = (int64_t)heads[i];
+      len = (i + 1 < nh ? (int64_t)heads[i + 1] : n) - st;
+      q = len < 2 ? -1 : len <= kSegWaveMax ? 0 : len <= kSegBlockMax ? 1 : len <= kSegBigMax ? 2 : 3;
+      if (q >= 0) r = atomicAdd(&s_n[q], 1u);
     }
-    if (ballot(down)) {
-      if (lane == 0) c->unordered = 1u;
-    }
-    // the first head of any later lane (n past the chunk: not in it)
-    const int64_t kNone = INT64_MAX;
-    int64_t nxt = hb ? p0 + (__ffs(hb) - 1) : kNone;
-    // suffix minimum over lanes > this one
-    int64_t after = kNone;
-    {
-      int64_t x = nxt;
-#pragma unroll
-      for (int o = 1; o < kWave; o <<= 1) {
-        const int64_t y = __shfl_down(x, o);
-        if (lane + o < kWave && y < x) x = y;
-      }
-      // x: min over lanes >= this one; after: min over lanes > this one
-      after = __shfl_down(x, 1);
-      if (lane == kWave - 1) after = kNone;
-    }
-    const int64_t cend = c0 + kChunk < n ? c0 + kChunk : n;
-    // the chunk's last head, if its segment reaches the chunk's end: measured
-    // by the whole wave (kNone when the chunk holds no head)
-    const int64_t last_head = [&] {
-      const uint64_t lm = ballot(hb != 0);
-      if (!lm) return kNone;
-      const int L = 63 - __clzll((long long)lm);
-      const uint32_t lb = (uint32_t)__shfl((int)hb, L);
-      return c0 + (int64_t)L * kSegPer + (31 - __clz(lb));
-    }();
-    int64_t last_end = 0;
-    if (last_head != kNone) {
-      // no head follows the last one in the chunk, so its segment holds the
-      // chunk's last id; a full chunk's segment may go on past it
-      const int lv = __shfl(v[kSegPer - 1], 63);
-      last_end = cend >= n ? n : seg_end_wave(tid, n, cend, lv);
-    }
-    // lengths and classes of this lane's heads; appends aggregated per class
-    int64_t hstart[kSegPer];
-    int64_t hlen[kSegPer];
-    int cls[kSegPer];
-#pragma unroll
-    for (int k = 0; k < kSegPer; k++) {
-      cls[k] = -1;
-      hstart[k] = 0;
-      hlen[k] = 0;
-      if ((hb >> k) & 1u) {
-        const uint32_t later = hb & ~((2u << k) - 1u);  // this lane's heads after k
-        int64_t end = later ? p0 + (__ffs(later) - 1) : after;
-        if (end == kNone) end = last_end;  // the chunk's last segment
-        const int64_t len = end - (p0 + k);
-        hstart[k] = p0 + k;
-        hlen[k] = len;
-        cls[k] = len < 2 ? -1 : len <= kSegWaveMax ? 0 : len <= kSegBlockMax ? 1
-               : len <= kSegBigMax ? 2 : 3;
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      uint32_t mine = 0;
-#pragma unroll
-      for (int k = 0; k < kSegPer; k++) mine += cls[k] == q ? 1u : 0u;
-      if (!ballot(mine != 0)) continue;
-      uint32_t inc = mine;
-#pragma unroll
-      for (int o = 1; o < kWave; o <<= 1) {
-        const uint32_t y = __shfl_up(inc, o);
-        if (lane >= o) inc += y;
-      }
-      const uint32_t wtot = __shfl(inc, kWave - 1);
-      unsigned long long at = 0;
-      if (lane == 0) at = atomicAdd(&c->nseg[q], (unsigned long long)wtot);
-      at = __shfl(at, 0);
-      uint64_t r = at + (inc - mine);
-#pragma unroll
-      for (int k = 0; k < kSegPer; k++) {
-        if (cls[k] != q) continue;
-        if (r < b.cap[q]) {
-          if (q == 0) {
-            b.small[r] = ((uint64_t)hstart[k] << 11) | (uint64_t)hlen[k];
-          } else {
-            b.big[2 * (b.off[q] + r)] = (uint64_t)hstart[k];
-            b.big[2 * (b.off[q] + r) + 1] = (uint64_t)hlen[k];
-          }
+    __syncthreads();
+    if (threadIdx.x < 4)
+      s_at[threadIdx.x] = s_n[threadIdx.x] ? atomicAdd(&c->nseg[threadIdx.x],
+                                                       (unsigned long long)s_n[threadIdx.x])
+                                           : 0ull;
+    __syncthreads();
+    if (q >= 0) {
+      const uint64_t at = s_at[q] + r;
+      if (at < b.cap[q]) {
+        if (q == 0) {
+          b.small[at] = ((uint64_t)st << 11) | (uint64_t)len;
+        } else {
+          b.big[2 * (b.off[q] + at)] = (uint64_t)st;
+          b.big[2 * (b.off[q] + at) + 1] = (uint64_t)len;
         }
-        r++;
       }
     }
   }
@@ -2772,29 +2768,26 @@ int pairs_canonical_sort(Context& ctx, int* fid, int* tid, int64_t n, hipStream_
   // lists: wave-class segments (8 B, at most n/2 of them), longer ones (16 B,
   // at most n/1025); the huge class's work array (4 B per pair)
   // (a class of segments longer than m holds at most n / (m + 1) of them)
-  SegBufs b;
-  b.cap[0] = (uint64_t)(n / 2 + 1);
-  b.cap[1] = (uint64_t)(n / (kSegWaveMax + 1) + 1);
-  b.cap[2] = (uint64_t)(n / (kSegBlockMax + 1) + 1);
-  b.cap[3] = (uint64_t)(n / (kSegBigMax + 1) + 1);
-  b.off[0] = 0;
-  b.off[1] = 0;
-  b.off[2] = b.cap[1];
-  b.off[3] = b.cap[1] + b.cap[2];
-  const uint64_t nbig = b.cap[1] + b.cap[2] + b.cap[3];
+  // chunk head counts and their offsets; then the head list and the class
+  // lists (sized once the number of heads is known); the huge class's work
+  // array (4 B per pair)
+  const int64_t nchunks = (n + kSegChunk - 1) / kSegChunk;
   if ((rc = ws.reserve(WsSortKeys0, (size_t)n * 4)) ||
-      (rc = ws.reserve(WsSortKeys1, (size_t)(b.cap[0] * 8 + nbig * 16))) ||
+      (rc = ws.reserve(WsSortVals0, (size_t)nchunks * 4)) ||
+      (rc = ws.reserve(WsSortVals1, (size_t)nchunks * 8)) ||
       (rc = ws.reserve(WsSortCtrl, sizeof(SegCtrl) > sizeof(BuildCtrl) ? sizeof(SegCtrl)
                                                                          : sizeof(BuildCtrl))))
     return rc;
   SegCtrl* c = (SegCtrl*)ws.ptr(WsSortCtrl);
-  b.small = (uint64_t*)ws.ptr(WsSortKeys1);
-  b.big = b.small + b.cap[0];
+  uint32_t* cnt = (uint32_t*)ws.ptr(WsSortVals0);
+  uint64_t* off = (uint64_t*)ws.ptr(WsSortVals1);
+  const unsigned cgrid = (unsigned)std::min<int64_t>(nchunks, 4096);
   Prof& pf = ctx.prof;
   pf.mark(s, "k_seg_find");
   hipLaunchKernelGGL(k_seg_init, dim3(1), dim3(64), 0, s, c);
-  hipLaunchKernelGGL(k_seg_find, dim3((unsigned)std::min<int64_t>((n + 2047) / 2048, 8192)),
-                     dim3(256), 0, s, n, (const int*)tid, c, b);
+  hipLaunchKernelGGL(k_seg_count, dim3(cgrid), dim3(256), 0, s, n, (const int*)tid, cnt, c);
+  hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(1024), 0, s, nchunks, (const uint32_t*)cnt, off,
+                     c);
   pf.mark(s, nullptr);
   CT_HIP(hipGetLastError());
   SegCtrl hc;
@@ -2802,27 +2795,40 @@ int pairs_canonical_sort(Context& ctx, int* fid, int* tid, int64_t n, hipStream_
   CT_HIP(hipStreamSynchronize(s));
   // query ids that ever decrease (e.g. the self-join's box ids): the full sort
   if (hc.unordered) return pairs_full_sort(ctx, fid, tid, n, s);
-  if (hc.nseg[0]) {
-    pf.mark(s, "k_seg_wave");
-    hipLaunchKernelGGL(k_seg_wave, dim3((unsigned)std::min<uint64_t>((hc.nseg[0] + 3) / 4, 2048)),
-                       dim3(256), 0, s, (const SegCtrl*)c, b, fid);
-  }
-  if (hc.nseg[1]) {
-    pf.mark(s, "k_seg_block");
-    hipLaunchKernelGGL((k_seg_block<4, 1>), dim3((unsigned)std::min<uint64_t>(hc.nseg[1], 1024)),
-                       dim3(256), 0, s, (const SegCtrl*)c, b, fid);
-  }
-  if (hc.nseg[2]) {
-    pf.mark(s, "k_seg_block16");
-    hipLaunchKernelGGL((k_seg_block<16, 2>), dim3((unsigned)std::min<uint64_t>(hc.nseg[2], 256)),
-                       dim3(1024), 0, s, (const SegCtrl*)c, b, fid);
-  }
-  if (hc.nseg[3]) {
-    pf.mark(s, "k_seg_huge");
-    hipLaunchKernelGGL(k_seg_huge, dim3((unsigned)std::min<uint64_t>(hc.nseg[3], 768)),
-                       dim3(256), 0, s, (const SegCtrl*)c, b, fid,
-                       (uint32_t*)ws.ptr(WsSortKeys0));
-  }
+  const uint64_t nh = hc.nheads;
+  // (a class of segments longer than m holds at most n / (m + 1) of them)
+  SegBufs b;
+  b.cap[0] = std::min<uint64_t>(nh, (uint64_t)(n / 2 + 1));
+  b.cap[1] = std::min<uint64_t>(nh, (uint64_t)(n / (kSegWaveMax + 1) + 1));
+  b.cap[2] = std::min<uint64_t>(nh, (uint64_t)(n / (kSegBlockMax + 1) + 1));
+  b.cap[3] = std::min<uint64_t>(nh, (uint64_t)(n / (kSegBigMax + 1) + 1));
+  b.off[0] = 0;
+  b.off[1] = 0;
+  b.off[2] = b.cap[1];
+  b.off[3] = b.cap[1] + b.cap[2];
+  const uint64_t nbig = b.cap[1] + b.cap[2] + b.cap[3];
+  if ((rc = ws.reserve(WsSortKeys1, (size_t)(nh * 8 + b.cap[0] * 8 + nbig * 16)))) return rc;
+  uint64_t* heads = (uint64_t*)ws.ptr(WsSortKeys1);
+  b.small = heads + nh;
+  b.big = b.small + b.cap[0];
+  pf.mark(s, "k_seg_find");
+  hipLaunchKernelGGL(k_seg_heads, dim3(cgrid), dim3(256), 0, s, n, (const int*)tid,
+                     (const uint64_t*)off, heads);
+  hipLaunchKernelGGL(k_seg_classify, dim3((unsigned)std::min<uint64_t>((nh + 255) / 256, 4096)),
+                     dim3(256), 0, s, n, (const uint64_t*)heads, c, b);
+  // every class launch reads its count on the device (an empty class exits)
+  pf.mark(s, "k_seg_wave");
+  hipLaunchKernelGGL(k_seg_wave, dim3((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((b.cap[0] + 3) / 4, 2048))),
+                     dim3(256), 0, s, (const SegCtrl*)c, b, fid);
+  pf.mark(s, "k_seg_block");
+  hipLaunchKernelGGL((k_seg_block<4, 1>), dim3((unsigned)std::max<uint64_t>(1, std::min<uint64_t>(b.cap[1], 1024))),
+                     dim3(256), 0, s, (const SegCtrl*)c, b, fid);
+  pf.mark(s, "k_seg_block16");
+  hipLaunchKernelGGL((k_seg_block<16, 2>), dim3((unsigned)std::max<uint64_t>(1, std::min<uint64_t>(b.cap[2], 256))),
+                     dim3(1024), 0, s, (const SegCtrl*)c, b, fid);
+  pf.mark(s, "k_seg_huge");
+  hipLaunchKernelGGL(k_seg_huge, dim3((unsigned)std::max<uint64_t>(1, std::min<uint64_t>(b.cap[3], 768))),
+                     dim3(256), 0, s, (const SegCtrl*)c, b, fid, (uint32_t*)ws.ptr(WsSortKeys0));
   pf.mark(s, nullptr);
   CT_HIP(hipGetLastError());
   return CURTREE_OK;

@@ -120,11 +120,14 @@ def test_more_than_2_30_pairs_properties():
 
 def test_unaligned_arrays_and_chunk_edges():
     # pairs that start one element into their buffers (no wide loads), with
-    # segment lengths around the scan's 512-position chunks
+    # segment lengths around the head scan's 4096-position chunks (heads on
+    # a chunk's first and last position, segments spanning many chunks, a run
+    # of one-pair segments)
     import curtree_amd as ct
 
     rng = np.random.default_rng(9)
-    lengths = [511, 1, 513, 512, 2, 1023, 1025, 4095, 4097, 16383, 16385, 3, 70_000]
+    lengths = ([511, 1, 513, 512, 2, 1023, 1025, 4095, 4097, 16383, 16385, 3, 70_000]
+               + [138, 4096, 1, 4094, 1, 8192, 12_287] + [1] * 5000 + [2, 40_000])
     fid, tid = _segments(lengths, rng)
     n = fid.shape[0]
     fp = _dev(ct, np.concatenate([[0], fid]))
