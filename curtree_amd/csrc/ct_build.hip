@@ -237,11 +237,16 @@ __device__ __forceinline__ uint32_t ref32(float a, float b, float cc, float d) {
   return ((spread16(f2u16_sat(cy)) << 1) | spread16(f2u16_sat(cx))) ^ 0x80000000u;
 }
 
+// The high half of morton64 from the high 16 bits of each quantised
+// coordinate (bit 16 + j of x lands on bit 2j, of y on bit 2j + 1): the same
+// bits as morton64(...) >> 32 with 32-bit spreads only.
 template <int MODE>
 __device__ __forceinline__ uint32_t sort_key32(float a, float b, float cc, float d,
                                                const KeyParams& p) {
   if (MODE == kKeyRef32) return ref32(a, b, cc, d);
-  return (uint32_t)(morton64(a, b, cc, d, p) >> 32);
+  const float cx = (a + cc) * 0.5f;
+  const float cy = (b + d) * 0.5f;
+  return (spread16(q32(cy, p.loy, p.sy) >> 16) << 1) | spread16(q32(cx, p.lox, p.sx) >> 16);
 }
 
 // The histograms of the four digits of the 32-bit sort keys (the first sort
@@ -449,7 +454,31 @@ constexpr uint32_t kFlagA = 1u << 30, kFlagP = 2u << 30, kValMask = (1u << 30) -
 constexpr int64_t kOnesweepGrid = 256;
 static_assert(kSortThreads >= 256, "one thread per digit");
 
+// Lanes of the wave whose 8-bit digit d equals this lane's (valid lanes
+// only): every valid lane ORs its bit into the digit's entry of a per-wave
+// table of 256 lane masks in LDS and reads the entry back; the lowest lane of
+// each digit then clears it, so the table is all zero again on return (it
+// must be on entry).  Three LDS operations per item instead of eight ballots
+// and the mask arithmetic around them.  The wave's LDS operations execute in
+// order; the fences keep the compiler from moving them.
+__device__ __forceinline__ uint64_t peer_mask(uint32_t d, bool valid, uint64_t* tab) {
+  if (valid) __hip_atomic_fetch_or(&tab[d], 1ull << lane_id(), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WAVEFRONT);
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  const uint64_t m =
+      valid ? __hip_atomic_load(&tab[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT) : 0ull;
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  if (valid && (m & lanemask_lt()) == 0)
+    __hip_atomic_store(&tab[d], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  return m;
+}
+
 struct SweepShared {
+  uint64_t match[kSortWaves][256];  // peer_mask tables (zero between items)
   uint32_t wh[kSortWaves][256];  // per-wave digit counts, then wave offsets
   uint32_t start[256];           // first tile-local position of each digit
   uint32_t adj[256];             // output position of tile-local position j: adj[d] + j
@@ -473,12 +502,7 @@ __device__ __forceinline__ void sweep_rank(const uint32_t (&dig)[I], uint32_t (&
     const int64_t idx = wbase + i * kWave + lane;
     const bool valid = idx < n;
     const uint32_t d = dig[i];
-    uint64_t m = ballot(valid);
-#pragma unroll
-    for (int b = 0; b < 8; b++) {
-      const uint64_t bb = ballot((d >> b) & 1u);
-      m &= ((d >> b) & 1u) ? bb : ~bb;
-    }
+    const uint64_t m = peer_mask(d, valid, sh.match[w]);
     const uint32_t old = sh.wh[w][d];
     pos[i] = old + (uint32_t)popc64(m & lt);
     if (valid && (m & lt) == 0) sh.wh[w][d] = old + (uint32_t)popc64(m);
@@ -567,7 +591,10 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
   // blocks loop over tickets when the grid is smaller than the tile count
   for (;;) {
     if (t == 0) sh.tile = atomicAdd(&c->ticket[step], 1u);
-    for (int i = t; i < kSortWaves * 256; i += kSortThreads) (&sh.wh[0][0])[i] = 0;
+    for (int i = t; i < kSortWaves * 256; i += kSortThreads) {
+      (&sh.wh[0][0])[i] = 0;
+      (&sh.match[0][0])[i] = 0;
+    }
     __syncthreads();
     const int64_t tile = sh.tile;
     if (tile >= ntiles) break;
@@ -809,7 +836,10 @@ __global__ __launch_bounds__(kSortThreads) void k_sweep_kv(
   const bool last = c->write_keys[step] == 0;
   const KeyParams kp = key_params<MODE>(c);
   if (t == 0) sh.tile = atomicAdd(&c->ticket[step], 1u);
-  for (int i = t; i < kSortWaves * 256; i += kSortThreads) (&sh.wh[0][0])[i] = 0;
+  for (int i = t; i < kSortWaves * 256; i += kSortThreads) {
+    (&sh.wh[0][0])[i] = 0;
+    (&sh.match[0][0])[i] = 0;
+  }
   __syncthreads();
   const int64_t tile = sh.tile;
   const int src = c->src[step];
@@ -1019,13 +1049,15 @@ __global__ __launch_bounds__(256) void k_fixup_short(int64_t n, BuildCtrl* c,
 // from a scan of the counts, the scatter and the reload through LDS.  Every
 // thread of the block calls it.
 constexpr int kRunPer = 16;  // items per lane of the LDS run sorts
-// One stable pass of lds_lsd over the 8-bit digit at bit `sh` of the items.
-// On return s_cnt[0][d] is the first position of digit d in the reordered
-// items, and x[k] the item at position wbase + 64 * k.
-template <int NW>
-__device__ __forceinline__ void lds_pass(uint64_t (&x)[kRunPer], int kUse, int wbase, int sh,
-                                         uint64_t* s_net, uint32_t (*s_cnt)[256],
-                                         uint32_t* s_wsum) {
+// One stable pass of lds_lsd over the 8-bit digit at bit `sh` of the items
+// (T: 64-bit items, a key above an index, or 32-bit keys alone).  On return
+// s_cnt[0][d] is the first position of digit d in the reordered items, and
+// x[k] the item at position wbase + 64 * k.  s_match (or null: ballots):
+// the waves' peer_mask tables, zero on entry and on return.
+template <int NW, typename T>
+__device__ __forceinline__ void lds_pass(T (&x)[kRunPer], int kUse, int wbase, int sh, T* s_net,
+                                         uint32_t (*s_cnt)[256], uint32_t* s_wsum,
+                                         uint64_t (*s_match)[256] = nullptr) {
   const int t = threadIdx.x, lane = lane_id(), w = t >> 6;
   const uint64_t lt = lanemask_lt();
 #pragma unroll
@@ -1036,10 +1068,14 @@ __device__ __forceinline__ void lds_pass(uint64_t (&x)[kRunPer], int kUse, int w
     if (k < kUse) {
       const uint32_t d = (uint32_t)(x[k] >> sh) & 255u;
       uint64_t m = ~0ull;
+      if (s_match) {
+        m = peer_mask(d, true, s_match[w]);
+      } else {
 #pragma unroll
-      for (int bit = 0; bit < 8; bit++) {
-        const uint64_t bb = ballot((d >> bit) & 1u);
-        m &= ((d >> bit) & 1u) ? bb : ~bb;
+        for (int bit = 0; bit < 8; bit++) {
+          const uint64_t bb = ballot((d >> bit) & 1u);
+          m &= ((d >> bit) & 1u) ? bb : ~bb;
+        }
       }
       const uint32_t before = s_cnt[w][d];
       rank[k] = before + (uint32_t)popc64(m & lt);
@@ -1084,14 +1120,16 @@ __device__ __forceinline__ void lds_pass(uint64_t (&x)[kRunPer], int kUse, int w
     if (k < kUse) x[k] = s_net[wbase + 64 * k];
 }
 
-template <int NW>
-__device__ __forceinline__ void lds_lsd(uint64_t (&x)[kRunPer], int kUse, int wbase,
-                                        uint64_t* s_net, uint32_t (*s_cnt)[256],
-                                        uint32_t* s_wsum) {
+// (64-bit items: the four digits of the high word; 32-bit keys: of the key)
+template <int NW, typename T>
+__device__ __forceinline__ void lds_lsd(T (&x)[kRunPer], int kUse, int wbase, T* s_net,
+                                        uint32_t (*s_cnt)[256], uint32_t* s_wsum,
+                                        uint64_t (*s_match)[256] = nullptr) {
   // (a pass reuses only its own wave's row of s_cnt before its first barrier,
   // and writes s_net only after two, so passes need no barrier between them)
+  constexpr int sh0 = sizeof(T) == 8 ? 32 : 0;
   for (int pass = 0; pass < 4; pass++)
-    lds_pass<NW>(x, kUse, wbase, 32 + 8 * pass, s_net, s_cnt, s_wsum);
+    lds_pass<NW, T>(x, kUse, wbase, sh0 + 8 * pass, s_net, s_cnt, s_wsum, s_match);
 }
 
 // Sorts the run of L <= NW * 64 * kRunPer boxes at position i by (low half,
@@ -1105,7 +1143,8 @@ __device__ __forceinline__ void lds_sort_run(int64_t i, int L, const KeyParams& 
                                              float* __restrict__ xmax, float* __restrict__ ymax,
                                              int* __restrict__ id, float4* __restrict__ stage,
                                              uint32_t* __restrict__ stage_id, uint64_t* s_net,
-                                             uint32_t (*s_cnt)[256], uint32_t* s_wsum) {
+                                             uint32_t (*s_cnt)[256], uint32_t* s_wsum,
+                                             uint64_t (*s_match)[256] = nullptr) {
   const int kUse = (L + NW * 64 - 1) / (NW * 64);  // items per lane
   const int wbase = (threadIdx.x >> 6) * 64 * kUse + lane_id();
   uint64_t x[kRunPer];
@@ -1123,7 +1162,7 @@ __device__ __forceinline__ void lds_sort_run(int64_t i, int L, const KeyParams& 
       x[k] = ((uint64_t)(uint32_t)morton64(b.x, b.y, b.z, b.w, kp) << 32) | (uint32_t)e;
     }
   }
-  lds_lsd<NW>(x, kUse, wbase, s_net, s_cnt, s_wsum);  // (its barriers order the copy)
+  lds_lsd<NW, uint64_t>(x, kUse, wbase, s_net, s_cnt, s_wsum, s_match);  // (its barriers order the copy)
 #pragma unroll
   for (int k = 0; k < kRunPer; k++) {
     const int e = wbase + 64 * k;
@@ -1156,7 +1195,7 @@ __device__ __forceinline__ void lds_sort_keys(const uint64_t* __restrict__ src,
     const int e = wbase + 64 * k;
     x[k] = k < kUse && e < L ? src[e] : ~0ull;
   }
-  lds_lsd<NW>(x, kUse, wbase, s_net, s_cnt, s_wsum);
+  lds_lsd<NW, uint64_t>(x, kUse, wbase, s_net, s_cnt, s_wsum);
 #pragma unroll
   for (int k = 0; k < kRunPer; k++) {
     const int e = wbase + 64 * k;
@@ -1175,7 +1214,7 @@ __device__ __forceinline__ void wave_sort_run(int64_t i, int L, const KeyParams&
                                               float* __restrict__ xmax, float* __restrict__ ymax,
                                               int* __restrict__ id, float4* __restrict__ stage,
                                               uint32_t* __restrict__ stage_id, uint64_t* s_w,
-                                              uint32_t* s_c) {
+                                              uint32_t* s_c, uint64_t* s_m) {
   const int lane = lane_id();
   const uint64_t lt = lanemask_lt();
   const int kUse = (L + 63) >> 6;
@@ -1205,12 +1244,7 @@ __device__ __forceinline__ void wave_sort_run(int64_t i, int L, const KeyParams&
     for (int k = 0; k < kRunPer; k++) {
       if (k < kUse) {
         const uint32_t d = (uint32_t)(x[k] >> sh) & 255u;
-        uint64_t m = ~0ull;
-#pragma unroll
-        for (int bit = 0; bit < 8; bit++) {
-          const uint64_t bb = ballot((d >> bit) & 1u);
-          m &= ((d >> bit) & 1u) ? bb : ~bb;
-        }
+        const uint64_t m = peer_mask(d, true, s_m);
         const uint32_t before = s_c[d];
         rank[k] = before + (uint32_t)popc64(m & lt);
         if ((m & lt) == 0) s_c[d] = before + (uint32_t)popc64(m);
@@ -1279,19 +1313,22 @@ __global__ __launch_bounds__(256) void k_fixup_long(int64_t n, BuildCtrl* c,
   __shared__ uint64_t s_net[kFixLong];
   __shared__ __align__(16) uint32_t s_cnt[4][256];  // per wave, per digit: count -> offset
   __shared__ uint32_t s_wsum[4];
+  __shared__ uint64_t s_match[4][256];  // peer_mask tables
   const uint32_t nl = c->nlong, nm = c->nmid;
   if ((blockIdx.x >= nl && blockIdx.x * 4u >= nm) || c->fallback) return;
   const KeyParams kp = key_params<kKeyMorton64>(c);
+  for (int i = threadIdx.x; i < 4 * 256; i += 256) (&s_match[0][0])[i] = 0;
+  __syncthreads();
   for (uint32_t r = blockIdx.x; r < nl; r += gridDim.x) {
     const uint32_t q = runs[3 * (size_t)runs_cap + r];
     lds_sort_run<4>(runs[q], (int)runs[runs_cap + q], kp, xmin, ymin, xmax, ymax, id, stage,
-                    stage_id, s_net, s_cnt, s_wsum);
+                    stage_id, s_net, s_cnt, s_wsum, s_match);
   }
   const int w = threadIdx.x >> 6;
   for (uint32_t r = blockIdx.x * 4u + w; r < nm; r += gridDim.x * 4u) {
     const uint32_t q = runs[2 * (size_t)runs_cap + r];
     wave_sort_run(runs[q], (int)runs[runs_cap + q], kp, xmin, ymin, xmax, ymax, id, stage,
-                  stage_id, s_net + kWaveSortMax * w, s_cnt[w]);
+                  stage_id, s_net + kWaveSortMax * w, s_cnt[w], s_match[w]);
   }
 }
 
@@ -2459,9 +2496,12 @@ __global__ __launch_bounds__(256) void k_seg_wave(const SegCtrl* c, SegBufs b,
                                                   int* __restrict__ fid) {
   __shared__ uint32_t s_w[4][kSegWaveMax];
   __shared__ __align__(16) uint32_t s_c[4][256];
+  __shared__ uint64_t s_m[4][256];  // peer_mask tables
   const uint64_t nseg = c->nseg[0] < b.cap[0] ? c->nseg[0] : b.cap[0];
   const int w = threadIdx.x >> 6, lane = lane_id();
   const uint64_t lt = lanemask_lt();
+#pragma unroll
+  for (int k = 0; k < 4; k++) s_m[w][lane + 64 * k] = 0;  // (each wave uses its own row)
   for (uint64_t sg = (uint64_t)blockIdx.x * 4 + w; sg < nseg; sg += (uint64_t)gridDim.x * 4) {
     const uint64_t e = b.small[sg];
     const int64_t st = (int64_t)(e >> 11);
@@ -2503,12 +2543,7 @@ __global__ __launch_bounds__(256) void k_seg_wave(const SegCtrl* c, SegBufs b,
       for (int k = 0; k < 16; k++) {
         if (k < kUse) {
           const uint32_t d = (x[k] >> sh) & 255u;
-          uint64_t m = ~0ull;
-#pragma unroll
-          for (int bit = 0; bit < 8; bit++) {
-            const uint64_t bb = ballot((d >> bit) & 1u);
-            m &= ((d >> bit) & 1u) ? bb : ~bb;
-          }
+          const uint64_t m = peer_mask(d, true, s_m[w]);
           const uint32_t before = s_c[w][d];
           rank[k] = before + (uint32_t)popc64(m & lt);
           if ((m & lt) == 0) s_c[w][d] = before + (uint32_t)popc64(m);
@@ -2553,26 +2588,31 @@ __global__ __launch_bounds__(256) void k_seg_wave(const SegCtrl* c, SegBufs b,
 template <int NW, int CLS>
 __global__ __launch_bounds__(NW * 64) void k_seg_block(const SegCtrl* c, SegBufs b,
                                                        int* __restrict__ fid) {
-  __shared__ uint64_t s_net[NW * 64 * kRunPer];
+  __shared__ uint32_t s_net[NW * 64 * kRunPer];
   __shared__ __align__(16) uint32_t s_cnt[NW][256];
   __shared__ uint32_t s_wsum[NW];
+  __shared__ uint64_t s_match[NW][256];  // peer_mask tables
   const uint64_t nseg = c->nseg[CLS] < b.cap[CLS] ? c->nseg[CLS] : b.cap[CLS];
+  for (int i = threadIdx.x; i < NW * 256; i += NW * 64) (&s_match[0][0])[i] = 0;
+  __syncthreads();
   for (uint64_t sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
     const int64_t st = (int64_t)b.big[2 * (b.off[CLS] + sg)];
     const int L = (int)b.big[2 * (b.off[CLS] + sg) + 1];
     const int kUse = (L + NW * 64 - 1) / (NW * 64);
     const int wbase = (threadIdx.x >> 6) * 64 * kUse + lane_id();
-    uint64_t x[kRunPer];
+    // (box ids alone: equal ids are indistinguishable, so no index rides
+    // along; slots past the segment hold the largest key and stay after it)
+    uint32_t x[kRunPer];
 #pragma unroll
     for (int k = 0; k < kRunPer; k++) {
       const int p = wbase + 64 * k;
-      x[k] = (k < kUse && p < L) ? ((uint64_t)id_key(fid[st + p]) << 32) | (uint32_t)p : ~0ull;
+      x[k] = (k < kUse && p < L) ? id_key(fid[st + p]) : 0xFFFFFFFFu;
     }
-    lds_lsd<NW>(x, kUse, wbase, s_net, s_cnt, s_wsum);
+    lds_lsd<NW, uint32_t>(x, kUse, wbase, s_net, s_cnt, s_wsum, s_match);
 #pragma unroll
     for (int k = 0; k < kRunPer; k++) {
       const int p = wbase + 64 * k;
-      if (k < kUse && p < L) fid[st + p] = key_id((uint32_t)(x[k] >> 32));
+      if (k < kUse && p < L) fid[st + p] = key_id(x[k]);
     }
     __syncthreads();  // s_net and s_cnt are reused by the next segment
   }
@@ -2591,7 +2631,9 @@ __global__ __launch_bounds__(256) void k_seg_huge(const SegCtrl* c, SegBufs b,
   __shared__ uint32_t s_hist[4][256];
   __shared__ uint32_t s_base[256];
   __shared__ int s_skip[4];
+  __shared__ uint64_t s_match[4][256];  // peer_mask tables
   const int t = threadIdx.x;
+  for (int i = t; i < 4 * 256; i += 256) (&s_match[0][0])[i] = 0;
   const uint64_t nseg = c->nseg[3] < b.cap[3] ? c->nseg[3] : b.cap[3];
   for (uint64_t sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
     const uint64_t slot = b.off[3] + sg;
@@ -2640,7 +2682,7 @@ __global__ __launch_bounds__(256) void k_seg_huge(const SegCtrl* c, SegBufs b,
                             : (uint32_t)fid[st + c0 + q];
           x[k] = (k < kUse && q < Lc) ? ((uint64_t)key << 32) | (uint32_t)q : ~0ull;
         }
-        lds_pass<4>(x, kUse, wbase, 32 + 8 * p, s_net, s_cnt, s_wsum);
+        lds_pass<4, uint64_t>(x, kUse, wbase, 32 + 8 * p, s_net, s_cnt, s_wsum, s_match);
         // item at chunk position q: digit d, output base[d] + q - start of d
         // in the chunk (the padding sorts after every item of the chunk)
 #pragma unroll
