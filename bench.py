@@ -680,7 +680,8 @@ def canonical_leg(ct, tree, work, q, H, fbuf, tbuf, steps):
     ct._check(ct.lib.curtree_pairs_canonical_sort(fbuf, tbuf, int(H), None))
     ct.Prof.enable(False)
     kernels = {}
-    for name in ("k_seg_find", "k_seg_wave", "k_seg_block", "k_seg_block16", "k_seg_huge"):
+    for name in ("k_seg_find", "k_seg_split", "k_seg_wave", "k_seg_block", "k_seg_block16",
+                 "k_seg_huge"):
         tot, cnt = ct.Prof.get(name)
         if cnt:
             kernels[name] = tot

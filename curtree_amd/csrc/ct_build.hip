@@ -2314,23 +2314,29 @@ constexpr int kSegBigMax = 16384;
 constexpr int kSegPer = 16;               // query ids per thread of a chunk
 constexpr int kSegChunk = 256 * kSegPer;  // pairs per chunk (one block step)
 struct SegCtrl {
-  unsigned long long nseg[4];  // listed segments: wave, block, big-block and huge class
+  // listed segments: wave, block, big-block and long class, and the buckets of
+  // long segments too long for a block
+  unsigned long long nseg[5];
   unsigned long long nheads;   // length of the head list
   uint32_t unordered;          // a query id is below the one before it
-  uint32_t pad;
+  uint32_t too_long;           // a segment holds 2^32 pairs or more (not sorted)
 };
+// A listed segment's ids are in the box id array, or (the buckets of a long
+// segment, kSegFromTmp) at the same positions of the work array.
 struct SegBufs {
-  uint64_t* small;  // wave class: start << 11 | length (length <= 1024)
-  uint64_t* big;    // the other classes: (start, length) pairs, a region each
-  uint64_t off[4];  // first entry of each class's region in `big` (classes 1..3)
-  uint64_t cap[4];  // entries each class's list holds
+  uint64_t* small;  // wave class: start << 12 | from_tmp << 11 | length (<= 1024)
+  uint64_t* big;    // the other classes: (start, length | from_tmp << 63), a region each
+  uint64_t off[5];  // first entry of each class's region in `big` (classes 1..4)
+  uint64_t cap[5];  // entries each class's list holds
 };
+constexpr uint64_t kSegFromTmp = 1ull << 63;
 
 __global__ void k_seg_init(SegCtrl* c) {
-  if (threadIdx.x < 4) c->nseg[threadIdx.x] = 0;
+  if (threadIdx.x < 5) c->nseg[threadIdx.x] = 0;
   if (threadIdx.x == 0) {
     c->nheads = 0;
     c->unordered = 0;
+    c->too_long = 0;
   }
 }
 
@@ -2466,6 +2472,10 @@ __global__ __launch_bounds__(256) void k_seg_classify(int64_t n, const uint64_t*
       st = (int64_t)heads[i];
       len = (i + 1 < nh ? (int64_t)heads[i + 1] : n) - st;
       q = len < 2 ? -1 : len <= kSegWaveMax ? 0 : len <= kSegBlockMax ? 1 : len <= kSegBigMax ? 2 : 3;
+      if (len > (int64_t)0xFFFFFFFFll) {  // (positions in a segment are 32-bit)
+        q = -1;
+        c->too_long = 1u;
+      }
       if (q >= 0) r = atomicAdd(&s_n[q], 1u);
     }
     __syncthreads();
@@ -2478,7 +2488,7 @@ __global__ __launch_bounds__(256) void k_seg_classify(int64_t n, const uint64_t*
       const uint64_t at = s_at[q] + r;
       if (at < b.cap[q]) {
         if (q == 0) {
-          b.small[at] = ((uint64_t)st << 11) | (uint64_t)len;
+          b.small[at] = ((uint64_t)st << 12) | (uint64_t)len;
         } else {
           b.big[2 * (b.off[q] + at)] = (uint64_t)st;
           b.big[2 * (b.off[q] + at) + 1] = (uint64_t)len;
@@ -2493,7 +2503,8 @@ __device__ __forceinline__ int key_id(uint32_t k) { return (int)(k ^ 0x80000000u
 
 // Segments of 2..1024 box ids, one wave each.
 __global__ __launch_bounds__(256) void k_seg_wave(const SegCtrl* c, SegBufs b,
-                                                  int* __restrict__ fid) {
+                                                  int* __restrict__ fid,
+                                                  const int* __restrict__ tmp) {
   __shared__ uint32_t s_w[4][kSegWaveMax];
   __shared__ __align__(16) uint32_t s_c[4][256];
   __shared__ uint64_t s_m[4][256];  // peer_mask tables
@@ -2504,12 +2515,13 @@ __global__ __launch_bounds__(256) void k_seg_wave(const SegCtrl* c, SegBufs b,
   for (int k = 0; k < 4; k++) s_m[w][lane + 64 * k] = 0;  // (each wave uses its own row)
   for (uint64_t sg = (uint64_t)blockIdx.x * 4 + w; sg < nseg; sg += (uint64_t)gridDim.x * 4) {
     const uint64_t e = b.small[sg];
-    const int64_t st = (int64_t)(e >> 11);
+    const int64_t st = (int64_t)(e >> 12);
     const int L = (int)(e & 2047u);
+    const int* src = ((e >> 11) & 1u) ? tmp : fid;
     if (L <= 64) {
       // bitonic network over the lanes (lanes past the segment hold the
       // largest key and stay at the end)
-      uint32_t x = lane < L ? id_key(fid[st + lane]) : 0xFFFFFFFFu;
+      uint32_t x = lane < L ? id_key(src[st + lane]) : 0xFFFFFFFFu;
 #pragma unroll
       for (int k = 2; k <= 64; k <<= 1) {
 #pragma unroll
@@ -2530,9 +2542,27 @@ __global__ __launch_bounds__(256) void k_seg_wave(const SegCtrl* c, SegBufs b,
 #pragma unroll
     for (int k = 0; k < 16; k++) {
       const int p = 64 * k + lane;
-      x[k] = (k < kUse && p < L) ? id_key(fid[st + p]) : 0xFFFFFFFFu;
+      x[k] = (k < kUse && p < L) ? id_key(src[st + p]) : 0xFFFFFFFFu;
     }
-    for (int pass = 0; pass < 4; pass++) {
+    // digits above the highest bit where the keys differ are shared by all
+    // of them (keys between the smallest and the largest share their common
+    // prefix): passes over them change nothing.  (The padding keys hold the
+    // largest digit in every pass, so they stay after the segment.)
+    uint32_t mn = 0xFFFFFFFFu, mx = 0u;
+#pragma unroll
+    for (int k = 0; k < 16; k++)
+      if (k < kUse && 64 * k + lane < L) {
+        mn = x[k] < mn ? x[k] : mn;
+        mx = x[k] > mx ? x[k] : mx;
+      }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const uint32_t a = (uint32_t)__shfl_xor((int)mn, o), z = (uint32_t)__shfl_xor((int)mx, o);
+      mn = a < mn ? a : mn;
+      mx = z > mx ? z : mx;
+    }
+    const int npass = mn != mx ? ((31 - __clz((int)(mn ^ mx))) >> 3) + 1 : 0;
+    for (int pass = 0; pass < npass; pass++) {
       const int sh = 8 * pass;
 #pragma unroll
       for (int k = 0; k < 4; k++) s_c[w][lane + 64 * k] = 0;
@@ -2587,17 +2617,21 @@ __global__ __launch_bounds__(256) void k_seg_wave(const SegCtrl* c, SegBufs b,
 // class 2), one block of NW waves each, sorted whole in LDS.
 template <int NW, int CLS>
 __global__ __launch_bounds__(NW * 64) void k_seg_block(const SegCtrl* c, SegBufs b,
-                                                       int* __restrict__ fid) {
+                                                       int* __restrict__ fid,
+                                                       const int* __restrict__ tmp) {
   __shared__ uint32_t s_net[NW * 64 * kRunPer];
   __shared__ __align__(16) uint32_t s_cnt[NW][256];
   __shared__ uint32_t s_wsum[NW];
   __shared__ uint64_t s_match[NW][256];  // peer_mask tables
+  __shared__ uint32_t s_mm[2][NW];
   const uint64_t nseg = c->nseg[CLS] < b.cap[CLS] ? c->nseg[CLS] : b.cap[CLS];
   for (int i = threadIdx.x; i < NW * 256; i += NW * 64) (&s_match[0][0])[i] = 0;
   __syncthreads();
   for (uint64_t sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
     const int64_t st = (int64_t)b.big[2 * (b.off[CLS] + sg)];
-    const int L = (int)b.big[2 * (b.off[CLS] + sg) + 1];
+    const uint64_t lf = b.big[2 * (b.off[CLS] + sg) + 1];
+    const int L = (int)(lf & ~kSegFromTmp);
+    const int* src = (lf & kSegFromTmp) ? tmp : fid;
     const int kUse = (L + NW * 64 - 1) / (NW * 64);
     const int wbase = (threadIdx.x >> 6) * 64 * kUse + lane_id();
     // (box ids alone: equal ids are indistinguishable, so no index rides
@@ -2606,9 +2640,43 @@ __global__ __launch_bounds__(NW * 64) void k_seg_block(const SegCtrl* c, SegBufs
 #pragma unroll
     for (int k = 0; k < kRunPer; k++) {
       const int p = wbase + 64 * k;
-      x[k] = (k < kUse && p < L) ? id_key(fid[st + p]) : 0xFFFFFFFFu;
+      x[k] = (k < kUse && p < L) ? id_key(src[st + p]) : 0xFFFFFFFFu;
     }
-    lds_lsd<NW, uint32_t>(x, kUse, wbase, s_net, s_cnt, s_wsum, s_match);
+    // buckets of long segments (their keys share the top bits): only the
+    // digits up to the highest bit where the keys differ (as in k_seg_wave);
+    // other segments take all four passes
+    int npass = 4;
+    if (lf & kSegFromTmp) {
+      uint32_t mn = 0xFFFFFFFFu, mx = 0u;
+#pragma unroll
+      for (int k = 0; k < kRunPer; k++)
+        if (k < kUse && wbase + 64 * k < L) {
+          mn = x[k] < mn ? x[k] : mn;
+          mx = x[k] > mx ? x[k] : mx;
+        }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t a = (uint32_t)__shfl_xor((int)mn, o), z = (uint32_t)__shfl_xor((int)mx, o);
+        mn = a < mn ? a : mn;
+        mx = z > mx ? z : mx;
+    }
+    if (lane_id() == 0) {
+      s_mm[0][threadIdx.x >> 6] = mn;
+      s_mm[1][threadIdx.x >> 6] = mx;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NW; i++) {
+      mn = s_mm[0][i] < mn ? s_mm[0][i] : mn;
+      mx = s_mm[1][i] > mx ? s_mm[1][i] : mx;
+    }
+    npass = mn != mx ? ((31 - __clz((int)(mn ^ mx))) >> 3) + 1 : 0;
+    }
+    // (a pass reuses only its own wave's row of s_cnt before its first
+    // barrier, and writes s_net only after two, so passes need no barrier
+    // between them)
+    for (int pass = 0; pass < npass; pass++)
+      lds_pass<NW, uint32_t>(x, kUse, wbase, 8 * pass, s_net, s_cnt, s_wsum, s_match);
 #pragma unroll
     for (int k = 0; k < kRunPer; k++) {
       const int p = wbase + 64 * k;
@@ -2618,10 +2686,133 @@ __global__ __launch_bounds__(NW * 64) void k_seg_block(const SegCtrl* c, SegBufs
   }
 }
 
-// Segments longer than 16384 box ids, one block each: stable LSD passes
-// between fid and the work array `tmp` (same positions).  A digit that every
-// id of the segment shares is skipped; an odd number of passes ends with a
-// copy back.
+// Segments longer than 16384 box ids, one block of 1024 threads each: the
+// ids' key range (a first read), a histogram of the top bits of (key - min)
+// over nb <= kSplitBins buckets chosen so that a bucket averages kSplitTarget
+// ids (a second read), and the ids scattered into their buckets in the work
+// array at the segment's positions (a third read).  Buckets hold consecutive
+// key ranges, so the segment is sorted once every bucket is: a bucket of one
+// id is copied back here, the others are listed in the class of their length,
+// marked as read from the work array, and sorted by the class kernels (a
+// bucket longer than kSegBigMax goes to k_seg_huge).
+constexpr int kSplitBins = 4096;
+constexpr int kSplitTarget = 2048;
+__global__ __launch_bounds__(1024) void k_seg_split(SegCtrl* c, SegBufs b, int* __restrict__ fid,
+                                                    int* __restrict__ tmp) {
+  __shared__ uint32_t s_cnt[kSplitBins];      // counts, cursors, then list slots
+  __shared__ uint32_t s_off[kSplitBins + 1];  // first position of each bucket
+  __shared__ uint32_t s_r[2][16];
+  __shared__ uint32_t s_n[5];
+  __shared__ unsigned long long s_at[5];
+  const int t = threadIdx.x, lane = lane_id(), w = t >> 6;
+  const uint64_t nseg = c->nseg[3] < b.cap[3] ? c->nseg[3] : b.cap[3];
+  for (uint64_t sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
+    const uint64_t slot = b.off[3] + sg;
+    const int64_t st = (int64_t)b.big[2 * slot];
+    const int64_t L = (int64_t)b.big[2 * slot + 1];  // (< 2^32: checked when listed)
+    // the key range
+    uint32_t mn = 0xFFFFFFFFu, mx = 0u;
+    for (int64_t i = t; i < L; i += 1024) {
+      const uint32_t k = id_key(fid[st + i]);
+      mn = k < mn ? k : mn;
+      mx = k > mx ? k : mx;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const uint32_t a = (uint32_t)__shfl_xor((int)mn, o), z = (uint32_t)__shfl_xor((int)mx, o);
+      mn = a < mn ? a : mn;
+      mx = z > mx ? z : mx;
+    }
+    __syncthreads();  // the previous segment is done with the shared words
+    if (lane == 0) {
+      s_r[0][w] = mn;
+      s_r[1][w] = mx;
+    }
+    if (t < 5) s_n[t] = 0;
+    __syncthreads();
+    mn = s_r[0][0];
+    mx = s_r[1][0];
+#pragma unroll
+    for (int i = 1; i < 16; i++) {
+      mn = s_r[0][i] < mn ? s_r[0][i] : mn;
+      mx = s_r[1][i] > mx ? s_r[1][i] : mx;
+    }
+    // nb = 2^lb buckets; bucket of key k: (k - mn) >> sh < nb
+    int lb = 1;
+    while (lb < 12 && ((int64_t)1 << lb) * kSplitTarget < L) lb++;
+    const int nb = 1 << lb;
+    const uint32_t span = mx - mn;
+    const int bits = span ? 32 - __clz((int)span) : 0;
+    const int sh = bits > lb ? bits - lb : 0;
+    for (int j = t; j < nb; j += 1024) s_cnt[j] = 0;
+    __syncthreads();
+    for (int64_t i = t; i < L; i += 1024)
+      atomicAdd(&s_cnt[(id_key(fid[st + i]) - mn) >> sh], 1u);
+    __syncthreads();
+    // exclusive scan of the counts (thread t: buckets 4t .. 4t+3), cursors 0
+    {
+      uint32_t v[4], sum = 0;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int j = 4 * t + q;
+        v[q] = j < nb ? s_cnt[j] : 0u;
+        sum += v[q];
+      }
+      uint32_t tot;
+      uint32_t ex = block_excl_scan<16>(sum, &s_r[0][0], &tot);
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int j = 4 * t + q;
+        if (j < nb) {
+          s_off[j] = ex;
+          s_cnt[j] = 0;
+        }
+        ex += v[q];
+      }
+      if (t == 0) s_off[nb] = tot;
+    }
+    __syncthreads();
+    for (int64_t i = t; i < L; i += 1024) {
+      const int v = fid[st + i];
+      const uint32_t d = (id_key(v) - mn) >> sh;
+      tmp[st + s_off[d] + atomicAdd(&s_cnt[d], 1u)] = v;
+    }
+    __syncthreads();  // (the block's stores to tmp are visible to it after the barrier)
+    // buckets: one id copied back; the others listed by the class of their
+    // length (local slot in s_cnt, then one reservation per class)
+    for (int j = t; j < nb; j += 1024) {
+      const uint32_t len = s_off[j + 1] - s_off[j];
+      if (len == 1) fid[st + s_off[j]] = tmp[st + s_off[j]];
+      const int q = len < 2 ? -1 : len <= (uint32_t)kSegWaveMax ? 0 : len <= (uint32_t)kSegBlockMax ? 1
+                  : len <= (uint32_t)kSegBigMax ? 2 : 4;
+      if (q >= 0) s_cnt[j] = atomicAdd(&s_n[q], 1u);
+    }
+    __syncthreads();
+    if (t < 5) s_at[t] = s_n[t] ? atomicAdd(&c->nseg[t], (unsigned long long)s_n[t]) : 0ull;
+    __syncthreads();
+    for (int j = t; j < nb; j += 1024) {
+      const uint32_t len = s_off[j + 1] - s_off[j];
+      const int q = len < 2 ? -1 : len <= (uint32_t)kSegWaveMax ? 0 : len <= (uint32_t)kSegBlockMax ? 1
+                  : len <= (uint32_t)kSegBigMax ? 2 : 4;
+      if (q < 0) continue;
+      const uint64_t at = s_at[q] + s_cnt[j];
+      if (at >= b.cap[q]) continue;  // (capacities cover every bucket)
+      const uint64_t bst = (uint64_t)(st + s_off[j]);
+      if (q == 0) {
+        b.small[at] = (bst << 12) | (1ull << 11) | (uint64_t)len;
+      } else {
+        b.big[2 * (b.off[q] + at)] = bst;
+        b.big[2 * (b.off[q] + at) + 1] = (uint64_t)len | kSegFromTmp;
+      }
+    }
+  }
+}
+
+// Buckets of long segments that are still longer than 16384 box ids (ids
+// crowded in a small part of their range), one block each: copied back from
+// the work array, then stable LSD passes between fid and the work array `tmp`
+// (same positions).  A digit that every id of the bucket shares is skipped;
+// an odd number of passes ends with a copy back.
 __global__ __launch_bounds__(256) void k_seg_huge(const SegCtrl* c, SegBufs b,
                                                   int* __restrict__ fid,
                                                   uint32_t* __restrict__ tmp) {
@@ -2634,11 +2825,14 @@ __global__ __launch_bounds__(256) void k_seg_huge(const SegCtrl* c, SegBufs b,
   __shared__ uint64_t s_match[4][256];  // peer_mask tables
   const int t = threadIdx.x;
   for (int i = t; i < 4 * 256; i += 256) (&s_match[0][0])[i] = 0;
-  const uint64_t nseg = c->nseg[3] < b.cap[3] ? c->nseg[3] : b.cap[3];
+  const uint64_t nseg = c->nseg[4] < b.cap[4] ? c->nseg[4] : b.cap[4];
   for (uint64_t sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
-    const uint64_t slot = b.off[3] + sg;
+    const uint64_t slot = b.off[4] + sg;
     const int64_t st = (int64_t)b.big[2 * slot];
-    const int64_t L = (int64_t)b.big[2 * slot + 1];
+    const uint64_t lf = b.big[2 * slot + 1];
+    const int64_t L = (int64_t)(lf & ~kSegFromTmp);
+    if (lf & kSegFromTmp)
+      for (int64_t i = t; i < L; i += 256) fid[st + i] = (int)tmp[st + i];
 #pragma unroll
     for (int p = 0; p < 4; p++) s_hist[p][t] = 0;
     __syncthreads();
@@ -2838,42 +3032,61 @@ int pairs_canonical_sort(Context& ctx, int* fid, int* tid, int64_t n, hipStream_
   // query ids that ever decrease (e.g. the self-join's box ids): the full sort
   if (hc.unordered) return pairs_full_sort(ctx, fid, tid, n, s);
   const uint64_t nh = hc.nheads;
-  // (a class of segments longer than m holds at most n / (m + 1) of them)
+  // (a class of segments longer than m holds at most n / (m + 1) of them; the
+  // buckets of long segments come on top: at most 2 + (segment length) / 1024
+  // per segment, so fewer than n / 1024 + n / 8192 in all)
   SegBufs b;
-  b.cap[0] = std::min<uint64_t>(nh, (uint64_t)(n / 2 + 1));
-  b.cap[1] = std::min<uint64_t>(nh, (uint64_t)(n / (kSegWaveMax + 1) + 1));
-  b.cap[2] = std::min<uint64_t>(nh, (uint64_t)(n / (kSegBlockMax + 1) + 1));
+  const uint64_t extra = (uint64_t)(n / 1024 + n / 8192 + 16);
+  b.cap[0] = std::min<uint64_t>(nh, (uint64_t)(n / 2 + 1)) + extra;
+  b.cap[1] = std::min<uint64_t>(nh, (uint64_t)(n / (kSegWaveMax + 1) + 1)) + extra;
+  b.cap[2] = std::min<uint64_t>(nh, (uint64_t)(n / (kSegBlockMax + 1) + 1)) + extra;
   b.cap[3] = std::min<uint64_t>(nh, (uint64_t)(n / (kSegBigMax + 1) + 1));
+  b.cap[4] = (uint64_t)(n / (kSegBigMax + 1) + 1);
   b.off[0] = 0;
   b.off[1] = 0;
   b.off[2] = b.cap[1];
-  b.off[3] = b.cap[1] + b.cap[2];
-  const uint64_t nbig = b.cap[1] + b.cap[2] + b.cap[3];
+  b.off[3] = b.off[2] + b.cap[2];
+  b.off[4] = b.off[3] + b.cap[3];
+  const uint64_t nbig = b.off[4] + b.cap[4];
   if ((rc = ws.reserve(WsSortKeys1, (size_t)(nh * 8 + b.cap[0] * 8 + nbig * 16)))) return rc;
   uint64_t* heads = (uint64_t*)ws.ptr(WsSortKeys1);
   b.small = heads + nh;
   b.big = b.small + b.cap[0];
+  int* tmp = (int*)ws.ptr(WsSortKeys0);
   pf.mark(s, "k_seg_find");
   hipLaunchKernelGGL(k_seg_heads, dim3(cgrid), dim3(256), 0, s, n, (const int*)tid,
                      (const uint64_t*)off, heads);
   hipLaunchKernelGGL(k_seg_classify, dim3((unsigned)std::min<uint64_t>((nh + 255) / 256, 4096)),
                      dim3(256), 0, s, n, (const uint64_t*)heads, c, b);
-  // every class launch reads its count on the device (an empty class exits)
+  // long segments: bucketed into the work array and listed in the classes
+  // below (every class launch reads its count on the device; an empty class
+  // exits)
+  auto grid = [](uint64_t want, uint64_t cap) {
+    return dim3((unsigned)std::max<uint64_t>(1, std::min<uint64_t>(want, cap)));
+  };
+  pf.mark(s, "k_seg_split");
+  hipLaunchKernelGGL(k_seg_split, grid(b.cap[3], 512), dim3(1024), 0, s, c, b, fid, tmp);
   pf.mark(s, "k_seg_wave");
-  hipLaunchKernelGGL(k_seg_wave, dim3((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((b.cap[0] + 3) / 4, 2048))),
-                     dim3(256), 0, s, (const SegCtrl*)c, b, fid);
+  hipLaunchKernelGGL(k_seg_wave, grid((b.cap[0] + 3) / 4, 2048), dim3(256), 0, s,
+                     (const SegCtrl*)c, b, fid, (const int*)tmp);
   pf.mark(s, "k_seg_block");
-  hipLaunchKernelGGL((k_seg_block<4, 1>), dim3((unsigned)std::max<uint64_t>(1, std::min<uint64_t>(b.cap[1], 1024))),
-                     dim3(256), 0, s, (const SegCtrl*)c, b, fid);
+  hipLaunchKernelGGL((k_seg_block<4, 1>), grid(b.cap[1], 1024), dim3(256), 0, s,
+                     (const SegCtrl*)c, b, fid, (const int*)tmp);
   pf.mark(s, "k_seg_block16");
-  hipLaunchKernelGGL((k_seg_block<16, 2>), dim3((unsigned)std::max<uint64_t>(1, std::min<uint64_t>(b.cap[2], 256))),
-                     dim3(1024), 0, s, (const SegCtrl*)c, b, fid);
+  hipLaunchKernelGGL((k_seg_block<16, 2>), grid(b.cap[2], 256), dim3(1024), 0, s,
+                     (const SegCtrl*)c, b, fid, (const int*)tmp);
   pf.mark(s, "k_seg_huge");
-  hipLaunchKernelGGL(k_seg_huge, dim3((unsigned)std::max<uint64_t>(1, std::min<uint64_t>(b.cap[3], 768))),
-                     dim3(256), 0, s, (const SegCtrl*)c, b, fid, (uint32_t*)ws.ptr(WsSortKeys0));
+  hipLaunchKernelGGL(k_seg_huge, grid(b.cap[4], 768), dim3(256), 0, s, (const SegCtrl*)c, b,
+                     fid, (uint32_t*)tmp);
   pf.mark(s, nullptr);
   CT_HIP(hipGetLastError());
+  CT_HIP(hipMemcpyAsync(&hc, c, sizeof hc, hipMemcpyDeviceToHost, s));
+  CT_HIP(hipStreamSynchronize(s));
+  if (hc.too_long)
+    return set_error(CURTREE_ERANGE,
+                     "canonical sort: a query holds 2^32 or more pairs (not sorted)");
   return CURTREE_OK;
 }
 
 }  // namespace ct
+

@@ -142,3 +142,23 @@ def test_unaligned_arrays_and_chunk_edges():
         ct.lib.curtree_free(ctypes.c_void_p(tp))
     cf, ctt = orc.canonical(fid, tid)
     assert np.array_equal(t, ctt) and np.array_equal(f, cf)
+
+
+def test_long_segments_with_crowded_ids():
+    # long segments are bucketed by the top bits of their key range; ids
+    # crowded into a small part of the range leave buckets longer than a
+    # block sorts (they take the LSD passes from the work array), all-equal
+    # ids one bucket, and a few far ids many tiny buckets
+    import curtree_amd as ct
+
+    rng = np.random.default_rng(11)
+    lengths = [300_000, 70_000, 40_000, 17_000, 5]
+    fid, tid = _segments(lengths, rng, 0, 999)
+    s1 = lengths[0]
+    fid[:5] = [2**31 - 1, -2**31, 123_456_789, -7, 2**30]  # wide range, most ids crowded
+    fid[s1:s1 + lengths[1]] = 42                            # one id repeated
+    s3 = s1 + lengths[1] + lengths[2]
+    fid[s3:s3 + lengths[3]] = rng.permutation(lengths[3]).astype(np.int32) * 4099  # spread evenly
+    f, t = _sort(ct, fid, tid)
+    cf, ctt = orc.canonical(fid, tid)
+    assert np.array_equal(t, ctt) and np.array_equal(f, cf)
