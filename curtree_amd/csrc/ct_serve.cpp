@@ -249,13 +249,20 @@ int Server::begin_batch(Slot& sl) {
   const Tree& tr = sl.tree;
   const int nj = (int)sl.reqs.size();
   const int64_t nwin = sl.nwin;
-  CT_HIP(hipMemcpyAsync(sl.jobs, sl.hjobs, (size_t)nj * sizeof(ServeJob), hipMemcpyHostToDevice,
-                        sl.s));
-  CT_HIP(hipMemcpyAsync(sl.first, sl.hfirst, (size_t)nj * 8, hipMemcpyHostToDevice, sl.s));
-  hipLaunchKernelGGL(k_stage, dim3((unsigned)std::min(nj, 1024)), dim3(256), 0, sl.s,
-                     (const ServeJob*)sl.jobs, nj, sl.win, sl.win_cap);
   curtree_bbox_f32 q{(int)nwin, sl.win, sl.win + sl.win_cap, sl.win + 2 * sl.win_cap,
                      sl.win + 3 * sl.win_cap, nullptr};
+  if (nj == 1) {
+    // one request: the run reads its windows where the client put them
+    const ServeJob& jb = sl.hjobs[0];
+    q = curtree_bbox_f32{(int)nwin, (float*)jb.src[0], (float*)jb.src[1], (float*)jb.src[2],
+                         (float*)jb.src[3], nullptr};
+  } else {
+    CT_HIP(hipMemcpyAsync(sl.jobs, sl.hjobs, (size_t)nj * sizeof(ServeJob),
+                          hipMemcpyHostToDevice, sl.s));
+    CT_HIP(hipMemcpyAsync(sl.first, sl.hfirst, (size_t)nj * 8, hipMemcpyHostToDevice, sl.s));
+    hipLaunchKernelGGL(k_stage, dim3((unsigned)std::min(nj, 1024)), dim3(256), 0, sl.s,
+                       (const ServeJob*)sl.jobs, nj, sl.win, sl.win_cap);
+  }
   // the tree's cached containment flags, AoS copies (if any) and layout check
   int rc;
   if ((rc = prepare_query(sl.qe, prof, &tr.rt, &tr.rbox, &q, 0, nwin, &sl.a, sl.s))) return rc;
@@ -317,13 +324,16 @@ int Server::enqueue_pass(Slot& sl) {
   int rc;
   QRunArgs& a = sl.a;
   const bool pass1 = sl.state == kPass1;
+  // per-window counts split a batch's pairs among its requests; a batch of
+  // one request takes them all (the run's total)
+  const bool split = pass1 && sl.reqs.size() > 1;
   if (a.nbox == 0 || sl.nwin == 0) {
-    enqueue_fill_counts(sl.counts, sl.nwin, sl.s);
+    if (split) enqueue_fill_counts(sl.counts, sl.nwin, sl.s);
     sl.qe.host_ctrl->total = 0;
     sl.qe.host_ctrl->overflow = 0;
   } else {
     a.write = true;
-    a.counts = pass1 ? (unsigned long long*)sl.counts : nullptr;
+    a.counts = split ? (unsigned long long*)sl.counts : nullptr;
     a.out_fid = pool_fid + sl.pool_off;
     a.out_tid = pool_tid + sl.pool_off;
     a.out_cap = sl.pool_len;
@@ -331,7 +341,7 @@ int Server::enqueue_pass(Slot& sl) {
                                        per_tile(a.G, a.steps) + 2)) ||
         (rc = sl.qe.reserve_pieces(pieces_needed(sl.qe, a))))
       return rc;
-    if (pass1) enqueue_fill_counts(sl.counts, sl.nwin, sl.s);
+    if (split) enqueue_fill_counts(sl.counts, sl.nwin, sl.s);
     enqueue_run(sl.qe, prof, a, sl.s);
     if (sl.reqs.size() > 1 && sl.pool_len > 0)
       hipLaunchKernelGGL(k_rebase_search, dim3(1024), dim3(256), 0, sl.s,
@@ -340,7 +350,7 @@ int Server::enqueue_pass(Slot& sl) {
     CT_HIP(hipMemcpyAsync(sl.qe.host_ctrl, sl.qe.ctrl, sizeof(QCtrl), hipMemcpyDeviceToHost,
                           sl.s));
   }
-  if (pass1)
+  if (split)
     CT_HIP(hipMemcpyAsync(sl.hcounts, sl.counts, (size_t)sl.nwin * 8, hipMemcpyDeviceToHost,
                           sl.s));
   CT_HIP(hipEventRecord(sl.ev, sl.s));
@@ -364,13 +374,15 @@ int Server::after_pass(Slot& sl) {
   }
   const int64_t H = ran ? (int64_t)hc.total : 0;
   if (sl.state == kPass1) {
-    // per-request slices from the per-window counts
+    // per-request slices from the per-window counts (one request: all pairs)
     int64_t p0 = 0;
     for (size_t j = 0; j < sl.reqs.size(); j++) {
       Request& r = reqs[sl.reqs[j]];
       const int64_t w0 = sl.hfirst[j];
       int64_t np = 0;
-      for (int64_t i = 0; i < r.win.sz; i++) np += sl.hcounts[w0 + i];
+      if (sl.reqs.size() == 1) np = H;
+      else
+        for (int64_t i = 0; i < r.win.sz; i++) np += sl.hcounts[w0 + i];
       r.pool_off = p0;  // relative until the final reservation is known
       r.npairs = np;
       r.state = CURTREE_REQ_RUNNING;
