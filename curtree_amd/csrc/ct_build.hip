@@ -2398,36 +2398,53 @@ __global__ __launch_bounds__(256) void k_seg_count(int64_t n, const int* __restr
 }
 
 // Exclusive scan of the chunk head counts into 64-bit offsets, by one block
-// of 1024 threads (each a contiguous slice); the total is the head list's
-// length.
+// of 1024 threads in rounds of 8192 counts (thread t loads counts 8t .. 8t+7
+// of the round as two 16-byte words), carrying the running total; the total
+// is the head list's length.
 __global__ __launch_bounds__(1024) void k_seg_scan(int64_t nchunks,
                                                    const uint32_t* __restrict__ cnt,
                                                    uint64_t* __restrict__ off, SegCtrl* c) {
   __shared__ uint64_t s_w[16];
   const int t = threadIdx.x, lane = lane_id(), w = t >> 6;
-  const int64_t per = (nchunks + 1023) / 1024;
-  const int64_t b0 = (int64_t)t * per, b1 = b0 + per < nchunks ? b0 + per : nchunks;
-  uint64_t sum = 0;
-  for (int64_t i = b0; i < b1; i++) sum += cnt[i];
-  uint64_t x = sum;
+  uint64_t carry = 0;
+  for (int64_t r0 = 0; r0 < nchunks; r0 += 8192) {
+    const int64_t i0 = r0 + 8 * (int64_t)t;
+    uint32_t v[8];
+    if (i0 + 8 <= nchunks) {
+      const uint4 a = *(const uint4*)(cnt + i0);
+      const uint4 z = *(const uint4*)(cnt + i0 + 4);
+      v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+      v[4] = z.x; v[5] = z.y; v[6] = z.z; v[7] = z.w;
+    } else {
 #pragma unroll
-  for (int o = 1; o < kWave; o <<= 1) {
-    const uint64_t y = __shfl_up(x, o);
-    if (lane >= o) x += y;
-  }
-  if (lane == kWave - 1) s_w[w] = x;
-  __syncthreads();
-  uint64_t before = x - sum, total = 0;
+      for (int k = 0; k < 8; k++) v[k] = i0 + k < nchunks ? cnt[i0 + k] : 0u;
+    }
+    uint64_t sum = 0;
 #pragma unroll
-  for (int i = 0; i < 16; i++) {
-    if (i < w) before += s_w[i];
-    total += s_w[i];
+    for (int k = 0; k < 8; k++) sum += v[k];
+    uint64_t x = sum;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+      const uint64_t y = __shfl_up(x, o);
+      if (lane >= o) x += y;
+    }
+    if (lane == kWave - 1) s_w[w] = x;
+    __syncthreads();
+    uint64_t before = carry + x - sum, total = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      if (i < w) before += s_w[i];
+      total += s_w[i];
+    }
+    __syncthreads();  // s_w is rewritten by the next round
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      if (i0 + k < nchunks) off[i0 + k] = before;
+      before += v[k];
+    }
+    carry += total;
   }
-  for (int64_t i = b0; i < b1; i++) {
-    off[i] = before;
-    before += cnt[i];
-  }
-  if (t == 0) c->nheads = total;
+  if (t == 0) c->nheads = carry;
 }
 
 __global__ __launch_bounds__(256) void k_seg_heads(int64_t n, const int* __restrict__ tid,
