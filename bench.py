@@ -827,6 +827,12 @@ def main():
     ap.add_argument("--expand-split", type=int, default=-1, choices=[-1, 0, 1],
                     help="internal levels: -1 by run size (library default), 0 chained scan, "
                          "1 count/scan/write")
+    ap.add_argument("--window-order", type=int, default=None, choices=[-1, 0, 1],
+                    help="windows of a large batch taken in Morton order of their centres: -1 "
+                         "by batch size (library default), 0 input order, 1 always")
+    ap.add_argument("--tile-xcd", type=int, default=None, choices=[0, 1],
+                    help="query tiles walked in contiguous eighths, one per XCD: 0 off (library "
+                         "default), 1 on")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -848,6 +854,10 @@ def main():
         ct.set_option("leaf_fused", args.leaf_fused)
     if args.box_staging is not None:
         ct.set_option("box_staging", args.box_staging)
+    if args.window_order is not None:
+        ct.set_option("window_order", args.window_order)
+    if args.tile_xcd is not None:
+        ct.set_option("tile_xcd", args.tile_xcd)
     args.boxes_set = args.boxes is not None
     args.windows_set = args.windows is not None
     if args.mode == "plain" and args.fanout is None:
@@ -951,7 +961,8 @@ def main():
     for name in ("k_build_init", "k_extent", "k_hist", "k_sort_plan", "k_sweep_kv", "k_final",
                  "k_run_len", "k_fixup_short", "k_fixup_long", "k_fixup_huge", "k_fixup_groups",
                  "k_fallback", "k_level",
-                 "k_safe", "k_stage4", "k_run_begin", "k_expand", "k_expand_count", "k_expand_write",
+                 "k_safe", "k_stage4", "k_win_keys", "k_pair_pack", "k_onesweep_pairs",
+                 "k_pair_unpack", "k_win_stage", "k_win_offsets", "k_run_begin", "k_expand", "k_expand_count", "k_expand_write",
                  "k_leaf_count", "k_tile_scan", "k_leaf_fused",
                  "k_leaf_write", "k_write_pieces"):
         tot, cnt = ct.Prof.get(name)

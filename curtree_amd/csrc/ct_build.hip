@@ -1214,7 +1214,7 @@ __device__ __forceinline__ void wave_sort_run(int64_t i, int L, const KeyParams&
                                               float* __restrict__ xmax, float* __restrict__ ymax,
                                               int* __restrict__ id, float4* __restrict__ stage,
                                               uint32_t* __restrict__ stage_id, uint64_t* s_w,
-                                              uint32_t* s_c, uint64_t* s_m) {
+                                              uint32_t* s_c) {
   const int lane = lane_id();
   const uint64_t lt = lanemask_lt();
   const int kUse = (L + 63) >> 6;
@@ -1244,7 +1244,14 @@ __device__ __forceinline__ void wave_sort_run(int64_t i, int L, const KeyParams&
     for (int k = 0; k < kRunPer; k++) {
       if (k < kUse) {
         const uint32_t d = (uint32_t)(x[k] >> sh) & 255u;
-        const uint64_t m = peer_mask(d, true, s_m);
+        // (ballots, not peer_mask: a run's keys share digits, and lanes that
+        // OR into one LDS entry are served one after another)
+        uint64_t m = ~0ull;
+#pragma unroll
+        for (int bit = 0; bit < 8; bit++) {
+          const uint64_t bb = ballot((d >> bit) & 1u);
+          m &= ((d >> bit) & 1u) ? bb : ~bb;
+        }
         const uint32_t before = s_c[d];
         rank[k] = before + (uint32_t)popc64(m & lt);
         if ((m & lt) == 0) s_c[d] = before + (uint32_t)popc64(m);
@@ -1313,22 +1320,21 @@ __global__ __launch_bounds__(256) void k_fixup_long(int64_t n, BuildCtrl* c,
   __shared__ uint64_t s_net[kFixLong];
   __shared__ __align__(16) uint32_t s_cnt[4][256];  // per wave, per digit: count -> offset
   __shared__ uint32_t s_wsum[4];
-  __shared__ uint64_t s_match[4][256];  // peer_mask tables
   const uint32_t nl = c->nlong, nm = c->nmid;
   if ((blockIdx.x >= nl && blockIdx.x * 4u >= nm) || c->fallback) return;
   const KeyParams kp = key_params<kKeyMorton64>(c);
-  for (int i = threadIdx.x; i < 4 * 256; i += 256) (&s_match[0][0])[i] = 0;
-  __syncthreads();
+  // (ballot ranking: the keys of one run share digits, which would serialise
+  // peer_mask's LDS ORs)
   for (uint32_t r = blockIdx.x; r < nl; r += gridDim.x) {
     const uint32_t q = runs[3 * (size_t)runs_cap + r];
     lds_sort_run<4>(runs[q], (int)runs[runs_cap + q], kp, xmin, ymin, xmax, ymax, id, stage,
-                    stage_id, s_net, s_cnt, s_wsum, s_match);
+                    stage_id, s_net, s_cnt, s_wsum);
   }
   const int w = threadIdx.x >> 6;
   for (uint32_t r = blockIdx.x * 4u + w; r < nm; r += gridDim.x * 4u) {
     const uint32_t q = runs[2 * (size_t)runs_cap + r];
     wave_sort_run(runs[q], (int)runs[runs_cap + q], kp, xmin, ymin, xmax, ymax, id, stage,
-                  stage_id, s_net + kWaveSortMax * w, s_cnt[w], s_match[w]);
+                  stage_id, s_net + kWaveSortMax * w, s_cnt[w]);
   }
 }
 
@@ -2972,7 +2978,9 @@ __global__ __launch_bounds__(256) void k_pair_unpack(int64_t n, const BuildCtrl*
 }
 
 // Any order of pairs: a full radix sort of (query id, box id) keys.
-static int pairs_full_sort(Context& ctx, int* fid, int* tid, int64_t n, hipStream_t s) {
+// tid_only: the pairs are already in fid order, so the stable passes over the
+// tid digits alone sort them.
+int pairs_full_sort(Context& ctx, int* fid, int* tid, int64_t n, hipStream_t s, bool tid_only) {
   if (n >= (int64_t)1 << 30)
     return set_error(CURTREE_ERANGE,
                      "canonical sort: pairs not in query order take at most 2^30-1 per call");
@@ -2996,9 +3004,10 @@ static int pairs_full_sort(Context& ctx, int* fid, int* tid, int64_t n, hipStrea
   hipLaunchKernelGGL(k_pair_pack, dim3(sgrid), dim3(256), 0, s, n, fid, tid, c, k0, status,
                      (int64_t)status_words);
   pf.mark(s, "k_sort_plan");
-  hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(256), 0, s, n, 8, c, 0, 8, 0x76543210ull, 1, 0,
-                     1, 0);
-  for (int p = 0; p < 8; p++) {
+  const int steps = tid_only ? 4 : 8;
+  hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(256), 0, s, n, 8, c, 0, steps,
+                     tid_only ? 0x7654ull : 0x76543210ull, 1, 0, 1, 0);
+  for (int p = 0; p < steps; p++) {
     pf.mark(s, "k_onesweep_pairs");
     hipLaunchKernelGGL(k_onesweep<uint64_t>, dim3((unsigned)std::min(ntiles, kOnesweepGrid)),
                        dim3(kSortThreads), 0, s, n, p, k0, k1, (uint32_t*)nullptr,

@@ -621,6 +621,11 @@ CURTREE_API int curtree_set_option(const char* name, int64_t value) {
     qtuning().fused = (int)value;
     return CURTREE_OK;
   }
+  if (!strcmp(name, "window_order")) {
+    if (value < -1 || value > 1) return set_error(CURTREE_EINVAL, "window_order: -1, 0 or 1");
+    qtuning().order = (int)value;
+    return CURTREE_OK;
+  }
   if (!strcmp(name, "tile_xcd")) {
     if (value < 0 || value > 1) return set_error(CURTREE_EINVAL, "tile_xcd: 0 or 1");
     qtuning().xcd = (int)value;

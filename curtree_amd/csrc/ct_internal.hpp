@@ -188,6 +188,10 @@ struct QEngine {
   float4* win4 = nullptr;
   int64_t cap_win4 = 0;
   uint32_t* canon = nullptr;            // the tree has the bulk-load layout (device word)
+  // windows in a run order (large writing runs): the order, per-window pair
+  // counts, each window's first pair in input and in run order, block sums
+  void* order = nullptr;
+  int64_t cap_order = 0;
   // fixed: every buffer above is a slice of memory the engine does not own
   // (the allocation-free entry point's scratch); a request beyond a buffer's
   // capacity fails with CURTREE_ERANGE instead of allocating
@@ -200,6 +204,7 @@ struct QEngine {
   int reserve_status(int64_t words);
   int reserve_safe(int64_t nodes);
   int reserve_plan(int64_t chunks);
+  int reserve_order(int64_t windows);
   int ensure_ctrl();
   void release();
   ~QEngine() { release(); }
@@ -253,11 +258,17 @@ int query_frontier_f32(Context& ctx, const curtree_rtree_f32* rt, const curtree_
 
 // Same, on an explicit engine and stream, for queries [q0, q1) of qbox (pair
 // tids stay global query indices).  contain: -1 auto, 0 off, 1 on.
+// ctx (or null): the library context, whose sort buffers order the windows
+// of large runs in Morton order of their centres (see ct_query.hip).
 int query_run_f32(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox,
                   const curtree_bbox_f32* qbox, int64_t q0, int64_t q1, int contain, QueryOut* qo,
-                  hipStream_t s);
+                  hipStream_t s, Context* ctx = nullptr);
 
 int pairs_canonical_sort(Context& ctx, int* fid, int* tid, int64_t n, hipStream_t s);
+// (query id, box id) order of any n < 2^30 pairs by a full radix sort of the
+// 64-bit (tid, fid) keys (signed order of both); no host synchronisation
+int pairs_full_sort(Context& ctx, int* fid, int* tid, int64_t n, hipStream_t s,
+                    bool tid_only = false);
 
 // query_into on the fixed scratch engine (no allocation; see ct_query.hip).
 int query_into_noalloc_f32(const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox,

@@ -843,7 +843,9 @@ template <int G, int STEPS>
 __global__ __launch_bounds__(kQThreads) void k_leaf_write(
     int slot, QCtrl* c, const QLeaf* __restrict__ est, const uint64_t* __restrict__ tbase_arr,
     const int* __restrict__ rid, int* __restrict__ out_fid, int* __restrict__ out_tid,
-    int64_t out_cap, QPiece* __restrict__ pieces, int64_t piece_cap, bool xcd) {
+    int64_t out_cap, QPiece* __restrict__ pieces, int64_t piece_cap, bool xcd,
+    const uint32_t* __restrict__ tmap, const int64_t* __restrict__ ooff,
+    const int64_t* __restrict__ poff) {
   using S = QShape<G, STEPS>;
   // a write tile is M count tiles (at least one entry per thread): its base is
   // the first count tile's scanned total
@@ -867,12 +869,13 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_write(
     __syncthreads();  // the previous tile is done with the shared arrays
     // the tile's entries, and scans of all pairs and of the flat ones
     // (partial entries and ranges up to kFlatMax, written by the block)
-    uint64_t loc[R];
+    uint64_t loc[R], shift[R];
     uint64_t sum = 0, fsum = 0;
 #pragma unroll
     for (int r = 0; r < R; r++) {
       const int li = t * R + r;
       loc[r] = 0;
+      shift[r] = 0;
       if (li < P) {
         const int64_t e = tile * P + li;
         uint32_t qf = kQNone;
@@ -882,6 +885,14 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_write(
           s_val[li] = x.val;
           s_first[li] = x.first;
           loc[r] = (qf & kQRange) ? x.val : (uint64_t)popc64(x.val);
+          // windows in a run order: the entry's query is a rank; its pairs
+          // move from the window's first pair in run order to its first pair
+          // in input order (modulo 2^64), and carry the input index
+          if (tmap) {
+            const uint32_t rk = qf & ~kQRange, qi = tmap[rk];
+            shift[r] = (uint64_t)ooff[rk] - (uint64_t)poff[rk];
+            qf = qi | (qf & kQRange);
+          }
         }
         s_q[li] = qf;
         sum += loc[r];
@@ -897,7 +908,7 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_write(
       if (li < P) {
         const uint32_t qf = s_q[li];
         const bool piece = (qf & kQRange) && qf != kQNone && loc[r] > kFlatMax;
-        s_off[li] = ex;
+        s_off[li] = ex + shift[r];
         s_foff[li] = (uint32_t)fex;
         if (!piece) fex += loc[r];
       }
@@ -1355,6 +1366,19 @@ int QEngine::reserve_plan(int64_t chunks) {
   return rc;
 }
 
+// Window order buffers for `windows` windows: the order and the ranks (4 B
+// each), the windows in run order (16 B), counts (8 B), first pair in input
+// and in run order (8 B each), and two block sums per kOrdBlock windows.
+constexpr int64_t kOrdBlock = 4096;
+int QEngine::reserve_order(int64_t windows) {
+  if (windows <= cap_order) return CURTREE_OK;
+  if (fixed) return fixed_full("the window order", windows, cap_order);
+  const int64_t nb = (windows + kOrdBlock - 1) / kOrdBlock;
+  int rc = dev_alloc(&order, (size_t)windows * 48 + (size_t)nb * 16 + 1024);
+  cap_order = rc ? 0 : windows;
+  return rc;
+}
+
 int QEngine::ensure_ctrl() {
   if (ctrl) return CURTREE_OK;
   if (fixed) return set_error(CURTREE_EINVAL, "query: fixed engine without a control block");
@@ -1370,8 +1394,8 @@ int QEngine::ensure_ctrl() {
 
 void QEngine::release() {
   if (fixed) return;  // slices of memory the engine does not own
-  void* ps[] = {front[0], front[1], pieces, status, ctrl,  safe, plan,
-                box4,     node4,    win4,   canon,  est,   ttot};
+  void* ps[] = {front[0], front[1], pieces, status, ctrl, safe, plan,
+                box4,     node4,    win4,   canon,  est,  ttot, order};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   box4 = node4 = win4 = nullptr;
@@ -1386,7 +1410,8 @@ void QEngine::release() {
   host_ctrl = nullptr;
   safe = nullptr;
   plan = nullptr;
-  cap_front = cap_pieces = status_words = cap_safe = cap_plan = 0;
+  order = nullptr;
+  cap_front = cap_pieces = status_words = cap_safe = cap_plan = cap_order = 0;
   needs_clear = true;
 }
 
@@ -1583,6 +1608,14 @@ static void warm_grids(int G, int steps) {
   }
 }
 
+// (window order kernels, below)
+__global__ void k_ord_sums(int64_t nq, const unsigned long long* __restrict__ cnt,
+                           const uint32_t* __restrict__ inv, int64_t* __restrict__ sums);
+__global__ void k_ord_top(int64_t nb, int64_t* __restrict__ sums);
+__global__ void k_ord_apply(int64_t nq, const unsigned long long* __restrict__ cnt,
+                            const uint32_t* __restrict__ inv, const int64_t* __restrict__ sums,
+                            int64_t* __restrict__ ooff, int64_t* __restrict__ poff);
+
 template <int G, int STEPS>
 void enqueue_run_g(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
   using S = QShape<G, STEPS>;
@@ -1664,12 +1697,24 @@ void enqueue_run_g(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
     pf.mark(s, "k_tile_scan");
     hipLaunchKernelGGL(k_tile_scan, dim3(64), dim3(kQThreads), 0, s, leaf + 1, leaf, S::kPerTile,
                        qe.ctrl, qe.ttot, qe.status);
+    if (a.tmap) {  // windows in a run order: their first pairs in both orders
+      const int64_t nq = a.q1 - a.q0, nb = (nq + kOrdBlock - 1) / kOrdBlock;
+      const uint32_t* inv = a.tmap + nq;  // the rank of each input window
+      pf.mark(s, "k_win_offsets");
+      hipLaunchKernelGGL(k_ord_sums, dim3((unsigned)nb), dim3(256), 0, s, nq,
+                         (const unsigned long long*)a.counts, inv, a.osum);
+      hipLaunchKernelGGL(k_ord_top, dim3(1), dim3(256), 0, s, nb, a.osum);
+      hipLaunchKernelGGL(k_ord_apply, dim3((unsigned)nb), dim3(256), 0, s, nq,
+                         (const unsigned long long*)a.counts, inv, (const int64_t*)a.osum,
+                         a.ooff, a.poff);
+    }
     pf.mark(s, "k_leaf_write");
     hipLaunchKernelGGL((k_leaf_write<G, STEPS>),
                        dim3(std::min<unsigned>(a.grid, leaf_write_grid<G, STEPS>())),
                        dim3(kQThreads), 0, s, leaf,
                        qe.ctrl, (const QLeaf*)qe.est, (const uint64_t*)qe.ttot, a.rid, a.out_fid,
-                       a.out_tid, a.out_cap, (QPiece*)qe.pieces, qe.cap_pieces, xcd);
+                       a.out_tid, a.out_cap, (QPiece*)qe.pieces, qe.cap_pieces, xcd, a.tmap,
+                       a.ooff, a.poff);
     pf.mark(s, "k_write_pieces");
     hipLaunchKernelGGL(k_write_pieces, dim3(4096), dim3(256), 0, s, (const QCtrl*)qe.ctrl,
                        (const QPiece*)qe.pieces, qe.cap_pieces, a.rid, a.out_fid, a.out_tid,
@@ -2007,9 +2052,149 @@ int prepare_query(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curt
   return CURTREE_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Windows in Morton order of their centres.  In a large batch each leaf-parent
+// is tested by many windows (about 14 at 100M boxes and 1M windows); with the
+// windows in input order those tests are far apart in time and every one reads
+// the leaf-parent's boxes from HBM again.  A run may take its windows in any
+// order as long as every window's pairs land where the input order puts them:
+// the windows are sorted by the Morton code of their centres (quantised over
+// the root MBR) and copied in that order, the run works on ranks (its frontier,
+// window loads and per-window counts), and the leaf write moves every entry
+// from its window's first pair in run order to the window's first pair in
+// input order and writes the input index.  Within a window the pairs keep
+// their (sorted box position) order, so the output is the same.
+constexpr int64_t kOrderMinWindows = 65536;
+
+// The sort keys (code ^ 0x80000000, so that the signed order of the pair sort
+// is the unsigned order of the code) and the window indices.
+__global__ __launch_bounds__(256) void k_win_keys(int64_t nq, QBoxes qb, QBoxes root,
+                                                  int* __restrict__ key, int* __restrict__ idx) {
+  const float rx0 = root.x0[0], ry0 = root.y0[0], rx1 = root.x1[0], ry1 = root.y1[0];
+  const float sx = rx1 > rx0 ? 65535.0f / (rx1 - rx0) : 0.0f;
+  const float sy = ry1 > ry0 ? 65535.0f / (ry1 - ry0) : 0.0f;
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < nq; i += (int64_t)gridDim.x * 256) {
+    const float cx = (qb.x0[i] + qb.x1[i]) * 0.5f, cy = (qb.y0[i] + qb.y1[i]) * 0.5f;
+    float fx = (cx - rx0) * sx, fy = (cy - ry0) * sy;
+    fx = fx > 0.0f ? (fx < 65535.0f ? fx : 65535.0f) : 0.0f;  // (NaN: 0)
+    fy = fy > 0.0f ? (fy < 65535.0f ? fy : 65535.0f) : 0.0f;
+    const uint32_t code = (spread16((uint32_t)fy) << 1) | spread16((uint32_t)fx);
+    key[i] = (int)(code ^ 0x80000000u);
+    idx[i] = (int)i;
+  }
+}
+
+// The windows copied in run order (win4[r] = window perm[r]) and the rank of
+// each input window (inv[perm[r]] = r).
+__global__ __launch_bounds__(256) void k_win_stage(int64_t nq, QBoxes qb,
+                                                   const uint32_t* __restrict__ perm,
+                                                   float4* __restrict__ win4,
+                                                   uint32_t* __restrict__ inv) {
+  for (int64_t r = blockIdx.x * 256ll + threadIdx.x; r < nq; r += (int64_t)gridDim.x * 256) {
+    const uint32_t i = perm[r];
+    win4[r] = make_float4(qb.x0[i], qb.y0[i], qb.x1[i], qb.y1[i]);
+    inv[i] = (uint32_t)r;
+  }
+}
+
+// Per block of kOrdBlock windows: their pair counts (by rank) summed in input
+// order (sums[b], windows b*kOrdBlock.. of the input) and in run order
+// (sums[nb + b], ranks b*kOrdBlock..).
+__global__ __launch_bounds__(256) void k_ord_sums(int64_t nq, const unsigned long long* __restrict__ cnt,
+                                                  const uint32_t* __restrict__ inv,
+                                                  int64_t* __restrict__ sums) {
+  __shared__ uint64_t s_a[kQWaves], s_b[kQWaves];
+  const int64_t nb = (nq + kOrdBlock - 1) / kOrdBlock;
+  const int64_t b = blockIdx.x;
+  uint64_t so = 0, sp = 0;
+  for (int64_t i = b * kOrdBlock + threadIdx.x; i < nq && i < (b + 1) * kOrdBlock; i += 256) {
+    so += cnt[inv[i]];
+    sp += cnt[i];
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    so += __shfl_xor(so, o);
+    sp += __shfl_xor(sp, o);
+  }
+  if (lane_id() == 0) {
+    s_a[threadIdx.x >> 6] = so;
+    s_b[threadIdx.x >> 6] = sp;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t x = 0, z = 0;
+#pragma unroll
+    for (int w = 0; w < kQWaves; w++) {
+      x += s_a[w];
+      z += s_b[w];
+    }
+    sums[b] = (int64_t)x;
+    sums[nb + b] = (int64_t)z;
+  }
+}
+
+// Exclusive scans of both rows of block sums in place (one block).
+__global__ __launch_bounds__(256) void k_ord_top(int64_t nb, int64_t* __restrict__ sums) {
+  __shared__ uint64_t s_scan[kQWaves];
+  for (int row = 0; row < 2; row++) {
+    int64_t* v = sums + row * nb;
+    uint64_t carry = 0;
+    for (int64_t r0 = 0; r0 < nb; r0 += 256) {
+      const int64_t i = r0 + threadIdx.x;
+      const uint64_t x = i < nb ? (uint64_t)v[i] : 0ull;
+      uint64_t tot;
+      const uint64_t ex = block_excl_scan64(x, s_scan, &tot);
+      if (i < nb) v[i] = (int64_t)(carry + ex);
+      carry += tot;
+    }
+  }
+}
+
+// Each window's first pair in input order and in run order, both stored by
+// rank (ooff[rank], poff[rank]: the leaf write reads them beside the rank's
+// input index, all in run order): thread t of block b takes 16 consecutive
+// windows of the block's kOrdBlock, in input order for ooff and in run order
+// for poff.
+__global__ __launch_bounds__(256) void k_ord_apply(int64_t nq, const unsigned long long* __restrict__ cnt,
+                                                   const uint32_t* __restrict__ inv,
+                                                   const int64_t* __restrict__ sums,
+                                                   int64_t* __restrict__ ooff,
+                                                   int64_t* __restrict__ poff) {
+  __shared__ uint64_t s_scan[kQWaves];
+  constexpr int kPer = kOrdBlock / 256;
+  const int64_t nb = (nq + kOrdBlock - 1) / kOrdBlock;
+  const int64_t b = blockIdx.x;
+  const int64_t i0 = b * kOrdBlock + (int64_t)threadIdx.x * kPer;
+  uint64_t co[kPer], cp[kPer];
+  uint32_t rk[kPer];
+  uint64_t so = 0, sp = 0;
+#pragma unroll
+  for (int k = 0; k < kPer; k++) {
+    const int64_t i = i0 + k;
+    rk[k] = i < nq ? inv[i] : 0u;
+    co[k] = i < nq ? cnt[rk[k]] : 0ull;
+    cp[k] = i < nq ? cnt[i] : 0ull;
+    so += co[k];
+    sp += cp[k];
+  }
+  uint64_t tot;
+  uint64_t eo = (uint64_t)sums[b] + block_excl_scan64(so, s_scan, &tot);
+  uint64_t ep = (uint64_t)sums[nb + b] + block_excl_scan64(sp, s_scan, &tot);
+#pragma unroll
+  for (int k = 0; k < kPer; k++) {
+    const int64_t i = i0 + k;
+    if (i < nq) {
+      ooff[rk[k]] = (int64_t)eo;
+      poff[i] = (int64_t)ep;
+    }
+    eo += co[k];
+    ep += cp[k];
+  }
+}
+
 int query_run_f32(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox,
                   const curtree_bbox_f32* qbox, int64_t q0, int64_t q1, int contain, QueryOut* qo,
-                  hipStream_t s) {
+                  hipStream_t s, Context* ctx) {
   if (!qo || !qbox) return set_error(CURTREE_EINVAL, "query: null argument");
   qo->n = 0;
   if (q0 < 0 || q1 > qbox->sz || q0 > q1) return set_error(CURTREE_EINVAL, "query: bad range");
@@ -2041,9 +2226,51 @@ int query_run_f32(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curt
       if (tid) (void)hipFree(tid);
     }
   } owned;
+  // windows in Morton order of their centres (large batches of the plain entry
+  // points; sorted and copied once, before the passes).  Per-window counts
+  // for the caller are by input index: such runs keep the input order.
+  const int ow = qtuning().order;
+  const bool order = ctx && q0 == 0 && !a.generic && !a.chained && !qe.fixed && !qo->counts &&
+                     qtuning().fused != 1 && nq >= kSplitMinQueries &&
+                     nq < ((int64_t)1 << 30) &&
+                     (ow == 1 || (ow < 0 && nq >= kOrderMinWindows));
+  uint32_t* perm = nullptr;
+  unsigned long long* ocnt = nullptr;
+  int64_t *ooff = nullptr, *poff = nullptr, *osum = nullptr;
+  if (order) {
+    if ((rc = qe.reserve_order(nq))) return rc;
+    char* base = (char*)qe.order;
+    const size_t at = ((size_t)nq * 8 + 255) & ~(size_t)255;
+    perm = (uint32_t*)base;  // then the rank of each input window (nq more)
+    float4* win4 = (float4*)(base + at);
+    ocnt = (unsigned long long*)(win4 + nq);
+    ooff = (int64_t*)(ocnt + nq);
+    poff = ooff + nq;
+    osum = poff + nq;
+    // sort keys in the counts' space (cleared before every writing run)
+    int* key = (int*)ocnt;
+    pf.mark(s, "k_win_keys");
+    const unsigned g = (unsigned)std::min<int64_t>((nq + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_win_keys, dim3(g), dim3(256), 0, s, nq, a.qb, a.tr.mbr, key, (int*)perm);
+    CT_HIP(hipGetLastError());
+    if ((rc = pairs_full_sort(*ctx, (int*)perm, key, nq, s, true))) return rc;
+    pf.mark(s, "k_win_stage");
+    hipLaunchKernelGGL(k_win_stage, dim3(g), dim3(256), 0, s, nq, a.qb, (const uint32_t*)perm,
+                       win4, perm + nq);
+    CT_HIP(hipGetLastError());
+    pf.mark(s, nullptr);
+    a.qb.b4 = win4;  // the run's windows are ranks
+    a.tmap = perm;
+  }
   for (int pass = 0; pass < 2; pass++) {
+    if (order && a.write) {  // counts per rank, and the offsets from them
+      a.counts = ocnt;
+      a.ooff = ooff;
+      a.poff = poff;
+      a.osum = osum;
+    }
     for (int attempt = 0;; attempt++) {
-      if (a.counts) enqueue_fill_counts(qo->counts + q0, nq, s);
+      if (a.counts) enqueue_fill_counts((int64_t*)a.counts + (a.counts == ocnt ? 0 : q0), nq, s);
       if ((rc = qe.reserve_status(std::max<int64_t>(qe.cap_front, nq) / per_tile(a.G, a.steps) +
                                   2)))
         return rc;
@@ -2116,7 +2343,8 @@ int query_run_f32(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curt
 int query_frontier_f32(Context& ctx, const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox,
                        const curtree_bbox_f32* qbox, QueryOut* qo, hipStream_t s) {
   if (!qbox) return set_error(CURTREE_EINVAL, "query: null argument");
-  return query_run_f32(ctx.qe, ctx.prof, rt, rbox, qbox, 0, qbox->sz, ctx.opt_contain, qo, s);
+  return query_run_f32(ctx.qe, ctx.prof, rt, rbox, qbox, 0, qbox->sz, ctx.opt_contain, qo, s,
+                       &ctx);
 }
 
 // ---------------------------------------------------------------------------

@@ -27,6 +27,7 @@ struct QTuning {
   int fused = 0;     // writing leaf stage: 1 one chained pass, 0 count / tile scan / write,
                      // -1 the chained pass for runs below kSplitMinQueries windows
   int xcd = 0;       // chain-free kernels: 1 each XCD walks a contiguous eighth of the tiles
+  int order = -1;    // windows in Morton order of their centres: -1 from kOrderMinWindows, 0, 1
 };
 QTuning& qtuning();
 
@@ -145,6 +146,14 @@ struct QRunArgs {
   int* out_tid = nullptr;
   int64_t out_cap = 0;
   unsigned long long* counts = nullptr;  // per-query totals (count pass)
+  // windows in a run order (tmap: the input window at each rank; the run's
+  // windows and query indices are ranks): a window's first pair in input
+  // order (ooff) and in run order (poff), both by rank; the leaf write moves
+  // each entry by their difference and writes input indices
+  const uint32_t* tmap = nullptr;
+  int64_t* ooff = nullptr;
+  int64_t* poff = nullptr;
+  int64_t* osum = nullptr;  // two block sums per kOrdBlock windows
 };
 
 int next_pow2_group(int F);

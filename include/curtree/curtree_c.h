@@ -315,6 +315,15 @@ CURTREE_API int curtree_query_last_stats(int64_t* entries, int cap, int* levels,
  *                  stores their pairs in one chained pass (as
  *                  CURTREE_ALGO_PERSISTENT does), instead of a count pass, a
  *                  tile scan and a write pass.  Results are identical.
+ *   "window_order" -1 (default: for batches of at least 65536 windows), 0
+ *                  off, 1 on: the plain entry points process a batch's windows
+ *                  in Morton order of their centres (windows that meet the
+ *                  same leaf-parents are tested close together, so repeated
+ *                  box reads hit in cache) and write every window's pairs
+ *                  where the input order puts them.  Results are identical.
+ *   "tile_xcd"     0 (default) or 1: the count and write kernels' blocks on
+ *                  one XCD walk a contiguous eighth of the tiles.  Results
+ *                  are identical.
  * Returns CURTREE_EINVAL for an unknown name or value. */
 CURTREE_API int curtree_set_option(const char* name, int64_t value);
 

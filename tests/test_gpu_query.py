@@ -511,3 +511,48 @@ def test_kept_tree_state_skips_flag_pass_and_forgets_changed_boxes():
     rt.free()
     d.free()
     q.free()
+
+
+@pytest.mark.parametrize("contain", [0, 1])
+def test_window_order_same_pairs(contain):
+    # a batch whose windows are processed in Morton order of their centres
+    # must give the same raw pair list (input order of the windows, then
+    # sorted box position) as input order and as the oracle's traversal:
+    # through the allocating query (count, then write), a caller buffer (one
+    # writing run) and the per-window counts; NaN and point windows included
+    import curtree_amd as ct
+
+    n, nq = 200_000, 70_000
+    bx = datagen.mixture_boxes(n, seed=161)
+    ids = np.arange(n, dtype=np.int32)
+    qs = [a.copy() for a in datagen.loguniform_windows(nq, seed=162, lo=1e-4, hi=1e-2)[:4]]
+    qs[0][:7] = np.nan
+    qs[2][7:20] = qs[0][7:20]
+    qs[3][7:20] = qs[1][7:20]
+    d, rt, q = setup(bx, ids, qs, 16, 1)
+    ct.set_option("containment", contain)
+    try:
+        ct.set_option("window_order", 0)
+        f0, t0 = ct.bfs_frontier(rt, d, q)
+        ct.set_option("window_order", 1)
+        f1, t1 = ct.bfs_frontier(rt, d, q)
+        H = f1.size
+        p = ctypes.c_void_p()
+        ct._check(ct.lib.curtree_malloc(ctypes.byref(p), 8 * (H + 1)))
+        got = ct.query_into(rt, d, q, p.value, p.value + 4 * (H + 1), H + 1)
+        f2 = ct._host_from_dev(p.value, got, np.int32)
+        t2 = ct._host_from_dev(p.value + 4 * (H + 1), got, np.int32)
+        ct.lib.curtree_free(p)
+        n3, c3 = ct.query_count(rt, d, q, per_query=True)
+    finally:
+        ct.set_option("window_order", -1)
+        ct.set_option("containment", -1)
+    assert np.array_equal(f0, f1) and np.array_equal(t0, t1)
+    assert got == H and np.array_equal(f2, f0) and np.array_equal(t2, t0)
+    assert n3 == H and np.array_equal(c3, np.bincount(t0, minlength=nq))
+    sb = d.to_host()
+    of, ot, _ = orc.bfs(rt.to_host(), sb, qs)
+    assert np.array_equal(f0, of) and np.array_equal(t0, ot)
+    rt.free()
+    d.free()
+    q.free()
