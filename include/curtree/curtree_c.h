@@ -315,7 +315,7 @@ CURTREE_API int curtree_query_last_stats(int64_t* entries, int cap, int* levels,
  *                  stores their pairs in one chained pass (as
  *                  CURTREE_ALGO_PERSISTENT does), instead of a count pass, a
  *                  tile scan and a write pass.  Results are identical.
- *   "window_order" -1 (default: for batches of at least 65536 windows), 0
+ *   "window_order" -1 (default: for batches of at least 4194304 windows), 0
  *                  off, 1 on: the plain entry points process a batch's windows
  *                  in Morton order of their centres (windows that meet the
  *                  same leaf-parents are tested close together, so repeated
