@@ -319,7 +319,7 @@ def _poisson_run(ct, sv, tree_ids, pool, npool, lam, duration, seed, drain_s=20.
             break
         if now - t0 >= duration + drain_s:
             break  # overloaded: the backlog does not drain
-        time.sleep(50e-6)
+        time.sleep(0)  # poll without a timer sleep (as the plain entry point's loop spins)
     done = len(lat)
     sst = sv.stats()
     pairs = sst["pairs_done"]
@@ -520,68 +520,78 @@ def serve_trees(args, ct, datagen, scale):
     return sizes, trees
 
 
-def serve_leg(args, ct, datagen, scale, seconds, rate_step, drain_s):
+def serve_leg(args, ct, datagen, scale, seconds, rate_step, drain_s, full=False):
     """Served vs plain per-request launches on the 8 resident trees: Poisson
     arrivals at a swept rate, requests of 1..65,536 windows (log-uniform
     count), side log-uniform [1e-5, 2e-3], result budget serve_budget_gb.
-    Each mode's rate rises by rate_step until it no longer keeps up; the
-    served pairs/s are compared at each baseline's knee p99."""
+    The baselines' rates rise by rate_step until they no longer keep up; the
+    served pairs/s are compared at each baseline's knee p99.  full: the served
+    rate also rises by rate_step from serve_rate0 to serve_rate_max (the whole
+    curve); otherwise it starts at the lower baseline knee and rises by
+    serve_fine_step until its p99 passes 1.5x the baselines' knee p99, which
+    is the part of the curve the equal-p99 comparison reads."""
     sizes, trees = serve_trees(args, ct, datagen, scale)
     npool = 4_000_000
     pool = ct.DeviceBoxes.from_host(*datagen.loguniform_windows(npool, seed=80, lo=1e-5, hi=2e-3))
     budget = args.serve_budget_gb << 30
-    modes = {"plain": dict(slots=1, batching=False),
-             "served": dict(slots=args.serve_slots, batching=True)}
     res = {}
-    for name, kw in modes.items():
-        sv = ct.Server(budget_bytes=budget, max_batch_windows=args.serve_batch_windows, **kw)
-        ids = [sv.add_tree(rt, d) for rt, d, _ in trees]
+
+    def sweep(name, run, lam, step, stop):
         pts = []
-        _poisson_run(ct, sv, ids, pool, npool, 200.0, 0.5, seed=1, drain_s=drain_s)  # warm-up
-        # the baseline's sweep starts lower, so that its knee lies inside it
-        lam = args.serve_rate0 / (4 if name == "plain" else 1)
         while lam <= args.serve_rate_max:
-            p = _poisson_run(ct, sv, ids, pool, npool, lam, seconds, seed=int(lam),
-                             drain_s=drain_s)
+            p = run(lam)
             pts.append(p)
             print(f"[bench] {name} {lam:.0f} req/s: {p['pairs_per_s'] / 1e9:.2f} Gpairs/s, "
-                  f"p50 {p['p50_ms']:.2f} ms, p99 {p['p99_ms']:.2f} ms, kept up {p['kept_up']}, "
-                  f"{p['batches']} batches, dispatcher launching {p['launch_busy']:.0%}",
-                  file=sys.stderr, flush=True)
-            if not p["kept_up"]:
+                  f"p50 {p['p50_ms']:.2f} ms, p99 {p['p99_ms']:.2f} ms, kept up {p['kept_up']}"
+                  + (f", {p['batches']} batches, dispatcher launching {p['launch_busy']:.0%}"
+                     if "batches" in p else ""), file=sys.stderr, flush=True)
+            if not p["kept_up"] or stop(p):
                 break
-            lam *= rate_step
+            lam *= step
+        return pts
+
+    def server_sweep(name, lam0, step, stop, **kw):
+        sv = ct.Server(budget_bytes=budget, max_batch_windows=args.serve_batch_windows, **kw)
+        ids = [sv.add_tree(rt, d) for rt, d, _ in trees]
+        _poisson_run(ct, sv, ids, pool, npool, 200.0, 0.5, seed=1, drain_s=drain_s)  # warm-up
+        pts = sweep(name, lambda lam: _poisson_run(ct, sv, ids, pool, npool, lam, seconds,
+                                                   seed=int(lam), drain_s=drain_s),
+                    lam0, step, stop)
         sv.close()
-        res[name] = pts
+        return pts
+
+    # the baselines first (their sweeps start lower, so that the knee lies inside)
+    res["plain"] = server_sweep("plain", args.serve_rate0 / 4, rate_step, lambda p: False,
+                                slots=1, batching=False)
     # the plain entry point per request (what a caller does without the server)
     cap = 256 << 20
     fb, tb = ctypes.c_void_p(), ctypes.c_void_p()
     ct._check(ct.lib.curtree_malloc(ctypes.byref(fb), 4 * cap))
     ct._check(ct.lib.curtree_malloc(ctypes.byref(tb), 4 * cap))
-    pts = []
     _plain_entry_run(ct, trees, pool, npool, 200.0, 0.5, 1, (fb.value, tb.value), cap,
                      drain_s=drain_s)
-    lam = args.serve_rate0 / 4
-    while lam <= args.serve_rate_max:
-        p = _plain_entry_run(ct, trees, pool, npool, lam, seconds, int(lam),
-                             (fb.value, tb.value), cap, drain_s=drain_s)
-        pts.append(p)
-        print(f"[bench] plain entry point {lam:.0f} req/s: {p['pairs_per_s'] / 1e9:.2f} "
-              f"Gpairs/s, p50 {p['p50_ms']:.2f} ms, p99 {p['p99_ms']:.2f} ms, kept up "
-              f"{p['kept_up']}", file=sys.stderr, flush=True)
-        if not p["kept_up"]:
-            break
-        lam *= rate_step
+    pts = sweep("plain entry point",
+                lambda lam: _plain_entry_run(ct, trees, pool, npool, lam, seconds, int(lam),
+                                             (fb.value, tb.value), cap, drain_s=drain_s),
+                args.serve_rate0 / 4, rate_step, lambda p: False)
     ct.lib.curtree_free(fb)
     ct.lib.curtree_free(tb)
     res["plain_entry"] = pts
+    ref = _knee(res["plain"])
+    ref_e = _knee(res["plain_entry"])
+    if full:
+        res["served"] = server_sweep("served", args.serve_rate0, rate_step, lambda p: False,
+                                     slots=args.serve_slots, batching=True)
+    else:
+        p99_max = max(ref["p99_ms"], ref_e["p99_ms"])
+        res["served"] = server_sweep("served", min(ref["rate"], ref_e["rate"]),
+                                     args.serve_fine_step, lambda p: p["p99_ms"] > 1.5 * p99_max,
+                                     slots=args.serve_slots, batching=True)
     pool.free()
     for rt, d, _ in trees:
         rt.free()
         d.free()
     # equal-p99 comparisons: served pairs/s at each baseline's knee p99
-    ref = _knee(res["plain"])
-    ref_e = _knee(res["plain_entry"])
     best = _at_p99(res["served"], ref["p99_ms"])
     best_e = _at_p99(res["served"], ref_e["p99_ms"])
     value = best_e["pairs_per_s"] if best_e else 0.0
@@ -591,7 +601,7 @@ def serve_leg(args, ct, datagen, scale, seconds, rate_step, drain_s):
                     "windows per request (log-uniform), side log-uniform [1e-5,2e-3], "
                     f"{args.serve_budget_gb} GB result budget",
         "trees": sizes, "fanout": args.serve_fanout, "seconds_per_rate": seconds,
-        "rate_step": rate_step,
+        "rate_step": rate_step, "served_rate_step": rate_step if full else args.serve_fine_step,
         "served_pairs_per_s": value,
         "plain": res["plain"], "served": res["served"], "plain_entry": res["plain_entry"],
         "vs_plain_entry_point": {
@@ -613,7 +623,7 @@ def serve_leg(args, ct, datagen, scale, seconds, rate_step, drain_s):
 
 def bench_serve(args, ct, datagen):
     sv = serve_leg(args, ct, datagen, args.serve_scale, args.serve_seconds, args.serve_rate_step,
-                   20.0)
+                   20.0, full=True)
     out = {
         "metric": METRIC, "value": sv["served_pairs_per_s"], "unit": "pairs/s", "n_gpus": 1,
         "steps": len(sv["plain"]) + len(sv["served"]), "warmup": 1,
@@ -791,6 +801,9 @@ def main():
     ap.add_argument("--serve-rate0", type=float, default=2000.0)
     ap.add_argument("--serve-rate-max", type=float, default=64000.0)
     ap.add_argument("--serve-rate-step", type=float, default=1.2)
+    ap.add_argument("--serve-fine-step", type=float, default=1.08,
+                    help="served rate step of the default run's serve leg (from the baselines' "
+                         "knee up to 1.5x their knee p99)")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-batch-windows", type=int, default=1 << 20,
                     help="serve mode: window cap of one batch (and of one request)")

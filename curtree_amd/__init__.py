@@ -464,10 +464,12 @@ class Server:
 
     def completed(self, cap=4096):
         """Ids of requests finished since the last call (completion queue)."""
-        ids = (ctypes.c_int64 * cap)()
-        n = ctypes.c_int()
-        _check(lib.curtree_server_completed(self.p, ids, cap, ctypes.byref(n)))
-        return [int(ids[i]) for i in range(n.value)]
+        if getattr(self, "_ids_cap", 0) < cap:  # one buffer per server: cheap to poll
+            self._ids = (ctypes.c_int64 * cap)()
+            self._ids_cap = cap
+            self._n = ctypes.c_int()
+        _check(lib.curtree_server_completed(self.p, self._ids, cap, ctypes.byref(self._n)))
+        return self._ids[:self._n.value]
 
     def release(self, rid):
         _check(lib.curtree_server_release(self.p, int(rid)))

@@ -78,6 +78,64 @@ double now_us() {
       .count();
 }
 
+// Slot streams with a hardware queue of their own.  The runtime shares a
+// small pool of hardware queues among ordinary streams (4 per process by
+// default, one of them the null stream's), so two slots on one queue would
+// run their kernels one after the other.  A stream created with a CU mask (the
+// mask enables every CU) is given a queue outside that pool.  The library
+// creates at most kDedicatedQueues of them per device, once, and lends them
+// to servers (a slot returns its stream when the server is destroyed); slots
+// beyond them take ordinary streams.
+constexpr int kDedicatedQueues = 4;
+std::mutex g_qmu;
+std::unordered_map<int, std::vector<hipStream_t>> g_qfree;  // per device: idle streams
+std::unordered_map<int, int> g_qmade;                       // per device: streams created
+
+hipStream_t acquire_queue(int dev) {
+  std::lock_guard<std::mutex> g(g_qmu);
+  auto& fr = g_qfree[dev];
+  if (!fr.empty()) {
+    hipStream_t s = fr.back();
+    fr.pop_back();
+    return s;
+  }
+  if (g_qmade[dev] >= kDedicatedQueues) return nullptr;
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  std::vector<uint32_t> mask((size_t)std::max(1, (ncu + 31) / 32), 0xFFFFFFFFu);
+  hipStream_t s = nullptr;
+  if (hipExtStreamCreateWithCUMask(&s, (uint32_t)mask.size(), mask.data()) != hipSuccess) {
+    (void)hipGetLastError();
+    g_qmade[dev] = kDedicatedQueues;  // not available: ordinary streams from now on
+    return nullptr;
+  }
+  g_qmade[dev]++;
+  return s;
+}
+
+void release_queue(int dev, hipStream_t s) {
+  std::lock_guard<std::mutex> g(g_qmu);
+  g_qfree[dev].push_back(s);
+}
+
+// A batch's completion: polled for up to kSpinUs (a batch usually ends within
+// a few hundred microseconds, and a poll sees it at once), then a blocking
+// wait.
+constexpr double kSpinUs = 2000.0;
+hipError_t wait_event(hipEvent_t ev) {
+  const double t0 = now_us();
+  for (;;) {
+    const hipError_t e = hipEventQuery(ev);
+    if (e != hipErrorNotReady) return e;
+    if (now_us() - t0 > kSpinUs) break;
+    std::this_thread::yield();
+  }
+  return hipEventSynchronize(ev);
+}
+
 struct Request {
   int64_t id = 0;
   int tree = 0;
@@ -109,6 +167,8 @@ enum SlotState { kIdle, kWaitPool1, kPass1, kWaitPool2, kPass2 };
 
 struct Slot {
   hipStream_t s = nullptr;
+  bool dedicated = false;  // s is a stream with a hardware queue of its own (acquire_queue)
+  int dev = 0;
   hipEvent_t ev = nullptr;
   QEngine qe;
   float* win = nullptr;  // staging: 4 arrays of win_cap windows
@@ -141,6 +201,7 @@ struct Server {
   std::unordered_map<int64_t, Request> reqs;
   std::deque<int64_t> queue;
   std::deque<int64_t> completed;  // finished request ids not yet reported
+  std::atomic<int64_t> ncompleted{0};  // completed.size(), readable without the lock
   Pool pool;
   int* pool_fid = nullptr;
   int* pool_tid = nullptr;
@@ -172,8 +233,13 @@ int Server::start() {
   int rc;
   slots.resize((size_t)opt.slots);
   const int64_t cap = std::max<int64_t>(opt.max_batch_windows, 1);
+  int dev = 0;
+  CT_HIP(hipGetDevice(&dev));
   for (auto& sl : slots) {
-    CT_HIP(hipStreamCreateWithFlags(&sl.s, hipStreamNonBlocking));
+    sl.dev = dev;
+    sl.s = acquire_queue(dev);
+    if (sl.s) sl.dedicated = true;
+    else CT_HIP(hipStreamCreateWithFlags(&sl.s, hipStreamNonBlocking));
     CT_HIP(hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming));
     CT_HIP(hipMalloc((void**)&sl.win, (size_t)cap * 16));
     CT_HIP(hipMalloc((void**)&sl.counts, (size_t)cap * 8));
@@ -219,6 +285,7 @@ void Server::finish(Slot& sl, int err) {
     completed.push_back(id);
     t_last = t;
   }
+  ncompleted.store((int64_t)completed.size(), std::memory_order_release);
   sl.reqs.clear();
   sl.pool_off = sl.pool_len = 0;
   sl.state = kIdle;
@@ -461,7 +528,7 @@ void Server::worker(Slot& sl) {
       launch_us += dt;
     } else if (sl.state == kPass1 || sl.state == kPass2) {
       lk.unlock();
-      const hipError_t e = hipEventSynchronize(sl.ev);
+      const hipError_t e = wait_event(sl.ev);
       lk.lock();
       rc = e != hipSuccess ? set_error(CURTREE_EHIP, "serve: %s", hipGetErrorString(e))
                            : after_pass(sl);
@@ -586,12 +653,19 @@ CURTREE_API int curtree_server_completed(curtree_server* s, int64_t* ids, int ca
   Server* sv = (Server*)s;
   if (!sv || !n || cap < 0 || (cap > 0 && !ids))
     return set_error(CURTREE_EINVAL, "server_completed: bad argument");
+  // an empty queue is seen without the lock, so a client may poll often
+  // without holding up the workers
+  if (sv->ncompleted.load(std::memory_order_acquire) == 0) {
+    *n = 0;
+    return CURTREE_OK;
+  }
   std::lock_guard<std::mutex> g(sv->mu);
   int k = 0;
   while (k < cap && !sv->completed.empty()) {
     ids[k++] = sv->completed.front();
     sv->completed.pop_front();
   }
+  sv->ncompleted.store((int64_t)sv->completed.size(), std::memory_order_release);
   *n = k;
   return CURTREE_OK;
 }
@@ -658,7 +732,8 @@ CURTREE_API int curtree_server_destroy(curtree_server* s) {
     if (w.joinable()) w.join();
   (void)hipDeviceSynchronize();
   for (auto& sl : sv->slots) {
-    if (sl.s) (void)hipStreamDestroy(sl.s);
+    if (sl.dedicated) release_queue(sl.dev, sl.s);  // kept for the next server
+    else if (sl.s) (void)hipStreamDestroy(sl.s);
     if (sl.ev) (void)hipEventDestroy(sl.ev);
     if (sl.win) (void)hipFree(sl.win);
     if (sl.counts) (void)hipFree(sl.counts);
