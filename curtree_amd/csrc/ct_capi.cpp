@@ -70,6 +70,9 @@ struct TreeState {
   int64_t sz = 0, nbox = 0;
   int fanout = 0;
   uint32_t* words = nullptr;  // device: [0] every box is well-formed
+  void* codes = nullptr;      // device: leaf codes (leaf_codes_bytes), or null
+  int64_t codes_bytes = 0;
+  bool codes_ready = false;
   bool overlaps(const char* p, int64_t bytes) const {
     for (int j = 0; j < 11; j++) {
       const char* a = (const char*)arr[j];
@@ -84,6 +87,17 @@ std::vector<TreeState> g_states;
 // words of forgotten entries, reused (never freed: 256 B each, and a free
 // could stall an asynchronous build or break a graph capture)
 std::vector<uint32_t*> g_spare_words;
+// leaf-code buffers of forgotten entries (bytes), reused by later trees
+std::vector<std::pair<void*, int64_t>> g_spare_codes;
+
+// Moves an entry's words and codes to the spare lists.
+void retire_locked(TreeState& e) {
+  g_spare_words.push_back(e.words);
+  if (e.codes) g_spare_codes.push_back({e.codes, e.codes_bytes});
+  e.codes = nullptr;
+  e.codes_bytes = 0;
+  e.codes_ready = false;
+}
 
 TreeState state_key(const curtree_rtree_f32* rt, const curtree_bbox_f32* box) {
   TreeState k;
@@ -103,7 +117,7 @@ void forget_locked(const void* const* a, int n) {
     for (int j = 0; j < 11 && !hit; j++)
       for (int k = 0; k < n && !hit; k++) hit = a[k] && g_states[i].arr[j] == a[k];
     if (hit) {
-      g_spare_words.push_back(g_states[i].words);
+      retire_locked(g_states[i]);
       g_states[i] = g_states.back();
       g_states.pop_back();
     } else {
@@ -133,16 +147,61 @@ uint32_t* tree_state_register(const curtree_rtree_f32* rt, const curtree_bbox_f3
   return k.words;
 }
 
-const uint32_t* tree_state_find(const curtree_rtree_f32* rt, const curtree_bbox_f32* box) {
-  std::lock_guard<std::mutex> g(g_state_mu);
+namespace {
+TreeState* find_locked(const curtree_rtree_f32* rt, const curtree_bbox_f32* box) {
   const TreeState k = state_key(rt, box);
-  for (const TreeState& e : g_states) {
+  for (TreeState& e : g_states) {
     if (e.sz != k.sz || e.nbox != k.nbox || e.fanout != k.fanout) continue;
     bool same = true;
     for (int j = 0; j < 11 && same; j++) same = e.arr[j] == k.arr[j];
-    if (same) return e.words;
+    if (same) return &e;
   }
   return nullptr;
+}
+}  // namespace
+
+const uint32_t* tree_state_find(const curtree_rtree_f32* rt, const curtree_bbox_f32* box) {
+  std::lock_guard<std::mutex> g(g_state_mu);
+  const TreeState* e = find_locked(rt, box);
+  return e ? e->words : nullptr;
+}
+
+const void* tree_state_codes(const curtree_rtree_f32* rt, const curtree_bbox_f32* box,
+                             int64_t bytes, const std::function<int(void*)>* build) {
+  std::lock_guard<std::mutex> g(g_state_mu);
+  TreeState* e = find_locked(rt, box);
+  if (!e) return nullptr;
+  if (e->codes_ready) return e->codes;
+  if (!build) return nullptr;
+  if (!e->codes) {
+    // the smallest spare buffer that holds them, else a new one
+    size_t best = g_spare_codes.size();
+    for (size_t i = 0; i < g_spare_codes.size(); i++)
+      if (g_spare_codes[i].second >= bytes &&
+          (best == g_spare_codes.size() || g_spare_codes[i].second < g_spare_codes[best].second))
+        best = i;
+    if (best < g_spare_codes.size()) {
+      // a query of the tree that had it may still be reading it
+      if (hipDeviceSynchronize() != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+      }
+      e->codes = g_spare_codes[best].first;
+      e->codes_bytes = g_spare_codes[best].second;
+      g_spare_codes[best] = g_spare_codes.back();
+      g_spare_codes.pop_back();
+    } else {
+      if (hipMalloc(&e->codes, (size_t)bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        e->codes = nullptr;
+        return nullptr;  // no codes: the boxes are tested directly
+      }
+      e->codes_bytes = bytes;
+    }
+  }
+  if ((*build)(e->codes) != CURTREE_OK) return nullptr;
+  e->codes_ready = true;
+  return e->codes;
 }
 
 void tree_state_forget(const void* const* arrays, int n) {
@@ -155,7 +214,7 @@ void tree_state_forget_range(const void* p, int64_t bytes) {
   std::lock_guard<std::mutex> g(g_state_mu);
   for (size_t i = 0; i < g_states.size();) {
     if (g_states[i].overlaps((const char*)p, bytes)) {
-      g_spare_words.push_back(g_states[i].words);
+      retire_locked(g_states[i]);
       g_states[i] = g_states.back();
       g_states.pop_back();
     } else {
@@ -624,6 +683,11 @@ CURTREE_API int curtree_set_option(const char* name, int64_t value) {
   if (!strcmp(name, "window_order")) {
     if (value < -1 || value > 1) return set_error(CURTREE_EINVAL, "window_order: -1, 0 or 1");
     qtuning().order = (int)value;
+    return CURTREE_OK;
+  }
+  if (!strcmp(name, "leaf_codes")) {
+    if (value < -1 || value > 1) return set_error(CURTREE_EINVAL, "leaf_codes: -1, 0 or 1");
+    qtuning().codes = (int)value;
     return CURTREE_OK;
   }
   if (!strcmp(name, "tile_xcd")) {

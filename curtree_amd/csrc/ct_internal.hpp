@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <map>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -58,6 +59,14 @@ void rt_free(curtree_rtree_f32* rt);
 uint32_t* tree_state_register(const curtree_rtree_f32* rt, const curtree_bbox_f32* box,
                               bool may_alloc);
 const uint32_t* tree_state_find(const curtree_rtree_f32* rt, const curtree_bbox_f32* box);
+// Leaf codes kept with a tree (`bytes` of them): the ready buffer, or null.
+// With `build` and none ready, a buffer is found (a spare one, after a device
+// synchronisation, since its last tree's queries may still read it, or a new
+// one) and build(buffer) fills it; build must finish its device work before
+// it returns.  Null when the tree is not kept or on any failure (the leaf
+// count then tests the boxes directly).
+const void* tree_state_codes(const curtree_rtree_f32* rt, const curtree_bbox_f32* box,
+                             int64_t bytes, const std::function<int(void*)>* build);
 // Forget every entry that names one of these arrays (null entries ignored).
 void tree_state_forget(const void* const* arrays, int n);
 // Forget every entry with an array that overlaps [p, p + bytes).

@@ -560,6 +560,218 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_count(
     atomicAdd((unsigned long long*)&c->total, (unsigned long long)block_total);
 }
 
+// Bit i of x (below 2^16) to bit 4i.
+__device__ __forceinline__ uint64_t spread4(uint32_t x) {
+  uint64_t v = x;
+  v = (v | (v << 24)) & 0x000000FF000000FFull;
+  v = (v | (v << 12)) & 0x000F000F000F000Full;
+  v = (v | (v << 6)) & 0x0303030303030303ull;
+  v = (v | (v << 3)) & 0x1111111111111111ull;
+  return v;
+}
+
+// The leaf count over leaf codes (tr.lframe / tr.lcode, G >= 4): the same
+// tiles of P entries and the same outputs as k_leaf_count, but G/4 lanes per
+// entry, each lane holding the cells of four consecutive child slots (one
+// 16-byte load), so one wave-step covers four times the entries and a whole
+// tile is one round of loads (entries, then windows, frames and cells).  A
+// test the cells leave open is settled at the end of the tile, in one more
+// round: the box and the window read, the float test.  A layout other than
+// the bulk-load one has no codes: every test is open (pos/len give the
+// children).
+#ifndef CT_CODED_WAVES
+#define CT_CODED_WAVES 6  // waves per SIMD asked of the compiler (0: its choice)
+#endif
+template <int G, int STEPS, bool STORE, bool QCOUNTS>
+__global__ __launch_bounds__(kQThreads)
+#if CT_CODED_WAVES > 0
+__attribute__((amdgpu_waves_per_eu(CT_CODED_WAVES)))
+#endif
+void k_leaf_count_coded(
+    int slot, QTree tr, QBoxes qb, QBoxes rb, QCtrl* c, const uint2* __restrict__ fin,
+    int64_t r_sz, int64_t nbox, QLeaf* __restrict__ est, uint64_t* __restrict__ ttot,
+    unsigned long long* counts, bool xcd) {
+  using S = QShape<G, STEPS>;
+  constexpr int P = S::kPerTile;
+  constexpr int GC = G / 4;           // lanes per entry
+  constexpr int EC = kWave / GC;      // entries per wave-step
+  constexpr int SC = S::kPerWave / EC;  // wave-steps per tile
+  static_assert(G >= 4 && SC * EC == S::kPerWave, "whole wave-steps");
+  constexpr uint32_t kNib = GC >= 32 ? 0xFFFFFFFFu : ((1u << GC) - 1u);
+  __shared__ uint64_t s_red[kQWaves];
+  __shared__ QLevels s_lv;
+  __shared__ QLeaf s_est[STORE ? P : 1];
+  __shared__ uint32_t s_cq[QCOUNTS ? P : 1];
+  __shared__ uint32_t s_cn[QCOUNTS ? P : 1];
+  __shared__ uint64_t s_scan[kQWaves];
+  if (c->overflow) return;
+  const int64_t n_in = (int64_t)c->count[slot];
+  const int64_t q0 = c->q0;
+  const int64_t ntiles = (n_in + P - 1) / P;
+  const int t = threadIdx.x, w = t >> 6, lane = lane_id();
+  if (ntiles == 0) return;
+  for (int i = t; i < (int)(sizeof(QLevels) / 4); i += kQThreads)
+    ((uint32_t*)&s_lv)[i] = ((const uint32_t*)&c->lv)[i];
+  const int sub = lane / GC, cg = lane % GC;  // entry of the wave-step, slot group
+  const int lead = sub * GC;
+  const bool canon = tr.canon && *tr.canon;
+  const int64_t ls = c->lv.lstart[0];
+  uint64_t block_total = 0;
+  const TileWalk tw = tile_walk(ntiles, xcd);
+  for (int64_t tile = tw.first; tile < tw.end; tile += tw.step) {
+    __syncthreads();
+    uint64_t wsum = 0;
+    const int64_t tbase = tile * P;
+    uint2 ent[SC];
+#pragma unroll
+    for (int k = 0; k < SC; k++) {
+      const int64_t e = tbase + w * S::kPerWave + k * EC + sub;
+      ent[k] = fin ? fin[e < n_in ? e : n_in - 1]
+                   : make_uint2((uint32_t)(q0 + (e < n_in ? e : n_in - 1)), 0u);
+    }
+    uint32_t q[SC];
+    int first[SC], cnt[SC], lpn[SC];
+    bool full[SC];
+#pragma unroll
+    for (int k = 0; k < SC; k++) {
+      q[k] = ent[k].x;
+      const uint32_t node = ent[k].y & ~kFull;
+      full[k] = (ent[k].y & kFull) != 0;
+      lpn[k] = -1;
+      if (full[k]) {
+        int64_t f, n;
+        node_range(node, &s_lv, nbox, &f, &n);
+        first[k] = (int)f;
+        cnt[k] = (int)n;
+      } else if (canon) {
+        const int64_t f = ((int64_t)node - ls) * tr.F;
+        first[k] = (int)f;
+        cnt[k] = (int)(nbox - f < tr.F ? nbox - f : tr.F);
+        lpn[k] = (int)((int64_t)node - ls);
+      } else {
+        first[k] = (int)((int64_t)tr.pos[node] - r_sz);
+        cnt[k] = tr.len[node];
+      }
+    }
+    // windows, frames and the cells of this lane's four slots
+    float4 wq[SC], fr[SC];
+    uint4 cd[SC];
+    uint32_t test[SC];  // bit j: slot 4*cg + j is a child box to test
+#pragma unroll
+    for (int k = 0; k < SC; k++) {
+      test[k] = 0;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int sl = 4 * cg + j, bi = first[k] + sl;
+        if (!full[k] && sl < cnt[k] && bi >= 0 && bi < nbox) test[k] |= 1u << j;
+      }
+      wq[k] = ld_box(qb, q[k]);
+      const bool coded = lpn[k] >= 0 && test[k] != 0;
+      fr[k] = coded ? tr.lframe[lpn[k]] : make_float4(0.f, 0.f, 0.f, 0.f);
+      cd[k] = coded ? ((const uint4*)tr.lcode)[(int64_t)lpn[k] * GC + cg] : make_uint4(0, 0, 0, 0);
+    }
+    // decisions from the cells; open tests settled by the boxes
+    uint32_t hit[SC], open[SC];
+#pragma unroll
+    for (int k = 0; k < SC; k++) {
+      const int64_t e = tbase + w * S::kPerWave + k * EC + sub;
+      hit[k] = open[k] = 0;
+      if (e < n_in && test[k]) {
+        if (fr[k].z > 0.f) {
+          const uint32_t wc = box_cells(wq[k].x, wq[k].y, wq[k].z, wq[k].w, fr[k]);
+          const uint32_t b4[4] = {cd[k].x, cd[k].y, cd[k].z, cd[k].w};
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            if (!((test[k] >> j) & 1u)) continue;
+            const int r = cells_intersect(wc, b4[j]);
+            if (r > 0) hit[k] |= 1u << j;
+            else if (r < 0) open[k] |= 1u << j;
+          }
+        } else {
+          open[k] = test[k];
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < SC; k++) {
+      if (open[k]) {
+        float4 bx[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          if ((open[k] >> j) & 1u) bx[j] = ld_box(rb, first[k] + 4 * cg + j);
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          if (((open[k] >> j) & 1u) &&
+              intersect(wq[k].x, wq[k].y, wq[k].z, wq[k].w, bx[j].x, bx[j].y, bx[j].z, bx[j].w))
+            hit[k] |= 1u << j;
+      }
+    }
+    // per entry: the hit mask over its slots (slot 4*cg + j from lane cg's bit j)
+#pragma unroll
+    for (int k = 0; k < SC; k++) {
+      const int64_t e = tbase + w * S::kPerWave + k * EC + sub;
+      const bool valid = e < n_in;
+      uint64_t gm = 0;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const uint64_t m = ballot((hit[k] >> j) & 1u);
+        gm |= spread4((uint32_t)(m >> lead) & kNib) << j;
+      }
+      const uint64_t nh = full[k] ? (uint64_t)cnt[k] : (uint64_t)popc64(gm);
+      if (cg == 0 && valid) {
+        if (STORE) {
+          QLeaf x;
+          x.val = full[k] ? nh : gm;
+          x.first = (uint32_t)first[k];
+          x.qf = q[k] | (full[k] ? kQRange : 0u);
+          s_est[e - tbase] = x;
+        }
+        if (QCOUNTS) {
+          s_cq[e - tbase] = q[k];
+          s_cn[e - tbase] = (uint32_t)nh;
+        }
+        wsum += nh;
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) wsum += __shfl_xor(wsum, o);
+    if (lane == 0) s_red[w] = wsum;
+    __syncthreads();
+    if (STORE)
+      for (int i = t; i < P; i += kQThreads)
+        if (tbase + i < n_in) est[tbase + i] = s_est[i];
+    if (t == 0) {
+      uint64_t tot = 0;
+#pragma unroll
+      for (int i = 0; i < kQWaves; i++) tot += s_red[i];
+      if (STORE) ttot[tile] = tot;
+      block_total += tot;
+    }
+    if (QCOUNTS) {
+      constexpr int R = (P + kQThreads - 1) / kQThreads;
+      const int nv = n_in - tbase < P ? (int)(n_in - tbase) : P;
+      uint64_t sum = 0;
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        if (t * R + r < nv) sum += s_cn[t * R + r];
+      uint64_t tile_sum;
+      uint64_t before = block_excl_scan64(sum, s_scan, &tile_sum);
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int i = t * R + r;
+        if (i >= nv) break;
+        const uint32_t qi = s_cq[i];
+        if (i == 0 || s_cq[i - 1] != qi) atomicAdd(counts + qi, 0ull - before);
+        before += s_cn[i];
+        if (i + 1 == nv || s_cq[i + 1] != qi) atomicAdd(counts + qi, before);
+      }
+    }
+  }
+  if (threadIdx.x == 0 && block_total)
+    atomicAdd((unsigned long long*)&c->total, (unsigned long long)block_total);
+}
+
+
 // Exclusive scan in place of the tile counts of frontier `cslot` (tiles of P
 // entries); `slot` names the launch's ticket and epoch.
 __global__ __launch_bounds__(kQThreads) void k_tile_scan(int slot, int cslot, int P, QCtrl* c,
@@ -1573,6 +1785,49 @@ int reserve_f4(float4** dst, int64_t* cap, int64_t n) {
   return CURTREE_OK;
 }
 
+// Leaf codes of a bulk-load tree: thread i codes box i in the frame of its
+// leaf-parent (node ls + i / F) into slot (i / F) * G + i % F (G >= F slots per
+// leaf-parent, so that four consecutive slots are one aligned 16-byte load);
+// the first box of each leaf-parent also writes the frame.  A frame needs a
+// finite low corner and a finite, positive extent on both axes with a finite
+// scale; otherwise its scales are 0 and the leaf count tests that node's
+// boxes directly.
+__global__ __launch_bounds__(256) void k_leaf_codes(int64_t nbox, int F, int G, int64_t ls,
+                                                    const float* __restrict__ nx1,
+                                                    const float* __restrict__ ny1,
+                                                    const float* __restrict__ nx2,
+                                                    const float* __restrict__ ny2, QBoxes rb,
+                                                    float4* __restrict__ frame,
+                                                    uint32_t* __restrict__ code) {
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < nbox; i += (int64_t)gridDim.x * 256) {
+    const int64_t lp = i / F;
+    const float a1 = nx1[ls + lp], b1 = ny1[ls + lp], a2 = nx2[ls + lp], b2 = ny2[ls + lp];
+    const float ex = a2 - a1, ey = b2 - b1;
+    float4 fr = make_float4(a1, b1, 0.f, 0.f);
+    if (isfinite(a1) && isfinite(b1) && isfinite(ex) && isfinite(ey) && ex > 0.f && ey > 0.f) {
+      const float sx = kCells / ex, sy = kCells / ey;
+      if (isfinite(sx) && isfinite(sy)) fr = make_float4(a1, b1, sx, sy);
+    }
+    if (i == lp * F) frame[lp] = fr;
+    code[lp * G + (i - lp * F)] = box_cells(rb.x0[i], rb.y0[i], rb.x1[i], rb.y1[i], fr);
+  }
+}
+
+int64_t leaf_codes_bytes(int64_t nleafparents, int G) {
+  return nleafparents * ((int64_t)sizeof(float4) + 4 * (int64_t)G);
+}
+
+void enqueue_leaf_codes(const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox, int64_t ls,
+                        int64_t nlp, int G, void* buf, hipStream_t s) {
+  const int64_t n = rbox->sz;
+  if (n <= 0) return;
+  const unsigned grid = (unsigned)std::min<int64_t>((n + 255) / 256, 16384);
+  hipLaunchKernelGGL(k_leaf_codes, dim3(grid), dim3(256), 0, s, n, rt->fanout, G, ls, rt->xmin,
+                     rt->ymin, rt->xmax, rt->ymax,
+                     QBoxes{rbox->xmin, rbox->ymin, rbox->xmax, rbox->ymax, nullptr},
+                     (float4*)buf, (uint32_t*)((float4*)buf + nlp));
+}
+
 int stage_boxes(float4** dst, int64_t* cap, const QBoxes& src, int64_t n, hipStream_t s) {
   int rc;
   if ((rc = reserve_f4(dst, cap, n))) return rc;
@@ -1620,6 +1875,8 @@ int compute_safe(QEngine& qe, Prof& pf, const QBoxes& rb, int F, int64_t r_sz, i
 
 // Runs of at least this many queries expand without a tile chain.
 constexpr int64_t kSplitMinQueries = 32768;
+// a query of at least this many windows builds the leaf codes of a kept tree
+constexpr int64_t kCodesMinWindows = 65536;
 
 // Blocks of `kernel` (kQThreads threads) that the device holds at once.  The
 // chain-free count and write kernels stride over their tiles with a fixed
@@ -1644,6 +1901,36 @@ static unsigned leaf_count_grid() {
   return g;
 }
 
+template <int G, int STEPS, bool STORE, bool QCOUNTS>
+static unsigned leaf_count_coded_grid() {
+  static const unsigned g =
+      resident_grid((const void*)k_leaf_count_coded<G, STEPS, STORE, QCOUNTS>);
+  return g;
+}
+
+// The leaf count: over the leaf codes when the run has them (G >= 4; tiles
+// of up to 8 wave-steps, two rounds of coded loads per tile), else the box
+// tests.
+template <int G, int STEPS, bool STORE, bool QCOUNTS>
+static void launch_leaf_count(const QRunArgs& a, int leaf, const uint2* cur, QCtrl* ctrl,
+                              QLeaf* est, uint64_t* ttot, unsigned long long* counts, bool xcd,
+                              hipStream_t s) {
+  if constexpr (G >= 4 && STEPS <= 8) {
+    if (a.tr.lcode) {
+      hipLaunchKernelGGL((k_leaf_count_coded<G, STEPS, STORE, QCOUNTS>),
+                         dim3(std::min<unsigned>(a.grid,
+                                                 leaf_count_coded_grid<G, STEPS, STORE, QCOUNTS>())),
+                         dim3(kQThreads), 0, s, leaf, a.tr, a.qb, a.rb, ctrl, cur, a.r_sz, a.nbox,
+                         est, ttot, counts, xcd);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_leaf_count<G, STEPS, STORE, QCOUNTS>),
+                     dim3(std::min<unsigned>(a.grid, leaf_count_grid<G, STEPS, STORE, QCOUNTS>())),
+                     dim3(kQThreads), 0, s, leaf, a.tr, a.qb, a.rb, ctrl, cur, a.r_sz, a.nbox, est,
+                     ttot, counts, xcd);
+}
+
 template <int G, int STEPS>
 static unsigned expand_count_grid() {
   static const unsigned g = resident_grid((const void*)k_expand_count<G, STEPS>);
@@ -1662,6 +1949,16 @@ static unsigned leaf_write_grid() {
   return g;
 }
 
+template <int G, int STEPS>
+static void warm_coded() {
+  if constexpr (G >= 4 && STEPS <= 8) {
+    (void)leaf_count_coded_grid<G, STEPS, true, false>();
+    (void)leaf_count_coded_grid<G, STEPS, true, true>();
+    (void)leaf_count_coded_grid<G, STEPS, false, false>();
+    (void)leaf_count_coded_grid<G, STEPS, false, true>();
+  }
+}
+
 // Queries the occupancy of the count kernels for (G, steps) ahead of any
 // graph capture (the first query of each is cached).
 template <int G>
@@ -1675,6 +1972,7 @@ static void warm_grids_g(int steps) {
     (void)leaf_count_grid<G, st, true, true>();   \
     (void)leaf_count_grid<G, st, false, false>(); \
     (void)leaf_count_grid<G, st, false, true>();  \
+    warm_coded<G, st>();                          \
     (void)leaf_write_grid<G, st>();               \
     break;
     CT_WARM(4) CT_WARM(16)
@@ -1771,15 +2069,11 @@ void enqueue_run_g(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
   } else if (a.write) {
     pf.mark(s, "k_leaf_count");
     if (a.counts)
-      hipLaunchKernelGGL((k_leaf_count<G, STEPS, true, true>),
-                         dim3(std::min<unsigned>(a.grid, leaf_count_grid<G, STEPS, true, true>())),
-                         dim3(kQThreads), 0, s, leaf, a.tr, a.qb, a.rb, qe.ctrl, cur, a.r_sz,
-                         a.nbox, (QLeaf*)qe.est, qe.ttot, a.counts, xcd);
+      launch_leaf_count<G, STEPS, true, true>(a, leaf, cur, qe.ctrl, (QLeaf*)qe.est, qe.ttot,
+                                              a.counts, xcd, s);
     else
-      hipLaunchKernelGGL((k_leaf_count<G, STEPS, true, false>),
-                         dim3(std::min<unsigned>(a.grid, leaf_count_grid<G, STEPS, true, false>())),
-                         dim3(kQThreads), 0, s, leaf, a.tr, a.qb, a.rb, qe.ctrl, cur, a.r_sz,
-                         a.nbox, (QLeaf*)qe.est, qe.ttot, (unsigned long long*)nullptr, xcd);
+      launch_leaf_count<G, STEPS, true, false>(a, leaf, cur, qe.ctrl, (QLeaf*)qe.est, qe.ttot,
+                                               (unsigned long long*)nullptr, xcd, s);
     pf.mark(s, "k_tile_scan");
     hipLaunchKernelGGL(k_tile_scan, dim3(64), dim3(kQThreads), 0, s, leaf + 1, leaf, S::kPerTile,
                        qe.ctrl, qe.ttot, qe.status);
@@ -1808,17 +2102,12 @@ void enqueue_run_g(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
   } else {
     pf.mark(s, "k_leaf_count");
     if (a.counts)
-      hipLaunchKernelGGL((k_leaf_count<G, STEPS, false, true>),
-                         dim3(std::min<unsigned>(a.grid, leaf_count_grid<G, STEPS, false, true>())),
-                         dim3(kQThreads), 0, s, leaf, a.tr, a.qb, a.rb, qe.ctrl, cur, a.r_sz,
-                         a.nbox, (QLeaf*)nullptr, (uint64_t*)nullptr, a.counts, xcd);
+      launch_leaf_count<G, STEPS, false, true>(a, leaf, cur, qe.ctrl, (QLeaf*)nullptr,
+                                               (uint64_t*)nullptr, a.counts, xcd, s);
     else
-      hipLaunchKernelGGL((k_leaf_count<G, STEPS, false, false>),
-                         dim3(std::min<unsigned>(a.grid,
-                                                 leaf_count_grid<G, STEPS, false, false>())),
-                         dim3(kQThreads), 0, s, leaf, a.tr, a.qb, a.rb, qe.ctrl, cur, a.r_sz,
-                         a.nbox, (QLeaf*)nullptr, (uint64_t*)nullptr,
-                         (unsigned long long*)nullptr, xcd);
+      launch_leaf_count<G, STEPS, false, false>(a, leaf, cur, qe.ctrl, (QLeaf*)nullptr,
+                                                (uint64_t*)nullptr, (unsigned long long*)nullptr,
+                                                xcd, s);
   }
   pf.mark(s, nullptr);
 }
@@ -2109,11 +2398,40 @@ int prepare_query(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curt
   const bool stage = !qe.fixed && st == 1;
   bool staged = false;
   if (stage && (rc = reserve_f4(&qe.box4, &qe.cap_box4, N))) return rc;
-  // a tree built by this library over these boxes: its build recorded whether
-  // every box is well-formed, so no per-node flags are computed (the layout is
-  // still checked: two small launches over pos/len)
-  const uint32_t* kept = use ? tree_state_find(rt, rbox) : nullptr;
-  if (kept) {
+  // A tree built by this library over these boxes keeps state with it: its
+  // build recorded whether every box is well-formed, so no per-node flags are
+  // computed (the layout is still checked: two small launches over pos/len),
+  // and its leaf codes, built once by the first large enough query (not under
+  // graph capture: the build synchronises), serve every later query.
+  const int cz = qtuning().codes;
+  const bool codes = !qe.fixed && !a->generic && a->G >= 4 && cz != 0 &&
+                     (cz == 1 || nq_hint >= 256);
+  const uint32_t* kept = use || codes ? tree_state_find(rt, rbox) : nullptr;
+  if (kept && codes) {
+    {
+      hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+      if (hipStreamIsCapturing(s, &cs) != hipSuccess) {
+        (void)hipGetLastError();
+        cs = hipStreamCaptureStatusActive;
+      }
+      const bool make = cs == hipStreamCaptureStatusNone && (cz == 1 || nq_hint >= kCodesMinWindows);
+      const int64_t nlp = lp.n[0], ls = a->lv.lstart[0];
+      const std::function<int(void*)> build = [&](void* buf) {
+        pf.mark(s, "k_leaf_codes");
+        enqueue_leaf_codes(rt, rbox, ls, nlp, a->G, buf, s);
+        pf.mark(s, nullptr);
+        CT_HIP(hipGetLastError());
+        CT_HIP(hipStreamSynchronize(s));
+        return CURTREE_OK;
+      };
+      const void* cb = tree_state_codes(rt, rbox, leaf_codes_bytes(nlp, a->G), make ? &build : nullptr);
+      if (cb) {
+        a->tr.lframe = (const float4*)cb;
+        a->tr.lcode = (const uint32_t*)((const float4*)cb + nlp);
+      }
+    }
+  }
+  if (kept && use) {
     a->tr.allwf = kept;
   } else if (use) {  // the flag pass also writes the AoS boxes when both are wanted
     if ((rc = compute_safe(qe, pf, a->rb, F, lp.r_sz, N, s, stage ? qe.box4 : nullptr, &staged)))
@@ -2131,7 +2449,7 @@ int prepare_query(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curt
     a->tr.mbr.b4 = qe.node4;
     a->qb.b4 = qe.win4;
   }
-  if (use || stage || qe.fixed) {
+  if (use || stage || qe.fixed || a->tr.lcode) {  // (the codes need the bulk-load layout)
     if ((rc = check_layout(&qe.canon, rt, N, s))) return rc;
     a->tr.canon = qe.canon;
   }

@@ -28,6 +28,7 @@ struct QTuning {
                      // -1 the chained pass for runs below kSplitMinQueries windows
   int xcd = 0;       // chain-free kernels: 1 each XCD walks a contiguous eighth of the tiles
   int order = -1;    // windows in Morton order of their centres: -1 from kOrderMinWindows, 0, 1
+  int codes = -1;    // leaf codes of kept trees: -1 built from kCodesMinWindows, 0 off, 1 on
 };
 QTuning& qtuning();
 
@@ -54,8 +55,49 @@ struct QTree {
   // (children of node i of a level are i*F .. i*F+F-1 of the level below):
   // child ranges are then computed, not loaded.  Null: always load.
   const uint32_t* canon = nullptr;
+  // Leaf codes (or both null): per leaf-parent a frame and G slots of four
+  // 8-bit cells of that frame per box (see box_cells); bulk-load layout only.
+  const float4* lframe = nullptr;
+  const uint32_t* lcode = nullptr;
   int F = 0;
 };
+
+// Leaf codes.  A leaf-parent's frame is (x0, y0, sx, sy): the low corner of
+// its MBR and kCells / extent per axis (sx = 0: no codes for this node).  A
+// coordinate v maps to a cell of that frame by one function, the same for
+// boxes and windows:
+//   t = (v - x0) * sx, rounded after each step (no contraction);
+//   t < 0: cell 0; t >= kCells: cell 255; else 1 + floor(t).
+// The map is monotone (v <= w gives cell(v) <= cell(w)).  The box test is
+// "not separated": !(lo > hi) for the four pairs (window low side, box high
+// side) and (box low side, window high side).  cell(lo) > cell(hi) proves a
+// separation, cell(lo) < cell(hi) proves there is none, equal cells leave the
+// pair open, and only then is the box itself read.  NaN takes cell 0 on the
+// low side (xmin, ymin) and cell 255 on the high side (xmax, ymax), so a pair
+// with a NaN is never proven separated, as the float compare (false) says.
+constexpr float kCells = 254.f;
+__device__ __forceinline__ uint32_t cell_of(float v, float x0, float sx, bool low_side) {
+  if (v != v) return low_side ? 0u : 255u;
+  const float t = __fmul_rn(__fsub_rn(v, x0), sx);
+  if (!(t >= 0.f)) return 0u;
+  if (t >= kCells) return 255u;
+  return 1u + (uint32_t)t;
+}
+// Cells of a box (or window): xmin | ymin << 8 | xmax << 16 | ymax << 24.
+__device__ __forceinline__ uint32_t box_cells(float x1, float y1, float x2, float y2,
+                                              const float4& fr) {
+  return cell_of(x1, fr.x, fr.z, true) | (cell_of(y1, fr.y, fr.w, true) << 8) |
+         (cell_of(x2, fr.x, fr.z, false) << 16) | (cell_of(y2, fr.y, fr.w, false) << 24);
+}
+// Intersection (no separation) of window cells w and box cells b: 1 proven,
+// 0 disproven (a separation is proven), -1 open.
+__device__ __forceinline__ int cells_intersect(uint32_t w, uint32_t b) {
+  const uint32_t wx1 = w & 255u, wy1 = (w >> 8) & 255u, wx2 = (w >> 16) & 255u, wy2 = w >> 24;
+  const uint32_t bx1 = b & 255u, by1 = (b >> 8) & 255u, bx2 = (b >> 16) & 255u, by2 = b >> 24;
+  if (wx1 > bx2 || bx1 > wx2 || wy1 > by2 || by1 > wy2) return 0;
+  if (wx1 < bx2 && bx1 < wx2 && wy1 < by2 && by1 < wy2) return 1;
+  return -1;
+}
 
 // Tiles of block blockIdx.x in a chain-free launch: first, first + step, ...
 // below end.  xcd: blocks are dispatched to the 8 XCDs round-robin (block b
@@ -178,5 +220,11 @@ void enqueue_run(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s);
 int64_t pieces_needed(const QEngine& qe, const QRunArgs& a);
 void enqueue_fill_counts(int64_t* counts, int64_t n, hipStream_t s);
 int read_ctrl(QEngine& qe, hipStream_t s);
+// Leaf codes of a bulk-load tree (nlp leaf-parents from node ls, G >= F
+// slots each): the frames of the leaf-parents, then G u32 cells per
+// leaf-parent, in one buffer of leaf_codes_bytes.
+int64_t leaf_codes_bytes(int64_t nleafparents, int G);
+void enqueue_leaf_codes(const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox, int64_t ls,
+                        int64_t nlp, int G, void* buf, hipStream_t s);
 
 }  // namespace ct

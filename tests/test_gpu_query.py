@@ -556,3 +556,62 @@ def test_window_order_same_pairs(contain):
     rt.free()
     d.free()
     q.free()
+
+
+@pytest.mark.parametrize("contain", [0, 1])
+def test_leaf_codes_same_pairs(contain):
+    # Leaf codes (8-bit cells of each box in its leaf-parent's frame) decide
+    # most leaf tests without reading the box; a test the cells leave open
+    # reads the box.  The raw pair list and the per-window counts must equal
+    # those of the direct tests and the oracle's traversal, with windows whose
+    # edges fall exactly on box edges (open cells), NaN, inverted, point and
+    # infinite boxes, NaN and point windows, and leaf-parents of zero or
+    # infinite extent (no frame: every test reads the box).
+    import curtree_amd as ct
+
+    n, nq = 200_000, 70_000
+    bx = [a.copy() for a in datagen.mixture_boxes(n, seed=171)]
+    r = np.random.default_rng(171)
+    bx[1][r.integers(0, n, 40)] = np.nan
+    bx[2][r.integers(0, n, 40)] = np.nan
+    inv = r.integers(0, n, 100)
+    bx[0][inv], bx[2][inv] = bx[2][inv] + np.float32(1e-4), bx[0][inv]
+    pts = r.integers(0, n, 2000)
+    bx[2][pts], bx[3][pts] = bx[0][pts], bx[1][pts]
+    bx[2][r.integers(0, n, 5)] = np.inf
+    same = np.arange(1000, 1200)  # 200 copies of one box: leaf-parents of zero extent
+    for a in bx:
+        a[same] = a[1000]
+    ids = np.arange(n, dtype=np.int32)
+    qs = [a.copy() for a in datagen.loguniform_windows(nq, seed=172, lo=1e-4, hi=1e-2)[:4]]
+    # windows whose edges are box edges (shifted by whole boxes)
+    k = r.integers(0, n, 20_000)
+    qs[0][:20_000], qs[1][:20_000] = bx[2][k], bx[3][k]
+    qs[2][:20_000], qs[3][:20_000] = bx[2][k] + np.float32(3e-3), bx[3][k] + np.float32(3e-3)
+    qs[0][20_000:20_500] = np.nan
+    qs[2][20_500:21_000], qs[3][20_500:21_000] = qs[0][20_500:21_000], qs[1][20_500:21_000]
+    d, rt, q = setup(tuple(bx), ids, qs, 16, 1)
+    ct.set_option("containment", contain)
+    ct.Prof.enable(True)
+    try:
+        ct.set_option("leaf_codes", 0)
+        f0, t0 = ct.bfs_frontier(rt, d, q)
+        ct.set_option("leaf_codes", 1)
+        ct.Prof.reset()
+        f1, t1 = ct.bfs_frontier(rt, d, q)
+        assert ct.Prof.get("k_leaf_codes")[1] == 1  # built once
+        f2, t2 = ct.bfs_frontier(rt, d, q)
+        n3, c3 = ct.query_count(rt, d, q, per_query=True)
+        assert ct.Prof.get("k_leaf_codes")[1] == 1  # then kept with the tree
+    finally:
+        ct.Prof.enable(False)
+        ct.set_option("leaf_codes", -1)
+        ct.set_option("containment", -1)
+    assert np.array_equal(f0, f1) and np.array_equal(t0, t1)
+    assert np.array_equal(f0, f2) and np.array_equal(t0, t2)
+    assert n3 == f0.size and np.array_equal(c3, np.bincount(t0, minlength=nq))
+    of, ot, _ = orc.bfs(rt.to_host(), d.to_host(), qs)
+    assert np.array_equal(f0, of) and np.array_equal(t0, ot)
+    rt.free()
+    d.free()
+    q.free()
