@@ -843,6 +843,10 @@ def main():
     ap.add_argument("--window-order", type=int, default=None, choices=[-1, 0, 1],
                     help="windows of a large batch taken in Morton order of their centres: -1 "
                          "by batch size (library default), 0 input order, 1 always")
+    ap.add_argument("--leaf-codes", type=int, default=None, choices=[-1, 0, 1],
+                    help="leaf codes of the tree (8-bit cells per box in its leaf-parent's "
+                         "frame) for the leaf count: -1 built by a query of 65,536 windows or "
+                         "more (library default), 0 off, 1 always")
     ap.add_argument("--tile-xcd", type=int, default=None, choices=[0, 1],
                     help="query tiles walked in contiguous eighths, one per XCD: 0 off (library "
                          "default), 1 on")
@@ -871,6 +875,8 @@ def main():
         ct.set_option("window_order", args.window_order)
     if args.tile_xcd is not None:
         ct.set_option("tile_xcd", args.tile_xcd)
+    if args.leaf_codes is not None:
+        ct.set_option("leaf_codes", args.leaf_codes)
     args.boxes_set = args.boxes is not None
     args.windows_set = args.windows is not None
     if args.mode == "plain" and args.fanout is None:
@@ -976,7 +982,7 @@ def main():
                  "k_fallback", "k_level",
                  "k_safe", "k_stage4", "k_win_keys", "k_pair_pack", "k_onesweep_pairs",
                  "k_pair_unpack", "k_win_stage", "k_win_offsets", "k_run_begin", "k_expand", "k_expand_count", "k_expand_write",
-                 "k_leaf_count", "k_tile_scan", "k_leaf_fused",
+                 "k_leaf_codes", "k_leaf_count", "k_tile_scan", "k_leaf_fused",
                  "k_leaf_write", "k_write_pieces"):
         tot, cnt = ct.Prof.get(name)
         if cnt:
@@ -1061,14 +1067,24 @@ def main():
                      "algorithmic_bytes_per_launch": dom_bytes,
                      "avg_launch_ms": dom_ms, "launches_timed": dom_cnt},
     }
-    # the query's dominant kernel, priced at the reference traversal's leaf
-    # level: 8 B per (window, leaf-parent) entry and 16 B per box test (F per
-    # entry), over its average duration in the per-kernel table
+    # the query's dominant kernel over its average duration in the per-kernel
+    # table, priced at what it moves: per (window, leaf-parent) entry of the
+    # run's leaf level the entry (8 B), the record it writes (16 B) and, with
+    # the tree's leaf codes, the leaf-parent's frame (16 B) and 4 B of cells
+    # per child slot (next power of two >= F), else 16 B per box test (F per
+    # entry); also priced at the reference traversal's leaf level (8 B per
+    # entry, 16 B per box test, containment off), the survey's figure
     lc = kernels.get("k_leaf_count")
+    codes_on = kernels.get("k_leaf_codes") is not None or (
+        args.leaf_codes == 1 or (args.leaf_codes in (None, -1) and nq >= 65536))
     if lc:
-        leaf_entries = ref_stats["entries"][-1]
-        lbytes = 8 * leaf_entries + 16 * F * leaf_entries
+        run_entries = run_stats["entries"][-1]
+        g = 1 << max(F - 1, 1).bit_length()
+        coded = codes_on and F <= 64 and g >= 4
+        lbytes = run_entries * (8 + 16 + (16 + 4 * g if coded else 16 * F))
         lgbs = lbytes / (lc["avg_ms"] * 1e-3) / 1e9
+        ref_entries = ref_stats["entries"][-1]
+        rbytes = 8 * ref_entries + 16 * F * ref_entries
         ltraffic = None
         if os.path.exists(os.path.join(ROOT, "profiles", "traffic.json")):
             with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
@@ -1079,6 +1095,9 @@ def main():
                                  "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                  "frac": lgbs / HBM_PEAK_GBS, "traffic": ltraffic,
                                  "algorithmic_bytes_per_launch": int(lbytes),
+                                 "leaf_codes": coded,
+                                 "reference_priced_bytes": int(rbytes),
+                                 "reference_priced_GBs": rbytes / (lc["avg_ms"] * 1e-3) / 1e9,
                                  "avg_launch_ms": lc["avg_ms"]}
     join_boxes = None
     join_r = 3e-4

@@ -204,6 +204,12 @@ const void* tree_state_codes(const curtree_rtree_f32* rt, const curtree_bbox_f32
   return e->codes;
 }
 
+void tree_state_free_spare_codes() {
+  std::lock_guard<std::mutex> g(g_state_mu);
+  for (auto& sp : g_spare_codes) (void)hipFree(sp.first);
+  g_spare_codes.clear();
+}
+
 void tree_state_forget(const void* const* arrays, int n) {
   std::lock_guard<std::mutex> g(g_state_mu);
   forget_locked(arrays, n);
@@ -635,6 +641,7 @@ CURTREE_API int curtree_release_workspace(void) {
   c.qe.release();
   c.batch.ws.release();
   c.batch.qe.release();
+  tree_state_free_spare_codes();
   return CURTREE_OK;
 }
 

@@ -67,6 +67,8 @@ const uint32_t* tree_state_find(const curtree_rtree_f32* rt, const curtree_bbox_
 // count then tests the boxes directly).
 const void* tree_state_codes(const curtree_rtree_f32* rt, const curtree_bbox_f32* box,
                              int64_t bytes, const std::function<int(void*)>* build);
+// Frees the leaf-code buffers of forgotten trees (the device must be idle).
+void tree_state_free_spare_codes();
 // Forget every entry that names one of these arrays (null entries ignored).
 void tree_state_forget(const void* const* arrays, int n);
 // Forget every entry with an array that overlaps [p, p + bytes).
