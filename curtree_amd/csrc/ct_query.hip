@@ -1785,29 +1785,61 @@ int reserve_f4(float4** dst, int64_t* cap, int64_t n) {
   return CURTREE_OK;
 }
 
-// Leaf codes of a bulk-load tree: thread i codes box i in the frame of its
+// Frame of a leaf-parent with MBR (a1, b1, a2, b2).  A frame needs a finite
+// low corner and a finite, positive extent on both axes with finite scales;
+// otherwise its scales are 0 and the leaf count tests that node's boxes
+// directly.
+__device__ __forceinline__ float4 leaf_frame(float a1, float b1, float a2, float b2) {
+  const float ex = a2 - a1, ey = b2 - b1;
+  if (isfinite(a1) && isfinite(b1) && isfinite(ex) && isfinite(ey) && ex > 0.f && ey > 0.f) {
+    const float sx = kCells / ex, sy = kCells / ey;
+    if (isfinite(sx) && isfinite(sy)) return make_float4(a1, b1, sx, sy);
+  }
+  return make_float4(a1, b1, 0.f, 0.f);
+}
+
+// Leaf codes of a bulk-load tree: box i is coded in the frame of its
 // leaf-parent (node ls + i / F) into slot (i / F) * G + i % F (G >= F slots per
 // leaf-parent, so that four consecutive slots are one aligned 16-byte load);
-// the first box of each leaf-parent also writes the frame.  A frame needs a
-// finite low corner and a finite, positive extent on both axes with a finite
-// scale; otherwise its scales are 0 and the leaf count tests that node's
-// boxes directly.
-__global__ __launch_bounds__(256) void k_leaf_codes(int64_t nbox, int F, int G, int64_t ls,
+// the first box of each leaf-parent also writes the frame.  vec (F == G, a
+// multiple of 4, box arrays 16-byte aligned): a thread codes four
+// consecutive boxes, read with 16-byte loads and written as one; otherwise
+// one box per thread.
+__global__ __launch_bounds__(256) void k_leaf_codes(int64_t nbox, int F, int G, bool vec,
+                                                    int64_t ls,
                                                     const float* __restrict__ nx1,
                                                     const float* __restrict__ ny1,
                                                     const float* __restrict__ nx2,
                                                     const float* __restrict__ ny2, QBoxes rb,
                                                     float4* __restrict__ frame,
                                                     uint32_t* __restrict__ code) {
-  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < nbox; i += (int64_t)gridDim.x * 256) {
-    const int64_t lp = i / F;
-    const float a1 = nx1[ls + lp], b1 = ny1[ls + lp], a2 = nx2[ls + lp], b2 = ny2[ls + lp];
-    const float ex = a2 - a1, ey = b2 - b1;
-    float4 fr = make_float4(a1, b1, 0.f, 0.f);
-    if (isfinite(a1) && isfinite(b1) && isfinite(ex) && isfinite(ey) && ex > 0.f && ey > 0.f) {
-      const float sx = kCells / ex, sy = kCells / ey;
-      if (isfinite(sx) && isfinite(sy)) fr = make_float4(a1, b1, sx, sy);
+  const int64_t step = (int64_t)gridDim.x * 256;
+  if (vec) {
+    // whole groups of four (a leaf-parent holds F = G boxes, so a group never
+    // straddles two); the last nbox % 4 boxes below
+    const int64_t n4 = nbox >> 2;
+    for (int64_t j = blockIdx.x * 256ll + threadIdx.x; j < n4; j += step) {
+      const int64_t i = 4 * j, lp = i / F;
+      const float4 fr = leaf_frame(nx1[ls + lp], ny1[ls + lp], nx2[ls + lp], ny2[ls + lp]);
+      if (i == lp * F) frame[lp] = fr;
+      const float4 x1 = ((const float4*)rb.x0)[j], y1 = ((const float4*)rb.y0)[j];
+      const float4 x2 = ((const float4*)rb.x1)[j], y2 = ((const float4*)rb.y1)[j];
+      ((uint4*)code)[j] = make_uint4(box_cells(x1.x, y1.x, x2.x, y2.x, fr),
+                                     box_cells(x1.y, y1.y, x2.y, y2.y, fr),
+                                     box_cells(x1.z, y1.z, x2.z, y2.z, fr),
+                                     box_cells(x1.w, y1.w, x2.w, y2.w, fr));
     }
+    for (int64_t i = 4 * n4 + blockIdx.x * 256ll + threadIdx.x; i < nbox; i += step) {
+      const int64_t lp = i / F;
+      const float4 fr = leaf_frame(nx1[ls + lp], ny1[ls + lp], nx2[ls + lp], ny2[ls + lp]);
+      if (i == lp * F) frame[lp] = fr;
+      code[i] = box_cells(rb.x0[i], rb.y0[i], rb.x1[i], rb.y1[i], fr);
+    }
+    return;
+  }
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < nbox; i += step) {
+    const int64_t lp = i / F;
+    const float4 fr = leaf_frame(nx1[ls + lp], ny1[ls + lp], nx2[ls + lp], ny2[ls + lp]);
     if (i == lp * F) frame[lp] = fr;
     code[lp * G + (i - lp * F)] = box_cells(rb.x0[i], rb.y0[i], rb.x1[i], rb.y1[i], fr);
   }
@@ -1821,8 +1853,13 @@ void enqueue_leaf_codes(const curtree_rtree_f32* rt, const curtree_bbox_f32* rbo
                         int64_t nlp, int G, void* buf, hipStream_t s) {
   const int64_t n = rbox->sz;
   if (n <= 0) return;
-  const unsigned grid = (unsigned)std::min<int64_t>((n + 255) / 256, 16384);
-  hipLaunchKernelGGL(k_leaf_codes, dim3(grid), dim3(256), 0, s, n, rt->fanout, G, ls, rt->xmin,
+  // (16-byte loads of the boxes need 16-byte aligned arrays: else one box per thread)
+  const bool al = ((uintptr_t)rbox->xmin | (uintptr_t)rbox->ymin | (uintptr_t)rbox->xmax |
+                   (uintptr_t)rbox->ymax) % 16 == 0;
+  const bool vec = al && rt->fanout == G && (G & 3) == 0;
+  const unsigned grid = (unsigned)std::min<int64_t>((n / (vec ? 4 : 1) + 255) / 256 + 1, 16384);
+  hipLaunchKernelGGL(k_leaf_codes, dim3(grid), dim3(256), 0, s, n, rt->fanout, G, vec, ls,
+                     rt->xmin,
                      rt->ymin, rt->xmax, rt->ymax,
                      QBoxes{rbox->xmin, rbox->ymin, rbox->xmax, rbox->ymax, nullptr},
                      (float4*)buf, (uint32_t*)((float4*)buf + nlp));
@@ -2324,6 +2361,36 @@ int read_ctrl(QEngine& qe, hipStream_t s) {
   return CURTREE_OK;
 }
 
+// The leaf codes of a kept tree (nlp leaf-parents from node ls, G slots
+// each): the ready buffer, or null when the tree is not kept, or none are
+// ready and make is false.  make: built now on stream s (a synchronisation).
+static const void* leaf_codes_for(Prof& pf, const curtree_rtree_f32* rt,
+                                  const curtree_bbox_f32* rbox, int G, int64_t nlp, int64_t ls,
+                                  bool make, hipStream_t s) {
+  const std::function<int(void*)> build = [&](void* buf) {
+    pf.mark(s, "k_leaf_codes");
+    enqueue_leaf_codes(rt, rbox, ls, nlp, G, buf, s);
+    pf.mark(s, nullptr);
+    CT_HIP(hipGetLastError());
+    CT_HIP(hipStreamSynchronize(s));
+    return CURTREE_OK;
+  };
+  return tree_state_codes(rt, rbox, leaf_codes_bytes(nlp, G), make ? &build : nullptr);
+}
+
+int ensure_leaf_codes(const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox, hipStream_t s) {
+  const int64_t N = rbox->sz;
+  const int F = rt->fanout;
+  if (N <= 0 || F < 2 || F > 64 || next_pow2_group(F) < 4 || qtuning().codes == 0)
+    return CURTREE_OK;
+  LevelPlan lp;
+  level_plan(N, F, &lp);
+  if (lp.r_sz != rt->sz) return CURTREE_OK;
+  Prof pf;  // (never enabled)
+  (void)leaf_codes_for(pf, rt, rbox, next_pow2_group(F), lp.n[0], lp.r_sz - lp.n[0], true, s);
+  return CURTREE_OK;
+}
+
 int prepare_query(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox,
                   const curtree_bbox_f32* qbox, int contain, int64_t nq_hint, QRunArgs* a,
                   hipStream_t s) {
@@ -2415,16 +2482,8 @@ int prepare_query(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curt
         cs = hipStreamCaptureStatusActive;
       }
       const bool make = cs == hipStreamCaptureStatusNone && (cz == 1 || nq_hint >= kCodesMinWindows);
-      const int64_t nlp = lp.n[0], ls = a->lv.lstart[0];
-      const std::function<int(void*)> build = [&](void* buf) {
-        pf.mark(s, "k_leaf_codes");
-        enqueue_leaf_codes(rt, rbox, ls, nlp, a->G, buf, s);
-        pf.mark(s, nullptr);
-        CT_HIP(hipGetLastError());
-        CT_HIP(hipStreamSynchronize(s));
-        return CURTREE_OK;
-      };
-      const void* cb = tree_state_codes(rt, rbox, leaf_codes_bytes(nlp, a->G), make ? &build : nullptr);
+      const int64_t nlp = lp.n[0];
+      const void* cb = leaf_codes_for(pf, rt, rbox, a->G, nlp, a->lv.lstart[0], make, s);
       if (cb) {
         a->tr.lframe = (const float4*)cb;
         a->tr.lcode = (const uint32_t*)((const float4*)cb + nlp);

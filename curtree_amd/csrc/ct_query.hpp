@@ -226,5 +226,9 @@ int read_ctrl(QEngine& qe, hipStream_t s);
 int64_t leaf_codes_bytes(int64_t nleafparents, int G);
 void enqueue_leaf_codes(const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox, int64_t ls,
                         int64_t nlp, int G, void* buf, hipStream_t s);
+// Builds the leaf codes of a tree built by this library (kept), if it has
+// none yet and they apply (fanout 3..64, option leaf_codes not 0); a tree
+// that is not kept is left as it is.  Synchronises s.
+int ensure_leaf_codes(const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox, hipStream_t s);
 
 }  // namespace ct

@@ -589,6 +589,9 @@ CURTREE_API int curtree_server_add_tree(curtree_server* s, const curtree_rtree_f
     if (!rc && qtuning().staging == 1) rc = stage_boxes(&t.box4, &cb, rb, rbox->sz, nullptr);
     if (!rc && qtuning().staging == 1) rc = stage_boxes(&t.node4, &cn, nb, rt->sz, nullptr);
     if (!rc) rc = check_layout(&t.canon, rt, rbox->sz, nullptr);
+    // the leaf codes of a tree built by this library, now rather than in
+    // the first large batch (a no-op for other trees)
+    if (!rc) rc = ensure_leaf_codes(rt, rbox, nullptr);
     if (!rc && hipDeviceSynchronize() != hipSuccess)
       rc = set_error(CURTREE_EHIP, "server_add_tree: staging failed");
     t.safe = qe.safe;
