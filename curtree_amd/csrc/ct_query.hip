@@ -968,6 +968,191 @@ __global__ __launch_bounds__(kQThreads) void k_expand_count(int slot, QTree tr, 
     atomicAdd((unsigned long long*)&c->count[slot + 1], (unsigned long long)block_total);
 }
 
+// Containment of box cells b in window cells w (window low side <= box low
+// side, box high side <= window high side): 1 proven, 0 disproven, -1 open.
+// Only for a window without NaN (a NaN window contains nothing; its cells
+// could prove otherwise); a NaN in the box takes cell 0 (low side) or 255
+// (high side) and is never proven contained.
+__device__ __forceinline__ int cells_contain(uint32_t w, uint32_t b) {
+  const uint32_t wx1 = w & 255u, wy1 = (w >> 8) & 255u, wx2 = (w >> 16) & 255u, wy2 = w >> 24;
+  const uint32_t bx1 = b & 255u, by1 = (b >> 8) & 255u, bx2 = (b >> 16) & 255u, by2 = b >> 24;
+  if (wx1 > bx1 || bx2 > wx2 || wy1 > by1 || by2 > wy2) return 0;
+  if (wx1 < bx1 && bx2 < wx2 && wy1 < by1 && by2 < wy2) return 1;
+  return -1;
+}
+
+// Internal level `slot` over the node codes (tr.nframe / tr.ncode, G >= 4):
+// the same tiles and QExp records as k_expand_count, but G/4 lanes per entry,
+// each lane the cells of four children (one 16-byte load), so a tile is one
+// round of loads.  A child whose cells leave the intersection open reads its
+// MBR at the end of the tile, and so does a child that intersects but whose
+// containment (the shortcut) the cells leave open.
+template <int G, int STEPS>
+__global__ __launch_bounds__(kQThreads)
+#if CT_CODED_WAVES > 0
+__attribute__((amdgpu_waves_per_eu(CT_CODED_WAVES)))
+#endif
+void k_expand_count_coded(int slot, QTree tr, QBoxes qb, QCtrl* c,
+                          const uint2* __restrict__ fin, QExp* __restrict__ est,
+                          uint64_t* __restrict__ ttot, int64_t r_sz, bool xcd) {
+  using S = QShape<G, STEPS>;
+  constexpr int P = S::kPerTile;
+  constexpr int GC = G / 4;             // lanes per entry
+  constexpr int EC = kWave / GC;        // entries per wave-step
+  constexpr int SC = S::kPerWave / EC;  // wave-steps per tile
+  static_assert(G >= 4 && SC * EC == S::kPerWave, "whole wave-steps");
+  constexpr uint32_t kNib = GC >= 32 ? 0xFFFFFFFFu : ((1u << GC) - 1u);
+  __shared__ uint64_t s_red[kQWaves];
+  __shared__ QExp s_est[P];
+  if (c->overflow) return;
+  const int64_t n_in = (int64_t)c->count[slot];
+  const int64_t q0 = c->q0;
+  const int64_t ntiles = (n_in + P - 1) / P;
+  const int t = threadIdx.x, w = t >> 6, lane = lane_id();
+  if (ntiles == 0) return;
+  const int sub = lane / GC, cg = lane % GC;
+  const int lead = sub * GC;
+  const bool canon = tr.canon && *tr.canon;
+  // (a tree with node codes is a kept tree: its shortcut needs no per-node flags)
+  const bool shortcut = shortcut_on(tr, canon) && tr.safe == nullptr;
+  const int lvl = c->lv.nl - 1 - slot;
+  const int64_t ls = c->lv.lstart[lvl];
+  const int64_t cs = lvl > 0 ? c->lv.lstart[lvl - 1] : 0;
+  const int64_t cc = lvl > 0 ? c->lv.lcount[lvl - 1] : 0;
+  uint64_t block_total = 0;
+  const TileWalk tw = tile_walk(ntiles, xcd);
+  for (int64_t tile = tw.first; tile < tw.end; tile += tw.step) {
+    __syncthreads();
+    uint64_t wsum = 0;
+    const int64_t tbase = tile * P;
+    uint2 ent[SC];
+#pragma unroll
+    for (int k = 0; k < SC; k++) {
+      const int64_t e = tbase + w * S::kPerWave + k * EC + sub;
+      ent[k] = fin ? fin[e < n_in ? e : n_in - 1]
+                   : make_uint2((uint32_t)(q0 + (e < n_in ? e : n_in - 1)), 0u);
+    }
+    int p[SC], L[SC];
+    bool coded[SC];
+#pragma unroll
+    for (int k = 0; k < SC; k++) {
+      const uint32_t node = ent[k].y & ~kFull;
+      if (canon) {
+        const int64_t f = ((int64_t)node - ls) * tr.F;
+        p[k] = (int)(cs + f);
+        L[k] = (int)(cc - f < tr.F ? cc - f : tr.F);
+      } else {
+        p[k] = tr.pos[node];
+        L[k] = tr.len[node];
+      }
+      coded[k] = canon && (ent[k].y & kFull) == 0;
+    }
+    float4 wq[SC], fr[SC];
+    uint4 cd[SC];
+    uint32_t test[SC];  // bit j: child slot 4*cg + j exists (children must be nodes)
+#pragma unroll
+    for (int k = 0; k < SC; k++) {
+      test[k] = 0;
+      const bool full = (ent[k].y & kFull) != 0;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int sl = 4 * cg + j;
+        if (!full && p[k] >= 0 && sl < L[k] && (int64_t)p[k] + L[k] <= r_sz) test[k] |= 1u << j;
+      }
+      wq[k] = ld_box(qb, ent[k].x);
+      const uint32_t node = ent[k].y & ~kFull;
+      const bool ld = coded[k] && test[k] != 0;
+      fr[k] = ld ? tr.nframe[node] : make_float4(0.f, 0.f, 0.f, 0.f);
+      cd[k] = ld ? ((const uint4*)tr.ncode)[(int64_t)node * GC + cg] : make_uint4(0, 0, 0, 0);
+    }
+    uint32_t hit[SC], cont[SC], open[SC];
+#pragma unroll
+    for (int k = 0; k < SC; k++) {
+      const int64_t e = tbase + w * S::kPerWave + k * EC + sub;
+      hit[k] = cont[k] = open[k] = 0;
+      if (e < n_in && test[k]) {
+        if (fr[k].z > 0.f) {
+          const uint32_t wc = box_cells(wq[k].x, wq[k].y, wq[k].z, wq[k].w, fr[k]);
+          const bool wnan = wq[k].x != wq[k].x || wq[k].y != wq[k].y || wq[k].z != wq[k].z ||
+                            wq[k].w != wq[k].w;
+          const uint32_t b4[4] = {cd[k].x, cd[k].y, cd[k].z, cd[k].w};
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            if (!((test[k] >> j) & 1u)) continue;
+            const int r = cells_intersect(wc, b4[j]);
+            if (r > 0) {
+              hit[k] |= 1u << j;
+              const int rc = shortcut && !wnan ? cells_contain(wc, b4[j]) : 0;
+              if (rc > 0) cont[k] |= 1u << j;
+              else if (rc < 0) open[k] |= 1u << j;  // (the MBR settles the containment)
+            } else if (r < 0) {
+              open[k] |= 1u << j;
+            }
+          }
+        } else {
+          open[k] = test[k];
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < SC; k++) {
+      if (open[k]) {
+        float4 mb[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          if ((open[k] >> j) & 1u) mb[j] = ld_box(tr.mbr, (uint32_t)(p[k] + 4 * cg + j));
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          if (((open[k] >> j) & 1u) &&
+              intersect(wq[k].x, wq[k].y, wq[k].z, wq[k].w, mb[j].x, mb[j].y, mb[j].z, mb[j].w)) {
+            hit[k] |= 1u << j;
+            if (shortcut &&
+                contains(wq[k].x, wq[k].y, wq[k].z, wq[k].w, mb[j].x, mb[j].y, mb[j].z, mb[j].w))
+              cont[k] |= 1u << j;
+          }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < SC; k++) {
+      const int64_t e = tbase + w * S::kPerWave + k * EC + sub;
+      const bool valid = e < n_in;
+      const bool full = (ent[k].y & kFull) != 0;
+      uint64_t gm = 0, fm = 0;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        gm |= spread4((uint32_t)(ballot((hit[k] >> j) & 1u) >> lead) & kNib) << j;
+        fm |= spread4((uint32_t)(ballot((cont[k] >> j) & 1u) >> lead) & kNib) << j;
+      }
+      if (full) gm = fm = 1;  // a contained entry is its own single child
+      if (cg == 0 && valid) {
+        QExp x;
+        x.hit = gm;
+        x.full = fm;
+        x.q = ent[k].x;
+        x.first = full ? (ent[k].y & ~kFull) : (uint32_t)p[k];
+        s_est[e - tbase] = x;
+        wsum += (uint64_t)popc64(gm);
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) wsum += __shfl_xor(wsum, o);
+    if (lane == 0) s_red[w] = wsum;
+    __syncthreads();
+    for (int i = t; i < P; i += kQThreads)
+      if (tbase + i < n_in) est[tbase + i] = s_est[i];
+    if (t == 0) {
+      uint64_t tot = 0;
+#pragma unroll
+      for (int i = 0; i < kQWaves; i++) tot += s_red[i];
+      ttot[tile] = tot;
+      block_total += tot;
+    }
+  }
+  if (t == 0 && block_total)
+    atomicAdd((unsigned long long*)&c->count[slot + 1], (unsigned long long)block_total);
+}
+
+
 template <int G, int STEPS>
 __global__ __launch_bounds__(kQThreads) void k_expand_write(int slot, QCtrl* c,
                                                             const QExp* __restrict__ est,
@@ -1845,14 +2030,45 @@ __global__ __launch_bounds__(256) void k_leaf_codes(int64_t nbox, int F, int G, 
   }
 }
 
-int64_t leaf_codes_bytes(int64_t nleafparents, int G) {
-  return nleafparents * ((int64_t)sizeof(float4) + 4 * (int64_t)G);
+// Node codes of one internal level: child j (node cs + j) is coded in the
+// frame of its parent (node ls + j / F, its MBR) into slot parent * G + j % F;
+// the first child of each parent writes the parent's frame at the parent's
+// node index.
+__global__ __launch_bounds__(256) void k_node_codes(int64_t cc, int F, int G, int64_t ls,
+                                                    int64_t cs, QBoxes nb,
+                                                    float4* __restrict__ frame,
+                                                    uint32_t* __restrict__ code) {
+  for (int64_t j = blockIdx.x * 256ll + threadIdx.x; j < cc; j += (int64_t)gridDim.x * 256) {
+    const int64_t par = ls + j / F, sl = j % F;
+    const float4 fr = leaf_frame(nb.x0[par], nb.y0[par], nb.x1[par], nb.y1[par]);
+    if (sl == 0) frame[par] = fr;
+    const int64_t ch = cs + j;
+    code[par * G + sl] = box_cells(nb.x0[ch], nb.y0[ch], nb.x1[ch], nb.y1[ch], fr);
+  }
 }
 
-void enqueue_leaf_codes(const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox, int64_t ls,
-                        int64_t nlp, int G, void* buf, hipStream_t s) {
+int64_t leaf_codes_bytes(int64_t nleafparents, int64_t nparents, int G) {
+  return (nleafparents + nparents) * ((int64_t)sizeof(float4) + 4 * (int64_t)G);
+}
+
+void enqueue_leaf_codes(const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox,
+                        const QLevels& lv, int G, void* buf, hipStream_t s) {
   const int64_t n = rbox->sz;
   if (n <= 0) return;
+  const int64_t nlp = lv.lcount[0], ls = lv.lstart[0];
+  // the internal levels' node codes, after the leaf codes
+  {
+    const int64_t np = ls;  // nodes above the leaf-parent level: 0 .. ls-1
+    float4* nf = (float4*)((char*)buf + nlp * ((int64_t)sizeof(float4) + 4 * (int64_t)G));
+    uint32_t* nc = (uint32_t*)(nf + np);
+    const QBoxes nb{rt->xmin, rt->ymin, rt->xmax, rt->ymax, nullptr};
+    for (int l = lv.nl - 1; l >= 1; l--) {
+      const int64_t cc = lv.lcount[l - 1];
+      const unsigned grid = (unsigned)std::min<int64_t>((cc + 255) / 256, 8192);
+      hipLaunchKernelGGL(k_node_codes, dim3(grid), dim3(256), 0, s, cc, rt->fanout, G,
+                         lv.lstart[l], lv.lstart[l - 1], nb, nf, nc);
+    }
+  }
   // (16-byte loads of the boxes need 16-byte aligned arrays: else one box per thread)
   const bool al = ((uintptr_t)rbox->xmin | (uintptr_t)rbox->ymin | (uintptr_t)rbox->xmax |
                    (uintptr_t)rbox->ymax) % 16 == 0;
@@ -1969,6 +2185,35 @@ static void launch_leaf_count(const QRunArgs& a, int leaf, const uint2* cur, QCt
 }
 
 template <int G, int STEPS>
+static unsigned expand_count_coded_grid() {
+  static const unsigned g = resident_grid((const void*)k_expand_count_coded<G, STEPS>);
+  return g;
+}
+
+template <int G, int STEPS>
+static unsigned expand_count_grid();
+
+// An internal level's count: over the node codes when the run has them (G >= 4,
+// tiles of up to 8 wave-steps), else the MBR tests.
+template <int G, int STEPS>
+static void launch_expand_count(const QRunArgs& a, int l, const uint2* cur, QEngine& qe, bool xcd,
+                                hipStream_t s) {
+  if constexpr (G >= 4 && STEPS <= 8) {
+    if (a.tr.ncode) {
+      hipLaunchKernelGGL((k_expand_count_coded<G, STEPS>),
+                         dim3(std::min<unsigned>(a.grid, expand_count_coded_grid<G, STEPS>())),
+                         dim3(kQThreads), 0, s, l, a.tr, a.qb, qe.ctrl, cur, (QExp*)qe.est,
+                         qe.ttot, a.r_sz, xcd);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_expand_count<G, STEPS>),
+                     dim3(std::min<unsigned>(a.grid, expand_count_grid<G, STEPS>())),
+                     dim3(kQThreads), 0, s, l, a.tr, a.qb, qe.ctrl, cur, (QExp*)qe.est, qe.ttot,
+                     a.r_sz, xcd);
+}
+
+template <int G, int STEPS>
 static unsigned expand_count_grid() {
   static const unsigned g = resident_grid((const void*)k_expand_count<G, STEPS>);
   return g;
@@ -1989,6 +2234,7 @@ static unsigned leaf_write_grid() {
 template <int G, int STEPS>
 static void warm_coded() {
   if constexpr (G >= 4 && STEPS <= 8) {
+    (void)expand_count_coded_grid<G, STEPS>();
     (void)leaf_count_coded_grid<G, STEPS, true, false>();
     (void)leaf_count_coded_grid<G, STEPS, true, true>();
     (void)leaf_count_coded_grid<G, STEPS, false, false>();
@@ -2065,10 +2311,7 @@ void enqueue_run_g(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s) {
     uint2* out = (uint2*)qe.front[which];
     if (split) {
       pf.mark(s, "k_expand_count");
-      hipLaunchKernelGGL((k_expand_count<G, STEPS>),
-                         dim3(std::min<unsigned>(a.grid, expand_count_grid<G, STEPS>())),
-                         dim3(kQThreads), 0, s, l,
-                         a.tr, a.qb, qe.ctrl, cur, (QExp*)qe.est, qe.ttot, a.r_sz, xcd);
+      launch_expand_count<G, STEPS>(a, l, cur, qe, xcd, s);
       pf.mark(s, "k_tile_scan");
       hipLaunchKernelGGL(k_tile_scan, dim3(64), dim3(kQThreads), 0, s, a.nl + 1 + l, l,
                          S::kPerTile, qe.ctrl, qe.ttot, qe.status);
@@ -2365,17 +2608,32 @@ int read_ctrl(QEngine& qe, hipStream_t s) {
 // each): the ready buffer, or null when the tree is not kept, or none are
 // ready and make is false.  make: built now on stream s (a synchronisation).
 static const void* leaf_codes_for(Prof& pf, const curtree_rtree_f32* rt,
-                                  const curtree_bbox_f32* rbox, int G, int64_t nlp, int64_t ls,
+                                  const curtree_bbox_f32* rbox, int G, const QLevels& lv,
                                   bool make, hipStream_t s) {
   const std::function<int(void*)> build = [&](void* buf) {
     pf.mark(s, "k_leaf_codes");
-    enqueue_leaf_codes(rt, rbox, ls, nlp, G, buf, s);
+    enqueue_leaf_codes(rt, rbox, lv, G, buf, s);
     pf.mark(s, nullptr);
     CT_HIP(hipGetLastError());
     CT_HIP(hipStreamSynchronize(s));
     return CURTREE_OK;
   };
-  return tree_state_codes(rt, rbox, leaf_codes_bytes(nlp, G), make ? &build : nullptr);
+  return tree_state_codes(rt, rbox, leaf_codes_bytes(lv.lcount[0], lv.lstart[0], G),
+                          make ? &build : nullptr);
+}
+
+// The level table of a tree over nbox boxes (root level last in lstart order:
+// level l starts at lstart[l], level 0 being the leaf-parents).
+static void level_table(int F, const LevelPlan& lp, QLevels* lv) {
+  lv->nl = lp.nl;
+  int64_t start = lp.r_sz, span = 1;
+  for (int l = 0; l < lp.nl; l++) {
+    start -= lp.n[l];
+    span = span > ((int64_t)1 << 40) / F ? ((int64_t)1 << 40) : span * F;
+    lv->lstart[l] = start;
+    lv->lcount[l] = lp.n[l];
+    lv->span[l] = span;
+  }
 }
 
 int ensure_leaf_codes(const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox, hipStream_t s) {
@@ -2386,8 +2644,10 @@ int ensure_leaf_codes(const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox,
   LevelPlan lp;
   level_plan(N, F, &lp);
   if (lp.r_sz != rt->sz) return CURTREE_OK;
+  QLevels lv{};
+  level_table(F, lp, &lv);
   Prof pf;  // (never enabled)
-  (void)leaf_codes_for(pf, rt, rbox, next_pow2_group(F), lp.n[0], lp.r_sz - lp.n[0], true, s);
+  (void)leaf_codes_for(pf, rt, rbox, next_pow2_group(F), lv, true, s);
   return CURTREE_OK;
 }
 
@@ -2444,17 +2704,7 @@ int prepare_query(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curt
   a->rid = rbox->id;
   a->r_sz = lp.r_sz;
   a->nl = lp.nl;
-  a->lv.nl = lp.nl;
-  {
-    int64_t start = lp.r_sz, span = 1;
-    for (int l = 0; l < lp.nl; l++) {
-      start -= lp.n[l];
-      span = span > ((int64_t)1 << 40) / F ? ((int64_t)1 << 40) : span * F;
-      a->lv.lstart[l] = start;
-      a->lv.lcount[l] = lp.n[l];
-      a->lv.span[l] = span;
-    }
-  }
+  level_table(F, lp, &a->lv);
   // the flags cost one pass over the boxes; worth it for larger batches
   // (a fixed engine holds no tree-sized buffers: no flags, no staging)
   const bool use = !qe.fixed && !a->generic && (contain == 1 || (contain < 0 && nq_hint >= 256));
@@ -2482,11 +2732,19 @@ int prepare_query(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curt
         cs = hipStreamCaptureStatusActive;
       }
       const bool make = cs == hipStreamCaptureStatusNone && (cz == 1 || nq_hint >= kCodesMinWindows);
-      const int64_t nlp = lp.n[0];
-      const void* cb = leaf_codes_for(pf, rt, rbox, a->G, nlp, a->lv.lstart[0], make, s);
+      const int64_t nlp = lp.n[0], np = a->lv.lstart[0];
+      const void* cb = leaf_codes_for(pf, rt, rbox, a->G, a->lv, make, s);
       if (cb) {
         a->tr.lframe = (const float4*)cb;
         a->tr.lcode = (const uint32_t*)((const float4*)cb + nlp);
+#ifndef CT_NODE_CODES
+#define CT_NODE_CODES 1  // internal levels over the node codes
+#endif
+        if (CT_NODE_CODES) {
+          a->tr.nframe = (const float4*)((const char*)cb +
+                                         nlp * ((int64_t)sizeof(float4) + 4 * (int64_t)a->G));
+          a->tr.ncode = (const uint32_t*)(a->tr.nframe + np);
+        }
       }
     }
   }

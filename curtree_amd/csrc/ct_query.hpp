@@ -59,6 +59,10 @@ struct QTree {
   // 8-bit cells of that frame per box (see box_cells); bulk-load layout only.
   const float4* lframe = nullptr;
   const uint32_t* lcode = nullptr;
+  // Node codes (with the leaf codes): per internal node (0 .. lstart[0]-1) a
+  // frame and G slots of its children's cells.
+  const float4* nframe = nullptr;
+  const uint32_t* ncode = nullptr;
   int F = 0;
 };
 
@@ -220,12 +224,13 @@ void enqueue_run(QEngine& qe, Prof& pf, const QRunArgs& a, hipStream_t s);
 int64_t pieces_needed(const QEngine& qe, const QRunArgs& a);
 void enqueue_fill_counts(int64_t* counts, int64_t n, hipStream_t s);
 int read_ctrl(QEngine& qe, hipStream_t s);
-// Leaf codes of a bulk-load tree (nlp leaf-parents from node ls, G >= F
-// slots each): the frames of the leaf-parents, then G u32 cells per
-// leaf-parent, in one buffer of leaf_codes_bytes.
-int64_t leaf_codes_bytes(int64_t nleafparents, int G);
-void enqueue_leaf_codes(const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox, int64_t ls,
-                        int64_t nlp, int G, void* buf, hipStream_t s);
+// Codes of a bulk-load tree (levels lv, G >= F slots per parent): the frames
+// of the leaf-parents, G u32 cells per leaf-parent, then the frames of the
+// internal nodes (0 .. lstart[0]-1) and G cells of their children each, in
+// one buffer of leaf_codes_bytes(leaf-parents, internal nodes, G).
+int64_t leaf_codes_bytes(int64_t nleafparents, int64_t nparents, int G);
+void enqueue_leaf_codes(const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox,
+                        const QLevels& lv, int G, void* buf, hipStream_t s);
 // Builds the leaf codes of a tree built by this library (kept), if it has
 // none yet and they apply (fanout 3..64, option leaf_codes not 0); a tree
 // that is not kept is left as it is.  Synchronises s.
