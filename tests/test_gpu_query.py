@@ -615,3 +615,45 @@ def test_leaf_codes_same_pairs(contain):
     rt.free()
     d.free()
     q.free()
+
+
+@pytest.mark.parametrize("fanout", [3, 4, 8, 10, 16, 32, 33, 64])
+def test_leaf_codes_fanouts(fanout):
+    # every lane shape of the coded kernels (G/4 lanes per entry for G = 4 ..
+    # 64, fanouts below G with padded slots) against the direct tests and the
+    # oracle, with the containment shortcut on; codes forced for any query
+    import curtree_amd as ct
+
+    n, nq = 60_000, 3_000
+    bx = [a.copy() for a in datagen.clustered_boxes(n, seed=181, frac=0.4, sigma=0.01, wmax=2e-3)]
+    bx[0][:50] = np.nan
+    ids = np.arange(n, dtype=np.int32)
+    qs = [a.copy() for a in datagen.loguniform_windows(nq, seed=182, lo=1e-3, hi=0.2)[:4]]
+    k = np.random.default_rng(183).integers(0, n, 500)
+    qs[0][:500], qs[1][:500], qs[2][:500], qs[3][:500] = bx[2][k], bx[3][k], bx[2][k] + 0.01, bx[3][k] + 0.01
+    qs[1][500:520] = np.nan
+    d, rt, q = setup(tuple(bx), ids, qs, fanout, 1)
+    ct.set_option("containment", 1)
+    ct.set_option("expand_split", 1)  # the count / scan / write levels (the coded kernels)
+    ct.Prof.enable(True)
+    try:
+        ct.set_option("leaf_codes", 0)
+        f0, t0 = ct.bfs_frontier(rt, d, q)
+        ct.set_option("leaf_codes", 1)
+        ct.Prof.reset()
+        f1, t1 = ct.bfs_frontier(rt, d, q)
+        built = ct.Prof.get("k_leaf_codes")[1]
+        n1, c1 = ct.query_count(rt, d, q, per_query=True)
+    finally:
+        ct.Prof.enable(False)
+        ct.set_option("leaf_codes", -1)
+        ct.set_option("expand_split", -1)
+        ct.set_option("containment", -1)
+    assert built == (1 if fanout >= 3 else 0)
+    assert np.array_equal(f0, f1) and np.array_equal(t0, t1)
+    assert n1 == f0.size and np.array_equal(c1, np.bincount(t0, minlength=nq))
+    of, ot, _ = orc.bfs(rt.to_host(), d.to_host(), qs)
+    assert np.array_equal(f0, of) and np.array_equal(t0, ot)
+    rt.free()
+    d.free()
+    q.free()
