@@ -560,21 +560,13 @@ __global__ __launch_bounds__(kQThreads) void k_leaf_count(
     atomicAdd((unsigned long long*)&c->total, (unsigned long long)block_total);
 }
 
-// Bit i of x (below 2^16) to bit 4i.
-__device__ __forceinline__ uint64_t spread4(uint32_t x) {
-  uint64_t v = x;
-  v = (v | (v << 24)) & 0x000000FF000000FFull;
-  v = (v | (v << 12)) & 0x000F000F000F000Full;
-  v = (v | (v << 6)) & 0x0303030303030303ull;
-  v = (v | (v << 3)) & 0x1111111111111111ull;
-  return v;
-}
-
 // The leaf count over leaf codes (tr.lframe / tr.lcode, G >= 4): the same
 // tiles of P entries and the same outputs as k_leaf_count, but G/4 lanes per
-// entry, each lane holding the cells of four consecutive child slots (one
-// 16-byte load), so one wave-step covers four times the entries and a whole
-// tile is one round of loads (entries, then windows, frames and cells).  A
+// entry, lane g holding the cells of child slots g, g + G/4, g + 2G/4,
+// g + 3G/4 (one 16-byte load, see cell_slot), so one wave-step covers four
+// times the entries and a whole tile is one round of loads (entries, then
+// windows, frames and cells); an entry's hit mask is then four shifted
+// nibbles of four ballots.  A
 // test the cells leave open is settled at the end of the tile, in one more
 // round: the box and the window read, the float test.  A layout other than
 // the bulk-load one has no codes: every test is open (pos/len give the
@@ -656,13 +648,13 @@ void k_leaf_count_coded(
     // windows, frames and the cells of this lane's four slots
     float4 wq[SC], fr[SC];
     uint4 cd[SC];
-    uint32_t test[SC];  // bit j: slot 4*cg + j is a child box to test
+    uint32_t test[SC];  // bit j: slot j * GC + cg is a child box to test
 #pragma unroll
     for (int k = 0; k < SC; k++) {
       test[k] = 0;
 #pragma unroll
       for (int j = 0; j < 4; j++) {
-        const int sl = 4 * cg + j, bi = first[k] + sl;
+        const int sl = j * GC + cg, bi = first[k] + sl;
         if (!full[k] && sl < cnt[k] && bi >= 0 && bi < nbox) test[k] |= 1u << j;
       }
       wq[k] = ld_box(qb, q[k]);
@@ -698,7 +690,7 @@ void k_leaf_count_coded(
         float4 bx[4];
 #pragma unroll
         for (int j = 0; j < 4; j++)
-          if ((open[k] >> j) & 1u) bx[j] = ld_box(rb, first[k] + 4 * cg + j);
+          if ((open[k] >> j) & 1u) bx[j] = ld_box(rb, first[k] + j * GC + cg);
 #pragma unroll
         for (int j = 0; j < 4; j++)
           if (((open[k] >> j) & 1u) &&
@@ -706,7 +698,7 @@ void k_leaf_count_coded(
             hit[k] |= 1u << j;
       }
     }
-    // per entry: the hit mask over its slots (slot 4*cg + j from lane cg's bit j)
+    // per entry: the hit mask over its slots (slot j * GC + cg from lane cg's bit j)
 #pragma unroll
     for (int k = 0; k < SC; k++) {
       const int64_t e = tbase + w * S::kPerWave + k * EC + sub;
@@ -715,7 +707,7 @@ void k_leaf_count_coded(
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         const uint64_t m = ballot((hit[k] >> j) & 1u);
-        gm |= spread4((uint32_t)(m >> lead) & kNib) << j;
+        gm |= (uint64_t)((uint32_t)(m >> lead) & kNib) << (j * GC);
       }
       const uint64_t nh = full[k] ? (uint64_t)cnt[k] : (uint64_t)popc64(gm);
       if (cg == 0 && valid) {
@@ -983,8 +975,8 @@ __device__ __forceinline__ int cells_contain(uint32_t w, uint32_t b) {
 
 // Internal level `slot` over the node codes (tr.nframe / tr.ncode, G >= 4):
 // the same tiles and QExp records as k_expand_count, but G/4 lanes per entry,
-// each lane the cells of four children (one 16-byte load), so a tile is one
-// round of loads.  A child whose cells leave the intersection open reads its
+// each lane the cells of four children (slots g + j * G/4, one 16-byte load),
+// so a tile is one round of loads.  A child whose cells leave the intersection open reads its
 // MBR at the end of the tile, and so does a child that intersects but whose
 // containment (the shortcut) the cells leave open.
 template <int G, int STEPS>
@@ -1049,14 +1041,14 @@ void k_expand_count_coded(int slot, QTree tr, QBoxes qb, QCtrl* c,
     }
     float4 wq[SC], fr[SC];
     uint4 cd[SC];
-    uint32_t test[SC];  // bit j: child slot 4*cg + j exists (children must be nodes)
+    uint32_t test[SC];  // bit j: child slot j * GC + cg exists (children must be nodes)
 #pragma unroll
     for (int k = 0; k < SC; k++) {
       test[k] = 0;
       const bool full = (ent[k].y & kFull) != 0;
 #pragma unroll
       for (int j = 0; j < 4; j++) {
-        const int sl = 4 * cg + j;
+        const int sl = j * GC + cg;
         if (!full && p[k] >= 0 && sl < L[k] && (int64_t)p[k] + L[k] <= r_sz) test[k] |= 1u << j;
       }
       wq[k] = ld_box(qb, ent[k].x);
@@ -1100,7 +1092,7 @@ void k_expand_count_coded(int slot, QTree tr, QBoxes qb, QCtrl* c,
         float4 mb[4];
 #pragma unroll
         for (int j = 0; j < 4; j++)
-          if ((open[k] >> j) & 1u) mb[j] = ld_box(tr.mbr, (uint32_t)(p[k] + 4 * cg + j));
+          if ((open[k] >> j) & 1u) mb[j] = ld_box(tr.mbr, (uint32_t)(p[k] + j * GC + cg));
 #pragma unroll
         for (int j = 0; j < 4; j++)
           if (((open[k] >> j) & 1u) &&
@@ -1120,8 +1112,8 @@ void k_expand_count_coded(int slot, QTree tr, QBoxes qb, QCtrl* c,
       uint64_t gm = 0, fm = 0;
 #pragma unroll
       for (int j = 0; j < 4; j++) {
-        gm |= spread4((uint32_t)(ballot((hit[k] >> j) & 1u) >> lead) & kNib) << j;
-        fm |= spread4((uint32_t)(ballot((cont[k] >> j) & 1u) >> lead) & kNib) << j;
+        gm |= (uint64_t)((uint32_t)(ballot((hit[k] >> j) & 1u) >> lead) & kNib) << (j * GC);
+        fm |= (uint64_t)((uint32_t)(ballot((cont[k] >> j) & 1u) >> lead) & kNib) << (j * GC);
       }
       if (full) gm = fm = 1;  // a contained entry is its own single child
       if (cg == 0 && valid) {
@@ -1970,6 +1962,15 @@ int reserve_f4(float4** dst, int64_t* cap, int64_t n) {
   return CURTREE_OK;
 }
 
+// Where the cells of child slot sl (of G slots per parent) are stored: slots
+// are transposed in groups of four, so that the 16 bytes at 16 * g hold slots
+// g, g + G/4, g + 2G/4, g + 3G/4 (one load per lane of a coded kernel, whose
+// lane g then finds slot j * G/4 + g in component j).
+__device__ __forceinline__ int cell_slot(int sl, int G) {
+  const int gc = G >> 2;
+  return (sl % gc) * 4 + sl / gc;
+}
+
 // Frame of a leaf-parent with MBR (a1, b1, a2, b2).  A frame needs a finite
 // low corner and a finite, positive extent on both axes with finite scales;
 // otherwise its scales are 0 and the leaf count tests that node's boxes
@@ -1984,12 +1985,10 @@ __device__ __forceinline__ float4 leaf_frame(float a1, float b1, float a2, float
 }
 
 // Leaf codes of a bulk-load tree: box i is coded in the frame of its
-// leaf-parent (node ls + i / F) into slot (i / F) * G + i % F (G >= F slots per
-// leaf-parent, so that four consecutive slots are one aligned 16-byte load);
-// the first box of each leaf-parent also writes the frame.  vec (F == G, a
-// multiple of 4, box arrays 16-byte aligned): a thread codes four
-// consecutive boxes, read with 16-byte loads and written as one; otherwise
-// one box per thread.
+// leaf-parent (node ls + i / F) into slot i % F of its G >= F slots (stored at
+// cell_slot); the first box of each leaf-parent also writes the frame.  vec
+// (F == G, a multiple of 4, box arrays 16-byte aligned): a thread codes four
+// consecutive boxes, read with 16-byte loads; otherwise one box per thread.
 __global__ __launch_bounds__(256) void k_leaf_codes(int64_t nbox, int F, int G, bool vec,
                                                     int64_t ls,
                                                     const float* __restrict__ nx1,
@@ -2009,16 +2008,19 @@ __global__ __launch_bounds__(256) void k_leaf_codes(int64_t nbox, int F, int G, 
       if (i == lp * F) frame[lp] = fr;
       const float4 x1 = ((const float4*)rb.x0)[j], y1 = ((const float4*)rb.y0)[j];
       const float4 x2 = ((const float4*)rb.x1)[j], y2 = ((const float4*)rb.y1)[j];
-      ((uint4*)code)[j] = make_uint4(box_cells(x1.x, y1.x, x2.x, y2.x, fr),
-                                     box_cells(x1.y, y1.y, x2.y, y2.y, fr),
-                                     box_cells(x1.z, y1.z, x2.z, y2.z, fr),
-                                     box_cells(x1.w, y1.w, x2.w, y2.w, fr));
+      uint32_t* cp = code + lp * G;
+      const int s0 = (int)(i - lp * F);
+      cp[cell_slot(s0, G)] = box_cells(x1.x, y1.x, x2.x, y2.x, fr);
+      cp[cell_slot(s0 + 1, G)] = box_cells(x1.y, y1.y, x2.y, y2.y, fr);
+      cp[cell_slot(s0 + 2, G)] = box_cells(x1.z, y1.z, x2.z, y2.z, fr);
+      cp[cell_slot(s0 + 3, G)] = box_cells(x1.w, y1.w, x2.w, y2.w, fr);
     }
     for (int64_t i = 4 * n4 + blockIdx.x * 256ll + threadIdx.x; i < nbox; i += step) {
       const int64_t lp = i / F;
       const float4 fr = leaf_frame(nx1[ls + lp], ny1[ls + lp], nx2[ls + lp], ny2[ls + lp]);
       if (i == lp * F) frame[lp] = fr;
-      code[i] = box_cells(rb.x0[i], rb.y0[i], rb.x1[i], rb.y1[i], fr);
+      code[lp * G + cell_slot((int)(i - lp * F), G)] =
+          box_cells(rb.x0[i], rb.y0[i], rb.x1[i], rb.y1[i], fr);
     }
     return;
   }
@@ -2026,14 +2028,15 @@ __global__ __launch_bounds__(256) void k_leaf_codes(int64_t nbox, int F, int G, 
     const int64_t lp = i / F;
     const float4 fr = leaf_frame(nx1[ls + lp], ny1[ls + lp], nx2[ls + lp], ny2[ls + lp]);
     if (i == lp * F) frame[lp] = fr;
-    code[lp * G + (i - lp * F)] = box_cells(rb.x0[i], rb.y0[i], rb.x1[i], rb.y1[i], fr);
+    code[lp * G + cell_slot((int)(i - lp * F), G)] =
+        box_cells(rb.x0[i], rb.y0[i], rb.x1[i], rb.y1[i], fr);
   }
 }
 
 // Node codes of one internal level: child j (node cs + j) is coded in the
-// frame of its parent (node ls + j / F, its MBR) into slot parent * G + j % F;
-// the first child of each parent writes the parent's frame at the parent's
-// node index.
+// frame of its parent (node ls + j / F, its MBR) into slot j % F of the
+// parent's G slots (stored at cell_slot); the first child of each parent
+// writes the parent's frame at the parent's node index.
 __global__ __launch_bounds__(256) void k_node_codes(int64_t cc, int F, int G, int64_t ls,
                                                     int64_t cs, QBoxes nb,
                                                     float4* __restrict__ frame,
@@ -2043,7 +2046,7 @@ __global__ __launch_bounds__(256) void k_node_codes(int64_t cc, int F, int G, in
     const float4 fr = leaf_frame(nb.x0[par], nb.y0[par], nb.x1[par], nb.y1[par]);
     if (sl == 0) frame[par] = fr;
     const int64_t ch = cs + j;
-    code[par * G + sl] = box_cells(nb.x0[ch], nb.y0[ch], nb.x1[ch], nb.y1[ch], fr);
+    code[par * G + cell_slot((int)sl, G)] = box_cells(nb.x0[ch], nb.y0[ch], nb.x1[ch], nb.y1[ch], fr);
   }
 }
 
