@@ -671,11 +671,12 @@ void k_leaf_count_coded(
       if (e < n_in && test[k]) {
         if (fr[k].z > 0.f) {
           const uint32_t wc = box_cells(wq[k].x, wq[k].y, wq[k].z, wq[k].w, fr[k]);
+          const ct_s16x2 wl = cells_low(wc), wh = cells_high(wc);
           const uint32_t b4[4] = {cd[k].x, cd[k].y, cd[k].z, cd[k].w};
 #pragma unroll
           for (int j = 0; j < 4; j++) {
             if (!((test[k] >> j) & 1u)) continue;
-            const int r = cells_intersect(wc, b4[j]);
+            const int r = cells_intersect(wl, wh, b4[j]);
             if (r > 0) hit[k] |= 1u << j;
             else if (r < 0) open[k] |= 1u << j;
           }
@@ -960,17 +961,14 @@ __global__ __launch_bounds__(kQThreads) void k_expand_count(int slot, QTree tr, 
     atomicAdd((unsigned long long*)&c->count[slot + 1], (unsigned long long)block_total);
 }
 
-// Containment of box cells b in window cells w (window low side <= box low
-// side, box high side <= window high side): 1 proven, 0 disproven, -1 open.
-// Only for a window without NaN (a NaN window contains nothing; its cells
-// could prove otherwise); a NaN in the box takes cell 0 (low side) or 255
-// (high side) and is never proven contained.
-__device__ __forceinline__ int cells_contain(uint32_t w, uint32_t b) {
-  const uint32_t wx1 = w & 255u, wy1 = (w >> 8) & 255u, wx2 = (w >> 16) & 255u, wy2 = w >> 24;
-  const uint32_t bx1 = b & 255u, by1 = (b >> 8) & 255u, bx2 = (b >> 16) & 255u, by2 = b >> 24;
-  if (wx1 > bx1 || bx2 > wx2 || wy1 > by1 || by2 > wy2) return 0;
-  if (wx1 < bx1 && bx2 < wx2 && wy1 < by1 && by2 < wy2) return 1;
-  return -1;
+// Containment of box cells b in window cells (low wl, high wh): window low
+// side <= box low side and box high side <= window high side; 1 proven, 0
+// disproven, -1 open.  Only for a window without NaN (a NaN window contains
+// nothing; its cells could prove otherwise); a NaN in the box takes cell 0
+// (low side) or 255 (high side) and is never proven contained.
+__device__ __forceinline__ int cells_contain(ct_s16x2 wl, ct_s16x2 wh, uint32_t b) {
+  const ct_s16x2 d1 = cells_low(b) - wl, d2 = wh - cells_high(b);
+  return lanes_sign(__builtin_elementwise_min(d1, d2));
 }
 
 // Internal level `slot` over the node codes (tr.nframe / tr.ncode, G >= 4):
@@ -1065,16 +1063,17 @@ void k_expand_count_coded(int slot, QTree tr, QBoxes qb, QCtrl* c,
       if (e < n_in && test[k]) {
         if (fr[k].z > 0.f) {
           const uint32_t wc = box_cells(wq[k].x, wq[k].y, wq[k].z, wq[k].w, fr[k]);
+          const ct_s16x2 wl = cells_low(wc), wh = cells_high(wc);
           const bool wnan = wq[k].x != wq[k].x || wq[k].y != wq[k].y || wq[k].z != wq[k].z ||
                             wq[k].w != wq[k].w;
           const uint32_t b4[4] = {cd[k].x, cd[k].y, cd[k].z, cd[k].w};
 #pragma unroll
           for (int j = 0; j < 4; j++) {
             if (!((test[k] >> j) & 1u)) continue;
-            const int r = cells_intersect(wc, b4[j]);
+            const int r = cells_intersect(wl, wh, b4[j]);
             if (r > 0) {
               hit[k] |= 1u << j;
-              const int rc = shortcut && !wnan ? cells_contain(wc, b4[j]) : 0;
+              const int rc = shortcut && !wnan ? cells_contain(wl, wh, b4[j]) : 0;
               if (rc > 0) cont[k] |= 1u << j;
               else if (rc < 0) open[k] |= 1u << j;  // (the MBR settles the containment)
             } else if (r < 0) {

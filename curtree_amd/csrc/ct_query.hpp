@@ -70,22 +70,21 @@ struct QTree {
 // its MBR and kCells / extent per axis (sx = 0: no codes for this node).  A
 // coordinate v maps to a cell of that frame by one function, the same for
 // boxes and windows:
-//   t = (v - x0) * sx, rounded after each step (no contraction);
-//   t < 0: cell 0; t >= kCells: cell 255; else 1 + floor(t).
-// The map is monotone (v <= w gives cell(v) <= cell(w)).  The box test is
-// "not separated": !(lo > hi) for the four pairs (window low side, box high
-// side) and (box low side, window high side).  cell(lo) > cell(hi) proves a
-// separation, cell(lo) < cell(hi) proves there is none, equal cells leave the
-// pair open, and only then is the box itself read.  NaN takes cell 0 on the
-// low side (xmin, ymin) and cell 255 on the high side (xmax, ymax), so a pair
-// with a NaN is never proven separated, as the float compare (false) says.
+//   t = (v - x0) * sx + 1, rounded after each step (no contraction);
+//   cell = t clamped to [0, 255] (v_med3), truncated.
+// The map is monotone (v <= w gives cell(v) <= cell(w)); nothing else about it
+// matters.  The box test is "not separated": !(lo > hi) for the four pairs
+// (window low side, box high side) and (box low side, window high side).
+// cell(lo) > cell(hi) proves a separation, cell(lo) < cell(hi) proves there is
+// none, equal cells leave the pair open, and only then is the box itself
+// read.  NaN takes cell 0 on the low side (xmin, ymin) and cell 255 on the
+// high side (xmax, ymax), so a pair with a NaN is never proven separated, as
+// the float compare (false) says.
 constexpr float kCells = 254.f;
 __device__ __forceinline__ uint32_t cell_of(float v, float x0, float sx, bool low_side) {
   if (v != v) return low_side ? 0u : 255u;
-  const float t = __fmul_rn(__fsub_rn(v, x0), sx);
-  if (!(t >= 0.f)) return 0u;
-  if (t >= kCells) return 255u;
-  return 1u + (uint32_t)t;
+  const float t = __fadd_rn(__fmul_rn(__fsub_rn(v, x0), sx), 1.f);
+  return (uint32_t)__builtin_amdgcn_fmed3f(t, 0.f, 255.f);
 }
 // Cells of a box (or window): xmin | ymin << 8 | xmax << 16 | ymax << 24.
 __device__ __forceinline__ uint32_t box_cells(float x1, float y1, float x2, float y2,
@@ -93,14 +92,32 @@ __device__ __forceinline__ uint32_t box_cells(float x1, float y1, float x2, floa
   return cell_of(x1, fr.x, fr.z, true) | (cell_of(y1, fr.y, fr.w, true) << 8) |
          (cell_of(x2, fr.x, fr.z, false) << 16) | (cell_of(y2, fr.y, fr.w, false) << 24);
 }
-// Intersection (no separation) of window cells w and box cells b: 1 proven,
-// 0 disproven (a separation is proven), -1 open.
+// The low (xmin, ymin) and high (xmax, ymax) cells of c as two 16-bit lanes
+// (x in the low half, y in the high half).
+typedef short ct_s16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ ct_s16x2 cells_low(uint32_t c) {
+  return __builtin_bit_cast(ct_s16x2, __builtin_amdgcn_perm(0u, c, 0x0c010c00u));
+}
+__device__ __forceinline__ ct_s16x2 cells_high(uint32_t c) {
+  return __builtin_bit_cast(ct_s16x2, __builtin_amdgcn_perm(0u, c, 0x0c030c02u));
+}
+// 1: both lanes of m positive; 0: a lane negative; -1: otherwise (a zero lane).
+__device__ __forceinline__ int lanes_sign(ct_s16x2 m) {
+  const uint32_t u = __builtin_bit_cast(uint32_t, m);
+  if (u & 0x80008000u) return 0;
+  const ct_s16x2 one = {1, 1};
+  return (__builtin_bit_cast(uint32_t, m - one) & 0x80008000u) ? -1 : 1;
+}
+// Intersection (no separation) of window cells (low wl, high wh) and box
+// cells b: 1 proven, 0 disproven (a separation is proven), -1 open.  Per axis
+// box high - window low and window high - box low: a negative difference
+// proves a separation, all positive prove none.
+__device__ __forceinline__ int cells_intersect(ct_s16x2 wl, ct_s16x2 wh, uint32_t b) {
+  const ct_s16x2 d1 = cells_high(b) - wl, d2 = wh - cells_low(b);
+  return lanes_sign(__builtin_elementwise_min(d1, d2));
+}
 __device__ __forceinline__ int cells_intersect(uint32_t w, uint32_t b) {
-  const uint32_t wx1 = w & 255u, wy1 = (w >> 8) & 255u, wx2 = (w >> 16) & 255u, wy2 = w >> 24;
-  const uint32_t bx1 = b & 255u, by1 = (b >> 8) & 255u, bx2 = (b >> 16) & 255u, by2 = b >> 24;
-  if (wx1 > bx2 || bx1 > wx2 || wy1 > by2 || by1 > wy2) return 0;
-  if (wx1 < bx2 && bx1 < wx2 && wy1 < by2 && by1 < wy2) return 1;
-  return -1;
+  return cells_intersect(cells_low(w), cells_high(w), b);
 }
 
 // Tiles of block blockIdx.x in a chain-free launch: first, first + step, ...
