@@ -657,3 +657,68 @@ def test_leaf_codes_fanouts(fanout):
     rt.free()
     d.free()
     q.free()
+
+
+def test_leaf_codes_forgotten_when_boxes_change():
+    # The leaf codes are kept with a tree built here; boxes rewritten through
+    # the library afterwards forget them with the rest of the tree's record (a
+    # stale code would decide a test from the old box).  One box moves after
+    # the codes are built: the next query (no record, so no codes) must see it
+    # where it now is, and a rebuild keeps the tree again with new codes.
+    import curtree_amd as ct
+
+    n, nq = 40_000, 2_000
+    bx = datagen.uniform_boxes(n, seed=191, wmax=2e-3)
+    ids = np.arange(n, dtype=np.int32)
+    d = ct.DeviceBoxes.from_host(*bx, ids)
+    rt = ct.build_rtree(d, 16, 1)
+    sb = list(d.to_host())
+    h = rt.to_host()
+    l0 = rt.sz - (n + 15) // 16
+    b = 123  # the box (sorted position) that moves
+    gx0, gy0 = float(h["xmin"][l0 + b // 16]), float(h["ymin"][l0 + b // 16])
+    ex = float(h["xmax"][l0 + b // 16]) - gx0
+    ws = [a.copy() for a in datagen.loguniform_windows(nq, seed=192, lo=1e-3, hi=0.1)[:4]]
+    # window 0 a little larger than the box (its cells prove a hit with the
+    # old box), window 1 a speck at the leaf-parent's low corner, where the
+    # box goes (its cells prove a miss with the old box)
+    m = ex / 50
+    ws[0][0], ws[1][0], ws[2][0], ws[3][0] = (sb[0][b] - m, sb[1][b] - m, sb[2][b] + m,
+                                              sb[3][b] + m)
+    ws[0][1], ws[1][1], ws[2][1], ws[3][1] = gx0 - 1e-7, gy0 - 1e-7, gx0 + 1e-7, gy0 + 1e-7
+    q = ct.DeviceBoxes.from_host(*ws)
+    ct.set_option("leaf_codes", 1)
+    ct.Prof.enable(True)
+    try:
+        ct.Prof.reset()
+        f0, t0 = ct.bfs_frontier(rt, d, q)
+        assert ct.Prof.get("k_leaf_codes")[1] == 1
+        of, ot, _ = orc.bfs(h, tuple(sb), ws)
+        assert np.array_equal(f0, of) and np.array_equal(t0, ot)
+        assert np.any((t0 == 0) & (f0 == sb[4][b])) and not np.any((t0 == 1) & (f0 == sb[4][b]))
+        # shrink the box to a point at its leaf-parent MBR's low corner: still
+        # inside the MBR, so the node MBRs stay valid, but its cells change
+        for k, a, v in ((0, "xmin", gx0), (1, "ymin", gy0), (2, "xmax", gx0), (3, "ymax", gy0)):
+            sb[k][b] = np.float32(v)
+            ct._check(ct.lib.curtree_memcpy_h2d(ctypes.c_void_p(getattr(d.s, a)),
+                                                sb[k].ctypes.data, sb[k].nbytes))
+        ct.Prof.reset()
+        f1, t1 = ct.bfs_frontier(rt, d, q)
+        assert ct.Prof.get("k_leaf_codes")[1] == 0  # the record, codes included, is gone
+        of1, ot1, _ = orc.bfs(h, tuple(sb), ws)
+        assert np.array_equal(f1, of1) and np.array_equal(t1, ot1)
+        assert not np.any((t1 == 0) & (f1 == sb[4][b])) and np.any((t1 == 1) & (f1 == sb[4][b]))
+        # a rebuild keeps the tree again, with codes from its first query
+        rt.free()
+        rt = ct.build_rtree(d, 16, 1)
+        ct.Prof.reset()
+        f2, t2 = ct.bfs_frontier(rt, d, q)
+        assert ct.Prof.get("k_leaf_codes")[1] == 1
+        of2, ot2, _ = orc.bfs(rt.to_host(), d.to_host(), ws)
+        assert np.array_equal(f2, of2) and np.array_equal(t2, ot2)
+    finally:
+        ct.Prof.enable(False)
+        ct.set_option("leaf_codes", -1)
+    rt.free()
+    d.free()
+    q.free()
