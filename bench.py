@@ -528,8 +528,9 @@ def serve_leg(args, ct, datagen, scale, seconds, rate_step, drain_s, full=False)
     served pairs/s are compared at each baseline's knee p99.  full: the served
     rate also rises by rate_step from serve_rate0 to serve_rate_max (the whole
     curve); otherwise it starts at the lower baseline knee and rises by
-    serve_fine_step until its p99 passes 1.5x the baselines' knee p99, which
-    is the part of the curve the equal-p99 comparison reads."""
+    serve_fine_step until its p99 passes 1.5x the baselines' knee p99 at two
+    rates in a row (or 3x at one), which is the part of the curve the
+    equal-p99 comparison reads."""
     sizes, trees = serve_trees(args, ct, datagen, scale)
     npool = 4_000_000
     pool = ct.DeviceBoxes.from_host(*datagen.loguniform_windows(npool, seed=80, lo=1e-5, hi=2e-3))
@@ -583,9 +584,17 @@ def serve_leg(args, ct, datagen, scale, seconds, rate_step, drain_s, full=False)
         res["served"] = server_sweep("served", args.serve_rate0, rate_step, lambda p: False,
                                      slots=args.serve_slots, batching=True)
     else:
+        # stop past the baselines' knee p99: two points in a row above 1.5x it, or one
+        # above 3x (a single hiccup of the host does not end the sweep)
         p99_max = max(ref["p99_ms"], ref_e["p99_ms"])
+        over = [0]
+
+        def past_knee(p):
+            over[0] = over[0] + 1 if p["p99_ms"] > 1.5 * p99_max else 0
+            return over[0] >= 2 or p["p99_ms"] > 3 * p99_max
+
         res["served"] = server_sweep("served", min(ref["rate"], ref_e["rate"]),
-                                     args.serve_fine_step, lambda p: p["p99_ms"] > 1.5 * p99_max,
+                                     args.serve_fine_step, past_knee,
                                      slots=args.serve_slots, batching=True)
     pool.free()
     for rt, d, _ in trees:
