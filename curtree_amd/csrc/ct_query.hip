@@ -2671,7 +2671,11 @@ int prepare_query(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curt
   // a block loops over tiles, so any grid is correct; small batches get a
   // small grid (cheaper launches that leave room for concurrent runs), also
   // under a tuned grid (tuned on large batches)
-  const int64_t by_size = std::max<int64_t>(512, nq_hint / 8);
+  // (at least 128 blocks: for a request of a few hundred windows most blocks
+  // of a chained launch only take a ticket and leave; with 128 rather than
+  // 512 the serving workload's p50 at 16k requests/s fell from 0.48 to 0.40 ms
+  // with the server and from 0.31 to 0.29 ms at 4.6k with the plain entry point)
+  const int64_t by_size = std::max<int64_t>(128, nq_hint / 8);
   a->grid = (int)std::min<int64_t>(qtuning().grid > 0 ? qtuning().grid : kDefaultChainGrid,
                                    by_size);
   a->nbox = N;
