@@ -2655,7 +2655,7 @@ int ensure_leaf_codes(const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox,
 
 int prepare_query(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox,
                   const curtree_bbox_f32* qbox, int contain, int64_t nq_hint, QRunArgs* a,
-                  hipStream_t s) {
+                  hipStream_t s, const uint32_t* canon) {
   if (!rt || !rbox || !qbox) return set_error(CURTREE_EINVAL, "query: null argument");
   const int64_t N = rbox->sz, Q = qbox->sz;
   const int F = rt->fanout;
@@ -2772,7 +2772,9 @@ int prepare_query(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curt
     a->tr.mbr.b4 = qe.node4;
     a->qb.b4 = qe.win4;
   }
-  if (use || stage || qe.fixed || a->tr.lcode) {  // (the codes need the bulk-load layout)
+  if (canon) {  // the caller checked this tree's layout once
+    a->tr.canon = canon;
+  } else if (use || stage || qe.fixed || a->tr.lcode) {  // (the codes need the bulk-load layout)
     if ((rc = check_layout(&qe.canon, rt, N, s))) return rc;
     a->tr.canon = qe.canon;
   }

@@ -222,9 +222,12 @@ struct QRunArgs {
 int next_pow2_group(int F);
 int per_tile(int G, int steps);
 // Validate, fill `a` (tree/boxes/levels) and compute the containment flags.
+// canon: the device word of a layout check the caller made once for this
+// tree (check_layout), used instead of checking again; null: checked here
+// where the run needs it.
 int prepare_query(QEngine& qe, Prof& pf, const curtree_rtree_f32* rt, const curtree_bbox_f32* rbox,
                   const curtree_bbox_f32* qbox, int contain, int64_t nq_hint, QRunArgs* a,
-                  hipStream_t s);
+                  hipStream_t s, const uint32_t* canon = nullptr);
 // Well-formedness flags of every node; with box4 (power-of-two F) the same
 // pass writes the AoS copy of the boxes and sets *staged.
 int compute_safe(QEngine& qe, Prof& pf, const QBoxes& rb, int F, int64_t r_sz, int64_t nbox,

@@ -332,11 +332,11 @@ int Server::begin_batch(Slot& sl) {
   }
   // the tree's cached containment flags, AoS copies (if any) and layout check
   int rc;
-  if ((rc = prepare_query(sl.qe, prof, &tr.rt, &tr.rbox, &q, 0, nwin, &sl.a, sl.s))) return rc;
+  if ((rc = prepare_query(sl.qe, prof, &tr.rt, &tr.rbox, &q, 0, nwin, &sl.a, sl.s, tr.canon)))
+    return rc;
   sl.a.tr.safe = tr.safe;
   sl.a.rb.b4 = tr.box4;
   sl.a.tr.mbr.b4 = tr.node4;
-  sl.a.tr.canon = tr.canon;
   sl.a.q0 = 0;
   sl.a.q1 = nwin;
   if ((rc = sl.qe.reserve_front(std::max<int64_t>(2 * nwin, 1 << 16)))) return rc;
