@@ -554,7 +554,9 @@ def serve_leg(args, ct, datagen, scale, seconds, rate_step, drain_s, full=False)
     def server_sweep(name, lam0, step, stop, **kw):
         sv = ct.Server(budget_bytes=budget, max_batch_windows=args.serve_batch_windows, **kw)
         ids = [sv.add_tree(rt, d) for rt, d, _ in trees]
-        _poisson_run(ct, sv, ids, pool, npool, 200.0, 0.5, seed=1, drain_s=drain_s)  # warm-up
+        # warm-up at a rate that makes batches large enough to grow every buffer
+        # once (a growth frees device memory, which waits for the whole device)
+        _poisson_run(ct, sv, ids, pool, npool, 2000.0, 1.0, seed=1, drain_s=drain_s)
         pts = sweep(name, lambda lam: _poisson_run(ct, sv, ids, pool, npool, lam, seconds,
                                                    seed=int(lam), drain_s=drain_s),
                     lam0, step, stop)
@@ -569,7 +571,7 @@ def serve_leg(args, ct, datagen, scale, seconds, rate_step, drain_s, full=False)
     fb, tb = ctypes.c_void_p(), ctypes.c_void_p()
     ct._check(ct.lib.curtree_malloc(ctypes.byref(fb), 4 * cap))
     ct._check(ct.lib.curtree_malloc(ctypes.byref(tb), 4 * cap))
-    _plain_entry_run(ct, trees, pool, npool, 200.0, 0.5, 1, (fb.value, tb.value), cap,
+    _plain_entry_run(ct, trees, pool, npool, 2000.0, 1.0, 1, (fb.value, tb.value), cap,
                      drain_s=drain_s)
     pts = sweep("plain entry point",
                 lambda lam: _plain_entry_run(ct, trees, pool, npool, lam, seconds, int(lam),
